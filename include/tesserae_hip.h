@@ -1,0 +1,197 @@
+/*
+ * tesserae_hip.h — C ABI of libtesserae_hip, the MI355X (gfx950) engine for the
+ * explicit material-point-method step of Tesserae.jl v0.7.5.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, returns a
+ * tess_status, never throws or aborts across the boundary, and enqueues its GPU
+ * work asynchronously on the stream bound to the context.  Only calls that say
+ * so synchronise (they read a small total back to the host, as the reference's
+ * GPU partition does at src/Parallel/gpu_block_strategy.jl:161-165).
+ *
+ * Conventions shared by all calls
+ *  - Arrays are caller-owned device memory (Julia-GC-owned CuArray/ROCArray on
+ *    the reference side).  The library never frees or reallocates them; its own
+ *    scratch (sort buffers, tile partials) belongs to the tess_ctx.
+ *  - A field is described by tess_field: component c of element e lives at
+ *    ptr + (e*elem_stride + c*comp_stride) scalars.  Julia-native layouts
+ *    (Vec{d,T} arrays: elem_stride=d, comp_stride=1; Mat{d,d,T}: elem_stride=d*d,
+ *    comp_stride=1, column-major components) and pure SoA (elem_stride=1,
+ *    comp_stride=n) are both accepted.
+ *  - Grid nodes are numbered x-fastest (Julia column-major Array{T,dim}).
+ *  - Scalars are the mesh's dtype (TESS_FLOAT32 / TESS_FLOAT64).
+ *
+ * Error classes mirror the exceptions the reference raises on the host before
+ * any launch (src/Transfer/runtime.jl:177-229, src/Basis/basis_weight.jl:714-752,
+ * src/Parallel/gpu_block_strategy.jl:147-148): the Julia binding rethrows
+ * ArgumentError / DimensionMismatch / AssertionError / ErrorException.
+ */
+#ifndef TESSERAE_HIP_H
+#define TESSERAE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TESS_ABI_VERSION 1
+
+typedef int32_t tess_status;
+enum {
+  TESS_OK = 0,
+  TESS_ERR_ARGUMENT = 1,    /* Julia ArgumentError */
+  TESS_ERR_DIMENSION = 2,   /* Julia DimensionMismatch */
+  TESS_ERR_ASSERTION = 3,   /* Julia AssertionError (the reference's @assert) */
+  TESS_ERR_STATE = 4,       /* Julia ErrorException (the reference's error(...)) */
+  TESS_ERR_DEVICE = 5,      /* HIP runtime failure (launch, allocation, copy) */
+  TESS_ERR_UNSUPPORTED = 6  /* program/basis/dimension outside the library's fixed set */
+};
+
+enum { TESS_FLOAT32 = 0, TESS_FLOAT64 = 1 };
+
+/* Bases: BSpline(Linear()), BSpline(Quadratic()), BSpline(Cubic()) and uGIMP()
+ * (src/Basis/bspline.jl:15-40, src/Basis/gimp.jl:16-40). */
+enum {
+  TESS_BSPLINE_LINEAR = 1,
+  TESS_BSPLINE_QUADRATIC = 2,
+  TESS_BSPLINE_CUBIC = 3,
+  TESS_UGIMP = 16
+};
+
+/* CartesianMesh{dim,T} (src/Mesh/cartesian.jl:38-42, ctor :89-103).
+ * xmin and h are the Float64 values the mesh was built from; the library uses
+ * T(xmin), T(h), T(h_inv) exactly as the device copy of the mesh holds them
+ * (src/adapt.jl:53-57), and node coordinates T(xmin + i*h). */
+typedef struct tess_mesh {
+  int32_t dim;              /* 2 or 3 */
+  int32_t dtype;            /* TESS_FLOAT32 / TESS_FLOAT64 */
+  int32_t block_size_log2;  /* cartesian.jl:11, blocks of 2^L cells per axis */
+  int32_t reserved;
+  int64_t dims[3];          /* nodes per axis, size(mesh) */
+  double xmin[3];
+  double h;                 /* spacing(mesh) */
+  double h_inv;             /* spacing_inv(mesh) = inv(h) */
+} tess_mesh;
+
+typedef struct tess_field {
+  void* ptr;
+  int64_t elem_stride;      /* in scalars */
+  int64_t comp_stride;      /* in scalars */
+} tess_field;
+
+/* Particle StructArray of the README program (README.md:53-54):
+ * {x::Vec, V::T, v::Vec, ∇v::Mat, F::Mat, σ::Mat}; l is the uGIMP particle
+ * length (src/Basis/gimp.jl:25); logE is an optional per-particle stiffness
+ * exponent, E_p = E*exp(logE_p) (NULL ptr: E_p = E). */
+typedef struct tess_particles {
+  int64_t n;
+  tess_field x, v, V, grad_v, F, sigma;
+  tess_field l;
+  tess_field logE;
+} tess_particles;
+
+/* Grid StructArray of the README program: {m::T, mv::Vec, f::Vec, vⁿ::Vec, v::Vec}. */
+typedef struct tess_grid {
+  tess_field m, mv, f, vn, v;
+} tess_grid;
+
+/* BasisWeightArray in stored mode (src/Basis/basis_weight.jl:345-358):
+ * w is Array{T,dim+1} of size (sw,...,sw,n), grad_w Array{Vec{dim,T},dim+1},
+ * window the per-particle CartesianIndices as dim (start, stop) Int64 pairs,
+ * 1-based inclusive, (1,0) when empty.  Entry ip of particle p is element
+ * ip + sw^dim * p, ip being the column-major position of the node inside the
+ * (possibly truncated) window placed at the origin of the sw^dim storage. */
+typedef struct tess_weights {
+  tess_field w;
+  tess_field grad_w;
+  int64_t* window;
+} tess_weights;
+
+/* Constants of the README program (README.md:46-49, 71-87) plus the config
+ * extras: gravity (applied as an acceleration on nodes with m>0) and slip walls
+ * (bit 2*d: low face of axis d, bit 2*d+1: high face; the normal component of
+ * the grid velocity v is zeroed on that face's nodes, docs/src/gpu.md:116-129). */
+typedef struct tess_params {
+  double dt;
+  double lambda;            /* Lamé λ = Eν/((1+ν)(1-2ν)) */
+  double mu;                /* Lamé μ = E/(2(1+ν)) */
+  double mp;                /* particle mass mₚ = ρ⁰V⁰ₚ */
+  double V0;                /* initial particle volume V⁰ₚ */
+  double gravity[3];
+  int32_t slip_walls;
+  int32_t reserved;
+} tess_params;
+
+typedef struct tess_ctx tess_ctx;
+
+/* ---- context, errors ---------------------------------------------------- */
+
+/* Replaces the backend glue of ext/TesseraeCUDAExt.jl:11-39 (backend <-> device,
+ * workgroup sizing, temporaries).  stream may be NULL (default stream). */
+tess_status tess_ctx_create(int32_t device, void* stream, tess_ctx** out);
+tess_status tess_ctx_destroy(tess_ctx* ctx);
+/* Blocks until the context's stream is idle (KernelAbstractions.synchronize). */
+tess_status tess_sync(tess_ctx* ctx);
+/* Message of the last failed call on this host thread ("" if none). */
+const char* tess_last_error(void);
+int32_t tess_abi_version(void);
+
+/* ---- basis weights ------------------------------------------------------ */
+
+/* update!(weights, particles, mesh) in stored mode
+ * (src/Basis/basis_weight.jl:714-752; per particle :634-654, B-spline fast path
+ * src/Basis/bspline.jl:102-170, truncated windows basis_weight.jl:624-632).
+ * Writes w, grad_w and window for particles 0..n-1. */
+tess_status tess_update_weights(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh,
+                                const tess_particles* particles, const tess_weights* weights);
+
+/* ---- partition ---------------------------------------------------------- */
+
+/* update!(partition, x) with the CPU block strategy's result
+ * (src/Parallel/cpu_block_strategy.jl:62-219): particleindices = inside-mesh
+ * particle ids (1-based) stably sorted by (block colour, linear block id);
+ * starts/stops (1-based, 0 for empty blocks) per block in linear block order;
+ * *nassigned (host) = number of assigned particles.  Synchronises. */
+tess_status tess_partition_update(tess_ctx* ctx, const tess_mesh* mesh, tess_field x, int64_t n,
+                                  int32_t* particleindices, int32_t* starts, int32_t* stops,
+                                  int64_t* nassigned);
+
+/* reorder_particles!(particles, partition; threshold)
+ * (src/Parallel/cpu_block_strategy.jl:248-355): permutes every particle field
+ * into the partition's block order when the block-ordered contiguity is below
+ * threshold (threshold 1 always reorders, 0 never).  particleindices/nassigned
+ * are those of tess_partition_update and are rewritten to the identity.
+ * *reordered (host) receives 1 if the particles were moved.  Synchronises. */
+tess_status tess_reorder_particles(tess_ctx* ctx, const tess_mesh* mesh, const tess_particles* particles,
+                                   int32_t* particleindices, int64_t nassigned, double threshold,
+                                   int32_t* reordered);
+
+/* ---- transfers ---------------------------------------------------------- */
+
+/* @P2G of the README program (README.md:71-77; lowering src/Transfer/p2g.jl:145-178,
+ * grid-node half :377-395): m = Σ w mₚ, mv = Σ w (mₚ v), f = Σ ((-V σ)·∇w), then on
+ * every node m⁻¹ = m>0 ? 1/m : 0, vⁿ = mv m⁻¹, v = vⁿ + Δt (f m⁻¹ + g[m>0]), slip
+ * walls.  weights == NULL evaluates the basis inside the transfer (deferred
+ * weights, src/Transfer/runtime.jl:21-24,55-130).  Deterministic: no float
+ * atomics; the same inputs give bitwise-identical grids. */
+tess_status tess_p2g(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                     const tess_particles* particles, const tess_weights* weights,
+                     const tess_params* params);
+
+/* @G2P of the README program (README.md:80-87; lowering src/Transfer/g2p.jl:110-157):
+ * v += Σ w (v_i - vⁿ_i); ∇v = Σ v_i ⊗ ∇w; x += Σ w v_i Δt; F = (I + Δt∇v)F;
+ * J = det F; σ = (μ(FFᵀ - I) + λ ln J I)/J; V = V⁰J (J, ln J, 1/J in Float64). */
+tess_status tess_g2p(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                     const tess_particles* particles, const tess_weights* weights,
+                     const tess_params* params);
+
+/* One README step with deferred weights: update!(wgts,…) + @P2G + @G2P
+ * (README.md:66-87), repeated nsteps times on the context's stream. */
+tess_status tess_step(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                      const tess_particles* particles, const tess_params* params, int32_t nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TESSERAE_HIP_H */

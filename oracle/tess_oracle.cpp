@@ -1,0 +1,810 @@
+// tess_oracle.cpp — TEST INFRASTRUCTURE ONLY.
+//
+// A CPU restatement of Tesserae.jl v0.7.5's explicit MPM step (the README
+// program, README.md:44-87) used as the parity checker for libtesserae_hip and
+// as the CPU baseline timed beside it.  Only tests/, __graft_entry__.smoke()
+// and bench.py's cpu_baseline leg may load this library; the product path never
+// does.  Every function names the reference file:line it follows.
+//
+// Parity status of each part (what pins it, beyond this restatement):
+//   mesh geometry, windows, findblock ........ pinned by test/mesh.jl:27-65 KATs
+//   B-spline / uGIMP values and gradients .... pinned by closed forms, partition of
+//                                              unity, linear reproduction, jet KATs
+//                                              (test/basis.jl:226-346), brute force
+//   CPU block partition ...................... pinned by brute-force grouping and
+//                                              colour-disjointness (test/partition.jl:53-158)
+//   P2G / grid update / G2P .................. pinned by conservation invariants, affine
+//                                              reproduction and a brute-force dense loop
+//   Neo-Hookean det/log (Tensorial.jl) ....... parity unpinned beyond closed-form
+//                                              checks (no reference test covers them)
+//
+// Arithmetic order follows the reference where it is visible in its source:
+// Horner evaluation with muladd (Julia evalpoly), left-associated tensor
+// products, hoisted particle factors (p2g.jl:272-307), per-particle temporaries
+// in G2P (g2p.jl:123-144).  Tensorial.jl's internal summation order for
+// Mat*Vec and det is not in the reference tree; left-to-right sums are used.
+//
+// Build: g++ -O2 -std=c++17 -fPIC -shared -pthread (see oracle/Makefile).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../include/tesserae_hip.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+template <typename T>
+struct Fld {
+  T* p;
+  int64_t es, cs;
+  Fld() : p(nullptr), es(0), cs(0) {}
+  explicit Fld(const tess_field& f) : p(static_cast<T*>(f.ptr)), es(f.elem_stride), cs(f.comp_stride) {}
+  T& operator()(int64_t e, int c = 0) const { return p[e * es + c * cs]; }
+  bool ok() const { return p != nullptr; }
+};
+
+// CartesianMesh as its device copy holds it (cartesian.jl:38-42, adapt.jl:53-57):
+// xmin, h, h⁻¹ cast to T; node coordinate T(xmin + i*h).
+template <typename T, int D>
+struct Mesh {
+  int64_t dims[D];
+  T xmin[D];
+  T h, h_inv;
+  double xmin_d[D], h_d;
+  int L;
+  explicit Mesh(const tess_mesh& m) {
+    for (int d = 0; d < D; ++d) {
+      dims[d] = m.dims[d];
+      xmin[d] = T(m.xmin[d]);
+      xmin_d[d] = m.xmin[d];
+    }
+    h = T(m.h);
+    h_inv = T(m.h_inv);
+    h_d = m.h;
+    L = m.block_size_log2;
+  }
+  T node(int d, int64_t i) const { return T(xmin_d[d] + double(i) * h_d); }
+  int64_t nnodes() const {
+    int64_t n = 1;
+    for (int d = 0; d < D; ++d) n *= dims[d];
+    return n;
+  }
+  int64_t lin(const int64_t* I) const {
+    int64_t l = 0;
+    for (int d = D - 1; d >= 0; --d) l = l * dims[d] + I[d];
+    return l;
+  }
+  int64_t nblocks(int d) const { return ((dims[d] - 1) >> L) + 1; }  // cartesian.jl:241-243
+};
+
+// ---- basis (bspline.jl, gimp.jl, basis_weight.jl) -------------------------
+
+constexpr int support_width(int basis) {
+  return basis == TESS_BSPLINE_LINEAR ? 2 : basis == TESS_BSPLINE_QUADRATIC ? 3 : basis == TESS_BSPLINE_CUBIC ? 4 : 3;
+}
+constexpr int degree_of(int basis) { return basis == TESS_UGIMP ? 2 : basis; }
+
+// Julia evalpoly(x, (a0, a1, ...)) = muladd Horner from the top coefficient.
+template <typename T>
+T horner(T x, const T* a, int n) {
+  T ex = a[n - 1];
+  for (int i = n - 2; i >= 0; --i) ex = std::fma(x, ex, a[i]);
+  return ex;
+}
+
+// Coefficient tables of _bspline_local_coeffs (bspline.jl:45-56) and their first
+// derivative (:58-66), ascending powers; trailing zeros trimmed as in
+// _evalpoly_expr (:68-75).  Rationals enter as T(num)/T(den).
+template <typename T>
+void bspline_axis_jet(int deg, T xi_local, T* val, T* der) {
+  auto c = [](double num, double den) { return den == 1 ? T(num) : T(num) / T(den); };
+  if (deg == 1) {
+    T a0[2] = {c(1, 1), c(-1, 1)}, a1[2] = {c(0, 1), c(1, 1)};
+    T d0[1] = {c(-1, 1)}, d1[1] = {c(1, 1)};
+    val[0] = horner(xi_local, a0, 2); val[1] = horner(xi_local, a1, 2);
+    der[0] = horner(xi_local, d0, 1); der[1] = horner(xi_local, d1, 1);
+  } else if (deg == 2) {
+    T a0[3] = {c(1, 2), c(-1, 1), c(1, 2)}, a1[3] = {c(1, 2), c(1, 1), c(-1, 1)}, a2[3] = {c(0, 1), c(0, 1), c(1, 2)};
+    T d0[2] = {c(-1, 1), c(1, 1)}, d1[2] = {c(1, 1), c(-2, 1)}, d2[2] = {c(0, 1), c(1, 1)};
+    val[0] = horner(xi_local, a0, 3); val[1] = horner(xi_local, a1, 3); val[2] = horner(xi_local, a2, 3);
+    der[0] = horner(xi_local, d0, 2); der[1] = horner(xi_local, d1, 2); der[2] = horner(xi_local, d2, 2);
+  } else {
+    T a0[4] = {c(1, 6), c(-1, 2), c(1, 2), c(-1, 6)}, a1[4] = {c(2, 3), c(0, 1), c(-1, 1), c(1, 2)};
+    T a2[4] = {c(1, 6), c(1, 2), c(1, 2), c(-1, 2)}, a3[4] = {c(0, 1), c(0, 1), c(0, 1), c(1, 6)};
+    T d0[3] = {c(-1, 2), c(1, 1), c(-1, 2)}, d1[3] = {c(0, 1), c(-2, 1), c(3, 2)};
+    T d2[3] = {c(1, 2), c(1, 1), c(-3, 2)}, d3[3] = {c(0, 1), c(0, 1), c(1, 2)};
+    val[0] = horner(xi_local, a0, 4); val[1] = horner(xi_local, a1, 4);
+    val[2] = horner(xi_local, a2, 4); val[3] = horner(xi_local, a3, 4);
+    der[0] = horner(xi_local, d0, 3); der[1] = horner(xi_local, d1, 3);
+    der[2] = horner(xi_local, d2, 3); der[3] = horner(xi_local, d3, 3);
+  }
+}
+
+// Piecewise value and its ForwardDiff derivative (bspline.jl:176-189,
+// gimp.jl:28-33, differentiated as basis_weight.jl:35-37 does): abs(Dual) is
+// signbit(x) ? -x : x, x^2 / x^3 are literal powers.
+template <typename T>
+void piecewise_jet(int basis, T xi, T l, T* val, T* der) {
+  T s = std::signbit(xi) ? T(-1) : T(1);
+  T a = std::fabs(xi);
+  if (basis == TESS_BSPLINE_LINEAR) {
+    if (a < T(1)) { *val = T(1) - a; *der = -s; }
+    else { *val = T(0); *der = T(0); }
+  } else if (basis == TESS_BSPLINE_QUADRATIC) {
+    if (a < T(0.5)) {
+      *val = (T(3) - T(4) * (a * a)) / T(4);
+      *der = -(T(4) * (T(2) * a * s)) / T(4);
+    } else if (a < T(1.5)) {
+      T u = T(3) - T(2) * a;
+      *val = (u * u) / T(8);
+      *der = (T(2) * u * (T(-2) * s)) / T(8);
+    } else { *val = T(0); *der = T(0); }
+  } else if (basis == TESS_BSPLINE_CUBIC) {
+    if (a < T(1)) {
+      T a2 = a * a, a3 = a * a * a;
+      *val = (T(3) * a3 - T(6) * a2 + T(4)) / T(6);
+      *der = (T(3) * (T(3) * a2 * s) - T(6) * (T(2) * a * s)) / T(6);
+    } else if (a < T(2)) {
+      T u = T(2) - a;
+      *val = (u * u * u) / T(6);
+      *der = (T(3) * (u * u) * (-s)) / T(6);
+    } else { *val = T(0); *der = T(0); }
+  } else {  // uGIMP, gimp.jl:28-33 with l normalised by h
+    if (a < l / T(2)) {
+      *val = T(1) - (T(4) * (a * a) + l * l) / (T(4) * l);
+      *der = -(T(4) * (T(2) * a * s)) / (T(4) * l);
+    } else if (a < T(1) - l / T(2)) {
+      *val = T(1) - a; *der = -s;
+    } else if (a < T(1) + l / T(2)) {
+      T u = T(1) + l / T(2) - a;
+      *val = (u * u) / (T(2) * l);
+      *der = (T(2) * u * (-s)) / (T(2) * l);
+    } else { *val = T(0); *der = T(0); }
+  }
+}
+
+// One particle's BasisWeight: window (0-based inclusive, clamped) and the values
+// at the local column-major positions of the clipped box inside sw^D storage.
+template <typename T, int D>
+struct PW {
+  static constexpr int MAXSW = 4;
+  int64_t lo[D], hi[D];    // clamped window, 0-based inclusive; empty if hi<lo
+  int64_t start[D];        // unclamped start (0-based)
+  bool empty;
+  bool full;
+  int sw;
+  T w[MAXSW * MAXSW * MAXSW];
+  T gw[MAXSW * MAXSW * MAXSW][D];
+};
+
+template <typename T, int D>
+bool isinside_xi(const T* xi, const Mesh<T, D>& mesh) {  // cartesian.jl:157-163, 227-234
+  for (int d = 0; d < D; ++d) {
+    T f = std::floor(xi[d]);
+    if (!(T(0) <= f && f <= T(mesh.dims[d] - 2))) return false;
+  }
+  return true;
+}
+
+// supportnodes (bspline.jl:27-40; gimp.jl:20-24 -> cartesian.jl:190-199).
+template <typename T, int D>
+void support_window(int basis, const T* x, T lp, const Mesh<T, D>& mesh, PW<T, D>& pw) {
+  T xi[D];
+  for (int d = 0; d < D; ++d) xi[d] = (x[d] - mesh.xmin[d]) * mesh.h_inv;  // normalize, cartesian.jl:146-150
+  pw.sw = support_width(basis);
+  pw.empty = !isinside_xi<T, D>(xi, mesh);
+  if (pw.empty) {
+    for (int d = 0; d < D; ++d) { pw.lo[d] = 0; pw.hi[d] = -1; pw.start[d] = 0; }
+    pw.full = false;
+    return;
+  }
+  bool full = true;
+  for (int d = 0; d < D; ++d) {
+    int64_t s, e;
+    if (basis == TESS_UGIMP) {
+      T l = lp / mesh.h;  // _normalized_particle_length, gimp.jl:25
+      T r = T(1) + l / T(2);
+      s = int64_t(std::floor(xi[d] - r)) + 1;
+      e = int64_t(std::floor(xi[d] + r));
+    } else {
+      int n = degree_of(basis);
+      T offset = T(n - 1) / T(2);
+      s = int64_t(std::floor(xi[d] - offset));
+      e = s + n;
+    }
+    pw.start[d] = s;
+    pw.lo[d] = std::max<int64_t>(s, 0);
+    pw.hi[d] = std::min<int64_t>(e, mesh.dims[d] - 1);
+    if (pw.hi[d] - pw.lo[d] + 1 != pw.sw) full = false;
+  }
+  pw.full = full;
+}
+
+// update_basis_weight! (basis_weight.jl:636-642): fast full-support B-spline
+// fill (bspline.jl:146-170) or the nodewise jet (basis_weight.jl:624-632).
+// `deferred` selects the transfer-time evaluation of deferred weights, which is
+// always nodewise (basis_weight.jl:442-446 -> bspline.jl:204-205).
+template <typename T, int D>
+void eval_weights(int basis, const T* x, T lp, const Mesh<T, D>& mesh, bool deferred, PW<T, D>& pw) {
+  support_window<T, D>(basis, x, lp, mesh, pw);
+  if (pw.empty) return;
+  const int sw = pw.sw;
+  int64_t ext[D];
+  for (int d = 0; d < D; ++d) ext[d] = pw.hi[d] - pw.lo[d] + 1;
+  T v1[D][PW<T, D>::MAXSW], d1[D][PW<T, D>::MAXSW];
+  if (pw.full && basis != TESS_UGIMP && !deferred) {
+    for (int d = 0; d < D; ++d) {
+      T xi = (x[d] - mesh.xmin[d]) * mesh.h_inv;
+      int n = degree_of(basis);
+      T y = (n % 2 == 1) ? xi : xi - T(0.5);
+      T loc = y - std::floor(y);  // fract
+      bspline_axis_jet<T>(n, loc, v1[d], d1[d]);
+    }
+  } else {
+    T l = basis == TESS_UGIMP ? lp / mesh.h : T(0);
+    for (int d = 0; d < D; ++d)
+      for (int64_t a = 0; a < ext[d]; ++a) {
+        T xi = (x[d] - mesh.node(d, pw.lo[d] + a)) * mesh.h_inv;  // separable_nodal_basis_jet :49-60
+        piecewise_jet<T>(basis, xi, l, &v1[d][a], &d1[d][a]);
+      }
+  }
+  // local column-major positions of the clipped box inside the sw^D storage
+  int64_t cnt = 1;
+  for (int d = 0; d < D; ++d) cnt *= ext[d];
+  for (int64_t k = 0; k < cnt; ++k) {
+    int64_t a[D], r = k;
+    for (int d = 0; d < D; ++d) { a[d] = r % ext[d]; r /= ext[d]; }
+    int64_t slot = 0;
+    for (int d = D - 1; d >= 0; --d) slot = slot * sw + a[d];
+    T w = v1[0][a[0]];
+    for (int d = 1; d < D; ++d) w = w * v1[d][a[d]];
+    pw.w[slot] = w;
+    for (int c = 0; c < D; ++c) {
+      T g = (c == 0) ? d1[0][a[0]] : v1[0][a[0]];
+      for (int d = 1; d < D; ++d) g = g * ((d == c) ? d1[d][a[d]] : v1[d][a[d]]);
+      pw.gw[slot][c] = g * mesh.h_inv;
+    }
+  }
+}
+
+// Iterate the window column-major (eachindex(window)), calling f(slot, node_linear).
+template <typename T, int D, typename Fn>
+void for_window(const PW<T, D>& pw, const Mesh<T, D>& mesh, Fn&& f) {
+  if (pw.empty) return;
+  int64_t ext[D];
+  int64_t cnt = 1;
+  for (int d = 0; d < D; ++d) { ext[d] = pw.hi[d] - pw.lo[d] + 1; cnt *= ext[d]; }
+  for (int64_t k = 0; k < cnt; ++k) {
+    int64_t a[D], I[D], r = k;
+    for (int d = 0; d < D; ++d) { a[d] = r % ext[d]; r /= ext[d]; I[d] = pw.lo[d] + a[d]; }
+    int64_t slot = 0;
+    for (int d = D - 1; d >= 0; --d) slot = slot * pw.sw + a[d];
+    f(int(slot), mesh.lin(I));
+  }
+}
+
+template <typename T, int D>
+void load_vec(const Fld<T>& f, int64_t e, T* out) {
+  for (int c = 0; c < D; ++c) out[c] = f(e, c);
+}
+
+// Weights for particle p: from stored arrays (weight_prop_view) or evaluated.
+template <typename T, int D>
+void particle_weights(int basis, const Mesh<T, D>& mesh, const tess_particles& P, int64_t p,
+                      const tess_weights* W, PW<T, D>& pw) {
+  Fld<T> X(P.x), Lf(P.l);
+  T x[D];
+  load_vec<T, D>(X, p, x);
+  T lp = Lf.ok() ? Lf(p) : T(0);
+  if (!W) {
+    eval_weights<T, D>(basis, x, lp, mesh, true, pw);
+    return;
+  }
+  // stored: window and values as update! wrote them (runtime.jl:15-18, 39-44)
+  pw.sw = support_width(basis);
+  const int64_t* win = W->window + p * 2 * D;
+  pw.empty = false;
+  for (int d = 0; d < D; ++d) {
+    pw.lo[d] = win[2 * d] - 1;
+    pw.hi[d] = win[2 * d + 1] - 1;
+    if (pw.hi[d] < pw.lo[d]) pw.empty = true;
+  }
+  int64_t nsl = 1;
+  for (int d = 0; d < D; ++d) nsl *= pw.sw;
+  Fld<T> Wf(W->w), Gf(W->grad_w);
+  for (int64_t s = 0; s < nsl; ++s) {
+    pw.w[s] = Wf(s + nsl * p);
+    for (int c = 0; c < D; ++c) pw.gw[s][c] = Gf(s + nsl * p, c);
+  }
+}
+
+// ---- the README program ----------------------------------------------------
+
+// Zero the `=` targets (p2g.jl:159, zero.jl:78-79).
+template <typename T, int D>
+void zero_grid(const Mesh<T, D>& mesh, const tess_grid& G) {
+  Fld<T> m(G.m), mv(G.mv), f(G.f);
+  int64_t nn = mesh.nnodes();
+  for (int64_t i = 0; i < nn; ++i) {
+    m(i) = T(0);
+    for (int c = 0; c < D; ++c) { mv(i, c) = T(0); f(i, c) = T(0); }
+  }
+}
+
+// The @∑ half for one particle (p2g.jl:145-178 with hoisting :272-307):
+// m += w*mₚ ; mv += w*(mₚ*v) ; f += ((-V)*σ)·∇w.
+template <typename T, int D>
+void p2g_particle(int basis, const Mesh<T, D>& mesh, const tess_grid& G, const tess_particles& P,
+                  const tess_weights* W, const tess_params& prm, int64_t p) {
+  Fld<T> m(G.m), mv(G.mv), f(G.f);
+  Fld<T> V(P.V), v(P.v), sg(P.sigma);
+  PW<T, D> pw;
+  particle_weights<T, D>(basis, mesh, P, p, W, pw);
+  if (pw.empty) return;
+  const T mp = T(prm.mp);
+  T mpv[D], A[D * D];
+  for (int c = 0; c < D; ++c) mpv[c] = mp * v(p, c);
+  const T negV = -V(p);
+  for (int k = 0; k < D * D; ++k) A[k] = negV * sg(p, k);
+  for_window<T, D>(pw, mesh, [&](int s, int64_t i) {
+    const T w = pw.w[s];
+    m(i) += w * mp;
+    for (int c = 0; c < D; ++c) mv(i, c) += w * mpv[c];
+    for (int a = 0; a < D; ++a) {
+      T acc = A[a] * pw.gw[s][0];
+      for (int b = 1; b < D; ++b) acc = acc + A[a + D * b] * pw.gw[s][b];
+      f(i, a) += acc;
+    }
+  });
+}
+
+// Grid-node half (p2g.jl:377-395 -> foreach.jl:140-180), guarded division as in
+// test/transfer.jl:102-104 with m⁻¹ in Float64, gravity and slip walls.
+template <typename T, int D>
+void grid_update_node(const Mesh<T, D>& mesh, const tess_grid& G, const tess_params& prm, int64_t i) {
+  Fld<T> m(G.m), mv(G.mv), f(G.f), vn(G.vn), v(G.v);
+  const double md = double(m(i));
+  const double minv = md > 0 ? 1.0 / md : 0.0;
+  const T dt = T(prm.dt);
+  int64_t I[D], r = i;
+  for (int d = 0; d < D; ++d) { I[d] = r % mesh.dims[d]; r /= mesh.dims[d]; }
+  for (int c = 0; c < D; ++c) {
+    T vnc = T(double(mv(i, c)) * minv);
+    T acc = T(double(f(i, c)) * minv) + (md > 0 ? T(prm.gravity[c]) : T(0));
+    T vc = vnc + acc * dt;
+    bool wall = ((prm.slip_walls >> (2 * c)) & 1) && I[c] == 0;
+    wall = wall || (((prm.slip_walls >> (2 * c + 1)) & 1) && I[c] == mesh.dims[c] - 1);
+    if (wall) vc = T(0);
+    vn(i, c) = vnc;
+    v(i, c) = vc;
+  }
+}
+
+template <typename T, int D>
+double det_d(const T* F) {
+  if (D == 2) return double(F[0]) * double(F[3]) - double(F[2]) * double(F[1]);
+  // F[a + 3b] = F_{ab}; cofactor expansion along the first row
+  double f11 = F[0], f21 = F[1], f31 = F[2], f12 = F[3], f22 = F[4], f32 = F[5], f13 = F[6], f23 = F[7], f33 = F[8];
+  return f11 * (f22 * f33 - f23 * f32) - f12 * (f21 * f33 - f23 * f31) + f13 * (f21 * f32 - f22 * f31);
+}
+
+// G2P for one particle (g2p.jl:110-157) with the README particle equations.
+template <typename T, int D>
+void g2p_particle(int basis, const Mesh<T, D>& mesh, const tess_grid& G, const tess_particles& P,
+                  const tess_weights* W, const tess_params& prm, int64_t p) {
+  Fld<T> gvn(G.vn), gv(G.v);
+  Fld<T> X(P.x), V(P.V), v(P.v), gradv(P.grad_v), Ff(P.F), sg(P.sigma), LE(P.logE);
+  PW<T, D> pw;
+  particle_weights<T, D>(basis, mesh, P, p, W, pw);
+  const T dt = T(prm.dt);
+  T dv[D] = {}, L[D * D] = {}, dx[D] = {};
+  for_window<T, D>(pw, mesh, [&](int s, int64_t i) {
+    const T w = pw.w[s];
+    T vi[D];
+    for (int c = 0; c < D; ++c) vi[c] = gv(i, c);
+    for (int c = 0; c < D; ++c) dv[c] += w * (vi[c] - gvn(i, c));
+    for (int b = 0; b < D; ++b)
+      for (int a = 0; a < D; ++a) L[a + D * b] += vi[a] * pw.gw[s][b];
+    for (int c = 0; c < D; ++c) dx[c] += (w * vi[c]) * dt;
+  });
+  for (int c = 0; c < D; ++c) v(p, c) += dv[c];
+  for (int k = 0; k < D * D; ++k) gradv(p, k) = L[k];
+  for (int c = 0; c < D; ++c) X(p, c) += dx[c];
+  // F = (I + ∇v Δt) F
+  T Fo[D * D], Mx[D * D], Fn[D * D];
+  for (int k = 0; k < D * D; ++k) Fo[k] = Ff(p, k);
+  for (int b = 0; b < D; ++b)
+    for (int a = 0; a < D; ++a) Mx[a + D * b] = (a == b ? T(1) : T(0)) + L[a + D * b] * dt;
+  for (int b = 0; b < D; ++b)
+    for (int a = 0; a < D; ++a) {
+      T acc = Mx[a] * Fo[D * b];
+      for (int k = 1; k < D; ++k) acc = acc + Mx[a + D * k] * Fo[k + D * b];
+      Fn[a + D * b] = acc;
+    }
+  for (int k = 0; k < D * D; ++k) Ff(p, k) = Fn[k];
+  // J, ln J, σ in Float64
+  const double J = det_d<T, D>(Fn);
+  double lam = prm.lambda, mu = prm.mu;
+  if (LE.ok()) {
+    double s = std::exp(double(LE(p)));
+    lam *= s;
+    mu *= s;
+  }
+  const double lnJ = std::log(J);
+  for (int b = 0; b < D; ++b)
+    for (int a = 0; a < D; ++a) {
+      double bb = 0;
+      for (int k = 0; k < D; ++k) bb += double(Fn[a + D * k]) * double(Fn[b + D * k]);
+      double s = mu * (bb - (a == b ? 1.0 : 0.0)) + (a == b ? lam * lnJ : 0.0);
+      sg(p, a + D * b) = T(s / J);
+    }
+  V(p) = T(prm.V0 * J);
+}
+
+template <typename T, int D>
+int p2g_seq(int basis, const tess_mesh& tm, const tess_grid& G, const tess_particles& P, const tess_weights* W,
+            const tess_params& prm) {
+  Mesh<T, D> mesh(tm);
+  zero_grid<T, D>(mesh, G);
+  for (int64_t p = 0; p < P.n; ++p) p2g_particle<T, D>(basis, mesh, G, P, W, prm, p);  // p2g.jl:322-324
+  const int64_t nn = mesh.nnodes();
+  for (int64_t i = 0; i < nn; ++i) grid_update_node<T, D>(mesh, G, prm, i);
+  return TESS_OK;
+}
+
+template <typename T, int D>
+int g2p_seq(int basis, const tess_mesh& tm, const tess_grid& G, const tess_particles& P, const tess_weights* W,
+            const tess_params& prm) {
+  Mesh<T, D> mesh(tm);
+  for (int64_t p = 0; p < P.n; ++p) g2p_particle<T, D>(basis, mesh, G, P, W, prm, p);
+  return TESS_OK;
+}
+
+template <typename T, int D>
+int update_weights_impl(int basis, const tess_mesh& tm, const tess_particles& P, const tess_weights& W,
+                        int64_t p0, int64_t p1) {
+  Mesh<T, D> mesh(tm);
+  Fld<T> X(P.x), Lf(P.l), Wf(W.w), Gf(W.grad_w);
+  const int sw = support_width(basis);
+  int64_t nsl = 1;
+  for (int d = 0; d < D; ++d) nsl *= sw;
+  for (int64_t p = p0; p < p1; ++p) {
+    T x[D];
+    load_vec<T, D>(X, p, x);
+    PW<T, D> pw;
+    eval_weights<T, D>(basis, x, Lf.ok() ? Lf(p) : T(0), mesh, false, pw);
+    int64_t* win = W.window + p * 2 * D;
+    for (int d = 0; d < D; ++d) {
+      if (pw.empty) { win[2 * d] = 1; win[2 * d + 1] = 0; }
+      else { win[2 * d] = pw.lo[d] + 1; win[2 * d + 1] = pw.hi[d] + 1; }
+    }
+    if (pw.empty) continue;
+    for_window<T, D>(pw, mesh, [&](int s, int64_t) {
+      Wf(s + nsl * p) = pw.w[s];
+      for (int c = 0; c < D; ++c) Gf(s + nsl * p, c) = pw.gw[s][c];
+    });
+  }
+  return TESS_OK;
+}
+
+// ---- CPU block partition (cpu_block_strategy.jl:62-219) --------------------
+
+template <typename T, int D>
+int64_t findblock_lin(const Mesh<T, D>& mesh, const T* x, int64_t* nb) {  // cartesian.jl:277-285
+  int64_t b[D];
+  for (int d = 0; d < D; ++d) {
+    T f = std::floor((x[d] - mesh.xmin[d]) * mesh.h_inv);
+    int64_t c0 = int64_t(f);
+    if (!(0 <= c0 && c0 <= mesh.dims[d] - 2)) return -1;
+    b[d] = c0 >> mesh.L;
+  }
+  int64_t l = 0;
+  for (int d = D - 1; d >= 0; --d) l = l * nb[d] + b[d];
+  return l;
+}
+
+template <int D>
+int block_color(int64_t lin, const int64_t* nb) {  // :197-203, 0-based colour
+  int c = 0;
+  for (int d = 0; d < D; ++d) {
+    int64_t bd = lin % nb[d];
+    lin /= nb[d];
+    c += int(bd & 1) << d;
+  }
+  return c;
+}
+
+struct PartitionOut {
+  std::vector<int32_t> pidx;           // 0-based particle ids in block order
+  std::vector<int64_t> start, stop;    // per block [start, stop) into pidx, start==stop if empty
+  std::vector<std::vector<int64_t>> groups;  // active blocks per colour, linear order
+  int64_t nassigned = 0;
+};
+
+template <typename T, int D>
+void partition_impl(const Mesh<T, D>& mesh, const Fld<T>& X, int64_t n, PartitionOut& out) {
+  int64_t nb[D], nbt = 1;
+  for (int d = 0; d < D; ++d) { nb[d] = mesh.nblocks(d); nbt *= nb[d]; }
+  std::vector<int64_t> blk(n), count(nbt, 0);
+  for (int64_t p = 0; p < n; ++p) {
+    T x[D];
+    load_vec<T, D>(X, p, x);
+    blk[p] = findblock_lin<T, D>(mesh, x, nb);
+    if (blk[p] >= 0) count[blk[p]]++;
+  }
+  out.groups.assign(1 << D, {});
+  for (int64_t b = 0; b < nbt; ++b)
+    if (count[b]) out.groups[block_color<D>(b, nb)].push_back(b);
+  out.start.assign(nbt, 0);
+  out.stop.assign(nbt, 0);
+  int64_t assigned = 0;
+  for (auto& g : out.groups)
+    for (int64_t b : g) { out.start[b] = assigned; assigned += count[b]; out.stop[b] = assigned; }
+  out.nassigned = assigned;
+  out.pidx.assign(assigned, 0);
+  std::vector<int64_t> cursor(out.start);
+  for (int64_t p = 0; p < n; ++p)
+    if (blk[p] >= 0) out.pidx[cursor[blk[p]]++] = int32_t(p);
+}
+
+// ---- threaded CPU baseline (region.jl:207-256, g2p.jl:104-108) -------------
+
+template <typename Fn>
+void parallel_for(int nthreads, int64_t n, Fn&& fn) {
+  if (nthreads <= 1 || n < 2) { fn(int64_t(0), n); return; }
+  std::vector<std::thread> th;
+  int64_t chunk = (n + nthreads - 1) / nthreads;
+  for (int t = 0; t < nthreads; ++t) {
+    int64_t a = t * chunk, b = std::min(n, a + chunk);
+    if (a >= b) break;
+    th.emplace_back([=, &fn] { fn(a, b); });
+  }
+  for (auto& x : th) x.join();
+}
+
+template <typename T, int D>
+int step_threaded_impl(int basis, const tess_mesh& tm, const tess_grid& G, const tess_particles& P,
+                       const tess_params& prm, int nthreads, int nsteps, tess_weights& W) {
+  Mesh<T, D> mesh(tm);
+  Fld<T> X(P.x);
+  if (support_width(basis) > (1 << mesh.L))
+    return fail(TESS_ERR_STATE, "Block size for Partition is too small for the basis");
+  const int64_t n = P.n, nn = mesh.nnodes();
+  for (int s = 0; s < nsteps; ++s) {
+    // update!(wgts, pcls, mesh): @threaded over particles, stored weights
+    parallel_for(nthreads, n, [&](int64_t a, int64_t b) { update_weights_impl<T, D>(basis, tm, P, W, a, b); });
+    // update!(partition, x)
+    PartitionOut part;
+    partition_impl<T, D>(mesh, X, n, part);
+    // @P2G with partition: parallel zero, one phase per colour, grid half
+    parallel_for(nthreads, nn, [&](int64_t a, int64_t b) {
+      Fld<T> m(G.m), mv(G.mv), f(G.f);
+      for (int64_t i = a; i < b; ++i) {
+        m(i) = T(0);
+        for (int c = 0; c < D; ++c) { mv(i, c) = T(0); f(i, c) = T(0); }
+      }
+    });
+    for (auto& group : part.groups) {
+      parallel_for(nthreads, int64_t(group.size()), [&](int64_t a, int64_t b) {
+        for (int64_t k = a; k < b; ++k) {
+          int64_t blk = group[k];
+          for (int64_t j = part.start[blk]; j < part.stop[blk]; ++j)
+            p2g_particle<T, D>(basis, mesh, G, P, &W, prm, part.pidx[j]);
+        }
+      });
+    }
+    parallel_for(nthreads, nn, [&](int64_t a, int64_t b) {
+      for (int64_t i = a; i < b; ++i) grid_update_node<T, D>(mesh, G, prm, i);
+    });
+    // @G2P: threaded over particles
+    parallel_for(nthreads, n, [&](int64_t a, int64_t b) {
+      for (int64_t p = a; p < b; ++p) g2p_particle<T, D>(basis, mesh, G, P, &W, prm, p);
+    });
+  }
+  return TESS_OK;
+}
+
+template <template <typename, int> class Fn, typename... Args>
+int dispatch(const tess_mesh* m, Args&&... args) {
+  if (!m) return fail(TESS_ERR_ARGUMENT, "mesh is NULL");
+  if (m->dim == 2 && m->dtype == TESS_FLOAT64) return Fn<double, 2>::run(*m, args...);
+  if (m->dim == 3 && m->dtype == TESS_FLOAT64) return Fn<double, 3>::run(*m, args...);
+  if (m->dim == 2 && m->dtype == TESS_FLOAT32) return Fn<float, 2>::run(*m, args...);
+  if (m->dim == 3 && m->dtype == TESS_FLOAT32) return Fn<float, 3>::run(*m, args...);
+  return fail(TESS_ERR_UNSUPPORTED, "unsupported dim/dtype");
+}
+
+bool basis_ok(int basis) {
+  return basis == TESS_BSPLINE_LINEAR || basis == TESS_BSPLINE_QUADRATIC || basis == TESS_BSPLINE_CUBIC ||
+         basis == TESS_UGIMP;
+}
+
+template <typename T, int D>
+struct RunP2G {
+  static int run(const tess_mesh& m, int basis, const tess_grid* G, const tess_particles* P, const tess_weights* W,
+                 const tess_params* prm) {
+    return p2g_seq<T, D>(basis, m, *G, *P, W, *prm);
+  }
+};
+template <typename T, int D>
+struct RunG2P {
+  static int run(const tess_mesh& m, int basis, const tess_grid* G, const tess_particles* P, const tess_weights* W,
+                 const tess_params* prm) {
+    return g2p_seq<T, D>(basis, m, *G, *P, W, *prm);
+  }
+};
+template <typename T, int D>
+struct RunStep {
+  static int run(const tess_mesh& m, int basis, const tess_grid* G, const tess_particles* P, const tess_params* prm,
+                 int nsteps) {
+    for (int s = 0; s < nsteps; ++s) {
+      p2g_seq<T, D>(basis, m, *G, *P, nullptr, *prm);
+      g2p_seq<T, D>(basis, m, *G, *P, nullptr, *prm);
+    }
+    return TESS_OK;
+  }
+};
+template <typename T, int D>
+struct RunWeights {
+  static int run(const tess_mesh& m, int basis, const tess_particles* P, const tess_weights* W) {
+    return update_weights_impl<T, D>(basis, m, *P, *W, 0, P->n);
+  }
+};
+template <typename T, int D>
+struct RunPartition {
+  static int run(const tess_mesh& m, tess_field x, int64_t n, int32_t* pidx, int32_t* starts, int32_t* stops,
+                 int64_t* nassigned) {
+    Mesh<T, D> mesh(m);
+    PartitionOut out;
+    partition_impl<T, D>(mesh, Fld<T>(x), n, out);
+    for (int64_t j = 0; j < out.nassigned; ++j) pidx[j] = out.pidx[j] + 1;  // 1-based ids
+    for (size_t b = 0; b < out.start.size(); ++b) {
+      bool nonempty = out.stop[b] > out.start[b];
+      starts[b] = nonempty ? int32_t(out.start[b] + 1) : 0;
+      stops[b] = nonempty ? int32_t(out.stop[b]) : 0;
+    }
+    *nassigned = out.nassigned;
+    return TESS_OK;
+  }
+};
+template <typename T, int D>
+struct RunThreaded {
+  static int run(const tess_mesh& m, int basis, const tess_grid* G, const tess_particles* P, const tess_params* prm,
+                 int nthreads, int nsteps, tess_weights* W) {
+    return step_threaded_impl<T, D>(basis, m, *G, *P, *prm, nthreads, nsteps, *W);
+  }
+};
+template <typename T, int D>
+struct RunJet {
+  static int run(const tess_mesh& m, int basis, const double* x, double l, int deferred, int64_t* window,
+                 double* w, double* gw) {
+    Mesh<T, D> mesh(m);
+    T xt[D];
+    for (int d = 0; d < D; ++d) xt[d] = T(x[d]);
+    PW<T, D> pw;
+    eval_weights<T, D>(basis, xt, T(l), mesh, deferred != 0, pw);
+    for (int d = 0; d < D; ++d) {
+      window[2 * d] = pw.empty ? 1 : pw.lo[d] + 1;
+      window[2 * d + 1] = pw.empty ? 0 : pw.hi[d] + 1;
+    }
+    int64_t nsl = 1;
+    for (int d = 0; d < D; ++d) nsl *= pw.sw;
+    for (int64_t s = 0; s < nsl; ++s) { w[s] = 0; for (int c = 0; c < D; ++c) gw[s * D + c] = 0; }
+    if (!pw.empty)
+      for_window<T, D>(pw, mesh, [&](int s, int64_t) {
+        w[s] = double(pw.w[s]);
+        for (int c = 0; c < D; ++c) gw[s * D + c] = double(pw.gw[s][c]);
+      });
+    return TESS_OK;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* oref_last_error(void) { return g_err.c_str(); }
+
+// One particle's BasisWeight (update!(bw, pt, mesh), basis_weight.jl:634-654):
+// window as dim (start, stop) 1-based pairs, w and ∇w over the sw^dim storage
+// (entries outside the window are zero).  deferred=1 gives the transfer-time
+// (nodewise) evaluation.
+int oref_basis_weight(const tess_mesh* m, int basis, const double* x, double l, int deferred, int64_t* window,
+                      double* w, double* gw) {
+  if (!basis_ok(basis)) return fail(TESS_ERR_UNSUPPORTED, "basis");
+  return dispatch<RunJet>(m, basis, x, l, deferred, window, w, gw);
+}
+
+// 1-D jet(Order(1), kernel, ξ[, l]) (basis_weight.jl:35-37).
+int oref_jet1d(int basis, double xi, double l, double* val, double* der) {
+  if (!basis_ok(basis)) return fail(TESS_ERR_UNSUPPORTED, "basis");
+  piecewise_jet<double>(basis, xi, l, val, der);
+  return TESS_OK;
+}
+
+// findblock(x, mesh) (cartesian.jl:277-285): 1-based Cartesian block index, or
+// all zeros when x is outside.
+int oref_findblock(const tess_mesh* m, const double* x, int64_t* blk) {
+  if (!m) return fail(TESS_ERR_ARGUMENT, "mesh");
+  const int D = m->dim;
+  for (int d = 0; d < D; ++d) {
+    double f = std::floor((x[d] - m->xmin[d]) * m->h_inv);
+    int64_t c0 = int64_t(f);
+    if (!(0 <= c0 && c0 <= m->dims[d] - 2)) {
+      for (int e = 0; e < D; ++e) blk[e] = 0;
+      return TESS_OK;
+    }
+    blk[d] = (c0 >> m->block_size_log2) + 1;
+  }
+  return TESS_OK;
+}
+
+// supportnodes(x, r, mesh) (cartesian.jl:190-199), 1-based ranges.
+int oref_supportnodes_r(const tess_mesh* m, const double* x, double r, int64_t* window) {
+  const int D = m->dim;
+  bool inside = true;
+  double xi[3];
+  for (int d = 0; d < D; ++d) {
+    xi[d] = (x[d] - m->xmin[d]) * m->h_inv;
+    double f = std::floor(xi[d]);
+    if (!(0 <= f && f <= double(m->dims[d] - 2))) inside = false;
+  }
+  for (int d = 0; d < D; ++d) {
+    if (!inside) { window[2 * d] = 1; window[2 * d + 1] = 0; continue; }
+    int64_t s = int64_t(std::floor(xi[d] - r)) + 2, e = int64_t(std::floor(xi[d] + r)) + 1;
+    window[2 * d] = std::max<int64_t>(s, 1);
+    window[2 * d + 1] = std::min<int64_t>(e, m->dims[d]);
+  }
+  return TESS_OK;
+}
+
+int oref_update_weights(int basis, const tess_mesh* m, const tess_particles* P, const tess_weights* W) {
+  if (!basis_ok(basis)) return fail(TESS_ERR_UNSUPPORTED, "basis");
+  return dispatch<RunWeights>(m, basis, P, W);
+}
+
+int oref_partition_update(const tess_mesh* m, tess_field x, int64_t n, int32_t* pidx, int32_t* starts,
+                          int32_t* stops, int64_t* nassigned) {
+  return dispatch<RunPartition>(m, x, n, pidx, starts, stops, nassigned);
+}
+
+int oref_p2g(int basis, const tess_mesh* m, const tess_grid* G, const tess_particles* P, const tess_weights* W,
+             const tess_params* prm) {
+  if (!basis_ok(basis)) return fail(TESS_ERR_UNSUPPORTED, "basis");
+  return dispatch<RunP2G>(m, basis, G, P, W, prm);
+}
+
+int oref_g2p(int basis, const tess_mesh* m, const tess_grid* G, const tess_particles* P, const tess_weights* W,
+             const tess_params* prm) {
+  if (!basis_ok(basis)) return fail(TESS_ERR_UNSUPPORTED, "basis");
+  return dispatch<RunG2P>(m, basis, G, P, W, prm);
+}
+
+// nsteps of the README step with deferred weights, sequential order.
+int oref_step(int basis, const tess_mesh* m, const tess_grid* G, const tess_particles* P, const tess_params* prm,
+              int nsteps) {
+  if (!basis_ok(basis)) return fail(TESS_ERR_UNSUPPORTED, "basis");
+  return dispatch<RunStep>(m, basis, G, P, prm, nsteps);
+}
+
+// The reference CPU algorithm with threads: stored weights, CPU block
+// partition, colour-phased P2G, threaded G2P.  W is caller-provided storage.
+int oref_step_threaded(int basis, const tess_mesh* m, const tess_grid* G, const tess_particles* P,
+                       const tess_params* prm, int nthreads, int nsteps, tess_weights* W) {
+  if (!basis_ok(basis)) return fail(TESS_ERR_UNSUPPORTED, "basis");
+  return dispatch<RunThreaded>(m, basis, G, P, prm, nthreads, nsteps, W);
+}
+
+}  // extern "C"

@@ -1,0 +1,273 @@
+// C ABI of libtesserae_hip: argument checks (mirroring the host-side checks of
+// src/Transfer/runtime.jl:177-229 and src/Basis/basis_weight.jl:714-752),
+// context management, and conversion of internal errors into tess_status +
+// a thread-local message.  No exception crosses this boundary.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <new>
+#include <string>
+
+#include "tess_internal.hpp"
+
+namespace tess {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void raise(tess_status code, const std::string& msg) { throw Error{code, msg}; }
+
+void check_hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) raise(TESS_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void* Buffer::get(size_t need) {
+  if (need == 0) need = 1;
+  if (need <= bytes) return ptr;
+  if (ptr) {
+    // the previous contents are dropped: callers treat scratch as uninitialised
+    TESS_HIP(hipDeviceSynchronize());
+    TESS_HIP(hipFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+  }
+  const size_t cap = need + need / 8;
+  TESS_HIP(hipMalloc(&ptr, cap));
+  bytes = cap;
+  return ptr;
+}
+
+void Buffer::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+namespace {
+
+template <typename F>
+tess_status guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return TESS_OK;
+  } catch (const Error& e) {
+    g_last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return TESS_ERR_DEVICE;
+  } catch (...) {
+    g_last_error = "unexpected internal error";
+    return TESS_ERR_STATE;
+  }
+}
+
+void check_ctx(const tess_ctx* ctx) {
+  if (!ctx) raise(TESS_ERR_ARGUMENT, "context is NULL");
+  TESS_HIP(hipSetDevice(ctx->device));
+}
+
+void check_mesh(const tess_mesh* m) {
+  if (!m) raise(TESS_ERR_ARGUMENT, "mesh is NULL");
+  if (m->dim != 2 && m->dim != 3) raise(TESS_ERR_UNSUPPORTED, "mesh dimension must be 2 or 3");
+  if (m->dtype != TESS_FLOAT32 && m->dtype != TESS_FLOAT64) raise(TESS_ERR_ARGUMENT, "mesh dtype must be Float32 or Float64");
+  if (m->block_size_log2 < 0)
+    raise(TESS_ERR_ARGUMENT, "block_size_log2 must be non-negative, got " + std::to_string(m->block_size_log2));
+  for (int d = 0; d < m->dim; ++d)
+    if (m->dims[d] < 2) raise(TESS_ERR_ARGUMENT, "a CartesianMesh needs at least 2 nodes per axis");
+  if (!(m->h > 0) || !std::isfinite(m->h)) raise(TESS_ERR_ARGUMENT, "mesh spacing must be positive");
+  if (!(m->h_inv > 0) || !std::isfinite(m->h_inv)) raise(TESS_ERR_ARGUMENT, "mesh inverse spacing must be positive");
+}
+
+void check_basis(int32_t basis) {
+  if (basis != TESS_BSPLINE_LINEAR && basis != TESS_BSPLINE_QUADRATIC && basis != TESS_BSPLINE_CUBIC &&
+      basis != TESS_UGIMP)
+    raise(TESS_ERR_UNSUPPORTED, "basis is not one of BSpline(Linear/Quadratic/Cubic) or uGIMP");
+}
+
+void need(const tess_field& f, const char* what) {
+  if (!f.ptr) raise(TESS_ERR_ARGUMENT, std::string("missing field ") + what);
+}
+
+void check_particles(const tess_particles* P, int32_t basis, bool transfer) {
+  if (!P) raise(TESS_ERR_ARGUMENT, "particles is NULL");
+  if (P->n < 0) raise(TESS_ERR_DIMENSION, "negative particle count");
+  need(P->x, "x");
+  if (transfer) {
+    need(P->v, "v");
+    need(P->V, "V");
+    need(P->grad_v, "∇v");
+    need(P->F, "F");
+    need(P->sigma, "σ");
+  }
+  if (basis == TESS_UGIMP && !P->l.ptr)
+    raise(TESS_ERR_ARGUMENT, "uGIMP requires the initial particle length `l` in the particle property");
+}
+
+void check_grid(const tess_grid* G) {
+  if (!G) raise(TESS_ERR_ARGUMENT, "grid is NULL");
+  need(G->m, "m");
+  need(G->mv, "mv");
+  need(G->f, "f");
+  need(G->vn, "vⁿ");
+  need(G->v, "v");
+}
+
+void check_weights(const tess_weights* W) {
+  if (!W) return;
+  if (!W->window) raise(TESS_ERR_ARGUMENT, "stored weights need the support-window array");
+  need(W->w, "w");
+  need(W->grad_w, "∇w");
+}
+
+void check_params(const tess_params* p) {
+  if (!p) raise(TESS_ERR_ARGUMENT, "params is NULL");
+  if (!std::isfinite(p->dt)) raise(TESS_ERR_ARGUMENT, "Δt must be finite");
+}
+
+}  // namespace
+}  // namespace tess
+
+using namespace tess;
+
+extern "C" {
+
+int32_t tess_abi_version(void) { return TESS_ABI_VERSION; }
+
+const char* tess_last_error(void) { return g_last_error.c_str(); }
+
+tess_status tess_ctx_create(int32_t device, void* stream, tess_ctx** out) {
+  return guarded([&] {
+    if (!out) raise(TESS_ERR_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    TESS_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) raise(TESS_ERR_ARGUMENT, "no such HIP device");
+    TESS_HIP(hipSetDevice(device));
+    tess_ctx* c = new tess_ctx();
+    c->device = device;
+    c->stream = static_cast<hipStream_t>(stream);
+    void* pinned = nullptr;
+    hipError_t e = hipHostMalloc(&pinned, 64 * sizeof(int64_t), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      delete c;
+      check_hip(e, "hipHostMalloc");
+    }
+    c->counters_host = static_cast<int64_t*>(pinned);
+    *out = c;
+  });
+}
+
+tess_status tess_ctx_destroy(tess_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    else (void)hipDeviceSynchronize();
+    tess::Buffer* bufs[] = {&ctx->keys,      &ctx->keys_sorted, &ctx->vals,         &ctx->keys_alt,
+                            &ctx->vals_alt,  &ctx->hist,        &ctx->scan_tmp,     &ctx->tile_start,
+                            &ctx->tile_end,  &ctx->tile_flag,   &ctx->tile_slot,    &ctx->active_tiles,
+                            &ctx->nb_flag,   &ctx->nb_slot,     &ctx->nb_list,      &ctx->nb_list_prev,
+                            &ctx->partials,  &ctx->counters,    &ctx->host_pinned_dummy, &ctx->reorder_tmp};
+    for (auto* b : bufs) b->release();
+    if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
+    delete ctx;
+  });
+}
+
+tess_status tess_sync(tess_ctx* ctx) {
+  return guarded([&] {
+    check_ctx(ctx);
+    TESS_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+tess_status tess_update_weights(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_particles* particles,
+                                const tess_weights* weights) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    check_basis(basis);
+    check_particles(particles, basis, false);
+    if (!weights) raise(TESS_ERR_ARGUMENT, "weights is NULL");
+    check_weights(weights);
+    launch_update_weights(ctx, basis, *mesh, *particles, *weights);
+  });
+}
+
+tess_status tess_partition_update(tess_ctx* ctx, const tess_mesh* mesh, tess_field x, int64_t n,
+                                  int32_t* particleindices, int32_t* starts, int32_t* stops, int64_t* nassigned) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    if (n < 0) raise(TESS_ERR_DIMENSION, "negative particle count");
+    if (!x.ptr && n > 0) raise(TESS_ERR_ARGUMENT, "missing field x");
+    if (!starts || !stops || !nassigned || (!particleindices && n > 0))
+      raise(TESS_ERR_ARGUMENT, "partition output arrays must not be NULL");
+    run_partition_update(ctx, *mesh, x, n, particleindices, starts, stops, nassigned);
+  });
+}
+
+tess_status tess_reorder_particles(tess_ctx* ctx, const tess_mesh* mesh, const tess_particles* particles,
+                                   int32_t* particleindices, int64_t nassigned, double threshold,
+                                   int32_t* reordered) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    if (!particles) raise(TESS_ERR_ARGUMENT, "particles is NULL");
+    if (!reordered) raise(TESS_ERR_ARGUMENT, "reordered is NULL");
+    if (nassigned < 0) raise(TESS_ERR_DIMENSION, "negative nassigned");
+    if (nassigned > 0 && !particleindices) raise(TESS_ERR_ARGUMENT, "particleindices is NULL");
+    run_reorder(ctx, *mesh, *particles, particleindices, nassigned, threshold, reordered);
+  });
+}
+
+tess_status tess_p2g(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                     const tess_particles* particles, const tess_weights* weights, const tess_params* params) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    check_basis(basis);
+    check_grid(grid);
+    check_particles(particles, basis, true);
+    check_weights(weights);
+    check_params(params);
+    run_p2g(ctx, basis, *mesh, *grid, *particles, weights, *params);
+  });
+}
+
+tess_status tess_g2p(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                     const tess_particles* particles, const tess_weights* weights, const tess_params* params) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    check_basis(basis);
+    check_grid(grid);
+    check_particles(particles, basis, true);
+    check_weights(weights);
+    check_params(params);
+    run_g2p(ctx, basis, *mesh, *grid, *particles, weights, *params, false);
+  });
+}
+
+tess_status tess_step(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                      const tess_particles* particles, const tess_params* params, int32_t nsteps) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    check_basis(basis);
+    check_grid(grid);
+    check_particles(particles, basis, true);
+    check_params(params);
+    if (nsteps < 0) raise(TESS_ERR_ARGUMENT, "nsteps must be non-negative");
+    for (int32_t s = 0; s < nsteps; ++s) {
+      run_p2g(ctx, basis, *mesh, *grid, *particles, nullptr, *params);
+      run_g2p(ctx, basis, *mesh, *grid, *particles, nullptr, *params, true);
+    }
+  });
+}
+
+}  // extern "C"

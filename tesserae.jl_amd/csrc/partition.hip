@@ -1,0 +1,419 @@
+// Stored basis weights in the reference layout, the CPU block partition
+// reproduced on the device, and particle reordering.
+//   update!(weights, particles, mesh)  src/Basis/basis_weight.jl:714-752
+//   update!(partition, x)              src/Parallel/cpu_block_strategy.jl:62-219
+//   reorder_particles!                 src/Parallel/cpu_block_strategy.jl:248-355
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "tess_device.hpp"
+#include "tess_internal.hpp"
+
+namespace tess {
+namespace {
+
+template <int N, int E>
+struct IPow {
+  static constexpr int v = N * IPow<N, E - 1>::v;
+};
+template <int N>
+struct IPow<N, 0> {
+  static constexpr int v = 1;
+};
+
+template <typename T, int D>
+struct WArgs {
+  DMesh<T, D> mesh;
+  DField<T> x, l, w, gw;
+  int64_t* window;
+  int64_t n;
+  uint32_t* err;
+};
+
+// Per particle: support window, then either the full-support polynomial fill
+// (bspline.jl:146-170) or the nodewise jet (basis_weight.jl:624-632).
+template <typename T, int D, int BASIS>
+__global__ __launch_bounds__(256) void k_update_weights(WArgs<T, D> a) {
+  constexpr int SW = support_width(BASIS);
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  T x[D], xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    x[d] = a.x(p, d);
+    xi[d] = (x[d] - a.mesh.xmin[d]) * a.mesh.h_inv;
+  }
+  int64_t* win = a.window + p * 2 * D;
+  if (!inside_xi<T, D>(xi, a.mesh)) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      win[2 * d] = 1;
+      win[2 * d + 1] = 0;
+    }
+    return;
+  }
+  T l = T(0);
+  if constexpr (BASIS == TESS_UGIMP) {
+    l = a.l(p) / a.mesh.h;
+    if (!(T(0) <= l && l <= T(1))) {
+      atomicOr(a.err, 1u);
+      return;
+    }
+  }
+  int lo[D], hi[D];
+  bool full = BASIS != TESS_UGIMP;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    axis_window<T, BASIS>(xi[d], l, a.mesh.dims[d], lo[d], hi[d]);
+    full = full && (hi[d] - lo[d] + 1 == SW);
+  }
+  T val[D][SW], der[D][SW];
+  if (full) {
+    if constexpr (BASIS != TESS_UGIMP) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const T y = (degree_of(BASIS) % 2 == 1) ? xi[d] : xi[d] - T(0.5);
+        bspline_local_jet<T, BASIS>(y - floor(y), val[d], der[d]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+      for (int k = 0; k < SW; ++k) {
+        if (k <= hi[d] - lo[d]) {
+          const T xin = (x[d] - a.mesh.node(d, lo[d] + k)) * a.mesh.h_inv;
+          piecewise_jet<T, BASIS>(xin, l, val[d][k], der[d][k]);
+        } else {
+          val[d][k] = T(0);
+          der[d][k] = T(0);
+        }
+      }
+  }
+  const T hinv = a.mesh.h_inv;
+#pragma unroll
+  for (int o = 0; o < IPow<SW, D>::v; ++o) {
+    int aa[D], r = o;
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      aa[d] = r % SW;
+      r /= SW;
+      ok = ok && aa[d] <= hi[d] - lo[d];
+    }
+    if (!ok) continue;
+    T w = val[0][aa[0]];
+#pragma unroll
+    for (int d = 1; d < D; ++d) w = w * val[d][aa[d]];
+    const int64_t e = int64_t(o) + int64_t(IPow<SW, D>::v) * p;
+    a.w(e) = w;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      T g = (c == 0) ? der[0][aa[0]] : val[0][aa[0]];
+#pragma unroll
+      for (int d = 1; d < D; ++d) g = g * ((d == c) ? der[d][aa[d]] : val[d][aa[d]]);
+      a.gw(e, c) = g * hinv;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    win[2 * d] = lo[d] + 1;
+    win[2 * d + 1] = hi[d] + 1;
+  }
+}
+
+// ---- partition ----------------------------------------------------------------
+
+// key = colour * nblocks + linear block id (cpu_block_strategy.jl:197-219: groups
+// by colour, blocks in linear order inside a group), or `outside` past the end.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_partition_keys(DMesh<T, D> mesh, DField<T> X, int64_t n, int L,
+                                                         int3 nb, uint32_t nbt, uint32_t* __restrict__ keys) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int nbd[3] = {nb.x, nb.y, nb.z};
+  uint32_t lin = 0, color = 0;
+  bool inside = true;
+  int b[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const T f = floor((X(p, d) - mesh.xmin[d]) * mesh.h_inv);  // _findblock, cartesian.jl:277-285
+    inside = inside && T(0) <= f && f <= T(mesh.dims[d] - 2);
+    b[d] = inside ? (int(f) >> L) : 0;
+  }
+  if (!inside) {
+    keys[p] = (uint32_t(1) << D) * nbt;
+    return;
+  }
+#pragma unroll
+  for (int d = D - 1; d >= 0; --d) lin = lin * uint32_t(nbd[d]) + uint32_t(b[d]);
+#pragma unroll
+  for (int d = 0; d < D; ++d) color += uint32_t(b[d] & 1) << d;  // block_color, :197-203
+  keys[p] = color * nbt + lin;
+}
+
+__global__ void k_partition_runs(const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ vals, int64_t n,
+                                 uint32_t nbt, uint32_t outside, int32_t* __restrict__ pidx,
+                                 int32_t* __restrict__ starts, int32_t* __restrict__ stops,
+                                 uint32_t* __restrict__ count) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = skeys[i];
+  const bool first = i == 0 || skeys[i - 1] != k;
+  const bool last = i == n - 1 || skeys[i + 1] != k;
+  if (k < outside) {
+    const uint32_t blk = k % nbt;
+    pidx[i] = int32_t(vals[i]) + 1;
+    if (first) starts[blk] = int32_t(i) + 1;
+    if (last) stops[blk] = int32_t(i) + 1;
+    if (i == n - 1) *count = uint32_t(n);
+  } else if (first) {
+    *count = uint32_t(i);
+  }
+}
+
+// ---- reorder ------------------------------------------------------------------
+
+__global__ void k_contiguity(const int32_t* __restrict__ pidx, int64_t na, uint32_t* __restrict__ count) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x + 1;
+  const bool hit = i < na && pidx[i] == pidx[i - 1] + 1;
+  const uint64_t m = __ballot(hit);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, uint32_t(__popcll(m)));
+}
+
+__global__ void k_mark_seen(const int32_t* __restrict__ pidx, int64_t na, int64_t n, uint32_t* __restrict__ seen,
+                            uint32_t* __restrict__ err) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= na) return;
+  const int64_t p = int64_t(pidx[i]) - 1;
+  if (p < 0 || p >= n) {
+    atomicOr(err, 1u);
+    return;
+  }
+  if (atomicAdd(&seen[p], 1u) != 0) atomicOr(err, 2u);
+}
+
+__global__ void k_build_perm(const int32_t* __restrict__ pidx, int64_t na, int64_t n,
+                             const uint32_t* __restrict__ seen, const uint32_t* __restrict__ unseen_pos,
+                             uint32_t* __restrict__ perm) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < na) perm[i] = uint32_t(pidx[i] - 1);
+  if (i < n && !seen[i]) perm[na + unseen_pos[i]] = uint32_t(i);
+}
+
+__global__ void k_not(const uint32_t* __restrict__ seen, uint32_t* __restrict__ out, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = seen[i] ? 0u : 1u;
+}
+
+template <typename U>
+__global__ void k_gather_field(const U* __restrict__ src, int64_t es, int64_t cs, int ncomp,
+                               const uint32_t* __restrict__ perm, int64_t n, U* __restrict__ tmp) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t p = perm[k];
+  for (int c = 0; c < ncomp; ++c) tmp[c * n + k] = src[p * es + c * cs];
+}
+
+template <typename U>
+__global__ void k_scatter_back(U* __restrict__ dst, int64_t es, int64_t cs, int ncomp, const U* __restrict__ tmp,
+                               int64_t n) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  for (int c = 0; c < ncomp; ++c) dst[k * es + c * cs] = tmp[c * n + k];
+}
+
+__global__ void k_iota1(int32_t* __restrict__ pidx, int64_t na) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < na) pidx[i] = int32_t(i) + 1;
+}
+
+inline unsigned nblk(int64_t n, int t) { return unsigned(std::max<int64_t>(1, ceil_div(n, t))); }
+
+template <typename T, int D>
+void update_weights_typed(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_particles& P,
+                          const tess_weights& W) {
+  WArgs<T, D> a;
+  a.mesh = dmesh<T, D>(mesh);
+  a.x = dfield<T>(P.x);
+  a.l = dfield<T>(P.l);
+  a.w = dfield<T>(W.w);
+  a.gw = dfield<T>(W.grad_w);
+  a.window = W.window;
+  a.n = P.n;
+  uint32_t* err = ctx->counters.as<uint32_t>(4);
+  a.err = err;
+  TESS_HIP(hipMemsetAsync(err, 0, sizeof(uint32_t), ctx->stream));
+  if (P.n == 0) return;
+  const dim3 g(nblk(P.n, 256)), b(256);
+  switch (basis) {
+    case TESS_BSPLINE_LINEAR:
+      hipLaunchKernelGGL((k_update_weights<T, D, TESS_BSPLINE_LINEAR>), g, b, 0, ctx->stream, a);
+      break;
+    case TESS_BSPLINE_QUADRATIC:
+      hipLaunchKernelGGL((k_update_weights<T, D, TESS_BSPLINE_QUADRATIC>), g, b, 0, ctx->stream, a);
+      break;
+    case TESS_BSPLINE_CUBIC:
+      hipLaunchKernelGGL((k_update_weights<T, D, TESS_BSPLINE_CUBIC>), g, b, 0, ctx->stream, a);
+      break;
+    default:
+      hipLaunchKernelGGL((k_update_weights<T, D, TESS_UGIMP>), g, b, 0, ctx->stream, a);
+      break;
+  }
+  TESS_HIP(hipGetLastError());
+  if (basis == TESS_UGIMP) {
+    uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
+    TESS_HIP(hipMemcpyAsync(hc, err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    TESS_HIP(hipStreamSynchronize(ctx->stream));
+    if (hc[0]) raise(TESS_ERR_ARGUMENT, "uGIMP requires 0 ≤ pt.l ≤ spacing(mesh)");
+  }
+}
+
+template <typename T, int D>
+void partition_typed(tess_ctx* ctx, const tess_mesh& mesh, tess_field x, int64_t n, int32_t* pidx, int32_t* starts,
+                     int32_t* stops, int64_t* nassigned) {
+  const int L = mesh.block_size_log2;
+  int nbd[3] = {1, 1, 1};
+  uint64_t nbt = 1;
+  for (int d = 0; d < D; ++d) {
+    nbd[d] = int(((mesh.dims[d] - 1) >> L) + 1);  // nblocks, cartesian.jl:241-243
+    nbt *= uint64_t(nbd[d]);
+  }
+  const uint64_t outside = (uint64_t(1) << D) * nbt;
+  if (outside >= uint64_t(UINT32_MAX) || n >= int64_t(INT32_MAX))
+    raise(TESS_ERR_ARGUMENT, "Partition block count exceeds packed block id capacity.");
+  TESS_HIP(hipMemsetAsync(starts, 0, nbt * sizeof(int32_t), ctx->stream));
+  TESS_HIP(hipMemsetAsync(stops, 0, nbt * sizeof(int32_t), ctx->stream));
+  uint32_t* cnt = ctx->counters.as<uint32_t>(4);
+  TESS_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), ctx->stream));
+  if (n > 0) {
+    uint32_t* keys = ctx->keys.as<uint32_t>(size_t(n));
+    uint32_t* skeys = ctx->keys_sorted.as<uint32_t>(size_t(n));
+    uint32_t* vals = ctx->vals.as<uint32_t>(size_t(n));
+    hipLaunchKernelGGL((k_partition_keys<T, D>), dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, dmesh<T, D>(mesh),
+                       dfield<T>(x), n, L, make_int3(nbd[0], nbd[1], nbd[2]), uint32_t(nbt), keys);
+    TESS_HIP(hipGetLastError());
+    int bits = 0;
+    while ((uint64_t(1) << bits) <= outside) ++bits;
+    radix_sort_pairs(ctx, keys, nullptr, skeys, vals, n, bits);
+    hipLaunchKernelGGL(k_partition_runs, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, skeys, vals, n,
+                       uint32_t(nbt), uint32_t(outside), pidx, starts, stops, cnt);
+    TESS_HIP(hipGetLastError());
+    // the internal tile binning used the same buffers
+    ctx->bin.valid = false;
+  }
+  uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
+  TESS_HIP(hipMemcpyAsync(hc, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  TESS_HIP(hipStreamSynchronize(ctx->stream));
+  *nassigned = n > 0 ? int64_t(hc[0]) : 0;
+}
+
+template <typename U>
+void permute_field(tess_ctx* ctx, const tess_field& f, int ncomp, const uint32_t* perm, int64_t n) {
+  if (!f.ptr || n == 0) return;
+  U* tmp = ctx->reorder_tmp.as<U>(size_t(n) * ncomp);
+  hipLaunchKernelGGL((k_gather_field<U>), dim3(nblk(n, 256)), dim3(256), 0, ctx->stream,
+                     static_cast<const U*>(f.ptr), f.elem_stride, f.comp_stride, ncomp, perm, n, tmp);
+  TESS_HIP(hipGetLastError());
+  hipLaunchKernelGGL((k_scatter_back<U>), dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, static_cast<U*>(f.ptr),
+                     f.elem_stride, f.comp_stride, ncomp, tmp, n);
+  TESS_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_update_weights(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_particles& P,
+                           const tess_weights& W) {
+  if (mesh.dim == 2 && mesh.dtype == TESS_FLOAT32) return update_weights_typed<float, 2>(ctx, basis, mesh, P, W);
+  if (mesh.dim == 3 && mesh.dtype == TESS_FLOAT32) return update_weights_typed<float, 3>(ctx, basis, mesh, P, W);
+  if (mesh.dim == 2 && mesh.dtype == TESS_FLOAT64) return update_weights_typed<double, 2>(ctx, basis, mesh, P, W);
+  if (mesh.dim == 3 && mesh.dtype == TESS_FLOAT64) return update_weights_typed<double, 3>(ctx, basis, mesh, P, W);
+  raise(TESS_ERR_UNSUPPORTED, "unsupported (dim, dtype)");
+}
+
+void run_partition_update(tess_ctx* ctx, const tess_mesh& mesh, tess_field x, int64_t n, int32_t* pidx,
+                          int32_t* starts, int32_t* stops, int64_t* nassigned) {
+  if (mesh.dim == 2 && mesh.dtype == TESS_FLOAT32)
+    return partition_typed<float, 2>(ctx, mesh, x, n, pidx, starts, stops, nassigned);
+  if (mesh.dim == 3 && mesh.dtype == TESS_FLOAT32)
+    return partition_typed<float, 3>(ctx, mesh, x, n, pidx, starts, stops, nassigned);
+  if (mesh.dim == 2 && mesh.dtype == TESS_FLOAT64)
+    return partition_typed<double, 2>(ctx, mesh, x, n, pidx, starts, stops, nassigned);
+  if (mesh.dim == 3 && mesh.dtype == TESS_FLOAT64)
+    return partition_typed<double, 3>(ctx, mesh, x, n, pidx, starts, stops, nassigned);
+  raise(TESS_ERR_UNSUPPORTED, "unsupported (dim, dtype)");
+}
+
+void run_reorder(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, int32_t* pidx, int64_t na,
+                 double threshold, int32_t* reordered) {
+  *reordered = 0;
+  const int64_t n = P.n;
+  if (!(threshold >= 0.0 && threshold <= 1.0)) raise(TESS_ERR_ARGUMENT, "threshold must be in [0, 1].");
+  if (threshold == 0.0) return;
+  if (na > n)
+    raise(TESS_ERR_STATE, "reorder_particles!: The block assignment contains more particle IDs than exist");
+  uint32_t* cnt = ctx->counters.as<uint32_t>(4);
+  uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
+  bool go = threshold == 1.0;
+  if (!go) {
+    double score = 1.0;
+    if (na > 1) {
+      TESS_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), ctx->stream));
+      hipLaunchKernelGGL(k_contiguity, dim3(nblk(na - 1, 256)), dim3(256), 0, ctx->stream, pidx, na, cnt);
+      TESS_HIP(hipGetLastError());
+      TESS_HIP(hipMemcpyAsync(hc, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+      TESS_HIP(hipStreamSynchronize(ctx->stream));
+      score = double(hc[0]) / double(na - 1);
+    }
+    go = score < threshold;
+  }
+  if (!go || n == 0) return;
+  uint32_t* perm = ctx->keys_alt.as<uint32_t>(size_t(n));
+  uint32_t* seen = ctx->vals_alt.as<uint32_t>(size_t(n));
+  uint32_t* pos = ctx->keys.as<uint32_t>(size_t(n));
+  TESS_HIP(hipMemsetAsync(seen, 0, size_t(n) * sizeof(uint32_t), ctx->stream));
+  TESS_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), ctx->stream));
+  if (na > 0) {
+    hipLaunchKernelGGL(k_mark_seen, dim3(nblk(na, 256)), dim3(256), 0, ctx->stream, pidx, na, n, seen, cnt);
+    TESS_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_not, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, seen, pos, n);
+  TESS_HIP(hipGetLastError());
+  exclusive_scan_u32(ctx, pos, n, nullptr);
+  TESS_HIP(hipMemcpyAsync(hc, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  TESS_HIP(hipStreamSynchronize(ctx->stream));
+  if (hc[0] & 1u) raise(TESS_ERR_STATE, "reorder_particles!: particle ID is out of range.");
+  if (hc[0] & 2u) raise(TESS_ERR_STATE, "reorder_particles!: particle is duplicated in the block assignment.");
+  hipLaunchKernelGGL(k_build_perm, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, pidx, na, n, seen, pos, perm);
+  TESS_HIP(hipGetLastError());
+  const int D = mesh.dim;
+  if (mesh.dtype == TESS_FLOAT32) {
+    permute_field<uint32_t>(ctx, P.x, D, perm, n);
+    permute_field<uint32_t>(ctx, P.v, D, perm, n);
+    permute_field<uint32_t>(ctx, P.V, 1, perm, n);
+    permute_field<uint32_t>(ctx, P.grad_v, D * D, perm, n);
+    permute_field<uint32_t>(ctx, P.F, D * D, perm, n);
+    permute_field<uint32_t>(ctx, P.sigma, D * D, perm, n);
+    permute_field<uint32_t>(ctx, P.l, 1, perm, n);
+    permute_field<uint32_t>(ctx, P.logE, 1, perm, n);
+  } else {
+    permute_field<uint64_t>(ctx, P.x, D, perm, n);
+    permute_field<uint64_t>(ctx, P.v, D, perm, n);
+    permute_field<uint64_t>(ctx, P.V, 1, perm, n);
+    permute_field<uint64_t>(ctx, P.grad_v, D * D, perm, n);
+    permute_field<uint64_t>(ctx, P.F, D * D, perm, n);
+    permute_field<uint64_t>(ctx, P.sigma, D * D, perm, n);
+    permute_field<uint64_t>(ctx, P.l, 1, perm, n);
+    permute_field<uint64_t>(ctx, P.logE, 1, perm, n);
+  }
+  if (na > 0) {
+    hipLaunchKernelGGL(k_iota1, dim3(nblk(na, 256)), dim3(256), 0, ctx->stream, pidx, na);
+    TESS_HIP(hipGetLastError());
+  }
+  ctx->bin.valid = false;
+  *reordered = 1;
+}
+
+}  // namespace tess

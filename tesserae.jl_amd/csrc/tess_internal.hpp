@@ -1,0 +1,105 @@
+// Host-side internals of libtesserae_hip: the context (stream + scratch owned by
+// the library), error reporting across the C ABI, and the launchers each
+// kernel file provides.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/tesserae_hip.h"
+
+namespace tess {
+
+// Error raised inside the library; converted to a tess_status + message at the
+// C boundary (nothing propagates across it).
+struct Error {
+  tess_status code;
+  std::string msg;
+};
+
+[[noreturn]] void raise(tess_status code, const std::string& msg);
+void check_hip(hipError_t e, const char* what);
+#define TESS_HIP(call) ::tess::check_hip((call), #call)
+
+// A device buffer owned by the context; grows on demand, never shrinks.
+struct Buffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void* get(size_t need);  // (re)allocates when need > bytes; contents undefined
+  void release();
+  template <typename U>
+  U* as(size_t count) { return static_cast<U*>(get(count * sizeof(U))); }
+};
+
+// Binning of one particle set into the P2G/G2P tiles, valid until the
+// particles move.  All arrays live in ctx scratch.
+struct Binning {
+  bool valid = false;
+  const void* x_ptr = nullptr;  // identity of the particle set it was built for
+  int64_t n = 0;
+  int32_t dim = 0, dtype = 0, basis = 0;
+  int32_t tile_log2 = 0;
+  int64_t ntiles_axis[3] = {1, 1, 1};
+  int64_t ntiles = 0;
+  int64_t n_inside = 0;     // particles with a non-empty window (host copy)
+  int64_t n_active = 0;     // tiles holding >= 1 particle (host copy)
+  int64_t n_nodeblocks = 0; // node blocks touched by some tile footprint (host copy)
+};
+
+}  // namespace tess
+
+struct tess_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // sort + binning: keys (unsorted), keys_sorted + vals (= perm) sorted output,
+  // *_alt ping-pong buffers of the radix sort
+  tess::Buffer keys, keys_sorted, vals, keys_alt, vals_alt, hist, scan_tmp;
+  tess::Buffer tile_start, tile_end, tile_flag, tile_slot, active_tiles;
+  tess::Buffer nb_flag, nb_slot, nb_list, nb_list_prev;
+  tess::Buffer partials;
+  tess::Buffer counters;   // small device scalars
+  tess::Buffer host_pinned_dummy;
+  int64_t* counters_host = nullptr;  // pinned readback slots
+  tess::Binning bin;
+  // zeroing bookkeeping for the grid of the previous P2G
+  const void* last_grid_m = nullptr;
+  int64_t prev_nodeblocks = 0;
+  int32_t prev_tile_log2 = -1;
+  int64_t prev_ntiles = -1;
+  int64_t prev_nnodes = -1;
+  // reorder scratch
+  tess::Buffer reorder_tmp;
+};
+
+namespace tess {
+
+// ---- scans and sorts (sort.hip) ----
+// Exclusive prefix sum of n uint32 values in place; returns nothing, total is
+// written to *total_dev (device) when non-null.
+void exclusive_scan_u32(tess_ctx* ctx, uint32_t* data, int64_t n, uint32_t* total_dev);
+// Stable LSD radix sort of (key, value) pairs on the low key_bits of key.
+// vals_in == nullptr means values 0..n-1.  Result lands in (keys_out, vals_out).
+void radix_sort_pairs(tess_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                      uint32_t* vals_out, int64_t n, int key_bits);
+
+// ---- reference-layout weights and partition (partition.hip) ----
+void launch_update_weights(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_particles& P,
+                           const tess_weights& W);
+void run_partition_update(tess_ctx* ctx, const tess_mesh& mesh, tess_field x, int64_t n, int32_t* pidx,
+                          int32_t* starts, int32_t* stops, int64_t* nassigned);
+void run_reorder(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, int32_t* pidx,
+                 int64_t nassigned, double threshold, int32_t* reordered);
+
+// ---- transfers (transfer.hip) ----
+void run_p2g(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+             const tess_weights* W, const tess_params& prm);
+// reuse_binning: the caller guarantees that the last binning on ctx was made
+// by run_p2g for these very particles and positions (within one step).
+void run_g2p(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+             const tess_weights* W, const tess_params& prm, bool reuse_binning);
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace tess

@@ -1,0 +1,1013 @@
+// The README step on gfx950: tile binning, deterministic P2G, fused grid
+// update, G2P.  Reference: README.md:66-87; @P2G lowering
+// src/Transfer/p2g.jl:145-178 (hoisting :272-307), grid-node half :377-395;
+// @G2P lowering src/Transfer/g2p.jl:110-157; deferred weights
+// src/Transfer/runtime.jl:21-24,55-130.
+//
+// Data flow of one P2G (no float atomics anywhere):
+//   k_bin_keys      particle -> tile whose footprint holds its support window
+//   radix sort      stable (tile, particle id) order -> perm
+//   k_tile_runs / k_tile_flags / scan / k_active_list   tile ranges + slots
+//   k_nb_mark / scan / k_nb_list    node blocks touched by some footprint
+//   k_p2g_tile      one wave per tile: lanes own the 64 window-start classes of
+//                   the tile, accumulate their particles' contributions in
+//                   registers and add them into an LDS tile of the
+//                   (BW+3)^dim footprint (distinct targets per instruction),
+//                   then store the tile to a per-tile partial buffer
+//   k_gather_grid   one workgroup per node block: each node sums the <= 2^dim
+//                   tile partials covering it in a fixed order, then applies
+//                   the grid-node half (Float64 m⁻¹, vⁿ, v, gravity, slip walls)
+// G2P: k_g2p_tile stages the footprint's vⁿ, v in LDS and updates the tile's
+// particles; k_g2p_outside handles particles outside the mesh (empty window).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "tess_device.hpp"
+#include "tess_internal.hpp"
+
+namespace tess {
+namespace {
+
+// Tile geometry: 64 window-start classes per tile in both dimensions.
+template <int D>
+struct Geo;
+template <>
+struct Geo<3> {
+  static constexpr int LT = 2, BW = 4, TS = 7, NFOOT = 343, NCLASS = 64, GNODES = 64;
+};
+template <>
+struct Geo<2> {
+  static constexpr int LT = 3, BW = 8, TS = 11, NFOOT = 121, NCLASS = 64, GNODES = 64;
+};
+
+template <int N, int E>
+struct Pow {
+  static constexpr int v = N * Pow<N, E - 1>::v;
+};
+template <int N>
+struct Pow<N, 0> {
+  static constexpr int v = 1;
+};
+
+enum { CNT_INSIDE = 0, CNT_ACTIVE = 1, CNT_NB = 2, CNT_ERR = 3, CNT_SLOTS = 4 };
+enum { ERR_GIMP_LENGTH = 1, ERR_WINDOW = 2 };
+
+template <typename T, int D>
+struct Args {
+  DMesh<T, D> mesh;
+  DField<T> x, v, V, gradv, F, sigma, l, logE;
+  DField<T> gm, gmv, gf, gvn, gv;
+  const int64_t* window;
+  DField<T> w, gw;
+  const uint32_t* perm;
+  const uint32_t* tile_start;
+  const uint32_t* tile_end;
+  const uint32_t* active;
+  const uint32_t* tile_flag;
+  const uint32_t* tile_slot;
+  const uint32_t* nb_list;
+  const uint32_t* nb_flag;
+  T* partials;
+  int64_t n, n_inside, n_active, n_nb;
+  int32_t ntiles[D];
+  T mp, dt;
+  double lambda, mu, V0;
+  T gravity[D];
+  int32_t slip;
+};
+
+// Workgroup i -> logical index; consecutive logical indices stay on one XCD
+// (workgroups are dealt to the 8 XCDs round-robin).
+__device__ __forceinline__ int64_t xcd_remap(int64_t i, int64_t N) {
+  const int64_t per = (N + 7) >> 3;
+  return (i & 7) * per + (i >> 3);
+}
+inline unsigned xcd_grid(int64_t N) { return unsigned(((N + 7) >> 3) << 3); }
+
+template <int D>
+__device__ __forceinline__ void unlin(int64_t l, const int32_t* dims, int* I) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    I[d] = int(l % dims[d]);
+    l /= dims[d];
+  }
+}
+
+// Particle window (0-based, clamped) from its position (deferred weights) or
+// from stored weights; false if the window is empty.
+template <typename T, int D, int BASIS, bool STORED>
+__device__ __forceinline__ bool particle_window(const Args<T, D>& a, int64_t p, T* x, T& l, int* lo, int* hi,
+                                                uint32_t* err) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) x[d] = a.x(p, d);
+  l = T(0);
+  if constexpr (BASIS == TESS_UGIMP) {
+    l = a.l(p) / a.mesh.h;  // _normalized_particle_length, gimp.jl:25
+  }
+  if constexpr (STORED) {
+    bool ok = true, bad = false;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      lo[d] = int(a.window[p * 2 * D + 2 * d]) - 1;
+      hi[d] = int(a.window[p * 2 * D + 2 * d + 1]) - 1;
+      ok = ok && hi[d] >= lo[d];
+      bad = bad || hi[d] - lo[d] >= support_width(BASIS) || lo[d] < 0 || hi[d] > a.mesh.dims[d] - 1;
+    }
+    if (ok && bad) {
+      if (err) atomicOr(err, uint32_t(ERR_WINDOW));
+      return false;
+    }
+    return ok;
+  } else {
+    T xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = (x[d] - a.mesh.xmin[d]) * a.mesh.h_inv;
+    if (!inside_xi<T, D>(xi, a.mesh)) return false;
+    if constexpr (BASIS == TESS_UGIMP) {
+      if (!(T(0) <= l && l <= T(1)) && err) atomicOr(err, uint32_t(ERR_GIMP_LENGTH));
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) axis_window<T, BASIS>(xi[d], l, a.mesh.dims[d], lo[d], hi[d]);
+    return true;
+  }
+}
+
+template <typename T, int D, int BASIS, bool STORED>
+__global__ __launch_bounds__(256) void k_bin_keys(Args<T, D> a, uint32_t* __restrict__ keys, uint32_t ntiles,
+                                                  uint32_t* __restrict__ counters) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  T x[D], l;
+  int lo[D], hi[D];
+  uint32_t key = ntiles;
+  if (particle_window<T, D, BASIS, STORED>(a, p, x, l, lo, hi, counters + CNT_ERR)) {
+    uint32_t k = 0;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) k = k * uint32_t(a.ntiles[d]) + uint32_t((lo[d] + 1) >> Geo<D>::LT);
+    key = k;
+  }
+  keys[p] = key;
+}
+
+__global__ void k_tile_runs(const uint32_t* __restrict__ skeys, int64_t n, uint32_t ntiles,
+                            uint32_t* __restrict__ tile_start, uint32_t* __restrict__ tile_end,
+                            uint32_t* __restrict__ counters) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = skeys[i];
+  const bool first = i == 0 || skeys[i - 1] != k;
+  const bool last = i == n - 1 || skeys[i + 1] != k;
+  if (k < ntiles) {
+    if (first) tile_start[k] = uint32_t(i);
+    if (last) tile_end[k] = uint32_t(i + 1);
+    if (i == n - 1) counters[CNT_INSIDE] = uint32_t(n);
+  } else if (first) {
+    counters[CNT_INSIDE] = uint32_t(i);
+  }
+}
+
+__global__ void k_tile_flags(const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_end,
+                             uint32_t ntiles, uint32_t* __restrict__ flag, uint32_t* __restrict__ slot) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const uint32_t f = tile_end[t] > tile_start[t] ? 1u : 0u;
+  flag[t] = f;
+  slot[t] = f;
+}
+
+__global__ void k_compact(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ slot, uint32_t count,
+                          uint32_t* __restrict__ list) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  if (flag[t]) list[slot[t]] = uint32_t(t);
+}
+
+// Node block B holds nodes BW*B .. BW*B+BW-1 per axis; tile t's footprint
+// [BW*t-1, BW*t+BW+1] touches node blocks t-1, t, t+1.
+template <int D>
+__global__ void k_nb_mark(const uint32_t* __restrict__ tile_flag, uint32_t ntiles, int3 nt,
+                          uint32_t* __restrict__ nb_flag) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= ntiles || !tile_flag[t]) return;
+  const int dims[3] = {nt.x, nt.y, nt.z};
+  int I[3] = {0, 0, 0};
+  int64_t r = t;
+  for (int d = 0; d < D; ++d) {
+    I[d] = int(r % dims[d]);
+    r /= dims[d];
+  }
+  for (int dz = (D == 3 ? -1 : 0); dz <= (D == 3 ? 1 : 0); ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int J[3] = {I[0] + dx, I[1] + dy, I[2] + dz};
+        bool ok = true;
+        for (int d = 0; d < D; ++d) ok = ok && J[d] >= 0 && J[d] < dims[d];
+        if (!ok) continue;
+        int64_t l = J[D - 1];
+        for (int d = D - 2; d >= 0; --d) l = l * dims[d] + J[d];
+        nb_flag[l] = 1u;
+      }
+}
+
+__global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+template <typename T, int D>
+__device__ __forceinline__ void zero_node(const Args<T, D>& a, int64_t i) {
+  a.gm(i) = T(0);
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    a.gmv(i, c) = T(0);
+    a.gf(i, c) = T(0);
+    a.gvn(i, c) = T(0);
+    a.gv(i, c) = T(0);
+  }
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_zero_grid(Args<T, D> a, int64_t nnodes) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < nnodes) zero_node(a, i);
+}
+
+// Zero the nodes of node blocks touched last call but not this call.
+template <typename T, int D>
+__global__ __launch_bounds__(64) void k_zero_stale(Args<T, D> a, const uint32_t* __restrict__ prev_list,
+                                                    int64_t n_prev) {
+  using G = Geo<D>;
+  const int64_t j = blockIdx.x;
+  if (j >= n_prev) return;
+  const uint32_t B = prev_list[j];
+  if (a.nb_flag[B]) return;
+  int Bi[D], I[D];
+  unlin<D>(B, a.ntiles, Bi);
+  int r = threadIdx.x;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    I[d] = Bi[d] * G::BW + r % G::BW;
+    r /= G::BW;
+    if (I[d] >= a.mesh.dims[d]) return;
+  }
+  zero_node(a, a.mesh.lin(I));
+}
+
+template <typename T, int D, int BASIS>
+__device__ __forceinline__ void axis_jets(T xd, int lo_local, const T* s_nodex, T l, T hinv, T* val, T* der) {
+  constexpr int SW = support_width(BASIS);
+#pragma unroll
+  for (int a = 0; a < SW; ++a) {
+    const T xi = (xd - s_nodex[lo_local + a]) * hinv;  // separable_nodal_basis_jet, basis_weight.jl:49-60
+    piecewise_jet<T, BASIS>(xi, l, val[a], der[a]);
+  }
+}
+
+// ---- P2G -------------------------------------------------------------------
+template <typename T, int D, int BASIS, bool STORED, int SEG>
+__global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a) {
+  using G = Geo<D>;
+  constexpr int SW = support_width(BASIS);
+  constexpr int NV = 1 + 2 * D;
+  constexpr int NSL = Pow<SW, D - 1>::v;  // offsets per slice of the last axis
+  const int lane = threadIdx.x;
+  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
+  if (j >= a.n_active) return;
+  const uint32_t tile = a.active[j];
+  const uint32_t start = a.tile_start[tile];
+  const int cnt = int(a.tile_end[tile] - start);
+  int tI[D], origin[D];
+  unlin<D>(tile, a.ntiles, tI);
+#pragma unroll
+  for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
+
+  __shared__ T s_tile[NV][G::NFOOT];
+  __shared__ T s_nodex[D][G::TS];
+  __shared__ T s_x[D][SEG];
+  __shared__ T s_mv[D][SEG];
+  __shared__ T s_A[D * D][SEG];
+  __shared__ T s_l[BASIS == TESS_UGIMP ? SEG : 1];
+  __shared__ uint32_t s_pid[STORED ? SEG : 1];
+  __shared__ uint8_t s_cls[SEG];
+  __shared__ uint8_t s_ext[SEG];
+  __shared__ uint16_t s_order[SEG];
+
+  for (int q = lane; q < NV * G::NFOOT; q += 64) (&s_tile[0][0])[q] = T(0);
+  for (int q = lane; q < D * G::TS; q += 64) {
+    const int d = q / G::TS, k = q % G::TS;
+    s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
+  }
+  // class coordinates of this lane: window start offset from the footprint origin
+  int ccoord[D];
+  {
+    int r = lane;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      ccoord[d] = r % G::BW;
+      r /= G::BW;
+    }
+  }
+  const T mp = a.mp, hinv = a.mesh.h_inv;
+  const int nseg = (cnt + SEG - 1) / SEG;
+  for (int s = 0; s < nseg; ++s) {
+    const int segcnt = (cnt - s + nseg - 1) / nseg;
+    __syncthreads();
+    // stage the segment (particles s, s+nseg, s+2nseg, ... of the tile)
+    for (int i = lane; i < segcnt; i += 64) {
+      const uint32_t pid = a.perm[start + uint32_t(s + i * nseg)];
+      T x[D], l;
+      int lo[D], hi[D];
+      particle_window<T, D, BASIS, STORED>(a, pid, x, l, lo, hi, nullptr);
+      int cls = 0, ext = 0;
+#pragma unroll
+      for (int d = D - 1; d >= 0; --d) {
+        cls = cls * G::BW + (lo[d] - origin[d]);
+        ext = (ext << 2) | (hi[d] - lo[d]);
+      }
+      s_cls[i] = uint8_t(cls);
+      s_ext[i] = uint8_t(ext);
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        s_x[d][i] = x[d];
+        s_mv[d][i] = mp * a.v(pid, d);  // hoisted run mₚ*v[p], p2g.jl:272-307
+      }
+      const T negV = -a.V(pid);
+#pragma unroll
+      for (int k = 0; k < D * D; ++k) s_A[k][i] = negV * a.sigma(pid, k);  // hoisted (-V[p])*σ[p]
+      if constexpr (BASIS == TESS_UGIMP) s_l[i] = l;
+      if constexpr (STORED) s_pid[i] = pid;
+    }
+    __syncthreads();
+    // counting sort of the segment by class: lane == class
+    int ccount = 0;
+    for (int i = 0; i < segcnt; ++i) ccount += (s_cls[i] == lane) ? 1 : 0;
+    int incl = ccount;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int u = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += u;
+    }
+    const int coff = incl - ccount;
+    {
+      int k = 0;
+      for (int i = 0; i < segcnt; ++i)
+        if (s_cls[i] == lane) s_order[coff + (k++)] = uint16_t(i);
+    }
+    int maxcnt = ccount;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) maxcnt = max(maxcnt, __shfl_xor(maxcnt, off, 64));
+    __syncthreads();
+    // one pass per offset along the last axis
+    for (int al = 0; al < SW; ++al) {
+      T acc[NSL][NV];
+#pragma unroll
+      for (int s2 = 0; s2 < NSL; ++s2)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[s2][v] = T(0);
+      for (int k = 0; k < maxcnt; ++k) {
+        if (k >= ccount) continue;
+        const int i = s_order[coff + k];
+        const int ext = s_ext[i];
+        int extd[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) extd[d] = (ext >> (2 * d)) & 3;
+        if (al > extd[D - 1]) continue;
+        T xv[D], mv[D], A[D * D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          xv[d] = s_x[d][i];
+          mv[d] = s_mv[d][i];
+        }
+#pragma unroll
+        for (int q = 0; q < D * D; ++q) A[q] = s_A[q][i];
+        T val[D][SW], der[D][SW];
+        if constexpr (!STORED) {
+          const T l = BASIS == TESS_UGIMP ? s_l[i] : T(0);
+#pragma unroll
+          for (int d = 0; d < D - 1; ++d) axis_jets<T, D, BASIS>(xv[d], ccoord[d], s_nodex[d], l, hinv, val[d], der[d]);
+          const T xi = (xv[D - 1] - s_nodex[D - 1][ccoord[D - 1] + al]) * hinv;
+          piecewise_jet<T, BASIS>(xi, l, val[D - 1][0], der[D - 1][0]);
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < NSL; ++s2) {
+          int aa[D];
+          int r = s2;
+          bool ok = true;
+#pragma unroll
+          for (int d = 0; d < D - 1; ++d) {
+            aa[d] = r % SW;
+            r /= SW;
+            ok = ok && aa[d] <= extd[d];
+          }
+          aa[D - 1] = al;
+          if (!ok) continue;
+          T w, g[D];
+          if constexpr (STORED) {
+            int slot = 0;
+#pragma unroll
+            for (int d = D - 1; d >= 0; --d) slot = slot * SW + aa[d];
+            const int64_t e = int64_t(slot) + int64_t(Pow<SW, D>::v) * s_pid[i];
+            w = a.w(e);
+#pragma unroll
+            for (int c = 0; c < D; ++c) g[c] = a.gw(e, c);
+          } else {
+            // w = Π_d N_d (left-assoc), ∇w_c = (N'_c Π_{d≠c} N_d) h⁻¹
+            w = val[0][aa[0]];
+#pragma unroll
+            for (int d = 1; d < D - 1; ++d) w = w * val[d][aa[d]];
+            w = w * val[D - 1][0];
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+              T gg = (c == 0) ? der[0][aa[0]] : val[0][aa[0]];
+#pragma unroll
+              for (int d = 1; d < D - 1; ++d) gg = gg * ((d == c) ? der[d][aa[d]] : val[d][aa[d]]);
+              gg = gg * ((c == D - 1) ? der[D - 1][0] : val[D - 1][0]);
+              g[c] = gg * hinv;
+            }
+          }
+          acc[s2][0] += w * mp;
+#pragma unroll
+          for (int c = 0; c < D; ++c) acc[s2][1 + c] += w * mv[c];
+#pragma unroll
+          for (int r2 = 0; r2 < D; ++r2) {
+            T f = A[r2] * g[0];
+#pragma unroll
+            for (int c = 1; c < D; ++c) f = f + A[r2 + D * c] * g[c];
+            acc[s2][1 + D + r2] += f;
+          }
+        }
+      }
+      // add this lane's class contributions into the LDS tile: all lanes are
+      // distinct classes, so for a fixed offset their targets are distinct
+      if (ccount > 0) {
+#pragma unroll
+        for (int s2 = 0; s2 < NSL; ++s2) {
+          int r = s2, q = 0, mul = 1;
+#pragma unroll
+          for (int d = 0; d < D - 1; ++d) {
+            q += (ccoord[d] + r % SW) * mul;
+            r /= SW;
+            mul *= G::TS;
+          }
+          q += (ccoord[D - 1] + al) * mul;
+#pragma unroll
+          for (int v = 0; v < NV; ++v) s_tile[v][q] += acc[s2][v];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  T* out = a.partials + int64_t(j) * NV * G::NFOOT;
+  for (int q = lane; q < NV * G::NFOOT; q += 64) out[q] = (&s_tile[0][0])[q];
+}
+
+// ---- gather + grid-node half ------------------------------------------------
+template <typename T, int D>
+__global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a) {
+  using G = Geo<D>;
+  constexpr int NV = 1 + 2 * D;
+  const int64_t j = xcd_remap(blockIdx.x, a.n_nb);
+  if (j >= a.n_nb) return;
+  const uint32_t B = a.nb_list[j];
+  int Bi[D], I[D], rr[D];
+  unlin<D>(B, a.ntiles, Bi);
+  {
+    int r = threadIdx.x;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      rr[d] = r % G::BW;
+      r /= G::BW;
+      I[d] = Bi[d] * G::BW + rr[d];
+      if (I[d] >= a.mesh.dims[d]) return;
+    }
+  }
+  T acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = T(0);
+  // tiles covering this node: B-1 if r<=1, B, B+1 if r==BW-1 (per axis),
+  // summed in a fixed order (last axis outermost)
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    lo[d] = (rr[d] <= 1 && Bi[d] > 0) ? -1 : 0;
+    hi[d] = (rr[d] == G::BW - 1 && Bi[d] + 1 < a.ntiles[d]) ? 1 : 0;
+  }
+  int o[3] = {0, 0, 0};
+  for (o[2] = lo[2]; o[2] <= hi[2]; ++o[2])
+    for (o[1] = lo[1]; o[1] <= hi[1]; ++o[1])
+      for (o[0] = lo[0]; o[0] <= hi[0]; ++o[0]) {
+        int64_t t = 0;
+        int q = 0;
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) {
+          const int b = Bi[d] + o[d];
+          t = t * a.ntiles[d] + b;
+        }
+        if (!a.tile_flag[t]) continue;
+        {
+          int mul = 1;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            q += (I[d] - ((Bi[d] + o[d]) * G::BW - 1)) * mul;
+            mul *= G::TS;
+          }
+        }
+        const T* P = a.partials + int64_t(a.tile_slot[t]) * NV * G::NFOOT + q;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] += P[int64_t(v) * G::NFOOT];
+      }
+  const int64_t i = a.mesh.lin(I);
+  // grid-node half with the guarded Float64 division (test/transfer.jl:102-104)
+  const double md = double(acc[0]);
+  const bool has = md > 0.0;
+  const double minv = has ? 1.0 / md : 0.0;
+  a.gm(i) = acc[0];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    const T mvc = acc[1 + c], fc = acc[1 + D + c];
+    const T vn = T(double(mvc) * minv);
+    const T accel = T(double(fc) * minv) + (has ? a.gravity[c] : T(0));
+    T vc = vn + accel * a.dt;
+    const bool wall = (((a.slip >> (2 * c)) & 1) && I[c] == 0) ||
+                      (((a.slip >> (2 * c + 1)) & 1) && I[c] == a.mesh.dims[c] - 1);
+    if (wall) vc = T(0);
+    a.gmv(i, c) = mvc;
+    a.gf(i, c) = fc;
+    a.gvn(i, c) = vn;
+    a.gv(i, c) = vc;
+  }
+}
+
+// ---- G2P -------------------------------------------------------------------
+template <typename T, int D>
+__device__ __forceinline__ double det_d(const T* F) {
+  if constexpr (D == 2) {
+    return double(F[0]) * double(F[3]) - double(F[2]) * double(F[1]);
+  } else {
+    const double f11 = F[0], f21 = F[1], f31 = F[2], f12 = F[3], f22 = F[4], f32 = F[5], f13 = F[6], f23 = F[7],
+                 f33 = F[8];
+    return f11 * (f22 * f33 - f23 * f32) - f12 * (f21 * f33 - f23 * f31) + f13 * (f21 * f32 - f22 * f31);
+  }
+}
+
+// Particle equations of the README @G2P block (README.md:81-86) after the sums.
+template <typename T, int D>
+__device__ __forceinline__ void particle_update(const Args<T, D>& a, int64_t p, const T* dv, const T* L,
+                                                const T* dx) {
+  const T dt = a.dt;
+#pragma unroll
+  for (int c = 0; c < D; ++c) a.v(p, c) += dv[c];
+#pragma unroll
+  for (int k = 0; k < D * D; ++k) a.gradv(p, k) = L[k];
+#pragma unroll
+  for (int c = 0; c < D; ++c) a.x(p, c) += dx[c];
+  T Fo[D * D], M[D * D], Fn[D * D];
+#pragma unroll
+  for (int k = 0; k < D * D; ++k) Fo[k] = a.F(p, k);
+#pragma unroll
+  for (int b = 0; b < D; ++b)
+#pragma unroll
+    for (int r = 0; r < D; ++r) M[r + D * b] = (r == b ? T(1) : T(0)) + L[r + D * b] * dt;
+#pragma unroll
+  for (int b = 0; b < D; ++b)
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      T s = M[r] * Fo[D * b];
+#pragma unroll
+      for (int k = 1; k < D; ++k) s = s + M[r + D * k] * Fo[k + D * b];
+      Fn[r + D * b] = s;
+    }
+#pragma unroll
+  for (int k = 0; k < D * D; ++k) a.F(p, k) = Fn[k];
+  const double J = det_d<T, D>(Fn);
+  double lam = a.lambda, mu = a.mu;
+  if (a.logE.p) {
+    const double s = exp(double(a.logE(p)));
+    lam *= s;
+    mu *= s;
+  }
+  const double lnJ = log(J);
+  const double invJ = 1.0 / J;
+#pragma unroll
+  for (int b = 0; b < D; ++b)
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      double bb = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) bb += double(Fn[r + D * k]) * double(Fn[b + D * k]);
+      const double s = mu * (bb - (r == b ? 1.0 : 0.0)) + (r == b ? lam * lnJ : 0.0);
+      a.sigma(p, r + D * b) = T(s * invJ);
+    }
+  a.V(p) = T(a.V0 * J);
+}
+
+template <typename T, int D, int BASIS, bool STORED>
+__global__ __launch_bounds__(256) void k_g2p_tile(Args<T, D> a) {
+  using G = Geo<D>;
+  constexpr int SW = support_width(BASIS);
+  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
+  if (j >= a.n_active) return;
+  const uint32_t tile = a.active[j];
+  const uint32_t start = a.tile_start[tile];
+  const int cnt = int(a.tile_end[tile] - start);
+  int tI[D], origin[D];
+  unlin<D>(tile, a.ntiles, tI);
+#pragma unroll
+  for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
+  __shared__ T s_vn[D][G::NFOOT];
+  __shared__ T s_v[D][G::NFOOT];
+  __shared__ T s_nodex[D][G::TS];
+  for (int q = threadIdx.x; q < G::NFOOT; q += blockDim.x) {
+    int I[D], r = q;
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      I[d] = origin[d] + r % G::TS;
+      r /= G::TS;
+      in = in && I[d] >= 0 && I[d] < a.mesh.dims[d];
+    }
+    if (in) {
+      const int64_t i = a.mesh.lin(I);
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        s_vn[c][q] = a.gvn(i, c);
+        s_v[c][q] = a.gv(i, c);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        s_vn[c][q] = T(0);
+        s_v[c][q] = T(0);
+      }
+    }
+  }
+  for (int q = threadIdx.x; q < D * G::TS; q += blockDim.x) {
+    const int d = q / G::TS, k = q % G::TS;
+    s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
+  }
+  __syncthreads();
+  const T hinv = a.mesh.h_inv, dt = a.dt;
+  for (int jj = threadIdx.x; jj < cnt; jj += blockDim.x) {
+    const uint32_t pid = a.perm[start + uint32_t(jj)];
+    T x[D], l;
+    int lo[D], hi[D];
+    particle_window<T, D, BASIS, STORED>(a, pid, x, l, lo, hi, nullptr);
+    int ll[D], ext[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      ll[d] = lo[d] - origin[d];
+      ext[d] = hi[d] - lo[d];
+    }
+    T val[D][SW], der[D][SW];
+    if constexpr (!STORED) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) axis_jets<T, D, BASIS>(x[d], ll[d], s_nodex[d], l, hinv, val[d], der[d]);
+    }
+    T dv[D], L[D * D], dx[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      dv[c] = T(0);
+      dx[c] = T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) L[k] = T(0);
+    // column-major walk of the window (eachindex(window))
+#pragma unroll
+    for (int o = 0; o < Pow<SW, D>::v; ++o) {
+      int aa[D], r = o;
+      bool ok = true;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        aa[d] = r % SW;
+        r /= SW;
+        ok = ok && aa[d] <= ext[d];
+      }
+      if (!ok) continue;
+      T w, g[D];
+      if constexpr (STORED) {
+        const int64_t e = int64_t(o) + int64_t(Pow<SW, D>::v) * pid;
+        w = a.w(e);
+#pragma unroll
+        for (int c = 0; c < D; ++c) g[c] = a.gw(e, c);
+      } else {
+        w = val[0][aa[0]];
+#pragma unroll
+        for (int d = 1; d < D; ++d) w = w * val[d][aa[d]];
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          T gg = (c == 0) ? der[0][aa[0]] : val[0][aa[0]];
+#pragma unroll
+          for (int d = 1; d < D; ++d) gg = gg * ((d == c) ? der[d][aa[d]] : val[d][aa[d]]);
+          g[c] = gg * hinv;
+        }
+      }
+      int q = 0, mul = 1;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        q += (ll[d] + aa[d]) * mul;
+        mul *= G::TS;
+      }
+      T vi[D];
+#pragma unroll
+      for (int c = 0; c < D; ++c) vi[c] = s_v[c][q];
+#pragma unroll
+      for (int c = 0; c < D; ++c) dv[c] += w * (vi[c] - s_vn[c][q]);
+#pragma unroll
+      for (int b = 0; b < D; ++b)
+#pragma unroll
+        for (int r2 = 0; r2 < D; ++r2) L[r2 + D * b] += vi[r2] * g[b];
+#pragma unroll
+      for (int c = 0; c < D; ++c) dx[c] += (w * vi[c]) * dt;
+    }
+    particle_update<T, D>(a, pid, dv, L, dx);
+  }
+}
+
+// Particles outside the mesh have an empty window: the sums are zero and only
+// the particle equations run (g2p.jl:123-153).
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_g2p_outside(Args<T, D> a) {
+  const int64_t j = a.n_inside + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= a.n) return;
+  const uint32_t pid = a.perm[j];
+  T dv[D], L[D * D], dx[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    dv[c] = T(0);
+    dx[c] = T(0);
+  }
+#pragma unroll
+  for (int k = 0; k < D * D; ++k) L[k] = T(0);
+  particle_update<T, D>(a, pid, dv, L, dx);
+}
+
+// ---- host side ----------------------------------------------------------------
+
+template <typename T, int D>
+Args<T, D> make_args(const tess_mesh& mesh, const tess_grid* G, const tess_particles& P, const tess_weights* W,
+                     const tess_params* prm) {
+  Args<T, D> a;
+  std::memset(static_cast<void*>(&a), 0, sizeof(a));
+  a.mesh = dmesh<T, D>(mesh);
+  a.x = dfield<T>(P.x);
+  a.v = dfield<T>(P.v);
+  a.V = dfield<T>(P.V);
+  a.gradv = dfield<T>(P.grad_v);
+  a.F = dfield<T>(P.F);
+  a.sigma = dfield<T>(P.sigma);
+  a.l = dfield<T>(P.l);
+  a.logE = dfield<T>(P.logE);
+  if (G) {
+    a.gm = dfield<T>(G->m);
+    a.gmv = dfield<T>(G->mv);
+    a.gf = dfield<T>(G->f);
+    a.gvn = dfield<T>(G->vn);
+    a.gv = dfield<T>(G->v);
+  }
+  if (W) {
+    a.window = W->window;
+    a.w = dfield<T>(W->w);
+    a.gw = dfield<T>(W->grad_w);
+  }
+  a.n = P.n;
+  if (prm) {
+    a.mp = T(prm->mp);
+    a.dt = T(prm->dt);
+    a.lambda = prm->lambda;
+    a.mu = prm->mu;
+    a.V0 = prm->V0;
+    for (int d = 0; d < D; ++d) a.gravity[d] = T(prm->gravity[d]);
+    a.slip = prm->slip_walls;
+  }
+  for (int d = 0; d < D; ++d) a.ntiles[d] = int32_t(((mesh.dims[d] - 1) >> Geo<D>::LT) + 1);
+  return a;
+}
+
+template <typename T, int D>
+void fill_binning_ptrs(tess_ctx* ctx, Args<T, D>& a) {
+  const Binning& b = ctx->bin;
+  a.perm = static_cast<const uint32_t*>(ctx->vals.ptr);
+  a.tile_start = static_cast<const uint32_t*>(ctx->tile_start.ptr);
+  a.tile_end = static_cast<const uint32_t*>(ctx->tile_end.ptr);
+  a.active = static_cast<const uint32_t*>(ctx->active_tiles.ptr);
+  a.tile_flag = static_cast<const uint32_t*>(ctx->tile_flag.ptr);
+  a.tile_slot = static_cast<const uint32_t*>(ctx->tile_slot.ptr);
+  a.nb_list = static_cast<const uint32_t*>(ctx->nb_list.ptr);
+  a.nb_flag = static_cast<const uint32_t*>(ctx->nb_flag.ptr);
+  a.n_inside = b.n_inside;
+  a.n_active = b.n_active;
+  a.n_nb = b.n_nodeblocks;
+}
+
+inline unsigned blocks_for(int64_t n, int threads) { return unsigned(std::max<int64_t>(1, ceil_div(n, threads))); }
+
+int key_bits_for(uint64_t maxkey) {
+  int b = 0;
+  while (b < 32 && (uint64_t(1) << b) <= maxkey) ++b;
+  return b;
+}
+
+template <typename T, int D, int BASIS, bool STORED>
+void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const tess_weights* W) {
+  Args<T, D> a = make_args<T, D>(mesh, nullptr, P, W, nullptr);
+  const int64_t n = P.n;
+  uint64_t ntiles = 1;
+  for (int d = 0; d < D; ++d) ntiles *= uint64_t(a.ntiles[d]);
+  if (ntiles >= uint64_t(UINT32_MAX) || n >= int64_t(UINT32_MAX))
+    raise(TESS_ERR_ARGUMENT, "mesh or particle count exceeds the 32-bit binning range");
+  Binning& b = ctx->bin;
+  b.valid = false;
+  uint32_t* counters = ctx->counters.as<uint32_t>(CNT_SLOTS);
+  TESS_HIP(hipMemsetAsync(counters, 0, CNT_SLOTS * sizeof(uint32_t), ctx->stream));
+  uint32_t* keys = ctx->keys.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
+  uint32_t* skeys = ctx->keys_sorted.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
+  uint32_t* perm = ctx->vals.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
+  uint32_t* tstart = ctx->tile_start.as<uint32_t>(ntiles);
+  uint32_t* tend = ctx->tile_end.as<uint32_t>(ntiles);
+  uint32_t* tflag = ctx->tile_flag.as<uint32_t>(ntiles);
+  uint32_t* tslot = ctx->tile_slot.as<uint32_t>(ntiles);
+  uint32_t* active = ctx->active_tiles.as<uint32_t>(ntiles);
+  uint32_t* nbflag = ctx->nb_flag.as<uint32_t>(ntiles);
+  uint32_t* nbslot = ctx->nb_slot.as<uint32_t>(ntiles);
+  uint32_t* nblist = ctx->nb_list.as<uint32_t>(ntiles);
+  if (n > 0) {
+    hipLaunchKernelGGL((k_bin_keys<T, D, BASIS, STORED>), dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, a,
+                       keys, uint32_t(ntiles), counters);
+    TESS_HIP(hipGetLastError());
+    radix_sort_pairs(ctx, keys, nullptr, skeys, perm, n, key_bits_for(ntiles));
+  }
+  TESS_HIP(hipMemsetAsync(tstart, 0, ntiles * sizeof(uint32_t), ctx->stream));
+  TESS_HIP(hipMemsetAsync(tend, 0, ntiles * sizeof(uint32_t), ctx->stream));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_tile_runs, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, skeys, n,
+                       uint32_t(ntiles), tstart, tend, counters);
+    TESS_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_tile_flags, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, tstart, tend,
+                     uint32_t(ntiles), tflag, tslot);
+  TESS_HIP(hipGetLastError());
+  exclusive_scan_u32(ctx, tslot, int64_t(ntiles), counters + CNT_ACTIVE);
+  hipLaunchKernelGGL(k_compact, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, tflag, tslot,
+                     uint32_t(ntiles), active);
+  TESS_HIP(hipGetLastError());
+  TESS_HIP(hipMemsetAsync(nbflag, 0, ntiles * sizeof(uint32_t), ctx->stream));
+  const int3 nt = make_int3(a.ntiles[0], a.ntiles[1], D == 3 ? a.ntiles[D == 3 ? 2 : 0] : 1);
+  hipLaunchKernelGGL(k_nb_mark<D>, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, tflag,
+                     uint32_t(ntiles), nt, nbflag);
+  TESS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_copy_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, nbflag, nbslot,
+                     int64_t(ntiles));
+  TESS_HIP(hipGetLastError());
+  exclusive_scan_u32(ctx, nbslot, int64_t(ntiles), counters + CNT_NB);
+  hipLaunchKernelGGL(k_compact, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, nbflag, nbslot,
+                     uint32_t(ntiles), nblist);
+  TESS_HIP(hipGetLastError());
+  // the one host readback of the binning: totals for the launch sizes
+  uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
+  TESS_HIP(hipMemcpyAsync(hc, counters, CNT_SLOTS * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  TESS_HIP(hipStreamSynchronize(ctx->stream));
+  if (hc[CNT_ERR] & ERR_GIMP_LENGTH) raise(TESS_ERR_ARGUMENT, "uGIMP requires 0 ≤ pt.l ≤ spacing(mesh)");
+  if (hc[CNT_ERR] & ERR_WINDOW)
+    raise(TESS_ERR_DIMENSION, "stored support window is larger than the basis support width");
+  b.n_inside = n > 0 ? int64_t(hc[CNT_INSIDE]) : 0;
+  b.n_active = int64_t(hc[CNT_ACTIVE]);
+  b.n_nodeblocks = int64_t(hc[CNT_NB]);
+  b.x_ptr = P.x.ptr;
+  b.n = n;
+  b.dim = D;
+  b.dtype = mesh.dtype;
+  b.basis = BASIS;
+  b.tile_log2 = Geo<D>::LT;
+  b.ntiles = int64_t(ntiles);
+  for (int d = 0; d < 3; ++d) b.ntiles_axis[d] = d < D ? a.ntiles[d] : 1;
+  b.valid = true;
+}
+
+template <typename T, int D, int BASIS, bool STORED>
+void p2g_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+              const tess_weights* W, const tess_params& prm) {
+  using Gm = Geo<D>;
+  constexpr int NV = 1 + 2 * D;
+  bin_particles<T, D, BASIS, STORED>(ctx, mesh, P, W);
+  Args<T, D> a = make_args<T, D>(mesh, &G, P, W, &prm);
+  fill_binning_ptrs<T, D>(ctx, a);
+  int64_t nnodes = 1;
+  for (int d = 0; d < D; ++d) nnodes *= mesh.dims[d];
+  // zero what this call will not overwrite: everything on first use of a
+  // grid, else the node blocks that were written last time and are not now
+  const bool fresh = ctx->last_grid_m != G.m.ptr || ctx->prev_tile_log2 != Gm::LT ||
+                     ctx->prev_ntiles != ctx->bin.ntiles || ctx->prev_nnodes != nnodes;
+  if (fresh) {
+    hipLaunchKernelGGL((k_zero_grid<T, D>), dim3(blocks_for(nnodes, 256)), dim3(256), 0, ctx->stream, a, nnodes);
+    TESS_HIP(hipGetLastError());
+  } else if (ctx->prev_nodeblocks > 0) {
+    hipLaunchKernelGGL((k_zero_stale<T, D>), dim3(unsigned(ctx->prev_nodeblocks)), dim3(64), 0, ctx->stream, a,
+                       static_cast<const uint32_t*>(ctx->nb_list_prev.ptr), ctx->prev_nodeblocks);
+    TESS_HIP(hipGetLastError());
+  }
+  const Binning& b = ctx->bin;
+  if (b.n_active > 0) {
+    a.partials = ctx->partials.as<T>(size_t(b.n_active) * NV * Gm::NFOOT);
+    constexpr int SEG = (sizeof(T) == 4 && D == 3) ? 256 : 128;
+    hipLaunchKernelGGL((k_p2g_tile<T, D, BASIS, STORED, SEG>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
+                       ctx->stream, a);
+    TESS_HIP(hipGetLastError());
+  }
+  if (b.n_nodeblocks > 0) {
+    hipLaunchKernelGGL((k_gather_grid<T, D>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a);
+    TESS_HIP(hipGetLastError());
+  }
+  // remember this call's node blocks for the next call's stale zeroing
+  uint32_t* prev = ctx->nb_list_prev.as<uint32_t>(size_t(std::max<int64_t>(b.ntiles, 1)));
+  if (b.n_nodeblocks > 0)
+    TESS_HIP(hipMemcpyAsync(prev, ctx->nb_list.ptr, size_t(b.n_nodeblocks) * sizeof(uint32_t),
+                            hipMemcpyDeviceToDevice, ctx->stream));
+  ctx->prev_nodeblocks = b.n_nodeblocks;
+  ctx->last_grid_m = G.m.ptr;
+  ctx->prev_tile_log2 = Gm::LT;
+  ctx->prev_ntiles = b.ntiles;
+  ctx->prev_nnodes = nnodes;
+}
+
+template <typename T, int D, int BASIS, bool STORED>
+void g2p_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+              const tess_weights* W, const tess_params& prm, bool reuse_binning) {
+  const Binning& b = ctx->bin;
+  // Only a step that just ran P2G on these particles may reuse its binning; a
+  // standalone G2P bins again (device addresses alone cannot prove that the
+  // particle set is unchanged).
+  const bool reuse = reuse_binning && b.valid && b.x_ptr == P.x.ptr && b.n == P.n && b.dim == D &&
+                     b.dtype == mesh.dtype && b.basis == BASIS;
+  if (!reuse) bin_particles<T, D, BASIS, STORED>(ctx, mesh, P, W);
+  Args<T, D> a = make_args<T, D>(mesh, &G, P, W, &prm);
+  fill_binning_ptrs<T, D>(ctx, a);
+  if (b.n_active > 0) {
+    hipLaunchKernelGGL((k_g2p_tile<T, D, BASIS, STORED>), dim3(xcd_grid(b.n_active)), dim3(256), 0, ctx->stream,
+                       a);
+    TESS_HIP(hipGetLastError());
+  }
+  if (P.n > b.n_inside) {
+    hipLaunchKernelGGL((k_g2p_outside<T, D>), dim3(blocks_for(P.n - b.n_inside, 256)), dim3(256), 0, ctx->stream,
+                       a);
+    TESS_HIP(hipGetLastError());
+  }
+  ctx->bin.valid = false;  // positions moved
+}
+
+template <template <typename, int, int, bool> class Op, typename... Args_>
+void dispatch(int basis, const tess_mesh& mesh, bool stored, Args_&&... args) {
+#define TESS_CASE(T_, D_, B_)                                     \
+  if (mesh.dim == D_ && basis == B_) {                            \
+    if (stored)                                                   \
+      Op<T_, D_, B_, true>::run(std::forward<Args_>(args)...);    \
+    else                                                          \
+      Op<T_, D_, B_, false>::run(std::forward<Args_>(args)...);   \
+    return;                                                       \
+  }
+#define TESS_CASES(T_, D_)                        \
+  TESS_CASE(T_, D_, TESS_BSPLINE_LINEAR)          \
+  TESS_CASE(T_, D_, TESS_BSPLINE_QUADRATIC)       \
+  TESS_CASE(T_, D_, TESS_BSPLINE_CUBIC)           \
+  TESS_CASE(T_, D_, TESS_UGIMP)
+  if (mesh.dtype == TESS_FLOAT32) {
+    TESS_CASES(float, 2)
+    TESS_CASES(float, 3)
+  } else {
+    TESS_CASES(double, 2)
+    TESS_CASES(double, 3)
+  }
+#undef TESS_CASES
+#undef TESS_CASE
+  raise(TESS_ERR_UNSUPPORTED, "unsupported (dim, dtype, basis) combination");
+}
+
+template <typename T, int D, int B, bool S>
+struct P2GOp {
+  static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
+                  const tess_weights* W, const tess_params& prm) {
+    p2g_impl<T, D, B, S>(ctx, m, G, P, W, prm);
+  }
+};
+template <typename T, int D, int B, bool S>
+struct G2POp {
+  static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
+                  const tess_weights* W, const tess_params& prm, bool reuse) {
+    g2p_impl<T, D, B, S>(ctx, m, G, P, W, prm, reuse);
+  }
+};
+
+}  // namespace
+
+void run_p2g(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+             const tess_weights* W, const tess_params& prm) {
+  dispatch<P2GOp>(basis, mesh, W != nullptr, ctx, mesh, G, P, W, prm);
+}
+
+void run_g2p(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+             const tess_weights* W, const tess_params& prm, bool reuse_binning) {
+  dispatch<G2POp>(basis, mesh, W != nullptr, ctx, mesh, G, P, W, prm, reuse_binning);
+}
+
+}  // namespace tess

@@ -1,0 +1,20 @@
+"""tesserae_hip — MI355X (gfx950) engine for Tesserae.jl's explicit MPM step.
+
+Host-side mirror of the reference API over the C ABI of libtesserae_hip
+(include/tesserae_hip.h).  Importing this package does not touch the GPU; the
+shared library is loaded on first use and its absence is an error.
+"""
+from .core import (ArgumentError, Basis, BasisWeights, BSpline, CartesianMesh, Context, Cubic, DeviceError,
+                   DimensionMismatch, ErrorException, G2P, Grid, Linear, MPMParams, P2G, Particles, Partition,
+                   Quadratic, TesseraeAssertionError, TesseraeError, UnsupportedError, default_context,
+                   generate_basis_weights, generate_grid, generate_particles, grid_sampling_points,
+                   reorder_particles, step, uGIMP, update)
+from ._abi import LIB_PATH, load_library
+
+__all__ = [
+    "ArgumentError", "Basis", "BasisWeights", "BSpline", "CartesianMesh", "Context", "Cubic", "DeviceError",
+    "DimensionMismatch", "ErrorException", "G2P", "Grid", "Linear", "MPMParams", "P2G", "Particles", "Partition",
+    "Quadratic", "TesseraeAssertionError", "TesseraeError", "UnsupportedError", "default_context",
+    "generate_basis_weights", "generate_grid", "generate_particles", "grid_sampling_points", "reorder_particles",
+    "step", "uGIMP", "update", "LIB_PATH", "load_library",
+]

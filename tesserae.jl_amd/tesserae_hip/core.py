@@ -1,0 +1,545 @@
+"""Host-side mirror of Tesserae.jl's API for the explicit MPM step, over the C ABI.
+
+Names follow the reference (src/Tesserae.jl:35-129): ``CartesianMesh``,
+``generate_grid``, ``generate_particles``, ``generate_basis_weights``,
+``update`` (Julia ``update!``), ``Partition``, ``reorder_particles``, and the
+README program's ``P2G`` / ``G2P`` transfers.  Device memory is held in torch
+tensors (plumbing only); every computation runs in libtesserae_hip.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from . import _abi
+from ._abi import (TESS_BSPLINE_CUBIC, TESS_BSPLINE_LINEAR, TESS_BSPLINE_QUADRATIC, TESS_FLOAT32, TESS_FLOAT64,
+                   TESS_UGIMP)
+
+
+# ---- exceptions mirroring the reference's host-side errors -----------------
+
+class TesseraeError(Exception):
+    """Base class of errors raised by libtesserae_hip."""
+
+
+class ArgumentError(TesseraeError, ValueError):
+    """Julia ArgumentError."""
+
+
+class DimensionMismatch(TesseraeError, ValueError):
+    """Julia DimensionMismatch."""
+
+
+class TesseraeAssertionError(TesseraeError, AssertionError):
+    """Julia AssertionError raised by the reference's @assert checks."""
+
+
+class ErrorException(TesseraeError, RuntimeError):
+    """Julia ErrorException (the reference's error(...))."""
+
+
+class DeviceError(TesseraeError, RuntimeError):
+    """HIP runtime failure."""
+
+
+class UnsupportedError(TesseraeError, NotImplementedError):
+    """Program, basis or dimension outside the library's fixed set."""
+
+
+_STATUS = {
+    _abi.TESS_ERR_ARGUMENT: ArgumentError,
+    _abi.TESS_ERR_DIMENSION: DimensionMismatch,
+    _abi.TESS_ERR_ASSERTION: TesseraeAssertionError,
+    _abi.TESS_ERR_STATE: ErrorException,
+    _abi.TESS_ERR_DEVICE: DeviceError,
+    _abi.TESS_ERR_UNSUPPORTED: UnsupportedError,
+}
+
+
+def _check(lib, status: int) -> None:
+    if status != _abi.TESS_OK:
+        msg = lib.tess_last_error().decode("utf-8", "replace")
+        raise _STATUS.get(status, TesseraeError)(msg)
+
+
+# ---- bases (src/Basis/bspline.jl:15-25, src/Basis/gimp.jl:16-18) ----------
+
+@dataclass(frozen=True)
+class Basis:
+    kind: int
+    name: str
+
+    @property
+    def support_width(self) -> int:
+        return {TESS_BSPLINE_LINEAR: 2, TESS_BSPLINE_QUADRATIC: 3, TESS_BSPLINE_CUBIC: 4, TESS_UGIMP: 3}[self.kind]
+
+    def __repr__(self) -> str:
+        return self.name
+
+
+class Linear:
+    degree = 1
+
+
+class Quadratic:
+    degree = 2
+
+
+class Cubic:
+    degree = 3
+
+
+def BSpline(degree) -> Basis:
+    """BSpline(Linear()/Quadratic()/Cubic()) (src/Basis/bspline.jl:15-17)."""
+    d = degree.degree if hasattr(degree, "degree") else int(degree)
+    if d not in (1, 2, 3):
+        raise UnsupportedError(f"BSpline degree {d} is outside the library's set (1, 2, 3)")
+    return Basis(d, f"BSpline({['', 'Linear', 'Quadratic', 'Cubic'][d]}())")
+
+
+def uGIMP() -> Basis:
+    """uGIMP() (src/Basis/gimp.jl:16); particles need the length field ``l``."""
+    return Basis(TESS_UGIMP, "uGIMP()")
+
+
+# ---- mesh (src/Mesh/cartesian.jl:38-103) ----------------------------------
+
+def _covered_axis(h: float, lims: tuple[float, float]) -> tuple[float, int]:
+    """Node count of range(xmin, xmax; step=h), expanded to cover xmax (:74-86)."""
+    xmin, xmax = float(lims[0]), float(lims[1])
+    n = int(math.floor((xmax - xmin) / h + 1e-9)) + 1
+    if not math.isclose(xmin + (n - 1) * h, xmax, rel_tol=math.sqrt(2.220446049250313e-16), abs_tol=0.0):
+        n += 1
+    return xmin, n
+
+
+class CartesianMesh:
+    """CartesianMesh([T,] h, (xmin, xmax), ...; block_size_log2=2)."""
+
+    def __init__(self, h: float, *minmax: Sequence[float], dtype: torch.dtype = torch.float64,
+                 block_size_log2: int = 2):
+        if len(minmax) not in (2, 3):
+            raise UnsupportedError("CartesianMesh: only 2D and 3D meshes are supported")
+        if not all(lo < hi for lo, hi in minmax):
+            raise TesseraeAssertionError("CartesianMesh: each axis needs xmin < xmax")
+        if not isinstance(block_size_log2, int) or block_size_log2 < 0:
+            raise ArgumentError(f"block_size_log2 must be non-negative, got {block_size_log2}")
+        if dtype not in (torch.float32, torch.float64):
+            raise ArgumentError("CartesianMesh dtype must be torch.float32 or torch.float64")
+        self.dim = len(minmax)
+        self.h = float(h)
+        self.h_inv = 1.0 / self.h
+        axes = [_covered_axis(self.h, lims) for lims in minmax]
+        self.xmin = tuple(a[0] for a in axes)
+        self.dims = tuple(a[1] for a in axes)
+        self.dtype = dtype
+        self.block_size_log2 = block_size_log2
+
+    # Julia accessors
+    def size(self) -> tuple[int, ...]:
+        return self.dims
+
+    def spacing(self) -> float:
+        return self.h
+
+    def nnodes(self) -> int:
+        return int(np.prod(self.dims))
+
+    def nblocks(self) -> tuple[int, ...]:
+        """nblocks(mesh) (cartesian.jl:241-243)."""
+        return tuple(((n - 1) >> self.block_size_log2) + 1 for n in self.dims)
+
+    def xmax(self) -> tuple[float, ...]:
+        return tuple(x0 + (n - 1) * self.h for x0, n in zip(self.xmin, self.dims))
+
+    def volume(self) -> float:
+        """volume(mesh) = prod(xmax - xmin) (cartesian.jl:66-71)."""
+        return float(np.prod([b - a for a, b in zip(self.xmin, self.xmax())]))
+
+    def axis(self, d: int) -> np.ndarray:
+        """Node coordinates along axis d, as the library computes them: T(xmin + i*h)."""
+        npdt = np.float32 if self.dtype == torch.float32 else np.float64
+        return (self.xmin[d] + np.arange(self.dims[d], dtype=np.float64) * self.h).astype(npdt)
+
+    @property
+    def tess_dtype(self) -> int:
+        return TESS_FLOAT32 if self.dtype == torch.float32 else TESS_FLOAT64
+
+    def abi(self) -> _abi.tess_mesh:
+        m = _abi.tess_mesh()
+        m.dim = self.dim
+        m.dtype = self.tess_dtype
+        m.block_size_log2 = self.block_size_log2
+        for d in range(3):
+            m.dims[d] = self.dims[d] if d < self.dim else 1
+            m.xmin[d] = self.xmin[d] if d < self.dim else 0.0
+        m.h = self.h
+        m.h_inv = self.h_inv
+        return m
+
+    def __repr__(self) -> str:
+        return f"CartesianMesh{self.dims}(h={self.h}, xmin={self.xmin}, T={self.dtype})"
+
+
+# ---- StructArray-like containers -------------------------------------------
+
+def _field(t: torch.Tensor | None, ncomp: int, layout: str) -> _abi.tess_field:
+    f = _abi.tess_field()
+    if t is None:
+        f.ptr = None
+        return f
+    if not t.is_contiguous():
+        raise ArgumentError("fields must be contiguous tensors")
+    f.ptr = t.data_ptr()
+    if layout == "soa" and ncomp > 1:
+        f.elem_stride, f.comp_stride = 1, t.shape[-1]
+    else:
+        f.elem_stride, f.comp_stride = ncomp, 1
+    return f
+
+
+class Particles:
+    """Particle StructArray of the README program (README.md:53-54).
+
+    Fields: x (n,d), v (n,d), V (n,), grad_v / F / sigma (n, d*d) with
+    column-major components (Tensorial Mat layout), optional l (n,) for uGIMP
+    and logE (n,) for per-particle stiffness.  ``layout="soa"`` stores each
+    field component-major ((ncomp, n)) instead.
+    """
+
+    FIELDS = ("x", "v", "V", "grad_v", "F", "sigma", "l", "logE")
+
+    def __init__(self, n: int, dim: int, dtype: torch.dtype, device="cuda", layout: str = "julia",
+                 with_l: bool = False, with_logE: bool = False):
+        self.n, self.dim, self.dtype, self.device, self.layout = int(n), dim, dtype, device, layout
+        self.tensors: dict[str, torch.Tensor | None] = {}
+        for name in self.FIELDS:
+            nc = self.ncomp(name)
+            if (name == "l" and not with_l) or (name == "logE" and not with_logE):
+                self.tensors[name] = None
+                continue
+            shape = (self.n,) if nc == 1 else ((nc, self.n) if layout == "soa" else (self.n, nc))
+            self.tensors[name] = torch.zeros(shape, dtype=dtype, device=device)
+
+    def ncomp(self, name: str) -> int:
+        return {"x": self.dim, "v": self.dim, "V": 1, "grad_v": self.dim ** 2, "F": self.dim ** 2,
+                "sigma": self.dim ** 2, "l": 1, "logE": 1}[name]
+
+    def __getattr__(self, name):
+        t = self.__dict__.get("tensors")
+        if t is not None and name in t:
+            return t[name]
+        raise AttributeError(name)
+
+    def __len__(self) -> int:
+        return self.n
+
+    def get(self, name: str) -> torch.Tensor:
+        """Field as (n, ncomp) (or (n,)) regardless of layout."""
+        t = self.tensors[name]
+        return t.t() if (self.layout == "soa" and t is not None and t.dim() == 2) else t
+
+    def set(self, name: str, value) -> None:
+        t = self.tensors[name]
+        v = torch.as_tensor(value, dtype=self.dtype).to(t.device)
+        if self.layout == "soa" and t.dim() == 2:
+            t.copy_(v.reshape(self.n, -1).t())
+        else:
+            t.copy_(v.reshape(t.shape))
+
+    def abi(self) -> _abi.tess_particles:
+        P = _abi.tess_particles()
+        P.n = self.n
+        for name in self.FIELDS:
+            setattr(P, name, _field(self.tensors[name], self.ncomp(name), self.layout))
+        return P
+
+
+class Grid:
+    """Grid StructArray of the README program: m (N,), mv / f / vn / v (N, d); x-fastest nodes."""
+
+    FIELDS = ("m", "mv", "f", "vn", "v")
+
+    def __init__(self, mesh: CartesianMesh, device="cuda", layout: str = "julia"):
+        self.mesh, self.layout = mesh, layout
+        N, d = mesh.nnodes(), mesh.dim
+        self.tensors = {}
+        for name in self.FIELDS:
+            if name == "m":
+                self.tensors[name] = torch.zeros((N,), dtype=mesh.dtype, device=device)
+            else:
+                shape = (d, N) if layout == "soa" else (N, d)
+                self.tensors[name] = torch.zeros(shape, dtype=mesh.dtype, device=device)
+
+    def __getattr__(self, name):
+        t = self.__dict__.get("tensors")
+        if t is not None and name in t:
+            return t[name]
+        raise AttributeError(name)
+
+    def get(self, name: str) -> torch.Tensor:
+        t = self.tensors[name]
+        return t.t() if (self.layout == "soa" and t.dim() == 2) else t
+
+    def abi(self) -> _abi.tess_grid:
+        G = _abi.tess_grid()
+        for name in self.FIELDS:
+            setattr(G, name, _field(self.tensors[name], 1 if name == "m" else self.mesh.dim, self.layout))
+        return G
+
+
+def generate_grid(mesh: CartesianMesh, device="cuda", layout: str = "julia") -> Grid:
+    """generate_grid(GridProp, mesh) for the README grid schema (src/Grids/grid.jl:70-86); fields zeroed."""
+    return Grid(mesh, device=device, layout=layout)
+
+
+def grid_sampling_points(mesh: CartesianMesh, spacing: float = 0.5) -> np.ndarray:
+    """GridSampling positions (src/particles.jl:9-18): axis (xmin+l/2):l:xmax, l = spacing*h, x fastest."""
+    l = spacing * mesh.h
+    axes = []
+    for x0, x1 in zip(mesh.xmin, mesh.xmax()):
+        start = x0 + l / 2
+        n = int(math.floor((x1 - start) / l + 1e-9)) + 1
+        axes.append(start + np.arange(n, dtype=np.float64) * l)
+    grids = np.meshgrid(*axes, indexing="ij")
+    return np.stack([g.reshape(-1, order="F") for g in grids], axis=1)
+
+
+def generate_particles(mesh: CartesianMesh, n: int | None = None, positions=None, device="cuda",
+                       layout: str = "julia", with_l: bool = False, with_logE: bool = False) -> Particles:
+    """generate_particles(ParticleProp, mesh) for the README particle schema.
+
+    With no ``positions`` the GridSampling points of the mesh are used; F starts at I.
+    """
+    pts = grid_sampling_points(mesh) if positions is None else np.asarray(positions, dtype=np.float64)
+    n = len(pts) if n is None else n
+    P = Particles(n, mesh.dim, mesh.dtype, device=device, layout=layout, with_l=with_l, with_logE=with_logE)
+    P.set("x", pts[:n])
+    eye = np.eye(mesh.dim).reshape(-1, order="F")
+    P.set("F", np.broadcast_to(eye, (n, mesh.dim ** 2)))
+    return P
+
+
+class BasisWeights:
+    """BasisWeightArray (src/Basis/basis_weight.jl:345-358) in stored or deferred mode."""
+
+    def __init__(self, basis: Basis, mesh: CartesianMesh, n: int, deferred: bool = False, device="cuda"):
+        self.basis, self.mesh, self.n, self.deferred = basis, mesh, int(n), deferred
+        sw, d = basis.support_width, mesh.dim
+        if deferred:
+            self.w = self.grad_w = self.window = None
+        else:
+            self.w = torch.zeros((self.n, sw ** d), dtype=mesh.dtype, device=device)
+            self.grad_w = torch.zeros((self.n, sw ** d, d), dtype=mesh.dtype, device=device)
+            self.window = torch.zeros((self.n, 2 * d), dtype=torch.int64, device=device)
+
+    def __len__(self) -> int:
+        return self.n
+
+    def abi(self) -> _abi.tess_weights | None:
+        if self.deferred:
+            return None
+        W = _abi.tess_weights()
+        W.w = _field(self.w, 1, "julia")
+        W.grad_w = _field(self.grad_w, self.mesh.dim, "julia")
+        W.window = self.window.data_ptr()
+        return W
+
+
+def generate_basis_weights(basis: Basis, mesh: CartesianMesh, n: int, deferred: bool = False,
+                           device="cuda") -> BasisWeights:
+    """generate_basis_weights([T,] basis, mesh, n; deferred) (basis_weight.jl:514-520)."""
+    return BasisWeights(basis, mesh, n, deferred=deferred, device=device)
+
+
+class Partition:
+    """Partition(mesh) with the CPU block strategy's ordering (src/Parallel/partition.jl:37-56)."""
+
+    def __init__(self, mesh: CartesianMesh, device="cuda"):
+        self.mesh = mesh
+        nb = int(np.prod(mesh.nblocks()))
+        self.starts = torch.zeros((nb,), dtype=torch.int32, device=device)
+        self.stops = torch.zeros((nb,), dtype=torch.int32, device=device)
+        self.particleindices = torch.zeros((0,), dtype=torch.int32, device=device)
+        self.nassigned = 0
+        self.device = device
+
+
+@dataclass
+class MPMParams:
+    """Constants of the README program (README.md:46-49,57) plus gravity and slip walls."""
+
+    dt: float
+    E: float
+    nu: float
+    rho0: float
+    V0: float
+    gravity: tuple = (0.0, 0.0, 0.0)
+    slip_walls: int = 0
+    lam: float = field(init=False)
+    mu: float = field(init=False)
+    mp: float = field(init=False)
+
+    def __post_init__(self):
+        E, nu = self.E, self.nu
+        self.lam = (E * nu) / ((1 + nu) * (1 - 2 * nu))
+        self.mu = E / (2 * (1 + nu))
+        self.mp = self.rho0 * self.V0
+
+    def abi(self) -> _abi.tess_params:
+        p = _abi.tess_params()
+        p.dt, p.lambda_, p.mu, p.mp, p.V0 = self.dt, self.lam, self.mu, self.mp, self.V0
+        g = list(self.gravity) + [0.0] * (3 - len(self.gravity))
+        for d in range(3):
+            p.gravity[d] = g[d]
+        p.slip_walls = int(self.slip_walls)
+        return p
+
+
+# ---- context ---------------------------------------------------------------
+
+class Context:
+    """Owns one tess_ctx bound to a device and a HIP stream."""
+
+    def __init__(self, device: int | None = None, stream=None):
+        self.lib = _abi.load_library()
+        self.device = torch.cuda.current_device() if device is None else int(device)
+        s = torch.cuda.current_stream(self.device) if stream is None else stream
+        self.stream = s
+        handle = ctypes.c_void_p()
+        _check(self.lib, self.lib.tess_ctx_create(self.device, ctypes.c_void_p(s.cuda_stream), ctypes.byref(handle)))
+        self.handle = handle
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            _check(self.lib, self.lib.tess_ctx_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self) -> None:
+        _check(self.lib, self.lib.tess_sync(self.handle))
+
+
+_contexts: dict = {}
+
+
+def default_context() -> Context:
+    dev = torch.cuda.current_device()
+    key = (dev, torch.cuda.current_stream(dev).cuda_stream)
+    if key not in _contexts:
+        _contexts[key] = Context(dev)
+    return _contexts[key]
+
+
+# ---- operations --------------------------------------------------------------
+
+def _require_device(*tensors) -> None:
+    for t in tensors:
+        if t is not None and not t.is_cuda:
+            raise ArgumentError("libtesserae_hip runs on the GPU: move grid, particles and weights to 'cuda'")
+
+
+def update(obj, *args, ctx: Context | None = None, **kwargs):
+    """Julia's ``update!``: ``update(weights, particles, mesh)`` or ``update(partition, x)``."""
+    if isinstance(obj, BasisWeights):
+        return _update_weights(obj, *args, ctx=ctx, **kwargs)
+    if isinstance(obj, Partition):
+        return _update_partition(obj, *args, ctx=ctx)
+    raise ArgumentError(f"update: no method for {type(obj).__name__}")
+
+
+def _update_weights(weights: BasisWeights, particles: Particles, mesh: CartesianMesh, ctx=None, deferred=None):
+    """update!(weights, particles, mesh) (basis_weight.jl:714-752)."""
+    if weights.deferred:
+        if deferred is False:
+            raise ErrorException("update!: these weights were built with `deferred=true` and have no storage for "
+                                 "basis values, so they cannot be materialized.")
+        return weights
+    if len(weights) < len(particles):
+        raise TesseraeAssertionError("length(weights) ≥ n")
+    _require_device(particles.x, weights.w)
+    ctx = ctx or default_context()
+    P = particles.abi()
+    _check(ctx.lib, ctx.lib.tess_update_weights(ctx.handle, weights.basis.kind, ctypes.byref(mesh.abi()),
+                                                ctypes.byref(P), ctypes.byref(weights.abi())))
+    return weights
+
+
+def _update_partition(partition: Partition, x: torch.Tensor, ctx=None, layout: str = "julia"):
+    """update!(partition, x) with CPUBlockStrategy ordering (cpu_block_strategy.jl:62-75)."""
+    _require_device(x)
+    ctx = ctx or default_context()
+    n = x.shape[0] if layout == "julia" else x.shape[-1]
+    if partition.particleindices.numel() != n:
+        partition.particleindices = torch.zeros((n,), dtype=torch.int32, device=x.device)
+    na = ctypes.c_int64(0)
+    f = _field(x, partition.mesh.dim, layout)
+    _check(ctx.lib, ctx.lib.tess_partition_update(ctx.handle, ctypes.byref(partition.mesh.abi()), f, n,
+                                                  ctypes.c_void_p(partition.particleindices.data_ptr()),
+                                                  ctypes.c_void_p(partition.starts.data_ptr()),
+                                                  ctypes.c_void_p(partition.stops.data_ptr()), ctypes.byref(na)))
+    partition.nassigned = int(na.value)
+    return partition
+
+
+def reorder_particles(particles: Particles, partition: Partition, threshold: float = 1.0, ctx=None) -> bool:
+    """reorder_particles!(particles, partition; threshold) (cpu_block_strategy.jl:248-269)."""
+    ctx = ctx or default_context()
+    out = ctypes.c_int32(0)
+    P = particles.abi()
+    _check(ctx.lib, ctx.lib.tess_reorder_particles(ctx.handle, ctypes.byref(partition.mesh.abi()), ctypes.byref(P),
+                                                   ctypes.c_void_p(partition.particleindices.data_ptr()),
+                                                   partition.nassigned, float(threshold), ctypes.byref(out)))
+    return bool(out.value)
+
+
+def P2G(grid: Grid, particles: Particles, weights, params: MPMParams, ctx=None) -> None:
+    """@P2G of the README program (README.md:71-77), deterministic."""
+    basis, Wp = _resolve_weights(weights)
+    _require_device(particles.x, grid.m)
+    ctx = ctx or default_context()
+    P, G = particles.abi(), grid.abi()
+    _check(ctx.lib, ctx.lib.tess_p2g(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
+                                     ctypes.byref(P), Wp, ctypes.byref(params.abi())))
+
+
+def G2P(grid: Grid, particles: Particles, weights, params: MPMParams, ctx=None) -> None:
+    """@G2P of the README program (README.md:80-87)."""
+    basis, Wp = _resolve_weights(weights)
+    _require_device(particles.x, grid.m)
+    ctx = ctx or default_context()
+    P, G = particles.abi(), grid.abi()
+    _check(ctx.lib, ctx.lib.tess_g2p(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
+                                     ctypes.byref(P), Wp, ctypes.byref(params.abi())))
+
+
+def _resolve_weights(weights):
+    """(basis, pointer-to-tess_weights or None for deferred evaluation)."""
+    if isinstance(weights, Basis):
+        return weights, None
+    if not isinstance(weights, BasisWeights):
+        raise ArgumentError(f"invalid `BasisWeight`s, got type {type(weights).__name__}")
+    W = weights.abi()
+    if W is None:
+        return weights.basis, None
+    weights._abi_keepalive = W
+    return weights.basis, ctypes.pointer(W)
+
+
+def step(grid: Grid, particles: Particles, basis: Basis, params: MPMParams, nsteps: int = 1, ctx=None) -> None:
+    """nsteps of the README loop body with deferred weights (README.md:66-87)."""
+    _require_device(particles.x, grid.m)
+    ctx = ctx or default_context()
+    P, G = particles.abi(), grid.abi()
+    _check(ctx.lib, ctx.lib.tess_step(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
+                                      ctypes.byref(P), ctypes.byref(params.abi()), int(nsteps)))

@@ -1,0 +1,131 @@
+"""Seeded synthetic scenes for the README step (host numpy state).
+
+Each builder returns a ``Scene``: the mesh, the basis, the program constants
+and the initial particle state as float64 numpy arrays in the Julia-native
+layout (x/v: (n, d); V: (n,); F/sigma/grad_v: (n, d*d) column-major).  The
+same arrays feed the GPU engine and the CPU oracle, so both start from
+identical inputs.  Random numbers come from splitmix64, a fixed published
+generator (Steele, Lea & Flood 2014), so every machine draws the same values.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .core import BSpline, Basis, CartesianMesh, MPMParams, Quadratic, grid_sampling_points
+
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+_G = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64(seed: int, n: int, offset: int = 0) -> np.ndarray:
+    """The n outputs of splitmix64 seeded with ``seed``, starting at draw ``offset``."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (np.arange(offset + 1, offset + n + 1, dtype=np.uint64) * _G)
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed: int, n: int, offset: int = 0) -> np.ndarray:
+    """Uniform doubles in [0, 1) from the top 53 bits of splitmix64."""
+    return (splitmix64(seed, n, offset) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def normal(seed: int, n: int, offset: int = 0) -> np.ndarray:
+    """Standard normals by Box-Muller over two uniform streams."""
+    u1 = uniform01(seed, n, offset)
+    u2 = uniform01(seed ^ 0x5DEECE66D, n, offset)
+    return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * math.pi * u2)
+
+
+@dataclass
+class Scene:
+    name: str
+    mesh: CartesianMesh
+    basis: Basis
+    params: MPMParams
+    x: np.ndarray
+    v: np.ndarray
+    V: np.ndarray
+    F: np.ndarray
+    sigma: np.ndarray
+    grad_v: np.ndarray
+    l: np.ndarray | None = None
+    logE: np.ndarray | None = None
+    steps: int = 1
+
+    @property
+    def n(self) -> int:
+        return len(self.x)
+
+    def copy(self) -> "Scene":
+        c = Scene(**{k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in self.__dict__.items()})
+        return c
+
+
+def _state(x, v, V0, dim):
+    n = len(x)
+    eye = np.eye(dim).reshape(-1, order="F")
+    return dict(x=np.ascontiguousarray(x, dtype=np.float64), v=np.ascontiguousarray(v, dtype=np.float64),
+                V=np.full(n, V0, dtype=np.float64), F=np.tile(eye, (n, 1)), sigma=np.zeros((n, dim * dim)),
+                grad_v=np.zeros((n, dim * dim)))
+
+
+def two_disks(dtype=torch.float64, h: float = 0.01, steps: int = 1000) -> Scene:
+    """README example (README.md:44-62): two elastic disks colliding on [0,1]^2."""
+    mesh = CartesianMesh(h, (0, 1), (0, 1), dtype=dtype)
+    pts = grid_sampling_points(mesh)
+    V0 = mesh.volume() / len(pts)  # volume / count before the filter
+    r = 0.2
+    keep = (np.linalg.norm(pts - r, axis=1) < r) | (np.linalg.norm(pts - (1 - r), axis=1) < r)
+    x = pts[keep]
+    v = np.where(x[:, :1] > 0.5, -0.1, 0.1) * np.ones((1, 2))
+    params = MPMParams(dt=1e-3, E=500.0, nu=0.3, rho0=1e3, V0=V0)
+    return Scene("two_disks", mesh, BSpline(Quadratic()), params, steps=steps, **_state(x, v, V0, 2))
+
+
+def block_drop(dtype=torch.float32, dx: float = 1 / 256, seed: int = 1, steps: int = 100,
+               box=((0.25, 0.75), (0.25, 0.75), (0.5, 0.9))) -> Scene:
+    """3D elastic block falling under gravity with slip walls (2x2x2 particles per cell, jittered)."""
+    mesh = CartesianMesh(dx, (0, 1), (0, 1), (0, 1), dtype=dtype)
+    s = dx / 2
+    axes = []
+    for lo, hi in box:
+        k = int(math.floor((hi - lo) / s - 0.5 + 1e-9)) + 1  # count of lo + (k+1/2) s < hi
+        axes.append(lo + (np.arange(k) + 0.5) * s)
+    g = np.meshgrid(*axes, indexing="ij")
+    x = np.stack([a.reshape(-1, order="F") for a in g], axis=1)
+    n = len(x)
+    jit = (uniform01(seed, 3 * n).reshape(n, 3) * 2.0 - 1.0) * (0.1 * dx)
+    x = x + jit
+    V0 = s ** 3
+    params = MPMParams(dt=1e-4, E=1e6, nu=0.3, rho0=1e3, V0=V0, gravity=(0.0, 0.0, -9.81), slip_walls=0b111111)
+    return Scene("block_drop", mesh, BSpline(Quadratic()), params, steps=steps,
+                 **_state(x, np.zeros_like(x), V0, 3))
+
+
+def random_box(dim: int, n: int, seed: int, basis: Basis, dtype=torch.float64, h: float = 0.1,
+               extent: float = 1.0, vscale: float = 0.3, lo: float = 0.0, hi: float = 1.0,
+               stress: float = 0.0, with_l: bool = False) -> Scene:
+    """n particles uniform in [lo, hi]^dim of a mesh on [0, extent]^dim, random v and a small random F."""
+    mesh = CartesianMesh(h, *[(0.0, extent)] * dim, dtype=dtype)
+    u = uniform01(seed, n * dim).reshape(n, dim)
+    x = lo + (hi - lo) * u
+    v = (uniform01(seed + 1, n * dim).reshape(n, dim) - 0.5) * 2 * vscale
+    V0 = (extent ** dim) / max(n, 1)
+    st = _state(x, v, V0, dim)
+    if stress:
+        d2 = dim * dim
+        pert = (uniform01(seed + 2, n * d2).reshape(n, d2) - 0.5) * 2 * stress
+        st["F"] = st["F"] + 0.05 * pert
+        sym = pert.reshape(n, dim, dim)
+        st["sigma"] = (sym + sym.transpose(0, 2, 1)).reshape(n, d2)
+    params = MPMParams(dt=1e-3, E=1e3, nu=0.3, rho0=1e3, V0=V0)
+    l = np.full(n, 0.5 * h) if with_l else None
+    return Scene(f"random_box_{dim}d", mesh, basis, params, l=l, **st)
