@@ -4,7 +4,8 @@
 // @G2P lowering src/Transfer/g2p.jl:110-157; deferred weights
 // src/Transfer/runtime.jl:21-24,55-130.
 //
-// Data flow of one P2G (no float atomics anywhere):
+// Data flow of one P2G (no global float atomics; every LDS add instruction
+// touches each node at most once, so the sums follow a fixed order):
 //   k_bin_keys      particle -> tile whose footprint holds its support window
 //   radix sort      stable (tile, particle id) order -> perm
 //   k_tile_runs / k_tile_flags / scan / k_active_list   tile ranges + slots
@@ -440,7 +441,12 @@ __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a) {
         }
       }
       // add this lane's class contributions into the LDS tile: all lanes are
-      // distinct classes, so for a fixed offset their targets are distinct
+      // distinct classes, so for a fixed offset their targets are distinct and
+      // every LDS add instruction touches each node at most once.  The adds are
+      // LDS atomics so that the compiler keeps each of them a single
+      // read-modify-write (a plain += would let it hoist one offset's load
+      // above another offset's store, which other lanes may target); the
+      // per-node order of the adds is the fixed program order.
       if (ccount > 0) {
 #pragma unroll
         for (int s2 = 0; s2 < NSL; ++s2) {
@@ -453,7 +459,7 @@ __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a) {
           }
           q += (ccoord[D - 1] + al) * mul;
 #pragma unroll
-          for (int v = 0; v < NV; ++v) s_tile[v][q] += acc[s2][v];
+          for (int v = 0; v < NV; ++v) atomicAdd(&s_tile[v][q], acc[s2][v]);
         }
       }
     }
