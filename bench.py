@@ -34,6 +34,7 @@ def parse():
     ap.add_argument("--layout", default="julia", choices=["julia", "soa"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--reorder", type=float, default=0.85, help="reorder_particles! threshold per step")
     return ap.parse_args()
 
 
@@ -81,15 +82,12 @@ def main():
     sc = block_drop(dtype=torch.float32, dx=args.dx)
     mesh, P, G = to_gpu(sc, layout=args.layout)
     n = sc.n
-    part = th.Partition(mesh)
     ctx = th.default_context()
-    # reorder once by block order, as a user following the reference's advice would
-    th.update(part, P.x)
-    th.reorder_particles(P, part, threshold=1.0)
     torch.cuda.synchronize()
 
+    # the reference loop's maintenance: reorder_particles!(...; threshold=0.85) every step
     def one_step():
-        th.step(G, P, sc.basis, sc.params, nsteps=1, ctx=ctx)
+        th.step(G, P, sc.basis, sc.params, nsteps=1, reorder_threshold=args.reorder, ctx=ctx)
 
     log(f"bench: n={n} warmup={args.warmup}")
     for _ in range(args.warmup):

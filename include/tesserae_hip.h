@@ -186,9 +186,17 @@ tess_status tess_g2p(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const 
                      const tess_params* params);
 
 /* One README step with deferred weights: update!(wgts,…) + @P2G + @G2P
- * (README.md:66-87), repeated nsteps times on the context's stream. */
+ * (README.md:66-87), repeated nsteps times on the context's stream.
+ * reorder_threshold plays the role of the reference loop's
+ * reorder_particles!(particles, partition; threshold)
+ * (src/Parallel/cpu_block_strategy.jl:248-269): before a step's P2G, the
+ * particle fields are permuted into the engine's tile order when the fraction
+ * of particles already in place (measured by the previous step) is below the
+ * threshold; 0 never reorders, 1 reorders every step.  Reordering permutes the
+ * caller's particle arrays, exactly like reorder_particles!. */
 tess_status tess_step(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
-                      const tess_particles* particles, const tess_params* params, int32_t nsteps);
+                      const tess_particles* particles, const tess_params* params, int32_t nsteps,
+                      double reorder_threshold);
 
 #ifdef __cplusplus
 }

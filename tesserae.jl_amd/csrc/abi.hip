@@ -249,12 +249,13 @@ tess_status tess_g2p(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const 
     check_particles(particles, basis, true);
     check_weights(weights);
     check_params(params);
-    run_g2p(ctx, basis, *mesh, *grid, *particles, weights, *params, false);
+    run_g2p(ctx, basis, *mesh, *grid, *particles, weights, *params);
   });
 }
 
 tess_status tess_step(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
-                      const tess_particles* particles, const tess_params* params, int32_t nsteps) {
+                      const tess_particles* particles, const tess_params* params, int32_t nsteps,
+                      double reorder_threshold) {
   return guarded([&] {
     check_ctx(ctx);
     check_mesh(mesh);
@@ -263,10 +264,9 @@ tess_status tess_step(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const
     check_particles(particles, basis, true);
     check_params(params);
     if (nsteps < 0) raise(TESS_ERR_ARGUMENT, "nsteps must be non-negative");
-    for (int32_t s = 0; s < nsteps; ++s) {
-      run_p2g(ctx, basis, *mesh, *grid, *particles, nullptr, *params);
-      run_g2p(ctx, basis, *mesh, *grid, *particles, nullptr, *params, true);
-    }
+    if (!(reorder_threshold >= 0.0 && reorder_threshold <= 1.0))
+      raise(TESS_ERR_ARGUMENT, "threshold must be in [0, 1].");
+    run_step(ctx, basis, *mesh, *grid, *particles, *params, nsteps, reorder_threshold);
   });
 }
 

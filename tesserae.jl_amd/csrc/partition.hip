@@ -346,6 +346,30 @@ void run_partition_update(tess_ctx* ctx, const tess_mesh& mesh, tess_field x, in
   raise(TESS_ERR_UNSUPPORTED, "unsupported (dim, dtype)");
 }
 
+void permute_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const uint32_t* perm) {
+  const int64_t n = P.n;
+  const int D = mesh.dim;
+  if (mesh.dtype == TESS_FLOAT32) {
+    permute_field<uint32_t>(ctx, P.x, D, perm, n);
+    permute_field<uint32_t>(ctx, P.v, D, perm, n);
+    permute_field<uint32_t>(ctx, P.V, 1, perm, n);
+    permute_field<uint32_t>(ctx, P.grad_v, D * D, perm, n);
+    permute_field<uint32_t>(ctx, P.F, D * D, perm, n);
+    permute_field<uint32_t>(ctx, P.sigma, D * D, perm, n);
+    permute_field<uint32_t>(ctx, P.l, 1, perm, n);
+    permute_field<uint32_t>(ctx, P.logE, 1, perm, n);
+  } else {
+    permute_field<uint64_t>(ctx, P.x, D, perm, n);
+    permute_field<uint64_t>(ctx, P.v, D, perm, n);
+    permute_field<uint64_t>(ctx, P.V, 1, perm, n);
+    permute_field<uint64_t>(ctx, P.grad_v, D * D, perm, n);
+    permute_field<uint64_t>(ctx, P.F, D * D, perm, n);
+    permute_field<uint64_t>(ctx, P.sigma, D * D, perm, n);
+    permute_field<uint64_t>(ctx, P.l, 1, perm, n);
+    permute_field<uint64_t>(ctx, P.logE, 1, perm, n);
+  }
+}
+
 void run_reorder(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, int32_t* pidx, int64_t na,
                  double threshold, int32_t* reordered) {
   *reordered = 0;
@@ -388,26 +412,7 @@ void run_reorder(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, 
   if (hc[0] & 2u) raise(TESS_ERR_STATE, "reorder_particles!: particle is duplicated in the block assignment.");
   hipLaunchKernelGGL(k_build_perm, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, pidx, na, n, seen, pos, perm);
   TESS_HIP(hipGetLastError());
-  const int D = mesh.dim;
-  if (mesh.dtype == TESS_FLOAT32) {
-    permute_field<uint32_t>(ctx, P.x, D, perm, n);
-    permute_field<uint32_t>(ctx, P.v, D, perm, n);
-    permute_field<uint32_t>(ctx, P.V, 1, perm, n);
-    permute_field<uint32_t>(ctx, P.grad_v, D * D, perm, n);
-    permute_field<uint32_t>(ctx, P.F, D * D, perm, n);
-    permute_field<uint32_t>(ctx, P.sigma, D * D, perm, n);
-    permute_field<uint32_t>(ctx, P.l, 1, perm, n);
-    permute_field<uint32_t>(ctx, P.logE, 1, perm, n);
-  } else {
-    permute_field<uint64_t>(ctx, P.x, D, perm, n);
-    permute_field<uint64_t>(ctx, P.v, D, perm, n);
-    permute_field<uint64_t>(ctx, P.V, 1, perm, n);
-    permute_field<uint64_t>(ctx, P.grad_v, D * D, perm, n);
-    permute_field<uint64_t>(ctx, P.F, D * D, perm, n);
-    permute_field<uint64_t>(ctx, P.sigma, D * D, perm, n);
-    permute_field<uint64_t>(ctx, P.l, 1, perm, n);
-    permute_field<uint64_t>(ctx, P.logE, 1, perm, n);
-  }
+  permute_particles(ctx, mesh, P, perm);
   if (na > 0) {
     hipLaunchKernelGGL(k_iota1, dim3(nblk(na, 256)), dim3(256), 0, ctx->stream, pidx, na);
     TESS_HIP(hipGetLastError());

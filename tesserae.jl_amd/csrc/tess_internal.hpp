@@ -69,8 +69,12 @@ struct tess_ctx {
   int32_t prev_tile_log2 = -1;
   int64_t prev_ntiles = -1;
   int64_t prev_nnodes = -1;
-  // reorder scratch
+  // reorder scratch and the layout score of the last step's P2G
   tess::Buffer reorder_tmp;
+  const void* misplaced_x_ptr = nullptr;
+  int64_t misplaced_n = 0;
+  int64_t last_misplaced = -1;
+  int64_t reorders = 0;
 };
 
 namespace tess {
@@ -95,10 +99,12 @@ void run_reorder(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, 
 // ---- transfers (transfer.hip) ----
 void run_p2g(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
              const tess_weights* W, const tess_params& prm);
-// reuse_binning: the caller guarantees that the last binning on ctx was made
-// by run_p2g for these very particles and positions (within one step).
 void run_g2p(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-             const tess_weights* W, const tess_params& prm, bool reuse_binning);
+             const tess_weights* W, const tess_params& prm);
+void run_step(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+              const tess_params& prm, int nsteps, double reorder_threshold);
+// Moves every particle field so that slot k receives particle order[k].
+void permute_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const uint32_t* order);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

@@ -52,7 +52,9 @@ struct Pow<N, 0> {
   static constexpr int v = 1;
 };
 
-enum { CNT_INSIDE = 0, CNT_ACTIVE = 1, CNT_NB = 2, CNT_ERR = 3, CNT_SLOTS = 4 };
+// device counters: binning totals (zeroed per binning), error flags, and the
+// misplaced-particle count of the last P2G (zeroed per P2G)
+enum { CNT_INSIDE = 0, CNT_ACTIVE = 1, CNT_NB = 2, CNT_ERR = 3, CNT_BIN_SLOTS = 4, CNT_MISPLACED = 8, CNT_SLOTS = 16 };
 enum { ERR_GIMP_LENGTH = 1, ERR_WINDOW = 2 };
 
 template <typename T, int D>
@@ -142,12 +144,17 @@ __global__ __launch_bounds__(256) void k_bin_keys(Args<T, D> a, uint32_t* __rest
   if (p >= a.n) return;
   T x[D], l;
   int lo[D], hi[D];
-  uint32_t key = ntiles;
+  // key = tile * 64 + class: the tile whose footprint [BW*t-1, BW*t+BW+1]
+  // holds the window, and the window start relative to the footprint origin
+  uint32_t key = ntiles * 64u;
   if (particle_window<T, D, BASIS, STORED>(a, p, x, l, lo, hi, counters + CNT_ERR)) {
-    uint32_t k = 0;
+    uint32_t t = 0, c = 0;
 #pragma unroll
-    for (int d = D - 1; d >= 0; --d) k = k * uint32_t(a.ntiles[d]) + uint32_t((lo[d] + 1) >> Geo<D>::LT);
-    key = k;
+    for (int d = D - 1; d >= 0; --d) {
+      t = t * uint32_t(a.ntiles[d]) + uint32_t((lo[d] + 1) >> Geo<D>::LT);
+      c = c * uint32_t(Geo<D>::BW) + uint32_t((lo[d] + 1) & (Geo<D>::BW - 1));
+    }
+    key = t * 64u + c;
   }
   keys[p] = key;
 }
@@ -157,9 +164,9 @@ __global__ void k_tile_runs(const uint32_t* __restrict__ skeys, int64_t n, uint3
                             uint32_t* __restrict__ counters) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t k = skeys[i];
-  const bool first = i == 0 || skeys[i - 1] != k;
-  const bool last = i == n - 1 || skeys[i + 1] != k;
+  const uint32_t k = skeys[i] >> 6;  // tile
+  const bool first = i == 0 || (skeys[i - 1] >> 6) != k;
+  const bool last = i == n - 1 || (skeys[i + 1] >> 6) != k;
   if (k < ntiles) {
     if (first) tile_start[k] = uint32_t(i);
     if (last) tile_end[k] = uint32_t(i + 1);
@@ -267,12 +274,57 @@ __device__ __forceinline__ void axis_jets(T xd, int lo_local, const T* s_nodex, 
 }
 
 // ---- P2G -------------------------------------------------------------------
-template <typename T, int D, int BASIS, bool STORED, int SEG>
-__global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a) {
+
+__device__ __forceinline__ uint64_t lanemask_lt64() { return (uint64_t(1) << (threadIdx.x & 63)) - 1; }
+
+// Per-class counts of one tile from its slice of the sorted keys (class = low 6
+// bits), by wave-level matching; lane c gets count and offset of class c.
+__device__ __forceinline__ void tile_class_counts(const uint32_t* __restrict__ skeys, uint32_t start, int cnt,
+                                                  uint32_t* s_cnt, int& ccount, int& coff, int& maxcnt) {
+  const int lane = threadIdx.x & 63;
+  s_cnt[lane] = 0;
+  __syncthreads();
+  for (int base = 0; base < cnt; base += 64) {
+    const int i = base + lane;
+    const bool valid = i < cnt;
+    const uint32_t c = valid ? (skeys[start + i] & 63u) : 0u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      const bool bit = (c >> b) & 1u;
+      const uint64_t m = __ballot(valid && bit);
+      peers &= bit ? m : ~m;
+    }
+    if (valid && (peers & lanemask_lt64()) == 0) atomicAdd(&s_cnt[c], uint32_t(__popcll(peers)));
+  }
+  __syncthreads();
+  ccount = int(s_cnt[lane]);
+  int incl = ccount;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += u;
+  }
+  coff = incl - ccount;
+  maxcnt = ccount;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) maxcnt = max(maxcnt, __shfl_xor(maxcnt, off, 64));
+}
+
+// One wave per tile.  The sort key orders each tile's particles by class
+// (window start relative to the footprint origin), so lane c owns class c: it
+// walks its particles, accumulates their contributions to the sw^dim nodes of
+// the class window in registers, and adds them into the LDS tile.  Lanes hold
+// distinct classes, so every LDS add instruction touches each node at most once
+// and the per-node order of the adds is the fixed program order.
+template <typename T, int D, int BASIS, bool STORED, int NPASS>
+__global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                  uint32_t* __restrict__ misplaced) {
   using G = Geo<D>;
   constexpr int SW = support_width(BASIS);
   constexpr int NV = 1 + 2 * D;
-  constexpr int NSL = Pow<SW, D - 1>::v;  // offsets per slice of the last axis
+  constexpr int NOFF = Pow<SW, D>::v;
+  constexpr int NSL = NOFF / NPASS;  // offsets per pass (NPASS is 1 or SW)
   const int lane = threadIdx.x;
   const int64_t j = xcd_remap(blockIdx.x, a.n_active);
   if (j >= a.n_active) return;
@@ -286,21 +338,14 @@ __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a) {
 
   __shared__ T s_tile[NV][G::NFOOT];
   __shared__ T s_nodex[D][G::TS];
-  __shared__ T s_x[D][SEG];
-  __shared__ T s_mv[D][SEG];
-  __shared__ T s_A[D * D][SEG];
-  __shared__ T s_l[BASIS == TESS_UGIMP ? SEG : 1];
-  __shared__ uint32_t s_pid[STORED ? SEG : 1];
-  __shared__ uint8_t s_cls[SEG];
-  __shared__ uint8_t s_ext[SEG];
-  __shared__ uint16_t s_order[SEG];
-
+  __shared__ uint32_t s_cnt[64];
   for (int q = lane; q < NV * G::NFOOT; q += 64) (&s_tile[0][0])[q] = T(0);
   for (int q = lane; q < D * G::TS; q += 64) {
     const int d = q / G::TS, k = q % G::TS;
     s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
   }
-  // class coordinates of this lane: window start offset from the footprint origin
+  int ccount, coff, maxcnt;
+  tile_class_counts(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);
   int ccoord[D];
   {
     int r = lane;
@@ -310,157 +355,106 @@ __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a) {
       r /= G::BW;
     }
   }
+  // particles whose storage index differs from the layered position
+  // (k-th particle of every class contiguous, classes ascending)
+  if (misplaced) {
+    uint32_t bad = 0, layer_off = 0;
+    for (int k = 0; k < maxcnt; ++k) {
+      const uint64_t has = __ballot(k < ccount);
+      if (k < ccount) {
+        const uint32_t want = start + layer_off + uint32_t(__popcll(has & lanemask_lt64()));
+        bad += a.perm[start + coff + k] != want ? 1u : 0u;
+      }
+      layer_off += uint32_t(__popcll(has));
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) bad += __shfl_xor(bad, off, 64);
+    if (lane == 0 && bad) atomicAdd(misplaced, bad);
+  }
   const T mp = a.mp, hinv = a.mesh.h_inv;
-  const int nseg = (cnt + SEG - 1) / SEG;
-  for (int s = 0; s < nseg; ++s) {
-    const int segcnt = (cnt - s + nseg - 1) / nseg;
-    __syncthreads();
-    // stage the segment (particles s, s+nseg, s+2nseg, ... of the tile)
-    for (int i = lane; i < segcnt; i += 64) {
-      const uint32_t pid = a.perm[start + uint32_t(s + i * nseg)];
+  for (int pass = 0; pass < NPASS; ++pass) {
+    T acc[NSL][NV];
+#pragma unroll
+    for (int s2 = 0; s2 < NSL; ++s2)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) acc[s2][v] = T(0);
+    for (int k = 0; k < maxcnt; ++k) {
+      if (k >= ccount) continue;
+      const uint32_t pid = a.perm[start + coff + k];
       T x[D], l;
       int lo[D], hi[D];
       particle_window<T, D, BASIS, STORED>(a, pid, x, l, lo, hi, nullptr);
-      int cls = 0, ext = 0;
+      int ext[D];
 #pragma unroll
-      for (int d = D - 1; d >= 0; --d) {
-        cls = cls * G::BW + (lo[d] - origin[d]);
-        ext = (ext << 2) | (hi[d] - lo[d]);
-      }
-      s_cls[i] = uint8_t(cls);
-      s_ext[i] = uint8_t(ext);
+      for (int d = 0; d < D; ++d) ext[d] = hi[d] - lo[d];
+      // hoisted particle factors (p2g.jl:272-307): mₚ*v[p] and (-V[p])*σ[p]
+      T mv[D], A[D * D];
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        s_x[d][i] = x[d];
-        s_mv[d][i] = mp * a.v(pid, d);  // hoisted run mₚ*v[p], p2g.jl:272-307
-      }
+      for (int d = 0; d < D; ++d) mv[d] = mp * a.v(pid, d);
       const T negV = -a.V(pid);
 #pragma unroll
-      for (int k = 0; k < D * D; ++k) s_A[k][i] = negV * a.sigma(pid, k);  // hoisted (-V[p])*σ[p]
-      if constexpr (BASIS == TESS_UGIMP) s_l[i] = l;
-      if constexpr (STORED) s_pid[i] = pid;
-    }
-    __syncthreads();
-    // counting sort of the segment by class: lane == class
-    int ccount = 0;
-    for (int i = 0; i < segcnt; ++i) ccount += (s_cls[i] == lane) ? 1 : 0;
-    int incl = ccount;
+      for (int q = 0; q < D * D; ++q) A[q] = negV * a.sigma(pid, q);
+      T val[D][SW], der[D][SW];
+      if constexpr (!STORED) {
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int u = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += u;
-    }
-    const int coff = incl - ccount;
-    {
-      int k = 0;
-      for (int i = 0; i < segcnt; ++i)
-        if (s_cls[i] == lane) s_order[coff + (k++)] = uint16_t(i);
-    }
-    int maxcnt = ccount;
+        for (int d = 0; d < D; ++d) axis_jets<T, D, BASIS>(x[d], ccoord[d], s_nodex[d], l, hinv, val[d], der[d]);
+      }
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) maxcnt = max(maxcnt, __shfl_xor(maxcnt, off, 64));
-    __syncthreads();
-    // one pass per offset along the last axis
-    for (int al = 0; al < SW; ++al) {
-      T acc[NSL][NV];
-#pragma unroll
-      for (int s2 = 0; s2 < NSL; ++s2)
-#pragma unroll
-        for (int v = 0; v < NV; ++v) acc[s2][v] = T(0);
-      for (int k = 0; k < maxcnt; ++k) {
-        if (k >= ccount) continue;
-        const int i = s_order[coff + k];
-        const int ext = s_ext[i];
-        int extd[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) extd[d] = (ext >> (2 * d)) & 3;
-        if (al > extd[D - 1]) continue;
-        T xv[D], mv[D], A[D * D];
+      for (int s2 = 0; s2 < NSL; ++s2) {
+        const int o = pass * NSL + s2;
+        int aa[D];
+        int r = o;
+        bool ok = true;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          xv[d] = s_x[d][i];
-          mv[d] = s_mv[d][i];
+          aa[d] = r % SW;
+          r /= SW;
+          ok = ok && aa[d] <= ext[d];
         }
+        if (!ok) continue;
+        T w, g[D];
+        if constexpr (STORED) {
+          const int64_t e = int64_t(o) + int64_t(NOFF) * pid;
+          w = a.w(e);
 #pragma unroll
-        for (int q = 0; q < D * D; ++q) A[q] = s_A[q][i];
-        T val[D][SW], der[D][SW];
-        if constexpr (!STORED) {
-          const T l = BASIS == TESS_UGIMP ? s_l[i] : T(0);
+          for (int c = 0; c < D; ++c) g[c] = a.gw(e, c);
+        } else {
+          // w = Π_d N_d (left-assoc), ∇w_c = (N'_c Π_{d≠c} N_d) h⁻¹
+          w = val[0][aa[0]];
 #pragma unroll
-          for (int d = 0; d < D - 1; ++d) axis_jets<T, D, BASIS>(xv[d], ccoord[d], s_nodex[d], l, hinv, val[d], der[d]);
-          const T xi = (xv[D - 1] - s_nodex[D - 1][ccoord[D - 1] + al]) * hinv;
-          piecewise_jet<T, BASIS>(xi, l, val[D - 1][0], der[D - 1][0]);
+          for (int d = 1; d < D; ++d) w = w * val[d][aa[d]];
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            T gg = (c == 0) ? der[0][aa[0]] : val[0][aa[0]];
+#pragma unroll
+            for (int d = 1; d < D; ++d) gg = gg * ((d == c) ? der[d][aa[d]] : val[d][aa[d]]);
+            g[c] = gg * hinv;
+          }
         }
+        acc[s2][0] += w * mp;
 #pragma unroll
-        for (int s2 = 0; s2 < NSL; ++s2) {
-          int aa[D];
-          int r = s2;
-          bool ok = true;
+        for (int c = 0; c < D; ++c) acc[s2][1 + c] += w * mv[c];
 #pragma unroll
-          for (int d = 0; d < D - 1; ++d) {
-            aa[d] = r % SW;
-            r /= SW;
-            ok = ok && aa[d] <= extd[d];
-          }
-          aa[D - 1] = al;
-          if (!ok) continue;
-          T w, g[D];
-          if constexpr (STORED) {
-            int slot = 0;
+        for (int r2 = 0; r2 < D; ++r2) {
+          T f = A[r2] * g[0];
 #pragma unroll
-            for (int d = D - 1; d >= 0; --d) slot = slot * SW + aa[d];
-            const int64_t e = int64_t(slot) + int64_t(Pow<SW, D>::v) * s_pid[i];
-            w = a.w(e);
-#pragma unroll
-            for (int c = 0; c < D; ++c) g[c] = a.gw(e, c);
-          } else {
-            // w = Π_d N_d (left-assoc), ∇w_c = (N'_c Π_{d≠c} N_d) h⁻¹
-            w = val[0][aa[0]];
-#pragma unroll
-            for (int d = 1; d < D - 1; ++d) w = w * val[d][aa[d]];
-            w = w * val[D - 1][0];
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-              T gg = (c == 0) ? der[0][aa[0]] : val[0][aa[0]];
-#pragma unroll
-              for (int d = 1; d < D - 1; ++d) gg = gg * ((d == c) ? der[d][aa[d]] : val[d][aa[d]]);
-              gg = gg * ((c == D - 1) ? der[D - 1][0] : val[D - 1][0]);
-              g[c] = gg * hinv;
-            }
-          }
-          acc[s2][0] += w * mp;
-#pragma unroll
-          for (int c = 0; c < D; ++c) acc[s2][1 + c] += w * mv[c];
-#pragma unroll
-          for (int r2 = 0; r2 < D; ++r2) {
-            T f = A[r2] * g[0];
-#pragma unroll
-            for (int c = 1; c < D; ++c) f = f + A[r2 + D * c] * g[c];
-            acc[s2][1 + D + r2] += f;
-          }
+          for (int c = 1; c < D; ++c) f = f + A[r2 + D * c] * g[c];
+          acc[s2][1 + D + r2] += f;
         }
       }
-      // add this lane's class contributions into the LDS tile: all lanes are
-      // distinct classes, so for a fixed offset their targets are distinct and
-      // every LDS add instruction touches each node at most once.  The adds are
-      // LDS atomics so that the compiler keeps each of them a single
-      // read-modify-write (a plain += would let it hoist one offset's load
-      // above another offset's store, which other lanes may target); the
-      // per-node order of the adds is the fixed program order.
-      if (ccount > 0) {
+    }
+    if (ccount > 0) {
 #pragma unroll
-        for (int s2 = 0; s2 < NSL; ++s2) {
-          int r = s2, q = 0, mul = 1;
+      for (int s2 = 0; s2 < NSL; ++s2) {
+        int r = pass * NSL + s2, q = 0, mul = 1;
 #pragma unroll
-          for (int d = 0; d < D - 1; ++d) {
-            q += (ccoord[d] + r % SW) * mul;
-            r /= SW;
-            mul *= G::TS;
-          }
-          q += (ccoord[D - 1] + al) * mul;
-#pragma unroll
-          for (int v = 0; v < NV; ++v) atomicAdd(&s_tile[v][q], acc[s2][v]);
+        for (int d = 0; d < D; ++d) {
+          q += (ccoord[d] + r % SW) * mul;
+          r /= SW;
+          mul *= G::TS;
         }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) atomicAdd(&s_tile[v][q], acc[s2][v]);
       }
     }
   }
@@ -609,10 +603,14 @@ __device__ __forceinline__ void particle_update(const Args<T, D>& a, int64_t p, 
   a.V(p) = T(a.V0 * J);
 }
 
+// One wave per tile, lanes own the tile's classes as in P2G (the walk is
+// coalesced when particles are stored in the layered order).  The footprint's
+// vⁿ and v are staged in LDS once per tile.
 template <typename T, int D, int BASIS, bool STORED>
-__global__ __launch_bounds__(256) void k_g2p_tile(Args<T, D> a) {
+__global__ __launch_bounds__(64) void k_g2p_tile(Args<T, D> a, const uint32_t* __restrict__ skeys) {
   using G = Geo<D>;
   constexpr int SW = support_width(BASIS);
+  const int lane = threadIdx.x;
   const int64_t j = xcd_remap(blockIdx.x, a.n_active);
   if (j >= a.n_active) return;
   const uint32_t tile = a.active[j];
@@ -625,7 +623,8 @@ __global__ __launch_bounds__(256) void k_g2p_tile(Args<T, D> a) {
   __shared__ T s_vn[D][G::NFOOT];
   __shared__ T s_v[D][G::NFOOT];
   __shared__ T s_nodex[D][G::TS];
-  for (int q = threadIdx.x; q < G::NFOOT; q += blockDim.x) {
+  __shared__ uint32_t s_cnt[64];
+  for (int q = lane; q < G::NFOOT; q += 64) {
     int I[D], r = q;
     bool in = true;
 #pragma unroll
@@ -634,42 +633,42 @@ __global__ __launch_bounds__(256) void k_g2p_tile(Args<T, D> a) {
       r /= G::TS;
       in = in && I[d] >= 0 && I[d] < a.mesh.dims[d];
     }
-    if (in) {
-      const int64_t i = a.mesh.lin(I);
+    const int64_t i = in ? a.mesh.lin(I) : 0;
 #pragma unroll
-      for (int c = 0; c < D; ++c) {
-        s_vn[c][q] = a.gvn(i, c);
-        s_v[c][q] = a.gv(i, c);
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        s_vn[c][q] = T(0);
-        s_v[c][q] = T(0);
-      }
+    for (int c = 0; c < D; ++c) {
+      s_vn[c][q] = in ? a.gvn(i, c) : T(0);
+      s_v[c][q] = in ? a.gv(i, c) : T(0);
     }
   }
-  for (int q = threadIdx.x; q < D * G::TS; q += blockDim.x) {
+  for (int q = lane; q < D * G::TS; q += 64) {
     const int d = q / G::TS, k = q % G::TS;
     s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
   }
-  __syncthreads();
+  int ccount, coff, maxcnt;
+  tile_class_counts(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);  // contains the barrier after staging
+  int ccoord[D];
+  {
+    int r = lane;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      ccoord[d] = r % G::BW;
+      r /= G::BW;
+    }
+  }
   const T hinv = a.mesh.h_inv, dt = a.dt;
-  for (int jj = threadIdx.x; jj < cnt; jj += blockDim.x) {
-    const uint32_t pid = a.perm[start + uint32_t(jj)];
+  for (int k = 0; k < maxcnt; ++k) {
+    if (k >= ccount) continue;
+    const uint32_t pid = a.perm[start + coff + k];
     T x[D], l;
     int lo[D], hi[D];
     particle_window<T, D, BASIS, STORED>(a, pid, x, l, lo, hi, nullptr);
-    int ll[D], ext[D];
+    int ext[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      ll[d] = lo[d] - origin[d];
-      ext[d] = hi[d] - lo[d];
-    }
+    for (int d = 0; d < D; ++d) ext[d] = hi[d] - lo[d];
     T val[D][SW], der[D][SW];
     if constexpr (!STORED) {
 #pragma unroll
-      for (int d = 0; d < D; ++d) axis_jets<T, D, BASIS>(x[d], ll[d], s_nodex[d], l, hinv, val[d], der[d]);
+      for (int d = 0; d < D; ++d) axis_jets<T, D, BASIS>(x[d], ccoord[d], s_nodex[d], l, hinv, val[d], der[d]);
     }
     T dv[D], L[D * D], dx[D];
 #pragma unroll
@@ -678,7 +677,7 @@ __global__ __launch_bounds__(256) void k_g2p_tile(Args<T, D> a) {
       dx[c] = T(0);
     }
 #pragma unroll
-    for (int k = 0; k < D * D; ++k) L[k] = T(0);
+    for (int q = 0; q < D * D; ++q) L[q] = T(0);
     // column-major walk of the window (eachindex(window))
 #pragma unroll
     for (int o = 0; o < Pow<SW, D>::v; ++o) {
@@ -712,7 +711,7 @@ __global__ __launch_bounds__(256) void k_g2p_tile(Args<T, D> a) {
       int q = 0, mul = 1;
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        q += (ll[d] + aa[d]) * mul;
+        q += (ccoord[d] + aa[d]) * mul;
         mul *= G::TS;
       }
       T vi[D];
@@ -821,12 +820,12 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
   const int64_t n = P.n;
   uint64_t ntiles = 1;
   for (int d = 0; d < D; ++d) ntiles *= uint64_t(a.ntiles[d]);
-  if (ntiles >= uint64_t(UINT32_MAX) || n >= int64_t(UINT32_MAX))
+  if (ntiles * 64 >= uint64_t(UINT32_MAX) || n >= int64_t(UINT32_MAX))
     raise(TESS_ERR_ARGUMENT, "mesh or particle count exceeds the 32-bit binning range");
   Binning& b = ctx->bin;
   b.valid = false;
   uint32_t* counters = ctx->counters.as<uint32_t>(CNT_SLOTS);
-  TESS_HIP(hipMemsetAsync(counters, 0, CNT_SLOTS * sizeof(uint32_t), ctx->stream));
+  TESS_HIP(hipMemsetAsync(counters, 0, CNT_BIN_SLOTS * sizeof(uint32_t), ctx->stream));
   uint32_t* keys = ctx->keys.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
   uint32_t* skeys = ctx->keys_sorted.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
   uint32_t* perm = ctx->vals.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
@@ -842,7 +841,7 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
     hipLaunchKernelGGL((k_bin_keys<T, D, BASIS, STORED>), dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, a,
                        keys, uint32_t(ntiles), counters);
     TESS_HIP(hipGetLastError());
-    radix_sort_pairs(ctx, keys, nullptr, skeys, perm, n, key_bits_for(ntiles));
+    radix_sort_pairs(ctx, keys, nullptr, skeys, perm, n, key_bits_for(ntiles * 64));
   }
   TESS_HIP(hipMemsetAsync(tstart, 0, ntiles * sizeof(uint32_t), ctx->stream));
   TESS_HIP(hipMemsetAsync(tend, 0, ntiles * sizeof(uint32_t), ctx->stream));
@@ -878,6 +877,8 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
   if (hc[CNT_ERR] & ERR_WINDOW)
     raise(TESS_ERR_DIMENSION, "stored support window is larger than the basis support width");
   b.n_inside = n > 0 ? int64_t(hc[CNT_INSIDE]) : 0;
+  // layout score of the previous P2G on this particle set (consumed by the step driver)
+  ctx->last_misplaced = (ctx->misplaced_x_ptr == P.x.ptr && ctx->misplaced_n == n) ? int64_t(hc[CNT_MISPLACED]) : -1;
   b.n_active = int64_t(hc[CNT_ACTIVE]);
   b.n_nodeblocks = int64_t(hc[CNT_NB]);
   b.x_ptr = P.x.ptr;
@@ -891,12 +892,18 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
   b.valid = true;
 }
 
+// P2G after binning: zeroing of what this call does not overwrite, the tile
+// kernel and the gather.  count_misplaced asks the tile kernel to count
+// particles not stored in the layered order (for the step driver's reorder).
 template <typename T, int D, int BASIS, bool STORED>
-void p2g_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-              const tess_weights* W, const tess_params& prm) {
+void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+                const tess_weights* W, const tess_params& prm, bool count_misplaced) {
   using Gm = Geo<D>;
   constexpr int NV = 1 + 2 * D;
-  bin_particles<T, D, BASIS, STORED>(ctx, mesh, P, W);
+  constexpr int SW = support_width(BASIS);
+  // all sw^dim offsets in one pass while the accumulators fit ~200 VGPRs,
+  // else one pass per offset along the last axis
+  constexpr int NPASS = (Pow<SW, D>::v * NV * int(sizeof(T)) / 4 <= 200) ? 1 : SW;
   Args<T, D> a = make_args<T, D>(mesh, &G, P, W, &prm);
   fill_binning_ptrs<T, D>(ctx, a);
   int64_t nnodes = 1;
@@ -914,11 +921,20 @@ void p2g_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const te
     TESS_HIP(hipGetLastError());
   }
   const Binning& b = ctx->bin;
+  uint32_t* counters = static_cast<uint32_t*>(ctx->counters.ptr);
+  uint32_t* misplaced = nullptr;
+  if (count_misplaced) {
+    misplaced = counters + CNT_MISPLACED;
+    TESS_HIP(hipMemsetAsync(misplaced, 0, sizeof(uint32_t), ctx->stream));
+    ctx->misplaced_x_ptr = P.x.ptr;
+    ctx->misplaced_n = P.n;
+  } else {
+    ctx->misplaced_x_ptr = nullptr;
+  }
   if (b.n_active > 0) {
     a.partials = ctx->partials.as<T>(size_t(b.n_active) * NV * Gm::NFOOT);
-    constexpr int SEG = (sizeof(T) == 4 && D == 3) ? 256 : 128;
-    hipLaunchKernelGGL((k_p2g_tile<T, D, BASIS, STORED, SEG>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
-                       ctx->stream, a);
+    hipLaunchKernelGGL((k_p2g_tile<T, D, BASIS, STORED, NPASS>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
+                       ctx->stream, a, static_cast<const uint32_t*>(ctx->keys_sorted.ptr), misplaced);
     TESS_HIP(hipGetLastError());
   }
   if (b.n_nodeblocks > 0) {
@@ -938,20 +954,14 @@ void p2g_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const te
 }
 
 template <typename T, int D, int BASIS, bool STORED>
-void g2p_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-              const tess_weights* W, const tess_params& prm, bool reuse_binning) {
+void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+                const tess_weights* W, const tess_params& prm) {
   const Binning& b = ctx->bin;
-  // Only a step that just ran P2G on these particles may reuse its binning; a
-  // standalone G2P bins again (device addresses alone cannot prove that the
-  // particle set is unchanged).
-  const bool reuse = reuse_binning && b.valid && b.x_ptr == P.x.ptr && b.n == P.n && b.dim == D &&
-                     b.dtype == mesh.dtype && b.basis == BASIS;
-  if (!reuse) bin_particles<T, D, BASIS, STORED>(ctx, mesh, P, W);
   Args<T, D> a = make_args<T, D>(mesh, &G, P, W, &prm);
   fill_binning_ptrs<T, D>(ctx, a);
   if (b.n_active > 0) {
-    hipLaunchKernelGGL((k_g2p_tile<T, D, BASIS, STORED>), dim3(xcd_grid(b.n_active)), dim3(256), 0, ctx->stream,
-                       a);
+    hipLaunchKernelGGL((k_g2p_tile<T, D, BASIS, STORED>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream, a,
+                       static_cast<const uint32_t*>(ctx->keys_sorted.ptr));
     TESS_HIP(hipGetLastError());
   }
   if (P.n > b.n_inside) {
@@ -960,6 +970,109 @@ void g2p_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const te
     TESS_HIP(hipGetLastError());
   }
   ctx->bin.valid = false;  // positions moved
+}
+
+// Layered storage order of the binned particles: inside each tile the k-th
+// particle of every class, classes ascending, then the next k.  Writes the
+// gather order (new storage slot -> old particle) and the sorted-slot -> new
+// storage index map.
+template <typename T, int D>
+__global__ __launch_bounds__(64) void k_layered_order(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                       uint32_t* __restrict__ order, uint32_t* __restrict__ perm_new) {
+  const int lane = threadIdx.x;
+  const int64_t j = blockIdx.x;
+  if (j >= a.n_active) return;
+  const uint32_t tile = a.active[j];
+  const uint32_t start = a.tile_start[tile];
+  const int cnt = int(a.tile_end[tile] - start);
+  __shared__ uint32_t s_cnt[64];
+  int ccount, coff, maxcnt;
+  tile_class_counts(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);
+  uint32_t layer_off = 0;
+  for (int k = 0; k < maxcnt; ++k) {
+    const uint64_t has = __ballot(k < ccount);
+    if (k < ccount) {
+      const uint32_t dest = start + layer_off + uint32_t(__popcll(has & lanemask_lt64()));
+      order[dest] = a.perm[start + coff + k];
+      perm_new[start + coff + k] = dest;
+    }
+    layer_off += uint32_t(__popcll(has));
+  }
+}
+
+__global__ void k_tail_identity(const uint32_t* __restrict__ perm, int64_t from, int64_t n,
+                                uint32_t* __restrict__ order, uint32_t* __restrict__ perm_new) {
+  const int64_t j = from + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  order[j] = perm[j];
+  perm_new[j] = uint32_t(j);
+}
+
+template <typename T, int D>
+void layered_reorder(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P) {
+  const Binning& b = ctx->bin;
+  const int64_t n = P.n;
+  if (n == 0) return;
+  Args<T, D> a = make_args<T, D>(mesh, nullptr, P, nullptr, nullptr);
+  fill_binning_ptrs<T, D>(ctx, a);
+  uint32_t* order = ctx->keys.as<uint32_t>(size_t(n));        // raw keys are dead after the sort
+  uint32_t* perm_new = ctx->keys_alt.as<uint32_t>(size_t(n));
+  if (b.n_active > 0) {
+    hipLaunchKernelGGL((k_layered_order<T, D>), dim3(unsigned(b.n_active)), dim3(64), 0, ctx->stream, a,
+                       static_cast<const uint32_t*>(ctx->keys_sorted.ptr), order, perm_new);
+    TESS_HIP(hipGetLastError());
+  }
+  if (n > b.n_inside) {
+    hipLaunchKernelGGL(k_tail_identity, dim3(blocks_for(n - b.n_inside, 256)), dim3(256), 0, ctx->stream,
+                       static_cast<const uint32_t*>(ctx->vals.ptr), b.n_inside, n, order, perm_new);
+    TESS_HIP(hipGetLastError());
+  }
+  permute_particles(ctx, mesh, P, order);
+  TESS_HIP(hipMemcpyAsync(ctx->vals.ptr, perm_new, size_t(n) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                          ctx->stream));
+}
+
+template <typename T, int D, int BASIS, bool STORED>
+void p2g_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+              const tess_weights* W, const tess_params& prm) {
+  bin_particles<T, D, BASIS, STORED>(ctx, mesh, P, W);
+  p2g_launch<T, D, BASIS, STORED>(ctx, mesh, G, P, W, prm, false);
+}
+
+template <typename T, int D, int BASIS, bool STORED>
+void g2p_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+              const tess_weights* W, const tess_params& prm) {
+  // a standalone G2P bins again: device addresses alone cannot prove that the
+  // particle set is unchanged since the last P2G
+  bin_particles<T, D, BASIS, STORED>(ctx, mesh, P, W);
+  g2p_launch<T, D, BASIS, STORED>(ctx, mesh, G, P, W, prm);
+}
+
+// nsteps of the README loop body (deferred weights).  With reorder_threshold
+// in (0, 1], particles are moved into the layered order when the fraction
+// stored in place, measured by the previous step's P2G, is below the
+// threshold (1 reorders every step), the role of
+// reorder_particles!(particles, partition; threshold) in the reference loop.
+template <typename T, int D, int BASIS, bool STORED>
+void step_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+               const tess_params& prm, int nsteps, double reorder_threshold) {
+  for (int s = 0; s < nsteps; ++s) {
+    bin_particles<T, D, BASIS, false>(ctx, mesh, P, nullptr);
+    const Binning& b = ctx->bin;
+    if (reorder_threshold > 0.0 && b.n_inside > 0) {
+      bool go = reorder_threshold >= 1.0;
+      if (!go && ctx->last_misplaced >= 0) {
+        const double score = 1.0 - double(ctx->last_misplaced) / double(b.n_inside);
+        go = score < reorder_threshold;
+      }
+      if (go) {
+        layered_reorder<T, D>(ctx, mesh, P);
+        ctx->reorders += 1;
+      }
+    }
+    p2g_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, reorder_threshold > 0.0);
+    g2p_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm);
+  }
 }
 
 template <template <typename, int, int, bool> class Op, typename... Args_>
@@ -999,8 +1112,15 @@ struct P2GOp {
 template <typename T, int D, int B, bool S>
 struct G2POp {
   static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
-                  const tess_weights* W, const tess_params& prm, bool reuse) {
-    g2p_impl<T, D, B, S>(ctx, m, G, P, W, prm, reuse);
+                  const tess_weights* W, const tess_params& prm) {
+    g2p_impl<T, D, B, S>(ctx, m, G, P, W, prm);
+  }
+};
+template <typename T, int D, int B, bool S>
+struct StepOp {
+  static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
+                  const tess_params& prm, int nsteps, double thr) {
+    if constexpr (!S) step_impl<T, D, B, false>(ctx, m, G, P, prm, nsteps, thr);
   }
 };
 
@@ -1012,8 +1132,13 @@ void run_p2g(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G
 }
 
 void run_g2p(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-             const tess_weights* W, const tess_params& prm, bool reuse_binning) {
-  dispatch<G2POp>(basis, mesh, W != nullptr, ctx, mesh, G, P, W, prm, reuse_binning);
+             const tess_weights* W, const tess_params& prm) {
+  dispatch<G2POp>(basis, mesh, W != nullptr, ctx, mesh, G, P, W, prm);
+}
+
+void run_step(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+              const tess_params& prm, int nsteps, double reorder_threshold) {
+  dispatch<StepOp>(basis, mesh, false, ctx, mesh, G, P, prm, nsteps, reorder_threshold);
 }
 
 }  // namespace tess

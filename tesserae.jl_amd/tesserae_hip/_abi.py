@@ -129,7 +129,7 @@ SIGNATURES = {
     "tess_step": (
         ctypes.c_int32,
         [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_particles), _P(tess_params),
-         ctypes.c_int32],
+         ctypes.c_int32, ctypes.c_double],
     ),
 }
 

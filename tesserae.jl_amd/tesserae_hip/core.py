@@ -536,10 +536,17 @@ def _resolve_weights(weights):
     return weights.basis, ctypes.pointer(W)
 
 
-def step(grid: Grid, particles: Particles, basis: Basis, params: MPMParams, nsteps: int = 1, ctx=None) -> None:
-    """nsteps of the README loop body with deferred weights (README.md:66-87)."""
+def step(grid: Grid, particles: Particles, basis: Basis, params: MPMParams, nsteps: int = 1,
+         reorder_threshold: float = 0.0, ctx=None) -> None:
+    """nsteps of the README loop body with deferred weights (README.md:66-87).
+
+    ``reorder_threshold`` acts like the reference loop's
+    ``reorder_particles!(particles, partition; threshold)``: particles are
+    permuted into the engine's tile order when the in-place fraction drops
+    below it (0 never, 1 every step)."""
     _require_device(particles.x, grid.m)
     ctx = ctx or default_context()
     P, G = particles.abi(), grid.abi()
     _check(ctx.lib, ctx.lib.tess_step(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
-                                      ctypes.byref(P), ctypes.byref(params.abi()), int(nsteps)))
+                                      ctypes.byref(P), ctypes.byref(params.abi()), int(nsteps),
+                                      float(reorder_threshold)))

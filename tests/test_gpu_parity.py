@@ -157,6 +157,23 @@ def test_two_disks_fp64_trajectory(oracle):
         assert err <= 1e-10, name
 
 
+def test_step_with_reorder_matches_oracle(oracle):
+    """Steps that permute particles into the engine's tile order (threshold 1: every
+    step) give the oracle's trajectory once particles are matched by identity."""
+    sc = two_disks(h=0.02, steps=30)
+    sc.l = np.arange(sc.n, dtype=np.float64)  # identity tag carried along by the permutation
+    mesh, P, G = to_gpu(sc)
+    th.step(G, P, sc.basis, sc.params, nsteps=30, reorder_threshold=1.0)
+    torch.cuda.synchronize()
+    got = particles_np(P)
+    order = np.argsort(got["l"])
+    assert np.array_equal(got["l"][order], np.arange(sc.n))
+    st = HostState(sc)
+    oracle.step(st, sc.basis, sc.params, 30)
+    for name in ("x", "v", "F", "sigma", "V"):
+        assert rel_err(got[name][order], getattr(st, name)) <= 1e-10, name
+
+
 def test_block_drop_fp32_vs_fp64_oracle(oracle):
     """3D block drop at dx=1/64 (reduced size), Float32 GPU vs Float64 oracle after 20 steps."""
     sc = block_drop(dx=1 / 64, steps=20)
