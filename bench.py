@@ -95,6 +95,7 @@ def main():
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     log(f"bench: timing {args.steps} steps")
+    ctx.profile(True)  # per-kernel HIP events on the library's stream
     torch.cuda.synchronize()
     start.record()
     t0 = time.perf_counter()
@@ -103,12 +104,24 @@ def main():
     stop.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    stats = ctx.stats()
     ms = start.elapsed_time(stop) / args.steps
     value = n / (ms * 1e-3)
     # algorithmic bytes per step (README layout, weights evaluated in-kernel):
     # particles s*(5d^2+6d+2), active grid nodes s*(8d+2)
     s, d = 4, 3
     bytes_p = s * (5 * d * d + 6 * d + 2)
+    bytes_node = s * (8 * d + 2)
+    step_bytes = bytes_p * n + bytes_node * max(stats["active_nodes"], 0)
+    kernels = {k: {"avg_us": v[0] / max(v[1], 1) * 1e3, "calls": v[1],
+                   "achieved_GBs": (v[2] / v[1]) / (v[0] / v[1] * 1e-3) / 1e9 if v[0] > 0 and v[2] > 0 else None}
+               for k, v in prof.items()}
+    dom = max((k for k in prof if prof[k][2] > 0), key=lambda k: prof[k][0])
+    dom_ms = prof[dom][0] / prof[dom][1]
+    dom_bytes = prof[dom][2] / prof[dom][1]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     out = {
         "metric": "3D MPM particle-steps/s at 13M pcls",
         "value": value,
@@ -124,8 +137,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": "3D elastic block drop, dx=1/256, quadratic B-spline, 13,434,880 particles, FP32",
                    "particles": n, "layout": args.layout},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
+        "step_hbm": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (ms * 1e-3) / 1e9,
+                     "frac": step_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), "active_nodes": stats["active_nodes"]},
+        "kernels": kernels,
         "wall_s": wall,
-        "step_hbm_frac_particles_only": (bytes_p * n / (ms * 1e-3)) / (HBM_PEAK_GBS * 1e9),
     }
     if not args.no_cpu:
         log("bench: CPU baseline")

@@ -198,6 +198,23 @@ tess_status tess_step(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const
                       const tess_particles* particles, const tess_params* params, int32_t nsteps,
                       double reorder_threshold);
 
+/* ---- measurement (no counterpart in the reference) ----------------------- */
+
+/* Per-kernel timing with HIP events recorded on the context's stream around
+ * each profiled launch (binning, k_p2g_tile, k_gather_grid, k_g2p_tile).
+ * Enabling (or disabling) clears the accumulated entries. */
+tess_status tess_profile_enable(tess_ctx* ctx, int32_t on);
+/* Entry `index` (0-based, in first-launch order): name, summed duration,
+ * launch count and summed algorithmic bytes.  TESS_ERR_ARGUMENT past the last
+ * entry.  Synchronises. */
+tess_status tess_profile_read(tess_ctx* ctx, int32_t index, char* name, int32_t name_len, double* total_ms,
+                              int64_t* calls, double* bytes);
+/* Host copies of the last binning and step: out[0] particles inside the mesh,
+ * out[1] active tiles, out[2] node blocks written, out[3] grid nodes with m>0
+ * written by the previous P2G, out[4] reorders done by tess_step, out[5]
+ * particles found out of the layered order by the previous P2G (-1 unknown). */
+tess_status tess_stats(tess_ctx* ctx, int64_t* out, int32_t count);
+
 #ifdef __cplusplus
 }
 #endif

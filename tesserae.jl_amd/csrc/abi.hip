@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <new>
 #include <string>
 
@@ -43,6 +44,48 @@ void Buffer::release() {
   if (ptr) (void)hipFree(ptr);
   ptr = nullptr;
   bytes = 0;
+}
+
+int prof_begin(tess_ctx* ctx, const char* name, double bytes) {
+  Profiler& P = ctx->prof;
+  if (!P.on) return -1;
+  int e = -1;
+  for (size_t i = 0; i < P.entries.size(); ++i)
+    if (P.entries[i].name == name) e = int(i);
+  if (e < 0) {
+    P.entries.push_back(ProfEntry{name});
+    e = int(P.entries.size() - 1);
+  }
+  P.entries[e].bytes += bytes;
+  P.entries[e].calls += 1;
+  while (P.pool.size() < P.next_event + 2) {
+    hipEvent_t ev;
+    TESS_HIP(hipEventCreate(&ev));
+    P.pool.push_back(ev);
+  }
+  hipEvent_t a = P.pool[P.next_event], b = P.pool[P.next_event + 1];
+  P.next_event += 2;
+  TESS_HIP(hipEventRecord(a, ctx->stream));
+  P.pending.push_back(Profiler::Pending{e, a, b});
+  return int(P.pending.size() - 1);
+}
+
+void prof_end(tess_ctx* ctx, int token) {
+  if (token < 0) return;
+  TESS_HIP(hipEventRecord(ctx->prof.pending[size_t(token)].b, ctx->stream));
+}
+
+void prof_flush(tess_ctx* ctx) {
+  Profiler& P = ctx->prof;
+  if (P.pending.empty()) return;
+  TESS_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& q : P.pending) {
+    float ms = 0.f;
+    TESS_HIP(hipEventElapsedTime(&ms, q.a, q.b));
+    P.entries[size_t(q.entry)].total_ms += ms;
+  }
+  P.pending.clear();
+  P.next_event = 0;
 }
 
 namespace {
@@ -157,6 +200,8 @@ tess_status tess_ctx_create(int32_t device, void* stream, tess_ctx** out) {
       check_hip(e, "hipHostMalloc");
     }
     c->counters_host = static_cast<int64_t*>(pinned);
+    // device counters start at zero (the binning reads slots written by the previous P2G)
+    TESS_HIP(hipMemset(c->counters.get(64 * sizeof(uint32_t)), 0, 64 * sizeof(uint32_t)));
     *out = c;
   });
 }
@@ -173,6 +218,7 @@ tess_status tess_ctx_destroy(tess_ctx* ctx) {
                             &ctx->nb_flag,   &ctx->nb_slot,     &ctx->nb_list,      &ctx->nb_list_prev,
                             &ctx->partials,  &ctx->counters,    &ctx->host_pinned_dummy, &ctx->reorder_tmp};
     for (auto* b : bufs) b->release();
+    for (auto ev : ctx->prof.pool) (void)hipEventDestroy(ev);
     if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
     delete ctx;
   });
@@ -267,6 +313,44 @@ tess_status tess_step(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const
     if (!(reorder_threshold >= 0.0 && reorder_threshold <= 1.0))
       raise(TESS_ERR_ARGUMENT, "threshold must be in [0, 1].");
     run_step(ctx, basis, *mesh, *grid, *particles, *params, nsteps, reorder_threshold);
+  });
+}
+
+tess_status tess_profile_enable(tess_ctx* ctx, int32_t on) {
+  return guarded([&] {
+    check_ctx(ctx);
+    prof_flush(ctx);
+    ctx->prof.on = on != 0;
+    ctx->prof.entries.clear();
+  });
+}
+
+tess_status tess_profile_read(tess_ctx* ctx, int32_t index, char* name, int32_t name_len, double* total_ms,
+                              int64_t* calls, double* bytes) {
+  return guarded([&] {
+    check_ctx(ctx);
+    prof_flush(ctx);
+    const auto& E = ctx->prof.entries;
+    if (index < 0 || size_t(index) >= E.size()) raise(TESS_ERR_ARGUMENT, "no such profile entry");
+    const ProfEntry& e = E[size_t(index)];
+    if (name && name_len > 0) {
+      const size_t n = std::min(size_t(name_len - 1), e.name.size());
+      e.name.copy(name, n);
+      name[n] = '\0';
+    }
+    if (total_ms) *total_ms = e.total_ms;
+    if (calls) *calls = e.calls;
+    if (bytes) *bytes = e.bytes;
+  });
+}
+
+tess_status tess_stats(tess_ctx* ctx, int64_t* out, int32_t count) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!out || count < 0) raise(TESS_ERR_ARGUMENT, "out is NULL");
+    const int64_t vals[] = {ctx->bin.n_inside, ctx->bin.n_active, ctx->bin.n_nodeblocks, ctx->last_active_nodes,
+                            ctx->reorders, ctx->last_misplaced};
+    for (int32_t i = 0; i < count && i < int32_t(sizeof(vals) / sizeof(vals[0])); ++i) out[i] = vals[i];
   });
 }
 

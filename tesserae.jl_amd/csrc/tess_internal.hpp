@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/tesserae_hip.h"
 
@@ -48,9 +49,30 @@ struct Binning {
   int64_t n_nodeblocks = 0; // node blocks touched by some tile footprint (host copy)
 };
 
+// Per-kernel timing with HIP events on the context's stream.  Disabled by
+// default; begin/end cost two event records per profiled launch when enabled.
+struct ProfEntry {
+  std::string name;
+  double total_ms = 0.0;
+  int64_t calls = 0;
+  double bytes = 0.0;  // algorithmic bytes summed over the calls
+};
+struct Profiler {
+  bool on = false;
+  std::vector<ProfEntry> entries;
+  std::vector<hipEvent_t> pool;
+  struct Pending {
+    int entry;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  size_t next_event = 0;
+};
+
 }  // namespace tess
 
 struct tess_ctx {
+  tess::Profiler prof;
   int device = 0;
   hipStream_t stream = nullptr;
   // sort + binning: keys (unsorted), keys_sorted + vals (= perm) sorted output,
@@ -75,6 +97,8 @@ struct tess_ctx {
   int64_t misplaced_n = 0;
   int64_t last_misplaced = -1;
   int64_t reorders = 0;
+  // nodes with m > 0 written by the last gather (read back with the next binning)
+  int64_t last_active_nodes = -1;
 };
 
 namespace tess {
@@ -107,5 +131,10 @@ void run_step(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& 
 void permute_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const uint32_t* order);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Profiling hooks (abi.hip): prof_begin returns a token (-1 when disabled).
+int prof_begin(tess_ctx* ctx, const char* name, double bytes);
+void prof_end(tess_ctx* ctx, int token);
+void prof_flush(tess_ctx* ctx);
 
 }  // namespace tess

@@ -54,7 +54,10 @@ struct Pow<N, 0> {
 
 // device counters: binning totals (zeroed per binning), error flags, and the
 // misplaced-particle count of the last P2G (zeroed per P2G)
-enum { CNT_INSIDE = 0, CNT_ACTIVE = 1, CNT_NB = 2, CNT_ERR = 3, CNT_BIN_SLOTS = 4, CNT_MISPLACED = 8, CNT_SLOTS = 16 };
+enum {
+  CNT_INSIDE = 0, CNT_ACTIVE = 1, CNT_NB = 2, CNT_ERR = 3, CNT_BIN_SLOTS = 4,
+  CNT_MISPLACED = 8, CNT_NODES = 9, CNT_SLOTS = 16
+};
 enum { ERR_GIMP_LENGTH = 1, ERR_WINDOW = 2 };
 
 template <typename T, int D>
@@ -465,7 +468,7 @@ __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a, const uint32_t* _
 
 // ---- gather + grid-node half ------------------------------------------------
 template <typename T, int D>
-__global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a) {
+__global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __restrict__ active_nodes) {
   using G = Geo<D>;
   constexpr int NV = 1 + 2 * D;
   const int64_t j = xcd_remap(blockIdx.x, a.n_nb);
@@ -522,6 +525,11 @@ __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a) {
   // grid-node half with the guarded Float64 division (test/transfer.jl:102-104)
   const double md = double(acc[0]);
   const bool has = md > 0.0;
+  {
+    const uint64_t m = __ballot(has);
+    if ((threadIdx.x & 63) == __ffsll((unsigned long long)__ballot(true)) - 1 && m)
+      atomicAdd(active_nodes, uint32_t(__popcll(m)));
+  }
   const double minv = has ? 1.0 / md : 0.0;
   a.gm(i) = acc[0];
 #pragma unroll
@@ -837,6 +845,7 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
   uint32_t* nbflag = ctx->nb_flag.as<uint32_t>(ntiles);
   uint32_t* nbslot = ctx->nb_slot.as<uint32_t>(ntiles);
   uint32_t* nblist = ctx->nb_list.as<uint32_t>(ntiles);
+  const int tok = prof_begin(ctx, "binning", 0.0);
   if (n > 0) {
     hipLaunchKernelGGL((k_bin_keys<T, D, BASIS, STORED>), dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, a,
                        keys, uint32_t(ntiles), counters);
@@ -869,6 +878,7 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
   hipLaunchKernelGGL(k_compact, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, nbflag, nbslot,
                      uint32_t(ntiles), nblist);
   TESS_HIP(hipGetLastError());
+  prof_end(ctx, tok);
   // the one host readback of the binning: totals for the launch sizes
   uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
   TESS_HIP(hipMemcpyAsync(hc, counters, CNT_SLOTS * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -877,6 +887,7 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
   if (hc[CNT_ERR] & ERR_WINDOW)
     raise(TESS_ERR_DIMENSION, "stored support window is larger than the basis support width");
   b.n_inside = n > 0 ? int64_t(hc[CNT_INSIDE]) : 0;
+  ctx->last_active_nodes = int64_t(hc[CNT_NODES]);
   // layout score of the previous P2G on this particle set (consumed by the step driver)
   ctx->last_misplaced = (ctx->misplaced_x_ptr == P.x.ptr && ctx->misplaced_n == n) ? int64_t(hc[CNT_MISPLACED]) : -1;
   b.n_active = int64_t(hc[CNT_ACTIVE]);
@@ -903,7 +914,7 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   constexpr int SW = support_width(BASIS);
   // all sw^dim offsets in one pass while the accumulators fit ~200 VGPRs,
   // else one pass per offset along the last axis
-  constexpr int NPASS = (Pow<SW, D>::v * NV * int(sizeof(T)) / 4 <= 200) ? 1 : SW;
+  constexpr int NPASS = (Pow<SW, D>::v * NV * int(sizeof(T)) / 4 <= 100) ? 1 : SW;
   Args<T, D> a = make_args<T, D>(mesh, &G, P, W, &prm);
   fill_binning_ptrs<T, D>(ctx, a);
   int64_t nnodes = 1;
@@ -933,13 +944,23 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   }
   if (b.n_active > 0) {
     a.partials = ctx->partials.as<T>(size_t(b.n_active) * NV * Gm::NFOOT);
+    // algorithmic bytes: x, v, V, σ per particle
+    const int tok = prof_begin(ctx, "k_p2g_tile", double(b.n_inside) * sizeof(T) * (2 * D + 1 + D * D));
     hipLaunchKernelGGL((k_p2g_tile<T, D, BASIS, STORED, NPASS>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
                        ctx->stream, a, static_cast<const uint32_t*>(ctx->keys_sorted.ptr), misplaced);
     TESS_HIP(hipGetLastError());
+    prof_end(ctx, tok);
   }
+  TESS_HIP(hipMemsetAsync(counters + CNT_NODES, 0, sizeof(uint32_t), ctx->stream));
   if (b.n_nodeblocks > 0) {
-    hipLaunchKernelGGL((k_gather_grid<T, D>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a);
+    // algorithmic bytes per active node: m, mv, f written, read back by the
+    // grid-node half, vⁿ and v written (the node count of the previous call)
+    const double nodes = double(std::max<int64_t>(ctx->last_active_nodes, 0));
+    const int tok = prof_begin(ctx, "k_gather_grid", nodes * sizeof(T) * (2 * (2 * D + 1) + 2 * D));
+    hipLaunchKernelGGL((k_gather_grid<T, D>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a,
+                       counters + CNT_NODES);
     TESS_HIP(hipGetLastError());
+    prof_end(ctx, tok);
   }
   // remember this call's node blocks for the next call's stale zeroing
   uint32_t* prev = ctx->nb_list_prev.as<uint32_t>(size_t(std::max<int64_t>(b.ntiles, 1)));
@@ -960,9 +981,16 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   Args<T, D> a = make_args<T, D>(mesh, &G, P, W, &prm);
   fill_binning_ptrs<T, D>(ctx, a);
   if (b.n_active > 0) {
+    // algorithmic bytes: x, v, F read and v, ∇v, x, F, σ, V written per
+    // particle; vⁿ, v read per active node
+    const double nodes = double(std::max<int64_t>(ctx->last_active_nodes, 0));
+    const int tok = prof_begin(ctx, "k_g2p_tile",
+                               double(b.n_inside) * sizeof(T) * (2 * D + D * D + 2 * D + 3 * D * D + 1) +
+                                   nodes * sizeof(T) * 2 * D);
     hipLaunchKernelGGL((k_g2p_tile<T, D, BASIS, STORED>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream, a,
                        static_cast<const uint32_t*>(ctx->keys_sorted.ptr));
     TESS_HIP(hipGetLastError());
+    prof_end(ctx, tok);
   }
   if (P.n > b.n_inside) {
     hipLaunchKernelGGL((k_g2p_outside<T, D>), dim3(blocks_for(P.n - b.n_inside, 256)), dim3(256), 0, ctx->stream,

@@ -131,6 +131,13 @@ SIGNATURES = {
         [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_particles), _P(tess_params),
          ctypes.c_int32, ctypes.c_double],
     ),
+    "tess_profile_enable": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "tess_profile_read": (
+        ctypes.c_int32,
+        [ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32, _P(ctypes.c_double), _P(ctypes.c_int64),
+         _P(ctypes.c_double)],
+    ),
+    "tess_stats": (ctypes.c_int32, [ctypes.c_void_p, _P(ctypes.c_int64), ctypes.c_int32]),
 }
 
 _lib = None

@@ -429,6 +429,30 @@ class Context:
     def sync(self) -> None:
         _check(self.lib, self.lib.tess_sync(self.handle))
 
+    def profile(self, on: bool = True) -> None:
+        """Start (or stop) per-kernel HIP-event timing on this context's stream."""
+        _check(self.lib, self.lib.tess_profile_enable(self.handle, int(on)))
+
+    def profile_read(self) -> dict:
+        """{kernel: (total_ms, calls, algorithmic_bytes)} accumulated since profile(True)."""
+        out, i = {}, 0
+        name = ctypes.create_string_buffer(64)
+        ms, calls, nbytes = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+        while True:
+            st = self.lib.tess_profile_read(self.handle, i, name, 64, ctypes.byref(ms), ctypes.byref(calls),
+                                            ctypes.byref(nbytes))
+            if st == _abi.TESS_ERR_ARGUMENT:
+                return out
+            _check(self.lib, st)
+            out[name.value.decode()] = (ms.value, calls.value, nbytes.value)
+            i += 1
+
+    def stats(self) -> dict:
+        vals = (ctypes.c_int64 * 6)()
+        _check(self.lib, self.lib.tess_stats(self.handle, vals, 6))
+        keys = ("n_inside", "active_tiles", "node_blocks", "active_nodes", "reorders", "misplaced")
+        return dict(zip(keys, list(vals)))
+
 
 _contexts: dict = {}
 
