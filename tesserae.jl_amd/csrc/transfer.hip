@@ -466,6 +466,412 @@ __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a, const uint32_t* _
   for (int q = lane; q < NV * G::NFOOT; q += 64) out[q] = (&s_tile[0][0])[q];
 }
 
+template <typename T, int D>
+__device__ __forceinline__ double det_d(const T* F) {
+  if constexpr (D == 2) {
+    return double(F[0]) * double(F[3]) - double(F[2]) * double(F[1]);
+  } else {
+    const double f11 = F[0], f21 = F[1], f31 = F[2], f12 = F[3], f22 = F[4], f32 = F[5], f13 = F[6], f23 = F[7],
+                 f33 = F[8];
+    return f11 * (f22 * f33 - f23 * f32) - f12 * (f21 * f33 - f23 * f31) + f13 * (f21 * f32 - f22 * f31);
+  }
+}
+
+// ---- deferred-weight fast paths ------------------------------------------------
+//
+// Both transfer kernels below evaluate the basis on the fly.  Along an axis
+// whose window is not truncated they use the reference's full-support
+// polynomial (bspline.jl:90-100,146-170); truncated axes and uGIMP use the
+// nodewise piecewise jet (basis_weight.jl:49-60,624-632).  Tensor-product sums
+// are factored by axis (products of the y/z jets are formed once per (y,z)
+// offset and reused along x); the results equal the reference's up to
+// floating-point rounding.
+
+// Loads component c of element e: Julia-native AoS fields (component stride 1,
+// element stride NC) get compile-time strides so neighbouring components
+// merge into vector loads.
+template <typename T, int NC, bool AOS>
+__device__ __forceinline__ T fld(const DField<T>& f, int64_t e, int c) {
+  if constexpr (AOS) return f.p[e * NC + c];
+  else return f(e, c);
+}
+template <typename T, int NC, bool AOS>
+__device__ __forceinline__ T& fldw(const DField<T>& f, int64_t e, int c) {
+  if constexpr (AOS) return f.p[e * NC + c];
+  else return f(e, c);
+}
+
+// 1-D jets at the window nodes lo..lo+SW-1 of one axis; entries past the
+// (truncated) window are zero.  Derivatives are returned times h⁻¹.
+template <typename T, int BASIS>
+__device__ __forceinline__ void axis_jets_mixed(T xd, T xmin, int lo_local, int ext, const T* s_nodex, T l, T hinv,
+                                                T* val, T* der) {
+  constexpr int SW = support_width(BASIS);
+  bool done = false;
+  if constexpr (BASIS != TESS_UGIMP) {
+    if (ext == SW - 1) {
+      const T xi = (xd - xmin) * hinv;
+      const T y = (degree_of(BASIS) % 2 == 1) ? xi : xi - T(0.5);
+      bspline_local_jet<T, BASIS>(y - floor(y), val, der);
+      done = true;
+    }
+  }
+  if (!done) {
+#pragma unroll
+    for (int q = 0; q < SW; ++q) {
+      T v0 = T(0), d0 = T(0);
+      if (q <= ext) piecewise_jet<T, BASIS>((xd - s_nodex[lo_local + q]) * hinv, l, v0, d0);
+      val[q] = v0;
+      der[q] = d0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < SW; ++q) der[q] *= hinv;
+}
+
+template <typename T, int D, int BASIS, int NPASS, bool AOS>
+__global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                  uint32_t* __restrict__ misplaced) {
+  using G = Geo<D>;
+  constexpr int SW = support_width(BASIS);
+  constexpr int NV = 1 + 2 * D;
+  constexpr int KN = SW / NPASS;                       // last-axis offsets per pass
+  constexpr int NSL = Pow<SW, D - 1>::v * KN;          // offsets per pass
+  const int lane = threadIdx.x;
+  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
+  if (j >= a.n_active) return;
+  const uint32_t tile = a.active[j];
+  const uint32_t start = a.tile_start[tile];
+  const int cnt = int(a.tile_end[tile] - start);
+  int tI[D], origin[D];
+  unlin<D>(tile, a.ntiles, tI);
+#pragma unroll
+  for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
+
+  __shared__ T s_tile[NV][G::NFOOT];
+  __shared__ T s_nodex[D][G::TS];
+  __shared__ uint32_t s_cnt[64];
+  for (int q = lane; q < NV * G::NFOOT; q += 64) (&s_tile[0][0])[q] = T(0);
+  for (int q = lane; q < D * G::TS; q += 64) {
+    const int d = q / G::TS, k = q % G::TS;
+    s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
+  }
+  int ccount, coff, maxcnt;
+  tile_class_counts(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);
+  int ccoord[D];
+  {
+    int r = lane;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      ccoord[d] = r % G::BW;
+      r /= G::BW;
+    }
+  }
+  const T mp = a.mp, hinv = a.mesh.h_inv;
+  uint32_t bad = 0, layer_off = 0;
+  for (int pass = 0; pass < NPASS; ++pass) {
+    T am[NSL], amv[D][NSL], af[D][NSL];
+#pragma unroll
+    for (int o = 0; o < NSL; ++o) {
+      am[o] = T(0);
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        amv[c][o] = T(0);
+        af[c][o] = T(0);
+      }
+    }
+    for (int k = 0; k < maxcnt; ++k) {
+      const uint64_t has = __ballot(k < ccount);
+      if (k < ccount) {
+        const uint32_t pid = a.perm[start + coff + k];
+        if (misplaced && pass == 0) {
+          const uint32_t want = start + layer_off + uint32_t(__popcll(has & lanemask_lt64()));
+          bad += pid != want ? 1u : 0u;
+        }
+        T x[D], mpv[D], A[D * D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) x[d] = fld<T, D, AOS>(a.x, pid, d);
+#pragma unroll
+        for (int d = 0; d < D; ++d) mpv[d] = mp * fld<T, D, AOS>(a.v, pid, d);  // hoisted mₚ*v[p]
+        const T negV = -a.V(pid);
+#pragma unroll
+        for (int q = 0; q < D * D; ++q) A[q] = negV * fld<T, D * D, AOS>(a.sigma, pid, q);  // hoisted (-V[p])*σ[p]
+        T l = T(0);
+        if constexpr (BASIS == TESS_UGIMP) l = a.l(pid) / a.mesh.h;
+        T val[D][SW], der[D][SW];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const T xi = (x[d] - a.mesh.xmin[d]) * hinv;
+          int lo, hi;
+          axis_window<T, BASIS>(xi, l, a.mesh.dims[d], lo, hi);
+          axis_jets_mixed<T, BASIS>(x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], l, hinv, val[d], der[d]);
+        }
+        if constexpr (D == 3) {
+#pragma unroll
+          for (int kz = 0; kz < KN; ++kz) {
+            const int kg = pass * KN + kz;
+#pragma unroll
+            for (int jy = 0; jy < SW; ++jy) {
+              const T YZ = val[1][jy] * val[2][kg];
+              const T YdZ = der[1][jy] * val[2][kg];
+              const T YZd = val[1][jy] * der[2][kg];
+              const T cm = YZ * mp;
+              T cmv[3], U[3], Wv[3];
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                cmv[c] = YZ * mpv[c];
+                U[c] = A[c] * YZ;
+                Wv[c] = A[c + 3] * YdZ + A[c + 6] * YZd;
+              }
+#pragma unroll
+              for (int ix = 0; ix < SW; ++ix) {
+                const int o = ix + SW * (jy + SW * kz);
+                am[o] += val[0][ix] * cm;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                  amv[c][o] += val[0][ix] * cmv[c];
+                  af[c][o] += der[0][ix] * U[c] + val[0][ix] * Wv[c];
+                }
+              }
+            }
+          }
+        } else {
+#pragma unroll
+          for (int jy = 0; jy < KN; ++jy) {
+            const int jg = pass * KN + jy;
+            const T Y = val[1][jg], Yd = der[1][jg];
+            const T cm = Y * mp;
+            T cmv[2], U[2], Wv[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+              cmv[c] = Y * mpv[c];
+              U[c] = A[c] * Y;
+              Wv[c] = A[c + 2] * Yd;
+            }
+#pragma unroll
+            for (int ix = 0; ix < SW; ++ix) {
+              const int o = ix + SW * jy;
+              am[o] += val[0][ix] * cm;
+#pragma unroll
+              for (int c = 0; c < 2; ++c) {
+                amv[c][o] += val[0][ix] * cmv[c];
+                af[c][o] += der[0][ix] * U[c] + val[0][ix] * Wv[c];
+              }
+            }
+          }
+        }
+      }
+      if (pass == 0) layer_off += uint32_t(__popcll(has));
+    }
+    // lanes are distinct classes: each LDS add instruction touches each node
+    // at most once, and the per-node order of the adds is program order
+    if (ccount > 0) {
+#pragma unroll
+      for (int o = 0; o < NSL; ++o) {
+        int r = o, q = 0, mul = 1;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const int off = (d == D - 1) ? pass * KN + r : r % SW;
+          q += (ccoord[d] + off) * mul;
+          r /= SW;
+          mul *= G::TS;
+        }
+        atomicAdd(&s_tile[0][q], am[o]);
+#pragma unroll
+        for (int c = 0; c < D; ++c) atomicAdd(&s_tile[1 + c][q], amv[c][o]);
+#pragma unroll
+        for (int c = 0; c < D; ++c) atomicAdd(&s_tile[1 + D + c][q], af[c][o]);
+      }
+    }
+  }
+  if (misplaced) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) bad += __shfl_xor(bad, off, 64);
+    if (lane == 0 && bad) atomicAdd(misplaced, bad);
+  }
+  __syncthreads();
+  T* out = a.partials + int64_t(j) * NV * G::NFOOT;
+  for (int q = lane; q < NV * G::NFOOT; q += 64) out[q] = (&s_tile[0][0])[q];
+}
+
+// G2P with the footprint's v and v - vⁿ staged per node in LDS; the FLIP
+// increment is formed from the node differences as the reference does.
+template <typename T, int D, int BASIS, bool AOS>
+__global__ __launch_bounds__(64) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys) {
+  using G = Geo<D>;
+  constexpr int SW = support_width(BASIS);
+  const int lane = threadIdx.x;
+  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
+  if (j >= a.n_active) return;
+  const uint32_t tile = a.active[j];
+  const uint32_t start = a.tile_start[tile];
+  const int cnt = int(a.tile_end[tile] - start);
+  int tI[D], origin[D];
+  unlin<D>(tile, a.ntiles, tI);
+#pragma unroll
+  for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
+  __shared__ T s_v[G::NFOOT][D];
+  __shared__ T s_dv[G::NFOOT][D];
+  __shared__ T s_nodex[D][G::TS];
+  __shared__ uint32_t s_cnt[64];
+  for (int q = lane; q < G::NFOOT; q += 64) {
+    int I[D], r = q;
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      I[d] = origin[d] + r % G::TS;
+      r /= G::TS;
+      in = in && I[d] >= 0 && I[d] < a.mesh.dims[d];
+    }
+    const int64_t i = in ? a.mesh.lin(I) : 0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      const T vv = in ? a.gv(i, c) : T(0);
+      const T vn = in ? a.gvn(i, c) : T(0);
+      s_v[q][c] = vv;
+      s_dv[q][c] = vv - vn;
+    }
+  }
+  for (int q = lane; q < D * G::TS; q += 64) {
+    const int d = q / G::TS, k = q % G::TS;
+    s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
+  }
+  int ccount, coff, maxcnt;
+  tile_class_counts(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);  // barrier after the staging
+  int ccoord[D];
+  {
+    int r = lane;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      ccoord[d] = r % G::BW;
+      r /= G::BW;
+    }
+  }
+  const T hinv = a.mesh.h_inv, dt = a.dt;
+  for (int k = 0; k < maxcnt; ++k) {
+    if (k >= ccount) continue;
+    const uint32_t pid = a.perm[start + coff + k];
+    T x[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) x[d] = fld<T, D, AOS>(a.x, pid, d);
+    T l = T(0);
+    if constexpr (BASIS == TESS_UGIMP) l = a.l(pid) / a.mesh.h;
+    T val[D][SW], der[D][SW];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const T xi = (x[d] - a.mesh.xmin[d]) * hinv;
+      int lo, hi;
+      axis_window<T, BASIS>(xi, l, a.mesh.dims[d], lo, hi);
+      axis_jets_mixed<T, BASIS>(x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], l, hinv, val[d], der[d]);
+    }
+    // X = Σ w v_i, DV = Σ w (v_i - vⁿ_i), L = Σ v_i ⊗ ∇w, factored by axis
+    T X[D], DV[D], L[D * D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      X[c] = T(0);
+      DV[c] = T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < D * D; ++q) L[q] = T(0);
+    if constexpr (D == 3) {
+#pragma unroll
+      for (int kz = 0; kz < SW; ++kz)
+#pragma unroll
+        for (int jy = 0; jy < SW; ++jy) {
+          T S1[3] = {T(0), T(0), T(0)}, S2[3] = {T(0), T(0), T(0)}, Sd[3] = {T(0), T(0), T(0)};
+#pragma unroll
+          for (int ix = 0; ix < SW; ++ix) {
+            const int q = (ccoord[0] + ix) + G::TS * ((ccoord[1] + jy) + G::TS * (ccoord[2] + kz));
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const T vv = s_v[q][c];
+              S1[c] += val[0][ix] * vv;
+              S2[c] += der[0][ix] * vv;
+              Sd[c] += val[0][ix] * s_dv[q][c];
+            }
+          }
+          const T YZ = val[1][jy] * val[2][kz];
+          const T YdZ = der[1][jy] * val[2][kz];
+          const T YZd = val[1][jy] * der[2][kz];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            X[c] += YZ * S1[c];
+            DV[c] += YZ * Sd[c];
+            L[c] += YZ * S2[c];
+            L[c + 3] += YdZ * S1[c];
+            L[c + 6] += YZd * S1[c];
+          }
+        }
+    } else {
+#pragma unroll
+      for (int jy = 0; jy < SW; ++jy) {
+        T S1[2] = {T(0), T(0)}, S2[2] = {T(0), T(0)}, Sd[2] = {T(0), T(0)};
+#pragma unroll
+        for (int ix = 0; ix < SW; ++ix) {
+          const int q = (ccoord[0] + ix) + G::TS * (ccoord[1] + jy);
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            const T vv = s_v[q][c];
+            S1[c] += val[0][ix] * vv;
+            S2[c] += der[0][ix] * vv;
+            Sd[c] += val[0][ix] * s_dv[q][c];
+          }
+        }
+        const T Y = val[1][jy], Yd = der[1][jy];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          X[c] += Y * S1[c];
+          DV[c] += Y * Sd[c];
+          L[c] += Y * S2[c];
+          L[c + 2] += Yd * S1[c];
+        }
+      }
+    }
+    // particle equations (README.md:81-86)
+#pragma unroll
+    for (int c = 0; c < D; ++c) fldw<T, D, AOS>(a.v, pid, c) += DV[c];
+#pragma unroll
+    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.gradv, pid, q) = L[q];
+#pragma unroll
+    for (int c = 0; c < D; ++c) fldw<T, D, AOS>(a.x, pid, c) = x[c] + X[c] * dt;
+    T Fo[D * D], Fn[D * D];
+#pragma unroll
+    for (int q = 0; q < D * D; ++q) Fo[q] = fld<T, D * D, AOS>(a.F, pid, q);
+#pragma unroll
+    for (int b = 0; b < D; ++b)
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        T s = Fo[r + D * b];  // (I + Δt∇v) F = F + Δt (∇v F)
+        T t = L[r] * Fo[D * b];
+#pragma unroll
+        for (int kk = 1; kk < D; ++kk) t += L[r + D * kk] * Fo[kk + D * b];
+        Fn[r + D * b] = s + dt * t;
+      }
+#pragma unroll
+    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.F, pid, q) = Fn[q];
+    const double J = det_d<T, D>(Fn);
+    double lam = a.lambda, mu = a.mu;
+    if (a.logE.p) {
+      const double sc = exp(double(a.logE(pid)));
+      lam *= sc;
+      mu *= sc;
+    }
+    const double lnJ = log(J);
+    const double invJ = 1.0 / J;
+#pragma unroll
+    for (int b = 0; b < D; ++b)
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        double bb = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < D; ++kk) bb += double(Fn[r + D * kk]) * double(Fn[b + D * kk]);
+        const double sg = mu * (bb - (r == b ? 1.0 : 0.0)) + (r == b ? lam * lnJ : 0.0);
+        fldw<T, D * D, AOS>(a.sigma, pid, r + D * b) = T(sg * invJ);
+      }
+    a.V(pid) = T(a.V0 * J);
+  }
+}
+
 // ---- gather + grid-node half ------------------------------------------------
 template <typename T, int D>
 __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __restrict__ active_nodes) {
@@ -549,16 +955,6 @@ __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __re
 }
 
 // ---- G2P -------------------------------------------------------------------
-template <typename T, int D>
-__device__ __forceinline__ double det_d(const T* F) {
-  if constexpr (D == 2) {
-    return double(F[0]) * double(F[3]) - double(F[2]) * double(F[1]);
-  } else {
-    const double f11 = F[0], f21 = F[1], f31 = F[2], f12 = F[3], f22 = F[4], f32 = F[5], f13 = F[6], f23 = F[7],
-                 f33 = F[8];
-    return f11 * (f22 * f33 - f23 * f32) - f12 * (f21 * f33 - f23 * f31) + f13 * (f21 * f32 - f22 * f31);
-  }
-}
 
 // Particle equations of the README @G2P block (README.md:81-86) after the sums.
 template <typename T, int D>
@@ -814,6 +1210,14 @@ void fill_binning_ptrs(tess_ctx* ctx, Args<T, D>& a) {
   a.n_nb = b.n_nodeblocks;
 }
 
+// True when every particle field is Julia-native AoS (components contiguous,
+// element stride = component count), the layout the vectorised kernels assume.
+inline bool julia_layout(const tess_particles& P, int D) {
+  auto ok = [](const tess_field& f, int nc) { return !f.ptr || (f.comp_stride == 1 && f.elem_stride == nc) ||
+                                                      (nc == 1 && f.elem_stride == 1); };
+  return ok(P.x, D) && ok(P.v, D) && ok(P.V, 1) && ok(P.grad_v, D * D) && ok(P.F, D * D) && ok(P.sigma, D * D);
+}
+
 inline unsigned blocks_for(int64_t n, int threads) { return unsigned(std::max<int64_t>(1, ceil_div(n, threads))); }
 
 int key_bits_for(uint64_t maxkey) {
@@ -946,8 +1350,19 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     a.partials = ctx->partials.as<T>(size_t(b.n_active) * NV * Gm::NFOOT);
     // algorithmic bytes: x, v, V, σ per particle
     const int tok = prof_begin(ctx, "k_p2g_tile", double(b.n_inside) * sizeof(T) * (2 * D + 1 + D * D));
-    hipLaunchKernelGGL((k_p2g_tile<T, D, BASIS, STORED, NPASS>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
-                       ctx->stream, a, static_cast<const uint32_t*>(ctx->keys_sorted.ptr), misplaced);
+    const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
+    if constexpr (STORED) {
+      hipLaunchKernelGGL((k_p2g_tile<T, D, BASIS, STORED, NPASS>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
+                         ctx->stream, a, sk, misplaced);
+    } else {
+      constexpr int NPF = (Pow<SW, D>::v * NV * int(sizeof(T)) / 4 <= 192) ? 1 : SW;
+      if (julia_layout(P, D))
+        hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, true>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
+                           ctx->stream, a, sk, misplaced);
+      else
+        hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, false>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
+                           ctx->stream, a, sk, misplaced);
+    }
     TESS_HIP(hipGetLastError());
     prof_end(ctx, tok);
   }
@@ -987,8 +1402,18 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     const int tok = prof_begin(ctx, "k_g2p_tile",
                                double(b.n_inside) * sizeof(T) * (2 * D + D * D + 2 * D + 3 * D * D + 1) +
                                    nodes * sizeof(T) * 2 * D);
-    hipLaunchKernelGGL((k_g2p_tile<T, D, BASIS, STORED>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream, a,
-                       static_cast<const uint32_t*>(ctx->keys_sorted.ptr));
+    const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
+    if constexpr (STORED) {
+      hipLaunchKernelGGL((k_g2p_tile<T, D, BASIS, STORED>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream, a,
+                         sk);
+    } else {
+      if (julia_layout(P, D))
+        hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream,
+                           a, sk);
+      else
+        hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, false>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream,
+                           a, sk);
+    }
     TESS_HIP(hipGetLastError());
     prof_end(ctx, tok);
   }
