@@ -339,10 +339,10 @@ __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a, const uint32_t* _
 #pragma unroll
   for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
 
-  __shared__ T s_tile[NV][G::NFOOT];
+  __shared__ __align__(16) T s_tile[G::NFOOT][8];
   __shared__ T s_nodex[D][G::TS];
   __shared__ uint32_t s_cnt[64];
-  for (int q = lane; q < NV * G::NFOOT; q += 64) (&s_tile[0][0])[q] = T(0);
+  for (int q = lane; q < 8 * G::NFOOT; q += 64) (&s_tile[0][0])[q] = T(0);
   for (int q = lane; q < D * G::TS; q += 64) {
     const int d = q / G::TS, k = q % G::TS;
     s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
@@ -457,13 +457,13 @@ __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a, const uint32_t* _
           mul *= G::TS;
         }
 #pragma unroll
-        for (int v = 0; v < NV; ++v) atomicAdd(&s_tile[v][q], acc[s2][v]);
+        for (int v = 0; v < NV; ++v) atomicAdd(&s_tile[q][v], acc[s2][v]);
       }
     }
   }
   __syncthreads();
-  T* out = a.partials + int64_t(j) * NV * G::NFOOT;
-  for (int q = lane; q < NV * G::NFOOT; q += 64) out[q] = (&s_tile[0][0])[q];
+  T* out = a.partials + int64_t(j) * 8 * G::NFOOT;
+  for (int q = lane; q < 8 * G::NFOOT; q += 64) out[q] = (&s_tile[0][0])[q];
 }
 
 template <typename T, int D>
@@ -548,10 +548,11 @@ __global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* _
 #pragma unroll
   for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
 
-  __shared__ T s_tile[NV][G::NFOOT];
+  // node-major tile, 8 slots per node: m, mv[D], f[D], padding
+  __shared__ __align__(16) T s_tile[G::NFOOT][8];
   __shared__ T s_nodex[D][G::TS];
   __shared__ uint32_t s_cnt[64];
-  for (int q = lane; q < NV * G::NFOOT; q += 64) (&s_tile[0][0])[q] = T(0);
+  for (int q = lane; q < 8 * G::NFOOT; q += 64) (&s_tile[0][0])[q] = T(0);
   for (int q = lane; q < D * G::TS; q += 64) {
     const int d = q / G::TS, k = q % G::TS;
     s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
@@ -663,8 +664,9 @@ __global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* _
       }
       if (pass == 0) layer_off += uint32_t(__popcll(has));
     }
-    // lanes are distinct classes: each LDS add instruction touches each node
-    // at most once, and the per-node order of the adds is program order
+    // lanes are distinct classes, so for one offset their target nodes are
+    // distinct; offsets are applied one after another (read, add, write, wait
+    // for the LDS queue), which fixes the per-node order of the additions
     if (ccount > 0) {
 #pragma unroll
       for (int o = 0; o < NSL; ++o) {
@@ -676,11 +678,18 @@ __global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* _
           r /= SW;
           mul *= G::TS;
         }
-        atomicAdd(&s_tile[0][q], am[o]);
+        T t[8];
 #pragma unroll
-        for (int c = 0; c < D; ++c) atomicAdd(&s_tile[1 + c][q], amv[c][o]);
+        for (int v = 0; v < 8; ++v) t[v] = s_tile[q][v];
+        t[0] += am[o];
 #pragma unroll
-        for (int c = 0; c < D; ++c) atomicAdd(&s_tile[1 + D + c][q], af[c][o]);
+        for (int c = 0; c < D; ++c) {
+          t[1 + c] += amv[c][o];
+          t[1 + D + c] += af[c][o];
+        }
+#pragma unroll
+        for (int v = 0; v < 8; ++v) s_tile[q][v] = t[v];
+        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
       }
     }
   }
@@ -690,8 +699,8 @@ __global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* _
     if (lane == 0 && bad) atomicAdd(misplaced, bad);
   }
   __syncthreads();
-  T* out = a.partials + int64_t(j) * NV * G::NFOOT;
-  for (int q = lane; q < NV * G::NFOOT; q += 64) out[q] = (&s_tile[0][0])[q];
+  T* out = a.partials + int64_t(j) * 8 * G::NFOOT;
+  for (int q = lane; q < 8 * G::NFOOT; q += 64) out[q] = (&s_tile[0][0])[q];
 }
 
 // G2P with the footprint's v and v - vⁿ staged per node in LDS; the FLIP
@@ -710,8 +719,9 @@ __global__ __launch_bounds__(64) void k_g2p_fast(Args<T, D> a, const uint32_t* _
   unlin<D>(tile, a.ntiles, tI);
 #pragma unroll
   for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
-  __shared__ T s_v[G::NFOOT][D];
-  __shared__ T s_dv[G::NFOOT][D];
+  // per node: v[D] and v-vⁿ[D], padded to a power-of-two stride for vector LDS reads
+  constexpr int NS = (D == 3) ? 8 : 4;
+  __shared__ __align__(16) T s_n[G::NFOOT][NS];
   __shared__ T s_nodex[D][G::TS];
   __shared__ uint32_t s_cnt[64];
   for (int q = lane; q < G::NFOOT; q += 64) {
@@ -728,8 +738,8 @@ __global__ __launch_bounds__(64) void k_g2p_fast(Args<T, D> a, const uint32_t* _
     for (int c = 0; c < D; ++c) {
       const T vv = in ? a.gv(i, c) : T(0);
       const T vn = in ? a.gvn(i, c) : T(0);
-      s_v[q][c] = vv;
-      s_dv[q][c] = vv - vn;
+      s_n[q][c] = vv;
+      s_n[q][NS / 2 + c] = vv - vn;
     }
   }
   for (int q = lane; q < D * G::TS; q += 64) {
@@ -782,12 +792,14 @@ __global__ __launch_bounds__(64) void k_g2p_fast(Args<T, D> a, const uint32_t* _
 #pragma unroll
           for (int ix = 0; ix < SW; ++ix) {
             const int q = (ccoord[0] + ix) + G::TS * ((ccoord[1] + jy) + G::TS * (ccoord[2] + kz));
+            T nd[NS];
+#pragma unroll
+            for (int u = 0; u < NS; ++u) nd[u] = s_n[q][u];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-              const T vv = s_v[q][c];
-              S1[c] += val[0][ix] * vv;
-              S2[c] += der[0][ix] * vv;
-              Sd[c] += val[0][ix] * s_dv[q][c];
+              S1[c] += val[0][ix] * nd[c];
+              S2[c] += der[0][ix] * nd[c];
+              Sd[c] += val[0][ix] * nd[NS / 2 + c];
             }
           }
           const T YZ = val[1][jy] * val[2][kz];
@@ -809,12 +821,14 @@ __global__ __launch_bounds__(64) void k_g2p_fast(Args<T, D> a, const uint32_t* _
 #pragma unroll
         for (int ix = 0; ix < SW; ++ix) {
           const int q = (ccoord[0] + ix) + G::TS * (ccoord[1] + jy);
+          T nd[NS];
+#pragma unroll
+          for (int u = 0; u < NS; ++u) nd[u] = s_n[q][u];
 #pragma unroll
           for (int c = 0; c < 2; ++c) {
-            const T vv = s_v[q][c];
-            S1[c] += val[0][ix] * vv;
-            S2[c] += der[0][ix] * vv;
-            Sd[c] += val[0][ix] * s_dv[q][c];
+            S1[c] += val[0][ix] * nd[c];
+            S2[c] += der[0][ix] * nd[c];
+            Sd[c] += val[0][ix] * nd[NS / 2 + c];
           }
         }
         const T Y = val[1][jy], Yd = der[1][jy];
@@ -923,9 +937,9 @@ __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __re
             mul *= G::TS;
           }
         }
-        const T* P = a.partials + int64_t(a.tile_slot[t]) * NV * G::NFOOT + q;
+        const T* P = a.partials + (int64_t(a.tile_slot[t]) * G::NFOOT + q) * 8;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) acc[v] += P[int64_t(v) * G::NFOOT];
+        for (int v = 0; v < NV; ++v) acc[v] += P[v];
       }
   const int64_t i = a.mesh.lin(I);
   // grid-node half with the guarded Float64 division (test/transfer.jl:102-104)
@@ -1347,7 +1361,7 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     ctx->misplaced_x_ptr = nullptr;
   }
   if (b.n_active > 0) {
-    a.partials = ctx->partials.as<T>(size_t(b.n_active) * NV * Gm::NFOOT);
+    a.partials = ctx->partials.as<T>(size_t(b.n_active) * 8 * Gm::NFOOT);
     // algorithmic bytes: x, v, V, σ per particle
     const int tok = prof_begin(ctx, "k_p2g_tile", double(b.n_inside) * sizeof(T) * (2 * D + 1 + D * D));
     const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
