@@ -282,12 +282,13 @@ __device__ __forceinline__ uint64_t lanemask_lt64() { return (uint64_t(1) << (th
 
 // Per-class counts of one tile from its slice of the sorted keys (class = low 6
 // bits), by wave-level matching; lane c gets count and offset of class c.
+template <int NW = 1>
 __device__ __forceinline__ void tile_class_counts(const uint32_t* __restrict__ skeys, uint32_t start, int cnt,
                                                   uint32_t* s_cnt, int& ccount, int& coff, int& maxcnt) {
-  const int lane = threadIdx.x & 63;
-  s_cnt[lane] = 0;
+  const int lane = threadIdx.x & 63, wave = int(threadIdx.x >> 6);
+  if (wave == 0) s_cnt[lane] = 0;
   __syncthreads();
-  for (int base = 0; base < cnt; base += 64) {
+  for (int base = wave * 64; base < cnt; base += 64 * NW) {
     const int i = base + lane;
     const bool valid = i < cnt;
     const uint32_t c = valid ? (skeys[start + i] & 63u) : 0u;
@@ -704,12 +705,33 @@ __global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* _
 }
 
 // G2P with the footprint's v and v - vⁿ staged per node in LDS; the FLIP
-// increment is formed from the node differences as the reference does.
+// increment is formed from the node differences as the reference does.  NW
+// waves share one tile: wave w walks the class-major order at k = w, w+NW, ...
+// and prefetches its next particle while updating the current one.
+template <typename T, int D, bool AOS>
+struct G2PIn {
+  uint32_t pid;
+  T x[D], v[D], F[D * D], l;
+};
+
 template <typename T, int D, int BASIS, bool AOS>
-__global__ __launch_bounds__(64) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys) {
+__device__ __forceinline__ void g2p_load(const Args<T, D>& a, uint32_t pid, G2PIn<T, D, AOS>& in) {
+  in.pid = pid;
+#pragma unroll
+  for (int c = 0; c < D; ++c) in.x[c] = fld<T, D, AOS>(a.x, pid, c);
+#pragma unroll
+  for (int c = 0; c < D; ++c) in.v[c] = fld<T, D, AOS>(a.v, pid, c);
+#pragma unroll
+  for (int q = 0; q < D * D; ++q) in.F[q] = fld<T, D * D, AOS>(a.F, pid, q);
+  in.l = T(0);
+  if constexpr (BASIS == TESS_UGIMP) in.l = a.l(pid) / a.mesh.h;
+}
+
+template <typename T, int D, int BASIS, bool AOS, int NW>
+__global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys) {
   using G = Geo<D>;
   constexpr int SW = support_width(BASIS);
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = int(threadIdx.x >> 6);
   const int64_t j = xcd_remap(blockIdx.x, a.n_active);
   if (j >= a.n_active) return;
   const uint32_t tile = a.active[j];
@@ -724,7 +746,7 @@ __global__ __launch_bounds__(64) void k_g2p_fast(Args<T, D> a, const uint32_t* _
   __shared__ __align__(16) T s_n[G::NFOOT][NS];
   __shared__ T s_nodex[D][G::TS];
   __shared__ uint32_t s_cnt[64];
-  for (int q = lane; q < G::NFOOT; q += 64) {
+  for (int q = int(threadIdx.x); q < G::NFOOT; q += 64 * NW) {
     int I[D], r = q;
     bool in = true;
 #pragma unroll
@@ -742,12 +764,12 @@ __global__ __launch_bounds__(64) void k_g2p_fast(Args<T, D> a, const uint32_t* _
       s_n[q][NS / 2 + c] = vv - vn;
     }
   }
-  for (int q = lane; q < D * G::TS; q += 64) {
+  for (int q = int(threadIdx.x); q < D * G::TS; q += 64 * NW) {
     const int d = q / G::TS, k = q % G::TS;
     s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
   }
   int ccount, coff, maxcnt;
-  tile_class_counts(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);  // barrier after the staging
+  tile_class_counts<NW>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);  // barrier after the staging
   int ccoord[D];
   {
     int r = lane;
@@ -758,21 +780,20 @@ __global__ __launch_bounds__(64) void k_g2p_fast(Args<T, D> a, const uint32_t* _
     }
   }
   const T hinv = a.mesh.h_inv, dt = a.dt;
-  for (int k = 0; k < maxcnt; ++k) {
+  G2PIn<T, D, AOS> cur, nxt;
+  if (wave < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + wave], nxt);
+  for (int k = wave; k < maxcnt; k += NW) {
+    cur = nxt;
+    if (k + NW < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + NW], nxt);
     if (k >= ccount) continue;
-    const uint32_t pid = a.perm[start + coff + k];
-    T x[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) x[d] = fld<T, D, AOS>(a.x, pid, d);
-    T l = T(0);
-    if constexpr (BASIS == TESS_UGIMP) l = a.l(pid) / a.mesh.h;
     T val[D][SW], der[D][SW];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const T xi = (x[d] - a.mesh.xmin[d]) * hinv;
+      const T xi = (cur.x[d] - a.mesh.xmin[d]) * hinv;
       int lo, hi;
-      axis_window<T, BASIS>(xi, l, a.mesh.dims[d], lo, hi);
-      axis_jets_mixed<T, BASIS>(x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], l, hinv, val[d], der[d]);
+      axis_window<T, BASIS>(xi, cur.l, a.mesh.dims[d], lo, hi);
+      axis_jets_mixed<T, BASIS>(cur.x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], cur.l, hinv, val[d],
+                                der[d]);
     }
     // X = Σ w v_i, DV = Σ w (v_i - vⁿ_i), L = Σ v_i ⊗ ∇w, factored by axis
     T X[D], DV[D], L[D * D];
@@ -841,25 +862,23 @@ __global__ __launch_bounds__(64) void k_g2p_fast(Args<T, D> a, const uint32_t* _
         }
       }
     }
+    const uint32_t pid = cur.pid;
     // particle equations (README.md:81-86)
 #pragma unroll
-    for (int c = 0; c < D; ++c) fldw<T, D, AOS>(a.v, pid, c) += DV[c];
+    for (int c = 0; c < D; ++c) fldw<T, D, AOS>(a.v, pid, c) = cur.v[c] + DV[c];
 #pragma unroll
     for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.gradv, pid, q) = L[q];
 #pragma unroll
-    for (int c = 0; c < D; ++c) fldw<T, D, AOS>(a.x, pid, c) = x[c] + X[c] * dt;
-    T Fo[D * D], Fn[D * D];
-#pragma unroll
-    for (int q = 0; q < D * D; ++q) Fo[q] = fld<T, D * D, AOS>(a.F, pid, q);
+    for (int c = 0; c < D; ++c) fldw<T, D, AOS>(a.x, pid, c) = cur.x[c] + X[c] * dt;
+    T Fn[D * D];
 #pragma unroll
     for (int b = 0; b < D; ++b)
 #pragma unroll
       for (int r = 0; r < D; ++r) {
-        T s = Fo[r + D * b];  // (I + Δt∇v) F = F + Δt (∇v F)
-        T t = L[r] * Fo[D * b];
+        T t = L[r] * cur.F[D * b];  // (I + Δt∇v) F = F + Δt (∇v F)
 #pragma unroll
-        for (int kk = 1; kk < D; ++kk) t += L[r + D * kk] * Fo[kk + D * b];
-        Fn[r + D * b] = s + dt * t;
+        for (int kk = 1; kk < D; ++kk) t += L[r + D * kk] * cur.F[kk + D * b];
+        Fn[r + D * b] = cur.F[r + D * b] + dt * t;
       }
 #pragma unroll
     for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.F, pid, q) = Fn[q];
@@ -887,15 +906,32 @@ __global__ __launch_bounds__(64) void k_g2p_fast(Args<T, D> a, const uint32_t* _
 }
 
 // ---- gather + grid-node half ------------------------------------------------
-template <typename T, int D>
+template <typename T, int D, bool GAOS>
 __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __restrict__ active_nodes) {
   using G = Geo<D>;
   constexpr int NV = 1 + 2 * D;
+  constexpr int NB = (D == 3) ? 27 : 9;
   const int64_t j = xcd_remap(blockIdx.x, a.n_nb);
   if (j >= a.n_nb) return;
   const uint32_t B = a.nb_list[j];
   int Bi[D], I[D], rr[D];
   unlin<D>(B, a.ntiles, Bi);
+  // slots of the 3^dim neighbour tiles (-1: no partial)
+  __shared__ int s_slot[NB];
+  if (threadIdx.x < NB) {
+    int r = int(threadIdx.x), t = 0, mul = 1;
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int b = Bi[d] + (r % 3) - 1;
+      r /= 3;
+      ok = ok && b >= 0 && b < a.ntiles[d];
+      t += b * mul;
+      mul *= a.ntiles[d];
+    }
+    s_slot[threadIdx.x] = (ok && a.tile_flag[t]) ? int(a.tile_slot[t]) : -1;
+  }
+  __syncthreads();
   {
     int r = threadIdx.x;
 #pragma unroll
@@ -914,30 +950,24 @@ __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __re
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    lo[d] = (rr[d] <= 1 && Bi[d] > 0) ? -1 : 0;
-    hi[d] = (rr[d] == G::BW - 1 && Bi[d] + 1 < a.ntiles[d]) ? 1 : 0;
+    lo[d] = rr[d] <= 1 ? -1 : 0;
+    hi[d] = rr[d] == G::BW - 1 ? 1 : 0;
   }
   int o[3] = {0, 0, 0};
   for (o[2] = lo[2]; o[2] <= hi[2]; ++o[2])
     for (o[1] = lo[1]; o[1] <= hi[1]; ++o[1])
       for (o[0] = lo[0]; o[0] <= hi[0]; ++o[0]) {
-        int64_t t = 0;
-        int q = 0;
+        int nb = 0, q = 0, mulb = 1, mulq = 1;
 #pragma unroll
-        for (int d = D - 1; d >= 0; --d) {
-          const int b = Bi[d] + o[d];
-          t = t * a.ntiles[d] + b;
+        for (int d = 0; d < D; ++d) {
+          nb += (o[d] + 1) * mulb;
+          mulb *= 3;
+          q += (rr[d] + 1 - o[d] * G::BW) * mulq;  // node index inside that tile's footprint
+          mulq *= G::TS;
         }
-        if (!a.tile_flag[t]) continue;
-        {
-          int mul = 1;
-#pragma unroll
-          for (int d = 0; d < D; ++d) {
-            q += (I[d] - ((Bi[d] + o[d]) * G::BW - 1)) * mul;
-            mul *= G::TS;
-          }
-        }
-        const T* P = a.partials + (int64_t(a.tile_slot[t]) * G::NFOOT + q) * 8;
+        const int slot = s_slot[nb];
+        if (slot < 0) continue;
+        const T* P = a.partials + (int64_t(slot) * G::NFOOT + q) * 8;
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] += P[v];
       }
@@ -952,6 +982,7 @@ __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __re
   }
   const double minv = has ? 1.0 / md : 0.0;
   a.gm(i) = acc[0];
+  T mvo[D], fo[D], vno[D], vo[D];
 #pragma unroll
   for (int c = 0; c < D; ++c) {
     const T mvc = acc[1 + c], fc = acc[1 + D + c];
@@ -961,10 +992,17 @@ __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __re
     const bool wall = (((a.slip >> (2 * c)) & 1) && I[c] == 0) ||
                       (((a.slip >> (2 * c + 1)) & 1) && I[c] == a.mesh.dims[c] - 1);
     if (wall) vc = T(0);
-    a.gmv(i, c) = mvc;
-    a.gf(i, c) = fc;
-    a.gvn(i, c) = vn;
-    a.gv(i, c) = vc;
+    mvo[c] = mvc;
+    fo[c] = fc;
+    vno[c] = vn;
+    vo[c] = vc;
+  }
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    fldw<T, D, GAOS>(a.gmv, i, c) = mvo[c];
+    fldw<T, D, GAOS>(a.gf, i, c) = fo[c];
+    fldw<T, D, GAOS>(a.gvn, i, c) = vno[c];
+    fldw<T, D, GAOS>(a.gv, i, c) = vo[c];
   }
 }
 
@@ -1232,6 +1270,11 @@ inline bool julia_layout(const tess_particles& P, int D) {
   return ok(P.x, D) && ok(P.v, D) && ok(P.V, 1) && ok(P.grad_v, D * D) && ok(P.F, D * D) && ok(P.sigma, D * D);
 }
 
+inline bool grid_julia_layout(const tess_grid& G, int D) {
+  auto ok = [D](const tess_field& f) { return f.comp_stride == 1 && f.elem_stride == D; };
+  return ok(G.mv) && ok(G.f) && ok(G.vn) && ok(G.v);
+}
+
 inline unsigned blocks_for(int64_t n, int threads) { return unsigned(std::max<int64_t>(1, ceil_div(n, threads))); }
 
 int key_bits_for(uint64_t maxkey) {
@@ -1386,8 +1429,12 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     // grid-node half, vⁿ and v written (the node count of the previous call)
     const double nodes = double(std::max<int64_t>(ctx->last_active_nodes, 0));
     const int tok = prof_begin(ctx, "k_gather_grid", nodes * sizeof(T) * (2 * (2 * D + 1) + 2 * D));
-    hipLaunchKernelGGL((k_gather_grid<T, D>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a,
-                       counters + CNT_NODES);
+    if (grid_julia_layout(G, D))
+      hipLaunchKernelGGL((k_gather_grid<T, D, true>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a,
+                         counters + CNT_NODES);
+    else
+      hipLaunchKernelGGL((k_gather_grid<T, D, false>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a,
+                         counters + CNT_NODES);
     TESS_HIP(hipGetLastError());
     prof_end(ctx, tok);
   }
@@ -1421,12 +1468,13 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
       hipLaunchKernelGGL((k_g2p_tile<T, D, BASIS, STORED>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream, a,
                          sk);
     } else {
+      constexpr int NW = 4;
       if (julia_layout(P, D))
-        hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream,
-                           a, sk);
+        hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
+                           ctx->stream, a, sk);
       else
-        hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, false>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream,
-                           a, sk);
+        hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, false, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
+                           ctx->stream, a, sk);
     }
     TESS_HIP(hipGetLastError());
     prof_end(ctx, tok);
