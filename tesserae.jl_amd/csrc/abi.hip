@@ -216,7 +216,8 @@ tess_status tess_ctx_destroy(tess_ctx* ctx) {
                             &ctx->vals_alt,  &ctx->hist,        &ctx->scan_tmp,     &ctx->tile_start,
                             &ctx->tile_end,  &ctx->tile_flag,   &ctx->tile_slot,    &ctx->active_tiles,
                             &ctx->nb_flag,   &ctx->nb_slot,     &ctx->nb_list,      &ctx->nb_list_prev,
-                            &ctx->partials,  &ctx->counters,    &ctx->host_pinned_dummy, &ctx->reorder_tmp};
+                            &ctx->partials,  &ctx->counters,    &ctx->host_pinned_dummy, &ctx->reorder_tmp,
+                            &ctx->per_tile,  &ctx->per_nodeblock};
     for (auto* b : bufs) b->release();
     for (auto ev : ctx->prof.pool) (void)hipEventDestroy(ev);
     if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);

@@ -176,11 +176,16 @@ __global__ void k_partition_runs(const uint32_t* __restrict__ skeys, const uint3
 
 // ---- reorder ------------------------------------------------------------------
 
-__global__ void k_contiguity(const int32_t* __restrict__ pidx, int64_t na, uint32_t* __restrict__ count) {
+// per 256-thread block: number of i with pidx[i] == pidx[i-1] + 1
+__global__ __launch_bounds__(256) void k_contiguity(const int32_t* __restrict__ pidx, int64_t na,
+                                                    uint32_t* __restrict__ per_block) {
+  __shared__ uint32_t s[4];
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x + 1;
   const bool hit = i < na && pidx[i] == pidx[i - 1] + 1;
   const uint64_t m = __ballot(hit);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, uint32_t(__popcll(m)));
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = uint32_t(__popcll(m));
+  __syncthreads();
+  if (threadIdx.x == 0) per_block[blockIdx.x] = s[0] + s[1] + s[2] + s[3];
 }
 
 __global__ void k_mark_seen(const int32_t* __restrict__ pidx, int64_t na, int64_t n, uint32_t* __restrict__ seen,
@@ -384,9 +389,11 @@ void run_reorder(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, 
   if (!go) {
     double score = 1.0;
     if (na > 1) {
-      TESS_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), ctx->stream));
-      hipLaunchKernelGGL(k_contiguity, dim3(nblk(na - 1, 256)), dim3(256), 0, ctx->stream, pidx, na, cnt);
+      const unsigned nb = nblk(na - 1, 256);
+      uint32_t* per_block = ctx->per_tile.as<uint32_t>(nb);
+      hipLaunchKernelGGL(k_contiguity, dim3(nb), dim3(256), 0, ctx->stream, pidx, na, per_block);
       TESS_HIP(hipGetLastError());
+      sum_u32(ctx, per_block, nb, cnt);
       TESS_HIP(hipMemcpyAsync(hc, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
       TESS_HIP(hipStreamSynchronize(ctx->stream));
       score = double(hc[0]) / double(na - 1);

@@ -185,7 +185,28 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* 
   (void)lane;
 }
 
+// *out = sum of v[0..n) (one workgroup; integer, so order-independent)
+__global__ __launch_bounds__(1024) void k_sum_u32(const uint32_t* __restrict__ v, int64_t n, uint32_t* __restrict__ out) {
+  __shared__ uint32_t s[16];
+  uint32_t acc = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) acc += v[i];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < 16; ++w) t += s[w];
+    *out = t;
+  }
+}
+
 }  // namespace
+
+void sum_u32(tess_ctx* ctx, const uint32_t* v, int64_t n, uint32_t* out) {
+  hipLaunchKernelGGL(k_sum_u32, dim3(1), dim3(1024), 0, ctx->stream, v, n, out);
+  TESS_HIP(hipGetLastError());
+}
 
 void exclusive_scan_u32(tess_ctx* ctx, uint32_t* data, int64_t n, uint32_t* total_dev) {
   if (n <= 0) {

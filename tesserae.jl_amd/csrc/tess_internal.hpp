@@ -81,6 +81,7 @@ struct tess_ctx {
   tess::Buffer tile_start, tile_end, tile_flag, tile_slot, active_tiles;
   tess::Buffer nb_flag, nb_slot, nb_list, nb_list_prev;
   tess::Buffer partials;
+  tess::Buffer per_tile, per_nodeblock;  // per-workgroup counts summed by a reduction
   tess::Buffer counters;   // small device scalars
   tess::Buffer host_pinned_dummy;
   int64_t* counters_host = nullptr;  // pinned readback slots
@@ -107,6 +108,8 @@ namespace tess {
 // Exclusive prefix sum of n uint32 values in place; returns nothing, total is
 // written to *total_dev (device) when non-null.
 void exclusive_scan_u32(tess_ctx* ctx, uint32_t* data, int64_t n, uint32_t* total_dev);
+// *out (device) = sum of v[0..n), one workgroup (for per-workgroup counts).
+void sum_u32(tess_ctx* ctx, const uint32_t* v, int64_t n, uint32_t* out);
 // Stable LSD radix sort of (key, value) pairs on the low key_bits of key.
 // vals_in == nullptr means values 0..n-1.  Result lands in (keys_out, vals_out).
 void radix_sort_pairs(tess_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,

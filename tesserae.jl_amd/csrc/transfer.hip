@@ -373,7 +373,7 @@ __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a, const uint32_t* _
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) bad += __shfl_xor(bad, off, 64);
-    if (lane == 0 && bad) atomicAdd(misplaced, bad);
+    if (lane == 0) misplaced[j] = bad;  // per tile, summed later
   }
   const T mp = a.mp, hinv = a.mesh.h_inv;
   for (int pass = 0; pass < NPASS; ++pass) {
@@ -697,7 +697,7 @@ __global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* _
   if (misplaced) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) bad += __shfl_xor(bad, off, 64);
-    if (lane == 0 && bad) atomicAdd(misplaced, bad);
+    if (lane == 0) misplaced[j] = bad;  // per tile, summed later
   }
   __syncthreads();
   T* out = a.partials + int64_t(j) * 8 * G::NFOOT;
@@ -907,7 +907,7 @@ __global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32
 
 // ---- gather + grid-node half ------------------------------------------------
 template <typename T, int D, bool GAOS>
-__global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __restrict__ active_nodes) {
+__global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __restrict__ nodes_per_block) {
   using G = Geo<D>;
   constexpr int NV = 1 + 2 * D;
   constexpr int NB = (D == 3) ? 27 : 9;
@@ -976,9 +976,10 @@ __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __re
   const double md = double(acc[0]);
   const bool has = md > 0.0;
   {
+    // nodes with m > 0 in this block (one plain store per block, summed later)
     const uint64_t m = __ballot(has);
-    if ((threadIdx.x & 63) == __ffsll((unsigned long long)__ballot(true)) - 1 && m)
-      atomicAdd(active_nodes, uint32_t(__popcll(m)));
+    if ((threadIdx.x & 63) == __ffsll((unsigned long long)__ballot(true)) - 1)
+      nodes_per_block[j] = uint32_t(__popcll(m));
   }
   const double minv = has ? 1.0 / md : 0.0;
   a.gm(i) = acc[0];
@@ -1395,9 +1396,10 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   const Binning& b = ctx->bin;
   uint32_t* counters = static_cast<uint32_t*>(ctx->counters.ptr);
   uint32_t* misplaced = nullptr;
+  uint32_t* per_tile = ctx->per_tile.as<uint32_t>(size_t(std::max<int64_t>(b.n_active, 1)));
+  uint32_t* per_nb = ctx->per_nodeblock.as<uint32_t>(size_t(std::max<int64_t>(b.n_nodeblocks, 1)));
   if (count_misplaced) {
-    misplaced = counters + CNT_MISPLACED;
-    TESS_HIP(hipMemsetAsync(misplaced, 0, sizeof(uint32_t), ctx->stream));
+    misplaced = per_tile;
     ctx->misplaced_x_ptr = P.x.ptr;
     ctx->misplaced_n = P.n;
   } else {
@@ -1422,6 +1424,9 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     }
     TESS_HIP(hipGetLastError());
     prof_end(ctx, tok);
+    if (misplaced) {
+      sum_u32(ctx, misplaced, b.n_active, counters + CNT_MISPLACED);
+    }
   }
   TESS_HIP(hipMemsetAsync(counters + CNT_NODES, 0, sizeof(uint32_t), ctx->stream));
   if (b.n_nodeblocks > 0) {
@@ -1431,12 +1436,13 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     const int tok = prof_begin(ctx, "k_gather_grid", nodes * sizeof(T) * (2 * (2 * D + 1) + 2 * D));
     if (grid_julia_layout(G, D))
       hipLaunchKernelGGL((k_gather_grid<T, D, true>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a,
-                         counters + CNT_NODES);
+                         per_nb);
     else
       hipLaunchKernelGGL((k_gather_grid<T, D, false>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a,
-                         counters + CNT_NODES);
+                         per_nb);
     TESS_HIP(hipGetLastError());
     prof_end(ctx, tok);
+    sum_u32(ctx, per_nb, b.n_nodeblocks, counters + CNT_NODES);
   }
   // remember this call's node blocks for the next call's stale zeroing
   uint32_t* prev = ctx->nb_list_prev.as<uint32_t>(size_t(std::max<int64_t>(b.ntiles, 1)));
