@@ -212,7 +212,9 @@ tess_status tess_profile_read(tess_ctx* ctx, int32_t index, char* name, int32_t 
 /* Host copies of the last binning and step: out[0] particles inside the mesh,
  * out[1] active tiles, out[2] node blocks written, out[3] grid nodes with m>0
  * written by the previous P2G, out[4] reorders done by tess_step, out[5]
- * particles found out of the layered order by the previous P2G (-1 unknown). */
+ * particles found out of the layered order by the previous P2G (-1 unknown),
+ * out[6] particles whose tile or window class changed at the last incremental
+ * re-binning (-1 after a full sort). */
 tess_status tess_stats(tess_ctx* ctx, int64_t* out, int32_t count);
 
 #ifdef __cplusplus

@@ -217,7 +217,9 @@ tess_status tess_ctx_destroy(tess_ctx* ctx) {
                             &ctx->tile_end,  &ctx->tile_flag,   &ctx->tile_slot,    &ctx->active_tiles,
                             &ctx->nb_flag,   &ctx->nb_slot,     &ctx->nb_list,      &ctx->nb_list_prev,
                             &ctx->partials,  &ctx->counters,    &ctx->host_pinned_dummy, &ctx->reorder_tmp,
-                            &ctx->per_tile,  &ctx->per_nodeblock};
+                            &ctx->per_tile,  &ctx->per_nodeblock, &ctx->skeys2, &ctx->perm2,
+                            &ctx->inc_nk,    &ctx->inc_spos,  &ctx->inc_mkey, &ctx->inc_mj,
+                            &ctx->inc_mks,   &ctx->inc_mr,    &ctx->inc_mjs};
     for (auto* b : bufs) b->release();
     for (auto ev : ctx->prof.pool) (void)hipEventDestroy(ev);
     if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
@@ -350,7 +352,7 @@ tess_status tess_stats(tess_ctx* ctx, int64_t* out, int32_t count) {
     check_ctx(ctx);
     if (!out || count < 0) raise(TESS_ERR_ARGUMENT, "out is NULL");
     const int64_t vals[] = {ctx->bin.n_inside, ctx->bin.n_active, ctx->bin.n_nodeblocks, ctx->last_active_nodes,
-                            ctx->reorders, ctx->last_misplaced};
+                            ctx->reorders, ctx->last_misplaced, ctx->last_movers};
     for (int32_t i = 0; i < count && i < int32_t(sizeof(vals) / sizeof(vals[0])); ++i) out[i] = vals[i];
   });
 }

@@ -308,6 +308,7 @@ void partition_typed(tess_ctx* ctx, const tess_mesh& mesh, tess_field x, int64_t
     TESS_HIP(hipGetLastError());
     // the internal tile binning used the same buffers
     ctx->bin.valid = false;
+    ctx->bin.state_valid = false;
   }
   uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
   TESS_HIP(hipMemcpyAsync(hc, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -425,6 +426,7 @@ void run_reorder(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, 
     TESS_HIP(hipGetLastError());
   }
   ctx->bin.valid = false;
+    ctx->bin.state_valid = false;
   *reordered = 1;
 }
 

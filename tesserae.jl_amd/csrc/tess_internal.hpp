@@ -47,6 +47,10 @@ struct Binning {
   int64_t n_inside = 0;     // particles with a non-empty window (host copy)
   int64_t n_active = 0;     // tiles holding >= 1 particle (host copy)
   int64_t n_nodeblocks = 0; // node blocks touched by some tile footprint (host copy)
+  // the sorted (key, perm) arrays are a valid previous order for incremental re-binning
+  bool state_valid = false;
+  int64_t state_n = -1, state_ntiles = -1;
+  int32_t state_dim = 0;
 };
 
 // Per-kernel timing with HIP events on the context's stream.  Disabled by
@@ -82,6 +86,9 @@ struct tess_ctx {
   tess::Buffer nb_flag, nb_slot, nb_list, nb_list_prev;
   tess::Buffer partials;
   tess::Buffer per_tile, per_nodeblock;  // per-workgroup counts summed by a reduction
+  // incremental re-binning scratch
+  tess::Buffer skeys2, perm2, inc_nk, inc_spos, inc_mkey, inc_mj, inc_mks, inc_mr, inc_mjs;
+  int64_t last_movers = -1;
   tess::Buffer counters;   // small device scalars
   tess::Buffer host_pinned_dummy;
   int64_t* counters_host = nullptr;  // pinned readback slots

@@ -56,7 +56,7 @@ struct Pow<N, 0> {
 // misplaced-particle count of the last P2G (zeroed per P2G)
 enum {
   CNT_INSIDE = 0, CNT_ACTIVE = 1, CNT_NB = 2, CNT_ERR = 3, CNT_BIN_SLOTS = 4,
-  CNT_MISPLACED = 8, CNT_NODES = 9, CNT_SLOTS = 16
+  CNT_MISPLACED = 8, CNT_NODES = 9, CNT_STAY = 10, CNT_SLOTS = 16
 };
 enum { ERR_GIMP_LENGTH = 1, ERR_WINDOW = 2 };
 
@@ -140,26 +140,138 @@ __device__ __forceinline__ bool particle_window(const Args<T, D>& a, int64_t p, 
   }
 }
 
+// key = tile * 64 + class: the tile whose footprint [BW*t-1, BW*t+BW+1] holds
+// the particle's window, and the window start relative to the footprint
+// origin; ntiles*64 for particles with an empty window.
+template <typename T, int D, int BASIS, bool STORED>
+__device__ __forceinline__ uint32_t particle_key(const Args<T, D>& a, int64_t p, uint32_t ntiles, uint32_t* err) {
+  T x[D], l;
+  int lo[D], hi[D];
+  if (!particle_window<T, D, BASIS, STORED>(a, p, x, l, lo, hi, err)) return ntiles * 64u;
+  uint32_t t = 0, c = 0;
+#pragma unroll
+  for (int d = D - 1; d >= 0; --d) {
+    t = t * uint32_t(a.ntiles[d]) + uint32_t((lo[d] + 1) >> Geo<D>::LT);
+    c = c * uint32_t(Geo<D>::BW) + uint32_t((lo[d] + 1) & (Geo<D>::BW - 1));
+  }
+  return t * 64u + c;
+}
+
 template <typename T, int D, int BASIS, bool STORED>
 __global__ __launch_bounds__(256) void k_bin_keys(Args<T, D> a, uint32_t* __restrict__ keys, uint32_t ntiles,
                                                   uint32_t* __restrict__ counters) {
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= a.n) return;
-  T x[D], l;
-  int lo[D], hi[D];
-  // key = tile * 64 + class: the tile whose footprint [BW*t-1, BW*t+BW+1]
-  // holds the window, and the window start relative to the footprint origin
-  uint32_t key = ntiles * 64u;
-  if (particle_window<T, D, BASIS, STORED>(a, p, x, l, lo, hi, counters + CNT_ERR)) {
-    uint32_t t = 0, c = 0;
-#pragma unroll
-    for (int d = D - 1; d >= 0; --d) {
-      t = t * uint32_t(a.ntiles[d]) + uint32_t((lo[d] + 1) >> Geo<D>::LT);
-      c = c * uint32_t(Geo<D>::BW) + uint32_t((lo[d] + 1) & (Geo<D>::BW - 1));
-    }
-    key = t * 64u + c;
+  keys[p] = particle_key<T, D, BASIS, STORED>(a, p, ntiles, counters + CNT_ERR);
+}
+
+// ---- incremental re-binning ---------------------------------------------------
+// New keys are taken in the previous sorted order j; the new order is the
+// stable sort of that order by the new keys.  Particles whose key is unchanged
+// ("stayers") keep their relative order; the few "movers" are sorted among
+// themselves and merged.  The full radix sort of (new key, previous order) is
+// the fallback when many particles move; both give the same permutation.
+
+template <typename T, int D, int BASIS, bool STORED>
+__global__ __launch_bounds__(256) void k_keys_sorted(Args<T, D> a, const uint32_t* __restrict__ perm,
+                                                     const uint32_t* __restrict__ skeys, uint32_t ntiles,
+                                                     uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
+                                                     uint32_t* __restrict__ counters) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j > a.n) return;
+  if (j == a.n) {
+    stay[j] = 0;  // scan slot holding the stayer total
+    return;
   }
-  keys[p] = key;
+  const uint32_t k = particle_key<T, D, BASIS, STORED>(a, perm[j], ntiles, counters + CNT_ERR);
+  nk[j] = k;
+  stay[j] = k == skeys[j] ? 1u : 0u;
+}
+
+__global__ void k_compact_movers(const uint32_t* __restrict__ nk, const uint32_t* __restrict__ spos, int64_t n,
+                                 uint32_t* __restrict__ mkey, uint32_t* __restrict__ mj) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= n || spos[j + 1] != spos[j]) return;
+  const int64_t r = j - spos[j];
+  mkey[r] = nk[j];
+  mj[r] = uint32_t(j);
+}
+
+// number of i in [lo, hi) with (k[i], jj[i]) < (K, J) for lexicographically sorted pairs
+__device__ __forceinline__ int64_t lower_bound_pair(const uint32_t* k, const uint32_t* jj, int64_t lo, int64_t hi,
+                                                    uint32_t K, uint32_t J) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    const bool less = k[mid] < K || (k[mid] == K && jj[mid] < J);
+    if (less) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int64_t bound_key(const uint32_t* k, int64_t n, uint32_t K, bool upper) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    const bool go = upper ? k[mid] <= K : k[mid] < K;
+    if (go) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Sorted mover i (key K, previous position j) lands after every stayer with a
+// smaller key or an equal key and smaller previous position, and after the
+// movers before it.
+__global__ void k_place_movers(const uint32_t* __restrict__ mks, const uint32_t* __restrict__ mr,
+                               const uint32_t* __restrict__ mj, const uint32_t* __restrict__ skeys_old,
+                               const uint32_t* __restrict__ perm_old, const uint32_t* __restrict__ spos, int64_t n,
+                               int64_t M, uint32_t* __restrict__ mjs, uint32_t* __restrict__ skeys_new,
+                               uint32_t* __restrict__ perm_new) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const uint32_t K = mks[i];
+  const int64_t j = mj[mr[i]];
+  mjs[i] = uint32_t(j);
+  const int64_t lb = bound_key(skeys_old, n, K, false), ub = bound_key(skeys_old, n, K, true);
+  const int64_t c = j < lb ? lb : (j > ub ? ub : j);
+  const int64_t pos = i + int64_t(spos[c]);
+  skeys_new[pos] = K;
+  perm_new[pos] = perm_old[j];
+}
+
+// Stayer j moves back by the movers removed before it and forward by the
+// movers inserted before it (sorted movers with a pair (key, j') < (key, j)).
+constexpr int STAY_BLOCK = 256, STAY_CAP = 2048;
+__global__ __launch_bounds__(STAY_BLOCK) void k_place_stayers(
+    const uint32_t* __restrict__ skeys_old, const uint32_t* __restrict__ perm_old, const uint32_t* __restrict__ spos,
+    int64_t n, const uint32_t* __restrict__ mks, const uint32_t* __restrict__ mjs, int64_t M,
+    uint32_t* __restrict__ skeys_new, uint32_t* __restrict__ perm_new) {
+  __shared__ uint32_t s_k[STAY_CAP], s_j[STAY_CAP];
+  __shared__ int64_t s_range[2];
+  const int64_t j0 = int64_t(blockIdx.x) * STAY_BLOCK;
+  const int64_t j1 = j0 + STAY_BLOCK < n ? j0 + STAY_BLOCK : n;
+  if (threadIdx.x == 0) {
+    s_range[0] = lower_bound_pair(mks, mjs, 0, M, skeys_old[j0], uint32_t(j0));
+    s_range[1] = lower_bound_pair(mks, mjs, s_range[0], M, skeys_old[j1 - 1], uint32_t(j1));
+  }
+  __syncthreads();
+  const int64_t a0 = s_range[0], cnt = s_range[1] - s_range[0];
+  const bool in_lds = cnt <= STAY_CAP;
+  if (in_lds)
+    for (int64_t t = threadIdx.x; t < cnt; t += STAY_BLOCK) {
+      s_k[t] = mks[a0 + t];
+      s_j[t] = mjs[a0 + t];
+    }
+  __syncthreads();
+  const int64_t j = j0 + threadIdx.x;
+  if (j >= j1 || spos[j + 1] == spos[j]) return;  // past the end, or a mover
+  const uint32_t K = skeys_old[j];
+  const int64_t ins = in_lds ? a0 + lower_bound_pair(s_k, s_j, 0, cnt, K, uint32_t(j))
+                             : lower_bound_pair(mks, mjs, a0, a0 + cnt, K, uint32_t(j));
+  const int64_t pos = int64_t(spos[j]) + ins;
+  skeys_new[pos] = K;
+  perm_new[pos] = perm_old[j];
 }
 
 __global__ void k_tile_runs(const uint32_t* __restrict__ skeys, int64_t n, uint32_t ntiles,
@@ -1308,11 +1420,56 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
   uint32_t* nbslot = ctx->nb_slot.as<uint32_t>(ntiles);
   uint32_t* nblist = ctx->nb_list.as<uint32_t>(ntiles);
   const int tok = prof_begin(ctx, "binning", 0.0);
-  if (n > 0) {
+  const int kbits = key_bits_for(ntiles * 64);
+  const bool incremental = b.state_valid && b.state_n == n && b.state_ntiles == int64_t(ntiles) &&
+                           b.state_dim == D && n > 0;
+  if (n > 0 && incremental) {
+    // keys in the previous sorted order, stayer flags -> stayer ranks
+    uint32_t* nk = ctx->inc_nk.as<uint32_t>(size_t(n));
+    uint32_t* spos = ctx->inc_spos.as<uint32_t>(size_t(n) + 1);
+    hipLaunchKernelGGL((k_keys_sorted<T, D, BASIS, STORED>), dim3(blocks_for(n + 1, 256)), dim3(256), 0,
+                       ctx->stream, a, perm, skeys, uint32_t(ntiles), nk, spos, counters);
+    TESS_HIP(hipGetLastError());
+    exclusive_scan_u32(ctx, spos, n + 1, counters + CNT_STAY);
+    uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
+    TESS_HIP(hipMemcpyAsync(hc + CNT_STAY, counters + CNT_STAY, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    TESS_HIP(hipStreamSynchronize(ctx->stream));
+    const int64_t M = n - int64_t(hc[CNT_STAY]);
+    ctx->last_movers = M;
+    if (M > 0) {
+      uint32_t* skeys2 = ctx->skeys2.as<uint32_t>(size_t(n));
+      uint32_t* perm2 = ctx->perm2.as<uint32_t>(size_t(n));
+      if (M <= n / 8) {
+        uint32_t* mkey = ctx->inc_mkey.as<uint32_t>(size_t(M));
+        uint32_t* mj = ctx->inc_mj.as<uint32_t>(size_t(M));
+        uint32_t* mks = ctx->inc_mks.as<uint32_t>(size_t(M));
+        uint32_t* mr = ctx->inc_mr.as<uint32_t>(size_t(M));
+        uint32_t* mjs = ctx->inc_mjs.as<uint32_t>(size_t(M));
+        hipLaunchKernelGGL(k_compact_movers, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, nk, spos, n, mkey,
+                           mj);
+        TESS_HIP(hipGetLastError());
+        radix_sort_pairs(ctx, mkey, nullptr, mks, mr, M, kbits);
+        hipLaunchKernelGGL(k_place_movers, dim3(blocks_for(M, 256)), dim3(256), 0, ctx->stream, mks, mr, mj, skeys,
+                           perm, spos, n, M, mjs, skeys2, perm2);
+        TESS_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_place_stayers, dim3(blocks_for(n, STAY_BLOCK)), dim3(STAY_BLOCK), 0, ctx->stream,
+                           skeys, perm, spos, n, mks, mjs, M, skeys2, perm2);
+        TESS_HIP(hipGetLastError());
+      } else {
+        radix_sort_pairs(ctx, nk, perm, skeys2, perm2, n, kbits);
+      }
+      std::swap(ctx->keys_sorted, ctx->skeys2);
+      std::swap(ctx->vals, ctx->perm2);
+      skeys = static_cast<uint32_t*>(ctx->keys_sorted.ptr);
+      perm = static_cast<uint32_t*>(ctx->vals.ptr);
+    }
+  } else if (n > 0) {
     hipLaunchKernelGGL((k_bin_keys<T, D, BASIS, STORED>), dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, a,
                        keys, uint32_t(ntiles), counters);
     TESS_HIP(hipGetLastError());
-    radix_sort_pairs(ctx, keys, nullptr, skeys, perm, n, key_bits_for(ntiles * 64));
+    radix_sort_pairs(ctx, keys, nullptr, skeys, perm, n, kbits);
+    ctx->last_movers = -1;
   }
   TESS_HIP(hipMemsetAsync(tstart, 0, ntiles * sizeof(uint32_t), ctx->stream));
   TESS_HIP(hipMemsetAsync(tend, 0, ntiles * sizeof(uint32_t), ctx->stream));
@@ -1363,6 +1520,10 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
   b.ntiles = int64_t(ntiles);
   for (int d = 0; d < 3; ++d) b.ntiles_axis[d] = d < D ? a.ntiles[d] : 1;
   b.valid = true;
+  b.state_valid = true;
+  b.state_n = n;
+  b.state_ntiles = int64_t(ntiles);
+  b.state_dim = D;
 }
 
 // P2G after binning: zeroing of what this call does not overwrite, the tile

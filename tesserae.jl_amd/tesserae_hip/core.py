@@ -448,9 +448,9 @@ class Context:
             i += 1
 
     def stats(self) -> dict:
-        vals = (ctypes.c_int64 * 6)()
-        _check(self.lib, self.lib.tess_stats(self.handle, vals, 6))
-        keys = ("n_inside", "active_tiles", "node_blocks", "active_nodes", "reorders", "misplaced")
+        vals = (ctypes.c_int64 * 7)()
+        _check(self.lib, self.lib.tess_stats(self.handle, vals, 7))
+        keys = ("n_inside", "active_tiles", "node_blocks", "active_nodes", "reorders", "misplaced", "movers")
         return dict(zip(keys, list(vals)))
 
 

@@ -142,6 +142,25 @@ def test_p2g_deterministic_and_layouts(oracle):
         np.testing.assert_array_equal(outs[0][name], outs[2][name])  # layout does not change the sums
 
 
+def test_incremental_binning_is_deterministic():
+    """Two contexts replaying the same step history (incremental re-binning after the
+    first step, reorders included) give bitwise-identical particles."""
+    sc = block_drop(dx=1 / 48, steps=25)
+    sc.v[:] = np.random.default_rng(3).normal(scale=2.0, size=sc.v.shape)  # many particles change class
+    outs = []
+    for _ in range(2):
+        ctx = th.Context()
+        mesh, P, G = to_gpu(sc, dtype=torch.float32)
+        th.step(G, P, sc.basis, sc.params, nsteps=25, reorder_threshold=0.9, ctx=ctx)
+        ctx.sync()
+        st = ctx.stats()
+        assert st["movers"] > 0
+        outs.append(particles_np(P))
+        ctx.close()
+    for name in outs[0]:
+        np.testing.assert_array_equal(outs[0][name], outs[1][name])
+
+
 def test_two_disks_fp64_trajectory(oracle):
     """README two-disk collision, Float64: 200 GPU steps against 200 oracle steps."""
     sc = two_disks(steps=200)
