@@ -198,6 +198,52 @@ tess_status tess_step(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const
                       const tess_particles* particles, const tess_params* params, int32_t nsteps,
                       double reorder_threshold);
 
+/* ---- sparse grid (SpArray / SpGrid) ------------------------------------- */
+
+/* SpIndices of a CartesianMesh (src/Grids/spindices.jl:57-140): nodes grouped
+ * in blocks of 2^L per axis (L = mesh->block_size_log2), nblocks(mesh) =
+ * ((dims-1)>>L)+1 per axis.  blocknumbering holds one int32 per block in
+ * column-major block order: 0 = inactive, k = the k-th active block in linear
+ * order (src/Grids/sparsity.jl:53-60).  A grid field of an SpGrid is the
+ * compact SpArray data vector: node I of an active block lives at element
+ * (k-1)*2^(L*dim) + local, local = column-major index of I & (2^L-1) inside the
+ * block (spindices.jl:89,96-102).  storage_len = active blocks * 2^(L*dim);
+ * capacity = elements the caller's grid arrays hold. */
+typedef struct tess_sparsity {
+  int32_t* blocknumbering;
+  int64_t storage_len;
+  int64_t capacity;
+} tess_sparsity;
+
+/* update_sparsity!(spinds, xₚ, mesh) (src/Grids/sparsity.jl:23-40,197-238):
+ * blocks holding a particle (findblock, cartesian.jl:277-285) are occupied,
+ * every block within ±1 block of an occupied one is active, and the active
+ * blocks are numbered in linear order.  Writes sp->blocknumbering and
+ * sp->storage_len; *changed (host, may be NULL) = 1 if the active set differs
+ * from the numbering passed in (the reference returns `nothing` otherwise).
+ * Synchronises (one readback of the active-block count). */
+tess_status tess_sparsity_update(tess_ctx* ctx, const tess_mesh* mesh, tess_field x, int64_t n,
+                                 tess_sparsity* sp, int32_t* changed);
+
+/* tess_p2g / tess_g2p / tess_step on an SpGrid: the grid fields are SpArray
+ * data vectors addressed through sp (src/Transfer/runtime.jl:132-170,
+ * basis_weight.jl:270-278).  P2G zeroes the whole storage [0, storage_len)
+ * before writing (the `=` targets, p2g.jl:159) and fails with TESS_ERR_STATE
+ * when a particle's support reaches an inactive block (the reference's
+ * "inactive SpGrid support node" error, runtime.jl:155-161).  G2P reads
+ * inactive nodes as zero.  tess_step_sparse runs update_sparsity! before each
+ * step's P2G, as the SpGrid loop does (src/Grids/grid.jl:97-101), and fails with TESS_ERR_STATE when the
+ * new storage_len exceeds sp->capacity (the caller resizes and calls again). */
+tess_status tess_p2g_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                            const tess_sparsity* sp, const tess_particles* particles, const tess_weights* weights,
+                            const tess_params* params);
+tess_status tess_g2p_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                            const tess_sparsity* sp, const tess_particles* particles, const tess_weights* weights,
+                            const tess_params* params);
+tess_status tess_step_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                             tess_sparsity* sp, const tess_particles* particles, const tess_params* params,
+                             int32_t nsteps, double reorder_threshold);
+
 /* ---- measurement (no counterpart in the reference) ----------------------- */
 
 /* Per-kernel timing with HIP events recorded on the context's stream around

@@ -679,6 +679,55 @@ struct RunPartition {
     return TESS_OK;
   }
 };
+// update_sparsity!(spinds, xₚ, mesh) on the CPU (src/Grids/sparsity.jl:23-40):
+// per-particle block ids (:120-139), occupied blocks, the ±1-block
+// neighbourhood scattered from each occupied block (:141-155), then the
+// sequential numbering of active blocks (:53-60).  *changed compares the new
+// active set with the numbering passed in (:262-267).
+template <typename T, int D>
+struct RunSparsity {
+  static int run(const tess_mesh& m, tess_field x, int64_t n, int32_t* numbering, int64_t* storage_len,
+                 int32_t* changed) {
+    Mesh<T, D> mesh(m);
+    int64_t nb[D], nbt = 1;
+    for (int d = 0; d < D; ++d) { nb[d] = mesh.nblocks(d); nbt *= nb[d]; }
+    Fld<T> X(x);
+    std::vector<uint8_t> occupied(nbt, 0), active(nbt, 0);
+    for (int64_t p = 0; p < n; ++p) {
+      T xp[D];
+      load_vec<T, D>(X, p, xp);
+      const int64_t b = findblock_lin<T, D>(mesh, xp, nb);
+      if (b >= 0) occupied[b] = 1;
+    }
+    for (int64_t b = 0; b < nbt; ++b) {
+      if (!occupied[b]) continue;
+      int64_t I[D], r = b;
+      for (int d = 0; d < D; ++d) { I[d] = r % nb[d]; r /= nb[d]; }
+      int64_t lo[D], hi[D], ext[D], tot = 1;
+      for (int d = 0; d < D; ++d) {
+        lo[d] = std::max<int64_t>(I[d] - 1, 0);
+        hi[d] = std::min<int64_t>(I[d] + 1, nb[d] - 1);
+        ext[d] = hi[d] - lo[d] + 1;
+        tot *= ext[d];
+      }
+      for (int64_t k = 0; k < tot; ++k) {
+        int64_t q = k, lin = 0, mul = 1;
+        for (int d = 0; d < D; ++d) { lin += (lo[d] + q % ext[d]) * mul; q /= ext[d]; mul *= nb[d]; }
+        active[lin] = 1;
+      }
+    }
+    int32_t count = 0;
+    bool diff = false;
+    for (int64_t b = 0; b < nbt; ++b) {
+      diff = diff || ((numbering[b] != 0) != (active[b] != 0));
+      numbering[b] = active[b] ? ++count : 0;
+    }
+    *storage_len = int64_t(count) << (mesh.L * D);
+    if (changed) *changed = diff ? 1 : 0;
+    return TESS_OK;
+  }
+};
+
 template <typename T, int D>
 struct RunThreaded {
   static int run(const tess_mesh& m, int basis, const tess_grid* G, const tess_particles* P, const tess_params* prm,
@@ -778,6 +827,11 @@ int oref_update_weights(int basis, const tess_mesh* m, const tess_particles* P, 
 int oref_partition_update(const tess_mesh* m, tess_field x, int64_t n, int32_t* pidx, int32_t* starts,
                           int32_t* stops, int64_t* nassigned) {
   return dispatch<RunPartition>(m, x, n, pidx, starts, stops, nassigned);
+}
+
+int oref_sparsity_update(const tess_mesh* m, tess_field x, int64_t n, int32_t* numbering, int64_t* storage_len,
+                         int32_t* changed) {
+  return dispatch<RunSparsity>(m, x, n, numbering, storage_len, changed);
 }
 
 int oref_p2g(int basis, const tess_mesh* m, const tess_grid* G, const tess_particles* P, const tess_weights* W,

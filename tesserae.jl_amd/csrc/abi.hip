@@ -171,6 +171,22 @@ void check_params(const tess_params* p) {
   if (!std::isfinite(p->dt)) raise(TESS_ERR_ARGUMENT, "Δt must be finite");
 }
 
+// SpIndices over the mesh's blocks; transfers additionally need the storage
+// to hold the active blocks.
+void check_sparsity(const tess_mesh* m, const tess_sparsity* sp, bool transfer) {
+  if (!sp) raise(TESS_ERR_ARGUMENT, "sparsity is NULL");
+  if (!sp->blocknumbering) raise(TESS_ERR_ARGUMENT, "blocknumbering is NULL");
+  if (m->block_size_log2 < 1 || m->block_size_log2 * m->dim > 30)
+    raise(TESS_ERR_UNSUPPORTED, "SpGrid blocks need 1 <= block_size_log2 and 2^(block_size_log2*dim) <= 2^30");
+  if (transfer) {
+    if (sp->storage_len < 0 || sp->capacity < 0) raise(TESS_ERR_DIMENSION, "negative SpArray storage length");
+    if (sp->storage_len > sp->capacity)
+      raise(TESS_ERR_DIMENSION, "SpArray storage (" + std::to_string(sp->capacity) +
+                                    " nodes) is smaller than the active blocks (" + std::to_string(sp->storage_len) +
+                                    " nodes)");
+  }
+}
+
 }  // namespace
 }  // namespace tess
 
@@ -219,7 +235,8 @@ tess_status tess_ctx_destroy(tess_ctx* ctx) {
                             &ctx->partials,  &ctx->counters,    &ctx->host_pinned_dummy, &ctx->reorder_tmp,
                             &ctx->per_tile,  &ctx->per_nodeblock, &ctx->skeys2, &ctx->perm2,
                             &ctx->inc_nk,    &ctx->inc_spos,  &ctx->inc_mkey, &ctx->inc_mj,
-                            &ctx->inc_mks,   &ctx->inc_mr,    &ctx->inc_mjs};
+                            &ctx->inc_mks,   &ctx->inc_mr,    &ctx->inc_mjs,
+                            &ctx->sp_occ,    &ctx->sp_act,    &ctx->sp_scan};
     for (auto* b : bufs) b->release();
     for (auto ev : ctx->prof.pool) (void)hipEventDestroy(ev);
     if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
@@ -316,6 +333,68 @@ tess_status tess_step(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const
     if (!(reorder_threshold >= 0.0 && reorder_threshold <= 1.0))
       raise(TESS_ERR_ARGUMENT, "threshold must be in [0, 1].");
     run_step(ctx, basis, *mesh, *grid, *particles, *params, nsteps, reorder_threshold);
+  });
+}
+
+tess_status tess_sparsity_update(tess_ctx* ctx, const tess_mesh* mesh, tess_field x, int64_t n, tess_sparsity* sp,
+                                 int32_t* changed) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    check_sparsity(mesh, sp, false);
+    if (n < 0) raise(TESS_ERR_DIMENSION, "negative particle count");
+    if (!x.ptr && n > 0) raise(TESS_ERR_ARGUMENT, "missing field x");
+    run_sparsity_update(ctx, *mesh, x, n, *sp, changed);
+  });
+}
+
+tess_status tess_p2g_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                            const tess_sparsity* sp, const tess_particles* particles, const tess_weights* weights,
+                            const tess_params* params) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    check_basis(basis);
+    check_grid(grid);
+    check_sparsity(mesh, sp, true);
+    check_particles(particles, basis, true);
+    check_weights(weights);
+    check_params(params);
+    run_p2g_sparse(ctx, basis, *mesh, *grid, *sp, *particles, weights, *params);
+  });
+}
+
+tess_status tess_g2p_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                            const tess_sparsity* sp, const tess_particles* particles, const tess_weights* weights,
+                            const tess_params* params) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    check_basis(basis);
+    check_grid(grid);
+    check_sparsity(mesh, sp, true);
+    check_particles(particles, basis, true);
+    check_weights(weights);
+    check_params(params);
+    run_g2p_sparse(ctx, basis, *mesh, *grid, *sp, *particles, weights, *params);
+  });
+}
+
+tess_status tess_step_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                             tess_sparsity* sp, const tess_particles* particles, const tess_params* params,
+                             int32_t nsteps, double reorder_threshold) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    check_basis(basis);
+    check_grid(grid);
+    check_sparsity(mesh, sp, false);
+    check_particles(particles, basis, true);
+    check_params(params);
+    if (nsteps < 0) raise(TESS_ERR_ARGUMENT, "nsteps must be non-negative");
+    if (!(reorder_threshold >= 0.0 && reorder_threshold <= 1.0))
+      raise(TESS_ERR_ARGUMENT, "threshold must be in [0, 1].");
+    run_step_sparse(ctx, basis, *mesh, *grid, *sp, *particles, *params, nsteps, reorder_threshold);
   });
 }
 

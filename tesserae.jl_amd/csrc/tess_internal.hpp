@@ -107,6 +107,8 @@ struct tess_ctx {
   int64_t reorders = 0;
   // nodes with m > 0 written by the last gather (read back with the next binning)
   int64_t last_active_nodes = -1;
+  // SpGrid block activation scratch (occupied flags, active flags, scan)
+  tess::Buffer sp_occ, sp_act, sp_scan;
 };
 
 namespace tess {
@@ -130,6 +132,12 @@ void run_partition_update(tess_ctx* ctx, const tess_mesh& mesh, tess_field x, in
 void run_reorder(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, int32_t* pidx,
                  int64_t nassigned, double threshold, int32_t* reordered);
 
+// ---- SpGrid block numbering (sparse.hip) ----
+// update_sparsity!(spinds, xₚ, mesh): writes sp.blocknumbering and
+// sp.storage_len, *changed (host, optional) = active set changed.  Synchronises.
+void run_sparsity_update(tess_ctx* ctx, const tess_mesh& mesh, tess_field x, int64_t n, tess_sparsity& sp,
+                         int32_t* changed);
+
 // ---- transfers (transfer.hip) ----
 void run_p2g(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
              const tess_weights* W, const tess_params& prm);
@@ -137,6 +145,13 @@ void run_g2p(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G
              const tess_weights* W, const tess_params& prm);
 void run_step(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
               const tess_params& prm, int nsteps, double reorder_threshold);
+// The same on an SpGrid: grid fields are SpArray data vectors addressed through sp.
+void run_p2g_sparse(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_sparsity& sp,
+                    const tess_particles& P, const tess_weights* W, const tess_params& prm);
+void run_g2p_sparse(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_sparsity& sp,
+                    const tess_particles& P, const tess_weights* W, const tess_params& prm);
+void run_step_sparse(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, tess_sparsity& sp,
+                     const tess_particles& P, const tess_params& prm, int nsteps, double reorder_threshold);
 // Moves every particle field so that slot k receives particle order[k].
 void permute_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const uint32_t* order);
 

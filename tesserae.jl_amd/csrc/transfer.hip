@@ -56,9 +56,9 @@ struct Pow<N, 0> {
 // misplaced-particle count of the last P2G (zeroed per P2G)
 enum {
   CNT_INSIDE = 0, CNT_ACTIVE = 1, CNT_NB = 2, CNT_ERR = 3, CNT_BIN_SLOTS = 4,
-  CNT_MISPLACED = 8, CNT_NODES = 9, CNT_STAY = 10, CNT_SLOTS = 16
+  CNT_MISPLACED = 8, CNT_NODES = 9, CNT_STAY = 10, CNT_P2GERR = 11, CNT_SLOTS = 16
 };
-enum { ERR_GIMP_LENGTH = 1, ERR_WINDOW = 2 };
+enum { ERR_GIMP_LENGTH = 1, ERR_WINDOW = 2, ERR_INACTIVE = 4 };
 
 template <typename T, int D>
 struct Args {
@@ -76,6 +76,11 @@ struct Args {
   const uint32_t* nb_list;
   const uint32_t* nb_flag;
   T* partials;
+  // SpGrid: block numbering (nullptr for a dense grid), block_size_log2, blocks per axis
+  const int32_t* numbering;
+  int32_t spL;
+  int32_t nblk[D];
+  uint32_t* p2g_err;
   int64_t n, n_inside, n_active, n_nb;
   int32_t ntiles[D];
   T mp, dt;
@@ -99,6 +104,57 @@ __device__ __forceinline__ void unlin(int64_t l, const int32_t* dims, int* I) {
     I[d] = int(l % dims[d]);
     l /= dims[d];
   }
+}
+
+// Element of grid node I in the caller's grid arrays: the column-major node
+// index of a dense grid, or on an SpGrid the SpArray storage index
+// (k-1)*2^(L*dim) + local of its block's number k (src/Grids/spindices.jl:89,
+// 96-102, 133-140), -1 when the block is inactive.
+template <typename T, int D>
+__device__ __forceinline__ int64_t grid_slot(const Args<T, D>& a, const int* I) {
+  if (a.numbering == nullptr) return a.mesh.lin(I);
+  const int L = a.spL;
+  int64_t b = 0, loc = 0;
+#pragma unroll
+  for (int d = D - 1; d >= 0; --d) {
+    b = b * a.nblk[d] + (I[d] >> L);
+    loc = (loc << L) | (I[d] & ((1 << L) - 1));
+  }
+  const int32_t k = a.numbering[b];
+  return k > 0 ? (int64_t(k - 1) << (L * D)) + loc : int64_t(-1);
+}
+
+// SpGrid: every node of a class window [lo, lo+sw-1] (clipped to the mesh)
+// must lie in an active block, the reference's "inactive SpGrid support node"
+// check (src/Transfer/runtime.jl:155-161).  The class window contains the
+// window of every particle of the class.
+template <typename T, int D, int SW>
+__device__ __forceinline__ void check_sparse_support(const Args<T, D>& a, const int* origin, const int* ccoord,
+                                                     int ccount) {
+  if (a.numbering == nullptr || ccount == 0) return;
+  const int L = a.spL;
+  int blo[D], ext[D], tot = 1;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const int lo = max(origin[d] + ccoord[d], 0);  // a class holding particles has lo >= 0
+    const int hi = min(lo + SW - 1, int(a.mesh.dims[d]) - 1);
+    blo[d] = lo >> L;
+    ext[d] = (hi >> L) - blo[d] + 1;
+    tot *= ext[d];
+  }
+  bool bad = false;
+  for (int k = 0; k < tot; ++k) {
+    int r = k;
+    int64_t b = 0, mul = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      b += int64_t(blo[d] + r % ext[d]) * mul;
+      r /= ext[d];
+      mul *= a.nblk[d];
+    }
+    bad = bad || a.numbering[b] == 0;
+  }
+  if (bad) atomicOr(a.p2g_err, uint32_t(ERR_INACTIVE));
 }
 
 // Particle window (0-based, clamped) from its position (deferred weights) or
@@ -375,7 +431,8 @@ __global__ __launch_bounds__(64) void k_zero_stale(Args<T, D> a, const uint32_t*
     r /= G::BW;
     if (I[d] >= a.mesh.dims[d]) return;
   }
-  zero_node(a, a.mesh.lin(I));
+  const int64_t i = grid_slot(a, I);
+  if (i >= 0) zero_node(a, i);
 }
 
 template <typename T, int D, int BASIS>
@@ -471,6 +528,7 @@ __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a, const uint32_t* _
       r /= G::BW;
     }
   }
+  check_sparse_support<T, D, SW>(a, origin, ccoord, ccount);
   // particles whose storage index differs from the layered position
   // (k-th particle of every class contiguous, classes ascending)
   if (misplaced) {
@@ -681,6 +739,7 @@ __global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* _
       r /= G::BW;
     }
   }
+  check_sparse_support<T, D, SW>(a, origin, ccoord, ccount);
   const T mp = a.mp, hinv = a.mesh.h_inv;
   uint32_t bad = 0, layer_off = 0;
   for (int pass = 0; pass < NPASS; ++pass) {
@@ -867,7 +926,8 @@ __global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32
       r /= G::TS;
       in = in && I[d] >= 0 && I[d] < a.mesh.dims[d];
     }
-    const int64_t i = in ? a.mesh.lin(I) : 0;
+    const int64_t i = in ? grid_slot(a, I) : -1;
+    in = i >= 0;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
       const T vv = in ? a.gv(i, c) : T(0);
@@ -1083,7 +1143,7 @@ __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __re
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] += P[v];
       }
-  const int64_t i = a.mesh.lin(I);
+  const int64_t i = grid_slot(a, I);
   // grid-node half with the guarded Float64 division (test/transfer.jl:102-104)
   const double md = double(acc[0]);
   const bool has = md > 0.0;
@@ -1094,6 +1154,7 @@ __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __re
       nodes_per_block[j] = uint32_t(__popcll(m));
   }
   const double minv = has ? 1.0 / md : 0.0;
+  if (i < 0) return;  // SpGrid node of an inactive block: no storage (and no support, checked by P2G)
   a.gm(i) = acc[0];
   T mvo[D], fo[D], vno[D], vo[D];
 #pragma unroll
@@ -1202,7 +1263,8 @@ __global__ __launch_bounds__(64) void k_g2p_tile(Args<T, D> a, const uint32_t* _
       r /= G::TS;
       in = in && I[d] >= 0 && I[d] < a.mesh.dims[d];
     }
-    const int64_t i = in ? a.mesh.lin(I) : 0;
+    const int64_t i = in ? grid_slot(a, I) : -1;
+    in = i >= 0;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
       s_vn[c][q] = in ? a.gvn(i, c) : T(0);
@@ -1357,6 +1419,14 @@ Args<T, D> make_args(const tess_mesh& mesh, const tess_grid* G, const tess_parti
   }
   for (int d = 0; d < D; ++d) a.ntiles[d] = int32_t(((mesh.dims[d] - 1) >> Geo<D>::LT) + 1);
   return a;
+}
+
+template <typename T, int D>
+void set_sparse(Args<T, D>& a, const tess_mesh& mesh, const tess_sparsity* sp) {
+  if (!sp) return;
+  a.numbering = sp->blocknumbering;
+  a.spL = mesh.block_size_log2;
+  for (int d = 0; d < D; ++d) a.nblk[d] = int32_t(((mesh.dims[d] - 1) >> mesh.block_size_log2) + 1);
 }
 
 template <typename T, int D>
@@ -1531,7 +1601,7 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
 // particles not stored in the layered order (for the step driver's reorder).
 template <typename T, int D, int BASIS, bool STORED>
 void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-                const tess_weights* W, const tess_params& prm, bool count_misplaced) {
+                const tess_weights* W, const tess_params& prm, bool count_misplaced, const tess_sparsity* sp) {
   using Gm = Geo<D>;
   constexpr int NV = 1 + 2 * D;
   constexpr int SW = support_width(BASIS);
@@ -1540,13 +1610,24 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   constexpr int NPASS = (Pow<SW, D>::v * NV * int(sizeof(T)) / 4 <= 100) ? 1 : SW;
   Args<T, D> a = make_args<T, D>(mesh, &G, P, W, &prm);
   fill_binning_ptrs<T, D>(ctx, a);
+  set_sparse<T, D>(a, mesh, sp);
+  uint32_t* counters = static_cast<uint32_t*>(ctx->counters.ptr);
   int64_t nnodes = 1;
   for (int d = 0; d < D; ++d) nnodes *= mesh.dims[d];
   // zero what this call will not overwrite: everything on first use of a
-  // grid, else the node blocks that were written last time and are not now
+  // grid, else the node blocks that were written last time and are not now.
+  // An SpGrid's storage is zeroed whole (its numbering may have changed).
   const bool fresh = ctx->last_grid_m != G.m.ptr || ctx->prev_tile_log2 != Gm::LT ||
                      ctx->prev_ntiles != ctx->bin.ntiles || ctx->prev_nnodes != nnodes;
-  if (fresh) {
+  if (sp) {
+    a.p2g_err = counters + CNT_P2GERR;
+    TESS_HIP(hipMemsetAsync(a.p2g_err, 0, sizeof(uint32_t), ctx->stream));
+    if (sp->storage_len > 0) {
+      hipLaunchKernelGGL((k_zero_grid<T, D>), dim3(blocks_for(sp->storage_len, 256)), dim3(256), 0, ctx->stream, a,
+                         sp->storage_len);
+      TESS_HIP(hipGetLastError());
+    }
+  } else if (fresh) {
     hipLaunchKernelGGL((k_zero_grid<T, D>), dim3(blocks_for(nnodes, 256)), dim3(256), 0, ctx->stream, a, nnodes);
     TESS_HIP(hipGetLastError());
   } else if (ctx->prev_nodeblocks > 0) {
@@ -1555,7 +1636,6 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     TESS_HIP(hipGetLastError());
   }
   const Binning& b = ctx->bin;
-  uint32_t* counters = static_cast<uint32_t*>(ctx->counters.ptr);
   uint32_t* misplaced = nullptr;
   uint32_t* per_tile = ctx->per_tile.as<uint32_t>(size_t(std::max<int64_t>(b.n_active, 1)));
   uint32_t* per_nb = ctx->per_nodeblock.as<uint32_t>(size_t(std::max<int64_t>(b.n_nodeblocks, 1)));
@@ -1611,18 +1691,27 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     TESS_HIP(hipMemcpyAsync(prev, ctx->nb_list.ptr, size_t(b.n_nodeblocks) * sizeof(uint32_t),
                             hipMemcpyDeviceToDevice, ctx->stream));
   ctx->prev_nodeblocks = b.n_nodeblocks;
-  ctx->last_grid_m = G.m.ptr;
+  ctx->last_grid_m = sp ? nullptr : G.m.ptr;  // a dense grid after an SpGrid call starts from a full zero
   ctx->prev_tile_log2 = Gm::LT;
   ctx->prev_ntiles = b.ntiles;
   ctx->prev_nnodes = nnodes;
+  if (sp) {
+    uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
+    TESS_HIP(hipMemcpyAsync(hc + CNT_P2GERR, counters + CNT_P2GERR, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    TESS_HIP(hipStreamSynchronize(ctx->stream));
+    if (hc[CNT_P2GERR] & ERR_INACTIVE)
+      raise(TESS_ERR_STATE, "@P2G: inactive SpGrid support node. Call update_sparsity! before @P2G.");
+  }
 }
 
 template <typename T, int D, int BASIS, bool STORED>
 void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-                const tess_weights* W, const tess_params& prm) {
+                const tess_weights* W, const tess_params& prm, const tess_sparsity* sp) {
   const Binning& b = ctx->bin;
   Args<T, D> a = make_args<T, D>(mesh, &G, P, W, &prm);
   fill_binning_ptrs<T, D>(ctx, a);
+  set_sparse<T, D>(a, mesh, sp);
   if (b.n_active > 0) {
     // algorithmic bytes: x, v, F read and v, ∇v, x, F, σ, V written per
     // particle; vⁿ, v read per active node
@@ -1716,18 +1805,18 @@ void layered_reorder(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles&
 
 template <typename T, int D, int BASIS, bool STORED>
 void p2g_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-              const tess_weights* W, const tess_params& prm) {
+              const tess_weights* W, const tess_params& prm, const tess_sparsity* sp) {
   bin_particles<T, D, BASIS, STORED>(ctx, mesh, P, W);
-  p2g_launch<T, D, BASIS, STORED>(ctx, mesh, G, P, W, prm, false);
+  p2g_launch<T, D, BASIS, STORED>(ctx, mesh, G, P, W, prm, false, sp);
 }
 
 template <typename T, int D, int BASIS, bool STORED>
 void g2p_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-              const tess_weights* W, const tess_params& prm) {
+              const tess_weights* W, const tess_params& prm, const tess_sparsity* sp) {
   // a standalone G2P bins again: device addresses alone cannot prove that the
   // particle set is unchanged since the last P2G
   bin_particles<T, D, BASIS, STORED>(ctx, mesh, P, W);
-  g2p_launch<T, D, BASIS, STORED>(ctx, mesh, G, P, W, prm);
+  g2p_launch<T, D, BASIS, STORED>(ctx, mesh, G, P, W, prm, sp);
 }
 
 // nsteps of the README loop body (deferred weights).  With reorder_threshold
@@ -1737,8 +1826,15 @@ void g2p_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const te
 // reorder_particles!(particles, partition; threshold) in the reference loop.
 template <typename T, int D, int BASIS, bool STORED>
 void step_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-               const tess_params& prm, int nsteps, double reorder_threshold) {
+               const tess_params& prm, int nsteps, double reorder_threshold, tess_sparsity* sp) {
   for (int s = 0; s < nsteps; ++s) {
+    if (sp) {
+      // update_sparsity!(grid, xₚ) at the top of the SpGrid loop (src/Grids/grid.jl:97-101)
+      run_sparsity_update(ctx, mesh, P.x, P.n, *sp, nullptr);
+      if (sp->storage_len > sp->capacity)
+        raise(TESS_ERR_STATE, "SpGrid storage holds " + std::to_string(sp->capacity) + " nodes but the active blocks need " +
+                                  std::to_string(sp->storage_len) + "; resize the grid arrays and call again");
+    }
     bin_particles<T, D, BASIS, false>(ctx, mesh, P, nullptr);
     const Binning& b = ctx->bin;
     if (reorder_threshold > 0.0 && b.n_inside > 0) {
@@ -1752,8 +1848,8 @@ void step_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const t
         ctx->reorders += 1;
       }
     }
-    p2g_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, reorder_threshold > 0.0);
-    g2p_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm);
+    p2g_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, reorder_threshold > 0.0, sp);
+    g2p_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, sp);
   }
 }
 
@@ -1787,22 +1883,22 @@ void dispatch(int basis, const tess_mesh& mesh, bool stored, Args_&&... args) {
 template <typename T, int D, int B, bool S>
 struct P2GOp {
   static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
-                  const tess_weights* W, const tess_params& prm) {
-    p2g_impl<T, D, B, S>(ctx, m, G, P, W, prm);
+                  const tess_weights* W, const tess_params& prm, const tess_sparsity* sp) {
+    p2g_impl<T, D, B, S>(ctx, m, G, P, W, prm, sp);
   }
 };
 template <typename T, int D, int B, bool S>
 struct G2POp {
   static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
-                  const tess_weights* W, const tess_params& prm) {
-    g2p_impl<T, D, B, S>(ctx, m, G, P, W, prm);
+                  const tess_weights* W, const tess_params& prm, const tess_sparsity* sp) {
+    g2p_impl<T, D, B, S>(ctx, m, G, P, W, prm, sp);
   }
 };
 template <typename T, int D, int B, bool S>
 struct StepOp {
   static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
-                  const tess_params& prm, int nsteps, double thr) {
-    if constexpr (!S) step_impl<T, D, B, false>(ctx, m, G, P, prm, nsteps, thr);
+                  const tess_params& prm, int nsteps, double thr, tess_sparsity* sp) {
+    if constexpr (!S) step_impl<T, D, B, false>(ctx, m, G, P, prm, nsteps, thr, sp);
   }
 };
 
@@ -1810,17 +1906,33 @@ struct StepOp {
 
 void run_p2g(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
              const tess_weights* W, const tess_params& prm) {
-  dispatch<P2GOp>(basis, mesh, W != nullptr, ctx, mesh, G, P, W, prm);
+  dispatch<P2GOp>(basis, mesh, W != nullptr, ctx, mesh, G, P, W, prm, static_cast<const tess_sparsity*>(nullptr));
 }
 
 void run_g2p(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
              const tess_weights* W, const tess_params& prm) {
-  dispatch<G2POp>(basis, mesh, W != nullptr, ctx, mesh, G, P, W, prm);
+  dispatch<G2POp>(basis, mesh, W != nullptr, ctx, mesh, G, P, W, prm, static_cast<const tess_sparsity*>(nullptr));
 }
 
 void run_step(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
               const tess_params& prm, int nsteps, double reorder_threshold) {
-  dispatch<StepOp>(basis, mesh, false, ctx, mesh, G, P, prm, nsteps, reorder_threshold);
+  dispatch<StepOp>(basis, mesh, false, ctx, mesh, G, P, prm, nsteps, reorder_threshold,
+                   static_cast<tess_sparsity*>(nullptr));
+}
+
+void run_p2g_sparse(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_sparsity& sp,
+                    const tess_particles& P, const tess_weights* W, const tess_params& prm) {
+  dispatch<P2GOp>(basis, mesh, W != nullptr, ctx, mesh, G, P, W, prm, &sp);
+}
+
+void run_g2p_sparse(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_sparsity& sp,
+                    const tess_particles& P, const tess_weights* W, const tess_params& prm) {
+  dispatch<G2POp>(basis, mesh, W != nullptr, ctx, mesh, G, P, W, prm, &sp);
+}
+
+void run_step_sparse(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, tess_sparsity& sp,
+                     const tess_particles& P, const tess_params& prm, int nsteps, double reorder_threshold) {
+  dispatch<StepOp>(basis, mesh, false, ctx, mesh, G, P, prm, nsteps, reorder_threshold, &sp);
 }
 
 }  // namespace tess

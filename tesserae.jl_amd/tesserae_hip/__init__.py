@@ -6,15 +6,17 @@ shared library is loaded on first use and its absence is an error.
 """
 from .core import (ArgumentError, Basis, BasisWeights, BSpline, CartesianMesh, Context, Cubic, DeviceError,
                    DimensionMismatch, ErrorException, G2P, Grid, Linear, MPMParams, P2G, Particles, Partition,
-                   Quadratic, TesseraeAssertionError, TesseraeError, UnsupportedError, default_context,
-                   generate_basis_weights, generate_grid, generate_particles, grid_sampling_points,
-                   reorder_particles, step, uGIMP, update)
+                   Quadratic, SpArray, SpGrid, SpIndices, TesseraeAssertionError, TesseraeError,
+                   UnsupportedError, default_context, generate_basis_weights, generate_grid, generate_particles,
+                   grid_sampling_points, reorder_particles, sparse_storage_index, step, uGIMP, update,
+                   update_sparsity)
 from ._abi import LIB_PATH, load_library
 
 __all__ = [
     "ArgumentError", "Basis", "BasisWeights", "BSpline", "CartesianMesh", "Context", "Cubic", "DeviceError",
     "DimensionMismatch", "ErrorException", "G2P", "Grid", "Linear", "MPMParams", "P2G", "Particles", "Partition",
-    "Quadratic", "TesseraeAssertionError", "TesseraeError", "UnsupportedError", "default_context",
+    "Quadratic", "SpArray", "SpGrid", "SpIndices", "sparse_storage_index", "update_sparsity",
+    "TesseraeAssertionError", "TesseraeError", "UnsupportedError", "default_context",
     "generate_basis_weights", "generate_grid", "generate_particles", "grid_sampling_points", "reorder_particles",
     "step", "uGIMP", "update", "LIB_PATH", "load_library",
 ]

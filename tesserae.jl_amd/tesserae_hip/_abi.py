@@ -81,6 +81,14 @@ class tess_weights(ctypes.Structure):
     ]
 
 
+class tess_sparsity(ctypes.Structure):
+    _fields_ = [
+        ("blocknumbering", ctypes.c_void_p),
+        ("storage_len", ctypes.c_int64),
+        ("capacity", ctypes.c_int64),
+    ]
+
+
 class tess_params(ctypes.Structure):
     _fields_ = [
         ("dt", ctypes.c_double),
@@ -130,6 +138,25 @@ SIGNATURES = {
         ctypes.c_int32,
         [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_particles), _P(tess_params),
          ctypes.c_int32, ctypes.c_double],
+    ),
+    "tess_sparsity_update": (
+        ctypes.c_int32,
+        [ctypes.c_void_p, _P(tess_mesh), tess_field, ctypes.c_int64, _P(tess_sparsity), _P(ctypes.c_int32)],
+    ),
+    "tess_p2g_sparse": (
+        ctypes.c_int32,
+        [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_sparsity), _P(tess_particles),
+         _P(tess_weights), _P(tess_params)],
+    ),
+    "tess_g2p_sparse": (
+        ctypes.c_int32,
+        [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_sparsity), _P(tess_particles),
+         _P(tess_weights), _P(tess_params)],
+    ),
+    "tess_step_sparse": (
+        ctypes.c_int32,
+        [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_sparsity), _P(tess_particles),
+         _P(tess_params), ctypes.c_int32, ctypes.c_double],
     ),
     "tess_profile_enable": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "tess_profile_read": (

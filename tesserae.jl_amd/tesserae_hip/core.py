@@ -293,9 +293,99 @@ class Grid:
         return G
 
 
-def generate_grid(mesh: CartesianMesh, device="cuda", layout: str = "julia") -> Grid:
-    """generate_grid(GridProp, mesh) for the README grid schema (src/Grids/grid.jl:70-86); fields zeroed."""
-    return Grid(mesh, device=device, layout=layout)
+class SpIndices:
+    """SpIndices(mesh) (src/Grids/spindices.jl:57-70): one int32 block number per
+    block of 2^L nodes per axis, column-major; 0 = inactive, k = k-th active block."""
+
+    def __init__(self, mesh: CartesianMesh, device="cuda"):
+        self.mesh = mesh
+        self.blocknumbering = torch.zeros((int(np.prod(mesh.nblocks())),), dtype=torch.int32, device=device)
+        self.storage_len = 0
+
+    def blocklength(self) -> int:
+        return 1 << (self.mesh.block_size_log2 * self.mesh.dim)
+
+    def numbering(self) -> np.ndarray:
+        """blocknumbering as a host array of shape nblocks (Julia column-major)."""
+        return self.blocknumbering.cpu().numpy().reshape(self.mesh.nblocks(), order="F")
+
+
+class SpGrid(Grid):
+    """generate_grid(SpArray, GridProp, mesh) (src/Grids/grid.jl:79-101): every field
+    is an SpArray data vector over the active blocks of ``spinds``."""
+
+    def __init__(self, mesh: CartesianMesh, device="cuda", layout: str = "julia"):
+        if layout != "julia":
+            raise ArgumentError("SpGrid storage uses the Julia layout")
+        self.mesh, self.layout, self.device = mesh, layout, device
+        self.spinds = SpIndices(mesh, device)
+        self.tensors = {}
+        self._allocate(0)
+
+    def _allocate(self, nodes: int) -> None:
+        d = self.mesh.dim
+        for name in self.FIELDS:
+            shape = (nodes,) if name == "m" else (nodes, d)
+            self.tensors[name] = torch.zeros(shape, dtype=self.mesh.dtype, device=self.device)
+        self.capacity = nodes
+
+    def resize_fillzero(self, nodes: int) -> None:
+        """resize_fillzero_data! of every field (src/Grids/sparray.jl:147-150): grow when
+        needed, then zero."""
+        if nodes > self.capacity:
+            self._allocate(nodes)
+        else:
+            for t in self.tensors.values():
+                t.zero_()
+
+    def sparsity_abi(self) -> _abi.tess_sparsity:
+        sp = _abi.tess_sparsity()
+        sp.blocknumbering = self.spinds.blocknumbering.data_ptr()
+        sp.storage_len = self.spinds.storage_len
+        sp.capacity = self.capacity
+        return sp
+
+    def to_dense(self, name: str) -> np.ndarray:
+        """Field over all mesh nodes (inactive nodes read as zero, as SpArray getindex does)."""
+        nb = self.mesh.nblocks()
+        num = self.spinds.numbering()
+        L, d = self.mesh.block_size_log2, self.mesh.dim
+        data = self.tensors[name].detach().double().cpu().numpy()
+        ncomp = 1 if name == "m" else d
+        out = np.zeros((self.mesh.nnodes(), ncomp))
+        idx = sparse_storage_index(self.mesh, num)
+        ok = idx >= 0
+        out[ok] = data.reshape(-1, ncomp)[idx[ok]]
+        return out[:, 0] if ncomp == 1 else out
+
+
+def sparse_storage_index(mesh: CartesianMesh, numbering: np.ndarray) -> np.ndarray:
+    """0-based SpArray storage index of every mesh node (x fastest), -1 if inactive:
+    (k-1)*2^(L*dim) + column-major local index (src/Grids/spindices.jl:89,96-102)."""
+    L, d = mesh.block_size_log2, mesh.dim
+    grids = np.meshgrid(*[np.arange(n) for n in mesh.dims], indexing="ij")
+    I = [g.reshape(-1, order="F") for g in grids]
+    blk = [i >> L for i in I]
+    loc = [i & ((1 << L) - 1) for i in I]
+    k = numbering[tuple(blk)]
+    lin_local = np.zeros_like(loc[0])
+    for a in reversed(range(d)):
+        lin_local = (lin_local << L) | loc[a]
+    return np.where(k > 0, (k.astype(np.int64) - 1) * (1 << (L * d)) + lin_local, -1)
+
+
+def generate_grid(mesh_or_kind, mesh: CartesianMesh | None = None, device="cuda", layout: str = "julia"):
+    """generate_grid(GridProp, mesh) / generate_grid(SpArray, GridProp, mesh) for the
+    README grid schema (src/Grids/grid.jl:70-86); fields zeroed."""
+    if isinstance(mesh_or_kind, CartesianMesh):
+        return Grid(mesh_or_kind, device=device, layout=layout)
+    if mesh_or_kind is SpArray:
+        return SpGrid(mesh, device=device, layout=layout)
+    raise ArgumentError("generate_grid: expected a mesh or (SpArray, mesh)")
+
+
+class SpArray:
+    """Marker for generate_grid(SpArray, mesh)."""
 
 
 def grid_sampling_points(mesh: CartesianMesh, spacing: float = 0.5) -> np.ndarray:
@@ -473,6 +563,25 @@ def _require_device(*tensors) -> None:
             raise ArgumentError("libtesserae_hip runs on the GPU: move grid, particles and weights to 'cuda'")
 
 
+def update_sparsity(grid: SpGrid, x: torch.Tensor, ctx: Context | None = None):
+    """update_sparsity!(grid, xₚ) (src/Grids/grid.jl:97-101, sparsity.jl:23-40): activates the
+    blocks within ±1 block of every occupied block, renumbers them, resizes and zeroes
+    the SpArray storage.  Returns the storage length when the active set changed, else
+    None (the reference's ``nothing``)."""
+    if not isinstance(grid, SpGrid):
+        raise ArgumentError("update_sparsity: expected an SpGrid")
+    _require_device(x)
+    ctx = ctx or default_context()
+    sp = grid.sparsity_abi()
+    changed = ctypes.c_int32(0)
+    f = _field(x, grid.mesh.dim, "julia")
+    _check(ctx.lib, ctx.lib.tess_sparsity_update(ctx.handle, ctypes.byref(grid.mesh.abi()), f, x.shape[0],
+                                                 ctypes.byref(sp), ctypes.byref(changed)))
+    grid.spinds.storage_len = int(sp.storage_len)
+    grid.resize_fillzero(grid.spinds.storage_len)
+    return grid.spinds.storage_len if changed.value else None
+
+
 def update(obj, *args, ctx: Context | None = None, **kwargs):
     """Julia's ``update!``: ``update(weights, particles, mesh)`` or ``update(partition, x)``."""
     if isinstance(obj, BasisWeights):
@@ -533,6 +642,11 @@ def P2G(grid: Grid, particles: Particles, weights, params: MPMParams, ctx=None) 
     _require_device(particles.x, grid.m)
     ctx = ctx or default_context()
     P, G = particles.abi(), grid.abi()
+    if isinstance(grid, SpGrid):
+        _check(ctx.lib, ctx.lib.tess_p2g_sparse(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
+                                                ctypes.byref(grid.sparsity_abi()), ctypes.byref(P), Wp,
+                                                ctypes.byref(params.abi())))
+        return
     _check(ctx.lib, ctx.lib.tess_p2g(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
                                      ctypes.byref(P), Wp, ctypes.byref(params.abi())))
 
@@ -543,6 +657,11 @@ def G2P(grid: Grid, particles: Particles, weights, params: MPMParams, ctx=None) 
     _require_device(particles.x, grid.m)
     ctx = ctx or default_context()
     P, G = particles.abi(), grid.abi()
+    if isinstance(grid, SpGrid):
+        _check(ctx.lib, ctx.lib.tess_g2p_sparse(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
+                                                ctypes.byref(grid.sparsity_abi()), ctypes.byref(P), Wp,
+                                                ctypes.byref(params.abi())))
+        return
     _check(ctx.lib, ctx.lib.tess_g2p(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
                                      ctypes.byref(P), Wp, ctypes.byref(params.abi())))
 
@@ -570,7 +689,24 @@ def step(grid: Grid, particles: Particles, basis: Basis, params: MPMParams, nste
     below it (0 never, 1 every step)."""
     _require_device(particles.x, grid.m)
     ctx = ctx or default_context()
-    P, G = particles.abi(), grid.abi()
+    P = particles.abi()
+    if isinstance(grid, SpGrid):
+        # update_sparsity!(grid, x) runs inside every step; the storage grows when the
+        # active blocks outgrow it and the step is retried (nothing of it has run yet)
+        s = 0
+        while s < nsteps:
+            sp, G = grid.sparsity_abi(), grid.abi()
+            st = ctx.lib.tess_step_sparse(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
+                                          ctypes.byref(sp), ctypes.byref(P), ctypes.byref(params.abi()), 1,
+                                          float(reorder_threshold))
+            grid.spinds.storage_len = int(sp.storage_len)
+            if st == _abi.TESS_ERR_STATE and sp.storage_len > grid.capacity:
+                grid.resize_fillzero(int(sp.storage_len * 1.25) + 1)
+                continue
+            _check(ctx.lib, st)
+            s += 1
+        return
+    G = grid.abi()
     _check(ctx.lib, ctx.lib.tess_step(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
                                       ctypes.byref(P), ctypes.byref(params.abi()), int(nsteps),
                                       float(reorder_threshold)))

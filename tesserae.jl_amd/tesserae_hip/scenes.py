@@ -110,6 +110,38 @@ def block_drop(dtype=torch.float32, dx: float = 1 / 256, seed: int = 1, steps: i
                  **_state(x, np.zeros_like(x), V0, 3))
 
 
+def sphere_spin(basis: Basis, dtype=torch.float32, dx: float = 1 / 512, seed: int = 7, steps: int = 50,
+                block_size_log2: int = 3, radius: float = 0.3) -> Scene:
+    """3D sparse-grid sphere: 2x2x2 particles per cell of [0,1]^3 (spacing dx/2) inside the
+    sphere of the given radius at the centre, v = 0.5*(rigid rotation about z at ω=2) +
+    0.05*N(0,1) per component, Neo-Hookean E=5e5, ν=0.35, ρ⁰=1e3, Δt=5e-5; SpArray blocks
+    of 2^block_size_log2 nodes; l = dx/2 for uGIMP.  Positions are generated one z-plane
+    of samples at a time."""
+    mesh = CartesianMesh(dx, (0, 1), (0, 1), (0, 1), dtype=dtype, block_size_log2=block_size_log2)
+    s = dx / 2
+    c = 0.5
+    k0 = int(math.floor((c - radius) / s - 0.5))
+    k1 = int(math.ceil((c + radius) / s - 0.5))
+    ax = (np.arange(k0, k1 + 1) + 0.5) * s
+    X, Y = np.meshgrid(ax, ax, indexing="ij")
+    X, Y = X.reshape(-1, order="F"), Y.reshape(-1, order="F")
+    r2xy = (X - c) ** 2 + (Y - c) ** 2
+    parts = []
+    for z in ax:
+        keep = r2xy + (z - c) ** 2 < radius * radius
+        if keep.any():
+            parts.append(np.stack([X[keep], Y[keep], np.full(int(keep.sum()), z)], axis=1))
+    x = np.concatenate(parts) if parts else np.zeros((0, 3))
+    n = len(x)
+    omega = 2.0
+    v = np.stack([-(x[:, 1] - c) * omega, (x[:, 0] - c) * omega, np.zeros(n)], axis=1) * 0.5
+    v = v + 0.05 * normal(seed, 3 * n).reshape(n, 3)
+    V0 = s ** 3
+    params = MPMParams(dt=5e-5, E=5e5, nu=0.35, rho0=1e3, V0=V0)
+    l = np.full(n, dx / 2) if basis.kind == 16 else None
+    return Scene("sphere_spin", mesh, basis, params, steps=steps, l=l, **_state(x, v, V0, 3))
+
+
 def random_box(dim: int, n: int, seed: int, basis: Basis, dtype=torch.float64, h: float = 0.1,
                extent: float = 1.0, vscale: float = 0.3, lo: float = 0.0, hi: float = 1.0,
                stress: float = 0.0, with_l: bool = False) -> Scene:

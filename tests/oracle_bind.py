@@ -123,6 +123,8 @@ class Oracle:
             "oref_update_weights": (ctypes.c_int, [ctypes.c_int, M, PP, W]),
             "oref_partition_update": (ctypes.c_int, [M, _abi.tess_field, _c64, ctypes.c_void_p, ctypes.c_void_p,
                                                      ctypes.c_void_p, _P(_c64)]),
+            "oref_sparsity_update": (ctypes.c_int, [M, _abi.tess_field, _c64, ctypes.c_void_p, _P(_c64),
+                                                    _P(ctypes.c_int32)]),
             "oref_p2g": (ctypes.c_int, [ctypes.c_int, M, G, PP, W, PR]),
             "oref_g2p": (ctypes.c_int, [ctypes.c_int, M, G, PP, W, PR]),
             "oref_step": (ctypes.c_int, [ctypes.c_int, M, G, PP, PR, ctypes.c_int]),
@@ -188,6 +190,18 @@ class Oracle:
         self._ok(self.lib.oref_partition_update(ctypes.byref(m), _field(x, mesh.dim), n, _np_ptr(pidx),
                                                 _np_ptr(starts), _np_ptr(stops), ctypes.byref(na)))
         return pidx[:na.value].copy(), starts, stops, na.value
+
+    def sparsity(self, mesh, x, numbering=None, dtype=np.float64):
+        """update_sparsity!(spinds, x, mesh): (numbering int32 (nblocks, column-major), storage_len, changed)."""
+        x = np.ascontiguousarray(x, dtype=dtype).reshape(-1, mesh.dim)
+        nb = int(np.prod(mesh.nblocks()))
+        num = np.zeros(nb, np.int32) if numbering is None else np.ascontiguousarray(numbering, np.int32).copy()
+        sl, ch = _c64(0), ctypes.c_int32(0)
+        m = mesh.abi()
+        m.dtype = _abi.TESS_FLOAT32 if dtype == np.float32 else _abi.TESS_FLOAT64
+        self._ok(self.lib.oref_sparsity_update(ctypes.byref(m), _field(x, mesh.dim), len(x), _np_ptr(num),
+                                               ctypes.byref(sl), ctypes.byref(ch)))
+        return num, sl.value, bool(ch.value)
 
     def p2g(self, st: HostState, basis, params, weights: HostWeights | None = None):
         P, G = st.particles_abi(), st.grid_abi()

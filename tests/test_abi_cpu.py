@@ -44,6 +44,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_abi.tess_grid) == 5 * 24
     assert ctypes.sizeof(_abi.tess_weights) == 2 * 24 + 8
     assert ctypes.sizeof(_abi.tess_params) == 5 * 8 + 3 * 8 + 8
+    assert ctypes.sizeof(_abi.tess_sparsity) == 3 * 8
     assert _abi.tess_mesh.dims.offset == 16 and _abi.tess_mesh.h.offset == 64
 
 
