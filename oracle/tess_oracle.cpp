@@ -616,6 +616,239 @@ int step_threaded_impl(int basis, const tess_mesh& tm, const tess_grid& G, const
   return TESS_OK;
 }
 
+// ---- adjoint of one README step (no reference counterpart) ------------------
+//
+// The engine's differentiation extension: the reverse of one step with
+// deferred weights and per-particle E = E0*exp(logE).  The state of a step is
+// (x, v, F) with V = V0 J and σ = (μ(FFᵀ-I) + λ lnJ I)/J functions of F, so
+// P = V σ = V0 (μ(FFᵀ-I) + λ lnJ I).  Given the adjoint (x̄', v̄', F̄') of the
+// state after the step, this returns the adjoint before it and adds ∂/∂logE.
+// Written sequentially over a dense grid with textbook B-spline jets; its
+// finite-difference check lives in tests/test_oracle_adjoint.py.
+
+// Centred B-spline of the basis at node-relative coordinate r = ξ - i, with
+// its first and second derivatives in r.
+inline void bspline_jet2(int basis, double r, double& N, double& dN, double& ddN) {
+  const double a = std::fabs(r), sg = std::signbit(r) ? -1.0 : 1.0;
+  N = dN = ddN = 0.0;
+  if (basis == TESS_BSPLINE_LINEAR) {
+    if (a < 1.0) { N = 1.0 - a; dN = -sg; }
+  } else if (basis == TESS_BSPLINE_QUADRATIC) {
+    if (a < 0.5) { N = 0.75 - r * r; dN = -2.0 * r; ddN = -2.0; }
+    else if (a < 1.5) { const double u = 1.5 - a; N = 0.5 * u * u; dN = -sg * u; ddN = 1.0; }
+  } else {  // cubic
+    if (a < 1.0) { N = 0.5 * a * a * a - r * r + 2.0 / 3.0; dN = 1.5 * sg * r * r - 2.0 * r; ddN = 3.0 * a - 2.0; }
+    else if (a < 2.0) { const double u = 2.0 - a; N = u * u * u / 6.0; dN = -sg * 0.5 * u * u; ddN = u; }
+  }
+}
+
+// w, ∇w, ∇∇w (in x) of the nodes of particle p's window, column-major.
+template <typename T, int D>
+struct Jets2 {
+  int64_t lo[D], ext[D], count;
+  bool empty;
+  double N[D][4], dN[D][4], ddN[D][4];
+  template <typename Fn>
+  void each(const Mesh<T, D>& mesh, Fn&& f) const {
+    for (int64_t k = 0; k < count; ++k) {
+      int64_t a[D], I[D], r = k;
+      for (int d = 0; d < D; ++d) { a[d] = r % ext[d]; r /= ext[d]; I[d] = lo[d] + a[d]; }
+      double w = 1.0, g[D], H[D][D];
+      for (int d = 0; d < D; ++d) w *= N[d][a[d]];
+      for (int c = 0; c < D; ++c) {
+        g[c] = 1.0;
+        for (int d = 0; d < D; ++d) g[c] *= (d == c) ? dN[d][a[d]] : N[d][a[d]];
+        g[c] *= 1.0 / mesh.h_d;
+        for (int e = 0; e < D; ++e) {
+          double h = 1.0;
+          for (int d = 0; d < D; ++d) {
+            if (c == e) h *= (d == c) ? ddN[d][a[d]] : N[d][a[d]];
+            else h *= (d == c || d == e) ? dN[d][a[d]] : N[d][a[d]];
+          }
+          H[c][e] = h / (mesh.h_d * mesh.h_d);
+        }
+      }
+      f(mesh.lin(I), w, g, H);
+    }
+  }
+};
+
+template <typename T, int D>
+void particle_jets2(int basis, const Mesh<T, D>& mesh, const T* x, Jets2<T, D>& J) {
+  PW<T, D> pw;
+  support_window<T, D>(basis, x, T(0), mesh, pw);
+  J.empty = pw.empty;
+  J.count = 0;
+  if (pw.empty) return;
+  J.count = 1;
+  for (int d = 0; d < D; ++d) {
+    J.lo[d] = pw.lo[d];
+    J.ext[d] = pw.hi[d] - pw.lo[d] + 1;
+    J.count *= J.ext[d];
+    for (int64_t a = 0; a < J.ext[d]; ++a) {
+      const double r = (double(x[d]) - (mesh.xmin_d[d] + double(pw.lo[d] + a) * mesh.h_d)) / mesh.h_d;
+      bspline_jet2(basis, r, J.N[d][a], J.dN[d][a], J.ddN[d][a]);
+    }
+  }
+}
+
+template <typename T, int D>
+void inverse_d(const double* F, double* Fi) {  // column-major
+  if (D == 2) {
+    const double det = F[0] * F[3] - F[2] * F[1];
+    Fi[0] = F[3] / det; Fi[1] = -F[1] / det; Fi[2] = -F[2] / det; Fi[3] = F[0] / det;
+    return;
+  }
+  auto f = [&](int r, int c) { return F[r + 3 * c]; };
+  double cof[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      cof[r + 3 * c] = f(r1, c1) * f(r2, c2) - f(r1, c2) * f(r2, c1);
+    }
+  const double det = f(0, 0) * cof[0] + f(0, 1) * cof[3] + f(0, 2) * cof[6];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) Fi[r + 3 * c] = cof[c + 3 * r] / det;  // adjugate = cofactorᵀ
+}
+
+template <typename T, int D>
+int step_adjoint_impl(int basis, const tess_mesh& tm, const tess_grid& G, const tess_particles& P,
+                      const tess_params& prm, double* ax, double* av, double* aF, double* ath) {
+  if (basis == TESS_UGIMP) return fail(TESS_ERR_UNSUPPORTED, "adjoint: B-spline bases only");
+  Mesh<T, D> mesh(tm);
+  p2g_seq<T, D>(basis, tm, G, P, nullptr, prm);  // the forward grid of this step
+  Fld<T> X(P.x), v(P.v), Ff(P.F), V(P.V), sg(P.sigma), LE(P.logE);
+  Fld<T> gm(G.m), gmv(G.mv), gf(G.f), gvn(G.vn), gv(G.v);
+  const int64_t nn = mesh.nnodes(), n = P.n;
+  const double dt = double(T(prm.dt)), mp = double(T(prm.mp));
+  std::vector<double> dvb(nn * D, 0.0), vgb(nn * D, 0.0);
+  // particle adjoints scattered to the nodes: Δv̄_i = Σ w v̄', v̄_i = Σ (L̄ ∇w + Δt w x̄')
+  for (int64_t p = 0; p < n; ++p) {
+    T x[D];
+    load_vec<T, D>(X, p, x);
+    Jets2<T, D> J;
+    particle_jets2<T, D>(basis, mesh, x, J);
+    if (J.empty) continue;
+    double Lb[D * D];
+    for (int r = 0; r < D; ++r)
+      for (int c = 0; c < D; ++c) {
+        double acc = 0.0;
+        for (int k = 0; k < D; ++k) acc += aF[p * D * D + r + D * k] * double(Ff(p, c + D * k));
+        Lb[r + D * c] = dt * acc;  // L̄ = Δt F̄' Fᵀ
+      }
+    J.each(mesh, [&](int64_t i, double w, const double* g, const double (*)[D]) {
+      for (int r = 0; r < D; ++r) {
+        dvb[i * D + r] += w * av[p * D + r];
+        double acc = dt * w * ax[p * D + r];
+        for (int c = 0; c < D; ++c) acc += Lb[r + D * c] * g[c];
+        vgb[i * D + r] += acc;
+      }
+    });
+  }
+  // grid update reversed: f̄, q̄ (of mv), m̄ per node, walls included
+  std::vector<double> fb(nn * D, 0.0), qb(nn * D, 0.0), mb(nn, 0.0);
+  for (int64_t i = 0; i < nn; ++i) {
+    const double m = double(gm(i));
+    if (!(m > 0)) continue;
+    int64_t I[D], r = i;
+    for (int d = 0; d < D; ++d) { I[d] = r % mesh.dims[d]; r /= mesh.dims[d]; }
+    double acc_m = 0.0;
+    for (int c = 0; c < D; ++c) {
+      bool wall = ((prm.slip_walls >> (2 * c)) & 1) && I[c] == 0;
+      wall = wall || (((prm.slip_walls >> (2 * c + 1)) & 1) && I[c] == mesh.dims[c] - 1);
+      const double a = wall ? 0.0 : vgb[i * D + c] + dvb[i * D + c];
+      const double vnb = a - dvb[i * D + c];
+      fb[i * D + c] = a * dt / m;
+      qb[i * D + c] = vnb / m;
+      acc_m += -a * dt * double(gf(i, c)) / (m * m) - vnb * double(gmv(i, c)) / (m * m);
+    }
+    mb[i] = acc_m;
+  }
+  // nodes gathered back to the particles
+  for (int64_t p = 0; p < n; ++p) {
+    T x[D];
+    load_vec<T, D>(X, p, x);
+    Jets2<T, D> J;
+    particle_jets2<T, D>(basis, mesh, x, J);
+    if (J.empty) continue;
+    double F[D * D], Fb1[D * D], Lb[D * D], Pm[D * D], vp[D];
+    for (int k = 0; k < D * D; ++k) {
+      F[k] = double(Ff(p, k));
+      Fb1[k] = aF[p * D * D + k];
+      Pm[k] = double(V(p)) * double(sg(p, k));
+    }
+    for (int c = 0; c < D; ++c) vp[c] = double(v(p, c));
+    for (int r = 0; r < D; ++r)
+      for (int c = 0; c < D; ++c) {
+        double acc = 0.0;
+        for (int k = 0; k < D; ++k) acc += Fb1[r + D * k] * F[c + D * k];
+        Lb[r + D * c] = dt * acc;
+      }
+    double gx[D] = {}, gvq[D] = {}, Pb[D * D] = {}, L[D * D] = {};
+    J.each(mesh, [&](int64_t i, double w, const double* g, const double (*H)[D]) {
+      double vg[D], dv[D], s = 0.0;
+      for (int c = 0; c < D; ++c) {
+        vg[c] = double(gv(i, c));
+        dv[c] = vg[c] - double(gvn(i, c));
+      }
+      for (int c = 0; c < D; ++c)
+        s += av[p * D + c] * dv[c] + dt * ax[p * D + c] * vg[c] + mp * qb[i * D + c] * vp[c];
+      s += mp * mb[i];
+      double u[D];  // L̄ᵀ vg - Pᵀ f̄
+      for (int c = 0; c < D; ++c) {
+        double acc = 0.0;
+        for (int r = 0; r < D; ++r) acc += Lb[r + D * c] * vg[r] - Pm[r + D * c] * fb[i * D + r];
+        u[c] = acc;
+      }
+      for (int c = 0; c < D; ++c) {
+        double acc = g[c] * s;
+        for (int e = 0; e < D; ++e) acc += H[c][e] * u[e];
+        gx[c] += acc;
+        gvq[c] += w * qb[i * D + c];
+      }
+      for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) {
+          Pb[r + D * c] -= fb[i * D + r] * g[c];
+          L[r + D * c] += vg[r] * g[c];
+        }
+    });
+    double lam = prm.lambda, mu = prm.mu;
+    if (LE.ok()) {
+      const double e = std::exp(double(LE(p)));
+      lam *= e;
+      mu *= e;
+    }
+    double Fi[D * D];
+    inverse_d<T, D>(F, Fi);
+    double trPb = 0.0, PbP = 0.0;
+    for (int c = 0; c < D; ++c) trPb += Pb[c + D * c];
+    for (int k = 0; k < D * D; ++k) PbP += Pb[k] * Pm[k];
+    for (int r = 0; r < D; ++r)
+      for (int c = 0; c < D; ++c) {
+        double acc = Fb1[r + D * c];  // (I + Δt L)ᵀ F̄'
+        for (int k = 0; k < D; ++k) acc += dt * L[k + D * r] * Fb1[k + D * c];
+        double sym = 0.0;  // μ (P̄ + P̄ᵀ) F
+        for (int k = 0; k < D; ++k) sym += (Pb[r + D * k] + Pb[k + D * r]) * F[k + D * c];
+        acc += prm.V0 * (mu * sym + lam * trPb * Fi[c + D * r]);
+        aF[p * D * D + r + D * c] = acc;
+      }
+    for (int c = 0; c < D; ++c) {
+      ax[p * D + c] += gx[c];
+      av[p * D + c] += mp * gvq[c];
+    }
+    if (ath) ath[p] += PbP;
+  }
+  return TESS_OK;
+}
+
+template <typename T, int D>
+struct RunAdjoint {
+  static int run(const tess_mesh& m, int basis, const tess_grid* G, const tess_particles* P, const tess_params* prm,
+                 double* ax, double* av, double* aF, double* ath) {
+    return step_adjoint_impl<T, D>(basis, m, *G, *P, *prm, ax, av, aF, ath);
+  }
+};
+
 template <template <typename, int> class Fn, typename... Args>
 int dispatch(const tess_mesh* m, Args&&... args) {
   if (!m) return fail(TESS_ERR_ARGUMENT, "mesh is NULL");
@@ -832,6 +1065,16 @@ int oref_partition_update(const tess_mesh* m, tess_field x, int64_t n, int32_t* 
 int oref_sparsity_update(const tess_mesh* m, tess_field x, int64_t n, int32_t* numbering, int64_t* storage_len,
                          int32_t* changed) {
   return dispatch<RunSparsity>(m, x, n, numbering, storage_len, changed);
+}
+
+// Reverse of one README step (deferred weights): the particle state is the one at
+// the start of the step (unchanged); ax/av/aF (Float64, Julia layout) hold the
+// adjoint after the step on entry and before it on return; ath += ∂/∂logE.
+// The grid receives the step's forward P2G.
+int oref_step_adjoint(int basis, const tess_mesh* m, const tess_grid* G, const tess_particles* P,
+                      const tess_params* prm, double* ax, double* av, double* aF, double* ath) {
+  if (!basis_ok(basis)) return fail(TESS_ERR_UNSUPPORTED, "basis");
+  return dispatch<RunAdjoint>(m, basis, G, P, prm, ax, av, aF, ath);
 }
 
 int oref_p2g(int basis, const tess_mesh* m, const tess_grid* G, const tess_particles* P, const tess_weights* W,

@@ -125,6 +125,8 @@ class Oracle:
                                                      ctypes.c_void_p, _P(_c64)]),
             "oref_sparsity_update": (ctypes.c_int, [M, _abi.tess_field, _c64, ctypes.c_void_p, _P(_c64),
                                                     _P(ctypes.c_int32)]),
+            "oref_step_adjoint": (ctypes.c_int, [ctypes.c_int, M, G, PP, PR, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_void_p]),
             "oref_p2g": (ctypes.c_int, [ctypes.c_int, M, G, PP, W, PR]),
             "oref_g2p": (ctypes.c_int, [ctypes.c_int, M, G, PP, W, PR]),
             "oref_step": (ctypes.c_int, [ctypes.c_int, M, G, PP, PR, ctypes.c_int]),
@@ -219,6 +221,17 @@ class Oracle:
         P, G = st.particles_abi(), st.grid_abi()
         self._ok(self.lib.oref_step(basis.kind, ctypes.byref(st.mesh_abi()), ctypes.byref(G), ctypes.byref(P),
                                     ctypes.byref(params.abi()), int(nsteps)))
+
+    def step_adjoint(self, st: HostState, basis, params, ax, av, aF, ath=None):
+        """Reverse of one step from the start state ``st`` (unchanged; its grid receives the
+        step's forward P2G): ax/av/aF (float64, updated in place) go from the adjoint after
+        the step to the adjoint before it; ath += ∂/∂logE."""
+        for a in (ax, av, aF) + ((ath,) if ath is not None else ()):
+            assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        P, G = st.particles_abi(), st.grid_abi()
+        self._ok(self.lib.oref_step_adjoint(basis.kind, ctypes.byref(st.mesh_abi()), ctypes.byref(G),
+                                            ctypes.byref(P), ctypes.byref(params.abi()), _np_ptr(ax), _np_ptr(av),
+                                            _np_ptr(aF), _np_ptr(ath)))
 
     def step_threaded(self, st: HostState, basis, params, nthreads, nsteps=1, weights: HostWeights | None = None):
         W = weights or HostWeights(basis, st.dim, st.n, st.npdt)
