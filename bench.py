@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--reorder", type=float, default=0.85, help="reorder_particles! threshold per step")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward ratio leg")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--checkpoint-every", type=int, default=20)
     return ap.parse_args()
 
 
@@ -62,6 +65,53 @@ def cpu_baseline(threads):
     return {"value": sc.n * 2 / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"block drop dx=1/128 ({sc.n} particles), Float32, 2 timed steps after 1 warm-up, "
                       f"{threads} threads, {dt:.2f} s"}
+
+
+def adjoint_ratio(nsteps, every):
+    """Differentiable inverse cube (dx=1/64, 132,651 particles, quadratic): time of a plain
+    forward of nsteps against a checkpointed rollout plus its pullback of the loss
+    mean ||x(T) - (x(0) + (0.1,0,0))||^2 (FP32 forward, FP64 adjoint)."""
+    import numpy as np
+    import torch
+
+    import tesserae_hip as th
+    from gpu_util import to_gpu
+    from tesserae_hip.scenes import inverse_cube
+
+    sc = inverse_cube(dtype=torch.float32)
+    ctx = th.Context()
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+
+    def timed(fn):
+        a, b = ev(), ev()
+        torch.cuda.synchronize()
+        a.record()
+        fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b)
+
+    target = torch.from_numpy(sc.x + np.array([0.1, 0.0, 0.0])).cuda()
+    # warm-up: one short forward and pullback (allocations, code loading)
+    mesh, P, G = to_gpu(sc)
+    th.rollout(G, P, sc.basis, sc.params, 2, checkpoint_every=every, ctx=ctx)
+    adj = th.Adjoint(sc.n, 3)
+    th.rollout_adjoint(G, sc.basis, sc.params, adj, ctx=ctx)
+    mesh, P, G = to_gpu(sc)
+    t_fwd = timed(lambda: th.step(G, P, sc.basis, sc.params, nsteps=nsteps, ctx=ctx))
+    mesh, P, G = to_gpu(sc)
+    t_roll = timed(lambda: th.rollout(G, P, sc.basis, sc.params, nsteps, checkpoint_every=every, ctx=ctx))
+    x = P.get("x").double()
+    loss = float(((x - target) ** 2).sum(dim=1).mean())
+    adj = th.Adjoint(sc.n, 3)
+    adj.x.copy_(2.0 * (x - target) / sc.n)
+    t_adj = timed(lambda: th.rollout_adjoint(G, sc.basis, sc.params, adj, ctx=ctx))
+    gnorm = float(adj.v.norm())
+    ctx.close()
+    return {"workload": f"inverse cube dx=1/64, {sc.n} particles, quadratic, {nsteps} steps, checkpoint every "
+                        f"{every}, FP32 forward / FP64 adjoint",
+            "t_forward_ms": t_fwd, "t_rollout_ms": t_roll, "t_pullback_ms": t_adj,
+            "ratio": (t_roll + t_adj) / t_fwd, "loss": loss, "grad_v0_norm": gnorm}
 
 
 def main():
@@ -145,6 +195,9 @@ def main():
         "kernels": kernels,
         "wall_s": wall,
     }
+    if not args.no_adjoint:
+        log("bench: adjoint/forward ratio")
+        out["adjoint"] = adjoint_ratio(args.adjoint_steps, args.checkpoint_every)
     if not args.no_cpu:
         log("bench: CPU baseline")
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

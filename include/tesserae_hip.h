@@ -244,6 +244,43 @@ tess_status tess_step_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh
                              tess_sparsity* sp, const tess_particles* particles, const tess_params* params,
                              int32_t nsteps, double reorder_threshold);
 
+/* ---- differentiation (no counterpart in the reference) ------------------- */
+
+/* Adjoint of the README step's state (x, v, F) and of the per-particle
+ * stiffness exponent logE (E_p = E exp(logE_p)); Float64 fields whatever the
+ * mesh dtype.  The step treats V = V0 J and σ = (μ(FFᵀ-I) + λ lnJ I)/J as
+ * functions of F (the README particle update), so a particle state must hold
+ * σ and V consistent with F (as every state after a step does, and F = I,
+ * σ = 0, V = V0 does).  logE.ptr may be NULL when its gradient is not wanted. */
+typedef struct tess_adjoint {
+  tess_field x, v, F;
+  tess_field logE;
+} tess_adjoint;
+
+/* Reverse of one tess_step with deferred weights and no reordering (B-spline
+ * bases): particles is the state at the START of the step (not modified); on
+ * entry adj holds the adjoint of the state after the step, on return the
+ * adjoint of the state before it, and adj->logE accumulates (+=) the step's
+ * contribution.  The grid receives the step's forward P2G (scratch).
+ * Deterministic (no float atomics outside LDS with a fixed order). */
+tess_status tess_step_adjoint(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                              const tess_particles* particles, const tess_params* params, tess_adjoint* adj);
+
+/* Forward rollout of nsteps README steps (deferred weights, no reordering) on
+ * the caller's particles, recording a checkpoint of (x, v, F, V, σ) every
+ * checkpoint_every steps in the context (the logE field is referenced, not
+ * copied: keep it alive and unchanged until tess_rollout_adjoint). */
+tess_status tess_rollout(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                         const tess_particles* particles, const tess_params* params, int32_t nsteps,
+                         int32_t checkpoint_every);
+
+/* Pullback of the context's last tess_rollout: adj holds the adjoint of the
+ * final state on entry and of the initial state on return; adj->logE
+ * accumulates ∂/∂logE over all steps.  Each segment is replayed from its
+ * checkpoint, then reversed step by step.  The grid is scratch. */
+tess_status tess_rollout_adjoint(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                                 const tess_params* params, tess_adjoint* adj);
+
 /* ---- measurement (no counterpart in the reference) ----------------------- */
 
 /* Per-kernel timing with HIP events recorded on the context's stream around

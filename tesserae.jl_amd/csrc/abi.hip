@@ -236,7 +236,9 @@ tess_status tess_ctx_destroy(tess_ctx* ctx) {
                             &ctx->per_tile,  &ctx->per_nodeblock, &ctx->skeys2, &ctx->perm2,
                             &ctx->inc_nk,    &ctx->inc_spos,  &ctx->inc_mkey, &ctx->inc_mj,
                             &ctx->inc_mks,   &ctx->inc_mr,    &ctx->inc_mjs,
-                            &ctx->sp_occ,    &ctx->sp_act,    &ctx->sp_scan};
+                            &ctx->sp_occ,    &ctx->sp_act,    &ctx->sp_scan,
+                            &ctx->adj_partials, &ctx->adj_nodes, &ctx->ro_checkpoints, &ctx->ro_segment,
+                            &ctx->ro_gradv};
     for (auto* b : bufs) b->release();
     for (auto ev : ctx->prof.pool) (void)hipEventDestroy(ev);
     if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
@@ -395,6 +397,67 @@ tess_status tess_step_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh
     if (!(reorder_threshold >= 0.0 && reorder_threshold <= 1.0))
       raise(TESS_ERR_ARGUMENT, "threshold must be in [0, 1].");
     run_step_sparse(ctx, basis, *mesh, *grid, *sp, *particles, *params, nsteps, reorder_threshold);
+  });
+}
+
+namespace tess {
+namespace {
+void check_adjoint_basis(int32_t basis) {
+  check_basis(basis);
+  if (basis == TESS_UGIMP) raise(TESS_ERR_UNSUPPORTED, "the adjoint supports the B-spline bases");
+}
+
+void check_adjoint(const tess_adjoint* adj, bool need_state) {
+  if (!adj) raise(TESS_ERR_ARGUMENT, "adjoint is NULL");
+  if (need_state) {
+    need(adj->x, "x̄");
+    need(adj->v, "v̄");
+    need(adj->F, "F̄");
+  }
+}
+}  // namespace
+}  // namespace tess
+
+tess_status tess_step_adjoint(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                              const tess_particles* particles, const tess_params* params, tess_adjoint* adj) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    check_adjoint_basis(basis);
+    check_grid(grid);
+    check_particles(particles, basis, true);
+    check_params(params);
+    check_adjoint(adj, true);
+    run_step_adjoint(ctx, basis, *mesh, *grid, *particles, *params, *adj);
+  });
+}
+
+tess_status tess_rollout(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                         const tess_particles* particles, const tess_params* params, int32_t nsteps,
+                         int32_t checkpoint_every) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    check_adjoint_basis(basis);
+    check_grid(grid);
+    check_particles(particles, basis, true);
+    check_params(params);
+    if (nsteps < 1) raise(TESS_ERR_ARGUMENT, "a rollout needs at least one step");
+    if (checkpoint_every < 1) raise(TESS_ERR_ARGUMENT, "checkpoint_every must be positive");
+    run_rollout(ctx, basis, *mesh, *grid, *particles, *params, nsteps, checkpoint_every);
+  });
+}
+
+tess_status tess_rollout_adjoint(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                                 const tess_params* params, tess_adjoint* adj) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    check_adjoint_basis(basis);
+    check_grid(grid);
+    check_params(params);
+    check_adjoint(adj, true);
+    run_rollout_adjoint(ctx, basis, *mesh, *grid, *params, *adj);
   });
 }
 

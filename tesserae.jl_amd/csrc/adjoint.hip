@@ -1,0 +1,683 @@
+// Reverse mode of the README step (the engine's differentiation extension; no
+// counterpart in the reference).  State of a step: (x, v, F); V = V0 J and
+// σ = (μ(FFᵀ-I) + λ lnJ I)/J follow from F and the per-particle stiffness
+// E_p = E exp(logE_p), so P = V σ = V0 (μ(FFᵀ-I) + λ lnJ I).
+//
+// One reversed step, given the state at its start and the adjoint (x̄', v̄', F̄')
+// of the state after it (Float64 throughout):
+//   bin + forward P2G      the step's grid (m, mv, f, vⁿ, v), recomputed
+//   k_adj_scatter          particles -> nodes through the same sorted tiles and
+//                          class-owner lanes as P2G (deterministic):
+//                          Δv̄_i = Σ w v̄',  v̄_i = Σ (L̄ ∇w + Δt w x̄'),  L̄ = Δt F̄' Fᵀ
+//   k_adj_grid             per node block: the grid update reversed (m>0 guard,
+//                          slip walls): f̄, q̄ (of mv), m̄
+//   k_adj_gather           per particle over its window, with ∇w and ∇∇w:
+//                          x̄ = x̄' + Σ ∇w (v̄'·Δv + Δt x̄'·v + mₚ m̄ + mₚ q̄·v_p)
+//                                  + Σ ∇∇w (L̄ᵀ v - Pᵀ f̄)
+//                          v̄ = v̄' + mₚ Σ w q̄
+//                          P̄ = -Σ f̄ ⊗ ∇w;  F̄ = (I+ΔtL)ᵀ F̄' + V0 (μ (P̄+P̄ᵀ) F + λ tr(P̄) F⁻ᵀ)
+//                          ∂/∂logE += P̄ : P
+// Rollouts keep checkpoints every k steps and replay each segment before
+// reversing it.
+#include "transfer_impl.hpp"
+
+namespace tess {
+namespace {
+
+// Centred B-spline at node-relative coordinate r = ξ - i with its first and
+// second derivatives in r (the textbook piecewise polynomials).
+template <int BASIS>
+__device__ __forceinline__ void bspline_jet2(double r, double& N, double& dN, double& ddN) {
+  const double a = fabs(r), sg = signbit(r) ? -1.0 : 1.0;
+  N = 0.0;
+  dN = 0.0;
+  ddN = 0.0;
+  if constexpr (BASIS == TESS_BSPLINE_LINEAR) {
+    if (a < 1.0) {
+      N = 1.0 - a;
+      dN = -sg;
+    }
+  } else if constexpr (BASIS == TESS_BSPLINE_QUADRATIC) {
+    if (a < 0.5) {
+      N = 0.75 - r * r;
+      dN = -2.0 * r;
+      ddN = -2.0;
+    } else if (a < 1.5) {
+      const double u = 1.5 - a;
+      N = 0.5 * u * u;
+      dN = -sg * u;
+      ddN = 1.0;
+    }
+  } else {
+    if (a < 1.0) {
+      N = 0.5 * a * a * a - r * r + 2.0 / 3.0;
+      dN = 1.5 * sg * r * r - 2.0 * r;
+      ddN = 3.0 * a - 2.0;
+    } else if (a < 2.0) {
+      const double u = 2.0 - a;
+      N = u * u * u / 6.0;
+      dN = -sg * 0.5 * u * u;
+      ddN = u;
+    }
+  }
+}
+
+// Per-axis jets of a particle's window: node lo[d]+k for k < SW (zero past hi).
+template <typename T, int D, int BASIS, bool SECOND>
+__device__ __forceinline__ void window_jets2(const Args<T, D>& a, const T* x, const int* lo, const int* hi,
+                                             double (*N)[support_width(BASIS)], double (*dN)[support_width(BASIS)],
+                                             double (*ddN)[support_width(BASIS)]) {
+  constexpr int SW = support_width(BASIS);
+  const double hinv = 1.0 / a.mesh.h_d;
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+#pragma unroll
+    for (int k = 0; k < SW; ++k) {
+      double n0 = 0.0, n1 = 0.0, n2 = 0.0;
+      if (lo[d] + k <= hi[d]) {
+        const double r = (double(x[d]) - (a.mesh.xmin_d[d] + double(lo[d] + k) * a.mesh.h_d)) * hinv;
+        bspline_jet2<BASIS>(r, n0, n1, n2);
+      }
+      N[d][k] = n0;
+      dN[d][k] = n1 * hinv;
+      if constexpr (SECOND) ddN[d][k] = n2 * hinv * hinv;
+    }
+}
+
+// Particle adjoints -> per-tile node partials [tile][NFOOT][2D]: Δv̄ (D) then v̄ (D).
+// One wave per tile, lane = window-start class, as in P2G; one pass per offset
+// along the last axis keeps the Float64 accumulators in registers.
+template <typename T, int D, int BASIS>
+__global__ __launch_bounds__(64) void k_adj_scatter(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                    DField<double> ax, DField<double> av, DField<double> aF,
+                                                    double* __restrict__ partials) {
+  using G = Geo<D>;
+  constexpr int SW = support_width(BASIS);
+  constexpr int NA = 2 * D;
+  constexpr int NSL = Pow<SW, D - 1>::v;
+  const int lane = threadIdx.x;
+  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
+  if (j >= a.n_active) return;
+  const uint32_t tile = a.active[j];
+  const uint32_t start = a.tile_start[tile];
+  const int cnt = int(a.tile_end[tile] - start);
+  int tI[D], origin[D];
+  unlin<D>(tile, a.ntiles, tI);
+#pragma unroll
+  for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
+  __shared__ double s_tile[G::NFOOT][NA];
+  __shared__ uint32_t s_cnt[64];
+  for (int q = lane; q < NA * G::NFOOT; q += 64) (&s_tile[0][0])[q] = 0.0;
+  int ccount, coff, maxcnt;
+  tile_class_counts(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);
+  int ccoord[D];
+  {
+    int r = lane;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      ccoord[d] = r % G::BW;
+      r /= G::BW;
+    }
+  }
+  const double dt = double(a.dt);
+  for (int pass = 0; pass < SW; ++pass) {
+    double acc[NSL][NA];
+#pragma unroll
+    for (int s = 0; s < NSL; ++s)
+#pragma unroll
+      for (int v = 0; v < NA; ++v) acc[s][v] = 0.0;
+    for (int k = 0; k < maxcnt; ++k) {
+      if (k >= ccount) continue;
+      const uint32_t pid = a.perm[start + coff + k];
+      T x[D], l;
+      int lo[D], hi[D];
+      particle_window<T, D, BASIS, false>(a, pid, x, l, lo, hi, nullptr);
+      double N[D][SW], dN[D][SW];
+      window_jets2<T, D, BASIS, false>(a, x, lo, hi, N, dN, nullptr);
+      double vb[D], xb[D], Lb[D * D];
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        vb[c] = av(pid, c);
+        xb[c] = dt * ax(pid, c);
+      }
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < D; ++q) s += aF(pid, r + D * q) * double(a.F(pid, c + D * q));
+          Lb[r + D * c] = dt * s;  // L̄ = Δt F̄' Fᵀ
+        }
+#pragma unroll
+      for (int s = 0; s < NSL; ++s) {
+        int o[D];
+        int r = s;
+#pragma unroll
+        for (int d = 0; d < D - 1; ++d) {
+          o[d] = r % SW;
+          r /= SW;
+        }
+        o[D - 1] = pass;
+        double w = 1.0, g[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) w *= N[d][o[d]];
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          double gg = 1.0;
+#pragma unroll
+          for (int d = 0; d < D; ++d) gg *= (d == c) ? dN[d][o[d]] : N[d][o[d]];
+          g[c] = gg;
+        }
+#pragma unroll
+        for (int r2 = 0; r2 < D; ++r2) {
+          acc[s][r2] += w * vb[r2];
+          double t = w * xb[r2];
+#pragma unroll
+          for (int c = 0; c < D; ++c) t += Lb[r2 + D * c] * g[c];
+          acc[s][D + r2] += t;
+        }
+      }
+    }
+    if (ccount > 0) {
+      // lanes own distinct classes: one add instruction never hits a node twice
+#pragma unroll
+      for (int s = 0; s < NSL; ++s) {
+        int r = s, q = 0, mul = 1;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const int od = (d == D - 1) ? pass : r % SW;
+          if (d < D - 1) r /= SW;
+          q += (ccoord[d] + od) * mul;
+          mul *= G::TS;
+        }
+#pragma unroll
+        for (int v = 0; v < NA; ++v) atomicAdd(&s_tile[q][v], acc[s][v]);
+      }
+    }
+  }
+  __syncthreads();
+  double* out = partials + int64_t(j) * NA * G::NFOOT;
+  for (int q = lane; q < NA * G::NFOOT; q += 64) out[q] = (&s_tile[0][0])[q];
+}
+
+// Node adjoints [node][2D+1] = f̄ (D), q̄ (D), m̄ of every node of the touched
+// node blocks: the covering tiles' partials summed in a fixed order, then the
+// grid update reversed.
+template <typename T, int D>
+__global__ __launch_bounds__(64) void k_adj_grid(Args<T, D> a, const double* __restrict__ partials,
+                                                 double* __restrict__ nodeadj) {
+  using G = Geo<D>;
+  constexpr int NA = 2 * D;
+  constexpr int NB = (D == 3) ? 27 : 9;
+  const int64_t j = xcd_remap(blockIdx.x, a.n_nb);
+  if (j >= a.n_nb) return;
+  const uint32_t B = a.nb_list[j];
+  int Bi[D], I[D], rr[D];
+  unlin<D>(B, a.ntiles, Bi);
+  __shared__ int s_slot[NB];
+  if (threadIdx.x < NB) {
+    int r = int(threadIdx.x), t = 0, mul = 1;
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int b = Bi[d] + (r % 3) - 1;
+      r /= 3;
+      ok = ok && b >= 0 && b < a.ntiles[d];
+      t += b * mul;
+      mul *= a.ntiles[d];
+    }
+    s_slot[threadIdx.x] = (ok && a.tile_flag[t]) ? int(a.tile_slot[t]) : -1;
+  }
+  __syncthreads();
+  {
+    int r = threadIdx.x;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      rr[d] = r % G::BW;
+      r /= G::BW;
+      I[d] = Bi[d] * G::BW + rr[d];
+      if (I[d] >= a.mesh.dims[d]) return;
+    }
+  }
+  double acc[NA];
+#pragma unroll
+  for (int v = 0; v < NA; ++v) acc[v] = 0.0;
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    lo[d] = rr[d] <= 1 ? -1 : 0;
+    hi[d] = rr[d] == G::BW - 1 ? 1 : 0;
+  }
+  int o[3] = {0, 0, 0};
+  for (o[2] = lo[2]; o[2] <= hi[2]; ++o[2])
+    for (o[1] = lo[1]; o[1] <= hi[1]; ++o[1])
+      for (o[0] = lo[0]; o[0] <= hi[0]; ++o[0]) {
+        int nb = 0, q = 0, mulb = 1, mulq = 1;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          nb += (o[d] + 1) * mulb;
+          mulb *= 3;
+          q += (rr[d] + 1 - o[d] * G::BW) * mulq;
+          mulq *= G::TS;
+        }
+        const int slot = s_slot[nb];
+        if (slot < 0) continue;
+        const double* P = partials + (int64_t(slot) * G::NFOOT + q) * NA;
+#pragma unroll
+        for (int v = 0; v < NA; ++v) acc[v] += P[v];
+      }
+  const int64_t i = a.mesh.lin(I);
+  double* out = nodeadj + i * (2 * D + 1);
+  const double m = double(a.gm(i));
+  if (!(m > 0.0)) {
+#pragma unroll
+    for (int v = 0; v < 2 * D + 1; ++v) out[v] = 0.0;
+    return;
+  }
+  const double dt = double(a.dt), im = 1.0 / m;
+  double mb = 0.0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    const bool wall = (((a.slip >> (2 * c)) & 1) && I[c] == 0) ||
+                      (((a.slip >> (2 * c + 1)) & 1) && I[c] == a.mesh.dims[c] - 1);
+    const double dvb = acc[c], vgb = acc[D + c];
+    const double s = wall ? 0.0 : vgb + dvb;  // adjoint reaching v through its update
+    const double vnb = s - dvb;               // vⁿ enters v and Δv = v - vⁿ
+    out[c] = s * dt * im;                     // f̄
+    out[D + c] = vnb * im;                    // q̄ (of mv)
+    mb -= (s * dt * double(a.gf(i, c)) + vnb * double(a.gmv(i, c))) * im * im;
+  }
+  out[2 * D] = mb;
+}
+
+// Nodes -> particle adjoints: the footprint's forward v, Δv and node adjoints
+// are staged in LDS; each thread reverses one particle of the tile.
+template <typename T, int D, int BASIS>
+__global__ __launch_bounds__(256) void k_adj_gather(Args<T, D> a, const double* __restrict__ nodeadj,
+                                                    DField<double> ax, DField<double> av, DField<double> aF,
+                                                    DField<double> ath) {
+  using G = Geo<D>;
+  constexpr int SW = support_width(BASIS);
+  constexpr int NS = 4 * D + 1;  // v (D), Δv (D), f̄ (D), q̄ (D), m̄
+  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
+  if (j >= a.n_active) return;
+  const uint32_t tile = a.active[j];
+  const uint32_t start = a.tile_start[tile];
+  const int cnt = int(a.tile_end[tile] - start);
+  int tI[D], origin[D];
+  unlin<D>(tile, a.ntiles, tI);
+#pragma unroll
+  for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
+  __shared__ double s_n[G::NFOOT][NS];
+  for (int q = int(threadIdx.x); q < G::NFOOT; q += int(blockDim.x)) {
+    int I[D], r = q;
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      I[d] = origin[d] + r % G::TS;
+      r /= G::TS;
+      in = in && I[d] >= 0 && I[d] < a.mesh.dims[d];
+    }
+    const int64_t i = in ? a.mesh.lin(I) : 0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      const double vv = in ? double(a.gv(i, c)) : 0.0;
+      s_n[q][c] = vv;
+      s_n[q][D + c] = in ? vv - double(a.gvn(i, c)) : 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < 2 * D + 1; ++v) s_n[q][2 * D + v] = in ? nodeadj[i * (2 * D + 1) + v] : 0.0;
+  }
+  __syncthreads();
+  const double dt = double(a.dt), mp = double(a.mp);
+  for (int t = int(threadIdx.x); t < cnt; t += int(blockDim.x)) {
+    const uint32_t p = a.perm[start + t];
+    T x[D], l;
+    int lo[D], hi[D];
+    particle_window<T, D, BASIS, false>(a, p, x, l, lo, hi, nullptr);
+    double N[D][SW], dN[D][SW], ddN[D][SW];
+    window_jets2<T, D, BASIS, true>(a, x, lo, hi, N, dN, ddN);
+    double xb[D], vb[D], Fb[D * D], F[D * D], Pm[D * D], vp[D], Lb[D * D];
+    const double Vp = double(a.V(p));
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      xb[c] = ax(p, c);
+      vb[c] = av(p, c);
+      vp[c] = double(a.v(p, c));
+    }
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) {
+      Fb[k] = aF(p, k);
+      F[k] = double(a.F(p, k));
+      Pm[k] = Vp * double(a.sigma(p, k));
+    }
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < D; ++q) s += Fb[r + D * q] * F[c + D * q];
+        Lb[r + D * c] = dt * s;
+      }
+    int base[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) base[d] = lo[d] - origin[d];
+    double gx[D], gq[D], Pb[D * D], L[D * D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) gx[c] = gq[c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) Pb[k] = L[k] = 0.0;
+    constexpr int NOFF = Pow<SW, D>::v;
+#pragma unroll
+    for (int s = 0; s < NOFF; ++s) {
+      int o[D], r = s, q = 0, mul = 1;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        o[d] = r % SW;
+        r /= SW;
+        q += (base[d] + o[d]) * mul;
+        mul *= G::TS;
+      }
+      double w = 1.0, g[D], H[D][D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) w *= N[d][o[d]];
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        double gg = 1.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) gg *= (d == c) ? dN[d][o[d]] : N[d][o[d]];
+        g[c] = gg;
+#pragma unroll
+        for (int e = 0; e < D; ++e) {
+          double h = 1.0;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            if (c == e) h *= (d == c) ? ddN[d][o[d]] : N[d][o[d]];
+            else h *= (d == c || d == e) ? dN[d][o[d]] : N[d][o[d]];
+          }
+          H[c][e] = h;
+        }
+      }
+      const double* nd = s_n[q];  // base[d] = class coordinate <= BW-1, so q stays inside the footprint
+      double sdot = mp * nd[4 * D];
+      double u[D];
+#pragma unroll
+      for (int c = 0; c < D; ++c) sdot += vb[c] * nd[D + c] + dt * xb[c] * nd[c] + mp * nd[3 * D + c] * vp[c];
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        double acc = 0.0;
+#pragma unroll
+        for (int r2 = 0; r2 < D; ++r2) acc += Lb[r2 + D * c] * nd[r2] - Pm[r2 + D * c] * nd[2 * D + r2];
+        u[c] = acc;
+      }
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        double acc = g[c] * sdot;
+#pragma unroll
+        for (int e = 0; e < D; ++e) acc += H[c][e] * u[e];
+        gx[c] += acc;
+        gq[c] += w * nd[3 * D + c];
+      }
+#pragma unroll
+      for (int r2 = 0; r2 < D; ++r2)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          Pb[r2 + D * c] -= nd[2 * D + r2] * g[c];
+          L[r2 + D * c] += nd[r2] * g[c];
+        }
+    }
+    double lam = a.lambda, mu = a.mu;
+    if (a.logE.p) {
+      const double e = exp(double(a.logE(p)));
+      lam *= e;
+      mu *= e;
+    }
+    // F⁻¹ by the adjugate
+    double Fi[D * D];
+    if constexpr (D == 2) {
+      const double det = F[0] * F[3] - F[2] * F[1];
+      Fi[0] = F[3] / det;
+      Fi[1] = -F[1] / det;
+      Fi[2] = -F[2] / det;
+      Fi[3] = F[0] / det;
+    } else {
+      double cof[9];
+#pragma unroll
+      for (int r2 = 0; r2 < 3; ++r2)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const int r1 = (r2 + 1) % 3, r3 = (r2 + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+          cof[r2 + 3 * c] = F[r1 + 3 * c1] * F[r3 + 3 * c2] - F[r1 + 3 * c2] * F[r3 + 3 * c1];
+        }
+      const double det = F[0] * cof[0] + F[3] * cof[3] + F[6] * cof[6];
+#pragma unroll
+      for (int r2 = 0; r2 < 3; ++r2)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Fi[r2 + 3 * c] = cof[c + 3 * r2] / det;
+    }
+    double trPb = 0.0, PbP = 0.0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) trPb += Pb[c + D * c];
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) PbP += Pb[k] * Pm[k];
+    double Fn[D * D];
+#pragma unroll
+    for (int r2 = 0; r2 < D; ++r2)
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        double acc = Fb[r2 + D * c];
+#pragma unroll
+        for (int k = 0; k < D; ++k) acc += dt * L[k + D * r2] * Fb[k + D * c];
+        double sym = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) sym += (Pb[r2 + D * k] + Pb[k + D * r2]) * F[k + D * c];
+        Fn[r2 + D * c] = acc + a.V0 * (mu * sym + lam * trPb * Fi[c + D * r2]);
+      }
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) aF(p, k) = Fn[k];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      ax(p, c) = xb[c] + gx[c];
+      av(p, c) = vb[c] + mp * gq[c];
+    }
+    if (ath.p) ath(p) += PbP;
+  }
+}
+
+inline DField<double> dfield_d(const tess_field& f) {
+  return DField<double>{static_cast<double*>(f.ptr), f.elem_stride, f.comp_stride};
+}
+
+// One reversed step: S is the state at the start of the step.
+template <typename T, int D, int BASIS>
+void step_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& S,
+                       const tess_params& prm, const tess_adjoint& adj) {
+  using Gm = Geo<D>;
+  bin_particles<T, D, BASIS, false>(ctx, mesh, S, nullptr);
+  p2g_launch<T, D, BASIS, false>(ctx, mesh, G, S, nullptr, prm, false, nullptr);
+  const Binning& b = ctx->bin;
+  if (b.n_active == 0) return;  // every particle outside: the step is the identity on the state
+  Args<T, D> a = make_args<T, D>(mesh, &G, S, nullptr, &prm);
+  fill_binning_ptrs<T, D>(ctx, a);
+  int64_t nnodes = 1;
+  for (int d = 0; d < D; ++d) nnodes *= mesh.dims[d];
+  double* partials = ctx->adj_partials.as<double>(size_t(b.n_active) * 2 * D * Gm::NFOOT);
+  double* nodeadj = ctx->adj_nodes.as<double>(size_t(nnodes) * (2 * D + 1));
+  const DField<double> ax = dfield_d(adj.x), av = dfield_d(adj.v), aF = dfield_d(adj.F), ath = dfield_d(adj.logE);
+  const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
+  int tok = prof_begin(ctx, "k_adj_scatter", 0.0);
+  hipLaunchKernelGGL((k_adj_scatter<T, D, BASIS>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream, a, sk, ax,
+                     av, aF, partials);
+  TESS_HIP(hipGetLastError());
+  prof_end(ctx, tok);
+  tok = prof_begin(ctx, "k_adj_grid", 0.0);
+  hipLaunchKernelGGL((k_adj_grid<T, D>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a, partials,
+                     nodeadj);
+  TESS_HIP(hipGetLastError());
+  prof_end(ctx, tok);
+  tok = prof_begin(ctx, "k_adj_gather", 0.0);
+  hipLaunchKernelGGL((k_adj_gather<T, D, BASIS>), dim3(xcd_grid(b.n_active)), dim3(256), 0, ctx->stream, a,
+                     nodeadj, ax, av, aF, ath);
+  TESS_HIP(hipGetLastError());
+  prof_end(ctx, tok);
+}
+
+// ---- checkpointed rollouts -----------------------------------------------------
+
+// A particle state stored by the context: x, v, F, V, σ packed field after field
+// in the Julia layout.
+template <typename T, int D>
+constexpr int64_t state_scalars() {
+  return 2 * D + 2 * D * D + 1;
+}
+
+template <typename T, int D>
+tess_particles state_view(T* base, int64_t n, const Rollout& ro, T* gradv) {
+  tess_particles P;
+  std::memset(static_cast<void*>(&P), 0, sizeof(P));
+  P.n = n;
+  auto fld = [](void* p, int64_t es) {
+    tess_field f;
+    f.ptr = p;
+    f.elem_stride = es;
+    f.comp_stride = 1;
+    return f;
+  };
+  T* q = base;
+  P.x = fld(q, D);
+  q += n * D;
+  P.v = fld(q, D);
+  q += n * D;
+  P.F = fld(q, D * D);
+  q += n * D * D;
+  P.V = fld(q, 1);
+  q += n;
+  P.sigma = fld(q, D * D);
+  P.grad_v = fld(gradv, D * D);
+  P.logE = ro.logE;
+  return P;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_field(DField<T> src, DField<T> dst, int64_t n, int nc) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  for (int c = 0; c < nc; ++c) dst(e, c) = src(e, c);
+}
+
+template <typename T, int D>
+void copy_state(tess_ctx* ctx, const tess_particles& src, const tess_particles& dst) {
+  const int64_t n = src.n;
+  if (n == 0) return;
+  const tess_field* s[] = {&src.x, &src.v, &src.F, &src.V, &src.sigma};
+  const tess_field* t[] = {&dst.x, &dst.v, &dst.F, &dst.V, &dst.sigma};
+  const int nc[] = {D, D, D * D, 1, D * D};
+  for (int k = 0; k < 5; ++k) {
+    hipLaunchKernelGGL(k_copy_field<T>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, dfield<T>(*s[k]),
+                       dfield<T>(*t[k]), n, nc[k]);
+    TESS_HIP(hipGetLastError());
+  }
+}
+
+template <typename T, int D, int BASIS>
+void rollout_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+                  const tess_params& prm, int nsteps, int every) {
+  Rollout& ro = ctx->rollout;
+  ro.valid = false;
+  const int64_t n = P.n;
+  const int ncheck = (nsteps + every - 1) / every;
+  const int64_t per = state_scalars<T, D>() * n;
+  T* ck = ctx->ro_checkpoints.as<T>(size_t(std::max<int64_t>(per * ncheck, 1)));
+  ro.logE = P.logE;
+  for (int s = 0; s < nsteps; ++s) {
+    if (s % every == 0) copy_state<T, D>(ctx, P, state_view<T, D>(ck + per * (s / every), n, ro, nullptr));
+    step_impl<T, D, BASIS, false>(ctx, mesh, G, P, prm, 1, 0.0, nullptr);
+  }
+  ro.valid = true;
+  ro.nsteps = nsteps;
+  ro.every = every;
+  ro.n = n;
+  ro.dim = D;
+  ro.dtype = mesh.dtype;
+  ro.basis = BASIS;
+  for (int d = 0; d < 3; ++d) ro.dims[d] = mesh.dims[d];
+}
+
+template <typename T, int D, int BASIS>
+void rollout_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_params& prm,
+                          const tess_adjoint& adj) {
+  Rollout& ro = ctx->rollout;
+  const int64_t n = ro.n;
+  const int64_t per = state_scalars<T, D>() * n;
+  const int ncheck = (ro.nsteps + ro.every - 1) / ro.every;
+  T* ck = static_cast<T*>(ctx->ro_checkpoints.ptr);
+  T* seg = ctx->ro_segment.as<T>(size_t(std::max<int64_t>(per * ro.every, 1)));
+  T* gradv = ctx->ro_gradv.as<T>(size_t(std::max<int64_t>(n * D * D, 1)));
+  for (int c = ncheck - 1; c >= 0; --c) {
+    const int s0 = c * ro.every;
+    const int len = std::min(ro.every, ro.nsteps - s0);
+    // replay the segment: slot k holds the state at the start of step s0 + k
+    TESS_HIP(hipMemcpyAsync(seg, ck + per * c, size_t(per) * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+    for (int k = 1; k < len; ++k) {
+      TESS_HIP(hipMemcpyAsync(seg + per * k, seg + per * (k - 1), size_t(per) * sizeof(T), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+      const tess_particles S = state_view<T, D>(seg + per * k, n, ro, gradv);
+      step_impl<T, D, BASIS, false>(ctx, mesh, G, S, prm, 1, 0.0, nullptr);
+    }
+    for (int k = len - 1; k >= 0; --k) {
+      const tess_particles S = state_view<T, D>(seg + per * k, n, ro, gradv);
+      step_adjoint_impl<T, D, BASIS>(ctx, mesh, G, S, prm, adj);
+    }
+  }
+}
+
+template <typename T, int D, int B, bool S>
+struct AdjointOp {
+  static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
+                  const tess_params& prm, const tess_adjoint& adj) {
+    if constexpr (B == TESS_UGIMP || S) raise(TESS_ERR_UNSUPPORTED, "the adjoint supports the B-spline bases");
+    else step_adjoint_impl<T, D, B>(ctx, m, G, P, prm, adj);
+  }
+};
+template <typename T, int D, int B, bool S>
+struct RolloutOp {
+  static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
+                  const tess_params& prm, int nsteps, int every) {
+    if constexpr (B == TESS_UGIMP || S) raise(TESS_ERR_UNSUPPORTED, "the adjoint supports the B-spline bases");
+    else rollout_impl<T, D, B>(ctx, m, G, P, prm, nsteps, every);
+  }
+};
+template <typename T, int D, int B, bool S>
+struct RolloutAdjointOp {
+  static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_params& prm,
+                  const tess_adjoint& adj) {
+    if constexpr (B == TESS_UGIMP || S) raise(TESS_ERR_UNSUPPORTED, "the adjoint supports the B-spline bases");
+    else rollout_adjoint_impl<T, D, B>(ctx, m, G, prm, adj);
+  }
+};
+
+}  // namespace
+
+void run_step_adjoint(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+                      const tess_params& prm, const tess_adjoint& adj) {
+  dispatch<AdjointOp>(basis, mesh, false, ctx, mesh, G, P, prm, adj);
+}
+
+void run_rollout(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+                 const tess_params& prm, int nsteps, int every) {
+  dispatch<RolloutOp>(basis, mesh, false, ctx, mesh, G, P, prm, nsteps, every);
+}
+
+void run_rollout_adjoint(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G,
+                         const tess_params& prm, const tess_adjoint& adj) {
+  const Rollout& ro = ctx->rollout;
+  if (!ro.valid) raise(TESS_ERR_STATE, "no recorded rollout on this context: call tess_rollout first");
+  bool same = ro.dim == mesh.dim && ro.dtype == mesh.dtype && ro.basis == basis;
+  for (int d = 0; d < mesh.dim; ++d) same = same && ro.dims[d] == mesh.dims[d];
+  if (!same) raise(TESS_ERR_ARGUMENT, "mesh, dtype or basis differ from the recorded rollout");
+  dispatch<RolloutAdjointOp>(basis, mesh, false, ctx, mesh, G, prm, adj);
+}
+
+}  // namespace tess

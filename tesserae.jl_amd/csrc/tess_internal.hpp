@@ -73,6 +73,17 @@ struct Profiler {
   size_t next_event = 0;
 };
 
+// The last tess_rollout of a context: checkpoints of (x, v, F, V, σ) every
+// `every` steps, replayed segment by segment by the adjoint.
+struct Rollout {
+  bool valid = false;
+  int nsteps = 0, every = 1;
+  int64_t n = 0;
+  int32_t dim = 0, dtype = 0, basis = 0;
+  int64_t dims[3] = {0, 0, 0};
+  tess_field logE{nullptr, 0, 0};
+};
+
 }  // namespace tess
 
 struct tess_ctx {
@@ -109,6 +120,9 @@ struct tess_ctx {
   int64_t last_active_nodes = -1;
   // SpGrid block activation scratch (occupied flags, active flags, scan)
   tess::Buffer sp_occ, sp_act, sp_scan;
+  // adjoint: per-tile Float64 partials, node adjoints, rollout checkpoints and replay
+  tess::Buffer adj_partials, adj_nodes, ro_checkpoints, ro_segment, ro_gradv;
+  tess::Rollout rollout;
 };
 
 namespace tess {
@@ -152,6 +166,13 @@ void run_g2p_sparse(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_
                     const tess_particles& P, const tess_weights* W, const tess_params& prm);
 void run_step_sparse(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, tess_sparsity& sp,
                      const tess_particles& P, const tess_params& prm, int nsteps, double reorder_threshold);
+// ---- adjoint (adjoint.hip) ----
+void run_step_adjoint(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+                      const tess_params& prm, const tess_adjoint& adj);
+void run_rollout(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+                 const tess_params& prm, int nsteps, int every);
+void run_rollout_adjoint(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G,
+                         const tess_params& prm, const tess_adjoint& adj);
 // Moves every particle field so that slot k receives particle order[k].
 void permute_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const uint32_t* order);
 

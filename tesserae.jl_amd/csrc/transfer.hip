@@ -1,1908 +1,8 @@
-// The README step on gfx950: tile binning, deterministic P2G, fused grid
-// update, G2P.  Reference: README.md:66-87; @P2G lowering
-// src/Transfer/p2g.jl:145-178 (hoisting :272-307), grid-node half :377-395;
-// @G2P lowering src/Transfer/g2p.jl:110-157; deferred weights
-// src/Transfer/runtime.jl:21-24,55-130.
-//
-// Data flow of one P2G (no global float atomics; every LDS add instruction
-// touches each node at most once, so the sums follow a fixed order):
-//   k_bin_keys      particle -> tile whose footprint holds its support window
-//   radix sort      stable (tile, particle id) order -> perm
-//   k_tile_runs / k_tile_flags / scan / k_active_list   tile ranges + slots
-//   k_nb_mark / scan / k_nb_list    node blocks touched by some footprint
-//   k_p2g_tile      one wave per tile: lanes own the 64 window-start classes of
-//                   the tile, accumulate their particles' contributions in
-//                   registers and add them into an LDS tile of the
-//                   (BW+3)^dim footprint (distinct targets per instruction),
-//                   then store the tile to a per-tile partial buffer
-//   k_gather_grid   one workgroup per node block: each node sums the <= 2^dim
-//                   tile partials covering it in a fixed order, then applies
-//                   the grid-node half (Float64 m⁻¹, vⁿ, v, gravity, slip walls)
-// G2P: k_g2p_tile stages the footprint's vⁿ, v in LDS and updates the tile's
-// particles; k_g2p_outside handles particles outside the mesh (empty window).
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cstring>
-
-#include "tess_device.hpp"
-#include "tess_internal.hpp"
+// Forward entry points of the README step (tess_p2g / tess_g2p / tess_step and
+// their SpGrid forms); the kernels and launchers live in transfer_impl.hpp.
+#include "transfer_impl.hpp"
 
 namespace tess {
-namespace {
-
-// Tile geometry: 64 window-start classes per tile in both dimensions.
-template <int D>
-struct Geo;
-template <>
-struct Geo<3> {
-  static constexpr int LT = 2, BW = 4, TS = 7, NFOOT = 343, NCLASS = 64, GNODES = 64;
-};
-template <>
-struct Geo<2> {
-  static constexpr int LT = 3, BW = 8, TS = 11, NFOOT = 121, NCLASS = 64, GNODES = 64;
-};
-
-template <int N, int E>
-struct Pow {
-  static constexpr int v = N * Pow<N, E - 1>::v;
-};
-template <int N>
-struct Pow<N, 0> {
-  static constexpr int v = 1;
-};
-
-// device counters: binning totals (zeroed per binning), error flags, and the
-// misplaced-particle count of the last P2G (zeroed per P2G)
-enum {
-  CNT_INSIDE = 0, CNT_ACTIVE = 1, CNT_NB = 2, CNT_ERR = 3, CNT_BIN_SLOTS = 4,
-  CNT_MISPLACED = 8, CNT_NODES = 9, CNT_STAY = 10, CNT_P2GERR = 11, CNT_SLOTS = 16
-};
-enum { ERR_GIMP_LENGTH = 1, ERR_WINDOW = 2, ERR_INACTIVE = 4 };
-
-template <typename T, int D>
-struct Args {
-  DMesh<T, D> mesh;
-  DField<T> x, v, V, gradv, F, sigma, l, logE;
-  DField<T> gm, gmv, gf, gvn, gv;
-  const int64_t* window;
-  DField<T> w, gw;
-  const uint32_t* perm;
-  const uint32_t* tile_start;
-  const uint32_t* tile_end;
-  const uint32_t* active;
-  const uint32_t* tile_flag;
-  const uint32_t* tile_slot;
-  const uint32_t* nb_list;
-  const uint32_t* nb_flag;
-  T* partials;
-  // SpGrid: block numbering (nullptr for a dense grid), block_size_log2, blocks per axis
-  const int32_t* numbering;
-  int32_t spL;
-  int32_t nblk[D];
-  uint32_t* p2g_err;
-  int64_t n, n_inside, n_active, n_nb;
-  int32_t ntiles[D];
-  T mp, dt;
-  double lambda, mu, V0;
-  T gravity[D];
-  int32_t slip;
-};
-
-// Workgroup i -> logical index; consecutive logical indices stay on one XCD
-// (workgroups are dealt to the 8 XCDs round-robin).
-__device__ __forceinline__ int64_t xcd_remap(int64_t i, int64_t N) {
-  const int64_t per = (N + 7) >> 3;
-  return (i & 7) * per + (i >> 3);
-}
-inline unsigned xcd_grid(int64_t N) { return unsigned(((N + 7) >> 3) << 3); }
-
-template <int D>
-__device__ __forceinline__ void unlin(int64_t l, const int32_t* dims, int* I) {
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    I[d] = int(l % dims[d]);
-    l /= dims[d];
-  }
-}
-
-// Element of grid node I in the caller's grid arrays: the column-major node
-// index of a dense grid, or on an SpGrid the SpArray storage index
-// (k-1)*2^(L*dim) + local of its block's number k (src/Grids/spindices.jl:89,
-// 96-102, 133-140), -1 when the block is inactive.
-template <typename T, int D>
-__device__ __forceinline__ int64_t grid_slot(const Args<T, D>& a, const int* I) {
-  if (a.numbering == nullptr) return a.mesh.lin(I);
-  const int L = a.spL;
-  int64_t b = 0, loc = 0;
-#pragma unroll
-  for (int d = D - 1; d >= 0; --d) {
-    b = b * a.nblk[d] + (I[d] >> L);
-    loc = (loc << L) | (I[d] & ((1 << L) - 1));
-  }
-  const int32_t k = a.numbering[b];
-  return k > 0 ? (int64_t(k - 1) << (L * D)) + loc : int64_t(-1);
-}
-
-// SpGrid: every node of a class window [lo, lo+sw-1] (clipped to the mesh)
-// must lie in an active block, the reference's "inactive SpGrid support node"
-// check (src/Transfer/runtime.jl:155-161).  The class window contains the
-// window of every particle of the class.
-template <typename T, int D, int SW>
-__device__ __forceinline__ void check_sparse_support(const Args<T, D>& a, const int* origin, const int* ccoord,
-                                                     int ccount) {
-  if (a.numbering == nullptr || ccount == 0) return;
-  const int L = a.spL;
-  int blo[D], ext[D], tot = 1;
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    const int lo = max(origin[d] + ccoord[d], 0);  // a class holding particles has lo >= 0
-    const int hi = min(lo + SW - 1, int(a.mesh.dims[d]) - 1);
-    blo[d] = lo >> L;
-    ext[d] = (hi >> L) - blo[d] + 1;
-    tot *= ext[d];
-  }
-  bool bad = false;
-  for (int k = 0; k < tot; ++k) {
-    int r = k;
-    int64_t b = 0, mul = 1;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      b += int64_t(blo[d] + r % ext[d]) * mul;
-      r /= ext[d];
-      mul *= a.nblk[d];
-    }
-    bad = bad || a.numbering[b] == 0;
-  }
-  if (bad) atomicOr(a.p2g_err, uint32_t(ERR_INACTIVE));
-}
-
-// Particle window (0-based, clamped) from its position (deferred weights) or
-// from stored weights; false if the window is empty.
-template <typename T, int D, int BASIS, bool STORED>
-__device__ __forceinline__ bool particle_window(const Args<T, D>& a, int64_t p, T* x, T& l, int* lo, int* hi,
-                                                uint32_t* err) {
-#pragma unroll
-  for (int d = 0; d < D; ++d) x[d] = a.x(p, d);
-  l = T(0);
-  if constexpr (BASIS == TESS_UGIMP) {
-    l = a.l(p) / a.mesh.h;  // _normalized_particle_length, gimp.jl:25
-  }
-  if constexpr (STORED) {
-    bool ok = true, bad = false;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      lo[d] = int(a.window[p * 2 * D + 2 * d]) - 1;
-      hi[d] = int(a.window[p * 2 * D + 2 * d + 1]) - 1;
-      ok = ok && hi[d] >= lo[d];
-      bad = bad || hi[d] - lo[d] >= support_width(BASIS) || lo[d] < 0 || hi[d] > a.mesh.dims[d] - 1;
-    }
-    if (ok && bad) {
-      if (err) atomicOr(err, uint32_t(ERR_WINDOW));
-      return false;
-    }
-    return ok;
-  } else {
-    T xi[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) xi[d] = (x[d] - a.mesh.xmin[d]) * a.mesh.h_inv;
-    if (!inside_xi<T, D>(xi, a.mesh)) return false;
-    if constexpr (BASIS == TESS_UGIMP) {
-      if (!(T(0) <= l && l <= T(1)) && err) atomicOr(err, uint32_t(ERR_GIMP_LENGTH));
-    }
-#pragma unroll
-    for (int d = 0; d < D; ++d) axis_window<T, BASIS>(xi[d], l, a.mesh.dims[d], lo[d], hi[d]);
-    return true;
-  }
-}
-
-// key = tile * 64 + class: the tile whose footprint [BW*t-1, BW*t+BW+1] holds
-// the particle's window, and the window start relative to the footprint
-// origin; ntiles*64 for particles with an empty window.
-template <typename T, int D, int BASIS, bool STORED>
-__device__ __forceinline__ uint32_t particle_key(const Args<T, D>& a, int64_t p, uint32_t ntiles, uint32_t* err) {
-  T x[D], l;
-  int lo[D], hi[D];
-  if (!particle_window<T, D, BASIS, STORED>(a, p, x, l, lo, hi, err)) return ntiles * 64u;
-  uint32_t t = 0, c = 0;
-#pragma unroll
-  for (int d = D - 1; d >= 0; --d) {
-    t = t * uint32_t(a.ntiles[d]) + uint32_t((lo[d] + 1) >> Geo<D>::LT);
-    c = c * uint32_t(Geo<D>::BW) + uint32_t((lo[d] + 1) & (Geo<D>::BW - 1));
-  }
-  return t * 64u + c;
-}
-
-template <typename T, int D, int BASIS, bool STORED>
-__global__ __launch_bounds__(256) void k_bin_keys(Args<T, D> a, uint32_t* __restrict__ keys, uint32_t ntiles,
-                                                  uint32_t* __restrict__ counters) {
-  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= a.n) return;
-  keys[p] = particle_key<T, D, BASIS, STORED>(a, p, ntiles, counters + CNT_ERR);
-}
-
-// ---- incremental re-binning ---------------------------------------------------
-// New keys are taken in the previous sorted order j; the new order is the
-// stable sort of that order by the new keys.  Particles whose key is unchanged
-// ("stayers") keep their relative order; the few "movers" are sorted among
-// themselves and merged.  The full radix sort of (new key, previous order) is
-// the fallback when many particles move; both give the same permutation.
-
-template <typename T, int D, int BASIS, bool STORED>
-__global__ __launch_bounds__(256) void k_keys_sorted(Args<T, D> a, const uint32_t* __restrict__ perm,
-                                                     const uint32_t* __restrict__ skeys, uint32_t ntiles,
-                                                     uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
-                                                     uint32_t* __restrict__ counters) {
-  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j > a.n) return;
-  if (j == a.n) {
-    stay[j] = 0;  // scan slot holding the stayer total
-    return;
-  }
-  const uint32_t k = particle_key<T, D, BASIS, STORED>(a, perm[j], ntiles, counters + CNT_ERR);
-  nk[j] = k;
-  stay[j] = k == skeys[j] ? 1u : 0u;
-}
-
-__global__ void k_compact_movers(const uint32_t* __restrict__ nk, const uint32_t* __restrict__ spos, int64_t n,
-                                 uint32_t* __restrict__ mkey, uint32_t* __restrict__ mj) {
-  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= n || spos[j + 1] != spos[j]) return;
-  const int64_t r = j - spos[j];
-  mkey[r] = nk[j];
-  mj[r] = uint32_t(j);
-}
-
-// number of i in [lo, hi) with (k[i], jj[i]) < (K, J) for lexicographically sorted pairs
-__device__ __forceinline__ int64_t lower_bound_pair(const uint32_t* k, const uint32_t* jj, int64_t lo, int64_t hi,
-                                                    uint32_t K, uint32_t J) {
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    const bool less = k[mid] < K || (k[mid] == K && jj[mid] < J);
-    if (less) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ int64_t bound_key(const uint32_t* k, int64_t n, uint32_t K, bool upper) {
-  int64_t lo = 0, hi = n;
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    const bool go = upper ? k[mid] <= K : k[mid] < K;
-    if (go) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-// Sorted mover i (key K, previous position j) lands after every stayer with a
-// smaller key or an equal key and smaller previous position, and after the
-// movers before it.
-__global__ void k_place_movers(const uint32_t* __restrict__ mks, const uint32_t* __restrict__ mr,
-                               const uint32_t* __restrict__ mj, const uint32_t* __restrict__ skeys_old,
-                               const uint32_t* __restrict__ perm_old, const uint32_t* __restrict__ spos, int64_t n,
-                               int64_t M, uint32_t* __restrict__ mjs, uint32_t* __restrict__ skeys_new,
-                               uint32_t* __restrict__ perm_new) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= M) return;
-  const uint32_t K = mks[i];
-  const int64_t j = mj[mr[i]];
-  mjs[i] = uint32_t(j);
-  const int64_t lb = bound_key(skeys_old, n, K, false), ub = bound_key(skeys_old, n, K, true);
-  const int64_t c = j < lb ? lb : (j > ub ? ub : j);
-  const int64_t pos = i + int64_t(spos[c]);
-  skeys_new[pos] = K;
-  perm_new[pos] = perm_old[j];
-}
-
-// Stayer j moves back by the movers removed before it and forward by the
-// movers inserted before it (sorted movers with a pair (key, j') < (key, j)).
-constexpr int STAY_BLOCK = 256, STAY_CAP = 2048;
-__global__ __launch_bounds__(STAY_BLOCK) void k_place_stayers(
-    const uint32_t* __restrict__ skeys_old, const uint32_t* __restrict__ perm_old, const uint32_t* __restrict__ spos,
-    int64_t n, const uint32_t* __restrict__ mks, const uint32_t* __restrict__ mjs, int64_t M,
-    uint32_t* __restrict__ skeys_new, uint32_t* __restrict__ perm_new) {
-  __shared__ uint32_t s_k[STAY_CAP], s_j[STAY_CAP];
-  __shared__ int64_t s_range[2];
-  const int64_t j0 = int64_t(blockIdx.x) * STAY_BLOCK;
-  const int64_t j1 = j0 + STAY_BLOCK < n ? j0 + STAY_BLOCK : n;
-  if (threadIdx.x == 0) {
-    s_range[0] = lower_bound_pair(mks, mjs, 0, M, skeys_old[j0], uint32_t(j0));
-    s_range[1] = lower_bound_pair(mks, mjs, s_range[0], M, skeys_old[j1 - 1], uint32_t(j1));
-  }
-  __syncthreads();
-  const int64_t a0 = s_range[0], cnt = s_range[1] - s_range[0];
-  const bool in_lds = cnt <= STAY_CAP;
-  if (in_lds)
-    for (int64_t t = threadIdx.x; t < cnt; t += STAY_BLOCK) {
-      s_k[t] = mks[a0 + t];
-      s_j[t] = mjs[a0 + t];
-    }
-  __syncthreads();
-  const int64_t j = j0 + threadIdx.x;
-  if (j >= j1 || spos[j + 1] == spos[j]) return;  // past the end, or a mover
-  const uint32_t K = skeys_old[j];
-  const int64_t ins = in_lds ? a0 + lower_bound_pair(s_k, s_j, 0, cnt, K, uint32_t(j))
-                             : lower_bound_pair(mks, mjs, a0, a0 + cnt, K, uint32_t(j));
-  const int64_t pos = int64_t(spos[j]) + ins;
-  skeys_new[pos] = K;
-  perm_new[pos] = perm_old[j];
-}
-
-__global__ void k_tile_runs(const uint32_t* __restrict__ skeys, int64_t n, uint32_t ntiles,
-                            uint32_t* __restrict__ tile_start, uint32_t* __restrict__ tile_end,
-                            uint32_t* __restrict__ counters) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t k = skeys[i] >> 6;  // tile
-  const bool first = i == 0 || (skeys[i - 1] >> 6) != k;
-  const bool last = i == n - 1 || (skeys[i + 1] >> 6) != k;
-  if (k < ntiles) {
-    if (first) tile_start[k] = uint32_t(i);
-    if (last) tile_end[k] = uint32_t(i + 1);
-    if (i == n - 1) counters[CNT_INSIDE] = uint32_t(n);
-  } else if (first) {
-    counters[CNT_INSIDE] = uint32_t(i);
-  }
-}
-
-__global__ void k_tile_flags(const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_end,
-                             uint32_t ntiles, uint32_t* __restrict__ flag, uint32_t* __restrict__ slot) {
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
-  const uint32_t f = tile_end[t] > tile_start[t] ? 1u : 0u;
-  flag[t] = f;
-  slot[t] = f;
-}
-
-__global__ void k_compact(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ slot, uint32_t count,
-                          uint32_t* __restrict__ list) {
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= count) return;
-  if (flag[t]) list[slot[t]] = uint32_t(t);
-}
-
-// Node block B holds nodes BW*B .. BW*B+BW-1 per axis; tile t's footprint
-// [BW*t-1, BW*t+BW+1] touches node blocks t-1, t, t+1.
-template <int D>
-__global__ void k_nb_mark(const uint32_t* __restrict__ tile_flag, uint32_t ntiles, int3 nt,
-                          uint32_t* __restrict__ nb_flag) {
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= ntiles || !tile_flag[t]) return;
-  const int dims[3] = {nt.x, nt.y, nt.z};
-  int I[3] = {0, 0, 0};
-  int64_t r = t;
-  for (int d = 0; d < D; ++d) {
-    I[d] = int(r % dims[d]);
-    r /= dims[d];
-  }
-  for (int dz = (D == 3 ? -1 : 0); dz <= (D == 3 ? 1 : 0); ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int J[3] = {I[0] + dx, I[1] + dy, I[2] + dz};
-        bool ok = true;
-        for (int d = 0; d < D; ++d) ok = ok && J[d] >= 0 && J[d] < dims[d];
-        if (!ok) continue;
-        int64_t l = J[D - 1];
-        for (int d = D - 2; d >= 0; --d) l = l * dims[d] + J[d];
-        nb_flag[l] = 1u;
-      }
-}
-
-__global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int64_t n) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[i];
-}
-
-template <typename T, int D>
-__device__ __forceinline__ void zero_node(const Args<T, D>& a, int64_t i) {
-  a.gm(i) = T(0);
-#pragma unroll
-  for (int c = 0; c < D; ++c) {
-    a.gmv(i, c) = T(0);
-    a.gf(i, c) = T(0);
-    a.gvn(i, c) = T(0);
-    a.gv(i, c) = T(0);
-  }
-}
-
-template <typename T, int D>
-__global__ __launch_bounds__(256) void k_zero_grid(Args<T, D> a, int64_t nnodes) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < nnodes) zero_node(a, i);
-}
-
-// Zero the nodes of node blocks touched last call but not this call.
-template <typename T, int D>
-__global__ __launch_bounds__(64) void k_zero_stale(Args<T, D> a, const uint32_t* __restrict__ prev_list,
-                                                    int64_t n_prev) {
-  using G = Geo<D>;
-  const int64_t j = blockIdx.x;
-  if (j >= n_prev) return;
-  const uint32_t B = prev_list[j];
-  if (a.nb_flag[B]) return;
-  int Bi[D], I[D];
-  unlin<D>(B, a.ntiles, Bi);
-  int r = threadIdx.x;
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    I[d] = Bi[d] * G::BW + r % G::BW;
-    r /= G::BW;
-    if (I[d] >= a.mesh.dims[d]) return;
-  }
-  const int64_t i = grid_slot(a, I);
-  if (i >= 0) zero_node(a, i);
-}
-
-template <typename T, int D, int BASIS>
-__device__ __forceinline__ void axis_jets(T xd, int lo_local, const T* s_nodex, T l, T hinv, T* val, T* der) {
-  constexpr int SW = support_width(BASIS);
-#pragma unroll
-  for (int a = 0; a < SW; ++a) {
-    const T xi = (xd - s_nodex[lo_local + a]) * hinv;  // separable_nodal_basis_jet, basis_weight.jl:49-60
-    piecewise_jet<T, BASIS>(xi, l, val[a], der[a]);
-  }
-}
-
-// ---- P2G -------------------------------------------------------------------
-
-__device__ __forceinline__ uint64_t lanemask_lt64() { return (uint64_t(1) << (threadIdx.x & 63)) - 1; }
-
-// Per-class counts of one tile from its slice of the sorted keys (class = low 6
-// bits), by wave-level matching; lane c gets count and offset of class c.
-template <int NW = 1>
-__device__ __forceinline__ void tile_class_counts(const uint32_t* __restrict__ skeys, uint32_t start, int cnt,
-                                                  uint32_t* s_cnt, int& ccount, int& coff, int& maxcnt) {
-  const int lane = threadIdx.x & 63, wave = int(threadIdx.x >> 6);
-  if (wave == 0) s_cnt[lane] = 0;
-  __syncthreads();
-  for (int base = wave * 64; base < cnt; base += 64 * NW) {
-    const int i = base + lane;
-    const bool valid = i < cnt;
-    const uint32_t c = valid ? (skeys[start + i] & 63u) : 0u;
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 6; ++b) {
-      const bool bit = (c >> b) & 1u;
-      const uint64_t m = __ballot(valid && bit);
-      peers &= bit ? m : ~m;
-    }
-    if (valid && (peers & lanemask_lt64()) == 0) atomicAdd(&s_cnt[c], uint32_t(__popcll(peers)));
-  }
-  __syncthreads();
-  ccount = int(s_cnt[lane]);
-  int incl = ccount;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int u = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += u;
-  }
-  coff = incl - ccount;
-  maxcnt = ccount;
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) maxcnt = max(maxcnt, __shfl_xor(maxcnt, off, 64));
-}
-
-// One wave per tile.  The sort key orders each tile's particles by class
-// (window start relative to the footprint origin), so lane c owns class c: it
-// walks its particles, accumulates their contributions to the sw^dim nodes of
-// the class window in registers, and adds them into the LDS tile.  Lanes hold
-// distinct classes, so every LDS add instruction touches each node at most once
-// and the per-node order of the adds is the fixed program order.
-template <typename T, int D, int BASIS, bool STORED, int NPASS>
-__global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a, const uint32_t* __restrict__ skeys,
-                                                  uint32_t* __restrict__ misplaced) {
-  using G = Geo<D>;
-  constexpr int SW = support_width(BASIS);
-  constexpr int NV = 1 + 2 * D;
-  constexpr int NOFF = Pow<SW, D>::v;
-  constexpr int NSL = NOFF / NPASS;  // offsets per pass (NPASS is 1 or SW)
-  const int lane = threadIdx.x;
-  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
-  if (j >= a.n_active) return;
-  const uint32_t tile = a.active[j];
-  const uint32_t start = a.tile_start[tile];
-  const int cnt = int(a.tile_end[tile] - start);
-  int tI[D], origin[D];
-  unlin<D>(tile, a.ntiles, tI);
-#pragma unroll
-  for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
-
-  __shared__ __align__(16) T s_tile[G::NFOOT][8];
-  __shared__ T s_nodex[D][G::TS];
-  __shared__ uint32_t s_cnt[64];
-  for (int q = lane; q < 8 * G::NFOOT; q += 64) (&s_tile[0][0])[q] = T(0);
-  for (int q = lane; q < D * G::TS; q += 64) {
-    const int d = q / G::TS, k = q % G::TS;
-    s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
-  }
-  int ccount, coff, maxcnt;
-  tile_class_counts(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);
-  int ccoord[D];
-  {
-    int r = lane;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      ccoord[d] = r % G::BW;
-      r /= G::BW;
-    }
-  }
-  check_sparse_support<T, D, SW>(a, origin, ccoord, ccount);
-  // particles whose storage index differs from the layered position
-  // (k-th particle of every class contiguous, classes ascending)
-  if (misplaced) {
-    uint32_t bad = 0, layer_off = 0;
-    for (int k = 0; k < maxcnt; ++k) {
-      const uint64_t has = __ballot(k < ccount);
-      if (k < ccount) {
-        const uint32_t want = start + layer_off + uint32_t(__popcll(has & lanemask_lt64()));
-        bad += a.perm[start + coff + k] != want ? 1u : 0u;
-      }
-      layer_off += uint32_t(__popcll(has));
-    }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) bad += __shfl_xor(bad, off, 64);
-    if (lane == 0) misplaced[j] = bad;  // per tile, summed later
-  }
-  const T mp = a.mp, hinv = a.mesh.h_inv;
-  for (int pass = 0; pass < NPASS; ++pass) {
-    T acc[NSL][NV];
-#pragma unroll
-    for (int s2 = 0; s2 < NSL; ++s2)
-#pragma unroll
-      for (int v = 0; v < NV; ++v) acc[s2][v] = T(0);
-    for (int k = 0; k < maxcnt; ++k) {
-      if (k >= ccount) continue;
-      const uint32_t pid = a.perm[start + coff + k];
-      T x[D], l;
-      int lo[D], hi[D];
-      particle_window<T, D, BASIS, STORED>(a, pid, x, l, lo, hi, nullptr);
-      int ext[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) ext[d] = hi[d] - lo[d];
-      // hoisted particle factors (p2g.jl:272-307): mₚ*v[p] and (-V[p])*σ[p]
-      T mv[D], A[D * D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) mv[d] = mp * a.v(pid, d);
-      const T negV = -a.V(pid);
-#pragma unroll
-      for (int q = 0; q < D * D; ++q) A[q] = negV * a.sigma(pid, q);
-      T val[D][SW], der[D][SW];
-      if constexpr (!STORED) {
-#pragma unroll
-        for (int d = 0; d < D; ++d) axis_jets<T, D, BASIS>(x[d], ccoord[d], s_nodex[d], l, hinv, val[d], der[d]);
-      }
-#pragma unroll
-      for (int s2 = 0; s2 < NSL; ++s2) {
-        const int o = pass * NSL + s2;
-        int aa[D];
-        int r = o;
-        bool ok = true;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          aa[d] = r % SW;
-          r /= SW;
-          ok = ok && aa[d] <= ext[d];
-        }
-        if (!ok) continue;
-        T w, g[D];
-        if constexpr (STORED) {
-          const int64_t e = int64_t(o) + int64_t(NOFF) * pid;
-          w = a.w(e);
-#pragma unroll
-          for (int c = 0; c < D; ++c) g[c] = a.gw(e, c);
-        } else {
-          // w = Π_d N_d (left-assoc), ∇w_c = (N'_c Π_{d≠c} N_d) h⁻¹
-          w = val[0][aa[0]];
-#pragma unroll
-          for (int d = 1; d < D; ++d) w = w * val[d][aa[d]];
-#pragma unroll
-          for (int c = 0; c < D; ++c) {
-            T gg = (c == 0) ? der[0][aa[0]] : val[0][aa[0]];
-#pragma unroll
-            for (int d = 1; d < D; ++d) gg = gg * ((d == c) ? der[d][aa[d]] : val[d][aa[d]]);
-            g[c] = gg * hinv;
-          }
-        }
-        acc[s2][0] += w * mp;
-#pragma unroll
-        for (int c = 0; c < D; ++c) acc[s2][1 + c] += w * mv[c];
-#pragma unroll
-        for (int r2 = 0; r2 < D; ++r2) {
-          T f = A[r2] * g[0];
-#pragma unroll
-          for (int c = 1; c < D; ++c) f = f + A[r2 + D * c] * g[c];
-          acc[s2][1 + D + r2] += f;
-        }
-      }
-    }
-    if (ccount > 0) {
-#pragma unroll
-      for (int s2 = 0; s2 < NSL; ++s2) {
-        int r = pass * NSL + s2, q = 0, mul = 1;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          q += (ccoord[d] + r % SW) * mul;
-          r /= SW;
-          mul *= G::TS;
-        }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) atomicAdd(&s_tile[q][v], acc[s2][v]);
-      }
-    }
-  }
-  __syncthreads();
-  T* out = a.partials + int64_t(j) * 8 * G::NFOOT;
-  for (int q = lane; q < 8 * G::NFOOT; q += 64) out[q] = (&s_tile[0][0])[q];
-}
-
-template <typename T, int D>
-__device__ __forceinline__ double det_d(const T* F) {
-  if constexpr (D == 2) {
-    return double(F[0]) * double(F[3]) - double(F[2]) * double(F[1]);
-  } else {
-    const double f11 = F[0], f21 = F[1], f31 = F[2], f12 = F[3], f22 = F[4], f32 = F[5], f13 = F[6], f23 = F[7],
-                 f33 = F[8];
-    return f11 * (f22 * f33 - f23 * f32) - f12 * (f21 * f33 - f23 * f31) + f13 * (f21 * f32 - f22 * f31);
-  }
-}
-
-// ---- deferred-weight fast paths ------------------------------------------------
-//
-// Both transfer kernels below evaluate the basis on the fly.  Along an axis
-// whose window is not truncated they use the reference's full-support
-// polynomial (bspline.jl:90-100,146-170); truncated axes and uGIMP use the
-// nodewise piecewise jet (basis_weight.jl:49-60,624-632).  Tensor-product sums
-// are factored by axis (products of the y/z jets are formed once per (y,z)
-// offset and reused along x); the results equal the reference's up to
-// floating-point rounding.
-
-// Loads component c of element e: Julia-native AoS fields (component stride 1,
-// element stride NC) get compile-time strides so neighbouring components
-// merge into vector loads.
-template <typename T, int NC, bool AOS>
-__device__ __forceinline__ T fld(const DField<T>& f, int64_t e, int c) {
-  if constexpr (AOS) return f.p[e * NC + c];
-  else return f(e, c);
-}
-template <typename T, int NC, bool AOS>
-__device__ __forceinline__ T& fldw(const DField<T>& f, int64_t e, int c) {
-  if constexpr (AOS) return f.p[e * NC + c];
-  else return f(e, c);
-}
-
-// 1-D jets at the window nodes lo..lo+SW-1 of one axis; entries past the
-// (truncated) window are zero.  Derivatives are returned times h⁻¹.
-template <typename T, int BASIS>
-__device__ __forceinline__ void axis_jets_mixed(T xd, T xmin, int lo_local, int ext, const T* s_nodex, T l, T hinv,
-                                                T* val, T* der) {
-  constexpr int SW = support_width(BASIS);
-  bool done = false;
-  if constexpr (BASIS != TESS_UGIMP) {
-    if (ext == SW - 1) {
-      const T xi = (xd - xmin) * hinv;
-      const T y = (degree_of(BASIS) % 2 == 1) ? xi : xi - T(0.5);
-      bspline_local_jet<T, BASIS>(y - floor(y), val, der);
-      done = true;
-    }
-  }
-  if (!done) {
-#pragma unroll
-    for (int q = 0; q < SW; ++q) {
-      T v0 = T(0), d0 = T(0);
-      if (q <= ext) piecewise_jet<T, BASIS>((xd - s_nodex[lo_local + q]) * hinv, l, v0, d0);
-      val[q] = v0;
-      der[q] = d0;
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < SW; ++q) der[q] *= hinv;
-}
-
-template <typename T, int D, int BASIS, int NPASS, bool AOS>
-__global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
-                                                  uint32_t* __restrict__ misplaced) {
-  using G = Geo<D>;
-  constexpr int SW = support_width(BASIS);
-  constexpr int NV = 1 + 2 * D;
-  constexpr int KN = SW / NPASS;                       // last-axis offsets per pass
-  constexpr int NSL = Pow<SW, D - 1>::v * KN;          // offsets per pass
-  const int lane = threadIdx.x;
-  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
-  if (j >= a.n_active) return;
-  const uint32_t tile = a.active[j];
-  const uint32_t start = a.tile_start[tile];
-  const int cnt = int(a.tile_end[tile] - start);
-  int tI[D], origin[D];
-  unlin<D>(tile, a.ntiles, tI);
-#pragma unroll
-  for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
-
-  // node-major tile, 8 slots per node: m, mv[D], f[D], padding
-  __shared__ __align__(16) T s_tile[G::NFOOT][8];
-  __shared__ T s_nodex[D][G::TS];
-  __shared__ uint32_t s_cnt[64];
-  for (int q = lane; q < 8 * G::NFOOT; q += 64) (&s_tile[0][0])[q] = T(0);
-  for (int q = lane; q < D * G::TS; q += 64) {
-    const int d = q / G::TS, k = q % G::TS;
-    s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
-  }
-  int ccount, coff, maxcnt;
-  tile_class_counts(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);
-  int ccoord[D];
-  {
-    int r = lane;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      ccoord[d] = r % G::BW;
-      r /= G::BW;
-    }
-  }
-  check_sparse_support<T, D, SW>(a, origin, ccoord, ccount);
-  const T mp = a.mp, hinv = a.mesh.h_inv;
-  uint32_t bad = 0, layer_off = 0;
-  for (int pass = 0; pass < NPASS; ++pass) {
-    T am[NSL], amv[D][NSL], af[D][NSL];
-#pragma unroll
-    for (int o = 0; o < NSL; ++o) {
-      am[o] = T(0);
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        amv[c][o] = T(0);
-        af[c][o] = T(0);
-      }
-    }
-    for (int k = 0; k < maxcnt; ++k) {
-      const uint64_t has = __ballot(k < ccount);
-      if (k < ccount) {
-        const uint32_t pid = a.perm[start + coff + k];
-        if (misplaced && pass == 0) {
-          const uint32_t want = start + layer_off + uint32_t(__popcll(has & lanemask_lt64()));
-          bad += pid != want ? 1u : 0u;
-        }
-        T x[D], mpv[D], A[D * D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) x[d] = fld<T, D, AOS>(a.x, pid, d);
-#pragma unroll
-        for (int d = 0; d < D; ++d) mpv[d] = mp * fld<T, D, AOS>(a.v, pid, d);  // hoisted mₚ*v[p]
-        const T negV = -a.V(pid);
-#pragma unroll
-        for (int q = 0; q < D * D; ++q) A[q] = negV * fld<T, D * D, AOS>(a.sigma, pid, q);  // hoisted (-V[p])*σ[p]
-        T l = T(0);
-        if constexpr (BASIS == TESS_UGIMP) l = a.l(pid) / a.mesh.h;
-        T val[D][SW], der[D][SW];
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const T xi = (x[d] - a.mesh.xmin[d]) * hinv;
-          int lo, hi;
-          axis_window<T, BASIS>(xi, l, a.mesh.dims[d], lo, hi);
-          axis_jets_mixed<T, BASIS>(x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], l, hinv, val[d], der[d]);
-        }
-        if constexpr (D == 3) {
-#pragma unroll
-          for (int kz = 0; kz < KN; ++kz) {
-            const int kg = pass * KN + kz;
-#pragma unroll
-            for (int jy = 0; jy < SW; ++jy) {
-              const T YZ = val[1][jy] * val[2][kg];
-              const T YdZ = der[1][jy] * val[2][kg];
-              const T YZd = val[1][jy] * der[2][kg];
-              const T cm = YZ * mp;
-              T cmv[3], U[3], Wv[3];
-#pragma unroll
-              for (int c = 0; c < 3; ++c) {
-                cmv[c] = YZ * mpv[c];
-                U[c] = A[c] * YZ;
-                Wv[c] = A[c + 3] * YdZ + A[c + 6] * YZd;
-              }
-#pragma unroll
-              for (int ix = 0; ix < SW; ++ix) {
-                const int o = ix + SW * (jy + SW * kz);
-                am[o] += val[0][ix] * cm;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                  amv[c][o] += val[0][ix] * cmv[c];
-                  af[c][o] += der[0][ix] * U[c] + val[0][ix] * Wv[c];
-                }
-              }
-            }
-          }
-        } else {
-#pragma unroll
-          for (int jy = 0; jy < KN; ++jy) {
-            const int jg = pass * KN + jy;
-            const T Y = val[1][jg], Yd = der[1][jg];
-            const T cm = Y * mp;
-            T cmv[2], U[2], Wv[2];
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-              cmv[c] = Y * mpv[c];
-              U[c] = A[c] * Y;
-              Wv[c] = A[c + 2] * Yd;
-            }
-#pragma unroll
-            for (int ix = 0; ix < SW; ++ix) {
-              const int o = ix + SW * jy;
-              am[o] += val[0][ix] * cm;
-#pragma unroll
-              for (int c = 0; c < 2; ++c) {
-                amv[c][o] += val[0][ix] * cmv[c];
-                af[c][o] += der[0][ix] * U[c] + val[0][ix] * Wv[c];
-              }
-            }
-          }
-        }
-      }
-      if (pass == 0) layer_off += uint32_t(__popcll(has));
-    }
-    // lanes are distinct classes, so for one offset their target nodes are
-    // distinct; offsets are applied one after another (read, add, write, wait
-    // for the LDS queue), which fixes the per-node order of the additions
-    if (ccount > 0) {
-#pragma unroll
-      for (int o = 0; o < NSL; ++o) {
-        int r = o, q = 0, mul = 1;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const int off = (d == D - 1) ? pass * KN + r : r % SW;
-          q += (ccoord[d] + off) * mul;
-          r /= SW;
-          mul *= G::TS;
-        }
-        T t[8];
-#pragma unroll
-        for (int v = 0; v < 8; ++v) t[v] = s_tile[q][v];
-        t[0] += am[o];
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-          t[1 + c] += amv[c][o];
-          t[1 + D + c] += af[c][o];
-        }
-#pragma unroll
-        for (int v = 0; v < 8; ++v) s_tile[q][v] = t[v];
-        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-      }
-    }
-  }
-  if (misplaced) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) bad += __shfl_xor(bad, off, 64);
-    if (lane == 0) misplaced[j] = bad;  // per tile, summed later
-  }
-  __syncthreads();
-  T* out = a.partials + int64_t(j) * 8 * G::NFOOT;
-  for (int q = lane; q < 8 * G::NFOOT; q += 64) out[q] = (&s_tile[0][0])[q];
-}
-
-// G2P with the footprint's v and v - vⁿ staged per node in LDS; the FLIP
-// increment is formed from the node differences as the reference does.  NW
-// waves share one tile: wave w walks the class-major order at k = w, w+NW, ...
-// and prefetches its next particle while updating the current one.
-template <typename T, int D, bool AOS>
-struct G2PIn {
-  uint32_t pid;
-  T x[D], v[D], F[D * D], l;
-};
-
-template <typename T, int D, int BASIS, bool AOS>
-__device__ __forceinline__ void g2p_load(const Args<T, D>& a, uint32_t pid, G2PIn<T, D, AOS>& in) {
-  in.pid = pid;
-#pragma unroll
-  for (int c = 0; c < D; ++c) in.x[c] = fld<T, D, AOS>(a.x, pid, c);
-#pragma unroll
-  for (int c = 0; c < D; ++c) in.v[c] = fld<T, D, AOS>(a.v, pid, c);
-#pragma unroll
-  for (int q = 0; q < D * D; ++q) in.F[q] = fld<T, D * D, AOS>(a.F, pid, q);
-  in.l = T(0);
-  if constexpr (BASIS == TESS_UGIMP) in.l = a.l(pid) / a.mesh.h;
-}
-
-template <typename T, int D, int BASIS, bool AOS, int NW>
-__global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys) {
-  using G = Geo<D>;
-  constexpr int SW = support_width(BASIS);
-  const int lane = threadIdx.x & 63, wave = int(threadIdx.x >> 6);
-  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
-  if (j >= a.n_active) return;
-  const uint32_t tile = a.active[j];
-  const uint32_t start = a.tile_start[tile];
-  const int cnt = int(a.tile_end[tile] - start);
-  int tI[D], origin[D];
-  unlin<D>(tile, a.ntiles, tI);
-#pragma unroll
-  for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
-  // per node: v[D] and v-vⁿ[D], padded to a power-of-two stride for vector LDS reads
-  constexpr int NS = (D == 3) ? 8 : 4;
-  __shared__ __align__(16) T s_n[G::NFOOT][NS];
-  __shared__ T s_nodex[D][G::TS];
-  __shared__ uint32_t s_cnt[64];
-  for (int q = int(threadIdx.x); q < G::NFOOT; q += 64 * NW) {
-    int I[D], r = q;
-    bool in = true;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      I[d] = origin[d] + r % G::TS;
-      r /= G::TS;
-      in = in && I[d] >= 0 && I[d] < a.mesh.dims[d];
-    }
-    const int64_t i = in ? grid_slot(a, I) : -1;
-    in = i >= 0;
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      const T vv = in ? a.gv(i, c) : T(0);
-      const T vn = in ? a.gvn(i, c) : T(0);
-      s_n[q][c] = vv;
-      s_n[q][NS / 2 + c] = vv - vn;
-    }
-  }
-  for (int q = int(threadIdx.x); q < D * G::TS; q += 64 * NW) {
-    const int d = q / G::TS, k = q % G::TS;
-    s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
-  }
-  int ccount, coff, maxcnt;
-  tile_class_counts<NW>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);  // barrier after the staging
-  int ccoord[D];
-  {
-    int r = lane;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      ccoord[d] = r % G::BW;
-      r /= G::BW;
-    }
-  }
-  const T hinv = a.mesh.h_inv, dt = a.dt;
-  G2PIn<T, D, AOS> cur, nxt;
-  if (wave < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + wave], nxt);
-  for (int k = wave; k < maxcnt; k += NW) {
-    cur = nxt;
-    if (k + NW < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + NW], nxt);
-    if (k >= ccount) continue;
-    T val[D][SW], der[D][SW];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const T xi = (cur.x[d] - a.mesh.xmin[d]) * hinv;
-      int lo, hi;
-      axis_window<T, BASIS>(xi, cur.l, a.mesh.dims[d], lo, hi);
-      axis_jets_mixed<T, BASIS>(cur.x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], cur.l, hinv, val[d],
-                                der[d]);
-    }
-    // X = Σ w v_i, DV = Σ w (v_i - vⁿ_i), L = Σ v_i ⊗ ∇w, factored by axis
-    T X[D], DV[D], L[D * D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      X[c] = T(0);
-      DV[c] = T(0);
-    }
-#pragma unroll
-    for (int q = 0; q < D * D; ++q) L[q] = T(0);
-    if constexpr (D == 3) {
-#pragma unroll
-      for (int kz = 0; kz < SW; ++kz)
-#pragma unroll
-        for (int jy = 0; jy < SW; ++jy) {
-          T S1[3] = {T(0), T(0), T(0)}, S2[3] = {T(0), T(0), T(0)}, Sd[3] = {T(0), T(0), T(0)};
-#pragma unroll
-          for (int ix = 0; ix < SW; ++ix) {
-            const int q = (ccoord[0] + ix) + G::TS * ((ccoord[1] + jy) + G::TS * (ccoord[2] + kz));
-            T nd[NS];
-#pragma unroll
-            for (int u = 0; u < NS; ++u) nd[u] = s_n[q][u];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              S1[c] += val[0][ix] * nd[c];
-              S2[c] += der[0][ix] * nd[c];
-              Sd[c] += val[0][ix] * nd[NS / 2 + c];
-            }
-          }
-          const T YZ = val[1][jy] * val[2][kz];
-          const T YdZ = der[1][jy] * val[2][kz];
-          const T YZd = val[1][jy] * der[2][kz];
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            X[c] += YZ * S1[c];
-            DV[c] += YZ * Sd[c];
-            L[c] += YZ * S2[c];
-            L[c + 3] += YdZ * S1[c];
-            L[c + 6] += YZd * S1[c];
-          }
-        }
-    } else {
-#pragma unroll
-      for (int jy = 0; jy < SW; ++jy) {
-        T S1[2] = {T(0), T(0)}, S2[2] = {T(0), T(0)}, Sd[2] = {T(0), T(0)};
-#pragma unroll
-        for (int ix = 0; ix < SW; ++ix) {
-          const int q = (ccoord[0] + ix) + G::TS * (ccoord[1] + jy);
-          T nd[NS];
-#pragma unroll
-          for (int u = 0; u < NS; ++u) nd[u] = s_n[q][u];
-#pragma unroll
-          for (int c = 0; c < 2; ++c) {
-            S1[c] += val[0][ix] * nd[c];
-            S2[c] += der[0][ix] * nd[c];
-            Sd[c] += val[0][ix] * nd[NS / 2 + c];
-          }
-        }
-        const T Y = val[1][jy], Yd = der[1][jy];
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          X[c] += Y * S1[c];
-          DV[c] += Y * Sd[c];
-          L[c] += Y * S2[c];
-          L[c + 2] += Yd * S1[c];
-        }
-      }
-    }
-    const uint32_t pid = cur.pid;
-    // particle equations (README.md:81-86)
-#pragma unroll
-    for (int c = 0; c < D; ++c) fldw<T, D, AOS>(a.v, pid, c) = cur.v[c] + DV[c];
-#pragma unroll
-    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.gradv, pid, q) = L[q];
-#pragma unroll
-    for (int c = 0; c < D; ++c) fldw<T, D, AOS>(a.x, pid, c) = cur.x[c] + X[c] * dt;
-    T Fn[D * D];
-#pragma unroll
-    for (int b = 0; b < D; ++b)
-#pragma unroll
-      for (int r = 0; r < D; ++r) {
-        T t = L[r] * cur.F[D * b];  // (I + Δt∇v) F = F + Δt (∇v F)
-#pragma unroll
-        for (int kk = 1; kk < D; ++kk) t += L[r + D * kk] * cur.F[kk + D * b];
-        Fn[r + D * b] = cur.F[r + D * b] + dt * t;
-      }
-#pragma unroll
-    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.F, pid, q) = Fn[q];
-    const double J = det_d<T, D>(Fn);
-    double lam = a.lambda, mu = a.mu;
-    if (a.logE.p) {
-      const double sc = exp(double(a.logE(pid)));
-      lam *= sc;
-      mu *= sc;
-    }
-    const double lnJ = log(J);
-    const double invJ = 1.0 / J;
-#pragma unroll
-    for (int b = 0; b < D; ++b)
-#pragma unroll
-      for (int r = 0; r < D; ++r) {
-        double bb = 0.0;
-#pragma unroll
-        for (int kk = 0; kk < D; ++kk) bb += double(Fn[r + D * kk]) * double(Fn[b + D * kk]);
-        const double sg = mu * (bb - (r == b ? 1.0 : 0.0)) + (r == b ? lam * lnJ : 0.0);
-        fldw<T, D * D, AOS>(a.sigma, pid, r + D * b) = T(sg * invJ);
-      }
-    a.V(pid) = T(a.V0 * J);
-  }
-}
-
-// ---- gather + grid-node half ------------------------------------------------
-template <typename T, int D, bool GAOS>
-__global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __restrict__ nodes_per_block) {
-  using G = Geo<D>;
-  constexpr int NV = 1 + 2 * D;
-  constexpr int NB = (D == 3) ? 27 : 9;
-  const int64_t j = xcd_remap(blockIdx.x, a.n_nb);
-  if (j >= a.n_nb) return;
-  const uint32_t B = a.nb_list[j];
-  int Bi[D], I[D], rr[D];
-  unlin<D>(B, a.ntiles, Bi);
-  // slots of the 3^dim neighbour tiles (-1: no partial)
-  __shared__ int s_slot[NB];
-  if (threadIdx.x < NB) {
-    int r = int(threadIdx.x), t = 0, mul = 1;
-    bool ok = true;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const int b = Bi[d] + (r % 3) - 1;
-      r /= 3;
-      ok = ok && b >= 0 && b < a.ntiles[d];
-      t += b * mul;
-      mul *= a.ntiles[d];
-    }
-    s_slot[threadIdx.x] = (ok && a.tile_flag[t]) ? int(a.tile_slot[t]) : -1;
-  }
-  __syncthreads();
-  {
-    int r = threadIdx.x;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      rr[d] = r % G::BW;
-      r /= G::BW;
-      I[d] = Bi[d] * G::BW + rr[d];
-      if (I[d] >= a.mesh.dims[d]) return;
-    }
-  }
-  T acc[NV];
-#pragma unroll
-  for (int v = 0; v < NV; ++v) acc[v] = T(0);
-  // tiles covering this node: B-1 if r<=1, B, B+1 if r==BW-1 (per axis),
-  // summed in a fixed order (last axis outermost)
-  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    lo[d] = rr[d] <= 1 ? -1 : 0;
-    hi[d] = rr[d] == G::BW - 1 ? 1 : 0;
-  }
-  int o[3] = {0, 0, 0};
-  for (o[2] = lo[2]; o[2] <= hi[2]; ++o[2])
-    for (o[1] = lo[1]; o[1] <= hi[1]; ++o[1])
-      for (o[0] = lo[0]; o[0] <= hi[0]; ++o[0]) {
-        int nb = 0, q = 0, mulb = 1, mulq = 1;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          nb += (o[d] + 1) * mulb;
-          mulb *= 3;
-          q += (rr[d] + 1 - o[d] * G::BW) * mulq;  // node index inside that tile's footprint
-          mulq *= G::TS;
-        }
-        const int slot = s_slot[nb];
-        if (slot < 0) continue;
-        const T* P = a.partials + (int64_t(slot) * G::NFOOT + q) * 8;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) acc[v] += P[v];
-      }
-  const int64_t i = grid_slot(a, I);
-  // grid-node half with the guarded Float64 division (test/transfer.jl:102-104)
-  const double md = double(acc[0]);
-  const bool has = md > 0.0;
-  {
-    // nodes with m > 0 in this block (one plain store per block, summed later)
-    const uint64_t m = __ballot(has);
-    if ((threadIdx.x & 63) == __ffsll((unsigned long long)__ballot(true)) - 1)
-      nodes_per_block[j] = uint32_t(__popcll(m));
-  }
-  const double minv = has ? 1.0 / md : 0.0;
-  if (i < 0) return;  // SpGrid node of an inactive block: no storage (and no support, checked by P2G)
-  a.gm(i) = acc[0];
-  T mvo[D], fo[D], vno[D], vo[D];
-#pragma unroll
-  for (int c = 0; c < D; ++c) {
-    const T mvc = acc[1 + c], fc = acc[1 + D + c];
-    const T vn = T(double(mvc) * minv);
-    const T accel = T(double(fc) * minv) + (has ? a.gravity[c] : T(0));
-    T vc = vn + accel * a.dt;
-    const bool wall = (((a.slip >> (2 * c)) & 1) && I[c] == 0) ||
-                      (((a.slip >> (2 * c + 1)) & 1) && I[c] == a.mesh.dims[c] - 1);
-    if (wall) vc = T(0);
-    mvo[c] = mvc;
-    fo[c] = fc;
-    vno[c] = vn;
-    vo[c] = vc;
-  }
-#pragma unroll
-  for (int c = 0; c < D; ++c) {
-    fldw<T, D, GAOS>(a.gmv, i, c) = mvo[c];
-    fldw<T, D, GAOS>(a.gf, i, c) = fo[c];
-    fldw<T, D, GAOS>(a.gvn, i, c) = vno[c];
-    fldw<T, D, GAOS>(a.gv, i, c) = vo[c];
-  }
-}
-
-// ---- G2P -------------------------------------------------------------------
-
-// Particle equations of the README @G2P block (README.md:81-86) after the sums.
-template <typename T, int D>
-__device__ __forceinline__ void particle_update(const Args<T, D>& a, int64_t p, const T* dv, const T* L,
-                                                const T* dx) {
-  const T dt = a.dt;
-#pragma unroll
-  for (int c = 0; c < D; ++c) a.v(p, c) += dv[c];
-#pragma unroll
-  for (int k = 0; k < D * D; ++k) a.gradv(p, k) = L[k];
-#pragma unroll
-  for (int c = 0; c < D; ++c) a.x(p, c) += dx[c];
-  T Fo[D * D], M[D * D], Fn[D * D];
-#pragma unroll
-  for (int k = 0; k < D * D; ++k) Fo[k] = a.F(p, k);
-#pragma unroll
-  for (int b = 0; b < D; ++b)
-#pragma unroll
-    for (int r = 0; r < D; ++r) M[r + D * b] = (r == b ? T(1) : T(0)) + L[r + D * b] * dt;
-#pragma unroll
-  for (int b = 0; b < D; ++b)
-#pragma unroll
-    for (int r = 0; r < D; ++r) {
-      T s = M[r] * Fo[D * b];
-#pragma unroll
-      for (int k = 1; k < D; ++k) s = s + M[r + D * k] * Fo[k + D * b];
-      Fn[r + D * b] = s;
-    }
-#pragma unroll
-  for (int k = 0; k < D * D; ++k) a.F(p, k) = Fn[k];
-  const double J = det_d<T, D>(Fn);
-  double lam = a.lambda, mu = a.mu;
-  if (a.logE.p) {
-    const double s = exp(double(a.logE(p)));
-    lam *= s;
-    mu *= s;
-  }
-  const double lnJ = log(J);
-  const double invJ = 1.0 / J;
-#pragma unroll
-  for (int b = 0; b < D; ++b)
-#pragma unroll
-    for (int r = 0; r < D; ++r) {
-      double bb = 0.0;
-#pragma unroll
-      for (int k = 0; k < D; ++k) bb += double(Fn[r + D * k]) * double(Fn[b + D * k]);
-      const double s = mu * (bb - (r == b ? 1.0 : 0.0)) + (r == b ? lam * lnJ : 0.0);
-      a.sigma(p, r + D * b) = T(s * invJ);
-    }
-  a.V(p) = T(a.V0 * J);
-}
-
-// One wave per tile, lanes own the tile's classes as in P2G (the walk is
-// coalesced when particles are stored in the layered order).  The footprint's
-// vⁿ and v are staged in LDS once per tile.
-template <typename T, int D, int BASIS, bool STORED>
-__global__ __launch_bounds__(64) void k_g2p_tile(Args<T, D> a, const uint32_t* __restrict__ skeys) {
-  using G = Geo<D>;
-  constexpr int SW = support_width(BASIS);
-  const int lane = threadIdx.x;
-  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
-  if (j >= a.n_active) return;
-  const uint32_t tile = a.active[j];
-  const uint32_t start = a.tile_start[tile];
-  const int cnt = int(a.tile_end[tile] - start);
-  int tI[D], origin[D];
-  unlin<D>(tile, a.ntiles, tI);
-#pragma unroll
-  for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
-  __shared__ T s_vn[D][G::NFOOT];
-  __shared__ T s_v[D][G::NFOOT];
-  __shared__ T s_nodex[D][G::TS];
-  __shared__ uint32_t s_cnt[64];
-  for (int q = lane; q < G::NFOOT; q += 64) {
-    int I[D], r = q;
-    bool in = true;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      I[d] = origin[d] + r % G::TS;
-      r /= G::TS;
-      in = in && I[d] >= 0 && I[d] < a.mesh.dims[d];
-    }
-    const int64_t i = in ? grid_slot(a, I) : -1;
-    in = i >= 0;
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      s_vn[c][q] = in ? a.gvn(i, c) : T(0);
-      s_v[c][q] = in ? a.gv(i, c) : T(0);
-    }
-  }
-  for (int q = lane; q < D * G::TS; q += 64) {
-    const int d = q / G::TS, k = q % G::TS;
-    s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
-  }
-  int ccount, coff, maxcnt;
-  tile_class_counts(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);  // contains the barrier after staging
-  int ccoord[D];
-  {
-    int r = lane;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      ccoord[d] = r % G::BW;
-      r /= G::BW;
-    }
-  }
-  const T hinv = a.mesh.h_inv, dt = a.dt;
-  for (int k = 0; k < maxcnt; ++k) {
-    if (k >= ccount) continue;
-    const uint32_t pid = a.perm[start + coff + k];
-    T x[D], l;
-    int lo[D], hi[D];
-    particle_window<T, D, BASIS, STORED>(a, pid, x, l, lo, hi, nullptr);
-    int ext[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) ext[d] = hi[d] - lo[d];
-    T val[D][SW], der[D][SW];
-    if constexpr (!STORED) {
-#pragma unroll
-      for (int d = 0; d < D; ++d) axis_jets<T, D, BASIS>(x[d], ccoord[d], s_nodex[d], l, hinv, val[d], der[d]);
-    }
-    T dv[D], L[D * D], dx[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      dv[c] = T(0);
-      dx[c] = T(0);
-    }
-#pragma unroll
-    for (int q = 0; q < D * D; ++q) L[q] = T(0);
-    // column-major walk of the window (eachindex(window))
-#pragma unroll
-    for (int o = 0; o < Pow<SW, D>::v; ++o) {
-      int aa[D], r = o;
-      bool ok = true;
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        aa[d] = r % SW;
-        r /= SW;
-        ok = ok && aa[d] <= ext[d];
-      }
-      if (!ok) continue;
-      T w, g[D];
-      if constexpr (STORED) {
-        const int64_t e = int64_t(o) + int64_t(Pow<SW, D>::v) * pid;
-        w = a.w(e);
-#pragma unroll
-        for (int c = 0; c < D; ++c) g[c] = a.gw(e, c);
-      } else {
-        w = val[0][aa[0]];
-#pragma unroll
-        for (int d = 1; d < D; ++d) w = w * val[d][aa[d]];
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-          T gg = (c == 0) ? der[0][aa[0]] : val[0][aa[0]];
-#pragma unroll
-          for (int d = 1; d < D; ++d) gg = gg * ((d == c) ? der[d][aa[d]] : val[d][aa[d]]);
-          g[c] = gg * hinv;
-        }
-      }
-      int q = 0, mul = 1;
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        q += (ccoord[d] + aa[d]) * mul;
-        mul *= G::TS;
-      }
-      T vi[D];
-#pragma unroll
-      for (int c = 0; c < D; ++c) vi[c] = s_v[c][q];
-#pragma unroll
-      for (int c = 0; c < D; ++c) dv[c] += w * (vi[c] - s_vn[c][q]);
-#pragma unroll
-      for (int b = 0; b < D; ++b)
-#pragma unroll
-        for (int r2 = 0; r2 < D; ++r2) L[r2 + D * b] += vi[r2] * g[b];
-#pragma unroll
-      for (int c = 0; c < D; ++c) dx[c] += (w * vi[c]) * dt;
-    }
-    particle_update<T, D>(a, pid, dv, L, dx);
-  }
-}
-
-// Particles outside the mesh have an empty window: the sums are zero and only
-// the particle equations run (g2p.jl:123-153).
-template <typename T, int D>
-__global__ __launch_bounds__(256) void k_g2p_outside(Args<T, D> a) {
-  const int64_t j = a.n_inside + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= a.n) return;
-  const uint32_t pid = a.perm[j];
-  T dv[D], L[D * D], dx[D];
-#pragma unroll
-  for (int c = 0; c < D; ++c) {
-    dv[c] = T(0);
-    dx[c] = T(0);
-  }
-#pragma unroll
-  for (int k = 0; k < D * D; ++k) L[k] = T(0);
-  particle_update<T, D>(a, pid, dv, L, dx);
-}
-
-// ---- host side ----------------------------------------------------------------
-
-template <typename T, int D>
-Args<T, D> make_args(const tess_mesh& mesh, const tess_grid* G, const tess_particles& P, const tess_weights* W,
-                     const tess_params* prm) {
-  Args<T, D> a;
-  std::memset(static_cast<void*>(&a), 0, sizeof(a));
-  a.mesh = dmesh<T, D>(mesh);
-  a.x = dfield<T>(P.x);
-  a.v = dfield<T>(P.v);
-  a.V = dfield<T>(P.V);
-  a.gradv = dfield<T>(P.grad_v);
-  a.F = dfield<T>(P.F);
-  a.sigma = dfield<T>(P.sigma);
-  a.l = dfield<T>(P.l);
-  a.logE = dfield<T>(P.logE);
-  if (G) {
-    a.gm = dfield<T>(G->m);
-    a.gmv = dfield<T>(G->mv);
-    a.gf = dfield<T>(G->f);
-    a.gvn = dfield<T>(G->vn);
-    a.gv = dfield<T>(G->v);
-  }
-  if (W) {
-    a.window = W->window;
-    a.w = dfield<T>(W->w);
-    a.gw = dfield<T>(W->grad_w);
-  }
-  a.n = P.n;
-  if (prm) {
-    a.mp = T(prm->mp);
-    a.dt = T(prm->dt);
-    a.lambda = prm->lambda;
-    a.mu = prm->mu;
-    a.V0 = prm->V0;
-    for (int d = 0; d < D; ++d) a.gravity[d] = T(prm->gravity[d]);
-    a.slip = prm->slip_walls;
-  }
-  for (int d = 0; d < D; ++d) a.ntiles[d] = int32_t(((mesh.dims[d] - 1) >> Geo<D>::LT) + 1);
-  return a;
-}
-
-template <typename T, int D>
-void set_sparse(Args<T, D>& a, const tess_mesh& mesh, const tess_sparsity* sp) {
-  if (!sp) return;
-  a.numbering = sp->blocknumbering;
-  a.spL = mesh.block_size_log2;
-  for (int d = 0; d < D; ++d) a.nblk[d] = int32_t(((mesh.dims[d] - 1) >> mesh.block_size_log2) + 1);
-}
-
-template <typename T, int D>
-void fill_binning_ptrs(tess_ctx* ctx, Args<T, D>& a) {
-  const Binning& b = ctx->bin;
-  a.perm = static_cast<const uint32_t*>(ctx->vals.ptr);
-  a.tile_start = static_cast<const uint32_t*>(ctx->tile_start.ptr);
-  a.tile_end = static_cast<const uint32_t*>(ctx->tile_end.ptr);
-  a.active = static_cast<const uint32_t*>(ctx->active_tiles.ptr);
-  a.tile_flag = static_cast<const uint32_t*>(ctx->tile_flag.ptr);
-  a.tile_slot = static_cast<const uint32_t*>(ctx->tile_slot.ptr);
-  a.nb_list = static_cast<const uint32_t*>(ctx->nb_list.ptr);
-  a.nb_flag = static_cast<const uint32_t*>(ctx->nb_flag.ptr);
-  a.n_inside = b.n_inside;
-  a.n_active = b.n_active;
-  a.n_nb = b.n_nodeblocks;
-}
-
-// True when every particle field is Julia-native AoS (components contiguous,
-// element stride = component count), the layout the vectorised kernels assume.
-inline bool julia_layout(const tess_particles& P, int D) {
-  auto ok = [](const tess_field& f, int nc) { return !f.ptr || (f.comp_stride == 1 && f.elem_stride == nc) ||
-                                                      (nc == 1 && f.elem_stride == 1); };
-  return ok(P.x, D) && ok(P.v, D) && ok(P.V, 1) && ok(P.grad_v, D * D) && ok(P.F, D * D) && ok(P.sigma, D * D);
-}
-
-inline bool grid_julia_layout(const tess_grid& G, int D) {
-  auto ok = [D](const tess_field& f) { return f.comp_stride == 1 && f.elem_stride == D; };
-  return ok(G.mv) && ok(G.f) && ok(G.vn) && ok(G.v);
-}
-
-inline unsigned blocks_for(int64_t n, int threads) { return unsigned(std::max<int64_t>(1, ceil_div(n, threads))); }
-
-int key_bits_for(uint64_t maxkey) {
-  int b = 0;
-  while (b < 32 && (uint64_t(1) << b) <= maxkey) ++b;
-  return b;
-}
-
-template <typename T, int D, int BASIS, bool STORED>
-void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const tess_weights* W) {
-  Args<T, D> a = make_args<T, D>(mesh, nullptr, P, W, nullptr);
-  const int64_t n = P.n;
-  uint64_t ntiles = 1;
-  for (int d = 0; d < D; ++d) ntiles *= uint64_t(a.ntiles[d]);
-  if (ntiles * 64 >= uint64_t(UINT32_MAX) || n >= int64_t(UINT32_MAX))
-    raise(TESS_ERR_ARGUMENT, "mesh or particle count exceeds the 32-bit binning range");
-  Binning& b = ctx->bin;
-  b.valid = false;
-  uint32_t* counters = ctx->counters.as<uint32_t>(CNT_SLOTS);
-  TESS_HIP(hipMemsetAsync(counters, 0, CNT_BIN_SLOTS * sizeof(uint32_t), ctx->stream));
-  uint32_t* keys = ctx->keys.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
-  uint32_t* skeys = ctx->keys_sorted.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
-  uint32_t* perm = ctx->vals.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
-  uint32_t* tstart = ctx->tile_start.as<uint32_t>(ntiles);
-  uint32_t* tend = ctx->tile_end.as<uint32_t>(ntiles);
-  uint32_t* tflag = ctx->tile_flag.as<uint32_t>(ntiles);
-  uint32_t* tslot = ctx->tile_slot.as<uint32_t>(ntiles);
-  uint32_t* active = ctx->active_tiles.as<uint32_t>(ntiles);
-  uint32_t* nbflag = ctx->nb_flag.as<uint32_t>(ntiles);
-  uint32_t* nbslot = ctx->nb_slot.as<uint32_t>(ntiles);
-  uint32_t* nblist = ctx->nb_list.as<uint32_t>(ntiles);
-  const int tok = prof_begin(ctx, "binning", 0.0);
-  const int kbits = key_bits_for(ntiles * 64);
-  const bool incremental = b.state_valid && b.state_n == n && b.state_ntiles == int64_t(ntiles) &&
-                           b.state_dim == D && n > 0;
-  if (n > 0 && incremental) {
-    // keys in the previous sorted order, stayer flags -> stayer ranks
-    uint32_t* nk = ctx->inc_nk.as<uint32_t>(size_t(n));
-    uint32_t* spos = ctx->inc_spos.as<uint32_t>(size_t(n) + 1);
-    hipLaunchKernelGGL((k_keys_sorted<T, D, BASIS, STORED>), dim3(blocks_for(n + 1, 256)), dim3(256), 0,
-                       ctx->stream, a, perm, skeys, uint32_t(ntiles), nk, spos, counters);
-    TESS_HIP(hipGetLastError());
-    exclusive_scan_u32(ctx, spos, n + 1, counters + CNT_STAY);
-    uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
-    TESS_HIP(hipMemcpyAsync(hc + CNT_STAY, counters + CNT_STAY, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                            ctx->stream));
-    TESS_HIP(hipStreamSynchronize(ctx->stream));
-    const int64_t M = n - int64_t(hc[CNT_STAY]);
-    ctx->last_movers = M;
-    if (M > 0) {
-      uint32_t* skeys2 = ctx->skeys2.as<uint32_t>(size_t(n));
-      uint32_t* perm2 = ctx->perm2.as<uint32_t>(size_t(n));
-      if (M <= n / 8) {
-        uint32_t* mkey = ctx->inc_mkey.as<uint32_t>(size_t(M));
-        uint32_t* mj = ctx->inc_mj.as<uint32_t>(size_t(M));
-        uint32_t* mks = ctx->inc_mks.as<uint32_t>(size_t(M));
-        uint32_t* mr = ctx->inc_mr.as<uint32_t>(size_t(M));
-        uint32_t* mjs = ctx->inc_mjs.as<uint32_t>(size_t(M));
-        hipLaunchKernelGGL(k_compact_movers, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, nk, spos, n, mkey,
-                           mj);
-        TESS_HIP(hipGetLastError());
-        radix_sort_pairs(ctx, mkey, nullptr, mks, mr, M, kbits);
-        hipLaunchKernelGGL(k_place_movers, dim3(blocks_for(M, 256)), dim3(256), 0, ctx->stream, mks, mr, mj, skeys,
-                           perm, spos, n, M, mjs, skeys2, perm2);
-        TESS_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_place_stayers, dim3(blocks_for(n, STAY_BLOCK)), dim3(STAY_BLOCK), 0, ctx->stream,
-                           skeys, perm, spos, n, mks, mjs, M, skeys2, perm2);
-        TESS_HIP(hipGetLastError());
-      } else {
-        radix_sort_pairs(ctx, nk, perm, skeys2, perm2, n, kbits);
-      }
-      std::swap(ctx->keys_sorted, ctx->skeys2);
-      std::swap(ctx->vals, ctx->perm2);
-      skeys = static_cast<uint32_t*>(ctx->keys_sorted.ptr);
-      perm = static_cast<uint32_t*>(ctx->vals.ptr);
-    }
-  } else if (n > 0) {
-    hipLaunchKernelGGL((k_bin_keys<T, D, BASIS, STORED>), dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, a,
-                       keys, uint32_t(ntiles), counters);
-    TESS_HIP(hipGetLastError());
-    radix_sort_pairs(ctx, keys, nullptr, skeys, perm, n, kbits);
-    ctx->last_movers = -1;
-  }
-  TESS_HIP(hipMemsetAsync(tstart, 0, ntiles * sizeof(uint32_t), ctx->stream));
-  TESS_HIP(hipMemsetAsync(tend, 0, ntiles * sizeof(uint32_t), ctx->stream));
-  if (n > 0) {
-    hipLaunchKernelGGL(k_tile_runs, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, skeys, n,
-                       uint32_t(ntiles), tstart, tend, counters);
-    TESS_HIP(hipGetLastError());
-  }
-  hipLaunchKernelGGL(k_tile_flags, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, tstart, tend,
-                     uint32_t(ntiles), tflag, tslot);
-  TESS_HIP(hipGetLastError());
-  exclusive_scan_u32(ctx, tslot, int64_t(ntiles), counters + CNT_ACTIVE);
-  hipLaunchKernelGGL(k_compact, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, tflag, tslot,
-                     uint32_t(ntiles), active);
-  TESS_HIP(hipGetLastError());
-  TESS_HIP(hipMemsetAsync(nbflag, 0, ntiles * sizeof(uint32_t), ctx->stream));
-  const int3 nt = make_int3(a.ntiles[0], a.ntiles[1], D == 3 ? a.ntiles[D == 3 ? 2 : 0] : 1);
-  hipLaunchKernelGGL(k_nb_mark<D>, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, tflag,
-                     uint32_t(ntiles), nt, nbflag);
-  TESS_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_copy_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, nbflag, nbslot,
-                     int64_t(ntiles));
-  TESS_HIP(hipGetLastError());
-  exclusive_scan_u32(ctx, nbslot, int64_t(ntiles), counters + CNT_NB);
-  hipLaunchKernelGGL(k_compact, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, nbflag, nbslot,
-                     uint32_t(ntiles), nblist);
-  TESS_HIP(hipGetLastError());
-  prof_end(ctx, tok);
-  // the one host readback of the binning: totals for the launch sizes
-  uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
-  TESS_HIP(hipMemcpyAsync(hc, counters, CNT_SLOTS * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-  TESS_HIP(hipStreamSynchronize(ctx->stream));
-  if (hc[CNT_ERR] & ERR_GIMP_LENGTH) raise(TESS_ERR_ARGUMENT, "uGIMP requires 0 ≤ pt.l ≤ spacing(mesh)");
-  if (hc[CNT_ERR] & ERR_WINDOW)
-    raise(TESS_ERR_DIMENSION, "stored support window is larger than the basis support width");
-  b.n_inside = n > 0 ? int64_t(hc[CNT_INSIDE]) : 0;
-  ctx->last_active_nodes = int64_t(hc[CNT_NODES]);
-  // layout score of the previous P2G on this particle set (consumed by the step driver)
-  ctx->last_misplaced = (ctx->misplaced_x_ptr == P.x.ptr && ctx->misplaced_n == n) ? int64_t(hc[CNT_MISPLACED]) : -1;
-  b.n_active = int64_t(hc[CNT_ACTIVE]);
-  b.n_nodeblocks = int64_t(hc[CNT_NB]);
-  b.x_ptr = P.x.ptr;
-  b.n = n;
-  b.dim = D;
-  b.dtype = mesh.dtype;
-  b.basis = BASIS;
-  b.tile_log2 = Geo<D>::LT;
-  b.ntiles = int64_t(ntiles);
-  for (int d = 0; d < 3; ++d) b.ntiles_axis[d] = d < D ? a.ntiles[d] : 1;
-  b.valid = true;
-  b.state_valid = true;
-  b.state_n = n;
-  b.state_ntiles = int64_t(ntiles);
-  b.state_dim = D;
-}
-
-// P2G after binning: zeroing of what this call does not overwrite, the tile
-// kernel and the gather.  count_misplaced asks the tile kernel to count
-// particles not stored in the layered order (for the step driver's reorder).
-template <typename T, int D, int BASIS, bool STORED>
-void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-                const tess_weights* W, const tess_params& prm, bool count_misplaced, const tess_sparsity* sp) {
-  using Gm = Geo<D>;
-  constexpr int NV = 1 + 2 * D;
-  constexpr int SW = support_width(BASIS);
-  // all sw^dim offsets in one pass while the accumulators fit ~200 VGPRs,
-  // else one pass per offset along the last axis
-  constexpr int NPASS = (Pow<SW, D>::v * NV * int(sizeof(T)) / 4 <= 100) ? 1 : SW;
-  Args<T, D> a = make_args<T, D>(mesh, &G, P, W, &prm);
-  fill_binning_ptrs<T, D>(ctx, a);
-  set_sparse<T, D>(a, mesh, sp);
-  uint32_t* counters = static_cast<uint32_t*>(ctx->counters.ptr);
-  int64_t nnodes = 1;
-  for (int d = 0; d < D; ++d) nnodes *= mesh.dims[d];
-  // zero what this call will not overwrite: everything on first use of a
-  // grid, else the node blocks that were written last time and are not now.
-  // An SpGrid's storage is zeroed whole (its numbering may have changed).
-  const bool fresh = ctx->last_grid_m != G.m.ptr || ctx->prev_tile_log2 != Gm::LT ||
-                     ctx->prev_ntiles != ctx->bin.ntiles || ctx->prev_nnodes != nnodes;
-  if (sp) {
-    a.p2g_err = counters + CNT_P2GERR;
-    TESS_HIP(hipMemsetAsync(a.p2g_err, 0, sizeof(uint32_t), ctx->stream));
-    if (sp->storage_len > 0) {
-      hipLaunchKernelGGL((k_zero_grid<T, D>), dim3(blocks_for(sp->storage_len, 256)), dim3(256), 0, ctx->stream, a,
-                         sp->storage_len);
-      TESS_HIP(hipGetLastError());
-    }
-  } else if (fresh) {
-    hipLaunchKernelGGL((k_zero_grid<T, D>), dim3(blocks_for(nnodes, 256)), dim3(256), 0, ctx->stream, a, nnodes);
-    TESS_HIP(hipGetLastError());
-  } else if (ctx->prev_nodeblocks > 0) {
-    hipLaunchKernelGGL((k_zero_stale<T, D>), dim3(unsigned(ctx->prev_nodeblocks)), dim3(64), 0, ctx->stream, a,
-                       static_cast<const uint32_t*>(ctx->nb_list_prev.ptr), ctx->prev_nodeblocks);
-    TESS_HIP(hipGetLastError());
-  }
-  const Binning& b = ctx->bin;
-  uint32_t* misplaced = nullptr;
-  uint32_t* per_tile = ctx->per_tile.as<uint32_t>(size_t(std::max<int64_t>(b.n_active, 1)));
-  uint32_t* per_nb = ctx->per_nodeblock.as<uint32_t>(size_t(std::max<int64_t>(b.n_nodeblocks, 1)));
-  if (count_misplaced) {
-    misplaced = per_tile;
-    ctx->misplaced_x_ptr = P.x.ptr;
-    ctx->misplaced_n = P.n;
-  } else {
-    ctx->misplaced_x_ptr = nullptr;
-  }
-  if (b.n_active > 0) {
-    a.partials = ctx->partials.as<T>(size_t(b.n_active) * 8 * Gm::NFOOT);
-    // algorithmic bytes: x, v, V, σ per particle
-    const int tok = prof_begin(ctx, "k_p2g_tile", double(b.n_inside) * sizeof(T) * (2 * D + 1 + D * D));
-    const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
-    if constexpr (STORED) {
-      hipLaunchKernelGGL((k_p2g_tile<T, D, BASIS, STORED, NPASS>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
-                         ctx->stream, a, sk, misplaced);
-    } else {
-      constexpr int NPF = (Pow<SW, D>::v * NV * int(sizeof(T)) / 4 <= 192) ? 1 : SW;
-      if (julia_layout(P, D))
-        hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, true>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
-                           ctx->stream, a, sk, misplaced);
-      else
-        hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, false>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
-                           ctx->stream, a, sk, misplaced);
-    }
-    TESS_HIP(hipGetLastError());
-    prof_end(ctx, tok);
-    if (misplaced) {
-      sum_u32(ctx, misplaced, b.n_active, counters + CNT_MISPLACED);
-    }
-  }
-  TESS_HIP(hipMemsetAsync(counters + CNT_NODES, 0, sizeof(uint32_t), ctx->stream));
-  if (b.n_nodeblocks > 0) {
-    // algorithmic bytes per active node: m, mv, f written, read back by the
-    // grid-node half, vⁿ and v written (the node count of the previous call)
-    const double nodes = double(std::max<int64_t>(ctx->last_active_nodes, 0));
-    const int tok = prof_begin(ctx, "k_gather_grid", nodes * sizeof(T) * (2 * (2 * D + 1) + 2 * D));
-    if (grid_julia_layout(G, D))
-      hipLaunchKernelGGL((k_gather_grid<T, D, true>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a,
-                         per_nb);
-    else
-      hipLaunchKernelGGL((k_gather_grid<T, D, false>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a,
-                         per_nb);
-    TESS_HIP(hipGetLastError());
-    prof_end(ctx, tok);
-    sum_u32(ctx, per_nb, b.n_nodeblocks, counters + CNT_NODES);
-  }
-  // remember this call's node blocks for the next call's stale zeroing
-  uint32_t* prev = ctx->nb_list_prev.as<uint32_t>(size_t(std::max<int64_t>(b.ntiles, 1)));
-  if (b.n_nodeblocks > 0)
-    TESS_HIP(hipMemcpyAsync(prev, ctx->nb_list.ptr, size_t(b.n_nodeblocks) * sizeof(uint32_t),
-                            hipMemcpyDeviceToDevice, ctx->stream));
-  ctx->prev_nodeblocks = b.n_nodeblocks;
-  ctx->last_grid_m = sp ? nullptr : G.m.ptr;  // a dense grid after an SpGrid call starts from a full zero
-  ctx->prev_tile_log2 = Gm::LT;
-  ctx->prev_ntiles = b.ntiles;
-  ctx->prev_nnodes = nnodes;
-  if (sp) {
-    uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
-    TESS_HIP(hipMemcpyAsync(hc + CNT_P2GERR, counters + CNT_P2GERR, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                            ctx->stream));
-    TESS_HIP(hipStreamSynchronize(ctx->stream));
-    if (hc[CNT_P2GERR] & ERR_INACTIVE)
-      raise(TESS_ERR_STATE, "@P2G: inactive SpGrid support node. Call update_sparsity! before @P2G.");
-  }
-}
-
-template <typename T, int D, int BASIS, bool STORED>
-void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-                const tess_weights* W, const tess_params& prm, const tess_sparsity* sp) {
-  const Binning& b = ctx->bin;
-  Args<T, D> a = make_args<T, D>(mesh, &G, P, W, &prm);
-  fill_binning_ptrs<T, D>(ctx, a);
-  set_sparse<T, D>(a, mesh, sp);
-  if (b.n_active > 0) {
-    // algorithmic bytes: x, v, F read and v, ∇v, x, F, σ, V written per
-    // particle; vⁿ, v read per active node
-    const double nodes = double(std::max<int64_t>(ctx->last_active_nodes, 0));
-    const int tok = prof_begin(ctx, "k_g2p_tile",
-                               double(b.n_inside) * sizeof(T) * (2 * D + D * D + 2 * D + 3 * D * D + 1) +
-                                   nodes * sizeof(T) * 2 * D);
-    const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
-    if constexpr (STORED) {
-      hipLaunchKernelGGL((k_g2p_tile<T, D, BASIS, STORED>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream, a,
-                         sk);
-    } else {
-      constexpr int NW = 4;
-      if (julia_layout(P, D))
-        hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
-                           ctx->stream, a, sk);
-      else
-        hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, false, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
-                           ctx->stream, a, sk);
-    }
-    TESS_HIP(hipGetLastError());
-    prof_end(ctx, tok);
-  }
-  if (P.n > b.n_inside) {
-    hipLaunchKernelGGL((k_g2p_outside<T, D>), dim3(blocks_for(P.n - b.n_inside, 256)), dim3(256), 0, ctx->stream,
-                       a);
-    TESS_HIP(hipGetLastError());
-  }
-  ctx->bin.valid = false;  // positions moved
-}
-
-// Layered storage order of the binned particles: inside each tile the k-th
-// particle of every class, classes ascending, then the next k.  Writes the
-// gather order (new storage slot -> old particle) and the sorted-slot -> new
-// storage index map.
-template <typename T, int D>
-__global__ __launch_bounds__(64) void k_layered_order(Args<T, D> a, const uint32_t* __restrict__ skeys,
-                                                       uint32_t* __restrict__ order, uint32_t* __restrict__ perm_new) {
-  const int lane = threadIdx.x;
-  const int64_t j = blockIdx.x;
-  if (j >= a.n_active) return;
-  const uint32_t tile = a.active[j];
-  const uint32_t start = a.tile_start[tile];
-  const int cnt = int(a.tile_end[tile] - start);
-  __shared__ uint32_t s_cnt[64];
-  int ccount, coff, maxcnt;
-  tile_class_counts(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);
-  uint32_t layer_off = 0;
-  for (int k = 0; k < maxcnt; ++k) {
-    const uint64_t has = __ballot(k < ccount);
-    if (k < ccount) {
-      const uint32_t dest = start + layer_off + uint32_t(__popcll(has & lanemask_lt64()));
-      order[dest] = a.perm[start + coff + k];
-      perm_new[start + coff + k] = dest;
-    }
-    layer_off += uint32_t(__popcll(has));
-  }
-}
-
-__global__ void k_tail_identity(const uint32_t* __restrict__ perm, int64_t from, int64_t n,
-                                uint32_t* __restrict__ order, uint32_t* __restrict__ perm_new) {
-  const int64_t j = from + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  order[j] = perm[j];
-  perm_new[j] = uint32_t(j);
-}
-
-template <typename T, int D>
-void layered_reorder(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P) {
-  const Binning& b = ctx->bin;
-  const int64_t n = P.n;
-  if (n == 0) return;
-  Args<T, D> a = make_args<T, D>(mesh, nullptr, P, nullptr, nullptr);
-  fill_binning_ptrs<T, D>(ctx, a);
-  uint32_t* order = ctx->keys.as<uint32_t>(size_t(n));        // raw keys are dead after the sort
-  uint32_t* perm_new = ctx->keys_alt.as<uint32_t>(size_t(n));
-  if (b.n_active > 0) {
-    hipLaunchKernelGGL((k_layered_order<T, D>), dim3(unsigned(b.n_active)), dim3(64), 0, ctx->stream, a,
-                       static_cast<const uint32_t*>(ctx->keys_sorted.ptr), order, perm_new);
-    TESS_HIP(hipGetLastError());
-  }
-  if (n > b.n_inside) {
-    hipLaunchKernelGGL(k_tail_identity, dim3(blocks_for(n - b.n_inside, 256)), dim3(256), 0, ctx->stream,
-                       static_cast<const uint32_t*>(ctx->vals.ptr), b.n_inside, n, order, perm_new);
-    TESS_HIP(hipGetLastError());
-  }
-  permute_particles(ctx, mesh, P, order);
-  TESS_HIP(hipMemcpyAsync(ctx->vals.ptr, perm_new, size_t(n) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
-                          ctx->stream));
-}
-
-template <typename T, int D, int BASIS, bool STORED>
-void p2g_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-              const tess_weights* W, const tess_params& prm, const tess_sparsity* sp) {
-  bin_particles<T, D, BASIS, STORED>(ctx, mesh, P, W);
-  p2g_launch<T, D, BASIS, STORED>(ctx, mesh, G, P, W, prm, false, sp);
-}
-
-template <typename T, int D, int BASIS, bool STORED>
-void g2p_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-              const tess_weights* W, const tess_params& prm, const tess_sparsity* sp) {
-  // a standalone G2P bins again: device addresses alone cannot prove that the
-  // particle set is unchanged since the last P2G
-  bin_particles<T, D, BASIS, STORED>(ctx, mesh, P, W);
-  g2p_launch<T, D, BASIS, STORED>(ctx, mesh, G, P, W, prm, sp);
-}
-
-// nsteps of the README loop body (deferred weights).  With reorder_threshold
-// in (0, 1], particles are moved into the layered order when the fraction
-// stored in place, measured by the previous step's P2G, is below the
-// threshold (1 reorders every step), the role of
-// reorder_particles!(particles, partition; threshold) in the reference loop.
-template <typename T, int D, int BASIS, bool STORED>
-void step_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-               const tess_params& prm, int nsteps, double reorder_threshold, tess_sparsity* sp) {
-  for (int s = 0; s < nsteps; ++s) {
-    if (sp) {
-      // update_sparsity!(grid, xₚ) at the top of the SpGrid loop (src/Grids/grid.jl:97-101)
-      run_sparsity_update(ctx, mesh, P.x, P.n, *sp, nullptr);
-      if (sp->storage_len > sp->capacity)
-        raise(TESS_ERR_STATE, "SpGrid storage holds " + std::to_string(sp->capacity) + " nodes but the active blocks need " +
-                                  std::to_string(sp->storage_len) + "; resize the grid arrays and call again");
-    }
-    bin_particles<T, D, BASIS, false>(ctx, mesh, P, nullptr);
-    const Binning& b = ctx->bin;
-    if (reorder_threshold > 0.0 && b.n_inside > 0) {
-      bool go = reorder_threshold >= 1.0;
-      if (!go && ctx->last_misplaced >= 0) {
-        const double score = 1.0 - double(ctx->last_misplaced) / double(b.n_inside);
-        go = score < reorder_threshold;
-      }
-      if (go) {
-        layered_reorder<T, D>(ctx, mesh, P);
-        ctx->reorders += 1;
-      }
-    }
-    p2g_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, reorder_threshold > 0.0, sp);
-    g2p_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, sp);
-  }
-}
-
-template <template <typename, int, int, bool> class Op, typename... Args_>
-void dispatch(int basis, const tess_mesh& mesh, bool stored, Args_&&... args) {
-#define TESS_CASE(T_, D_, B_)                                     \
-  if (mesh.dim == D_ && basis == B_) {                            \
-    if (stored)                                                   \
-      Op<T_, D_, B_, true>::run(std::forward<Args_>(args)...);    \
-    else                                                          \
-      Op<T_, D_, B_, false>::run(std::forward<Args_>(args)...);   \
-    return;                                                       \
-  }
-#define TESS_CASES(T_, D_)                        \
-  TESS_CASE(T_, D_, TESS_BSPLINE_LINEAR)          \
-  TESS_CASE(T_, D_, TESS_BSPLINE_QUADRATIC)       \
-  TESS_CASE(T_, D_, TESS_BSPLINE_CUBIC)           \
-  TESS_CASE(T_, D_, TESS_UGIMP)
-  if (mesh.dtype == TESS_FLOAT32) {
-    TESS_CASES(float, 2)
-    TESS_CASES(float, 3)
-  } else {
-    TESS_CASES(double, 2)
-    TESS_CASES(double, 3)
-  }
-#undef TESS_CASES
-#undef TESS_CASE
-  raise(TESS_ERR_UNSUPPORTED, "unsupported (dim, dtype, basis) combination");
-}
-
-template <typename T, int D, int B, bool S>
-struct P2GOp {
-  static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
-                  const tess_weights* W, const tess_params& prm, const tess_sparsity* sp) {
-    p2g_impl<T, D, B, S>(ctx, m, G, P, W, prm, sp);
-  }
-};
-template <typename T, int D, int B, bool S>
-struct G2POp {
-  static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
-                  const tess_weights* W, const tess_params& prm, const tess_sparsity* sp) {
-    g2p_impl<T, D, B, S>(ctx, m, G, P, W, prm, sp);
-  }
-};
-template <typename T, int D, int B, bool S>
-struct StepOp {
-  static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
-                  const tess_params& prm, int nsteps, double thr, tess_sparsity* sp) {
-    if constexpr (!S) step_impl<T, D, B, false>(ctx, m, G, P, prm, nsteps, thr, sp);
-  }
-};
-
-}  // namespace
 
 void run_p2g(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
              const tess_weights* W, const tess_params& prm) {

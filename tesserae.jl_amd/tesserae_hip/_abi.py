@@ -89,6 +89,15 @@ class tess_sparsity(ctypes.Structure):
     ]
 
 
+class tess_adjoint(ctypes.Structure):
+    _fields_ = [
+        ("x", tess_field),
+        ("v", tess_field),
+        ("F", tess_field),
+        ("logE", tess_field),
+    ]
+
+
 class tess_params(ctypes.Structure):
     _fields_ = [
         ("dt", ctypes.c_double),
@@ -157,6 +166,20 @@ SIGNATURES = {
         ctypes.c_int32,
         [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_sparsity), _P(tess_particles),
          _P(tess_params), ctypes.c_int32, ctypes.c_double],
+    ),
+    "tess_step_adjoint": (
+        ctypes.c_int32,
+        [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_particles), _P(tess_params),
+         _P(tess_adjoint)],
+    ),
+    "tess_rollout": (
+        ctypes.c_int32,
+        [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_particles), _P(tess_params),
+         ctypes.c_int32, ctypes.c_int32],
+    ),
+    "tess_rollout_adjoint": (
+        ctypes.c_int32,
+        [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_params), _P(tess_adjoint)],
     ),
     "tess_profile_enable": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "tess_profile_read": (

@@ -710,3 +710,58 @@ def step(grid: Grid, particles: Particles, basis: Basis, params: MPMParams, nste
     _check(ctx.lib, ctx.lib.tess_step(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
                                       ctypes.byref(P), ctypes.byref(params.abi()), int(nsteps),
                                       float(reorder_threshold)))
+
+
+# ---- differentiation --------------------------------------------------------------
+
+class Adjoint:
+    """Float64 adjoint of the step state: x̄, v̄ (n, d), F̄ (n, d*d) column-major, and the
+    accumulated ∂/∂logE (n,)."""
+
+    def __init__(self, n: int, dim: int, device="cuda"):
+        self.n, self.dim = int(n), dim
+        self.x = torch.zeros((self.n, dim), dtype=torch.float64, device=device)
+        self.v = torch.zeros((self.n, dim), dtype=torch.float64, device=device)
+        self.F = torch.zeros((self.n, dim * dim), dtype=torch.float64, device=device)
+        self.logE = torch.zeros((self.n,), dtype=torch.float64, device=device)
+
+    def abi(self) -> _abi.tess_adjoint:
+        A = _abi.tess_adjoint()
+        A.x = _field(self.x, self.dim, "julia")
+        A.v = _field(self.v, self.dim, "julia")
+        A.F = _field(self.F, self.dim * self.dim, "julia")
+        A.logE = _field(self.logE, 1, "julia")
+        return A
+
+
+def step_adjoint(grid: Grid, particles: Particles, basis: Basis, params: MPMParams, adj: Adjoint, ctx=None) -> None:
+    """Reverse of one step from the start state ``particles`` (unchanged): ``adj`` goes from
+    the adjoint after the step to the adjoint before it; ``adj.logE`` accumulates."""
+    _require_device(particles.x, grid.m, adj.x)
+    ctx = ctx or default_context()
+    P, G, A = particles.abi(), grid.abi(), adj.abi()
+    _check(ctx.lib, ctx.lib.tess_step_adjoint(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
+                                              ctypes.byref(P), ctypes.byref(params.abi()), ctypes.byref(A)))
+
+
+def rollout(grid: Grid, particles: Particles, basis: Basis, params: MPMParams, nsteps: int,
+            checkpoint_every: int = 20, ctx=None) -> None:
+    """nsteps forward steps (no reordering) with a checkpoint every ``checkpoint_every`` steps,
+    recorded for :func:`rollout_adjoint`."""
+    _require_device(particles.x, grid.m)
+    ctx = ctx or default_context()
+    P, G = particles.abi(), grid.abi()
+    ctx._rollout_keepalive = particles.tensors.get("logE")
+    _check(ctx.lib, ctx.lib.tess_rollout(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
+                                         ctypes.byref(P), ctypes.byref(params.abi()), int(nsteps),
+                                         int(checkpoint_every)))
+
+
+def rollout_adjoint(grid: Grid, basis: Basis, params: MPMParams, adj: Adjoint, ctx=None) -> None:
+    """Pullback of the context's last rollout: ``adj`` holds the adjoint of the final state
+    on entry and of the initial state on return; ``adj.logE`` accumulates ∂/∂logE."""
+    _require_device(grid.m, adj.x)
+    ctx = ctx or default_context()
+    G, A = grid.abi(), adj.abi()
+    _check(ctx.lib, ctx.lib.tess_rollout_adjoint(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()),
+                                                 ctypes.byref(G), ctypes.byref(params.abi()), ctypes.byref(A)))

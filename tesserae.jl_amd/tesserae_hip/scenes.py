@@ -142,6 +142,29 @@ def sphere_spin(basis: Basis, dtype=torch.float32, dx: float = 1 / 512, seed: in
     return Scene("sphere_spin", mesh, basis, params, steps=steps, l=l, **_state(x, v, V0, 3))
 
 
+def inverse_cube(dtype=torch.float32, dx: float = 1 / 64, seed: int = 5, steps: int = 200,
+                 lo: float = 0.3, side: float = 0.4, drift: float = 0.25, noise: float = 0.05) -> Scene:
+    """Differentiable 3D inverse problem: a cube of the given side at [lo, lo+side]^3 with 2x2x2
+    particles per cell of [0,1]^3 (dx), quadratic B-spline, Δt=2e-4, Neo-Hookean with base
+    E=1e5, ν=0.3, ρ⁰=1e3 and per-particle logE = 0 (E_p = E exp(logE_p)).  The initial
+    velocity is a drift along x plus seeded noise, v = (drift, 0, 0) + noise*N(0,1).  The
+    loss target is x(0) + (0.1, 0, 0)."""
+    mesh = CartesianMesh(dx, (0, 1), (0, 1), (0, 1), dtype=dtype)
+    s = dx / 2
+    k = int(math.floor(side / s - 0.5 + 1e-9)) + 1
+    ax = lo + (np.arange(k) + 0.5) * s
+    g = np.meshgrid(ax, ax, ax, indexing="ij")
+    x = np.stack([a.reshape(-1, order="F") for a in g], axis=1)
+    n = len(x)
+    v = np.zeros((n, 3))
+    v[:, 0] = drift
+    v = v + noise * normal(seed, 3 * n).reshape(n, 3)
+    V0 = s ** 3
+    params = MPMParams(dt=2e-4, E=1e5, nu=0.3, rho0=1e3, V0=V0)
+    return Scene("inverse_cube", mesh, BSpline(Quadratic()), params, steps=steps, logE=np.zeros(n),
+                 **_state(x, v, V0, 3))
+
+
 def random_box(dim: int, n: int, seed: int, basis: Basis, dtype=torch.float64, h: float = 0.1,
                extent: float = 1.0, vscale: float = 0.3, lo: float = 0.0, hi: float = 1.0,
                stress: float = 0.0, with_l: bool = False) -> Scene:
