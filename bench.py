@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--reorder", type=float, default=0.85, help="reorder_particles! threshold per step")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward ratio leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (reference for HBM counter scaling)")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--checkpoint-every", type=int, default=20)
     return ap.parse_args()
@@ -128,6 +130,16 @@ def main():
     from tesserae_hip.scenes import block_drop
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counter readings scale the
+        # engine's kernels' readings to bytes
+        src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"bench: copy probe {args.copy_probe} MiB x3")
     log("bench: building scene")
     sc = block_drop(dtype=torch.float32, dx=args.dx)
     mesh, P, G = to_gpu(sc, layout=args.layout)
@@ -173,7 +185,7 @@ def main():
     dom_bytes = prof[dom][2] / prof[dom][1]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     out = {
-        "metric": "3D MPM particle-steps/s at 13M pcls",
+        "metric": "3D MPM particle-steps/s at 13M pcls, % of HBM peak; adjoint/forward time ratio",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,

@@ -62,6 +62,20 @@ __device__ __forceinline__ void bspline_jet2(double r, double& N, double& dN, do
   }
 }
 
+// Jets of node lo+k along axis d (zero past hi), derivatives in x.
+template <typename T, int D, int BASIS>
+__device__ __forceinline__ void node_jet2(const Args<T, D>& a, T x, int d, int node, int hi, double& n0, double& n1,
+                                          double& n2) {
+  n0 = n1 = n2 = 0.0;
+  if (node <= hi) {
+    const double hinv = 1.0 / a.mesh.h_d;
+    const double r = (double(x) - (a.mesh.xmin_d[d] + double(node) * a.mesh.h_d)) * hinv;
+    bspline_jet2<BASIS>(r, n0, n1, n2);
+    n1 *= hinv;
+    n2 *= hinv * hinv;
+  }
+}
+
 // Per-axis jets of a particle's window: node lo[d]+k for k < SW (zero past hi).
 template <typename T, int D, int BASIS, bool SECOND>
 __device__ __forceinline__ void window_jets2(const Args<T, D>& a, const T* x, const int* lo, const int* hi,
@@ -331,102 +345,203 @@ __global__ __launch_bounds__(256) void k_adj_gather(Args<T, D> a, const double* 
   }
   __syncthreads();
   const double dt = double(a.dt), mp = double(a.mp);
+  constexpr int NR = Pow<SW, D - 1>::v;
   for (int t = int(threadIdx.x); t < cnt; t += int(blockDim.x)) {
     const uint32_t p = a.perm[start + t];
     T x[D], l;
     int lo[D], hi[D];
     particle_window<T, D, BASIS, false>(a, p, x, l, lo, hi, nullptr);
-    double N[D][SW], dN[D][SW], ddN[D][SW];
-    window_jets2<T, D, BASIS, true>(a, x, lo, hi, N, dN, ddN);
-    double xb[D], vb[D], Fb[D * D], F[D * D], Pm[D * D], vp[D], Lb[D * D];
-    const double Vp = double(a.V(p));
+    // x-axis jets kept in registers; the other axes are evaluated per offset
+    double N[1][SW], dN[1][SW], ddN[1][SW];
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-      xb[c] = ax(p, c);
-      vb[c] = av(p, c);
-      vp[c] = double(a.v(p, c));
-    }
-#pragma unroll
-    for (int k = 0; k < D * D; ++k) {
-      Fb[k] = aF(p, k);
-      F[k] = double(a.F(p, k));
-      Pm[k] = Vp * double(a.sigma(p, k));
-    }
-#pragma unroll
-    for (int r = 0; r < D; ++r)
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < D; ++q) s += Fb[r + D * q] * F[c + D * q];
-        Lb[r + D * c] = dt * s;
-      }
+    for (int k = 0; k < SW; ++k) node_jet2<T, D, BASIS>(a, x[0], 0, lo[0] + k, hi[0], N[0][k], dN[0][k], ddN[0][k]);
     int base[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) base[d] = lo[d] - origin[d];
-    double gx[D], gq[D], Pb[D * D], L[D * D];
+    for (int d = 0; d < D; ++d) base[d] = lo[d] - origin[d];  // the class coordinate, <= BW-1
+    // Sums are factored by axis: for each offset of the other axes (rest), x-sums
+    // of node data weighted by N_x (A), N_x' (B), N_x'' (C), combined with the
+    // rest's jets.  With R = Π_{d>0} N_d, R_c = N_c' Π_{d>0,d!=c} N_d,
+    // R_cc = N_c'' Π.., R_ce = N_c' N_e' Π..:  w = N_x R, ∇w = (N_x' R, N_x R_c),
+    // ∇∇w = [[N_x'' R, N_x' R_c], [N_x' R_c, N_x R_cc | N_x R_ce]].
+    // Two passes keep the Float64 working set in registers: the ∇∇w terms of x̄
+    // (with L̄ and P), then the ∇w / w terms (x̄, v̄, P̄, L).
+    double gx[D];
 #pragma unroll
-    for (int c = 0; c < D; ++c) gx[c] = gq[c] = 0.0;
-#pragma unroll
-    for (int k = 0; k < D * D; ++k) Pb[k] = L[k] = 0.0;
-    constexpr int NOFF = Pow<SW, D>::v;
-#pragma unroll
-    for (int s = 0; s < NOFF; ++s) {
-      int o[D], r = s, q = 0, mul = 1;
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        o[d] = r % SW;
-        r /= SW;
-        q += (base[d] + o[d]) * mul;
-        mul *= G::TS;
-      }
-      double w = 1.0, g[D], H[D][D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) w *= N[d][o[d]];
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        double gg = 1.0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) gg *= (d == c) ? dN[d][o[d]] : N[d][o[d]];
-        g[c] = gg;
-#pragma unroll
-        for (int e = 0; e < D; ++e) {
-          double h = 1.0;
-#pragma unroll
-          for (int d = 0; d < D; ++d) {
-            if (c == e) h *= (d == c) ? ddN[d][o[d]] : N[d][o[d]];
-            else h *= (d == c || d == e) ? dN[d][o[d]] : N[d][o[d]];
-          }
-          H[c][e] = h;
-        }
-      }
-      const double* nd = s_n[q];  // base[d] = class coordinate <= BW-1, so q stays inside the footprint
-      double sdot = mp * nd[4 * D];
-      double u[D];
-#pragma unroll
-      for (int c = 0; c < D; ++c) sdot += vb[c] * nd[D + c] + dt * xb[c] * nd[c] + mp * nd[3 * D + c] * vp[c];
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        double acc = 0.0;
-#pragma unroll
-        for (int r2 = 0; r2 < D; ++r2) acc += Lb[r2 + D * c] * nd[r2] - Pm[r2 + D * c] * nd[2 * D + r2];
-        u[c] = acc;
-      }
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        double acc = g[c] * sdot;
-#pragma unroll
-        for (int e = 0; e < D; ++e) acc += H[c][e] * u[e];
-        gx[c] += acc;
-        gq[c] += w * nd[3 * D + c];
-      }
+    for (int c = 0; c < D; ++c) gx[c] = 0.0;
+    {
+      double Lb[D * D], Pm[D * D];
+      const double Vp = double(a.V(p));
 #pragma unroll
       for (int r2 = 0; r2 < D; ++r2)
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-          Pb[r2 + D * c] -= nd[2 * D + r2] * g[c];
-          L[r2 + D * c] += nd[r2] * g[c];
+          double s2 = 0.0;
+#pragma unroll
+          for (int q = 0; q < D; ++q) s2 += aF(p, r2 + D * q) * double(a.F(p, c + D * q));
+          Lb[r2 + D * c] = dt * s2;  // L̄ = Δt F̄' Fᵀ
+          Pm[r2 + D * c] = Vp * double(a.sigma(p, r2 + D * c));
         }
+#pragma unroll 1
+      for (int rs = 0; rs < NR; ++rs) {
+        int o[D];
+        o[0] = 0;
+        int r = rs, qrow = 0, mul = G::TS;
+#pragma unroll
+        for (int d = 1; d < D; ++d) {
+          o[d] = r % SW;
+          r /= SW;
+          qrow += (base[d] + o[d]) * mul;
+          mul *= G::TS;
+        }
+        double rn[D], rd[D], rdd[D];
+#pragma unroll
+        for (int d = 1; d < D; ++d) node_jet2<T, D, BASIS>(a, x[d], d, lo[d] + o[d], hi[d], rn[d], rd[d], rdd[d]);
+        // A/B x-sums of v and f̄, and C of u0 = (L̄ᵀ v - Pᵀ f̄)_0
+        double Av[D], Bv[D], Af[D], Bf[D], Cu0 = 0.0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) Av[c] = Bv[c] = Af[c] = Bf[c] = 0.0;
+#pragma unroll
+        for (int ix = 0; ix < SW; ++ix) {
+          const double* nd = s_n[qrow + base[0] + ix];
+          const double n0 = N[0][ix], n1 = dN[0][ix], n2 = ddN[0][ix];
+          double u0 = 0.0;
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            const double vg = nd[c], fb = nd[2 * D + c];
+            u0 += Lb[c] * vg - Pm[c] * fb;
+            Av[c] += n0 * vg;
+            Bv[c] += n1 * vg;
+            Af[c] += n0 * fb;
+            Bf[c] += n1 * fb;
+          }
+          Cu0 += n2 * u0;
+        }
+        double UA[D], UB[D];  // U(S)_e = Σ_r L̄_re S[v]_r - P_re S[f̄]_r
+#pragma unroll
+        for (int e = 0; e < D; ++e) {
+          double ua = 0.0, ub = 0.0;
+#pragma unroll
+          for (int r2 = 0; r2 < D; ++r2) {
+            ua += Lb[r2 + D * e] * Av[r2] - Pm[r2 + D * e] * Af[r2];
+            ub += Lb[r2 + D * e] * Bv[r2] - Pm[r2 + D * e] * Bf[r2];
+          }
+          UA[e] = ua;
+          UB[e] = ub;
+        }
+        double R = 1.0;
+#pragma unroll
+        for (int d = 1; d < D; ++d) R *= rn[d];
+        gx[0] += R * Cu0;
+#pragma unroll
+        for (int c = 1; c < D; ++c) {
+          double Rc = rd[c], Rcc = rdd[c];
+#pragma unroll
+          for (int d = 1; d < D; ++d)
+            if (d != c) {
+              Rc *= rn[d];
+              Rcc *= rn[d];
+            }
+          gx[0] += Rc * UB[c];
+          double tt = Rc * UB[0] + Rcc * UA[c];
+#pragma unroll
+          for (int e = 1; e < D; ++e)
+            if (e != c) {
+              double pce = rd[c] * rd[e];
+#pragma unroll
+              for (int d = 1; d < D; ++d)
+                if (d != c && d != e) pce *= rn[d];
+              tt += pce * UA[e];
+            }
+          gx[c] += tt;
+        }
+      }
+    }
+    double gq[D], Pb[D * D], L[D * D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) gq[c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) Pb[k] = L[k] = 0.0;
+    {
+      double vb[D], dtxb[D], mpvp[D];
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        vb[c] = av(p, c);
+        dtxb[c] = dt * ax(p, c);
+        mpvp[c] = mp * double(a.v(p, c));
+      }
+#pragma unroll 1
+      for (int rs = 0; rs < NR; ++rs) {
+        int o[D];
+        o[0] = 0;
+        int r = rs, qrow = 0, mul = G::TS;
+#pragma unroll
+        for (int d = 1; d < D; ++d) {
+          o[d] = r % SW;
+          r /= SW;
+          qrow += (base[d] + o[d]) * mul;
+          mul *= G::TS;
+        }
+        double rn[D], rd[D], rdd_unused;
+#pragma unroll
+        for (int d = 1; d < D; ++d) node_jet2<T, D, BASIS>(a, x[d], d, lo[d] + o[d], hi[d], rn[d], rd[d], rdd_unused);
+        // A/B x-sums of v, f̄ and s = v̄'·Δv + Δt x̄'·v + mₚ m̄ + mₚ q̄·v_p, A of q̄
+        double Av[D], Bv[D], Af[D], Bf[D], Aq[D], As = 0.0, Bs = 0.0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) Av[c] = Bv[c] = Af[c] = Bf[c] = Aq[c] = 0.0;
+#pragma unroll
+        for (int ix = 0; ix < SW; ++ix) {
+          const double* nd = s_n[qrow + base[0] + ix];
+          const double n0 = N[0][ix], n1 = dN[0][ix];
+          double sv = mp * nd[4 * D];
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            const double vg = nd[c], fb = nd[2 * D + c], qb = nd[3 * D + c];
+            sv += vb[c] * nd[D + c] + dtxb[c] * vg + mpvp[c] * qb;
+            Av[c] += n0 * vg;
+            Bv[c] += n1 * vg;
+            Af[c] += n0 * fb;
+            Bf[c] += n1 * fb;
+            Aq[c] += n0 * qb;
+          }
+          As += n0 * sv;
+          Bs += n1 * sv;
+        }
+        double R = 1.0;
+#pragma unroll
+        for (int d = 1; d < D; ++d) R *= rn[d];
+        gx[0] += R * Bs;
+#pragma unroll
+        for (int r2 = 0; r2 < D; ++r2) {
+          gq[r2] += R * Aq[r2];
+          L[r2] += R * Bv[r2];
+          Pb[r2] -= R * Bf[r2];
+        }
+#pragma unroll
+        for (int c = 1; c < D; ++c) {
+          double Rc = rd[c];
+#pragma unroll
+          for (int d = 1; d < D; ++d)
+            if (d != c) Rc *= rn[d];
+          gx[c] += Rc * As;
+#pragma unroll
+          for (int r2 = 0; r2 < D; ++r2) {
+            L[r2 + D * c] += Rc * Av[r2];
+            Pb[r2 + D * c] -= Rc * Af[r2];
+          }
+        }
+      }
+    }
+    // particle side: x̄, v̄, F̄ = (I+ΔtL)ᵀ F̄' + V0 (μ (P̄+P̄ᵀ) F + λ tr(P̄) F⁻ᵀ), ∂/∂logE += P̄ : P
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      ax(p, c) = ax(p, c) + gx[c];
+      av(p, c) = av(p, c) + mp * gq[c];
+    }
+    double F[D * D], Fb[D * D];
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) {
+      F[k] = double(a.F(p, k));
+      Fb[k] = aF(p, k);
     }
     double lam = a.lambda, mu = a.mu;
     if (a.logE.p) {
@@ -434,8 +549,7 @@ __global__ __launch_bounds__(256) void k_adj_gather(Args<T, D> a, const double* 
       lam *= e;
       mu *= e;
     }
-    // F⁻¹ by the adjugate
-    double Fi[D * D];
+    double Fi[D * D];  // F⁻¹ by the adjugate
     if constexpr (D == 2) {
       const double det = F[0] * F[3] - F[2] * F[1];
       Fi[0] = F[3] / det;
@@ -457,11 +571,12 @@ __global__ __launch_bounds__(256) void k_adj_gather(Args<T, D> a, const double* 
 #pragma unroll
         for (int c = 0; c < 3; ++c) Fi[r2 + 3 * c] = cof[c + 3 * r2] / det;
     }
+    const double Vp = double(a.V(p));
     double trPb = 0.0, PbP = 0.0;
 #pragma unroll
     for (int c = 0; c < D; ++c) trPb += Pb[c + D * c];
 #pragma unroll
-    for (int k = 0; k < D * D; ++k) PbP += Pb[k] * Pm[k];
+    for (int k = 0; k < D * D; ++k) PbP += Pb[k] * (Vp * double(a.sigma(p, k)));
     double Fn[D * D];
 #pragma unroll
     for (int r2 = 0; r2 < D; ++r2)
@@ -477,11 +592,6 @@ __global__ __launch_bounds__(256) void k_adj_gather(Args<T, D> a, const double* 
       }
 #pragma unroll
     for (int k = 0; k < D * D; ++k) aF(p, k) = Fn[k];
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      ax(p, c) = xb[c] + gx[c];
-      av(p, c) = vb[c] + mp * gq[c];
-    }
     if (ath.p) ath(p) += PbP;
   }
 }

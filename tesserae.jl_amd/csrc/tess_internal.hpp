@@ -51,6 +51,8 @@ struct Binning {
   bool state_valid = false;
   int64_t state_n = -1, state_ntiles = -1;
   int32_t state_dim = 0;
+  // the last G2P wrote the next keys and stay flags at the sorted positions
+  bool keys_ready = false;
 };
 
 // Per-kernel timing with HIP events on the context's stream.  Disabled by

@@ -216,6 +216,25 @@ __device__ __forceinline__ uint32_t particle_key(const Args<T, D>& a, int64_t p,
   return t * 64u + c;
 }
 
+// The same key from a position and particle length held in registers
+// (deferred weights); equals particle_key of the stored position.
+template <typename T, int D, int BASIS>
+__device__ __forceinline__ uint32_t key_of_position(const Args<T, D>& a, const T* x, T l, uint32_t ntiles) {
+  T xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = (x[d] - a.mesh.xmin[d]) * a.mesh.h_inv;
+  if (!inside_xi<T, D>(xi, a.mesh)) return ntiles * 64u;
+  uint32_t t = 0, c = 0;
+#pragma unroll
+  for (int d = D - 1; d >= 0; --d) {
+    int lo, hi;
+    axis_window<T, BASIS>(xi[d], l, a.mesh.dims[d], lo, hi);
+    t = t * uint32_t(a.ntiles[d]) + uint32_t((lo + 1) >> Geo<D>::LT);
+    c = c * uint32_t(Geo<D>::BW) + uint32_t((lo + 1) & (Geo<D>::BW - 1));
+  }
+  return t * 64u + c;
+}
+
 template <typename T, int D, int BASIS, bool STORED>
 __global__ __launch_bounds__(256) void k_bin_keys(Args<T, D> a, uint32_t* __restrict__ keys, uint32_t ntiles,
                                                   uint32_t* __restrict__ counters) {
@@ -703,15 +722,42 @@ __device__ __forceinline__ void axis_jets_mixed(T xd, T xmin, int lo_local, int 
   for (int q = 0; q < SW; ++q) der[q] *= hinv;
 }
 
+// One particle's P2G inputs, loaded one iteration ahead of their use.
+template <typename T, int D>
+struct P2GIn {
+  uint32_t pid;
+  T x[D], v[D], V, S[D * D], l;
+};
+
+template <typename T, int D, int BASIS, bool AOS>
+__device__ __forceinline__ void p2g_load(const Args<T, D>& a, uint32_t pid, P2GIn<T, D>& in) {
+  in.pid = pid;
+#pragma unroll
+  for (int d = 0; d < D; ++d) in.x[d] = fld<T, D, AOS>(a.x, pid, d);
+#pragma unroll
+  for (int d = 0; d < D; ++d) in.v[d] = fld<T, D, AOS>(a.v, pid, d);
+  in.V = a.V(pid);
+#pragma unroll
+  for (int q = 0; q < D * D; ++q) in.S[q] = fld<T, D * D, AOS>(a.sigma, pid, q);
+  in.l = T(0);
+  if constexpr (BASIS == TESS_UGIMP) in.l = a.l(pid) / a.mesh.h;
+}
+
+// NPASS waves per tile: wave w accumulates the offsets w*KN .. w*KN+KN-1
+// along the last axis for every class (the particles are read by each wave;
+// the registers of one wave hold 1/NPASS of the stencil, which keeps several
+// waves per SIMD resident).  The waves add their stencils into the LDS tile
+// one after another, so the per-node order of the additions is fixed.
 template <typename T, int D, int BASIS, int NPASS, bool AOS>
-__global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
-                                                  uint32_t* __restrict__ misplaced) {
+__global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                          uint32_t* __restrict__ misplaced) {
   using G = Geo<D>;
   constexpr int SW = support_width(BASIS);
   constexpr int NV = 1 + 2 * D;
   constexpr int KN = SW / NPASS;                       // last-axis offsets per pass
   constexpr int NSL = Pow<SW, D - 1>::v * KN;          // offsets per pass
-  const int lane = threadIdx.x;
+  constexpr int NT = 64 * NPASS;
+  const int lane = threadIdx.x & 63, wave = int(threadIdx.x >> 6);
   const int64_t j = xcd_remap(blockIdx.x, a.n_active);
   if (j >= a.n_active) return;
   const uint32_t tile = a.active[j];
@@ -726,13 +772,13 @@ __global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* _
   __shared__ __align__(16) T s_tile[G::NFOOT][8];
   __shared__ T s_nodex[D][G::TS];
   __shared__ uint32_t s_cnt[64];
-  for (int q = lane; q < 8 * G::NFOOT; q += 64) (&s_tile[0][0])[q] = T(0);
-  for (int q = lane; q < D * G::TS; q += 64) {
+  for (int q = int(threadIdx.x); q < 8 * G::NFOOT; q += NT) (&s_tile[0][0])[q] = T(0);
+  for (int q = int(threadIdx.x); q < D * G::TS; q += NT) {
     const int d = q / G::TS, k = q % G::TS;
     s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
   }
   int ccount, coff, maxcnt;
-  tile_class_counts(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);
+  tile_class_counts<NPASS>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);  // barrier after the staging
   int ccoord[D];
   {
     int r = lane;
@@ -742,10 +788,11 @@ __global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* _
       r /= G::BW;
     }
   }
-  check_sparse_support<T, D, SW>(a, origin, ccoord, ccount);
+  if (wave == 0) check_sparse_support<T, D, SW>(a, origin, ccoord, ccount);
   const T mp = a.mp, hinv = a.mesh.h_inv;
   uint32_t bad = 0, layer_off = 0;
-  for (int pass = 0; pass < NPASS; ++pass) {
+  {
+    const int pass = wave;
     T am[NSL], amv[D][NSL], af[D][NSL];
 #pragma unroll
     for (int o = 0; o < NSL; ++o) {
@@ -756,24 +803,27 @@ __global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* _
         af[c][o] = T(0);
       }
     }
+    P2GIn<T, D> cur, nxt;
+    if (ccount > 0) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff], nxt);
     for (int k = 0; k < maxcnt; ++k) {
       const uint64_t has = __ballot(k < ccount);
+      cur = nxt;
+      if (k + 1 < ccount) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + 1], nxt);
       if (k < ccount) {
-        const uint32_t pid = a.perm[start + coff + k];
+        const uint32_t pid = cur.pid;
         if (misplaced && pass == 0) {
           const uint32_t want = start + layer_off + uint32_t(__popcll(has & lanemask_lt64()));
           bad += pid != want ? 1u : 0u;
         }
         T x[D], mpv[D], A[D * D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) x[d] = fld<T, D, AOS>(a.x, pid, d);
+        for (int d = 0; d < D; ++d) x[d] = cur.x[d];
 #pragma unroll
-        for (int d = 0; d < D; ++d) mpv[d] = mp * fld<T, D, AOS>(a.v, pid, d);  // hoisted mₚ*v[p]
-        const T negV = -a.V(pid);
+        for (int d = 0; d < D; ++d) mpv[d] = mp * cur.v[d];  // hoisted mₚ*v[p]
+        const T negV = -cur.V;
 #pragma unroll
-        for (int q = 0; q < D * D; ++q) A[q] = negV * fld<T, D * D, AOS>(a.sigma, pid, q);  // hoisted (-V[p])*σ[p]
-        T l = T(0);
-        if constexpr (BASIS == TESS_UGIMP) l = a.l(pid) / a.mesh.h;
+        for (int q = 0; q < D * D; ++q) A[q] = negV * cur.S[q];  // hoisted (-V[p])*σ[p]
+        const T l = cur.l;
         T val[D][SW], der[D][SW];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -841,8 +891,10 @@ __global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* _
     }
     // lanes are distinct classes, so for one offset their target nodes are
     // distinct; offsets are applied one after another (read, add, write, wait
-    // for the LDS queue), which fixes the per-node order of the additions
-    if (ccount > 0) {
+    // for the LDS queue) and the waves one after another, which fixes the
+    // per-node order of the additions
+    for (int w = 0; w < NPASS; ++w) {
+      if (w == wave && ccount > 0) {
 #pragma unroll
       for (int o = 0; o < NSL; ++o) {
         int r = o, q = 0, mul = 1;
@@ -866,16 +918,17 @@ __global__ __launch_bounds__(64) void k_p2g_fast(Args<T, D> a, const uint32_t* _
         for (int v = 0; v < 8; ++v) s_tile[q][v] = t[v];
         __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
       }
+      }
+      __syncthreads();
     }
   }
-  if (misplaced) {
+  if (misplaced && wave == 0) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) bad += __shfl_xor(bad, off, 64);
     if (lane == 0) misplaced[j] = bad;  // per tile, summed later
   }
-  __syncthreads();
   T* out = a.partials + int64_t(j) * 8 * G::NFOOT;
-  for (int q = lane; q < 8 * G::NFOOT; q += 64) out[q] = (&s_tile[0][0])[q];
+  for (int q = int(threadIdx.x); q < 8 * G::NFOOT; q += NT) out[q] = (&s_tile[0][0])[q];
 }
 
 // G2P with the footprint's v and v - vⁿ staged per node in LDS; the FLIP
@@ -901,8 +954,13 @@ __device__ __forceinline__ void g2p_load(const Args<T, D>& a, uint32_t pid, G2PI
   if constexpr (BASIS == TESS_UGIMP) in.l = a.l(pid) / a.mesh.h;
 }
 
+// With nk != nullptr the kernel also writes, at each particle's sorted position
+// j, the sort key of its new position (nk[j]) and whether it equals the
+// current one (stay[j]): the next step's incremental binning starts from them.
 template <typename T, int D, int BASIS, bool AOS, int NW>
-__global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys) {
+__global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                       uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
+                                                       uint32_t ntiles) {
   using G = Geo<D>;
   constexpr int SW = support_width(BASIS);
   const int lane = threadIdx.x & 63, wave = int(threadIdx.x >> 6);
@@ -1043,8 +1101,18 @@ __global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32
     for (int c = 0; c < D; ++c) fldw<T, D, AOS>(a.v, pid, c) = cur.v[c] + DV[c];
 #pragma unroll
     for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.gradv, pid, q) = L[q];
+    T xn[D];
 #pragma unroll
-    for (int c = 0; c < D; ++c) fldw<T, D, AOS>(a.x, pid, c) = cur.x[c] + X[c] * dt;
+    for (int c = 0; c < D; ++c) {
+      xn[c] = cur.x[c] + X[c] * dt;
+      fldw<T, D, AOS>(a.x, pid, c) = xn[c];
+    }
+    if (nk) {
+      const uint32_t jj = start + uint32_t(coff + k);
+      const uint32_t key = key_of_position<T, D, BASIS>(a, xn, cur.l, ntiles);
+      nk[jj] = key;
+      stay[jj] = key == skeys[jj] ? 1u : 0u;
+    }
     T Fn[D * D];
 #pragma unroll
     for (int b = 0; b < D; ++b)
@@ -1367,10 +1435,15 @@ __global__ __launch_bounds__(64) void k_g2p_tile(Args<T, D> a, const uint32_t* _
 // Particles outside the mesh have an empty window: the sums are zero and only
 // the particle equations run (g2p.jl:123-153).
 template <typename T, int D>
-__global__ __launch_bounds__(256) void k_g2p_outside(Args<T, D> a) {
+__global__ __launch_bounds__(256) void k_g2p_outside(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                     uint32_t* __restrict__ nk, uint32_t* __restrict__ stay) {
   const int64_t j = a.n_inside + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= a.n) return;
   const uint32_t pid = a.perm[j];
+  if (nk) {  // the position does not change: the key stays
+    nk[j] = skeys[j];
+    stay[j] = 1u;
+  }
   T dv[D], L[D * D], dx[D];
 #pragma unroll
   for (int c = 0; c < D; ++c) {
@@ -1470,7 +1543,8 @@ int key_bits_for(uint64_t maxkey) {
 }
 
 template <typename T, int D, int BASIS, bool STORED>
-void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const tess_weights* W) {
+void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const tess_weights* W,
+                   bool keys_from_g2p = false) {
   Args<T, D> a = make_args<T, D>(mesh, nullptr, P, W, nullptr);
   const int64_t n = P.n;
   uint64_t ntiles = 1;
@@ -1496,13 +1570,20 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
   const int kbits = key_bits_for(ntiles * 64);
   const bool incremental = b.state_valid && b.state_n == n && b.state_ntiles == int64_t(ntiles) &&
                            b.state_dim == D && n > 0;
+  const bool precomputed = keys_from_g2p && b.keys_ready;
+  b.keys_ready = false;
   if (n > 0 && incremental) {
-    // keys in the previous sorted order, stayer flags -> stayer ranks
+    // keys in the previous sorted order, stayer flags -> stayer ranks; the
+    // previous G2P of the same step loop may have written them already
     uint32_t* nk = ctx->inc_nk.as<uint32_t>(size_t(n));
     uint32_t* spos = ctx->inc_spos.as<uint32_t>(size_t(n) + 1);
-    hipLaunchKernelGGL((k_keys_sorted<T, D, BASIS, STORED>), dim3(blocks_for(n + 1, 256)), dim3(256), 0,
-                       ctx->stream, a, perm, skeys, uint32_t(ntiles), nk, spos, counters);
-    TESS_HIP(hipGetLastError());
+    if (precomputed) {
+      TESS_HIP(hipMemsetAsync(spos + n, 0, sizeof(uint32_t), ctx->stream));
+    } else {
+      hipLaunchKernelGGL((k_keys_sorted<T, D, BASIS, STORED>), dim3(blocks_for(n + 1, 256)), dim3(256), 0,
+                         ctx->stream, a, perm, skeys, uint32_t(ntiles), nk, spos, counters);
+      TESS_HIP(hipGetLastError());
+    }
     exclusive_scan_u32(ctx, spos, n + 1, counters + CNT_STAY);
     uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
     TESS_HIP(hipMemcpyAsync(hc + CNT_STAY, counters + CNT_STAY, sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -1658,12 +1739,13 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
       hipLaunchKernelGGL((k_p2g_tile<T, D, BASIS, STORED, NPASS>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
                          ctx->stream, a, sk, misplaced);
     } else {
-      constexpr int NPF = (Pow<SW, D>::v * NV * int(sizeof(T)) / 4 <= 192) ? 1 : SW;
+      // 3D: one wave per offset along z; 2D: one wave while the stencil fits
+      constexpr int NPF = (D == 3 || Pow<SW, D>::v * NV * int(sizeof(T)) / 4 > 96) ? SW : 1;
       if (julia_layout(P, D))
-        hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, true>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
+        hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, true>), dim3(xcd_grid(b.n_active)), dim3(64 * NPF), 0,
                            ctx->stream, a, sk, misplaced);
       else
-        hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, false>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
+        hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, false>), dim3(xcd_grid(b.n_active)), dim3(64 * NPF), 0,
                            ctx->stream, a, sk, misplaced);
     }
     TESS_HIP(hipGetLastError());
@@ -1710,11 +1792,19 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
 
 template <typename T, int D, int BASIS, bool STORED>
 void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-                const tess_weights* W, const tess_params& prm, const tess_sparsity* sp) {
+                const tess_weights* W, const tess_params& prm, const tess_sparsity* sp, bool write_keys = false) {
   const Binning& b = ctx->bin;
   Args<T, D> a = make_args<T, D>(mesh, &G, P, W, &prm);
   fill_binning_ptrs<T, D>(ctx, a);
   set_sparse<T, D>(a, mesh, sp);
+  // next-step keys at the sorted positions (deferred weights only)
+  uint32_t* nk = nullptr;
+  uint32_t* stay = nullptr;
+  if (write_keys && !STORED && P.n > 0) {
+    nk = ctx->inc_nk.as<uint32_t>(size_t(P.n));
+    stay = ctx->inc_spos.as<uint32_t>(size_t(P.n) + 1);
+  }
+  const uint32_t* skeys_now = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
   if (b.n_active > 0) {
     // algorithmic bytes: x, v, F read and v, ∇v, x, F, σ, V written per
     // particle; vⁿ, v read per active node
@@ -1728,22 +1818,24 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
                          sk);
     } else {
       constexpr int NW = 4;
+      const uint32_t nt = uint32_t(b.ntiles);
       if (julia_layout(P, D))
         hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
-                           ctx->stream, a, sk);
+                           ctx->stream, a, sk, nk, stay, nt);
       else
         hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, false, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
-                           ctx->stream, a, sk);
+                           ctx->stream, a, sk, nk, stay, nt);
     }
     TESS_HIP(hipGetLastError());
     prof_end(ctx, tok);
   }
   if (P.n > b.n_inside) {
     hipLaunchKernelGGL((k_g2p_outside<T, D>), dim3(blocks_for(P.n - b.n_inside, 256)), dim3(256), 0, ctx->stream,
-                       a);
+                       a, skeys_now, nk, stay);
     TESS_HIP(hipGetLastError());
   }
   ctx->bin.valid = false;  // positions moved
+  ctx->bin.keys_ready = nk != nullptr;
 }
 
 // Layered storage order of the binned particles: inside each tile the k-th
@@ -1838,7 +1930,9 @@ void step_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const t
         raise(TESS_ERR_STATE, "SpGrid storage holds " + std::to_string(sp->capacity) + " nodes but the active blocks need " +
                                   std::to_string(sp->storage_len) + "; resize the grid arrays and call again");
     }
-    bin_particles<T, D, BASIS, false>(ctx, mesh, P, nullptr);
+    // from the second step on, the previous G2P wrote the keys (same call, so
+    // nothing else moved the particles in between)
+    bin_particles<T, D, BASIS, false>(ctx, mesh, P, nullptr, s > 0);
     const Binning& b = ctx->bin;
     if (reorder_threshold > 0.0 && b.n_inside > 0) {
       bool go = reorder_threshold >= 1.0;
@@ -1852,7 +1946,7 @@ void step_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const t
       }
     }
     p2g_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, reorder_threshold > 0.0, sp);
-    g2p_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, sp);
+    g2p_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, sp, s + 1 < nsteps);
   }
 }
 

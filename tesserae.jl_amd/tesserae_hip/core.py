@@ -693,6 +693,8 @@ def step(grid: Grid, particles: Particles, basis: Basis, params: MPMParams, nste
     if isinstance(grid, SpGrid):
         # update_sparsity!(grid, x) runs inside every step; the storage grows when the
         # active blocks outgrow it and the step is retried (nothing of it has run yet)
+        if grid.capacity == 0:
+            update_sparsity(grid, particles.x, ctx=ctx)
         s = 0
         while s < nsteps:
             sp, G = grid.sparsity_abi(), grid.abi()

@@ -147,7 +147,7 @@ def test_incremental_binning_matches_oracle_and_is_deterministic(oracle):
     step (incremental re-binning by merge), with reorders: two contexts give
     bitwise-identical particles, and the trajectory equals the oracle's once
     particles are matched by identity."""
-    sc = block_drop(dx=1 / 48, steps=25)
+    sc = block_drop(dtype=torch.float64, dx=1 / 48, steps=25)
     # a drift of a third of a cell over the run plus noise
     sc.v[:] = np.random.default_rng(3).normal(scale=2.0, size=sc.v.shape) + np.array([3.0, -2.0, 1.0])
     sc.l = np.arange(sc.n, dtype=np.float64)  # identity tag carried along by the permutation

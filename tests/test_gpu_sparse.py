@@ -44,7 +44,7 @@ def test_sparsity_numbering_bit_exact(oracle, dim, L, dtype):
 @pytest.mark.parametrize("dim", [2, 3])
 def test_sparse_p2g_g2p_equal_dense(oracle, basis, dim):
     print(f"[sparse p2g/g2p] dim={dim} basis={basis}", flush=True)
-    sc = random_box(dim, 6000 if dim == 3 else 3000, 17, basis, dtype=torch.float64, h=0.05, lo=0.2, hi=0.55,
+    sc = random_box(dim, 6000 if dim == 3 else 3000, 17, basis, dtype=torch.float64, h=0.02, lo=0.2, hi=0.55,
                     stress=1.0, with_l=basis.kind == 16)
     sc.mesh.block_size_log2 = 3
     mesh, P, Gd = to_gpu(sc)
