@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--reorder", type=float, default=0.85, help="reorder_particles! threshold per step")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward ratio leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
+    ap.add_argument("--no-sphere", action="store_true", help="skip the sparse-grid sphere leg")
+    ap.add_argument("--sphere-dx", type=float, default=1 / 512)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (reference for HBM counter scaling)")
     ap.add_argument("--adjoint-steps", type=int, default=200)
@@ -114,6 +117,104 @@ def adjoint_ratio(nsteps, every):
                         f"{every}, FP32 forward / FP64 adjoint",
             "t_forward_ms": t_fwd, "t_rollout_ms": t_roll, "t_pullback_ms": t_adj,
             "ratio": (t_roll + t_adj) / t_fwd, "loss": loss, "grad_v0_norm": gnorm}
+
+
+def batch_leg(steps):
+    """256 uneven 2D scenes (quadratic, FP32) stepped as one batch, after the autotuner
+    picks the launch variant."""
+    import torch
+
+    import tesserae_hip as th
+    from gpu_util import to_gpu
+    from tesserae_hip.scenes import batched_scenes
+
+    scenes = batched_scenes(256, dtype=torch.float32)
+    members = []
+    for sc in scenes:
+        mesh, P, G = to_gpu(sc)
+        members.append((G, P, sc.params))
+    ntot = sum(sc.n for sc in scenes)
+    ctx = th.Context()
+    batch = th.Batch(members, th.BSpline(th.Quadratic()), ctx=ctx)
+    variant = batch.autotune(2)
+    batch.step(1)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    a.record()
+    batch.step(steps)
+    b.record()
+    torch.cuda.synchronize()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ms = a.elapsed_time(b) / steps
+    stats = ctx.stats()
+    bytes_step = 4 * (5 * 4 + 6 * 2 + 2) * ntot + 4 * (8 * 2 + 2) * max(stats["active_nodes"], 0)
+    batch.close()
+    ctx.close()
+    return {"workload": f"256 uneven 2D scenes (grid 64-512 cells, 2e3-2e5 particles each, {ntot} particles), "
+                        "quadratic, FP32, per-scene dt, one launch per kernel for the batch",
+            "particles": ntot, "ms_per_step": ms, "particle_steps_per_s": ntot / (ms * 1e-3),
+            "hbm_frac": bytes_step / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), "autotuned_variant": variant,
+            "kernels_avg_us": {k: v[0] / max(v[1], 1) * 1e3 for k, v in prof.items()}}
+
+
+def sphere_leg(dx, steps):
+    """Sparse-grid sphere (SpArray 8^3 blocks): 2x2x2 particles per cell inside r=0.3 at
+    dx, rigid rotation plus noise, cubic B-spline and uGIMP, FP32.  Positions are generated
+    on the device."""
+    import math
+
+    import torch
+
+    import tesserae_hip as th
+
+    out = {}
+    s = dx / 2
+    k0, k1 = int(math.floor(0.2 / s - 0.5)), int(math.ceil(0.8 / s - 0.5))
+    ax = (torch.arange(k0, k1 + 1, device="cuda", dtype=torch.float64) + 0.5) * s
+    parts = []
+    for z in ax.split(64):
+        X, Y, Z = torch.meshgrid(ax, ax, z, indexing="ij")
+        keep = (X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2 < 0.09
+        parts.append(torch.stack([X[keep], Y[keep], Z[keep]], dim=1).float())
+    x = torch.cat(parts)
+    del parts
+    n = x.shape[0]
+    for basis in (th.BSpline(th.Cubic()), th.uGIMP()):
+        mesh = th.CartesianMesh(dx, (0, 1), (0, 1), (0, 1), dtype=torch.float32, block_size_log2=3)
+        P = th.Particles(n, 3, torch.float32, with_l=basis.kind == 16)
+        P.x.copy_(x)
+        g = torch.Generator(device="cuda")
+        g.manual_seed(7)
+        v = torch.stack([-(x[:, 1] - 0.5), x[:, 0] - 0.5, torch.zeros_like(x[:, 0])], dim=1)  # ω = 2, times 0.5
+        P.v.copy_(v + 0.05 * torch.randn(n, 3, device="cuda", generator=g))
+        P.V.fill_(s ** 3)
+        P.F.copy_(torch.eye(3, device="cuda").reshape(1, 9).expand(n, 9))
+        if basis.kind == 16:
+            P.l.fill_(dx / 2)
+        params = th.MPMParams(dt=5e-5, E=5e5, nu=0.35, rho0=1e3, V0=s ** 3)
+        G = th.generate_grid(th.SpArray, mesh)
+        ctx = th.Context()
+        th.step(G, P, basis, params, nsteps=1, ctx=ctx)  # warm-up: storage, code loading
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        th.step(G, P, basis, params, nsteps=steps, ctx=ctx)
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / steps
+        st = ctx.stats()
+        # 260 B per particle (3D FP32), plus l read by both transfers for uGIMP; 104 B per active node
+        bytes_step = (260 + (8 if basis.kind == 16 else 0)) * n + 104 * max(st["active_nodes"], 0)
+        out[repr(basis)] = {"ms_per_step": ms, "particle_steps_per_s": n / (ms * 1e-3),
+                            "hbm_frac": bytes_step / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+                            "storage_nodes": G.spinds.storage_len, "dense_nodes": mesh.nnodes()}
+        ctx.close()
+        del P, G
+        torch.cuda.empty_cache()
+    return {"workload": f"sparse-grid sphere dx={dx:g}, {n} particles, SpArray 8^3 blocks, FP32", "particles": n,
+            "by_basis": out}
 
 
 def main():
@@ -207,6 +308,12 @@ def main():
         "kernels": kernels,
         "wall_s": wall,
     }
+    if not args.no_batch:
+        log("bench: batched scenes")
+        out["batch"] = batch_leg(20)
+    if not args.no_sphere:
+        log("bench: sparse-grid sphere")
+        out["sphere"] = sphere_leg(args.sphere_dx, 3)
     if not args.no_adjoint:
         log("bench: adjoint/forward ratio")
         out["adjoint"] = adjoint_ratio(args.adjoint_steps, args.checkpoint_every)

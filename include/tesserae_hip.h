@@ -281,6 +281,35 @@ tess_status tess_rollout(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, co
 tess_status tess_rollout_adjoint(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
                                  const tess_params* params, tess_adjoint* adj);
 
+/* ---- batches of scenes (no counterpart in the reference) ----------------- */
+
+/* One scene of a batch: its own mesh, grid, particles and constants (Δt
+ * included).  Fields use the Julia-native layout. */
+typedef struct tess_scene {
+  tess_mesh mesh;
+  tess_grid grid;
+  tess_particles particles;
+  tess_params params;
+} tess_scene;
+
+typedef struct tess_batch tess_batch;
+
+/* Many independent scenes of one dimension, dtype and basis, stepped together:
+ * the scenes' tiles and particles share one index space, so every kernel of a
+ * step runs once for the whole batch.  The scene arrays are referenced, not
+ * copied (keep them alive and in place).  Deferred weights, no reordering. */
+tess_status tess_batch_create(tess_ctx* ctx, int32_t basis, int32_t nscenes, const tess_scene* scenes,
+                              tess_batch** out);
+/* nsteps README steps of every scene of the batch (each with its own Δt). */
+tess_status tess_batch_step(tess_ctx* ctx, tess_batch* batch, int32_t nsteps);
+/* Times each launch variant (waves per tile in P2G and G2P) over
+ * steps_per_variant steps of the batch after one untimed step, keeps the
+ * fastest and writes its number to *variant.  The tuning steps are steps of
+ * the simulation: the scenes advance by (variants * (steps_per_variant + 1))
+ * steps.  Synchronises. */
+tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_per_variant, int32_t* variant);
+tess_status tess_batch_destroy(tess_batch* batch);
+
 /* ---- measurement (no counterpart in the reference) ----------------------- */
 
 /* Per-kernel timing with HIP events recorded on the context's stream around

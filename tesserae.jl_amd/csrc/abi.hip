@@ -461,6 +461,74 @@ tess_status tess_rollout_adjoint(tess_ctx* ctx, int32_t basis, const tess_mesh* 
   });
 }
 
+tess_status tess_batch_create(tess_ctx* ctx, int32_t basis, int32_t nscenes, const tess_scene* scenes,
+                              tess_batch** out) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_basis(basis);
+    if (!out) raise(TESS_ERR_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (nscenes < 1 || !scenes) raise(TESS_ERR_ARGUMENT, "a batch needs at least one scene");
+    for (int32_t s = 0; s < nscenes; ++s) {
+      check_mesh(&scenes[s].mesh);
+      check_grid(&scenes[s].grid);
+      check_particles(&scenes[s].particles, basis, true);
+      check_params(&scenes[s].params);
+    }
+    run_batch_create(ctx, basis, nscenes, scenes, out);
+  });
+}
+
+tess_status tess_batch_step(tess_ctx* ctx, tess_batch* batch, int32_t nsteps) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!batch) raise(TESS_ERR_ARGUMENT, "batch is NULL");
+    if (nsteps < 0) raise(TESS_ERR_ARGUMENT, "nsteps must be non-negative");
+    run_batch_step(ctx, batch, nsteps);
+  });
+}
+
+tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_per_variant, int32_t* variant) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!batch) raise(TESS_ERR_ARGUMENT, "batch is NULL");
+    if (steps_per_variant < 1) raise(TESS_ERR_ARGUMENT, "steps_per_variant must be positive");
+    const int dim = batch->dim;
+    const int candidates[4] = {0, 1, 2, 3};
+    const int ncand = dim == 2 ? 4 : 2;  // 3D: the P2G split is fixed, only the G2P waves vary
+    hipEvent_t e0, e1;
+    TESS_HIP(hipEventCreate(&e0));
+    TESS_HIP(hipEventCreate(&e1));
+    float best = 0.0f;
+    int best_v = run_batch_variant(batch);
+    for (int k = 0; k < ncand; ++k) {
+      const int v = dim == 2 ? candidates[k] : candidates[k] * 2;
+      run_batch_set_variant(batch, v);
+      run_batch_step(ctx, batch, 1);  // untimed: code loading and first-use allocations
+      TESS_HIP(hipEventRecord(e0, ctx->stream));
+      run_batch_step(ctx, batch, steps_per_variant);
+      TESS_HIP(hipEventRecord(e1, ctx->stream));
+      TESS_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      TESS_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (k == 0 || ms < best) {
+        best = ms;
+        best_v = v;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    run_batch_set_variant(batch, best_v);
+    if (variant) *variant = best_v;
+  });
+}
+
+tess_status tess_batch_destroy(tess_batch* batch) {
+  return guarded([&] {
+    if (batch) run_batch_destroy(batch);
+  });
+}
+
 tess_status tess_profile_enable(tess_ctx* ctx, int32_t on) {
   return guarded([&] {
     check_ctx(ctx);

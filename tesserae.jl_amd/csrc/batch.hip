@@ -1,0 +1,302 @@
+// Batches of independent scenes (the engine's batching extension; no
+// counterpart in the reference).  Scenes of one dimension, dtype and basis but
+// their own mesh, particle count, constants and Δt are stepped together: the
+// tiles and particles of all scenes share one index space (scene s owns tiles
+// [toff_s, toff_s+1) and particles [poff_s, poff_s+1)), so each kernel of a
+// step — keys, sort, tile runs, P2G, gather, G2P — is one launch for the whole
+// batch.  Tile kernels look their scene up from the tile index
+// (scene_of/bind_scene in transfer_impl.hpp).  An autotuner times the launch
+// variants (waves per tile in P2G and G2P) on the batch itself.
+#include "transfer_impl.hpp"
+
+#include <vector>
+
+namespace tess {
+namespace {
+
+template <typename T, int D>
+SceneRec<T, D> scene_record(const tess_scene& sc, int64_t toff, int64_t poff) {
+  SceneRec<T, D> r;
+  std::memset(static_cast<void*>(&r), 0, sizeof(r));
+  const Args<T, D> a = make_args<T, D>(sc.mesh, &sc.grid, sc.particles, nullptr, &sc.params);
+  r.mesh = a.mesh;
+  r.x = a.x;
+  r.v = a.v;
+  r.V = a.V;
+  r.gradv = a.gradv;
+  r.F = a.F;
+  r.sigma = a.sigma;
+  r.l = a.l;
+  r.logE = a.logE;
+  r.gm = a.gm;
+  r.gmv = a.gmv;
+  r.gf = a.gf;
+  r.gvn = a.gvn;
+  r.gv = a.gv;
+  r.mp = a.mp;
+  r.dt = a.dt;
+  r.lambda = a.lambda;
+  r.mu = a.mu;
+  r.V0 = a.V0;
+  for (int d = 0; d < D; ++d) {
+    r.gravity[d] = a.gravity[d];
+    r.ntiles[d] = a.ntiles[d];
+  }
+  r.slip = a.slip;
+  r.toff = toff;
+  r.poff = poff;
+  r.n = sc.particles.n;
+  r.nnodes = 1;
+  for (int d = 0; d < D; ++d) r.nnodes *= sc.mesh.dims[d];
+  return r;
+}
+
+template <typename T, int D>
+void batch_create_impl(tess_ctx* ctx, tess_batch& b) {
+  std::vector<SceneRec<T, D>> recs;
+  for (int s = 0; s < b.nscenes; ++s) recs.push_back(scene_record<T, D>(b.scenes[size_t(s)], b.toff[size_t(s)], b.poff[size_t(s)]));
+  void* p = b.recs.get(recs.size() * sizeof(SceneRec<T, D>));
+  TESS_HIP(hipMemcpyAsync(p, recs.data(), recs.size() * sizeof(SceneRec<T, D>), hipMemcpyHostToDevice, ctx->stream));
+  TESS_HIP(hipMemcpyAsync(b.d_toff.get(b.toff.size() * 8), b.toff.data(), b.toff.size() * 8, hipMemcpyHostToDevice,
+                          ctx->stream));
+  TESS_HIP(hipMemcpyAsync(b.d_poff.get(b.poff.size() * 8), b.poff.data(), b.poff.size() * 8, hipMemcpyHostToDevice,
+                          ctx->stream));
+  TESS_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+template <typename T, int D>
+Args<T, D> batch_args(tess_ctx* ctx, tess_batch& b) {
+  Args<T, D> a;
+  std::memset(static_cast<void*>(&a), 0, sizeof(a));
+  a.scenes = static_cast<const SceneRec<T, D>*>(b.recs.ptr);
+  a.scene_toff = static_cast<const int64_t*>(b.d_toff.ptr);
+  a.scene_poff = static_cast<const int64_t*>(b.d_poff.ptr);
+  a.nscenes = b.nscenes;
+  a.n = b.ntot;
+  fill_binning_ptrs<T, D>(ctx, a);
+  return a;
+}
+
+// Full sort of the batch's keys every step (the batch-wide tile space), then the
+// tile runs, active tiles and touched node blocks, as bin_particles does.
+template <typename T, int D, int BASIS>
+void batch_bin(tess_ctx* ctx, tess_batch& B) {
+  const int64_t n = B.ntot;
+  const uint64_t ntiles = uint64_t(B.tiles_total);
+  Binning& b = ctx->bin;
+  b.valid = false;
+  b.state_valid = false;  // the single-scene incremental state does not describe a batch
+  b.keys_ready = false;
+  uint32_t* counters = ctx->counters.as<uint32_t>(CNT_SLOTS);
+  TESS_HIP(hipMemsetAsync(counters, 0, CNT_BIN_SLOTS * sizeof(uint32_t), ctx->stream));
+  uint32_t* keys = ctx->keys.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
+  uint32_t* skeys = ctx->keys_sorted.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
+  uint32_t* perm = ctx->vals.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
+  uint32_t* tstart = ctx->tile_start.as<uint32_t>(ntiles);
+  uint32_t* tend = ctx->tile_end.as<uint32_t>(ntiles);
+  uint32_t* tflag = ctx->tile_flag.as<uint32_t>(ntiles);
+  uint32_t* tslot = ctx->tile_slot.as<uint32_t>(ntiles);
+  uint32_t* active = ctx->active_tiles.as<uint32_t>(ntiles);
+  uint32_t* nbflag = ctx->nb_flag.as<uint32_t>(ntiles);
+  uint32_t* nbslot = ctx->nb_slot.as<uint32_t>(ntiles);
+  uint32_t* nblist = ctx->nb_list.as<uint32_t>(ntiles);
+  Args<T, D> a = batch_args<T, D>(ctx, B);
+  const int tok = prof_begin(ctx, "binning", 0.0);
+  hipLaunchKernelGGL((k_bin_keys<T, D, BASIS, false, true>), dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, a,
+                     keys, uint32_t(ntiles), counters);
+  TESS_HIP(hipGetLastError());
+  radix_sort_pairs(ctx, keys, nullptr, skeys, perm, n, key_bits_for(ntiles * 64));
+  TESS_HIP(hipMemsetAsync(tstart, 0, ntiles * sizeof(uint32_t), ctx->stream));
+  TESS_HIP(hipMemsetAsync(tend, 0, ntiles * sizeof(uint32_t), ctx->stream));
+  hipLaunchKernelGGL(k_tile_runs, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, skeys, n, uint32_t(ntiles),
+                     tstart, tend, counters);
+  TESS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_tile_flags, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, tstart, tend,
+                     uint32_t(ntiles), tflag, tslot);
+  TESS_HIP(hipGetLastError());
+  exclusive_scan_u32(ctx, tslot, int64_t(ntiles), counters + CNT_ACTIVE);
+  hipLaunchKernelGGL(k_compact, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, tflag, tslot,
+                     uint32_t(ntiles), active);
+  TESS_HIP(hipGetLastError());
+  TESS_HIP(hipMemsetAsync(nbflag, 0, ntiles * sizeof(uint32_t), ctx->stream));
+  hipLaunchKernelGGL((k_nb_mark_batch<T, D>), dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, a, tflag,
+                     uint32_t(ntiles), nbflag);
+  TESS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_copy_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, nbflag, nbslot,
+                     int64_t(ntiles));
+  TESS_HIP(hipGetLastError());
+  exclusive_scan_u32(ctx, nbslot, int64_t(ntiles), counters + CNT_NB);
+  hipLaunchKernelGGL(k_compact, dim3(blocks_for(ntiles, 256)), dim3(256), 0, ctx->stream, nbflag, nbslot,
+                     uint32_t(ntiles), nblist);
+  TESS_HIP(hipGetLastError());
+  prof_end(ctx, tok);
+  uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
+  TESS_HIP(hipMemcpyAsync(hc, counters, CNT_SLOTS * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  TESS_HIP(hipStreamSynchronize(ctx->stream));
+  if (hc[CNT_ERR] & ERR_GIMP_LENGTH) raise(TESS_ERR_ARGUMENT, "uGIMP requires 0 ≤ pt.l ≤ spacing(mesh)");
+  b.n_inside = n > 0 ? int64_t(hc[CNT_INSIDE]) : 0;
+  ctx->last_active_nodes = int64_t(hc[CNT_NODES]);  // written by the previous step's gather
+  b.n_active = int64_t(hc[CNT_ACTIVE]);
+  b.n_nodeblocks = int64_t(hc[CNT_NB]);
+  b.ntiles = int64_t(ntiles);
+  b.n = n;
+  b.valid = true;
+}
+
+template <typename T, int D, int BASIS, int NP>
+void launch_p2g(tess_ctx* ctx, const Args<T, D>& a, int64_t n_active, const uint32_t* sk) {
+  hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NP, true, true>), dim3(xcd_grid(n_active)), dim3(64 * NP), 0,
+                     ctx->stream, a, sk, static_cast<uint32_t*>(nullptr));
+  TESS_HIP(hipGetLastError());
+}
+
+template <typename T, int D, int BASIS, int NW>
+void launch_g2p(tess_ctx* ctx, const Args<T, D>& a, int64_t n_active, const uint32_t* sk) {
+  hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW, true>), dim3(xcd_grid(n_active)), dim3(64 * NW), 0,
+                     ctx->stream, a, sk, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
+                     uint32_t(ctx->bin.ntiles));
+  TESS_HIP(hipGetLastError());
+}
+
+template <typename T, int D, int BASIS>
+void batch_step_impl(tess_ctx* ctx, tess_batch& B, int nsteps) {
+  using Gm = Geo<D>;
+  constexpr int SW = support_width(BASIS);
+  for (int s = 0; s < nsteps; ++s) {
+    batch_bin<T, D, BASIS>(ctx, B);
+    const Binning& b = ctx->bin;
+    Args<T, D> a = batch_args<T, D>(ctx, B);
+    // zero what this step does not overwrite: every grid the first time, then
+    // the node blocks written last step and not now
+    if (!B.zeroed) {
+      hipLaunchKernelGGL((k_zero_grid_batch<T, D>), dim3(blocks_for(B.max_nodes, 256), unsigned(B.nscenes)), dim3(256),
+                         0, ctx->stream, a);
+      TESS_HIP(hipGetLastError());
+      B.zeroed = true;
+    } else if (B.prev_nb > 0) {
+      hipLaunchKernelGGL((k_zero_stale<T, D, true>), dim3(unsigned(B.prev_nb)), dim3(64), 0, ctx->stream, a,
+                         static_cast<const uint32_t*>(B.prev_list.ptr), B.prev_nb);
+      TESS_HIP(hipGetLastError());
+    }
+    const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
+    uint32_t* counters = static_cast<uint32_t*>(ctx->counters.ptr);
+    if (b.n_active > 0) {
+      a.partials = ctx->partials.as<T>(size_t(b.n_active) * 8 * Gm::NFOOT);
+      const int tok = prof_begin(ctx, "k_p2g_tile", double(b.n_inside) * sizeof(T) * (2 * D + 1 + D * D));
+      if (D == 3 || (B.variant & 1)) launch_p2g<T, D, BASIS, SW>(ctx, a, b.n_active, sk);
+      else launch_p2g<T, D, BASIS, 1>(ctx, a, b.n_active, sk);
+      prof_end(ctx, tok);
+    }
+    if (b.n_nodeblocks > 0) {
+      uint32_t* per_nb = ctx->per_nodeblock.as<uint32_t>(size_t(b.n_nodeblocks));
+      const int tok = prof_begin(ctx, "k_gather_grid", 0.0);
+      hipLaunchKernelGGL((k_gather_grid<T, D, true, true>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream,
+                         a, per_nb);
+      TESS_HIP(hipGetLastError());
+      prof_end(ctx, tok);
+      sum_u32(ctx, per_nb, b.n_nodeblocks, counters + CNT_NODES);
+      uint32_t* prev = B.prev_list.as<uint32_t>(size_t(b.n_nodeblocks));
+      TESS_HIP(hipMemcpyAsync(prev, ctx->nb_list.ptr, size_t(b.n_nodeblocks) * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    B.prev_nb = b.n_nodeblocks;
+    if (b.n_active > 0) {
+      const int tok = prof_begin(ctx, "k_g2p_tile",
+                                 double(b.n_inside) * sizeof(T) * (2 * D + D * D + 2 * D + 3 * D * D + 1));
+      if (B.variant & 2) launch_g2p<T, D, BASIS, 2>(ctx, a, b.n_active, sk);
+      else launch_g2p<T, D, BASIS, 4>(ctx, a, b.n_active, sk);
+      prof_end(ctx, tok);
+    }
+    if (B.ntot > b.n_inside) {
+      hipLaunchKernelGGL((k_g2p_outside<T, D, true>), dim3(blocks_for(B.ntot - b.n_inside, 256)), dim3(256), 0,
+                         ctx->stream, a, sk, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr));
+      TESS_HIP(hipGetLastError());
+    }
+  }
+  // the single-scene zeroing bookkeeping no longer describes any grid
+  ctx->last_grid_m = nullptr;
+  ctx->bin.valid = false;
+}
+
+template <typename T, int D, int B_, bool S>
+struct BatchStepOp {
+  static void run(tess_ctx* ctx, const tess_mesh&, tess_batch& b, int nsteps) {
+    if constexpr (S) raise(TESS_ERR_UNSUPPORTED, "batches evaluate the basis inside the transfers");
+    else batch_step_impl<T, D, B_>(ctx, b, nsteps);
+  }
+};
+template <typename T, int D, int B_, bool S>
+struct BatchCreateOp {
+  static void run(tess_ctx* ctx, const tess_mesh&, tess_batch& b) {
+    if constexpr (!S) batch_create_impl<T, D>(ctx, b);
+  }
+};
+
+bool julia_fields(const tess_scene& sc, int D) {
+  auto pok = [](const tess_field& f, int nc) { return !f.ptr || (f.comp_stride == 1 && f.elem_stride == nc); };
+  const tess_particles& P = sc.particles;
+  const tess_grid& G = sc.grid;
+  return pok(P.x, D) && pok(P.v, D) && pok(P.V, 1) && pok(P.grad_v, D * D) && pok(P.F, D * D) &&
+         pok(P.sigma, D * D) && pok(P.l, 1) && pok(P.logE, 1) && pok(G.m, 1) && pok(G.mv, D) && pok(G.f, D) &&
+         pok(G.vn, D) && pok(G.v, D);
+}
+
+}  // namespace
+
+void run_batch_create(tess_ctx* ctx, int basis, int nscenes, const tess_scene* scenes, tess_batch** out) {
+  auto* b = new tess_batch();
+  try {
+    b->basis = basis;
+    b->nscenes = nscenes;
+    b->dim = scenes[0].mesh.dim;
+    b->dtype = scenes[0].mesh.dtype;
+    b->scenes.assign(scenes, scenes + nscenes);
+    b->toff.assign(size_t(nscenes) + 1, 0);
+    b->poff.assign(size_t(nscenes) + 1, 0);
+    const int LT = b->dim == 3 ? Geo<3>::LT : Geo<2>::LT;
+    for (int s = 0; s < nscenes; ++s) {
+      const tess_scene& sc = scenes[s];
+      if (sc.mesh.dim != b->dim || sc.mesh.dtype != b->dtype)
+        raise(TESS_ERR_ARGUMENT, "all scenes of a batch share the mesh dimension and dtype");
+      if (!julia_fields(sc, b->dim))
+        raise(TESS_ERR_UNSUPPORTED, "batched scenes use the Julia-native field layout");
+      int64_t nt = 1, nn = 1;
+      for (int d = 0; d < b->dim; ++d) {
+        nt *= ((sc.mesh.dims[d] - 1) >> LT) + 1;
+        nn *= sc.mesh.dims[d];
+      }
+      b->toff[size_t(s) + 1] = b->toff[size_t(s)] + nt;
+      b->poff[size_t(s) + 1] = b->poff[size_t(s)] + sc.particles.n;
+      b->max_nodes = std::max(b->max_nodes, nn);
+    }
+    b->ntot = b->poff.back();
+    b->tiles_total = b->toff.back();
+    if (uint64_t(b->tiles_total) * 64 >= uint64_t(UINT32_MAX) || b->ntot >= int64_t(UINT32_MAX))
+      raise(TESS_ERR_ARGUMENT, "the batch exceeds the 32-bit binning range");
+    dispatch<BatchCreateOp>(basis, scenes[0].mesh, false, ctx, scenes[0].mesh, *b);
+  } catch (...) {
+    b->recs.release();
+    b->d_toff.release();
+    b->d_poff.release();
+    delete b;
+    throw;
+  }
+  *out = b;
+}
+
+void run_batch_step(tess_ctx* ctx, tess_batch* b, int nsteps) {
+  dispatch<BatchStepOp>(b->basis, b->scenes[0].mesh, false, ctx, b->scenes[0].mesh, *b, nsteps);
+}
+
+void run_batch_set_variant(tess_batch* b, int variant) { b->variant = variant; }
+int run_batch_variant(const tess_batch* b) { return b->variant; }
+
+void run_batch_destroy(tess_batch* b) {
+  b->recs.release();
+  b->d_toff.release();
+  b->d_poff.release();
+  b->prev_list.release();
+  delete b;
+}
+
+}  // namespace tess

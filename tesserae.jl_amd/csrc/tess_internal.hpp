@@ -127,6 +127,19 @@ struct tess_ctx {
   tess::Rollout rollout;
 };
 
+// A batch of scenes (batch.hip): the scenes as given, where each scene's tiles
+// and particles start in the batch-wide index spaces, and the device copies.
+struct tess_batch {
+  int32_t basis = 0, dim = 0, dtype = 0, nscenes = 0;
+  int64_t ntot = 0, tiles_total = 0, max_nodes = 0;
+  std::vector<tess_scene> scenes;
+  std::vector<int64_t> toff, poff;
+  tess::Buffer recs, d_toff, d_poff, prev_list;
+  int64_t prev_nb = 0;
+  bool zeroed = false;
+  int variant = 0;  // bit 0: P2G one wave per last-axis offset (2D), bit 1: G2P with 2 waves instead of 4
+};
+
 namespace tess {
 
 // ---- scans and sorts (sort.hip) ----
@@ -175,6 +188,12 @@ void run_rollout(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_gri
                  const tess_params& prm, int nsteps, int every);
 void run_rollout_adjoint(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G,
                          const tess_params& prm, const tess_adjoint& adj);
+// ---- batches (batch.hip) ----
+void run_batch_create(tess_ctx* ctx, int basis, int nscenes, const tess_scene* scenes, tess_batch** out);
+void run_batch_step(tess_ctx* ctx, tess_batch* b, int nsteps);
+void run_batch_set_variant(tess_batch* b, int variant);
+int run_batch_variant(const tess_batch* b);
+void run_batch_destroy(tess_batch* b);
 // Moves every particle field so that slot k receives particle order[k].
 void permute_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const uint32_t* order);
 

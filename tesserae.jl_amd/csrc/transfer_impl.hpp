@@ -64,6 +64,9 @@ enum {
 enum { ERR_GIMP_LENGTH = 1, ERR_WINDOW = 2, ERR_INACTIVE = 4 };
 
 template <typename T, int D>
+struct SceneRec;
+
+template <typename T, int D>
 struct Args {
   DMesh<T, D> mesh;
   DField<T> x, v, V, gradv, F, sigma, l, logE;
@@ -84,6 +87,12 @@ struct Args {
   int32_t spL;
   int32_t nblk[D];
   uint32_t* p2g_err;
+  // batches of scenes: per-scene records and the first global tile / particle
+  // of each scene (nscenes+1 entries); nullptr for a single scene
+  const SceneRec<T, D>* scenes;
+  const int64_t* scene_toff;
+  const int64_t* scene_poff;
+  int32_t nscenes;
   int64_t n, n_inside, n_active, n_nb;
   int32_t ntiles[D];
   T mp, dt;
@@ -91,6 +100,78 @@ struct Args {
   T gravity[D];
   int32_t slip;
 };
+
+// One scene of a batch: what differs between scenes in Args, plus where the
+// scene's tiles and particles start in the batch-wide index spaces.
+template <typename T, int D>
+struct SceneRec {
+  DMesh<T, D> mesh;
+  DField<T> x, v, V, gradv, F, sigma, l, logE;
+  DField<T> gm, gmv, gf, gvn, gv;
+  T mp, dt;
+  double lambda, mu, V0;
+  T gravity[D];
+  int32_t slip;
+  int32_t ntiles[D];
+  int64_t toff, poff, n, nnodes;
+};
+
+// Largest s with off[s] <= g (off ascending, off[0] = 0).
+__device__ __forceinline__ int scene_of(const int64_t* __restrict__ off, int ns, int64_t g) {
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Replace the scene-dependent part of a by scene s of the batch.
+template <typename T, int D>
+__device__ __forceinline__ const SceneRec<T, D>& bind_scene(Args<T, D>& a, int s) {
+  const SceneRec<T, D>& r = a.scenes[s];
+  a.mesh = r.mesh;
+  a.x = r.x;
+  a.v = r.v;
+  a.V = r.V;
+  a.gradv = r.gradv;
+  a.F = r.F;
+  a.sigma = r.sigma;
+  a.l = r.l;
+  a.logE = r.logE;
+  a.gm = r.gm;
+  a.gmv = r.gmv;
+  a.gf = r.gf;
+  a.gvn = r.gvn;
+  a.gv = r.gv;
+  a.mp = r.mp;
+  a.dt = r.dt;
+  a.lambda = r.lambda;
+  a.mu = r.mu;
+  a.V0 = r.V0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    a.gravity[d] = r.gravity[d];
+    a.ntiles[d] = r.ntiles[d];
+  }
+  a.slip = r.slip;
+  return r;
+}
+
+// A tile kernel's tile: in a batch, binds the scene holding global tile gtile
+// and returns the scene-local tile, with the scene's first particle in poff.
+template <bool BATCH, typename T, int D>
+__device__ __forceinline__ uint32_t bind_tile(Args<T, D>& a, uint32_t gtile, uint32_t& poff) {
+  poff = 0;
+  if constexpr (!BATCH) {
+    return gtile;
+  } else {
+    const SceneRec<T, D>& r = bind_scene(a, scene_of(a.scene_toff, a.nscenes, gtile));
+    poff = uint32_t(r.poff);
+    return gtile - uint32_t(r.toff);
+  }
+}
 
 // Workgroup i -> logical index; consecutive logical indices stay on one XCD
 // (workgroups are dealt to the 8 XCDs round-robin).
@@ -235,11 +316,20 @@ __device__ __forceinline__ uint32_t key_of_position(const Args<T, D>& a, const T
   return t * 64u + c;
 }
 
-template <typename T, int D, int BASIS, bool STORED>
+template <typename T, int D, int BASIS, bool STORED, bool BATCH = false>
 __global__ __launch_bounds__(256) void k_bin_keys(Args<T, D> a, uint32_t* __restrict__ keys, uint32_t ntiles,
                                                   uint32_t* __restrict__ counters) {
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= a.n) return;
+  if constexpr (BATCH) {  // batch: scene-local key, shifted to the scene's first global tile
+    const SceneRec<T, D>& r = bind_scene(a, scene_of(a.scene_poff, a.nscenes, p));
+    uint32_t nl = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) nl *= uint32_t(r.ntiles[d]);
+    const uint32_t k = particle_key<T, D, BASIS, STORED>(a, p - r.poff, nl, counters + CNT_ERR);
+    keys[p] = k == nl * 64u ? ntiles * 64u : k + uint32_t(r.toff) * 64u;
+    return;
+  }
   keys[p] = particle_key<T, D, BASIS, STORED>(a, p, ntiles, counters + CNT_ERR);
 }
 
@@ -412,6 +502,34 @@ __global__ void k_nb_mark(const uint32_t* __restrict__ tile_flag, uint32_t ntile
       }
 }
 
+// The same for a batch: neighbours are taken in the tile's own scene.
+template <typename T, int D>
+__global__ void k_nb_mark_batch(Args<T, D> a, const uint32_t* __restrict__ tile_flag, uint32_t ntiles,
+                                uint32_t* __restrict__ nb_flag) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= ntiles || !tile_flag[t]) return;
+  uint32_t poff;
+  const uint32_t lt = bind_tile<true>(a, uint32_t(t), poff);
+  const uint32_t toff = uint32_t(t) - lt;
+  int I[3] = {0, 0, 0};
+  int64_t r = lt;
+  for (int d = 0; d < D; ++d) {
+    I[d] = int(r % a.ntiles[d]);
+    r /= a.ntiles[d];
+  }
+  for (int dz = (D == 3 ? -1 : 0); dz <= (D == 3 ? 1 : 0); ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int J[3] = {I[0] + dx, I[1] + dy, I[2] + dz};
+        bool ok = true;
+        for (int d = 0; d < D; ++d) ok = ok && J[d] >= 0 && J[d] < a.ntiles[d];
+        if (!ok) continue;
+        int64_t l = J[D - 1];
+        for (int d = D - 2; d >= 0; --d) l = l * a.ntiles[d] + J[d];
+        nb_flag[toff + l] = 1u;
+      }
+}
+
 __global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int64_t n) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
@@ -435,8 +553,16 @@ __global__ __launch_bounds__(256) void k_zero_grid(Args<T, D> a, int64_t nnodes)
   if (i < nnodes) zero_node(a, i);
 }
 
-// Zero the nodes of node blocks touched last call but not this call.
+// Every node of every scene of a batch (blockIdx.y = scene).
 template <typename T, int D>
+__global__ __launch_bounds__(256) void k_zero_grid_batch(Args<T, D> a) {
+  const SceneRec<T, D>& r = bind_scene(a, int(blockIdx.y));
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < r.nnodes) zero_node(a, i);
+}
+
+// Zero the nodes of node blocks touched last call but not this call.
+template <typename T, int D, bool BATCH = false>
 __global__ __launch_bounds__(64) void k_zero_stale(Args<T, D> a, const uint32_t* __restrict__ prev_list,
                                                     int64_t n_prev) {
   using G = Geo<D>;
@@ -444,8 +570,10 @@ __global__ __launch_bounds__(64) void k_zero_stale(Args<T, D> a, const uint32_t*
   if (j >= n_prev) return;
   const uint32_t B = prev_list[j];
   if (a.nb_flag[B]) return;
+  uint32_t poff;
+  const uint32_t Bl = bind_tile<BATCH>(a, B, poff);
   int Bi[D], I[D];
-  unlin<D>(B, a.ntiles, Bi);
+  unlin<D>(Bl, a.ntiles, Bi);
   int r = threadIdx.x;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
@@ -748,7 +876,7 @@ __device__ __forceinline__ void p2g_load(const Args<T, D>& a, uint32_t pid, P2GI
 // the registers of one wave hold 1/NPASS of the stencil, which keeps several
 // waves per SIMD resident).  The waves add their stencils into the LDS tile
 // one after another, so the per-node order of the additions is fixed.
-template <typename T, int D, int BASIS, int NPASS, bool AOS>
+template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false>
 __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                           uint32_t* __restrict__ misplaced) {
   using G = Geo<D>;
@@ -763,8 +891,10 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
   const uint32_t tile = a.active[j];
   const uint32_t start = a.tile_start[tile];
   const int cnt = int(a.tile_end[tile] - start);
+  uint32_t poff;  // a batch's scene: local tile, first particle
+  const uint32_t ltile = bind_tile<BATCH>(a, tile, poff);
   int tI[D], origin[D];
-  unlin<D>(tile, a.ntiles, tI);
+  unlin<D>(ltile, a.ntiles, tI);
 #pragma unroll
   for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
 
@@ -804,11 +934,11 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
       }
     }
     P2GIn<T, D> cur, nxt;
-    if (ccount > 0) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff], nxt);
+    if (ccount > 0) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff] - poff, nxt);
     for (int k = 0; k < maxcnt; ++k) {
       const uint64_t has = __ballot(k < ccount);
       cur = nxt;
-      if (k + 1 < ccount) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + 1], nxt);
+      if (k + 1 < ccount) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + 1] - poff, nxt);
       if (k < ccount) {
         const uint32_t pid = cur.pid;
         if (misplaced && pass == 0) {
@@ -957,7 +1087,7 @@ __device__ __forceinline__ void g2p_load(const Args<T, D>& a, uint32_t pid, G2PI
 // With nk != nullptr the kernel also writes, at each particle's sorted position
 // j, the sort key of its new position (nk[j]) and whether it equals the
 // current one (stay[j]): the next step's incremental binning starts from them.
-template <typename T, int D, int BASIS, bool AOS, int NW>
+template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false>
 __global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                        uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
                                                        uint32_t ntiles) {
@@ -969,8 +1099,10 @@ __global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32
   const uint32_t tile = a.active[j];
   const uint32_t start = a.tile_start[tile];
   const int cnt = int(a.tile_end[tile] - start);
+  uint32_t poff;  // a batch's scene: local tile, first particle
+  const uint32_t ltile = bind_tile<BATCH>(a, tile, poff);
   int tI[D], origin[D];
-  unlin<D>(tile, a.ntiles, tI);
+  unlin<D>(ltile, a.ntiles, tI);
 #pragma unroll
   for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
   // per node: v[D] and v-vⁿ[D], padded to a power-of-two stride for vector LDS reads
@@ -1014,10 +1146,10 @@ __global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32
   }
   const T hinv = a.mesh.h_inv, dt = a.dt;
   G2PIn<T, D, AOS> cur, nxt;
-  if (wave < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + wave], nxt);
+  if (wave < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + wave] - poff, nxt);
   for (int k = wave; k < maxcnt; k += NW) {
     cur = nxt;
-    if (k + NW < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + NW], nxt);
+    if (k + NW < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + NW] - poff, nxt);
     if (k >= ccount) continue;
     T val[D][SW], der[D][SW];
 #pragma unroll
@@ -1109,7 +1241,11 @@ __global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32
     }
     if (nk) {
       const uint32_t jj = start + uint32_t(coff + k);
-      const uint32_t key = key_of_position<T, D, BASIS>(a, xn, cur.l, ntiles);
+      uint32_t nl = 1;  // the scene's tiles; in a batch its keys start at its first global tile
+#pragma unroll
+      for (int d = 0; d < D; ++d) nl *= uint32_t(a.ntiles[d]);
+      const uint32_t kl = key_of_position<T, D, BASIS>(a, xn, cur.l, nl);
+      const uint32_t key = kl == nl * 64u ? ntiles * 64u : kl + (tile - ltile) * 64u;
       nk[jj] = key;
       stay[jj] = key == skeys[jj] ? 1u : 0u;
     }
@@ -1149,7 +1285,7 @@ __global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32
 }
 
 // ---- gather + grid-node half ------------------------------------------------
-template <typename T, int D, bool GAOS>
+template <typename T, int D, bool GAOS, bool BATCH = false>
 __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __restrict__ nodes_per_block) {
   using G = Geo<D>;
   constexpr int NV = 1 + 2 * D;
@@ -1157,8 +1293,11 @@ __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __re
   const int64_t j = xcd_remap(blockIdx.x, a.n_nb);
   if (j >= a.n_nb) return;
   const uint32_t B = a.nb_list[j];
+  uint32_t poff;  // a batch's scene: local node block; tiles of the scene start at toff
+  const uint32_t Bl = bind_tile<BATCH>(a, B, poff);
+  const uint32_t toff = B - Bl;
   int Bi[D], I[D], rr[D];
-  unlin<D>(B, a.ntiles, Bi);
+  unlin<D>(Bl, a.ntiles, Bi);
   // slots of the 3^dim neighbour tiles (-1: no partial)
   __shared__ int s_slot[NB];
   if (threadIdx.x < NB) {
@@ -1172,7 +1311,7 @@ __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __re
       t += b * mul;
       mul *= a.ntiles[d];
     }
-    s_slot[threadIdx.x] = (ok && a.tile_flag[t]) ? int(a.tile_slot[t]) : -1;
+    s_slot[threadIdx.x] = (ok && a.tile_flag[toff + t]) ? int(a.tile_slot[toff + t]) : -1;
   }
   __syncthreads();
   {
@@ -1434,12 +1573,13 @@ __global__ __launch_bounds__(64) void k_g2p_tile(Args<T, D> a, const uint32_t* _
 
 // Particles outside the mesh have an empty window: the sums are zero and only
 // the particle equations run (g2p.jl:123-153).
-template <typename T, int D>
+template <typename T, int D, bool BATCH = false>
 __global__ __launch_bounds__(256) void k_g2p_outside(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                      uint32_t* __restrict__ nk, uint32_t* __restrict__ stay) {
   const int64_t j = a.n_inside + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= a.n) return;
-  const uint32_t pid = a.perm[j];
+  uint32_t pid = a.perm[j];
+  if constexpr (BATCH) pid -= uint32_t(bind_scene(a, scene_of(a.scene_poff, a.nscenes, pid)).poff);
   if (nk) {  // the position does not change: the key stays
     nk[j] = skeys[j];
     stay[j] = 1u;

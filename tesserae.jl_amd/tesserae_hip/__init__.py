@@ -4,7 +4,7 @@ Host-side mirror of the reference API over the C ABI of libtesserae_hip
 (include/tesserae_hip.h).  Importing this package does not touch the GPU; the
 shared library is loaded on first use and its absence is an error.
 """
-from .core import (Adjoint, ArgumentError, Basis, BasisWeights, BSpline, CartesianMesh, Context, Cubic, DeviceError,
+from .core import (Adjoint, ArgumentError, Batch, Basis, BasisWeights, BSpline, CartesianMesh, Context, Cubic, DeviceError,
                    DimensionMismatch, ErrorException, G2P, Grid, Linear, MPMParams, P2G, Particles, Partition,
                    Quadratic, SpArray, SpGrid, SpIndices, TesseraeAssertionError, TesseraeError,
                    UnsupportedError, default_context, generate_basis_weights, generate_grid, generate_particles,
@@ -13,7 +13,7 @@ from .core import (Adjoint, ArgumentError, Basis, BasisWeights, BSpline, Cartesi
 from ._abi import LIB_PATH, load_library
 
 __all__ = [
-    "Adjoint", "step_adjoint", "rollout", "rollout_adjoint", "ArgumentError", "Basis", "BasisWeights", "BSpline", "CartesianMesh", "Context", "Cubic", "DeviceError",
+    "Adjoint", "Batch", "step_adjoint", "rollout", "rollout_adjoint", "ArgumentError", "Basis", "BasisWeights", "BSpline", "CartesianMesh", "Context", "Cubic", "DeviceError",
     "DimensionMismatch", "ErrorException", "G2P", "Grid", "Linear", "MPMParams", "P2G", "Particles", "Partition",
     "Quadratic", "SpArray", "SpGrid", "SpIndices", "sparse_storage_index", "update_sparsity",
     "TesseraeAssertionError", "TesseraeError", "UnsupportedError", "default_context",

@@ -111,6 +111,15 @@ class tess_params(ctypes.Structure):
     ]
 
 
+class tess_scene(ctypes.Structure):
+    _fields_ = [
+        ("mesh", tess_mesh),
+        ("grid", tess_grid),
+        ("particles", tess_particles),
+        ("params", tess_params),
+    ]
+
+
 # Every symbol include/tesserae_hip.h declares, with its ctypes signature.
 _P = ctypes.POINTER
 SIGNATURES = {
@@ -181,6 +190,13 @@ SIGNATURES = {
         ctypes.c_int32,
         [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_params), _P(tess_adjoint)],
     ),
+    "tess_batch_create": (
+        ctypes.c_int32,
+        [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _P(tess_scene), _P(ctypes.c_void_p)],
+    ),
+    "tess_batch_step": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
+    "tess_batch_autotune": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, _P(ctypes.c_int32)]),
+    "tess_batch_destroy": (ctypes.c_int32, [ctypes.c_void_p]),
     "tess_profile_enable": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "tess_profile_read": (
         ctypes.c_int32,

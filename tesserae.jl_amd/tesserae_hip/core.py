@@ -767,3 +767,48 @@ def rollout_adjoint(grid: Grid, basis: Basis, params: MPMParams, adj: Adjoint, c
     G, A = grid.abi(), adj.abi()
     _check(ctx.lib, ctx.lib.tess_rollout_adjoint(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()),
                                                  ctypes.byref(G), ctypes.byref(params.abi()), ctypes.byref(A)))
+
+
+# ---- batches of scenes ---------------------------------------------------------
+
+class Batch:
+    """Independent scenes (one dimension, dtype and basis) stepped together: every
+    kernel of a step is one launch for the whole batch.  ``scenes`` is a list of
+    (grid, particles, params) with Julia-layout fields; the arrays are referenced."""
+
+    def __init__(self, scenes, basis: Basis, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.basis = basis
+        self.members = list(scenes)
+        arr = (_abi.tess_scene * len(self.members))()
+        for k, (grid, particles, params) in enumerate(self.members):
+            _require_device(particles.x, grid.m)
+            arr[k].mesh = grid.mesh.abi()
+            arr[k].grid = grid.abi()
+            arr[k].particles = particles.abi()
+            arr[k].params = params.abi()
+        self._scenes = arr
+        handle = ctypes.c_void_p()
+        _check(self.ctx.lib, self.ctx.lib.tess_batch_create(self.ctx.handle, basis.kind, len(self.members), arr,
+                                                            ctypes.byref(handle)))
+        self.handle = handle
+
+    def step(self, nsteps: int = 1) -> None:
+        _check(self.ctx.lib, self.ctx.lib.tess_batch_step(self.ctx.handle, self.handle, int(nsteps)))
+
+    def autotune(self, steps_per_variant: int = 2) -> int:
+        v = ctypes.c_int32(0)
+        _check(self.ctx.lib, self.ctx.lib.tess_batch_autotune(self.ctx.handle, self.handle, int(steps_per_variant),
+                                                              ctypes.byref(v)))
+        return int(v.value)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            _check(self.ctx.lib, self.ctx.lib.tess_batch_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -165,6 +165,38 @@ def inverse_cube(dtype=torch.float32, dx: float = 1 / 64, seed: int = 5, steps: 
                  **_state(x, v, V0, 3))
 
 
+def batched_scenes(nscenes: int = 256, seed: int = 11, dtype=torch.float32, cells=(64, 512),
+                   particles=(2e3, 2e5), area=(0.2, 0.6), stiffness=(1e3, 1e6), steps: int = 200) -> list:
+    """Uneven 2D scenes for batching: per scene a grid of N cells per side on [0,1]^2 with N
+    log-uniform in ``cells``, n_p particles (log-uniform in ``particles``) uniform in a random
+    axis-aligned box covering a fraction of the unit square drawn from ``area``, velocities
+    U(-0.2, 0.2), E log-uniform in ``stiffness``, ν=0.3, ρ⁰=1e3, V⁰ = box area / n_p, quadratic
+    B-spline, Δt = 0.3 h / (c + max|v|) with c = sqrt((λ+2μ)/ρ⁰)."""
+    out = []
+    for s in range(nscenes):
+        u = uniform01(seed * 1000003 + s, 8)
+        ncell = int(round(math.exp(math.log(cells[0]) + u[0] * (math.log(cells[1]) - math.log(cells[0])))))
+        npart = int(round(math.exp(math.log(particles[0]) + u[1] * (math.log(particles[1]) - math.log(particles[0])))))
+        frac = area[0] + u[2] * (area[1] - area[0])
+        aspect = math.exp((u[3] - 0.5) * math.log(2.0))  # width/height within [1/sqrt2, sqrt2]
+        w = min(math.sqrt(frac * aspect), 0.98)
+        hgt = min(frac / w, 0.98)
+        x0, y0 = 0.01 + u[4] * (0.98 - w), 0.01 + u[5] * (0.98 - hgt)
+        E = math.exp(math.log(stiffness[0]) + u[6] * (math.log(stiffness[1]) - math.log(stiffness[0])))
+        h = 1.0 / ncell
+        mesh = CartesianMesh(h, (0, 1), (0, 1), dtype=dtype)
+        r = uniform01(seed * 7919 + s, 4 * npart).reshape(npart, 4)
+        x = np.stack([x0 + w * r[:, 0], y0 + hgt * r[:, 1]], axis=1)
+        v = (r[:, 2:4] - 0.5) * 0.4
+        V0 = w * hgt / npart
+        probe = MPMParams(dt=1.0, E=E, nu=0.3, rho0=1e3, V0=V0)
+        c = math.sqrt((probe.lam + 2 * probe.mu) / 1e3)
+        dt = 0.3 * h / (c + float(np.linalg.norm(v, axis=1).max()))
+        params = MPMParams(dt=dt, E=E, nu=0.3, rho0=1e3, V0=V0)
+        out.append(Scene(f"batch_{s}", mesh, BSpline(Quadratic()), params, steps=steps, **_state(x, v, V0, 2)))
+    return out
+
+
 def random_box(dim: int, n: int, seed: int, basis: Basis, dtype=torch.float64, h: float = 0.1,
                extent: float = 1.0, vscale: float = 0.3, lo: float = 0.0, hi: float = 1.0,
                stress: float = 0.0, with_l: bool = False) -> Scene:

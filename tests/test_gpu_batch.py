@@ -1,0 +1,66 @@
+"""Batches of uneven scenes: one launch per kernel for all scenes, each scene
+equal to the oracle's run of that scene alone (Float64, 1e-10)."""
+import numpy as np
+import pytest
+import torch
+
+import tesserae_hip as th
+from gpu_util import particles_np, rel_err, to_gpu
+from oracle_bind import HostState
+from tesserae_hip.scenes import batched_scenes, random_box
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_and_check(oracle, scenes, basis, nsteps, tune=0):
+    members, gpu = [], []
+    for sc in scenes:
+        mesh, P, G = to_gpu(sc)
+        members.append((G, P, sc.params))
+        gpu.append(P)
+    ctx = th.Context()
+    batch = th.Batch(members, basis, ctx=ctx)
+    extra = 0
+    if tune:
+        v = batch.autotune(tune)
+        assert v in (0, 1, 2, 3)
+        extra = (4 if scenes[0].mesh.dim == 2 else 2) * (tune + 1)
+    batch.step(nsteps)
+    ctx.sync()
+    for sc, P in zip(scenes, gpu):
+        st = HostState(sc)
+        oracle.step(st, basis, sc.params, nsteps + extra)
+        got = particles_np(P)
+        for name in ("x", "v", "F", "sigma", "V"):
+            assert rel_err(got[name], getattr(st, name)) <= 1e-10, (sc.name, name)
+    batch.close()
+    ctx.close()
+
+
+def test_batch_2d_uneven_scenes_match_oracle(oracle):
+    scenes = batched_scenes(12, seed=3, dtype=torch.float64, cells=(16, 64), particles=(100, 3000))
+    print(f"[batch2d] n={[s.n for s in scenes]}", flush=True)
+    _run_and_check(oracle, scenes, th.BSpline(th.Quadratic()), 5)
+
+
+def test_batch_autotune_then_steps(oracle):
+    scenes = batched_scenes(5, seed=9, dtype=torch.float64, cells=(16, 48), particles=(100, 2000))
+    _run_and_check(oracle, scenes, th.BSpline(th.Quadratic()), 2, tune=1)
+
+
+@pytest.mark.parametrize("basis", [th.BSpline(th.Linear()), th.BSpline(th.Cubic())], ids=["linear", "cubic"])
+def test_batch_3d_matches_oracle(oracle, basis):
+    scenes = [random_box(3, 800 + 300 * k, 40 + k, basis, dtype=torch.float64, h=0.1 / (k + 1), lo=0.1, hi=0.8,
+                         stress=0.5) for k in range(3)]
+    _run_and_check(oracle, scenes, basis, 3)
+
+
+def test_batch_rejects_mixed_dimensions():
+    a = random_box(2, 100, 1, th.BSpline(th.Quadratic()), dtype=torch.float64)
+    b = random_box(3, 100, 2, th.BSpline(th.Quadratic()), dtype=torch.float64)
+    members = []
+    for sc in (a, b):
+        mesh, P, G = to_gpu(sc)
+        members.append((G, P, sc.params))
+    with pytest.raises(th.ArgumentError):
+        th.Batch(members, th.BSpline(th.Quadratic()))
