@@ -1110,6 +1110,12 @@ __global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32
   __shared__ __align__(16) T s_n[G::NFOOT][NS];
   __shared__ T s_nodex[D][G::TS];
   __shared__ uint32_t s_cnt[64];
+  // per wave: the outputs of its 64 lanes, written back as contiguous runs when
+  // the wave's particles are stored contiguously (the layered order)
+  constexpr bool STAGE = AOS && sizeof(T) == 4;
+  constexpr int NOUT = 2 * D + 3 * D * D + 1;  // v, ∇v, x, F, σ, V
+  constexpr int NOP = NOUT | 1;                // odd row stride: lanes hit distinct banks
+  __shared__ T s_out[STAGE ? NW : 1][STAGE ? 64 : 1][STAGE ? NOP : 1];
   for (int q = int(threadIdx.x); q < G::NFOOT; q += 64 * NW) {
     int I[D], r = q;
     bool in = true;
@@ -1146,11 +1152,19 @@ __global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32
   }
   const T hinv = a.mesh.h_inv, dt = a.dt;
   G2PIn<T, D, AOS> cur, nxt;
+  std::memset(static_cast<void*>(&nxt), 0, sizeof(nxt));  // lanes without particles compute on zeros
   if (wave < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + wave] - poff, nxt);
   for (int k = wave; k < maxcnt; k += NW) {
     cur = nxt;
     if (k + NW < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + NW] - poff, nxt);
-    if (k >= ccount) continue;
+    const bool act = k < ccount;
+    // do the wave's active lanes update a contiguous run of storage slots?
+    const uint64_t has = __ballot(act);
+    const int rank = __popcll(has & lanemask_lt64());
+    const int first = __ffsll((unsigned long long)has) - 1;
+    const uint32_t base = uint32_t(__shfl(int(cur.pid) - rank, first, 64));
+    const bool contiguous = __ballot(act && cur.pid != base + uint32_t(rank)) == 0;
+    if (!STAGE && !act) continue;
     T val[D][SW], der[D][SW];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -1228,18 +1242,34 @@ __global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32
       }
     }
     const uint32_t pid = cur.pid;
-    // particle equations (README.md:81-86)
+    // particle equations (README.md:81-86).  Outputs go through one destination
+    // per field: the lane's LDS row in a staged wave (inactive lanes write the
+    // spare row 63, unused when a lane is inactive), else the particle itself.
+    DField<T> ov = a.v, og = a.gradv, ox = a.x, oF = a.F, os = a.sigma, oV = a.V;
+    int64_t oi = pid;
+    if constexpr (STAGE) {
+      // every lane parks its outputs in its LDS row (inactive lanes use the
+      // spare row 63, unused whenever a lane is inactive); the wave stores them below
+      T* row = s_out[STAGE ? wave : 0][STAGE ? (act ? rank : 63) : 0];
+      ov = DField<T>{row, D, 1};
+      og = DField<T>{row + D, D * D, 1};
+      ox = DField<T>{row + D + D * D, D, 1};
+      oF = DField<T>{row + 2 * D + D * D, D * D, 1};
+      os = DField<T>{row + 2 * D + 2 * D * D, D * D, 1};
+      oV = DField<T>{row + NOUT - 1, 1, 1};
+      oi = 0;
+    }
 #pragma unroll
-    for (int c = 0; c < D; ++c) fldw<T, D, AOS>(a.v, pid, c) = cur.v[c] + DV[c];
+    for (int c = 0; c < D; ++c) fldw<T, D, AOS>(ov, oi, c) = cur.v[c] + DV[c];
 #pragma unroll
-    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.gradv, pid, q) = L[q];
+    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(og, oi, q) = L[q];
     T xn[D];
 #pragma unroll
     for (int c = 0; c < D; ++c) {
       xn[c] = cur.x[c] + X[c] * dt;
-      fldw<T, D, AOS>(a.x, pid, c) = xn[c];
+      fldw<T, D, AOS>(ox, oi, c) = xn[c];
     }
-    if (nk) {
+    if (nk && act) {
       const uint32_t jj = start + uint32_t(coff + k);
       uint32_t nl = 1;  // the scene's tiles; in a batch its keys start at its first global tile
 #pragma unroll
@@ -1260,10 +1290,10 @@ __global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32
         Fn[r + D * b] = cur.F[r + D * b] + dt * t;
       }
 #pragma unroll
-    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.F, pid, q) = Fn[q];
+    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(oF, oi, q) = Fn[q];
     const double J = det_d<T, D>(Fn);
     double lam = a.lambda, mu = a.mu;
-    if (a.logE.p) {
+    if (a.logE.p && act) {
       const double sc = exp(double(a.logE(pid)));
       lam *= sc;
       mu *= sc;
@@ -1278,9 +1308,42 @@ __global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32
 #pragma unroll
         for (int kk = 0; kk < D; ++kk) bb += double(Fn[r + D * kk]) * double(Fn[b + D * kk]);
         const double sg = mu * (bb - (r == b ? 1.0 : 0.0)) + (r == b ? lam * lnJ : 0.0);
-        fldw<T, D * D, AOS>(a.sigma, pid, r + D * b) = T(sg * invJ);
+        fldw<T, D * D, AOS>(os, oi, r + D * b) = T(sg * invJ);
       }
-    a.V(pid) = T(a.V0 * J);
+    oV(oi) = T(a.V0 * J);
+    if constexpr (STAGE) {
+      __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      const int nact = __popcll(has);
+      T(*sw)[NOP] = s_out[STAGE ? wave : 0];
+      if (contiguous) {
+        // each field as one contiguous run of nact*ncomp scalars
+        auto run = [&](T* dst, int nc, int off) {
+#pragma unroll 1
+          for (int i = lane; i < nact * nc; i += 64) dst[int64_t(base) * nc + i] = sw[i / nc][off + i % nc];
+        };
+        run(a.v.p, D, 0);
+        run(a.gradv.p, D * D, D);
+        run(a.x.p, D, D + D * D);
+        run(a.F.p, D * D, 2 * D + D * D);
+        run(a.sigma.p, D * D, 2 * D + 2 * D * D);
+        run(a.V.p, 1, NOUT - 1);
+      } else if (act) {
+        // scattered slots: each lane stores its own row
+        const T* r = sw[rank];
+#pragma unroll
+        for (int c = 0; c < D; ++c) a.v.p[int64_t(pid) * D + c] = r[c];
+#pragma unroll
+        for (int q = 0; q < D * D; ++q) a.gradv.p[int64_t(pid) * D * D + q] = r[D + q];
+#pragma unroll
+        for (int c = 0; c < D; ++c) a.x.p[int64_t(pid) * D + c] = r[D + D * D + c];
+#pragma unroll
+        for (int q = 0; q < D * D; ++q) a.F.p[int64_t(pid) * D * D + q] = r[2 * D + D * D + q];
+#pragma unroll
+        for (int q = 0; q < D * D; ++q) a.sigma.p[int64_t(pid) * D * D + q] = r[2 * D + 2 * D * D + q];
+        a.V.p[pid] = r[NOUT - 1];
+      }
+      __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    }
   }
 }
 
