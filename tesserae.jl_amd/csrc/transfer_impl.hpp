@@ -902,6 +902,14 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
   __shared__ __align__(16) T s_tile[G::NFOOT][8];
   __shared__ T s_nodex[D][G::TS];
   __shared__ uint32_t s_cnt[64];
+  // Layer k (the k-th particle of every class) is stored contiguously in the
+  // layered order: the whole workgroup loads it with coalesced runs into one of
+  // two LDS buffers while the waves work on the other.
+  constexpr bool PSTAGE = AOS && sizeof(T) == 4;
+  constexpr int NIN = 2 * D + 1 + D * D;  // x, v, V, σ
+  constexpr int NIP = NIN | 1;
+  constexpr int PER = (64 * NIN + NT - 1) / NT;
+  __shared__ T s_in[PSTAGE ? 2 : 1][PSTAGE ? 64 : 1][PSTAGE ? NIP : 1];
   for (int q = int(threadIdx.x); q < 8 * G::NFOOT; q += NT) (&s_tile[0][0])[q] = T(0);
   for (int q = int(threadIdx.x); q < D * G::TS; q += NT) {
     const int d = q / G::TS, k = q % G::TS;
@@ -934,11 +942,91 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
       }
     }
     P2GIn<T, D> cur, nxt;
-    if (ccount > 0) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff] - poff, nxt);
+    std::memset(static_cast<void*>(&nxt), 0, sizeof(nxt));
+    // layer description, identical in every wave: active lanes, first storage
+    // slot, whether the active lanes' slots are base, base+1, ...
+    auto layer = [&](int k, uint64_t& has, uint32_t& base, bool& contig) {
+      const bool act = k < ccount;
+      has = __ballot(act);
+      const uint32_t pid = act ? a.perm[start + coff + k] - poff : 0u;
+      const int rank = __popcll(has & lanemask_lt64());
+      const int first = has ? __ffsll((unsigned long long)has) - 1 : 0;
+      base = uint32_t(__shfl(int(pid) - rank, first, 64));
+      contig = has != 0 && __ballot(act && pid != base + uint32_t(rank)) == 0;
+    };
+    // flattened runs of one layer: x (nD), v (nD), V (n), σ (nD²)
+    auto decode = [&](int i, int n, const T*& src, int& nc, int& off, int& e) {
+      if (i < n * D) { src = a.x.p; nc = D; off = 0; e = i; }
+      else if (i < 2 * n * D) { src = a.v.p; nc = D; off = D; e = i - n * D; }
+      else if (i < n * (2 * D + 1)) { src = a.V.p; nc = 1; off = 2 * D; e = i - 2 * n * D; }
+      else { src = a.sigma.p; nc = D * D; off = 2 * D + 1; e = i - n * (2 * D + 1); }
+    };
+    T pre[PSTAGE ? PER : 1];
+    uint64_t has_n = 0;
+    uint32_t base_n = 0;
+    bool contig_n = false;
+    if constexpr (PSTAGE) {
+      layer(0, has_n, base_n, contig_n);
+      if (contig_n) {
+        const int n = __popcll(has_n);
+#pragma unroll
+        for (int m = 0; m < PER; ++m) {
+          const int i = int(threadIdx.x) + m * NT;
+          if (i < n * NIN) {
+            const T* src;
+            int nc, off, e;
+            decode(i, n, src, nc, off, e);
+            s_in[0][e / nc][off + e % nc] = src[int64_t(base_n) * nc + e];
+          }
+        }
+      }
+      __syncthreads();
+    } else {
+      if (ccount > 0) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff] - poff, nxt);
+    }
     for (int k = 0; k < maxcnt; ++k) {
       const uint64_t has = __ballot(k < ccount);
-      cur = nxt;
-      if (k + 1 < ccount) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + 1] - poff, nxt);
+      bool contig_k = false, contig_next = false;
+      if constexpr (PSTAGE) {
+        contig_k = contig_n;
+        // issue the next layer's loads before this layer's arithmetic
+        layer(k + 1, has_n, base_n, contig_n);
+        contig_next = contig_n;
+        if (contig_next) {
+          const int n = __popcll(has_n);
+#pragma unroll
+          for (int m = 0; m < PER; ++m) {
+            const int i = int(threadIdx.x) + m * NT;
+            if (i < n * NIN) {
+              const T* src;
+              int nc, off, e;
+              decode(i, n, src, nc, off, e);
+              pre[m] = src[int64_t(base_n) * nc + e];
+            }
+          }
+        }
+        if (k < ccount) {
+          if (contig_k) {
+            const T* r = s_in[PSTAGE ? (k & 1) : 0][PSTAGE ? __popcll(has & lanemask_lt64()) : 0];
+            cur.pid = a.perm[start + coff + k] - poff;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              cur.x[d] = r[d];
+              cur.v[d] = r[D + d];
+            }
+            cur.V = r[2 * D];
+#pragma unroll
+            for (int q = 0; q < D * D; ++q) cur.S[q] = r[2 * D + 1 + q];
+            cur.l = T(0);
+            if constexpr (BASIS == TESS_UGIMP) cur.l = a.l(cur.pid) / a.mesh.h;
+          } else {
+            p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k] - poff, cur);
+          }
+        }
+      } else {
+        cur = nxt;
+        if (k + 1 < ccount) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + 1] - poff, nxt);
+      }
       if (k < ccount) {
         const uint32_t pid = cur.pid;
         if (misplaced && pass == 0) {
@@ -1018,6 +1106,22 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
         }
       }
       if (pass == 0) layer_off += uint32_t(__popcll(has));
+      if constexpr (PSTAGE) {
+        if (contig_next) {
+          const int n = __popcll(has_n);
+#pragma unroll
+          for (int m = 0; m < PER; ++m) {
+            const int i = int(threadIdx.x) + m * NT;
+            if (i < n * NIN) {
+              const T* src;
+              int nc, off, e;
+              decode(i, n, src, nc, off, e);
+              s_in[PSTAGE ? ((k + 1) & 1) : 0][e / nc][off + e % nc] = pre[m];
+            }
+          }
+        }
+        __syncthreads();  // next layer parked; this layer's buffer free again
+      }
     }
     // lanes are distinct classes, so for one offset their target nodes are
     // distinct; offsets are applied one after another (read, add, write, wait
