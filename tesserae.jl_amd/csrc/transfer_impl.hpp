@@ -1192,7 +1192,7 @@ __device__ __forceinline__ void g2p_load(const Args<T, D>& a, uint32_t pid, G2PI
 // j, the sort key of its new position (nk[j]) and whether it equals the
 // current one (stay[j]): the next step's incremental binning starts from them.
 template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false>
-__global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
+__global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? 3 : 1) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                        uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
                                                        uint32_t ntiles) {
   using G = Geo<D>;
@@ -1256,18 +1256,100 @@ __global__ __launch_bounds__(64 * NW) void k_g2p_fast(Args<T, D> a, const uint32
   }
   const T hinv = a.mesh.h_inv, dt = a.dt;
   G2PIn<T, D, AOS> cur, nxt;
-  std::memset(static_cast<void*>(&nxt), 0, sizeof(nxt));  // lanes without particles compute on zeros
-  if (wave < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + wave] - poff, nxt);
-  for (int k = wave; k < maxcnt; k += NW) {
-    cur = nxt;
-    if (k + NW < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + NW] - poff, nxt);
+  std::memset(static_cast<void*>(&cur), 0, sizeof(cur));  // lanes without particles compute on zeros
+  std::memset(static_cast<void*>(&nxt), 0, sizeof(nxt));
+  // layer k of the wave: active lanes, first storage slot, and whether the
+  // active lanes' slots are base, base+1, ... (the layered order)
+  auto layer = [&](int k, uint64_t& has, uint32_t& base, bool& contig) {
     const bool act = k < ccount;
-    // do the wave's active lanes update a contiguous run of storage slots?
+    has = __ballot(act);
+    const uint32_t pid = act ? a.perm[start + coff + k] - poff : 0u;
+    const int rank = __popcll(has & lanemask_lt64());
+    const int first = has ? __ffsll((unsigned long long)has) - 1 : 0;
+    base = uint32_t(__shfl(int(pid) - rank, first, 64));
+    contig = has != 0 && __ballot(act && pid != base + uint32_t(rank)) == 0;
+  };
+  // a staged wave loads a contiguous layer as runs of x (nD), v (nD), F (nD²)
+  constexpr int NIN = 2 * D + D * D;
+  auto decode = [&](int i, int n, const T*& src, int& nc, int& off, int& e) {
+    if (i < n * D) { src = a.x.p; nc = D; off = 0; e = i; }
+    else if (i < 2 * n * D) { src = a.v.p; nc = D; off = D; e = i - n * D; }
+    else { src = a.F.p; nc = D * D; off = 2 * D; e = i - 2 * n * D; }
+  };
+  T raw[STAGE ? NIN : 1];
+  uint64_t has_n = 0;
+  uint32_t base_n = 0;
+  bool contig_n = false;
+  auto issue = [&]() {
+    const int n = __popcll(has_n);
+#pragma unroll
+    for (int m = 0; m < NIN; ++m) {
+      const int i = lane + 64 * m;
+      if (i < n * NIN) {
+        const T* src;
+        int nc, off, e;
+        decode(i, n, src, nc, off, e);
+        raw[STAGE ? m : 0] = src[int64_t(base_n) * nc + e];
+      }
+    }
+  };
+  if constexpr (STAGE) {
+    layer(wave, has_n, base_n, contig_n);
+    if (contig_n) issue();
+  } else {
+    if (wave < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + wave] - poff, nxt);
+  }
+  for (int k = wave; k < maxcnt; k += NW) {
+    const bool act = k < ccount;
     const uint64_t has = __ballot(act);
     const int rank = __popcll(has & lanemask_lt64());
-    const int first = __ffsll((unsigned long long)has) - 1;
-    const uint32_t base = uint32_t(__shfl(int(cur.pid) - rank, first, 64));
-    const bool contiguous = __ballot(act && cur.pid != base + uint32_t(rank)) == 0;
+    uint32_t base = 0;
+    bool contiguous = false;
+    if constexpr (STAGE) {
+      base = base_n;
+      contiguous = contig_n;
+      if (contiguous) {
+        // park the prefetched runs by particle rank, then read the lane's own row
+        const int n = __popcll(has);
+        T(*sw)[NOP] = s_out[STAGE ? wave : 0];
+#pragma unroll
+        for (int m = 0; m < NIN; ++m) {
+          const int i = lane + 64 * m;
+          if (i < n * NIN) {
+            const T* src;
+            int nc, off, e;
+            decode(i, n, src, nc, off, e);
+            sw[e / nc][off + e % nc] = raw[STAGE ? m : 0];
+          }
+        }
+        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        if (act) {
+          const T* r = sw[rank];
+          cur.pid = base + uint32_t(rank);
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            cur.x[c] = r[c];
+            cur.v[c] = r[D + c];
+          }
+#pragma unroll
+          for (int q = 0; q < D * D; ++q) cur.F[q] = r[2 * D + q];
+          cur.l = T(0);
+          if constexpr (BASIS == TESS_UGIMP) cur.l = a.l(cur.pid) / a.mesh.h;
+        }
+      } else if (act) {
+        // a layer out of the layered order: each lane loads its own particle
+        g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k] - poff, cur);
+      }
+      // the next layer of this wave: coalesced runs in flight during the arithmetic
+      layer(k + NW, has_n, base_n, contig_n);
+      if (contig_n) issue();
+    } else {
+      cur = nxt;
+      if (k + NW < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + NW] - poff, nxt);
+      const int first = __ffsll((unsigned long long)has) - 1;
+      base = uint32_t(__shfl(int(cur.pid) - rank, first, 64));
+      contiguous = __ballot(act && cur.pid != base + uint32_t(rank)) == 0;
+    }
     if (!STAGE && !act) continue;
     T val[D][SW], der[D][SW];
 #pragma unroll
