@@ -5,7 +5,10 @@ Workload: 3D elastic block drop, dx=1/256 on [0,1]^3, quadratic B-spline,
 jittered +-0.1dx, gravity, slip walls, Neo-Hookean E=1e6 nu=0.3 rho0=1e3,
 dt=1e-4, Float32.  One step = update!(weights) (deferred) + @P2G (with grid
 update) + @G2P, exactly the README loop body, with the reference's
-reorder_particles!(threshold=0.85) maintenance check every step.
+reorder_particles!(threshold=0.85) maintenance check every step.  The warm-up
+steps and the timed steps are each one call of the step loop (tess_step with
+nsteps = W, then K): inside a call every binning after the first starts from
+the keys the previous G2P wrote.
 
 Prints one JSON line (metric particle-steps/s) with the roofline fraction of
 the dominant kernel and the CPU baseline (the oracle's threaded restatement of
@@ -249,12 +252,12 @@ def main():
     torch.cuda.synchronize()
 
     # the reference loop's maintenance: reorder_particles!(...; threshold=0.85) every step
-    def one_step():
-        th.step(G, P, sc.basis, sc.params, nsteps=1, reorder_threshold=args.reorder, ctx=ctx)
+    def steps(k):
+        th.step(G, P, sc.basis, sc.params, nsteps=k, reorder_threshold=args.reorder, ctx=ctx)
 
     log(f"bench: n={n} warmup={args.warmup}")
-    for _ in range(args.warmup):
-        one_step()
+    if args.warmup > 0:
+        steps(args.warmup)
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     log(f"bench: timing {args.steps} steps")
@@ -262,8 +265,7 @@ def main():
     torch.cuda.synchronize()
     start.record()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        one_step()
+    steps(args.steps)
     stop.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -306,6 +308,10 @@ def main():
         "step_hbm": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (ms * 1e-3) / 1e9,
                      "frac": step_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), "active_nodes": stats["active_nodes"]},
         "kernels": kernels,
+        # binnings of the context so far that kept the previous tile layout (no particle
+        # changed its key), and the movers merged at the last incremental re-binning
+        "binning": {"kept_layout": stats["kept_binnings"], "movers_last": stats["movers"],
+                    "reorders": stats["reorders"]},
         "wall_s": wall,
     }
     if not args.no_batch:

@@ -326,7 +326,8 @@ tess_status tess_profile_read(tess_ctx* ctx, int32_t index, char* name, int32_t 
  * written by the previous P2G, out[4] reorders done by tess_step, out[5]
  * particles found out of the layered order by the previous P2G (-1 unknown),
  * out[6] particles whose tile or window class changed at the last incremental
- * re-binning (-1 after a full sort). */
+ * re-binning (-1 after a full sort), out[7] binnings that kept the previous
+ * tile layout because no particle changed its key. */
 tess_status tess_stats(tess_ctx* ctx, int64_t* out, int32_t count);
 
 #ifdef __cplusplus

@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <algorithm>
 #include <new>
 #include <string>
@@ -209,6 +210,10 @@ tess_status tess_ctx_create(int32_t device, void* stream, tess_ctx** out) {
     tess_ctx* c = new tess_ctx();
     c->device = device;
     c->stream = static_cast<hipStream_t>(stream);
+    if (const char* st = std::getenv("TESS_LAYER_STAGING")) {
+      const int v = std::atoi(st);
+      if (v >= 0 && v <= 3) c->staging = v;
+    }
     void* pinned = nullptr;
     hipError_t e = hipHostMalloc(&pinned, 64 * sizeof(int64_t), hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -562,7 +567,7 @@ tess_status tess_stats(tess_ctx* ctx, int64_t* out, int32_t count) {
     check_ctx(ctx);
     if (!out || count < 0) raise(TESS_ERR_ARGUMENT, "out is NULL");
     const int64_t vals[] = {ctx->bin.n_inside, ctx->bin.n_active, ctx->bin.n_nodeblocks, ctx->last_active_nodes,
-                            ctx->reorders, ctx->last_misplaced, ctx->last_movers};
+                            ctx->reorders, ctx->last_misplaced, ctx->last_movers, ctx->kept_binnings};
     for (int32_t i = 0; i < count && i < int32_t(sizeof(vals) / sizeof(vals[0])); ++i) out[i] = vals[i];
   });
 }

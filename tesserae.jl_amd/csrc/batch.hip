@@ -154,7 +154,7 @@ template <typename T, int D, int BASIS, int NW>
 void launch_g2p(tess_ctx* ctx, const Args<T, D>& a, int64_t n_active, const uint32_t* sk) {
   hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW, true>), dim3(xcd_grid(n_active)), dim3(64 * NW), 0,
                      ctx->stream, a, sk, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
-                     uint32_t(ctx->bin.ntiles));
+                     uint32_t(ctx->bin.ntiles), static_cast<uint32_t*>(nullptr));
   TESS_HIP(hipGetLastError());
 }
 
@@ -181,7 +181,7 @@ void batch_step_impl(tess_ctx* ctx, tess_batch& B, int nsteps) {
     const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
     uint32_t* counters = static_cast<uint32_t*>(ctx->counters.ptr);
     if (b.n_active > 0) {
-      a.partials = ctx->partials.as<T>(size_t(b.n_active) * 8 * Gm::NFOOT);
+      a.partials = ctx->partials.as<T>(size_t(b.n_active) * Part<D, SW>::SIZE);
       const int tok = prof_begin(ctx, "k_p2g_tile", double(b.n_inside) * sizeof(T) * (2 * D + 1 + D * D));
       if (D == 3 || (B.variant & 1)) launch_p2g<T, D, BASIS, SW>(ctx, a, b.n_active, sk);
       else launch_p2g<T, D, BASIS, 1>(ctx, a, b.n_active, sk);
@@ -190,7 +190,7 @@ void batch_step_impl(tess_ctx* ctx, tess_batch& B, int nsteps) {
     if (b.n_nodeblocks > 0) {
       uint32_t* per_nb = ctx->per_nodeblock.as<uint32_t>(size_t(b.n_nodeblocks));
       const int tok = prof_begin(ctx, "k_gather_grid", 0.0);
-      hipLaunchKernelGGL((k_gather_grid<T, D, true, true>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream,
+      hipLaunchKernelGGL((k_gather_grid<T, D, SW, true, true>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream,
                          a, per_nb);
       TESS_HIP(hipGetLastError());
       prof_end(ctx, tok);

@@ -102,6 +102,7 @@ struct tess_ctx {
   // incremental re-binning scratch
   tess::Buffer skeys2, perm2, inc_nk, inc_spos, inc_mkey, inc_mj, inc_mks, inc_mr, inc_mjs;
   int64_t last_movers = -1;
+  int64_t kept_binnings = 0;  // binnings that kept the previous layout (no particle changed its key)
   tess::Buffer counters;   // small device scalars
   tess::Buffer host_pinned_dummy;
   int64_t* counters_host = nullptr;  // pinned readback slots
@@ -125,6 +126,10 @@ struct tess_ctx {
   // adjoint: per-tile Float64 partials, node adjoints, rollout checkpoints and replay
   tess::Buffer adj_partials, adj_nodes, ro_checkpoints, ro_segment, ro_gradv;
   tess::Rollout rollout;
+  // layer staging through LDS in the single-scene transfers (FP32, Julia
+  // layout): bit 0 P2G, bit 1 G2P.  Set from TESS_LAYER_STAGING (0-3) when the
+  // context is created; both settings give bitwise identical results.
+  int staging = 0;
 };
 
 // A batch of scenes (batch.hip): the scenes as given, where each scene's tiles

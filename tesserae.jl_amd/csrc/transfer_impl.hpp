@@ -40,16 +40,43 @@ struct Geo;
 template <>
 struct Geo<3> {
   static constexpr int LT = 2, BW = 4, TS = 7, NFOOT = 343, NCLASS = 64, GNODES = 64;
+  // padded LDS footprint, one array per component: node (x, y, z) at
+  // x + PY*y + PZ*z.  For the 64 class lanes (4x4x4) at any stencil offset
+  // the 64 addresses fall in distinct banks of a 64-bank LDS (two per bank
+  // with 32 banks), which the unpadded pitches 7/49 do not give.
+  static constexpr int PY = 12, PZ = 80, PSZ = TS * PZ;
 };
 template <>
 struct Geo<2> {
   static constexpr int LT = 3, BW = 8, TS = 11, NFOOT = 121, NCLASS = 64, GNODES = 64;
+  static constexpr int PY = 24, PZ = 0, PSZ = TS * PY;  // 8x8 class lanes, as in 3D
+};
+
+// padded LDS index of footprint node (x, y[, z])
+template <int D>
+__device__ __forceinline__ constexpr int pad_node(int x, int y, int z) {
+  return x + Geo<D>::PY * y + Geo<D>::PZ * z;
+}
+
+template <int N, int E>
+struct Pow;
+
+// A tile's P2G partial in HBM: the nodes its classes reach, E = BW + SW - 1
+// per axis from the footprint origin, x fastest, NV values per node
+// (m, mv[D], f[D]).
+template <int D, int SW>
+struct Part {
+  static constexpr int E = Geo<D>::BW + SW - 1;
+  static constexpr int NV = 1 + 2 * D;
+  static constexpr int NODES = Pow<E, D>::v;
+  static constexpr int SIZE = NODES * NV;
 };
 
 template <int N, int E>
 struct Pow {
   static constexpr int v = N * Pow<N, E - 1>::v;
 };
+
 template <int N>
 struct Pow<N, 0> {
   static constexpr int v = 1;
@@ -59,7 +86,7 @@ struct Pow<N, 0> {
 // misplaced-particle count of the last P2G (zeroed per P2G)
 enum {
   CNT_INSIDE = 0, CNT_ACTIVE = 1, CNT_NB = 2, CNT_ERR = 3, CNT_BIN_SLOTS = 4,
-  CNT_MISPLACED = 8, CNT_NODES = 9, CNT_STAY = 10, CNT_P2GERR = 11, CNT_SLOTS = 16
+  CNT_MISPLACED = 8, CNT_NODES = 9, CNT_STAY = 10, CNT_P2GERR = 11, CNT_MOVERS = 12, CNT_SLOTS = 16
 };
 enum { ERR_GIMP_LENGTH = 1, ERR_WINDOW = 2, ERR_INACTIVE = 4 };
 
@@ -599,12 +626,51 @@ __device__ __forceinline__ void axis_jets(T xd, int lo_local, const T* s_nodex, 
 
 __device__ __forceinline__ uint64_t lanemask_lt64() { return (uint64_t(1) << (threadIdx.x & 63)) - 1; }
 
+// One field of a contiguous layer: n ≤ 64 particles of NC scalars each, stored
+// at src[base*NC ...] as one run of n*NC scalars.  NT threads load the run with
+// consecutive addresses into registers, then park it in LDS rows (particle
+// rank r, columns OFF .. OFF+NC-1).  NC is a compile-time constant, so the
+// element -> (row, column) split is a multiply-shift.
+template <typename T, int NC, int NT>
+struct LayerRun {
+  static constexpr int R = (64 * NC + NT - 1) / NT;
+  T r[R];
+  __device__ __forceinline__ void load(const T* __restrict__ src, uint32_t base, int n, int tid) {
+    const T* p = src + int64_t(base) * NC;
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int i = tid + m * NT;
+      if (i < n * NC) r[m] = p[i];
+    }
+  }
+  template <int OFF, int NP>
+  __device__ __forceinline__ void park(T (*rows)[NP], int n, int tid) const {
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int i = tid + m * NT;
+      if (i < n * NC) rows[i / NC][OFF + i % NC] = r[m];
+    }
+  }
+};
+
+// The reverse: LDS rows (columns OFF ..) of n particles stored as one run.
+template <typename T, int NC, int OFF, int NP>
+__device__ __forceinline__ void store_run(T* __restrict__ dst, uint32_t base, int n, const T (*rows)[NP], int lane) {
+  T* p = dst + int64_t(base) * NC;
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    const int i = lane + 64 * m;
+    if (i < n * NC) p[i] = rows[i / NC][OFF + i % NC];
+  }
+}
+
 // Per-class counts of one tile from its slice of the sorted keys (class = low 6
 // bits), by wave-level matching; lane c gets count and offset of class c.
 template <int NW = 1>
 __device__ __forceinline__ void tile_class_counts(const uint32_t* __restrict__ skeys, uint32_t start, int cnt,
                                                   uint32_t* s_cnt, int& ccount, int& coff, int& maxcnt) {
-  const int lane = threadIdx.x & 63, wave = int(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform: scalar branches
   if (wave == 0) s_cnt[lane] = 0;
   __syncthreads();
   for (int base = wave * 64; base < cnt; base += 64 * NW) {
@@ -783,8 +849,13 @@ __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a, const uint32_t* _
     }
   }
   __syncthreads();
-  T* out = a.partials + int64_t(j) * 8 * G::NFOOT;
-  for (int q = lane; q < 8 * G::NFOOT; q += 64) out[q] = (&s_tile[0][0])[q];
+  using PT = Part<D, SW>;
+  T* out = a.partials + int64_t(j) * PT::SIZE;
+  for (int e = lane; e < PT::SIZE; e += 64) {
+    const int n = e / NV, v = e % NV;
+    const int x = n % PT::E, y = (n / PT::E) % PT::E, z = D == 3 ? n / (PT::E * PT::E) : 0;
+    out[e] = s_tile[x + G::TS * (y + G::TS * z)][v];
+  }
 }
 
 template <typename T, int D>
@@ -796,6 +867,38 @@ __device__ __forceinline__ double det_d(const T* F) {
                  f33 = F[8];
     return f11 * (f22 * f33 - f23 * f32) - f12 * (f21 * f33 - f23 * f31) + f13 * (f21 * f32 - f22 * f31);
   }
+}
+
+// Neo-Hookean Cauchy stress σ = (μ(FFᵀ-I) + λ lnJ I)/J and V = V0 J of the
+// updated F (README.md:84-86), stiffness scaled by exp(logE) when has_logE,
+// evaluated in Float64 for both storage types (cofactor determinant, log(J)).
+// Float64 vector instructions issue at the rate of unpacked Float32 ones here,
+// and this form is shorter than a Float32 one that keeps the relative accuracy
+// of small strains (log1p of J - 1 from the minors of F - I).  σ is written as
+// sig[r + D*b].
+template <typename T, int D>
+__device__ __forceinline__ void neo_hookean(const T* Fn, double lam0, double mu0, double V0, bool has_logE, T logE,
+                                            T* sig, T& Vout) {
+  const double J = det_d<T, D>(Fn);
+  double lam = lam0, mu = mu0;
+  if (has_logE) {
+    const double sc = exp(double(logE));
+    lam *= sc;
+    mu *= sc;
+  }
+  const double lnJ = log(J);
+  const double invJ = 1.0 / J;
+#pragma unroll
+  for (int b = 0; b < D; ++b)
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      double bb = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) bb += double(Fn[r + D * k]) * double(Fn[b + D * k]);
+      const double sg = mu * (bb - (r == b ? 1.0 : 0.0)) + (r == b ? lam * lnJ : 0.0);
+      sig[r + D * b] = T(sg * invJ);
+    }
+  Vout = T(V0 * J);
 }
 
 // ---- deferred-weight fast paths ------------------------------------------------
@@ -876,7 +979,16 @@ __device__ __forceinline__ void p2g_load(const Args<T, D>& a, uint32_t pid, P2GI
 // the registers of one wave hold 1/NPASS of the stencil, which keeps several
 // waves per SIMD resident).  The waves add their stencils into the LDS tile
 // one after another, so the per-node order of the additions is fixed.
-template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false>
+// layers of particles the unstaged P2G loads ahead of the one it accumulates
+#ifndef TESS_P2G_AHEAD
+#define TESS_P2G_AHEAD 1
+#endif
+// wait for each offset's LDS writes before the next offset's reads (not
+// needed for correctness: the LDS executes one wave's accesses in order)
+#ifndef TESS_P2G_WRITE_WAIT
+#define TESS_P2G_WRITE_WAIT 0
+#endif
+template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false, bool STG = true>
 __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                           uint32_t* __restrict__ misplaced) {
   using G = Geo<D>;
@@ -885,7 +997,9 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
   constexpr int KN = SW / NPASS;                       // last-axis offsets per pass
   constexpr int NSL = Pow<SW, D - 1>::v * KN;          // offsets per pass
   constexpr int NT = 64 * NPASS;
-  const int lane = threadIdx.x & 63, wave = int(threadIdx.x >> 6);
+  constexpr int P2G_AHEAD = TESS_P2G_AHEAD;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform: scalar branches
   const int64_t j = xcd_remap(blockIdx.x, a.n_active);
   if (j >= a.n_active) return;
   const uint32_t tile = a.active[j];
@@ -898,19 +1012,22 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
 #pragma unroll
   for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
 
-  // node-major tile, 8 slots per node: m, mv[D], f[D], padding
-  __shared__ __align__(16) T s_tile[G::NFOOT][8];
+  // the tile's node sums m, mv[D], f[D], one array per component over the
+  // padded footprint (bank-conflict-free for the class lanes)
+  __shared__ T s_tile[NV][G::PSZ];
   __shared__ T s_nodex[D][G::TS];
   __shared__ uint32_t s_cnt[64];
   // Layer k (the k-th particle of every class) is stored contiguously in the
   // layered order: the whole workgroup loads it with coalesced runs into one of
   // two LDS buffers while the waves work on the other.
-  constexpr bool PSTAGE = AOS && sizeof(T) == 4;
+  constexpr bool PSTAGE = STG && AOS && sizeof(T) == 4;
   constexpr int NIN = 2 * D + 1 + D * D;  // x, v, V, σ
   constexpr int NIP = NIN | 1;
-  constexpr int PER = (64 * NIN + NT - 1) / NT;
   __shared__ T s_in[PSTAGE ? 2 : 1][PSTAGE ? 64 : 1][PSTAGE ? NIP : 1];
-  for (int q = int(threadIdx.x); q < 8 * G::NFOOT; q += NT) (&s_tile[0][0])[q] = T(0);
+  // per-layer axis jets shared by the waves (NPASS > 1): [layer parity][axis][val SW, der SW][lane]
+  constexpr bool SHJ = NPASS > 1;
+  __shared__ T s_jet[SHJ ? 2 : 1][SHJ ? D : 1][SHJ ? 2 * SW : 1][SHJ ? 64 : 1];
+  for (int q = int(threadIdx.x); q < NV * G::PSZ; q += NT) (&s_tile[0][0])[q] = T(0);
   for (int q = int(threadIdx.x); q < D * G::TS; q += NT) {
     const int d = q / G::TS, k = q % G::TS;
     s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
@@ -941,8 +1058,12 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
         af[c][o] = T(0);
       }
     }
-    P2GIn<T, D> cur, nxt;
+    P2GIn<T, D> cur, nxt, nxt2;
+    // lanes past their class's count compute on these zeros with zero mass and
+    // stress factors (finite weights, exact zero contributions)
+    std::memset(static_cast<void*>(&cur), 0, sizeof(cur));
     std::memset(static_cast<void*>(&nxt), 0, sizeof(nxt));
+    std::memset(static_cast<void*>(&nxt2), 0, sizeof(nxt2));
     // layer description, identical in every wave: active lanes, first storage
     // slot, whether the active lanes' slots are base, base+1, ...
     auto layer = [&](int k, uint64_t& has, uint32_t& base, bool& contig) {
@@ -954,35 +1075,41 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
       base = uint32_t(__shfl(int(pid) - rank, first, 64));
       contig = has != 0 && __ballot(act && pid != base + uint32_t(rank)) == 0;
     };
-    // flattened runs of one layer: x (nD), v (nD), V (n), σ (nD²)
-    auto decode = [&](int i, int n, const T*& src, int& nc, int& off, int& e) {
-      if (i < n * D) { src = a.x.p; nc = D; off = 0; e = i; }
-      else if (i < 2 * n * D) { src = a.v.p; nc = D; off = D; e = i - n * D; }
-      else if (i < n * (2 * D + 1)) { src = a.V.p; nc = 1; off = 2 * D; e = i - 2 * n * D; }
-      else { src = a.sigma.p; nc = D * D; off = 2 * D + 1; e = i - n * (2 * D + 1); }
+    // the runs of one layer: x (nD), v (nD), V (n), σ (nD²)
+    constexpr int SR = PSTAGE ? 1 : 0;  // register arrays collapse when not staged
+    LayerRun<T, SR ? D : 1, NT> rx, rv;
+    LayerRun<T, 1, NT> rV;
+    LayerRun<T, SR ? D * D : 1, NT> rs;
+    const int tid = int(threadIdx.x);
+    auto issue = [&](uint32_t base, int n) {
+      rx.load(a.x.p, base, n, tid);
+      rv.load(a.v.p, base, n, tid);
+      rV.load(a.V.p, base, n, tid);
+      rs.load(a.sigma.p, base, n, tid);
     };
-    T pre[PSTAGE ? PER : 1];
+    auto park = [&](T (*rows)[PSTAGE ? NIP : 1], int n) {
+      if constexpr (PSTAGE) {
+        rx.template park<0, NIP>(rows, n, tid);
+        rv.template park<D, NIP>(rows, n, tid);
+        rV.template park<2 * D, NIP>(rows, n, tid);
+        rs.template park<2 * D + 1, NIP>(rows, n, tid);
+      }
+    };
     uint64_t has_n = 0;
     uint32_t base_n = 0;
     bool contig_n = false;
     if constexpr (PSTAGE) {
       layer(0, has_n, base_n, contig_n);
       if (contig_n) {
-        const int n = __popcll(has_n);
-#pragma unroll
-        for (int m = 0; m < PER; ++m) {
-          const int i = int(threadIdx.x) + m * NT;
-          if (i < n * NIN) {
-            const T* src;
-            int nc, off, e;
-            decode(i, n, src, nc, off, e);
-            s_in[0][e / nc][off + e % nc] = src[int64_t(base_n) * nc + e];
-          }
-        }
+        issue(base_n, __popcll(has_n));
+        park(s_in[0], __popcll(has_n));
       }
       __syncthreads();
     } else {
       if (ccount > 0) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff] - poff, nxt);
+      if constexpr (P2G_AHEAD == 2) {
+        if (ccount > 1) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + 1] - poff, nxt2);
+      }
     }
     for (int k = 0; k < maxcnt; ++k) {
       const uint64_t has = __ballot(k < ccount);
@@ -992,19 +1119,7 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
         // issue the next layer's loads before this layer's arithmetic
         layer(k + 1, has_n, base_n, contig_n);
         contig_next = contig_n;
-        if (contig_next) {
-          const int n = __popcll(has_n);
-#pragma unroll
-          for (int m = 0; m < PER; ++m) {
-            const int i = int(threadIdx.x) + m * NT;
-            if (i < n * NIN) {
-              const T* src;
-              int nc, off, e;
-              decode(i, n, src, nc, off, e);
-              pre[m] = src[int64_t(base_n) * nc + e];
-            }
-          }
-        }
+        if (contig_next) issue(base_n, __popcll(has_n));
         if (k < ccount) {
           if (contig_k) {
             const T* r = s_in[PSTAGE ? (k & 1) : 0][PSTAGE ? __popcll(has & lanemask_lt64()) : 0];
@@ -1025,41 +1140,92 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
         }
       } else {
         cur = nxt;
-        if (k + 1 < ccount) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + 1] - poff, nxt);
+        if constexpr (P2G_AHEAD == 2) {
+          nxt = nxt2;
+          if (k + 2 < ccount) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + 2] - poff, nxt2);
+        } else {
+          if (k + 1 < ccount) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + 1] - poff, nxt);
+        }
       }
-      if (k < ccount) {
+      // the accumulation runs on every lane: a branch around it made the
+      // compiler copy all accumulators at each iteration
+      {
+        const bool act = k < ccount;
         const uint32_t pid = cur.pid;
-        if (misplaced && pass == 0) {
+        if (misplaced && pass == 0 && act) {
           const uint32_t want = start + layer_off + uint32_t(__popcll(has & lanemask_lt64()));
           bad += pid != want ? 1u : 0u;
         }
         T x[D], mpv[D], A[D * D];
+        const T mpe = act ? mp : T(0);
 #pragma unroll
         for (int d = 0; d < D; ++d) x[d] = cur.x[d];
 #pragma unroll
-        for (int d = 0; d < D; ++d) mpv[d] = mp * cur.v[d];  // hoisted mₚ*v[p]
-        const T negV = -cur.V;
+        for (int d = 0; d < D; ++d) mpv[d] = mpe * cur.v[d];  // hoisted mₚ*v[p]
+        const T negV = act ? -cur.V : T(0);
 #pragma unroll
         for (int q = 0; q < D * D; ++q) A[q] = negV * cur.S[q];  // hoisted (-V[p])*σ[p]
         const T l = cur.l;
-        T val[D][SW], der[D][SW];
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
+        // jets of axes 0..D-2 at all SW offsets, of the last axis at this
+        // wave's KN offsets (vl, dl)
+        T val[D][SW], der[D][SW], vl[KN], dl[KN];
+        auto axis_jets = [&](auto dc, T* jv, T* jd) {
+          constexpr int d = decltype(dc)::value;
           const T xi = (x[d] - a.mesh.xmin[d]) * hinv;
           int lo, hi;
           axis_window<T, BASIS>(xi, l, a.mesh.dims[d], lo, hi);
-          axis_jets_mixed<T, BASIS>(x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], l, hinv, val[d], der[d]);
+          axis_jets_mixed<T, BASIS>(x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], l, hinv, jv, jd);
+        };
+        if constexpr (NPASS == 1) {
+          axis_jets(std::integral_constant<int, 0>{}, val[0], der[0]);
+          axis_jets(std::integral_constant<int, 1>{}, val[1], der[1]);
+          if constexpr (D == 3) axis_jets(std::integral_constant<int, D - 1>{}, val[D - 1], der[D - 1]);
+#pragma unroll
+          for (int q = 0; q < KN; ++q) {
+            vl[q] = val[D - 1][q];
+            dl[q] = der[D - 1][q];
+          }
+        } else {
+          // the waves hold the same particles: wave d % NPASS evaluates axis d
+          // and shares it through LDS (double-buffered by layer, one barrier)
+          const int buf = k & 1;
+          auto share = [&](auto dc) {
+            T jv[SW], jd[SW];
+            axis_jets(dc, jv, jd);
+#pragma unroll
+            for (int q = 0; q < SW; ++q) {
+              s_jet[buf][decltype(dc)::value][q][lane] = jv[q];
+              s_jet[buf][decltype(dc)::value][SW + q][lane] = jd[q];
+            }
+          };
+          if (wave == 0 % NPASS) share(std::integral_constant<int, 0>{});
+          if (wave == 1 % NPASS) share(std::integral_constant<int, 1>{});
+          if constexpr (D == 3) {
+            if (wave == 2 % NPASS) share(std::integral_constant<int, D - 1>{});
+          }
+          __syncthreads();
+#pragma unroll
+          for (int d = 0; d < D - 1; ++d)
+#pragma unroll
+            for (int q = 0; q < SW; ++q) {
+              val[d][q] = s_jet[buf][d][q][lane];
+              der[d][q] = s_jet[buf][d][SW + q][lane];
+            }
+#pragma unroll
+          for (int q = 0; q < KN; ++q) {
+            vl[q] = s_jet[buf][D - 1][pass * KN + q][lane];
+            dl[q] = s_jet[buf][D - 1][SW + pass * KN + q][lane];
+          }
         }
         if constexpr (D == 3) {
 #pragma unroll
           for (int kz = 0; kz < KN; ++kz) {
-            const int kg = pass * KN + kz;
 #pragma unroll
             for (int jy = 0; jy < SW; ++jy) {
-              const T YZ = val[1][jy] * val[2][kg];
-              const T YdZ = der[1][jy] * val[2][kg];
-              const T YZd = val[1][jy] * der[2][kg];
-              const T cm = YZ * mp;
+              const T YZ = val[1][jy] * vl[kz];
+              const T YdZ = der[1][jy] * vl[kz];
+              const T YZd = val[1][jy] * dl[kz];
+              const T cm = YZ * mpe;
               T cmv[3], U[3], Wv[3];
 #pragma unroll
               for (int c = 0; c < 3; ++c) {
@@ -1074,7 +1240,7 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                   amv[c][o] += val[0][ix] * cmv[c];
-                  af[c][o] += der[0][ix] * U[c] + val[0][ix] * Wv[c];
+                  af[c][o] = fma(val[0][ix], Wv[c], fma(der[0][ix], U[c], af[c][o]));
                 }
               }
             }
@@ -1082,9 +1248,8 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
         } else {
 #pragma unroll
           for (int jy = 0; jy < KN; ++jy) {
-            const int jg = pass * KN + jy;
-            const T Y = val[1][jg], Yd = der[1][jg];
-            const T cm = Y * mp;
+            const T Y = vl[jy], Yd = dl[jy];
+            const T cm = Y * mpe;
             T cmv[2], U[2], Wv[2];
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
@@ -1099,7 +1264,7 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
 #pragma unroll
               for (int c = 0; c < 2; ++c) {
                 amv[c][o] += val[0][ix] * cmv[c];
-                af[c][o] += der[0][ix] * U[c] + val[0][ix] * Wv[c];
+                af[c][o] = fma(val[0][ix], Wv[c], fma(der[0][ix], U[c], af[c][o]));
               }
             }
           }
@@ -1107,19 +1272,7 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
       }
       if (pass == 0) layer_off += uint32_t(__popcll(has));
       if constexpr (PSTAGE) {
-        if (contig_next) {
-          const int n = __popcll(has_n);
-#pragma unroll
-          for (int m = 0; m < PER; ++m) {
-            const int i = int(threadIdx.x) + m * NT;
-            if (i < n * NIN) {
-              const T* src;
-              int nc, off, e;
-              decode(i, n, src, nc, off, e);
-              s_in[PSTAGE ? ((k + 1) & 1) : 0][e / nc][off + e % nc] = pre[m];
-            }
-          }
-        }
+        if (contig_next) park(s_in[(k + 1) & 1], __popcll(has_n));
         __syncthreads();  // next layer parked; this layer's buffer free again
       }
     }
@@ -1131,17 +1284,17 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
       if (w == wave && ccount > 0) {
 #pragma unroll
       for (int o = 0; o < NSL; ++o) {
-        int r = o, q = 0, mul = 1;
+        int r = o, nI[3] = {0, 0, 0};
 #pragma unroll
         for (int d = 0; d < D; ++d) {
           const int off = (d == D - 1) ? pass * KN + r : r % SW;
-          q += (ccoord[d] + off) * mul;
+          nI[d] = ccoord[d] + off;
           r /= SW;
-          mul *= G::TS;
         }
-        T t[8];
+        const int q = pad_node<D>(nI[0], nI[1], nI[2]);
+        T t[NV];
 #pragma unroll
-        for (int v = 0; v < 8; ++v) t[v] = s_tile[q][v];
+        for (int v = 0; v < NV; ++v) t[v] = s_tile[v][q];
         t[0] += am[o];
 #pragma unroll
         for (int c = 0; c < D; ++c) {
@@ -1149,8 +1302,12 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
           t[1 + D + c] += af[c][o];
         }
 #pragma unroll
-        for (int v = 0; v < 8; ++v) s_tile[q][v] = t[v];
+        for (int v = 0; v < NV; ++v) s_tile[v][q] = t[v];
+#if TESS_P2G_WRITE_WAIT
         __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#else
+        __asm__ volatile("" ::: "memory");  // keep the offsets in order (LDS executes a wave's accesses in order)
+#endif
       }
       }
       __syncthreads();
@@ -1161,8 +1318,13 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
     for (int off = 32; off >= 1; off >>= 1) bad += __shfl_xor(bad, off, 64);
     if (lane == 0) misplaced[j] = bad;  // per tile, summed later
   }
-  T* out = a.partials + int64_t(j) * 8 * G::NFOOT;
-  for (int q = int(threadIdx.x); q < 8 * G::NFOOT; q += NT) out[q] = (&s_tile[0][0])[q];
+  using PT = Part<D, SW>;
+  T* out = a.partials + int64_t(j) * PT::SIZE;
+  for (int e = int(threadIdx.x); e < PT::SIZE; e += NT) {
+    const int n = e / NV, v = e % NV;
+    const int q = pad_node<D>(n % PT::E, (n / PT::E) % PT::E, D == 3 ? n / (PT::E * PT::E) : 0);
+    out[e] = s_tile[v][q];
+  }
 }
 
 // G2P with the footprint's v and v - vⁿ staged per node in LDS; the FLIP
@@ -1191,13 +1353,18 @@ __device__ __forceinline__ void g2p_load(const Args<T, D>& a, uint32_t pid, G2PI
 // With nk != nullptr the kernel also writes, at each particle's sorted position
 // j, the sort key of its new position (nk[j]) and whether it equals the
 // current one (stay[j]): the next step's incremental binning starts from them.
-template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false>
-__global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? 3 : 1) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
+// waves per SIMD the Float32 G2P is compiled for (VGPR budget 512 / n)
+#ifndef TESS_G2P_WAVES_PER_SIMD
+#define TESS_G2P_WAVES_PER_SIMD 3
+#endif
+template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false, bool STG = true>
+__global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD : 1) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                        uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
-                                                       uint32_t ntiles) {
+                                                       uint32_t ntiles, uint32_t* __restrict__ movers) {
   using G = Geo<D>;
   constexpr int SW = support_width(BASIS);
-  const int lane = threadIdx.x & 63, wave = int(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform: scalar branches
   const int64_t j = xcd_remap(blockIdx.x, a.n_active);
   if (j >= a.n_active) return;
   const uint32_t tile = a.active[j];
@@ -1209,14 +1376,14 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? 3 : 1) void k_g2p_fast(Ar
   unlin<D>(ltile, a.ntiles, tI);
 #pragma unroll
   for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
-  // per node: v[D] and v-vⁿ[D], padded to a power-of-two stride for vector LDS reads
-  constexpr int NS = (D == 3) ? 8 : 4;
-  __shared__ __align__(16) T s_n[G::NFOOT][NS];
+  // per component (v[D], then v-vⁿ[D]) the footprint in the padded
+  // bank-conflict-free node order of Geo
+  __shared__ T s_n[2 * D][G::PSZ];
   __shared__ T s_nodex[D][G::TS];
   __shared__ uint32_t s_cnt[64];
   // per wave: the outputs of its 64 lanes, written back as contiguous runs when
   // the wave's particles are stored contiguously (the layered order)
-  constexpr bool STAGE = AOS && sizeof(T) == 4;
+  constexpr bool STAGE = STG && AOS && sizeof(T) == 4;
   constexpr int NOUT = 2 * D + 3 * D * D + 1;  // v, ∇v, x, F, σ, V
   constexpr int NOP = NOUT | 1;                // odd row stride: lanes hit distinct banks
   __shared__ T s_out[STAGE ? NW : 1][STAGE ? 64 : 1][STAGE ? NOP : 1];
@@ -1231,12 +1398,13 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? 3 : 1) void k_g2p_fast(Ar
     }
     const int64_t i = in ? grid_slot(a, I) : -1;
     in = i >= 0;
+    const int pq = pad_node<D>(q % G::TS, (q / G::TS) % G::TS, D == 3 ? q / (G::TS * G::TS) : 0);
 #pragma unroll
     for (int c = 0; c < D; ++c) {
       const T vv = in ? a.gv(i, c) : T(0);
       const T vn = in ? a.gvn(i, c) : T(0);
-      s_n[q][c] = vv;
-      s_n[q][NS / 2 + c] = vv - vn;
+      s_n[c][pq] = vv;
+      s_n[D + c][pq] = vv - vn;
     }
   }
   for (int q = int(threadIdx.x); q < D * G::TS; q += 64 * NW) {
@@ -1270,28 +1438,17 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? 3 : 1) void k_g2p_fast(Ar
     contig = has != 0 && __ballot(act && pid != base + uint32_t(rank)) == 0;
   };
   // a staged wave loads a contiguous layer as runs of x (nD), v (nD), F (nD²)
-  constexpr int NIN = 2 * D + D * D;
-  auto decode = [&](int i, int n, const T*& src, int& nc, int& off, int& e) {
-    if (i < n * D) { src = a.x.p; nc = D; off = 0; e = i; }
-    else if (i < 2 * n * D) { src = a.v.p; nc = D; off = D; e = i - n * D; }
-    else { src = a.F.p; nc = D * D; off = 2 * D; e = i - 2 * n * D; }
-  };
-  T raw[STAGE ? NIN : 1];
+  constexpr int SR = STAGE ? 1 : 0;  // register arrays collapse when not staged
+  LayerRun<T, SR ? D : 1, 64> rx, rv;
+  LayerRun<T, SR ? D * D : 1, 64> rF;
   uint64_t has_n = 0;
   uint32_t base_n = 0;
   bool contig_n = false;
   auto issue = [&]() {
     const int n = __popcll(has_n);
-#pragma unroll
-    for (int m = 0; m < NIN; ++m) {
-      const int i = lane + 64 * m;
-      if (i < n * NIN) {
-        const T* src;
-        int nc, off, e;
-        decode(i, n, src, nc, off, e);
-        raw[STAGE ? m : 0] = src[int64_t(base_n) * nc + e];
-      }
-    }
+    rx.load(a.x.p, base_n, n, lane);
+    rv.load(a.v.p, base_n, n, lane);
+    rF.load(a.F.p, base_n, n, lane);
   };
   if constexpr (STAGE) {
     layer(wave, has_n, base_n, contig_n);
@@ -1312,15 +1469,10 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? 3 : 1) void k_g2p_fast(Ar
         // park the prefetched runs by particle rank, then read the lane's own row
         const int n = __popcll(has);
         T(*sw)[NOP] = s_out[STAGE ? wave : 0];
-#pragma unroll
-        for (int m = 0; m < NIN; ++m) {
-          const int i = lane + 64 * m;
-          if (i < n * NIN) {
-            const T* src;
-            int nc, off, e;
-            decode(i, n, src, nc, off, e);
-            sw[e / nc][off + e % nc] = raw[STAGE ? m : 0];
-          }
+        if constexpr (STAGE) {
+          rx.template park<0, NOP>(sw, n, lane);
+          rv.template park<D, NOP>(sw, n, lane);
+          rF.template park<2 * D, NOP>(sw, n, lane);
         }
         __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         if (act) {
@@ -1377,15 +1529,15 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? 3 : 1) void k_g2p_fast(Ar
           T S1[3] = {T(0), T(0), T(0)}, S2[3] = {T(0), T(0), T(0)}, Sd[3] = {T(0), T(0), T(0)};
 #pragma unroll
           for (int ix = 0; ix < SW; ++ix) {
-            const int q = (ccoord[0] + ix) + G::TS * ((ccoord[1] + jy) + G::TS * (ccoord[2] + kz));
-            T nd[NS];
+            const int q = pad_node<3>(ccoord[0] + ix, ccoord[1] + jy, ccoord[2] + kz);
+            T nd[6];
 #pragma unroll
-            for (int u = 0; u < NS; ++u) nd[u] = s_n[q][u];
+            for (int u = 0; u < 6; ++u) nd[u] = s_n[u][q];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
               S1[c] += val[0][ix] * nd[c];
               S2[c] += der[0][ix] * nd[c];
-              Sd[c] += val[0][ix] * nd[NS / 2 + c];
+              Sd[c] += val[0][ix] * nd[3 + c];
             }
           }
           const T YZ = val[1][jy] * val[2][kz];
@@ -1406,15 +1558,15 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? 3 : 1) void k_g2p_fast(Ar
         T S1[2] = {T(0), T(0)}, S2[2] = {T(0), T(0)}, Sd[2] = {T(0), T(0)};
 #pragma unroll
         for (int ix = 0; ix < SW; ++ix) {
-          const int q = (ccoord[0] + ix) + G::TS * (ccoord[1] + jy);
-          T nd[NS];
+          const int q = pad_node<2>(ccoord[0] + ix, ccoord[1] + jy, 0);
+          T nd[4];
 #pragma unroll
-          for (int u = 0; u < NS; ++u) nd[u] = s_n[q][u];
+          for (int u = 0; u < 4; ++u) nd[u] = s_n[u][q];
 #pragma unroll
           for (int c = 0; c < 2; ++c) {
             S1[c] += val[0][ix] * nd[c];
             S2[c] += der[0][ix] * nd[c];
-            Sd[c] += val[0][ix] * nd[NS / 2 + c];
+            Sd[c] += val[0][ix] * nd[2 + c];
           }
         }
         const T Y = val[1][jy], Yd = der[1][jy];
@@ -1455,15 +1607,23 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? 3 : 1) void k_g2p_fast(Ar
       xn[c] = cur.x[c] + X[c] * dt;
       fldw<T, D, AOS>(ox, oi, c) = xn[c];
     }
-    if (nk && act) {
-      const uint32_t jj = start + uint32_t(coff + k);
-      uint32_t nl = 1;  // the scene's tiles; in a batch its keys start at its first global tile
+    if (nk) {
+      bool moved = false;
+      if (act) {
+        const uint32_t jj = start + uint32_t(coff + k);
+        uint32_t nl = 1;  // the scene's tiles; in a batch its keys start at its first global tile
 #pragma unroll
-      for (int d = 0; d < D; ++d) nl *= uint32_t(a.ntiles[d]);
-      const uint32_t kl = key_of_position<T, D, BASIS>(a, xn, cur.l, nl);
-      const uint32_t key = kl == nl * 64u ? ntiles * 64u : kl + (tile - ltile) * 64u;
-      nk[jj] = key;
-      stay[jj] = key == skeys[jj] ? 1u : 0u;
+        for (int d = 0; d < D; ++d) nl *= uint32_t(a.ntiles[d]);
+        const uint32_t kl = key_of_position<T, D, BASIS>(a, xn, cur.l, nl);
+        const uint32_t key = kl == nl * 64u ? ntiles * 64u : kl + (tile - ltile) * 64u;
+        nk[jj] = key;
+        moved = key != skeys[jj];
+        stay[jj] = moved ? 0u : 1u;
+      }
+      // particles whose key changed, counted per wave by its lowest live lane
+      // (no atomic in the common case of none)
+      const uint64_t mv = __ballot(moved);
+      if (mv && (__ballot(true) & lanemask_lt64()) == 0) atomicAdd(movers, uint32_t(__popcll(mv)));
     }
     T Fn[D * D];
 #pragma unroll
@@ -1477,42 +1637,26 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? 3 : 1) void k_g2p_fast(Ar
       }
 #pragma unroll
     for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(oF, oi, q) = Fn[q];
-    const double J = det_d<T, D>(Fn);
-    double lam = a.lambda, mu = a.mu;
-    if (a.logE.p && act) {
-      const double sc = exp(double(a.logE(pid)));
-      lam *= sc;
-      mu *= sc;
+    {
+      const bool le = a.logE.p && act;
+      T sg[D * D], Vn;
+      neo_hookean<T, D>(Fn, a.lambda, a.mu, a.V0, le, le ? a.logE(pid) : T(0), sg, Vn);
+#pragma unroll
+      for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(os, oi, q) = sg[q];
+      oV(oi) = Vn;
     }
-    const double lnJ = log(J);
-    const double invJ = 1.0 / J;
-#pragma unroll
-    for (int b = 0; b < D; ++b)
-#pragma unroll
-      for (int r = 0; r < D; ++r) {
-        double bb = 0.0;
-#pragma unroll
-        for (int kk = 0; kk < D; ++kk) bb += double(Fn[r + D * kk]) * double(Fn[b + D * kk]);
-        const double sg = mu * (bb - (r == b ? 1.0 : 0.0)) + (r == b ? lam * lnJ : 0.0);
-        fldw<T, D * D, AOS>(os, oi, r + D * b) = T(sg * invJ);
-      }
-    oV(oi) = T(a.V0 * J);
     if constexpr (STAGE) {
       __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
       const int nact = __popcll(has);
       T(*sw)[NOP] = s_out[STAGE ? wave : 0];
       if (contiguous) {
         // each field as one contiguous run of nact*ncomp scalars
-        auto run = [&](T* dst, int nc, int off) {
-#pragma unroll 1
-          for (int i = lane; i < nact * nc; i += 64) dst[int64_t(base) * nc + i] = sw[i / nc][off + i % nc];
-        };
-        run(a.v.p, D, 0);
-        run(a.gradv.p, D * D, D);
-        run(a.x.p, D, D + D * D);
-        run(a.F.p, D * D, 2 * D + D * D);
-        run(a.sigma.p, D * D, 2 * D + 2 * D * D);
-        run(a.V.p, 1, NOUT - 1);
+        store_run<T, D, 0, NOP>(a.v.p, base, nact, sw, lane);
+        store_run<T, D * D, D, NOP>(a.gradv.p, base, nact, sw, lane);
+        store_run<T, D, D + D * D, NOP>(a.x.p, base, nact, sw, lane);
+        store_run<T, D * D, 2 * D + D * D, NOP>(a.F.p, base, nact, sw, lane);
+        store_run<T, D * D, 2 * D + 2 * D * D, NOP>(a.sigma.p, base, nact, sw, lane);
+        store_run<T, 1, NOUT - 1, NOP>(a.V.p, base, nact, sw, lane);
       } else if (act) {
         // scattered slots: each lane stores its own row
         const T* r = sw[rank];
@@ -1534,9 +1678,10 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? 3 : 1) void k_g2p_fast(Ar
 }
 
 // ---- gather + grid-node half ------------------------------------------------
-template <typename T, int D, bool GAOS, bool BATCH = false>
+template <typename T, int D, int SW, bool GAOS, bool BATCH = false>
 __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __restrict__ nodes_per_block) {
   using G = Geo<D>;
+  using PT = Part<D, SW>;
   constexpr int NV = 1 + 2 * D;
   constexpr int NB = (D == 3) ? 27 : 9;
   const int64_t j = xcd_remap(blockIdx.x, a.n_nb);
@@ -1573,35 +1718,47 @@ __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __re
       if (I[d] >= a.mesh.dims[d]) return;
     }
   }
+  // Tiles covering this node, per axis: B-1 when the node lies in the
+  // footprint of the tile below (r < SW-2), B, and B+1 when r == BW-1; at
+  // most two per axis.  All their partial values are loaded first, then
+  // summed in a fixed order (last axis outermost, ascending tiles).
+  int lo[3] = {0, 0, 0}, two[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    lo[d] = rr[d] < SW - 2 ? -1 : 0;
+    two[d] = (lo[d] < 0 || rr[d] == G::BW - 1) ? 1 : 0;
+  }
+  constexpr int NC = 1 << D;
+  T pv[NC][NV];
+  bool use[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    bool ok = true;
+    int nb = 0, q = 0, mulb = 1, mulq = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int k = (c >> d) & 1;
+      ok = ok && k <= two[d];
+      const int o = lo[d] + k;
+      nb += (o + 1) * mulb;
+      mulb *= 3;
+      q += (rr[d] + 1 - o * G::BW) * mulq;  // node index inside that tile's partial
+      mulq *= PT::E;
+    }
+    const int slot = ok ? s_slot[nb] : -1;
+    use[c] = slot >= 0;
+    // a skipped candidate reads tile slot 0's first node (a valid address) and is not added
+    const T* P = a.partials + (use[c] ? int64_t(slot) * PT::SIZE + int64_t(q) * NV : 0);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) pv[c][v] = P[v];
+  }
   T acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = T(0);
-  // tiles covering this node: B-1 if r<=1, B, B+1 if r==BW-1 (per axis),
-  // summed in a fixed order (last axis outermost)
-  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
 #pragma unroll
-  for (int d = 0; d < D; ++d) {
-    lo[d] = rr[d] <= 1 ? -1 : 0;
-    hi[d] = rr[d] == G::BW - 1 ? 1 : 0;
-  }
-  int o[3] = {0, 0, 0};
-  for (o[2] = lo[2]; o[2] <= hi[2]; ++o[2])
-    for (o[1] = lo[1]; o[1] <= hi[1]; ++o[1])
-      for (o[0] = lo[0]; o[0] <= hi[0]; ++o[0]) {
-        int nb = 0, q = 0, mulb = 1, mulq = 1;
+  for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          nb += (o[d] + 1) * mulb;
-          mulb *= 3;
-          q += (rr[d] + 1 - o[d] * G::BW) * mulq;  // node index inside that tile's footprint
-          mulq *= G::TS;
-        }
-        const int slot = s_slot[nb];
-        if (slot < 0) continue;
-        const T* P = a.partials + (int64_t(slot) * G::NFOOT + q) * 8;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) acc[v] += P[v];
-      }
+    for (int v = 0; v < NV; ++v) acc[v] = use[c] ? acc[v] + pv[c][v] : acc[v];
   const int64_t i = grid_slot(a, I);
   // grid-node half with the guarded Float64 division (test/transfer.jl:102-104)
   const double md = double(acc[0]);
@@ -1670,26 +1827,11 @@ __device__ __forceinline__ void particle_update(const Args<T, D>& a, int64_t p, 
     }
 #pragma unroll
   for (int k = 0; k < D * D; ++k) a.F(p, k) = Fn[k];
-  const double J = det_d<T, D>(Fn);
-  double lam = a.lambda, mu = a.mu;
-  if (a.logE.p) {
-    const double s = exp(double(a.logE(p)));
-    lam *= s;
-    mu *= s;
-  }
-  const double lnJ = log(J);
-  const double invJ = 1.0 / J;
+  T sg[D * D], Vn;
+  neo_hookean<T, D>(Fn, a.lambda, a.mu, a.V0, a.logE.p != nullptr, a.logE.p ? a.logE(p) : T(0), sg, Vn);
 #pragma unroll
-  for (int b = 0; b < D; ++b)
-#pragma unroll
-    for (int r = 0; r < D; ++r) {
-      double bb = 0.0;
-#pragma unroll
-      for (int k = 0; k < D; ++k) bb += double(Fn[r + D * k]) * double(Fn[b + D * k]);
-      const double s = mu * (bb - (r == b ? 1.0 : 0.0)) + (r == b ? lam * lnJ : 0.0);
-      a.sigma(p, r + D * b) = T(s * invJ);
-    }
-  a.V(p) = T(a.V0 * J);
+  for (int k = 0; k < D * D; ++k) a.sigma(p, k) = sg[k];
+  a.V(p) = Vn;
 }
 
 // One wave per tile, lanes own the tile's classes as in P2G (the walk is
@@ -1943,7 +2085,6 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
   Binning& b = ctx->bin;
   b.valid = false;
   uint32_t* counters = ctx->counters.as<uint32_t>(CNT_SLOTS);
-  TESS_HIP(hipMemsetAsync(counters, 0, CNT_BIN_SLOTS * sizeof(uint32_t), ctx->stream));
   uint32_t* keys = ctx->keys.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
   uint32_t* skeys = ctx->keys_sorted.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
   uint32_t* perm = ctx->vals.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
@@ -1961,6 +2102,25 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
                            b.state_dim == D && n > 0;
   const bool precomputed = keys_from_g2p && b.keys_ready;
   b.keys_ready = false;
+  if (n > 0 && incremental && precomputed && b.x_ptr == P.x.ptr && b.tile_log2 == Geo<D>::LT &&
+      b.basis == BASIS && b.dtype == mesh.dtype) {
+    // The previous G2P counted the particles whose key changed.  With none,
+    // the sorted keys, the order and every tile and node-block list of the
+    // previous binning are what binning again would produce: keep them.
+    uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
+    TESS_HIP(hipMemcpyAsync(hc, counters, CNT_SLOTS * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    TESS_HIP(hipStreamSynchronize(ctx->stream));
+    if (hc[CNT_MOVERS] == 0) {
+      ctx->last_movers = 0;
+      ctx->last_active_nodes = int64_t(hc[CNT_NODES]);
+      ctx->last_misplaced = (ctx->misplaced_x_ptr == P.x.ptr && ctx->misplaced_n == n) ? int64_t(hc[CNT_MISPLACED]) : -1;
+      ctx->kept_binnings += 1;
+      b.valid = true;
+      prof_end(ctx, tok);
+      return;
+    }
+  }
+  TESS_HIP(hipMemsetAsync(counters, 0, CNT_BIN_SLOTS * sizeof(uint32_t), ctx->stream));
   if (n > 0 && incremental) {
     // keys in the previous sorted order, stayer flags -> stayer ranks; the
     // previous G2P of the same step loop may have written them already
@@ -2120,7 +2280,7 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     ctx->misplaced_x_ptr = nullptr;
   }
   if (b.n_active > 0) {
-    a.partials = ctx->partials.as<T>(size_t(b.n_active) * 8 * Gm::NFOOT);
+    a.partials = ctx->partials.as<T>(size_t(b.n_active) * Part<D, SW>::SIZE);
     // algorithmic bytes: x, v, V, σ per particle
     const int tok = prof_begin(ctx, "k_p2g_tile", double(b.n_inside) * sizeof(T) * (2 * D + 1 + D * D));
     const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
@@ -2130,9 +2290,12 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     } else {
       // 3D: one wave per offset along z; 2D: one wave while the stencil fits
       constexpr int NPF = (D == 3 || Pow<SW, D>::v * NV * int(sizeof(T)) / 4 > 96) ? SW : 1;
-      if (julia_layout(P, D))
+      if (julia_layout(P, D) && (ctx->staging & 1))
         hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, true>), dim3(xcd_grid(b.n_active)), dim3(64 * NPF), 0,
                            ctx->stream, a, sk, misplaced);
+      else if (julia_layout(P, D))
+        hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, true, false, false>), dim3(xcd_grid(b.n_active)),
+                           dim3(64 * NPF), 0, ctx->stream, a, sk, misplaced);
       else
         hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, false>), dim3(xcd_grid(b.n_active)), dim3(64 * NPF), 0,
                            ctx->stream, a, sk, misplaced);
@@ -2150,10 +2313,10 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     const double nodes = double(std::max<int64_t>(ctx->last_active_nodes, 0));
     const int tok = prof_begin(ctx, "k_gather_grid", nodes * sizeof(T) * (2 * (2 * D + 1) + 2 * D));
     if (grid_julia_layout(G, D))
-      hipLaunchKernelGGL((k_gather_grid<T, D, true>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a,
+      hipLaunchKernelGGL((k_gather_grid<T, D, SW, true>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a,
                          per_nb);
     else
-      hipLaunchKernelGGL((k_gather_grid<T, D, false>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a,
+      hipLaunchKernelGGL((k_gather_grid<T, D, SW, false>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a,
                          per_nb);
     TESS_HIP(hipGetLastError());
     prof_end(ctx, tok);
@@ -2189,9 +2352,12 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   // next-step keys at the sorted positions (deferred weights only)
   uint32_t* nk = nullptr;
   uint32_t* stay = nullptr;
+  uint32_t* mv = nullptr;  // movers: particles whose key changes
   if (write_keys && !STORED && P.n > 0) {
     nk = ctx->inc_nk.as<uint32_t>(size_t(P.n));
     stay = ctx->inc_spos.as<uint32_t>(size_t(P.n) + 1);
+    mv = ctx->counters.as<uint32_t>(CNT_SLOTS) + CNT_MOVERS;
+    TESS_HIP(hipMemsetAsync(mv, 0, sizeof(uint32_t), ctx->stream));
   }
   const uint32_t* skeys_now = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
   if (b.n_active > 0) {
@@ -2208,12 +2374,15 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     } else {
       constexpr int NW = 4;
       const uint32_t nt = uint32_t(b.ntiles);
-      if (julia_layout(P, D))
+      if (julia_layout(P, D) && (ctx->staging & 2))
         hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
-                           ctx->stream, a, sk, nk, stay, nt);
+                           ctx->stream, a, sk, nk, stay, nt, mv);
+      else if (julia_layout(P, D))
+        hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW, false, false>), dim3(xcd_grid(b.n_active)),
+                           dim3(64 * NW), 0, ctx->stream, a, sk, nk, stay, nt, mv);
       else
         hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, false, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
-                           ctx->stream, a, sk, nk, stay, nt);
+                           ctx->stream, a, sk, nk, stay, nt, mv);
     }
     TESS_HIP(hipGetLastError());
     prof_end(ctx, tok);

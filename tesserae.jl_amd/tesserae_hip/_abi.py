@@ -210,11 +210,12 @@ _lib = None
 
 
 def load_library(path: str | None = None) -> ctypes.CDLL:
-    """Load libtesserae_hip.so (built in-tree by __graft_entry__.build())."""
+    """Load libtesserae_hip.so (built in-tree by __graft_entry__.build()); the environment
+    variable TESS_HIP_LIBRARY names another build of the same library to load instead."""
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = path or LIB_PATH
+    p = path or os.environ.get("TESS_HIP_LIBRARY") or LIB_PATH
     if not os.path.exists(p):
         raise ImportError(
             f"{LIB_NAME} not found at {p}; build it with `python -c 'import __graft_entry__ as g; g.build()'` "

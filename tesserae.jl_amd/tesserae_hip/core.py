@@ -538,9 +538,10 @@ class Context:
             i += 1
 
     def stats(self) -> dict:
-        vals = (ctypes.c_int64 * 7)()
-        _check(self.lib, self.lib.tess_stats(self.handle, vals, 7))
-        keys = ("n_inside", "active_tiles", "node_blocks", "active_nodes", "reorders", "misplaced", "movers")
+        vals = (ctypes.c_int64 * 8)()
+        _check(self.lib, self.lib.tess_stats(self.handle, vals, 8))
+        keys = ("n_inside", "active_tiles", "node_blocks", "active_nodes", "reorders", "misplaced", "movers",
+                "kept_binnings")
         return dict(zip(keys, list(vals)))
 
 
