@@ -26,6 +26,19 @@ for p in (os.path.join(ROOT, "tesserae.jl_amd"), os.path.join(ROOT, "tests")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the block-drop kernels from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE) scaled by a plain 1 GiB copy; written next to the
+# rocprofv3 summaries it comes from
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "block_drop_hbm_traffic.json")
+
+
+def profiled_traffic(kernel):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)["kernels"][kernel]
+        return float(rec["bytes"])
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
 
 
 def parse():
@@ -76,9 +89,11 @@ def cpu_baseline(threads):
 
 
 def adjoint_ratio(nsteps, every):
-    """Differentiable inverse cube (dx=1/64, 132,651 particles, quadratic): time of a plain
-    forward of nsteps against a checkpointed rollout plus its pullback of the loss
-    mean ||x(T) - (x(0) + (0.1,0,0))||^2 (FP32 forward, FP64 adjoint)."""
+    """Differentiable inverse cube (dx=1/64, 132,651 particles, quadratic), nsteps steps.
+    ratio: the checkpointed pullback of the loss mean ||x(T) - (x(0) + (0.1,0,0))||^2
+    (replay of every segment from its checkpoint plus the reversed steps; FP64 adjoint)
+    over the plain FP32 forward of the same steps.  ratio_with_rollout adds the
+    checkpointed forward (the rollout that records the checkpoints) to the numerator."""
     import numpy as np
     import torch
 
@@ -119,7 +134,8 @@ def adjoint_ratio(nsteps, every):
     return {"workload": f"inverse cube dx=1/64, {sc.n} particles, quadratic, {nsteps} steps, checkpoint every "
                         f"{every}, FP32 forward / FP64 adjoint",
             "t_forward_ms": t_fwd, "t_rollout_ms": t_roll, "t_pullback_ms": t_adj,
-            "ratio": (t_roll + t_adj) / t_fwd, "loss": loss, "grad_v0_norm": gnorm}
+            "ratio": t_adj / t_fwd, "ratio_with_rollout": (t_roll + t_adj) / t_fwd, "loss": loss,
+            "grad_v0_norm": gnorm}
 
 
 def batch_leg(steps):
@@ -303,7 +319,8 @@ def main():
         "config": {"workload": "3D elastic block drop, dx=1/256, quadratic B-spline, 13,434,880 particles, FP32",
                    "particles": n, "layout": args.layout},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom),
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
                      "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
         "step_hbm": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (ms * 1e-3) / 1e9,
                      "frac": step_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), "active_nodes": stats["active_nodes"]},

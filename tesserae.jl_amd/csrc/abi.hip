@@ -6,7 +6,6 @@
 
 #include <cmath>
 #include <cstdio>
-#include <cstdlib>
 #include <algorithm>
 #include <new>
 #include <string>
@@ -210,12 +209,9 @@ tess_status tess_ctx_create(int32_t device, void* stream, tess_ctx** out) {
     tess_ctx* c = new tess_ctx();
     c->device = device;
     c->stream = static_cast<hipStream_t>(stream);
-    if (const char* st = std::getenv("TESS_LAYER_STAGING")) {
-      const int v = std::atoi(st);
-      if (v >= 0 && v <= 3) c->staging = v;
-    }
     void* pinned = nullptr;
-    hipError_t e = hipHostMalloc(&pinned, 64 * sizeof(int64_t), hipHostMallocDefault);
+    // pinned readback slots: counters (words 0-63), mover counts (words 64-319)
+    hipError_t e = hipHostMalloc(&pinned, 512 * sizeof(int64_t), hipHostMallocDefault);
     if (e != hipSuccess) {
       delete c;
       check_hip(e, "hipHostMalloc");
@@ -243,7 +239,7 @@ tess_status tess_ctx_destroy(tess_ctx* ctx) {
                             &ctx->inc_mks,   &ctx->inc_mr,    &ctx->inc_mjs,
                             &ctx->sp_occ,    &ctx->sp_act,    &ctx->sp_scan,
                             &ctx->adj_partials, &ctx->adj_nodes, &ctx->ro_checkpoints, &ctx->ro_segment,
-                            &ctx->ro_gradv};
+                            &ctx->ro_gradv,  &ctx->ro_snapshots, &ctx->movers};
     for (auto* b : bufs) b->release();
     for (auto ev : ctx->prof.pool) (void)hipEventDestroy(ev);
     if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);

@@ -99,17 +99,20 @@ __device__ __forceinline__ void window_jets2(const Args<T, D>& a, const T* x, co
 }
 
 // Particle adjoints -> per-tile node partials [tile][NFOOT][2D]: Δv̄ (D) then v̄ (D).
-// One wave per tile, lane = window-start class, as in P2G; one pass per offset
-// along the last axis keeps the Float64 accumulators in registers.
+// SW waves per tile, lane = window-start class, as in P2G: wave w accumulates
+// the offsets w along the last axis (the Float64 accumulators of one offset
+// plane fit in registers); the waves add into the LDS tile one after another.
 template <typename T, int D, int BASIS>
-__global__ __launch_bounds__(64) void k_adj_scatter(Args<T, D> a, const uint32_t* __restrict__ skeys,
+__global__ __launch_bounds__(64 * support_width(BASIS)) void k_adj_scatter(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                     DField<double> ax, DField<double> av, DField<double> aF,
                                                     double* __restrict__ partials) {
   using G = Geo<D>;
   constexpr int SW = support_width(BASIS);
   constexpr int NA = 2 * D;
   constexpr int NSL = Pow<SW, D - 1>::v;
-  const int lane = threadIdx.x;
+  constexpr int NT = 64 * SW;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
   const int64_t j = xcd_remap(blockIdx.x, a.n_active);
   if (j >= a.n_active) return;
   const uint32_t tile = a.active[j];
@@ -121,9 +124,9 @@ __global__ __launch_bounds__(64) void k_adj_scatter(Args<T, D> a, const uint32_t
   for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
   __shared__ double s_tile[G::NFOOT][NA];
   __shared__ uint32_t s_cnt[64];
-  for (int q = lane; q < NA * G::NFOOT; q += 64) (&s_tile[0][0])[q] = 0.0;
+  for (int q = int(threadIdx.x); q < NA * G::NFOOT; q += NT) (&s_tile[0][0])[q] = 0.0;
   int ccount, coff, maxcnt;
-  tile_class_counts(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);
+  tile_class_counts<SW>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);
   int ccoord[D];
   {
     int r = lane;
@@ -134,7 +137,8 @@ __global__ __launch_bounds__(64) void k_adj_scatter(Args<T, D> a, const uint32_t
     }
   }
   const double dt = double(a.dt);
-  for (int pass = 0; pass < SW; ++pass) {
+  {
+    const int pass = wave;
     double acc[NSL][NA];
 #pragma unroll
     for (int s = 0; s < NSL; ++s)
@@ -193,26 +197,29 @@ __global__ __launch_bounds__(64) void k_adj_scatter(Args<T, D> a, const uint32_t
         }
       }
     }
-    if (ccount > 0) {
-      // lanes own distinct classes: one add instruction never hits a node twice
+    // lanes own distinct classes: one add instruction never hits a node twice;
+    // the waves add one after another, so the per-node order is fixed
+    for (int w = 0; w < SW; ++w) {
+      if (w == wave && ccount > 0) {
 #pragma unroll
-      for (int s = 0; s < NSL; ++s) {
-        int r = s, q = 0, mul = 1;
+        for (int s = 0; s < NSL; ++s) {
+          int r = s, q = 0, mul = 1;
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const int od = (d == D - 1) ? pass : r % SW;
-          if (d < D - 1) r /= SW;
-          q += (ccoord[d] + od) * mul;
-          mul *= G::TS;
+          for (int d = 0; d < D; ++d) {
+            const int od = (d == D - 1) ? pass : r % SW;
+            if (d < D - 1) r /= SW;
+            q += (ccoord[d] + od) * mul;
+            mul *= G::TS;
+          }
+#pragma unroll
+          for (int v = 0; v < NA; ++v) atomicAdd(&s_tile[q][v], acc[s][v]);
         }
-#pragma unroll
-        for (int v = 0; v < NA; ++v) atomicAdd(&s_tile[q][v], acc[s][v]);
       }
+      __syncthreads();
     }
   }
-  __syncthreads();
   double* out = partials + int64_t(j) * NA * G::NFOOT;
-  for (int q = lane; q < NA * G::NFOOT; q += 64) out[q] = (&s_tile[0][0])[q];
+  for (int q = int(threadIdx.x); q < NA * G::NFOOT; q += NT) out[q] = (&s_tile[0][0])[q];
 }
 
 // Node adjoints [node][2D+1] = f̄ (D), q̄ (D), m̄ of every node of the touched
@@ -600,13 +607,12 @@ inline DField<double> dfield_d(const tess_field& f) {
   return DField<double>{static_cast<double*>(f.ptr), f.elem_stride, f.comp_stride};
 }
 
-// One reversed step: S is the state at the start of the step.
+// The reversed kernels of one step, after the binning of S and its forward P2G
+// grid are in place.
 template <typename T, int D, int BASIS>
-void step_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& S,
-                       const tess_params& prm, const tess_adjoint& adj) {
+void adjoint_kernels(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& S,
+                     const tess_params& prm, const tess_adjoint& adj) {
   using Gm = Geo<D>;
-  bin_particles<T, D, BASIS, false>(ctx, mesh, S, nullptr);
-  p2g_launch<T, D, BASIS, false>(ctx, mesh, G, S, nullptr, prm, false, nullptr);
   const Binning& b = ctx->bin;
   if (b.n_active == 0) return;  // every particle outside: the step is the identity on the state
   Args<T, D> a = make_args<T, D>(mesh, &G, S, nullptr, &prm);
@@ -618,7 +624,8 @@ void step_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G,
   const DField<double> ax = dfield_d(adj.x), av = dfield_d(adj.v), aF = dfield_d(adj.F), ath = dfield_d(adj.logE);
   const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
   int tok = prof_begin(ctx, "k_adj_scatter", 0.0);
-  hipLaunchKernelGGL((k_adj_scatter<T, D, BASIS>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream, a, sk, ax,
+  hipLaunchKernelGGL((k_adj_scatter<T, D, BASIS>), dim3(xcd_grid(b.n_active)), dim3(64 * support_width(BASIS)), 0,
+                     ctx->stream, a, sk, ax,
                      av, aF, partials);
   TESS_HIP(hipGetLastError());
   prof_end(ctx, tok);
@@ -632,6 +639,100 @@ void step_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G,
                      nodeadj, ax, av, aF, ath);
   TESS_HIP(hipGetLastError());
   prof_end(ctx, tok);
+}
+
+// One reversed step: S is the state at the start of the step.
+template <typename T, int D, int BASIS>
+void step_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& S,
+                       const tess_params& prm, const tess_adjoint& adj) {
+  bin_particles<T, D, BASIS, false>(ctx, mesh, S, nullptr);
+  p2g_launch<T, D, BASIS, false>(ctx, mesh, G, S, nullptr, prm, false, nullptr);
+  adjoint_kernels<T, D, BASIS>(ctx, mesh, G, S, prm, adj);
+}
+
+// ---- snapshots of a step's binning and forward grid ------------------------------
+//
+// While a segment is replayed, each step's binning arrays and P2G grid are
+// copied aside, so that the reversed step restores them instead of binning and
+// scattering again.  One launch copies every array (4-byte words).
+
+struct CopyTask {
+  const uint32_t* src;
+  uint32_t* dst;
+  int64_t words;
+};
+struct CopyTable {
+  CopyTask t[16];
+};
+
+__global__ __launch_bounds__(256) void k_copy_multi(CopyTable tab) {
+  const CopyTask c = tab.t[blockIdx.y];
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < c.words; i += int64_t(gridDim.x) * 256)
+    c.dst[i] = c.src[i];
+}
+
+// words of a field of nc components over n elements, whatever its strides
+template <typename T>
+int64_t field_words(const tess_field& f, int64_t n, int nc) {
+  if (!f.ptr || n == 0) return 0;
+  const int64_t span = (n - 1) * f.elem_stride + int64_t(nc - 1) * f.comp_stride + 1;
+  return span * int64_t(sizeof(T)) / 4;
+}
+
+// The arrays of one snapshot: binning (perm, sorted keys, tile runs and lists)
+// and the grid fields m, mv, f, vⁿ, v.
+template <typename T, int D>
+int snapshot_arrays(tess_ctx* ctx, const tess_grid& G, int64_t n, int64_t ntiles, int64_t nnodes, void* ptrs[14],
+                    int64_t words[14]) {
+  tess::Buffer* bufs[] = {&ctx->vals,      &ctx->keys_sorted, &ctx->tile_start, &ctx->tile_end, &ctx->active_tiles,
+                          &ctx->tile_flag, &ctx->tile_slot,   &ctx->nb_list,    &ctx->nb_flag};
+  const int64_t lens[] = {n, n, ntiles, ntiles, ntiles, ntiles, ntiles, ntiles, ntiles};
+  int k = 0;
+  for (int i = 0; i < 9; ++i) {
+    ptrs[k] = bufs[i]->ptr;
+    words[k++] = lens[i];
+  }
+  const tess_field* gf[] = {&G.m, &G.mv, &G.f, &G.vn, &G.v};
+  const int gc[] = {1, D, D, D, D};
+  for (int i = 0; i < 5; ++i) {
+    ptrs[k] = gf[i]->ptr;
+    words[k++] = field_words<T>(*gf[i], nnodes, gc[i]);
+  }
+  return k;
+}
+
+template <typename T, int D>
+int64_t snapshot_words(tess_ctx* ctx, const tess_grid& G, int64_t n, int64_t ntiles, int64_t nnodes) {
+  void* p[14];
+  int64_t w[14];
+  const int k = snapshot_arrays<T, D>(ctx, G, n, ntiles, nnodes, p, w);
+  int64_t t = 0;
+  for (int i = 0; i < k; ++i) t += w[i];
+  return t;
+}
+
+// save (to_store) or restore the snapshot at store
+template <typename T, int D>
+void snapshot_copy(tess_ctx* ctx, const tess_grid& G, int64_t n, int64_t ntiles, int64_t nnodes, uint32_t* store,
+                   bool to_store) {
+  void* p[14];
+  int64_t w[14];
+  const int k = snapshot_arrays<T, D>(ctx, G, n, ntiles, nnodes, p, w);
+  CopyTable tab;
+  std::memset(static_cast<void*>(&tab), 0, sizeof(tab));
+  int64_t off = 0, most = 0;
+  for (int i = 0; i < k; ++i) {
+    uint32_t* arr = static_cast<uint32_t*>(p[i]);
+    tab.t[i].src = to_store ? arr : store + off;
+    tab.t[i].dst = to_store ? store + off : arr;
+    tab.t[i].words = (arr && w[i] > 0) ? w[i] : 0;
+    off += w[i];
+    most = std::max(most, w[i]);
+  }
+  if (most == 0) return;
+  const unsigned gx = unsigned(std::min<int64_t>(blocks_for(most, 256), 2048));
+  hipLaunchKernelGGL(k_copy_multi, dim3(gx, unsigned(k)), dim3(256), 0, ctx->stream, tab);
+  TESS_HIP(hipGetLastError());
 }
 
 // ---- checkpointed rollouts -----------------------------------------------------
@@ -701,9 +802,10 @@ void rollout_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, cons
   const int64_t per = state_scalars<T, D>() * n;
   T* ck = ctx->ro_checkpoints.as<T>(size_t(std::max<int64_t>(per * ncheck, 1)));
   ro.logE = P.logE;
-  for (int s = 0; s < nsteps; ++s) {
-    if (s % every == 0) copy_state<T, D>(ctx, P, state_view<T, D>(ck + per * (s / every), n, ro, nullptr));
-    step_impl<T, D, BASIS, false>(ctx, mesh, G, P, prm, 1, 0.0, nullptr);
+  // one step loop per segment between checkpoints
+  for (int c = 0; c < ncheck; ++c) {
+    copy_state<T, D>(ctx, P, state_view<T, D>(ck + per * c, n, ro, nullptr));
+    step_impl<T, D, BASIS, false>(ctx, mesh, G, P, prm, std::min(every, nsteps - c * every), 0.0, nullptr);
   }
   ro.valid = true;
   ro.nsteps = nsteps;
@@ -725,6 +827,15 @@ void rollout_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid&
   T* ck = static_cast<T*>(ctx->ro_checkpoints.ptr);
   T* seg = ctx->ro_segment.as<T>(size_t(std::max<int64_t>(per * ro.every, 1)));
   T* gradv = ctx->ro_gradv.as<T>(size_t(std::max<int64_t>(n * D * D, 1)));
+  int64_t nnodes = 1;
+  for (int d = 0; d < D; ++d) nnodes *= mesh.dims[d];
+  // The replay of step k bins state k and scatters its grid: both are kept for
+  // the reversed step k when the every-1 snapshots of a segment fit in 4 GiB;
+  // otherwise the reversed step bins and scatters again.
+  constexpr int64_t kSnapBytes = int64_t(4) << 30;
+  int64_t snap_words = -1;  // per snapshot; 0: not kept
+  uint32_t* snaps = nullptr;
+  ro.snap_bins.assign(size_t(std::max(ro.every - 1, 0)), Binning{});
   for (int c = ncheck - 1; c >= 0; --c) {
     const int s0 = c * ro.every;
     const int len = std::min(ro.every, ro.nsteps - s0);
@@ -735,10 +846,31 @@ void rollout_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid&
                               ctx->stream));
       const tess_particles S = state_view<T, D>(seg + per * k, n, ro, gradv);
       step_impl<T, D, BASIS, false>(ctx, mesh, G, S, prm, 1, 0.0, nullptr);
+      // the context now holds the binning of state k-1 and G its P2G grid
+      if (snap_words < 0) {
+        const int64_t w = snapshot_words<T, D>(ctx, G, n, ctx->bin.ntiles, nnodes);
+        snap_words = w * 4 * int64_t(ro.every - 1) <= kSnapBytes ? w : 0;
+        if (snap_words > 0) snaps = ctx->ro_snapshots.as<uint32_t>(size_t(snap_words) * size_t(ro.every - 1));
+      }
+      if (snap_words > 0) {
+        snapshot_copy<T, D>(ctx, G, n, ctx->bin.ntiles, nnodes, snaps + snap_words * (k - 1), true);
+        ro.snap_bins[size_t(k - 1)] = ctx->bin;
+      }
     }
     for (int k = len - 1; k >= 0; --k) {
       const tess_particles S = state_view<T, D>(seg + per * k, n, ro, gradv);
-      step_adjoint_impl<T, D, BASIS>(ctx, mesh, G, S, prm, adj);
+      if (snap_words > 0 && k < len - 1) {
+        Binning& b = ctx->bin;
+        b = ro.snap_bins[size_t(k)];
+        snapshot_copy<T, D>(ctx, G, n, b.ntiles, nnodes, snaps + snap_words * k, false);
+        b.valid = true;
+        b.keys_ready = false;
+        b.x_ptr = S.x.ptr;
+        ctx->last_grid_m = nullptr;  // G holds a restored grid: the next P2G zeroes it whole
+        adjoint_kernels<T, D, BASIS>(ctx, mesh, G, S, prm, adj);
+      } else {
+        step_adjoint_impl<T, D, BASIS>(ctx, mesh, G, S, prm, adj);
+      }
     }
   }
 }

@@ -80,14 +80,34 @@ Args<T, D> batch_args(tess_ctx* ctx, tess_batch& b) {
 // Full sort of the batch's keys every step (the batch-wide tile space), then the
 // tile runs, active tiles and touched node blocks, as bin_particles does.
 template <typename T, int D, int BASIS>
-void batch_bin(tess_ctx* ctx, tess_batch& B) {
+void batch_bin(tess_ctx* ctx, tess_batch& B, bool keys_from_g2p) {
   const int64_t n = B.ntot;
   const uint64_t ntiles = uint64_t(B.tiles_total);
   Binning& b = ctx->bin;
+  const bool incremental = keys_from_g2p && B.keys_ready && B.binned && b.n == n && b.ntiles == int64_t(ntiles) &&
+                           n > 0;
+  B.keys_ready = false;
   b.valid = false;
   b.state_valid = false;  // the single-scene incremental state does not describe a batch
   b.keys_ready = false;
   uint32_t* counters = ctx->counters.as<uint32_t>(CNT_SLOTS);
+  uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
+  const int kbits = key_bits_for(ntiles * 64);
+  int64_t movers = -1;
+  if (incremental) {
+    // the last G2P counted the particles whose key changed; with none the
+    // previous order and tile lists stand
+    TESS_HIP(hipMemcpyAsync(hc, counters, CNT_SLOTS * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    movers_readback(ctx);
+    TESS_HIP(hipStreamSynchronize(ctx->stream));
+    movers = movers_total(ctx);
+    if (movers == 0) {
+      ctx->last_active_nodes = int64_t(hc[CNT_NODES]);
+      B.kept_binnings += 1;
+      b.valid = true;
+      return;
+    }
+  }
   TESS_HIP(hipMemsetAsync(counters, 0, CNT_BIN_SLOTS * sizeof(uint32_t), ctx->stream));
   uint32_t* keys = ctx->keys.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
   uint32_t* skeys = ctx->keys_sorted.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
@@ -102,10 +122,24 @@ void batch_bin(tess_ctx* ctx, tess_batch& B) {
   uint32_t* nblist = ctx->nb_list.as<uint32_t>(ntiles);
   Args<T, D> a = batch_args<T, D>(ctx, B);
   const int tok = prof_begin(ctx, "binning", 0.0);
-  hipLaunchKernelGGL((k_bin_keys<T, D, BASIS, false, true>), dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, a,
-                     keys, uint32_t(ntiles), counters);
-  TESS_HIP(hipGetLastError());
-  radix_sort_pairs(ctx, keys, nullptr, skeys, perm, n, key_bits_for(ntiles * 64));
+  if (incremental) {
+    // movers merged into the previous sorted order
+    uint32_t* nk = ctx->inc_nk.as<uint32_t>(size_t(n));
+    uint32_t* spos = ctx->inc_spos.as<uint32_t>(size_t(n) + 1);
+    TESS_HIP(hipMemsetAsync(spos + n, 0, sizeof(uint32_t), ctx->stream));
+    exclusive_scan_u32(ctx, spos, n + 1, counters + CNT_STAY);
+    TESS_HIP(hipMemcpyAsync(hc + CNT_STAY, counters + CNT_STAY, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    TESS_HIP(hipStreamSynchronize(ctx->stream));
+    const int64_t M = n - int64_t(hc[CNT_STAY]);
+    if (M > 0) merge_movers(ctx, nk, spos, n, M, kbits);
+    skeys = static_cast<uint32_t*>(ctx->keys_sorted.ptr);
+    perm = static_cast<uint32_t*>(ctx->vals.ptr);
+  } else {
+    hipLaunchKernelGGL((k_bin_keys<T, D, BASIS, false, true>), dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, a,
+                       keys, uint32_t(ntiles), counters);
+    TESS_HIP(hipGetLastError());
+    radix_sort_pairs(ctx, keys, nullptr, skeys, perm, n, kbits);
+  }
   TESS_HIP(hipMemsetAsync(tstart, 0, ntiles * sizeof(uint32_t), ctx->stream));
   TESS_HIP(hipMemsetAsync(tend, 0, ntiles * sizeof(uint32_t), ctx->stream));
   hipLaunchKernelGGL(k_tile_runs, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, skeys, n, uint32_t(ntiles),
@@ -130,7 +164,6 @@ void batch_bin(tess_ctx* ctx, tess_batch& B) {
                      uint32_t(ntiles), nblist);
   TESS_HIP(hipGetLastError());
   prof_end(ctx, tok);
-  uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
   TESS_HIP(hipMemcpyAsync(hc, counters, CNT_SLOTS * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
   TESS_HIP(hipStreamSynchronize(ctx->stream));
   if (hc[CNT_ERR] & ERR_GIMP_LENGTH) raise(TESS_ERR_ARGUMENT, "uGIMP requires 0 ≤ pt.l ≤ spacing(mesh)");
@@ -141,6 +174,7 @@ void batch_bin(tess_ctx* ctx, tess_batch& B) {
   b.ntiles = int64_t(ntiles);
   b.n = n;
   b.valid = true;
+  B.binned = true;
 }
 
 template <typename T, int D, int BASIS, int NP>
@@ -151,10 +185,10 @@ void launch_p2g(tess_ctx* ctx, const Args<T, D>& a, int64_t n_active, const uint
 }
 
 template <typename T, int D, int BASIS, int NW>
-void launch_g2p(tess_ctx* ctx, const Args<T, D>& a, int64_t n_active, const uint32_t* sk) {
+void launch_g2p(tess_ctx* ctx, const Args<T, D>& a, int64_t n_active, const uint32_t* sk, uint32_t* nk, uint32_t* stay,
+                uint32_t* movers) {
   hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW, true>), dim3(xcd_grid(n_active)), dim3(64 * NW), 0,
-                     ctx->stream, a, sk, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
-                     uint32_t(ctx->bin.ntiles), static_cast<uint32_t*>(nullptr));
+                     ctx->stream, a, sk, nk, stay, uint32_t(ctx->bin.ntiles), movers);
   TESS_HIP(hipGetLastError());
 }
 
@@ -163,7 +197,8 @@ void batch_step_impl(tess_ctx* ctx, tess_batch& B, int nsteps) {
   using Gm = Geo<D>;
   constexpr int SW = support_width(BASIS);
   for (int s = 0; s < nsteps; ++s) {
-    batch_bin<T, D, BASIS>(ctx, B);
+    // from the second step of the call on, the previous G2P wrote the keys
+    batch_bin<T, D, BASIS>(ctx, B, s > 0);
     const Binning& b = ctx->bin;
     Args<T, D> a = batch_args<T, D>(ctx, B);
     // zero what this step does not overwrite: every grid the first time, then
@@ -200,18 +235,29 @@ void batch_step_impl(tess_ctx* ctx, tess_batch& B, int nsteps) {
                               hipMemcpyDeviceToDevice, ctx->stream));
     }
     B.prev_nb = b.n_nodeblocks;
+    // next-step keys at the sorted positions while the call goes on
+    uint32_t* nk = nullptr;
+    uint32_t* stay = nullptr;
+    uint32_t* mv = nullptr;
+    if (s + 1 < nsteps && B.ntot > 0) {
+      nk = ctx->inc_nk.as<uint32_t>(size_t(B.ntot));
+      stay = ctx->inc_spos.as<uint32_t>(size_t(B.ntot) + 1);
+      mv = movers_counters(ctx);
+      movers_zero(ctx);
+    }
     if (b.n_active > 0) {
       const int tok = prof_begin(ctx, "k_g2p_tile",
                                  double(b.n_inside) * sizeof(T) * (2 * D + D * D + 2 * D + 3 * D * D + 1));
-      if (B.variant & 2) launch_g2p<T, D, BASIS, 2>(ctx, a, b.n_active, sk);
-      else launch_g2p<T, D, BASIS, 4>(ctx, a, b.n_active, sk);
+      if (B.variant & 2) launch_g2p<T, D, BASIS, 2>(ctx, a, b.n_active, sk, nk, stay, mv);
+      else launch_g2p<T, D, BASIS, 4>(ctx, a, b.n_active, sk, nk, stay, mv);
       prof_end(ctx, tok);
     }
     if (B.ntot > b.n_inside) {
       hipLaunchKernelGGL((k_g2p_outside<T, D, true>), dim3(blocks_for(B.ntot - b.n_inside, 256)), dim3(256), 0,
-                         ctx->stream, a, sk, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr));
+                         ctx->stream, a, sk, nk, stay);
       TESS_HIP(hipGetLastError());
     }
+    B.keys_ready = nk != nullptr;
   }
   // the single-scene zeroing bookkeeping no longer describes any grid
   ctx->last_grid_m = nullptr;

@@ -84,6 +84,9 @@ struct Rollout {
   int32_t dim = 0, dtype = 0, basis = 0;
   int64_t dims[3] = {0, 0, 0};
   tess_field logE{nullptr, 0, 0};
+  // binnings of the replayed steps of one segment (host part; the arrays and
+  // the P2G grid of each step live in tess_ctx::ro_snapshots)
+  std::vector<Binning> snap_bins;
 };
 
 }  // namespace tess
@@ -104,6 +107,7 @@ struct tess_ctx {
   int64_t last_movers = -1;
   int64_t kept_binnings = 0;  // binnings that kept the previous layout (no particle changed its key)
   tess::Buffer counters;   // small device scalars
+  tess::Buffer movers;     // G2P's changed-key counts (spread over cache lines)
   tess::Buffer host_pinned_dummy;
   int64_t* counters_host = nullptr;  // pinned readback slots
   tess::Binning bin;
@@ -124,12 +128,8 @@ struct tess_ctx {
   // SpGrid block activation scratch (occupied flags, active flags, scan)
   tess::Buffer sp_occ, sp_act, sp_scan;
   // adjoint: per-tile Float64 partials, node adjoints, rollout checkpoints and replay
-  tess::Buffer adj_partials, adj_nodes, ro_checkpoints, ro_segment, ro_gradv;
+  tess::Buffer adj_partials, adj_nodes, ro_checkpoints, ro_segment, ro_gradv, ro_snapshots;
   tess::Rollout rollout;
-  // layer staging through LDS in the single-scene transfers (FP32, Julia
-  // layout): bit 0 P2G, bit 1 G2P.  Set from TESS_LAYER_STAGING (0-3) when the
-  // context is created; both settings give bitwise identical results.
-  int staging = 0;
 };
 
 // A batch of scenes (batch.hip): the scenes as given, where each scene's tiles
@@ -142,6 +142,10 @@ struct tess_batch {
   tess::Buffer recs, d_toff, d_poff, prev_list;
   int64_t prev_nb = 0;
   bool zeroed = false;
+  // the binning in the context is this batch's, and the last G2P wrote every
+  // particle's next key (stepping within one call)
+  bool binned = false, keys_ready = false;
+  int64_t kept_binnings = 0;
   int variant = 0;  // bit 0: P2G one wave per last-axis offset (2D), bit 1: G2P with 2 waves instead of 4
 };
 

@@ -86,8 +86,32 @@ struct Pow<N, 0> {
 // misplaced-particle count of the last P2G (zeroed per P2G)
 enum {
   CNT_INSIDE = 0, CNT_ACTIVE = 1, CNT_NB = 2, CNT_ERR = 3, CNT_BIN_SLOTS = 4,
-  CNT_MISPLACED = 8, CNT_NODES = 9, CNT_STAY = 10, CNT_P2GERR = 11, CNT_MOVERS = 12, CNT_SLOTS = 16
+  CNT_MISPLACED = 8, CNT_NODES = 9, CNT_STAY = 10, CNT_P2GERR = 11, CNT_SLOTS = 16
 };
+// G2P counts the particles whose key changed into MOVER_LANES counters, one per
+// 128-byte line, chosen by workgroup (no single contended address)
+constexpr int MOVER_LANES = 8, MOVER_STRIDE = 32;
+
+inline uint32_t* movers_counters(tess_ctx* ctx) {
+  return ctx->movers.as<uint32_t>(size_t(MOVER_LANES * MOVER_STRIDE));
+}
+// zero them (before a G2P that counts)
+inline void movers_zero(tess_ctx* ctx) {
+  TESS_HIP(hipMemsetAsync(movers_counters(ctx), 0, MOVER_LANES * MOVER_STRIDE * sizeof(uint32_t), ctx->stream));
+}
+// enqueue their readback into the pinned slots after the counters
+inline void movers_readback(tess_ctx* ctx) {
+  uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host) + 64;
+  TESS_HIP(hipMemcpyAsync(hc, movers_counters(ctx), MOVER_LANES * MOVER_STRIDE * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, ctx->stream));
+}
+// the sum, after a synchronisation that followed movers_readback
+inline int64_t movers_total(tess_ctx* ctx) {
+  const uint32_t* hc = reinterpret_cast<const uint32_t*>(ctx->counters_host) + 64;
+  int64_t t = 0;
+  for (int i = 0; i < MOVER_LANES; ++i) t += hc[i * MOVER_STRIDE];
+  return t;
+}
 enum { ERR_GIMP_LENGTH = 1, ERR_WINDOW = 2, ERR_INACTIVE = 4 };
 
 template <typename T, int D>
@@ -626,44 +650,6 @@ __device__ __forceinline__ void axis_jets(T xd, int lo_local, const T* s_nodex, 
 
 __device__ __forceinline__ uint64_t lanemask_lt64() { return (uint64_t(1) << (threadIdx.x & 63)) - 1; }
 
-// One field of a contiguous layer: n ≤ 64 particles of NC scalars each, stored
-// at src[base*NC ...] as one run of n*NC scalars.  NT threads load the run with
-// consecutive addresses into registers, then park it in LDS rows (particle
-// rank r, columns OFF .. OFF+NC-1).  NC is a compile-time constant, so the
-// element -> (row, column) split is a multiply-shift.
-template <typename T, int NC, int NT>
-struct LayerRun {
-  static constexpr int R = (64 * NC + NT - 1) / NT;
-  T r[R];
-  __device__ __forceinline__ void load(const T* __restrict__ src, uint32_t base, int n, int tid) {
-    const T* p = src + int64_t(base) * NC;
-#pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const int i = tid + m * NT;
-      if (i < n * NC) r[m] = p[i];
-    }
-  }
-  template <int OFF, int NP>
-  __device__ __forceinline__ void park(T (*rows)[NP], int n, int tid) const {
-#pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const int i = tid + m * NT;
-      if (i < n * NC) rows[i / NC][OFF + i % NC] = r[m];
-    }
-  }
-};
-
-// The reverse: LDS rows (columns OFF ..) of n particles stored as one run.
-template <typename T, int NC, int OFF, int NP>
-__device__ __forceinline__ void store_run(T* __restrict__ dst, uint32_t base, int n, const T (*rows)[NP], int lane) {
-  T* p = dst + int64_t(base) * NC;
-#pragma unroll
-  for (int m = 0; m < NC; ++m) {
-    const int i = lane + 64 * m;
-    if (i < n * NC) p[i] = rows[i / NC][OFF + i % NC];
-  }
-}
-
 // Per-class counts of one tile from its slice of the sorted keys (class = low 6
 // bits), by wave-level matching; lane c gets count and offset of class c.
 template <int NW = 1>
@@ -979,25 +965,16 @@ __device__ __forceinline__ void p2g_load(const Args<T, D>& a, uint32_t pid, P2GI
 // the registers of one wave hold 1/NPASS of the stencil, which keeps several
 // waves per SIMD resident).  The waves add their stencils into the LDS tile
 // one after another, so the per-node order of the additions is fixed.
-// layers of particles the unstaged P2G loads ahead of the one it accumulates
-#ifndef TESS_P2G_AHEAD
-#define TESS_P2G_AHEAD 1
-#endif
-// wait for each offset's LDS writes before the next offset's reads (not
-// needed for correctness: the LDS executes one wave's accesses in order)
-#ifndef TESS_P2G_WRITE_WAIT
-#define TESS_P2G_WRITE_WAIT 0
-#endif
-template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false, bool STG = true>
+template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false>
 __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                           uint32_t* __restrict__ misplaced) {
   using G = Geo<D>;
+  using PT = Part<D, support_width(BASIS)>;
   constexpr int SW = support_width(BASIS);
   constexpr int NV = 1 + 2 * D;
   constexpr int KN = SW / NPASS;                       // last-axis offsets per pass
   constexpr int NSL = Pow<SW, D - 1>::v * KN;          // offsets per pass
   constexpr int NT = 64 * NPASS;
-  constexpr int P2G_AHEAD = TESS_P2G_AHEAD;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform: scalar branches
   const int64_t j = xcd_remap(blockIdx.x, a.n_active);
@@ -1014,20 +991,15 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
 
   // the tile's node sums m, mv[D], f[D], one array per component over the
   // padded footprint (bank-conflict-free for the class lanes)
-  __shared__ T s_tile[NV][G::PSZ];
+  __shared__ __align__(16) T s_tile[NV][G::PSZ];
   __shared__ T s_nodex[D][G::TS];
   __shared__ uint32_t s_cnt[64];
-  // Layer k (the k-th particle of every class) is stored contiguously in the
-  // layered order: the whole workgroup loads it with coalesced runs into one of
-  // two LDS buffers while the waves work on the other.
-  constexpr bool PSTAGE = STG && AOS && sizeof(T) == 4;
-  constexpr int NIN = 2 * D + 1 + D * D;  // x, v, V, σ
-  constexpr int NIP = NIN | 1;
-  __shared__ T s_in[PSTAGE ? 2 : 1][PSTAGE ? 64 : 1][PSTAGE ? NIP : 1];
   // per-layer axis jets shared by the waves (NPASS > 1): [layer parity][axis][val SW, der SW][lane]
   constexpr bool SHJ = NPASS > 1;
   __shared__ T s_jet[SHJ ? 2 : 1][SHJ ? D : 1][SHJ ? 2 * SW : 1][SHJ ? 64 : 1];
-  for (int q = int(threadIdx.x); q < NV * G::PSZ; q += NT) (&s_tile[0][0])[q] = T(0);
+  static_assert(NV * G::PSZ * sizeof(T) % 16 == 0, "LDS tile in 16-byte words");
+  for (int q = int(threadIdx.x); q < int(NV * G::PSZ * sizeof(T) / 16); q += NT)
+    reinterpret_cast<uint4*>(&s_tile[0][0])[q] = make_uint4(0u, 0u, 0u, 0u);
   for (int q = int(threadIdx.x); q < D * G::TS; q += NT) {
     const int d = q / G::TS, k = q % G::TS;
     s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
@@ -1058,257 +1030,172 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
         af[c][o] = T(0);
       }
     }
-    P2GIn<T, D> cur, nxt, nxt2;
-    // lanes past their class's count compute on these zeros with zero mass and
-    // stress factors (finite weights, exact zero contributions)
-    std::memset(static_cast<void*>(&cur), 0, sizeof(cur));
-    std::memset(static_cast<void*>(&nxt), 0, sizeof(nxt));
-    std::memset(static_cast<void*>(&nxt2), 0, sizeof(nxt2));
-    // layer description, identical in every wave: active lanes, first storage
-    // slot, whether the active lanes' slots are base, base+1, ...
-    auto layer = [&](int k, uint64_t& has, uint32_t& base, bool& contig) {
-      const bool act = k < ccount;
-      has = __ballot(act);
-      const uint32_t pid = act ? a.perm[start + coff + k] - poff : 0u;
-      const int rank = __popcll(has & lanemask_lt64());
-      const int first = has ? __ffsll((unsigned long long)has) - 1 : 0;
-      base = uint32_t(__shfl(int(pid) - rank, first, 64));
-      contig = has != 0 && __ballot(act && pid != base + uint32_t(rank)) == 0;
-    };
-    // the runs of one layer: x (nD), v (nD), V (n), σ (nD²)
-    constexpr int SR = PSTAGE ? 1 : 0;  // register arrays collapse when not staged
-    LayerRun<T, SR ? D : 1, NT> rx, rv;
-    LayerRun<T, 1, NT> rV;
-    LayerRun<T, SR ? D * D : 1, NT> rs;
-    const int tid = int(threadIdx.x);
-    auto issue = [&](uint32_t base, int n) {
-      rx.load(a.x.p, base, n, tid);
-      rv.load(a.v.p, base, n, tid);
-      rV.load(a.V.p, base, n, tid);
-      rs.load(a.sigma.p, base, n, tid);
-    };
-    auto park = [&](T (*rows)[PSTAGE ? NIP : 1], int n) {
-      if constexpr (PSTAGE) {
-        rx.template park<0, NIP>(rows, n, tid);
-        rv.template park<D, NIP>(rows, n, tid);
-        rV.template park<2 * D, NIP>(rows, n, tid);
-        rs.template park<2 * D + 1, NIP>(rows, n, tid);
-      }
-    };
-    uint64_t has_n = 0;
-    uint32_t base_n = 0;
-    bool contig_n = false;
-    if constexpr (PSTAGE) {
-      layer(0, has_n, base_n, contig_n);
-      if (contig_n) {
-        issue(base_n, __popcll(has_n));
-        park(s_in[0], __popcll(has_n));
-      }
-      __syncthreads();
-    } else {
-      if (ccount > 0) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff] - poff, nxt);
-      if constexpr (P2G_AHEAD == 2) {
-        if (ccount > 1) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + 1] - poff, nxt2);
-      }
-    }
-    for (int k = 0; k < maxcnt; ++k) {
+    // two register sets that alternate between the layer being accumulated and
+    // the one being loaded.  Lanes past their class's count compute on the
+    // finite data they hold (zeros, or an earlier particle) with zero mass and
+    // stress factors, so their contributions are exact zeros.
+    P2GIn<T, D> pa, pb;
+    std::memset(static_cast<void*>(&pa), 0, sizeof(pa));
+    std::memset(static_cast<void*>(&pb), 0, sizeof(pb));
+    if (ccount > 0) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff] - poff, pa);
+    // one layer: the accumulation runs on every lane (a branch around it made
+    // the compiler copy all accumulators at each iteration)
+    auto layer = [&](int k, const P2GIn<T, D>& cur, P2GIn<T, D>& nxt) {
+      if (k + 1 < ccount) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + 1] - poff, nxt);
       const uint64_t has = __ballot(k < ccount);
-      bool contig_k = false, contig_next = false;
-      if constexpr (PSTAGE) {
-        contig_k = contig_n;
-        // issue the next layer's loads before this layer's arithmetic
-        layer(k + 1, has_n, base_n, contig_n);
-        contig_next = contig_n;
-        if (contig_next) issue(base_n, __popcll(has_n));
-        if (k < ccount) {
-          if (contig_k) {
-            const T* r = s_in[PSTAGE ? (k & 1) : 0][PSTAGE ? __popcll(has & lanemask_lt64()) : 0];
-            cur.pid = a.perm[start + coff + k] - poff;
+      const bool act = k < ccount;
+      if (misplaced && pass == 0 && act) {
+        const uint32_t want = start + layer_off + uint32_t(__popcll(has & lanemask_lt64()));
+        bad += cur.pid != want ? 1u : 0u;
+      }
+      T x[D], mpv[D], A[D * D];
+      const T mpe = act ? mp : T(0);
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-              cur.x[d] = r[d];
-              cur.v[d] = r[D + d];
-            }
-            cur.V = r[2 * D];
+      for (int d = 0; d < D; ++d) x[d] = cur.x[d];
 #pragma unroll
-            for (int q = 0; q < D * D; ++q) cur.S[q] = r[2 * D + 1 + q];
-            cur.l = T(0);
-            if constexpr (BASIS == TESS_UGIMP) cur.l = a.l(cur.pid) / a.mesh.h;
-          } else {
-            p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k] - poff, cur);
-          }
+      for (int d = 0; d < D; ++d) mpv[d] = mpe * cur.v[d];  // hoisted mₚ*v[p]
+      const T negV = act ? -cur.V : T(0);
+#pragma unroll
+      for (int q = 0; q < D * D; ++q) A[q] = negV * cur.S[q];  // hoisted (-V[p])*σ[p]
+      const T l = cur.l;
+      // jets of axes 0..D-2 at all SW offsets, of the last axis at this
+      // wave's KN offsets (vl, dl)
+      T val[D][SW], der[D][SW], vl[KN], dl[KN];
+      auto axis_jets = [&](auto dc, T* jv, T* jd) {
+        constexpr int d = decltype(dc)::value;
+        const T xi = (x[d] - a.mesh.xmin[d]) * hinv;
+        int lo, hi;
+        axis_window<T, BASIS>(xi, l, a.mesh.dims[d], lo, hi);
+        axis_jets_mixed<T, BASIS>(x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], l, hinv, jv, jd);
+      };
+      if constexpr (NPASS == 1) {
+        axis_jets(std::integral_constant<int, 0>{}, val[0], der[0]);
+        axis_jets(std::integral_constant<int, 1>{}, val[1], der[1]);
+        if constexpr (D == 3) axis_jets(std::integral_constant<int, D - 1>{}, val[D - 1], der[D - 1]);
+#pragma unroll
+        for (int q = 0; q < KN; ++q) {
+          vl[q] = val[D - 1][q];
+          dl[q] = der[D - 1][q];
         }
       } else {
-        cur = nxt;
-        if constexpr (P2G_AHEAD == 2) {
-          nxt = nxt2;
-          if (k + 2 < ccount) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + 2] - poff, nxt2);
-        } else {
-          if (k + 1 < ccount) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + 1] - poff, nxt);
+        // the waves hold the same particles: wave d % NPASS evaluates axis d
+        // and shares it through LDS (double-buffered by layer, one barrier)
+        const int buf = k & 1;
+        auto share = [&](auto dc) {
+          T jv[SW], jd[SW];
+          axis_jets(dc, jv, jd);
+#pragma unroll
+          for (int q = 0; q < SW; ++q) {
+            s_jet[buf][decltype(dc)::value][q][lane] = jv[q];
+            s_jet[buf][decltype(dc)::value][SW + q][lane] = jd[q];
+          }
+        };
+        if (wave == 0 % NPASS) share(std::integral_constant<int, 0>{});
+        if (wave == 1 % NPASS) share(std::integral_constant<int, 1>{});
+        if constexpr (D == 3) {
+          if (wave == 2 % NPASS) share(std::integral_constant<int, D - 1>{});
+        }
+        __syncthreads();
+#pragma unroll
+        for (int d = 0; d < D - 1; ++d)
+#pragma unroll
+          for (int q = 0; q < SW; ++q) {
+            val[d][q] = s_jet[buf][d][q][lane];
+            der[d][q] = s_jet[buf][d][SW + q][lane];
+          }
+#pragma unroll
+        for (int q = 0; q < KN; ++q) {
+          vl[q] = s_jet[buf][D - 1][pass * KN + q][lane];
+          dl[q] = s_jet[buf][D - 1][SW + pass * KN + q][lane];
         }
       }
-      // the accumulation runs on every lane: a branch around it made the
-      // compiler copy all accumulators at each iteration
-      {
-        const bool act = k < ccount;
-        const uint32_t pid = cur.pid;
-        if (misplaced && pass == 0 && act) {
-          const uint32_t want = start + layer_off + uint32_t(__popcll(has & lanemask_lt64()));
-          bad += pid != want ? 1u : 0u;
-        }
-        T x[D], mpv[D], A[D * D];
-        const T mpe = act ? mp : T(0);
+      if constexpr (D == 3) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) x[d] = cur.x[d];
+        for (int kz = 0; kz < KN; ++kz) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) mpv[d] = mpe * cur.v[d];  // hoisted mₚ*v[p]
-        const T negV = act ? -cur.V : T(0);
+          for (int jy = 0; jy < SW; ++jy) {
+            const T YZ = val[1][jy] * vl[kz];
+            const T YdZ = der[1][jy] * vl[kz];
+            const T YZd = val[1][jy] * dl[kz];
+            const T cm = YZ * mpe;
+            T cmv[3], U[3], Wv[3];
 #pragma unroll
-        for (int q = 0; q < D * D; ++q) A[q] = negV * cur.S[q];  // hoisted (-V[p])*σ[p]
-        const T l = cur.l;
-        // jets of axes 0..D-2 at all SW offsets, of the last axis at this
-        // wave's KN offsets (vl, dl)
-        T val[D][SW], der[D][SW], vl[KN], dl[KN];
-        auto axis_jets = [&](auto dc, T* jv, T* jd) {
-          constexpr int d = decltype(dc)::value;
-          const T xi = (x[d] - a.mesh.xmin[d]) * hinv;
-          int lo, hi;
-          axis_window<T, BASIS>(xi, l, a.mesh.dims[d], lo, hi);
-          axis_jets_mixed<T, BASIS>(x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], l, hinv, jv, jd);
-        };
-        if constexpr (NPASS == 1) {
-          axis_jets(std::integral_constant<int, 0>{}, val[0], der[0]);
-          axis_jets(std::integral_constant<int, 1>{}, val[1], der[1]);
-          if constexpr (D == 3) axis_jets(std::integral_constant<int, D - 1>{}, val[D - 1], der[D - 1]);
-#pragma unroll
-          for (int q = 0; q < KN; ++q) {
-            vl[q] = val[D - 1][q];
-            dl[q] = der[D - 1][q];
-          }
-        } else {
-          // the waves hold the same particles: wave d % NPASS evaluates axis d
-          // and shares it through LDS (double-buffered by layer, one barrier)
-          const int buf = k & 1;
-          auto share = [&](auto dc) {
-            T jv[SW], jd[SW];
-            axis_jets(dc, jv, jd);
-#pragma unroll
-            for (int q = 0; q < SW; ++q) {
-              s_jet[buf][decltype(dc)::value][q][lane] = jv[q];
-              s_jet[buf][decltype(dc)::value][SW + q][lane] = jd[q];
-            }
-          };
-          if (wave == 0 % NPASS) share(std::integral_constant<int, 0>{});
-          if (wave == 1 % NPASS) share(std::integral_constant<int, 1>{});
-          if constexpr (D == 3) {
-            if (wave == 2 % NPASS) share(std::integral_constant<int, D - 1>{});
-          }
-          __syncthreads();
-#pragma unroll
-          for (int d = 0; d < D - 1; ++d)
-#pragma unroll
-            for (int q = 0; q < SW; ++q) {
-              val[d][q] = s_jet[buf][d][q][lane];
-              der[d][q] = s_jet[buf][d][SW + q][lane];
-            }
-#pragma unroll
-          for (int q = 0; q < KN; ++q) {
-            vl[q] = s_jet[buf][D - 1][pass * KN + q][lane];
-            dl[q] = s_jet[buf][D - 1][SW + pass * KN + q][lane];
-          }
-        }
-        if constexpr (D == 3) {
-#pragma unroll
-          for (int kz = 0; kz < KN; ++kz) {
-#pragma unroll
-            for (int jy = 0; jy < SW; ++jy) {
-              const T YZ = val[1][jy] * vl[kz];
-              const T YdZ = der[1][jy] * vl[kz];
-              const T YZd = val[1][jy] * dl[kz];
-              const T cm = YZ * mpe;
-              T cmv[3], U[3], Wv[3];
-#pragma unroll
-              for (int c = 0; c < 3; ++c) {
-                cmv[c] = YZ * mpv[c];
-                U[c] = A[c] * YZ;
-                Wv[c] = A[c + 3] * YdZ + A[c + 6] * YZd;
-              }
-#pragma unroll
-              for (int ix = 0; ix < SW; ++ix) {
-                const int o = ix + SW * (jy + SW * kz);
-                am[o] += val[0][ix] * cm;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                  amv[c][o] += val[0][ix] * cmv[c];
-                  af[c][o] = fma(val[0][ix], Wv[c], fma(der[0][ix], U[c], af[c][o]));
-                }
-              }
-            }
-          }
-        } else {
-#pragma unroll
-          for (int jy = 0; jy < KN; ++jy) {
-            const T Y = vl[jy], Yd = dl[jy];
-            const T cm = Y * mpe;
-            T cmv[2], U[2], Wv[2];
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-              cmv[c] = Y * mpv[c];
-              U[c] = A[c] * Y;
-              Wv[c] = A[c + 2] * Yd;
+            for (int c = 0; c < 3; ++c) {
+              cmv[c] = YZ * mpv[c];
+              U[c] = A[c] * YZ;
+              Wv[c] = A[c + 3] * YdZ + A[c + 6] * YZd;
             }
 #pragma unroll
             for (int ix = 0; ix < SW; ++ix) {
-              const int o = ix + SW * jy;
+              const int o = ix + SW * (jy + SW * kz);
               am[o] += val[0][ix] * cm;
 #pragma unroll
-              for (int c = 0; c < 2; ++c) {
+              for (int c = 0; c < 3; ++c) {
                 amv[c][o] += val[0][ix] * cmv[c];
                 af[c][o] = fma(val[0][ix], Wv[c], fma(der[0][ix], U[c], af[c][o]));
               }
             }
           }
         }
+      } else {
+#pragma unroll
+        for (int jy = 0; jy < KN; ++jy) {
+          const T Y = vl[jy], Yd = dl[jy];
+          const T cm = Y * mpe;
+          T cmv[2], U[2], Wv[2];
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            cmv[c] = Y * mpv[c];
+            U[c] = A[c] * Y;
+            Wv[c] = A[c + 2] * Yd;
+          }
+#pragma unroll
+          for (int ix = 0; ix < SW; ++ix) {
+            const int o = ix + SW * jy;
+            am[o] += val[0][ix] * cm;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+              amv[c][o] += val[0][ix] * cmv[c];
+              af[c][o] = fma(val[0][ix], Wv[c], fma(der[0][ix], U[c], af[c][o]));
+            }
+          }
+        }
       }
       if (pass == 0) layer_off += uint32_t(__popcll(has));
-      if constexpr (PSTAGE) {
-        if (contig_next) park(s_in[(k + 1) & 1], __popcll(has_n));
-        __syncthreads();  // next layer parked; this layer's buffer free again
-      }
+    };
+    for (int k = 0; k < maxcnt; k += 2) {
+      layer(k, pa, pb);
+      if (k + 1 < maxcnt) layer(k + 1, pb, pa);
     }
     // lanes are distinct classes, so for one offset their target nodes are
-    // distinct; offsets are applied one after another (read, add, write, wait
-    // for the LDS queue) and the waves one after another, which fixes the
-    // per-node order of the additions
+    // distinct; offsets are applied one after another (read, add, write: the
+    // LDS executes one wave's accesses in order) and the waves one after
+    // another, which fixes the per-node order of the additions
     for (int w = 0; w < NPASS; ++w) {
       if (w == wave && ccount > 0) {
 #pragma unroll
-      for (int o = 0; o < NSL; ++o) {
-        int r = o, nI[3] = {0, 0, 0};
+        for (int o = 0; o < NSL; ++o) {
+          int r = o, nI[3] = {0, 0, 0};
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const int off = (d == D - 1) ? pass * KN + r : r % SW;
-          nI[d] = ccoord[d] + off;
-          r /= SW;
+          for (int d = 0; d < D; ++d) {
+            const int off = (d == D - 1) ? pass * KN + r : r % SW;
+            nI[d] = ccoord[d] + off;
+            r /= SW;
+          }
+          const int q = pad_node<D>(nI[0], nI[1], nI[2]);
+          T t[NV];
+#pragma unroll
+          for (int v = 0; v < NV; ++v) t[v] = s_tile[v][q];
+          t[0] += am[o];
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            t[1 + c] += amv[c][o];
+            t[1 + D + c] += af[c][o];
+          }
+#pragma unroll
+          for (int v = 0; v < NV; ++v) s_tile[v][q] = t[v];
+          __asm__ volatile("" ::: "memory");  // keep the offsets in program order
         }
-        const int q = pad_node<D>(nI[0], nI[1], nI[2]);
-        T t[NV];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) t[v] = s_tile[v][q];
-        t[0] += am[o];
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-          t[1 + c] += amv[c][o];
-          t[1 + D + c] += af[c][o];
-        }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) s_tile[v][q] = t[v];
-#if TESS_P2G_WRITE_WAIT
-        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-#else
-        __asm__ volatile("" ::: "memory");  // keep the offsets in order (LDS executes a wave's accesses in order)
-#endif
-      }
       }
       __syncthreads();
     }
@@ -1318,12 +1205,13 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
     for (int off = 32; off >= 1; off >>= 1) bad += __shfl_xor(bad, off, 64);
     if (lane == 0) misplaced[j] = bad;  // per tile, summed later
   }
-  using PT = Part<D, SW>;
+  // the partial: one thread per node writes its NV values
   T* out = a.partials + int64_t(j) * PT::SIZE;
-  for (int e = int(threadIdx.x); e < PT::SIZE; e += NT) {
-    const int n = e / NV, v = e % NV;
+  for (int n = int(threadIdx.x); n < PT::NODES; n += NT) {
     const int q = pad_node<D>(n % PT::E, (n / PT::E) % PT::E, D == 3 ? n / (PT::E * PT::E) : 0);
-    out[e] = s_tile[v][q];
+    T* o = out + int64_t(n) * NV;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) o[v] = s_tile[v][q];
   }
 }
 
@@ -1353,11 +1241,15 @@ __device__ __forceinline__ void g2p_load(const Args<T, D>& a, uint32_t pid, G2PI
 // With nk != nullptr the kernel also writes, at each particle's sorted position
 // j, the sort key of its new position (nk[j]) and whether it equals the
 // current one (stay[j]): the next step's incremental binning starts from them.
-// waves per SIMD the Float32 G2P is compiled for (VGPR budget 512 / n)
+// waves per SIMD the Float32 G2P is compiled for (VGPR budget 512 / n), and
+// waves per tile of the single-scene G2P
 #ifndef TESS_G2P_WAVES_PER_SIMD
 #define TESS_G2P_WAVES_PER_SIMD 3
 #endif
-template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false, bool STG = true>
+#ifndef TESS_G2P_WAVES_PER_TILE
+#define TESS_G2P_WAVES_PER_TILE 4
+#endif
+template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false>
 __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD : 1) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                        uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
                                                        uint32_t ntiles, uint32_t* __restrict__ movers) {
@@ -1381,12 +1273,6 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
   __shared__ T s_n[2 * D][G::PSZ];
   __shared__ T s_nodex[D][G::TS];
   __shared__ uint32_t s_cnt[64];
-  // per wave: the outputs of its 64 lanes, written back as contiguous runs when
-  // the wave's particles are stored contiguously (the layered order)
-  constexpr bool STAGE = STG && AOS && sizeof(T) == 4;
-  constexpr int NOUT = 2 * D + 3 * D * D + 1;  // v, ∇v, x, F, σ, V
-  constexpr int NOP = NOUT | 1;                // odd row stride: lanes hit distinct banks
-  __shared__ T s_out[STAGE ? NW : 1][STAGE ? 64 : 1][STAGE ? NOP : 1];
   for (int q = int(threadIdx.x); q < G::NFOOT; q += 64 * NW) {
     int I[D], r = q;
     bool in = true;
@@ -1423,86 +1309,16 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
     }
   }
   const T hinv = a.mesh.h_inv, dt = a.dt;
-  G2PIn<T, D, AOS> cur, nxt;
-  std::memset(static_cast<void*>(&cur), 0, sizeof(cur));  // lanes without particles compute on zeros
-  std::memset(static_cast<void*>(&nxt), 0, sizeof(nxt));
-  // layer k of the wave: active lanes, first storage slot, and whether the
-  // active lanes' slots are base, base+1, ... (the layered order)
-  auto layer = [&](int k, uint64_t& has, uint32_t& base, bool& contig) {
+  // two register sets alternate between the particle being updated and the
+  // one being loaded (wave w walks k = w, w+NW, ...)
+  G2PIn<T, D, AOS> pa, pb;
+  std::memset(static_cast<void*>(&pa), 0, sizeof(pa));
+  std::memset(static_cast<void*>(&pb), 0, sizeof(pb));
+  if (wave < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + wave] - poff, pa);
+  auto particle = [&](int k, const G2PIn<T, D, AOS>& cur, G2PIn<T, D, AOS>& nxt) {
     const bool act = k < ccount;
-    has = __ballot(act);
-    const uint32_t pid = act ? a.perm[start + coff + k] - poff : 0u;
-    const int rank = __popcll(has & lanemask_lt64());
-    const int first = has ? __ffsll((unsigned long long)has) - 1 : 0;
-    base = uint32_t(__shfl(int(pid) - rank, first, 64));
-    contig = has != 0 && __ballot(act && pid != base + uint32_t(rank)) == 0;
-  };
-  // a staged wave loads a contiguous layer as runs of x (nD), v (nD), F (nD²)
-  constexpr int SR = STAGE ? 1 : 0;  // register arrays collapse when not staged
-  LayerRun<T, SR ? D : 1, 64> rx, rv;
-  LayerRun<T, SR ? D * D : 1, 64> rF;
-  uint64_t has_n = 0;
-  uint32_t base_n = 0;
-  bool contig_n = false;
-  auto issue = [&]() {
-    const int n = __popcll(has_n);
-    rx.load(a.x.p, base_n, n, lane);
-    rv.load(a.v.p, base_n, n, lane);
-    rF.load(a.F.p, base_n, n, lane);
-  };
-  if constexpr (STAGE) {
-    layer(wave, has_n, base_n, contig_n);
-    if (contig_n) issue();
-  } else {
-    if (wave < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + wave] - poff, nxt);
-  }
-  for (int k = wave; k < maxcnt; k += NW) {
-    const bool act = k < ccount;
-    const uint64_t has = __ballot(act);
-    const int rank = __popcll(has & lanemask_lt64());
-    uint32_t base = 0;
-    bool contiguous = false;
-    if constexpr (STAGE) {
-      base = base_n;
-      contiguous = contig_n;
-      if (contiguous) {
-        // park the prefetched runs by particle rank, then read the lane's own row
-        const int n = __popcll(has);
-        T(*sw)[NOP] = s_out[STAGE ? wave : 0];
-        if constexpr (STAGE) {
-          rx.template park<0, NOP>(sw, n, lane);
-          rv.template park<D, NOP>(sw, n, lane);
-          rF.template park<2 * D, NOP>(sw, n, lane);
-        }
-        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        if (act) {
-          const T* r = sw[rank];
-          cur.pid = base + uint32_t(rank);
-#pragma unroll
-          for (int c = 0; c < D; ++c) {
-            cur.x[c] = r[c];
-            cur.v[c] = r[D + c];
-          }
-#pragma unroll
-          for (int q = 0; q < D * D; ++q) cur.F[q] = r[2 * D + q];
-          cur.l = T(0);
-          if constexpr (BASIS == TESS_UGIMP) cur.l = a.l(cur.pid) / a.mesh.h;
-        }
-      } else if (act) {
-        // a layer out of the layered order: each lane loads its own particle
-        g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k] - poff, cur);
-      }
-      // the next layer of this wave: coalesced runs in flight during the arithmetic
-      layer(k + NW, has_n, base_n, contig_n);
-      if (contig_n) issue();
-    } else {
-      cur = nxt;
-      if (k + NW < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + NW] - poff, nxt);
-      const int first = __ffsll((unsigned long long)has) - 1;
-      base = uint32_t(__shfl(int(cur.pid) - rank, first, 64));
-      contiguous = __ballot(act && cur.pid != base + uint32_t(rank)) == 0;
-    }
-    if (!STAGE && !act) continue;
+    if (k + NW < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + NW] - poff, nxt);
+    if (!act) return;
     T val[D][SW], der[D][SW];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -1580,50 +1396,32 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
       }
     }
     const uint32_t pid = cur.pid;
-    // particle equations (README.md:81-86).  Outputs go through one destination
-    // per field: the lane's LDS row in a staged wave (inactive lanes write the
-    // spare row 63, unused when a lane is inactive), else the particle itself.
-    DField<T> ov = a.v, og = a.gradv, ox = a.x, oF = a.F, os = a.sigma, oV = a.V;
-    int64_t oi = pid;
-    if constexpr (STAGE) {
-      // every lane parks its outputs in its LDS row (inactive lanes use the
-      // spare row 63, unused whenever a lane is inactive); the wave stores them below
-      T* row = s_out[STAGE ? wave : 0][STAGE ? (act ? rank : 63) : 0];
-      ov = DField<T>{row, D, 1};
-      og = DField<T>{row + D, D * D, 1};
-      ox = DField<T>{row + D + D * D, D, 1};
-      oF = DField<T>{row + 2 * D + D * D, D * D, 1};
-      os = DField<T>{row + 2 * D + 2 * D * D, D * D, 1};
-      oV = DField<T>{row + NOUT - 1, 1, 1};
-      oi = 0;
-    }
+    // particle equations (README.md:81-86)
 #pragma unroll
-    for (int c = 0; c < D; ++c) fldw<T, D, AOS>(ov, oi, c) = cur.v[c] + DV[c];
+    for (int c = 0; c < D; ++c) fldw<T, D, AOS>(a.v, pid, c) = cur.v[c] + DV[c];
 #pragma unroll
-    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(og, oi, q) = L[q];
+    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.gradv, pid, q) = L[q];
     T xn[D];
 #pragma unroll
     for (int c = 0; c < D; ++c) {
       xn[c] = cur.x[c] + X[c] * dt;
-      fldw<T, D, AOS>(ox, oi, c) = xn[c];
+      fldw<T, D, AOS>(a.x, pid, c) = xn[c];
     }
     if (nk) {
-      bool moved = false;
-      if (act) {
-        const uint32_t jj = start + uint32_t(coff + k);
-        uint32_t nl = 1;  // the scene's tiles; in a batch its keys start at its first global tile
+      const uint32_t jj = start + uint32_t(coff + k);
+      uint32_t nl = 1;  // the scene's tiles; in a batch its keys start at its first global tile
 #pragma unroll
-        for (int d = 0; d < D; ++d) nl *= uint32_t(a.ntiles[d]);
-        const uint32_t kl = key_of_position<T, D, BASIS>(a, xn, cur.l, nl);
-        const uint32_t key = kl == nl * 64u ? ntiles * 64u : kl + (tile - ltile) * 64u;
-        nk[jj] = key;
-        moved = key != skeys[jj];
-        stay[jj] = moved ? 0u : 1u;
-      }
+      for (int d = 0; d < D; ++d) nl *= uint32_t(a.ntiles[d]);
+      const uint32_t kl = key_of_position<T, D, BASIS>(a, xn, cur.l, nl);
+      const uint32_t key = kl == nl * 64u ? ntiles * 64u : kl + (tile - ltile) * 64u;
+      nk[jj] = key;
+      const bool moved = key != skeys[jj];
+      stay[jj] = moved ? 0u : 1u;
       // particles whose key changed, counted per wave by its lowest live lane
       // (no atomic in the common case of none)
       const uint64_t mv = __ballot(moved);
-      if (mv && (__ballot(true) & lanemask_lt64()) == 0) atomicAdd(movers, uint32_t(__popcll(mv)));
+      if (mv && (__ballot(true) & lanemask_lt64()) == 0)
+        atomicAdd(movers + (blockIdx.x % MOVER_LANES) * MOVER_STRIDE, uint32_t(__popcll(mv)));
     }
     T Fn[D * D];
 #pragma unroll
@@ -1636,44 +1434,16 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
         Fn[r + D * b] = cur.F[r + D * b] + dt * t;
       }
 #pragma unroll
-    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(oF, oi, q) = Fn[q];
-    {
-      const bool le = a.logE.p && act;
-      T sg[D * D], Vn;
-      neo_hookean<T, D>(Fn, a.lambda, a.mu, a.V0, le, le ? a.logE(pid) : T(0), sg, Vn);
+    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.F, pid, q) = Fn[q];
+    T sg[D * D], Vn;
+    neo_hookean<T, D>(Fn, a.lambda, a.mu, a.V0, a.logE.p != nullptr, a.logE.p ? a.logE(pid) : T(0), sg, Vn);
 #pragma unroll
-      for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(os, oi, q) = sg[q];
-      oV(oi) = Vn;
-    }
-    if constexpr (STAGE) {
-      __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-      const int nact = __popcll(has);
-      T(*sw)[NOP] = s_out[STAGE ? wave : 0];
-      if (contiguous) {
-        // each field as one contiguous run of nact*ncomp scalars
-        store_run<T, D, 0, NOP>(a.v.p, base, nact, sw, lane);
-        store_run<T, D * D, D, NOP>(a.gradv.p, base, nact, sw, lane);
-        store_run<T, D, D + D * D, NOP>(a.x.p, base, nact, sw, lane);
-        store_run<T, D * D, 2 * D + D * D, NOP>(a.F.p, base, nact, sw, lane);
-        store_run<T, D * D, 2 * D + 2 * D * D, NOP>(a.sigma.p, base, nact, sw, lane);
-        store_run<T, 1, NOUT - 1, NOP>(a.V.p, base, nact, sw, lane);
-      } else if (act) {
-        // scattered slots: each lane stores its own row
-        const T* r = sw[rank];
-#pragma unroll
-        for (int c = 0; c < D; ++c) a.v.p[int64_t(pid) * D + c] = r[c];
-#pragma unroll
-        for (int q = 0; q < D * D; ++q) a.gradv.p[int64_t(pid) * D * D + q] = r[D + q];
-#pragma unroll
-        for (int c = 0; c < D; ++c) a.x.p[int64_t(pid) * D + c] = r[D + D * D + c];
-#pragma unroll
-        for (int q = 0; q < D * D; ++q) a.F.p[int64_t(pid) * D * D + q] = r[2 * D + D * D + q];
-#pragma unroll
-        for (int q = 0; q < D * D; ++q) a.sigma.p[int64_t(pid) * D * D + q] = r[2 * D + 2 * D * D + q];
-        a.V.p[pid] = r[NOUT - 1];
-      }
-      __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    }
+    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.sigma, pid, q) = sg[q];
+    a.V(pid) = Vn;
+  };
+  for (int k = wave; k < maxcnt; k += 2 * NW) {
+    particle(k, pa, pb);
+    if (k + NW < maxcnt) particle(k + NW, pb, pa);
   }
 }
 
@@ -2073,6 +1843,38 @@ int key_bits_for(uint64_t maxkey) {
   return b;
 }
 
+// Incremental re-binning with M > 0 movers: nk[j] is the new key of the
+// particle at sorted position j, spos[j] its stayer rank (exclusive scan of the
+// stayer flags, spos[n] = stayers).  Few movers are sorted among themselves
+// and merged into the stayers' previous order; many are sorted with the
+// stayers.  The merged order replaces (keys_sorted, vals).
+inline void merge_movers(tess_ctx* ctx, const uint32_t* nk, const uint32_t* spos, int64_t n, int64_t M, int kbits) {
+  uint32_t* skeys = static_cast<uint32_t*>(ctx->keys_sorted.ptr);
+  uint32_t* perm = static_cast<uint32_t*>(ctx->vals.ptr);
+  uint32_t* skeys2 = ctx->skeys2.as<uint32_t>(size_t(n));
+  uint32_t* perm2 = ctx->perm2.as<uint32_t>(size_t(n));
+  if (M <= n / 8) {
+    uint32_t* mkey = ctx->inc_mkey.as<uint32_t>(size_t(M));
+    uint32_t* mj = ctx->inc_mj.as<uint32_t>(size_t(M));
+    uint32_t* mks = ctx->inc_mks.as<uint32_t>(size_t(M));
+    uint32_t* mr = ctx->inc_mr.as<uint32_t>(size_t(M));
+    uint32_t* mjs = ctx->inc_mjs.as<uint32_t>(size_t(M));
+    hipLaunchKernelGGL(k_compact_movers, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, nk, spos, n, mkey, mj);
+    TESS_HIP(hipGetLastError());
+    radix_sort_pairs(ctx, mkey, nullptr, mks, mr, M, kbits);
+    hipLaunchKernelGGL(k_place_movers, dim3(blocks_for(M, 256)), dim3(256), 0, ctx->stream, mks, mr, mj, skeys, perm,
+                       spos, n, M, mjs, skeys2, perm2);
+    TESS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_place_stayers, dim3(blocks_for(n, STAY_BLOCK)), dim3(STAY_BLOCK), 0, ctx->stream, skeys,
+                       perm, spos, n, mks, mjs, M, skeys2, perm2);
+    TESS_HIP(hipGetLastError());
+  } else {
+    radix_sort_pairs(ctx, nk, perm, skeys2, perm2, n, kbits);
+  }
+  std::swap(ctx->keys_sorted, ctx->skeys2);
+  std::swap(ctx->vals, ctx->perm2);
+}
+
 template <typename T, int D, int BASIS, bool STORED>
 void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const tess_weights* W,
                    bool keys_from_g2p = false) {
@@ -2109,8 +1911,9 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
     // previous binning are what binning again would produce: keep them.
     uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
     TESS_HIP(hipMemcpyAsync(hc, counters, CNT_SLOTS * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    movers_readback(ctx);
     TESS_HIP(hipStreamSynchronize(ctx->stream));
-    if (hc[CNT_MOVERS] == 0) {
+    if (movers_total(ctx) == 0) {
       ctx->last_movers = 0;
       ctx->last_active_nodes = int64_t(hc[CNT_NODES]);
       ctx->last_misplaced = (ctx->misplaced_x_ptr == P.x.ptr && ctx->misplaced_n == n) ? int64_t(hc[CNT_MISPLACED]) : -1;
@@ -2135,35 +1938,23 @@ void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P
     }
     exclusive_scan_u32(ctx, spos, n + 1, counters + CNT_STAY);
     uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
-    TESS_HIP(hipMemcpyAsync(hc + CNT_STAY, counters + CNT_STAY, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                            ctx->stream));
+    TESS_HIP(hipMemcpyAsync(hc, counters, CNT_SLOTS * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     TESS_HIP(hipStreamSynchronize(ctx->stream));
     const int64_t M = n - int64_t(hc[CNT_STAY]);
     ctx->last_movers = M;
+    if (M == 0 && b.tile_log2 == Geo<D>::LT && b.basis == BASIS && b.dtype == mesh.dtype &&
+        !(hc[CNT_ERR] & ERR_GIMP_LENGTH)) {
+      // no particle changed its key: the previous order and tile lists stand
+      ctx->last_active_nodes = int64_t(hc[CNT_NODES]);
+      ctx->last_misplaced = (ctx->misplaced_x_ptr == P.x.ptr && ctx->misplaced_n == n) ? int64_t(hc[CNT_MISPLACED]) : -1;
+      ctx->kept_binnings += 1;
+      b.x_ptr = P.x.ptr;
+      b.valid = true;
+      prof_end(ctx, tok);
+      return;
+    }
     if (M > 0) {
-      uint32_t* skeys2 = ctx->skeys2.as<uint32_t>(size_t(n));
-      uint32_t* perm2 = ctx->perm2.as<uint32_t>(size_t(n));
-      if (M <= n / 8) {
-        uint32_t* mkey = ctx->inc_mkey.as<uint32_t>(size_t(M));
-        uint32_t* mj = ctx->inc_mj.as<uint32_t>(size_t(M));
-        uint32_t* mks = ctx->inc_mks.as<uint32_t>(size_t(M));
-        uint32_t* mr = ctx->inc_mr.as<uint32_t>(size_t(M));
-        uint32_t* mjs = ctx->inc_mjs.as<uint32_t>(size_t(M));
-        hipLaunchKernelGGL(k_compact_movers, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, nk, spos, n, mkey,
-                           mj);
-        TESS_HIP(hipGetLastError());
-        radix_sort_pairs(ctx, mkey, nullptr, mks, mr, M, kbits);
-        hipLaunchKernelGGL(k_place_movers, dim3(blocks_for(M, 256)), dim3(256), 0, ctx->stream, mks, mr, mj, skeys,
-                           perm, spos, n, M, mjs, skeys2, perm2);
-        TESS_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_place_stayers, dim3(blocks_for(n, STAY_BLOCK)), dim3(STAY_BLOCK), 0, ctx->stream,
-                           skeys, perm, spos, n, mks, mjs, M, skeys2, perm2);
-        TESS_HIP(hipGetLastError());
-      } else {
-        radix_sort_pairs(ctx, nk, perm, skeys2, perm2, n, kbits);
-      }
-      std::swap(ctx->keys_sorted, ctx->skeys2);
-      std::swap(ctx->vals, ctx->perm2);
+      merge_movers(ctx, nk, spos, n, M, kbits);
       skeys = static_cast<uint32_t*>(ctx->keys_sorted.ptr);
       perm = static_cast<uint32_t*>(ctx->vals.ptr);
     }
@@ -2290,12 +2081,9 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     } else {
       // 3D: one wave per offset along z; 2D: one wave while the stencil fits
       constexpr int NPF = (D == 3 || Pow<SW, D>::v * NV * int(sizeof(T)) / 4 > 96) ? SW : 1;
-      if (julia_layout(P, D) && (ctx->staging & 1))
+      if (julia_layout(P, D))
         hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, true>), dim3(xcd_grid(b.n_active)), dim3(64 * NPF), 0,
                            ctx->stream, a, sk, misplaced);
-      else if (julia_layout(P, D))
-        hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, true, false, false>), dim3(xcd_grid(b.n_active)),
-                           dim3(64 * NPF), 0, ctx->stream, a, sk, misplaced);
       else
         hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, false>), dim3(xcd_grid(b.n_active)), dim3(64 * NPF), 0,
                            ctx->stream, a, sk, misplaced);
@@ -2356,8 +2144,8 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   if (write_keys && !STORED && P.n > 0) {
     nk = ctx->inc_nk.as<uint32_t>(size_t(P.n));
     stay = ctx->inc_spos.as<uint32_t>(size_t(P.n) + 1);
-    mv = ctx->counters.as<uint32_t>(CNT_SLOTS) + CNT_MOVERS;
-    TESS_HIP(hipMemsetAsync(mv, 0, sizeof(uint32_t), ctx->stream));
+    mv = movers_counters(ctx);
+    movers_zero(ctx);
   }
   const uint32_t* skeys_now = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
   if (b.n_active > 0) {
@@ -2372,14 +2160,11 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
       hipLaunchKernelGGL((k_g2p_tile<T, D, BASIS, STORED>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream, a,
                          sk);
     } else {
-      constexpr int NW = 4;
+      constexpr int NW = TESS_G2P_WAVES_PER_TILE;
       const uint32_t nt = uint32_t(b.ntiles);
-      if (julia_layout(P, D) && (ctx->staging & 2))
+      if (julia_layout(P, D))
         hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
                            ctx->stream, a, sk, nk, stay, nt, mv);
-      else if (julia_layout(P, D))
-        hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW, false, false>), dim3(xcd_grid(b.n_active)),
-                           dim3(64 * NW), 0, ctx->stream, a, sk, nk, stay, nt, mv);
       else
         hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, false, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
                            ctx->stream, a, sk, nk, stay, nt, mv);
