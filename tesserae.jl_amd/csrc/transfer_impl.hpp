@@ -1221,7 +1221,7 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
 // and prefetches its next particle while updating the current one.
 template <typename T, int D, bool AOS>
 struct G2PIn {
-  uint32_t pid;
+  uint32_t pid, cls;  // particle, window-start class in the tile
   T x[D], v[D], F[D * D], l;
 };
 
@@ -1299,34 +1299,49 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
   }
   int ccount, coff, maxcnt;
   tile_class_counts<NW>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);  // barrier after the staging
-  int ccoord[D];
-  {
-    int r = lane;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      ccoord[d] = r % G::BW;
-      r /= G::BW;
-    }
-  }
   const T hinv = a.mesh.h_inv, dt = a.dt;
+  // Two walks of the tile's particles.  Class order (lane = class, iteration k =
+  // the k-th particle of every class) reads the layered storage contiguously
+  // and takes maxcnt iterations.  Sorted order (iteration k = sorted positions
+  // 64k .. 64k+63) takes ceil(cnt/64); a tile whose classes are uneven or
+  // sparse uses it when that saves a third of the iterations.
+  const int n_sorted = (cnt + 63) / 64;
+  const bool by_sorted = 3 * n_sorted < 2 * maxcnt;
+  const int kend = by_sorted ? n_sorted : maxcnt;
+  auto live = [&](int k) { return by_sorted ? k * 64 + lane < cnt : k < ccount; };
+  auto slot = [&](int k) { return start + uint32_t(by_sorted ? k * 64 + lane : coff + k); };  // sorted position
+  auto fetch = [&](int k, G2PIn<T, D, AOS>& in) {
+    if (!live(k)) return;
+    const uint32_t jj = slot(k);
+    g2p_load<T, D, BASIS, AOS>(a, a.perm[jj] - poff, in);
+    in.cls = by_sorted ? skeys[jj] & 63u : uint32_t(lane);
+  };
   // two register sets alternate between the particle being updated and the
   // one being loaded (wave w walks k = w, w+NW, ...)
   G2PIn<T, D, AOS> pa, pb;
   std::memset(static_cast<void*>(&pa), 0, sizeof(pa));
   std::memset(static_cast<void*>(&pb), 0, sizeof(pb));
-  if (wave < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + wave] - poff, pa);
+  fetch(wave, pa);
   auto particle = [&](int k, const G2PIn<T, D, AOS>& cur, G2PIn<T, D, AOS>& nxt) {
-    const bool act = k < ccount;
-    if (k + NW < ccount) g2p_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + NW] - poff, nxt);
+    const bool act = live(k);
+    fetch(k + NW, nxt);
     if (!act) return;
+    int cc[D];  // the particle's class coordinates (its window's offset in the footprint)
+    {
+      uint32_t r = cur.cls;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        cc[d] = int(r % G::BW);
+        r /= G::BW;
+      }
+    }
     T val[D][SW], der[D][SW];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const T xi = (cur.x[d] - a.mesh.xmin[d]) * hinv;
       int lo, hi;
       axis_window<T, BASIS>(xi, cur.l, a.mesh.dims[d], lo, hi);
-      axis_jets_mixed<T, BASIS>(cur.x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], cur.l, hinv, val[d],
-                                der[d]);
+      axis_jets_mixed<T, BASIS>(cur.x[d], a.mesh.xmin[d], cc[d], hi - lo, s_nodex[d], cur.l, hinv, val[d], der[d]);
     }
     // X = Σ w v_i, DV = Σ w (v_i - vⁿ_i), L = Σ v_i ⊗ ∇w, factored by axis
     T X[D], DV[D], L[D * D];
@@ -1345,7 +1360,7 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
           T S1[3] = {T(0), T(0), T(0)}, S2[3] = {T(0), T(0), T(0)}, Sd[3] = {T(0), T(0), T(0)};
 #pragma unroll
           for (int ix = 0; ix < SW; ++ix) {
-            const int q = pad_node<3>(ccoord[0] + ix, ccoord[1] + jy, ccoord[2] + kz);
+            const int q = pad_node<3>(cc[0] + ix, cc[1] + jy, cc[2] + kz);
             T nd[6];
 #pragma unroll
             for (int u = 0; u < 6; ++u) nd[u] = s_n[u][q];
@@ -1374,7 +1389,7 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
         T S1[2] = {T(0), T(0)}, S2[2] = {T(0), T(0)}, Sd[2] = {T(0), T(0)};
 #pragma unroll
         for (int ix = 0; ix < SW; ++ix) {
-          const int q = pad_node<2>(ccoord[0] + ix, ccoord[1] + jy, 0);
+          const int q = pad_node<2>(cc[0] + ix, cc[1] + jy, 0);
           T nd[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) nd[u] = s_n[u][q];
@@ -1408,7 +1423,7 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
       fldw<T, D, AOS>(a.x, pid, c) = xn[c];
     }
     if (nk) {
-      const uint32_t jj = start + uint32_t(coff + k);
+      const uint32_t jj = slot(k);
       uint32_t nl = 1;  // the scene's tiles; in a batch its keys start at its first global tile
 #pragma unroll
       for (int d = 0; d < D; ++d) nl *= uint32_t(a.ntiles[d]);
@@ -1441,9 +1456,9 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
     for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.sigma, pid, q) = sg[q];
     a.V(pid) = Vn;
   };
-  for (int k = wave; k < maxcnt; k += 2 * NW) {
+  for (int k = wave; k < kend; k += 2 * NW) {
     particle(k, pa, pb);
-    if (k + NW < maxcnt) particle(k + NW, pb, pa);
+    if (k + NW < kend) particle(k + NW, pb, pa);
   }
 }
 
