@@ -185,6 +185,16 @@ tess_status tess_g2p(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const 
                      const tess_particles* particles, const tess_weights* weights,
                      const tess_params* params);
 
+/* @G2P2G of the README program (src/Transfer/g2p2g.jl:100-159): one pass
+ * doing the G2P half above, then the P2G half (m, mv, f of the updated v, V, σ)
+ * and the grid-node half of tess_p2g.  Both halves use the weights at the
+ * positions the particles have when the call starts: the macro binds them once
+ * and the P2G half does not rebind them after x is written
+ * (g2p2g.jl:107-114).  weights == NULL evaluates them inside the pass. */
+tess_status tess_g2p2g(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                       const tess_particles* particles, const tess_weights* weights,
+                       const tess_params* params);
+
 /* One README step with deferred weights: update!(wgts,…) + @P2G + @G2P
  * (README.md:66-87), repeated nsteps times on the context's stream.
  * reorder_threshold plays the role of the reference loop's

@@ -878,6 +878,39 @@ struct RunG2P {
     return g2p_seq<T, D>(basis, m, *G, *P, W, *prm);
   }
 };
+// @G2P2G of the README program (src/Transfer/g2p2g.jl:100-159): one set of
+// weights, evaluated at the positions before the G2P half, serves both halves
+// (the P2G half does not rebind them, g2p2g.jl:107-114).  The G2P half updates
+// every particle; the P2G half zeroes m, mv, f and scatters the updated v, V,
+// σ with those weights in particle order; the grid-node half follows.  That is
+// the reference test's manual loop (test/transfer.jl:267-314): G2P, then P2G
+// with the same weights.  Deferred (W == NULL) evaluates the weights into
+// scratch first.
+template <typename T, int D>
+struct RunG2P2G {
+  static int run(const tess_mesh& m, int basis, const tess_grid* G, const tess_particles* P, const tess_weights* W,
+                 const tess_params* prm) {
+    std::vector<T> w, gw;
+    std::vector<int64_t> win;
+    tess_weights own;
+    if (!W) {
+      int64_t nsl = 1;
+      for (int d = 0; d < D; ++d) nsl *= support_width(basis);
+      w.assign(size_t(nsl * P->n), T(0));
+      gw.assign(size_t(nsl * P->n * D), T(0));
+      win.assign(size_t(2 * D * P->n), 0);
+      own.w = tess_field{w.data(), 1, 0};
+      own.grad_w = tess_field{gw.data(), D, 1};
+      own.window = win.data();
+      const int st = update_weights_impl<T, D>(basis, m, *P, own, 0, P->n);
+      if (st != TESS_OK) return st;
+      W = &own;
+    }
+    const int st = g2p_seq<T, D>(basis, m, *G, *P, W, *prm);
+    if (st != TESS_OK) return st;
+    return p2g_seq<T, D>(basis, m, *G, *P, W, *prm);
+  }
+};
 template <typename T, int D>
 struct RunStep {
   static int run(const tess_mesh& m, int basis, const tess_grid* G, const tess_particles* P, const tess_params* prm,
@@ -1087,6 +1120,12 @@ int oref_g2p(int basis, const tess_mesh* m, const tess_grid* G, const tess_parti
              const tess_params* prm) {
   if (!basis_ok(basis)) return fail(TESS_ERR_UNSUPPORTED, "basis");
   return dispatch<RunG2P>(m, basis, G, P, W, prm);
+}
+
+int oref_g2p2g(int basis, const tess_mesh* m, const tess_grid* G, const tess_particles* P, const tess_weights* W,
+               const tess_params* prm) {
+  if (!basis_ok(basis)) return fail(TESS_ERR_UNSUPPORTED, "basis");
+  return dispatch<RunG2P2G>(m, basis, G, P, W, prm);
 }
 
 // nsteps of the README step with deferred weights, sequential order.

@@ -239,7 +239,7 @@ tess_status tess_ctx_destroy(tess_ctx* ctx) {
                             &ctx->inc_mks,   &ctx->inc_mr,    &ctx->inc_mjs,
                             &ctx->sp_occ,    &ctx->sp_act,    &ctx->sp_scan,
                             &ctx->adj_partials, &ctx->adj_nodes, &ctx->ro_checkpoints, &ctx->ro_segment,
-                            &ctx->ro_gradv,  &ctx->ro_snapshots, &ctx->movers};
+                            &ctx->ro_gradv,  &ctx->ro_snapshots, &ctx->movers, &ctx->g2p2g_x};
     for (auto* b : bufs) b->release();
     for (auto ev : ctx->prof.pool) (void)hipEventDestroy(ev);
     if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
@@ -305,6 +305,20 @@ tess_status tess_p2g(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const 
     check_weights(weights);
     check_params(params);
     run_p2g(ctx, basis, *mesh, *grid, *particles, weights, *params);
+  });
+}
+
+tess_status tess_g2p2g(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                       const tess_particles* particles, const tess_weights* weights, const tess_params* params) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    check_basis(basis);
+    check_grid(grid);
+    check_particles(particles, basis, true);
+    check_weights(weights);
+    check_params(params);
+    run_g2p2g(ctx, basis, *mesh, *grid, *particles, weights, *params);
   });
 }
 

@@ -313,9 +313,14 @@ __global__ __launch_bounds__(64) void k_adj_grid(Args<T, D> a, const double* __r
 }
 
 // Nodes -> particle adjoints: the footprint's forward v, Δv and node adjoints
-// are staged in LDS; each thread reverses one particle of the tile.
+// are staged in LDS; each thread reverses one particle of the tile (512
+// threads: a tile's particles at 8 per cell in one pass).
+#ifndef TESS_ADJ_GATHER_THREADS
+#define TESS_ADJ_GATHER_THREADS 512
+#endif
+constexpr int ADJ_GATHER_THREADS = TESS_ADJ_GATHER_THREADS;
 template <typename T, int D, int BASIS>
-__global__ __launch_bounds__(256) void k_adj_gather(Args<T, D> a, const double* __restrict__ nodeadj,
+__global__ __launch_bounds__(ADJ_GATHER_THREADS) void k_adj_gather(Args<T, D> a, const double* __restrict__ nodeadj,
                                                     DField<double> ax, DField<double> av, DField<double> aF,
                                                     DField<double> ath) {
   using G = Geo<D>;
@@ -635,7 +640,8 @@ void adjoint_kernels(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, c
   TESS_HIP(hipGetLastError());
   prof_end(ctx, tok);
   tok = prof_begin(ctx, "k_adj_gather", 0.0);
-  hipLaunchKernelGGL((k_adj_gather<T, D, BASIS>), dim3(xcd_grid(b.n_active)), dim3(256), 0, ctx->stream, a,
+  hipLaunchKernelGGL((k_adj_gather<T, D, BASIS>), dim3(xcd_grid(b.n_active)), dim3(ADJ_GATHER_THREADS), 0,
+                     ctx->stream, a,
                      nodeadj, ax, av, aF, ath);
   TESS_HIP(hipGetLastError());
   prof_end(ctx, tok);

@@ -107,6 +107,7 @@ struct tess_ctx {
   int64_t last_movers = -1;
   int64_t kept_binnings = 0;  // binnings that kept the previous layout (no particle changed its key)
   tess::Buffer counters;   // small device scalars
+  tess::Buffer g2p2g_x;    // positions before a G2P2G pass (its P2G half's weights)
   tess::Buffer movers;     // G2P's changed-key counts (spread over cache lines)
   tess::Buffer host_pinned_dummy;
   int64_t* counters_host = nullptr;  // pinned readback slots
@@ -181,6 +182,8 @@ void run_p2g(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G
              const tess_weights* W, const tess_params& prm);
 void run_g2p(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
              const tess_weights* W, const tess_params& prm);
+void run_g2p2g(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+               const tess_weights* W, const tess_params& prm);
 void run_step(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
               const tess_params& prm, int nsteps, double reorder_threshold);
 // The same on an SpGrid: grid fields are SpArray data vectors addressed through sp.

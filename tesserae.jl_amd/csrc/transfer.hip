@@ -1,5 +1,6 @@
-// Forward entry points of the README step (tess_p2g / tess_g2p / tess_step and
-// their SpGrid forms); the kernels and launchers live in transfer_impl.hpp.
+// Forward entry points of the README step (tess_p2g / tess_g2p / tess_g2p2g /
+// tess_step and their SpGrid forms); the kernels and launchers live in
+// transfer_impl.hpp.
 #include "transfer_impl.hpp"
 
 namespace tess {
@@ -12,6 +13,11 @@ void run_p2g(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G
 void run_g2p(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
              const tess_weights* W, const tess_params& prm) {
   dispatch<G2POp>(basis, mesh, W != nullptr, ctx, mesh, G, P, W, prm, static_cast<const tess_sparsity*>(nullptr));
+}
+
+void run_g2p2g(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+               const tess_weights* W, const tess_params& prm) {
+  dispatch<G2P2GOp>(basis, mesh, W != nullptr, ctx, mesh, G, P, W, prm);
 }
 
 void run_step(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,

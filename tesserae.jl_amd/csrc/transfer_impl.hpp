@@ -1771,6 +1771,15 @@ __global__ __launch_bounds__(256) void k_g2p_outside(Args<T, D> a, const uint32_
   particle_update<T, D>(a, pid, dv, L, dx);
 }
 
+// x of every particle into a dense (x, y[, z]) array
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_keep_positions(Args<T, D> a, T* __restrict__ xo) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+#pragma unroll
+  for (int d = 0; d < D; ++d) xo[p * D + d] = a.x(p, d);
+}
+
 // ---- host side ----------------------------------------------------------------
 
 template <typename T, int D>
@@ -2163,13 +2172,14 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     movers_zero(ctx);
   }
   const uint32_t* skeys_now = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
+  const bool aos = julia_layout(P, D);
   if (b.n_active > 0) {
     // algorithmic bytes: x, v, F read and v, ∇v, x, F, σ, V written per
     // particle; vⁿ, v read per active node
     const double nodes = double(std::max<int64_t>(ctx->last_active_nodes, 0));
+    const int per_p = 2 * D + D * D + 2 * D + 3 * D * D + 1;
     const int tok = prof_begin(ctx, "k_g2p_tile",
-                               double(b.n_inside) * sizeof(T) * (2 * D + D * D + 2 * D + 3 * D * D + 1) +
-                                   nodes * sizeof(T) * 2 * D);
+                               double(b.n_inside) * sizeof(T) * per_p + nodes * sizeof(T) * 2 * D);
     const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
     if constexpr (STORED) {
       hipLaunchKernelGGL((k_g2p_tile<T, D, BASIS, STORED>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream, a,
@@ -2177,7 +2187,7 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     } else {
       constexpr int NW = TESS_G2P_WAVES_PER_TILE;
       const uint32_t nt = uint32_t(b.ntiles);
-      if (julia_layout(P, D))
+      if (aos)
         hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
                            ctx->stream, a, sk, nk, stay, nt, mv);
       else
@@ -2272,6 +2282,30 @@ void g2p_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const te
   g2p_launch<T, D, BASIS, STORED>(ctx, mesh, G, P, W, prm, sp);
 }
 
+// @G2P2G of the README program (src/Transfer/g2p2g.jl:100-159): one set of
+// weights, at the positions before the pass, serves the G2P half and the P2G
+// half.  One binning at those positions orders both halves: the G2P kernel
+// updates the particles, then the P2G kernels scatter the updated v, V, σ with
+// the weights of the old positions (stored windows, or the positions kept
+// aside before the G2P half moved them), and the grid-node half follows.
+template <typename T, int D, int BASIS, bool STORED>
+void g2p2g_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+                const tess_weights* W, const tess_params& prm) {
+  bin_particles<T, D, BASIS, STORED>(ctx, mesh, P, W);
+  tess_particles Pold = P;
+  if (!STORED && P.n > 0) {
+    T* xo = ctx->g2p2g_x.as<T>(size_t(P.n) * D);
+    Args<T, D> a = make_args<T, D>(mesh, nullptr, P, nullptr, nullptr);
+    hipLaunchKernelGGL((k_keep_positions<T, D>), dim3(blocks_for(P.n, 256)), dim3(256), 0, ctx->stream, a, xo);
+    TESS_HIP(hipGetLastError());
+    Pold.x = tess_field{xo, D, 1};
+  }
+  g2p_launch<T, D, BASIS, STORED>(ctx, mesh, G, P, W, prm, nullptr);
+  ctx->bin.valid = true;  // the binning describes the old positions the P2G half reads
+  p2g_launch<T, D, BASIS, STORED>(ctx, mesh, G, Pold, W, prm, false, nullptr);
+  ctx->bin.valid = false;  // the particles have moved
+}
+
 // nsteps of the README loop body (deferred weights).  With reorder_threshold
 // in (0, 1], particles are moved into the layered order when the fraction
 // stored in place, measured by the previous step's P2G, is below the
@@ -2347,6 +2381,13 @@ struct G2POp {
   static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
                   const tess_weights* W, const tess_params& prm, const tess_sparsity* sp) {
     g2p_impl<T, D, B, S>(ctx, m, G, P, W, prm, sp);
+  }
+};
+template <typename T, int D, int B, bool S>
+struct G2P2GOp {
+  static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
+                  const tess_weights* W, const tess_params& prm) {
+    g2p2g_impl<T, D, B, S>(ctx, m, G, P, W, prm);
   }
 };
 template <typename T, int D, int B, bool S>

@@ -147,6 +147,11 @@ SIGNATURES = {
         [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_particles), _P(tess_weights),
          _P(tess_params)],
     ),
+    "tess_g2p2g": (
+        ctypes.c_int32,
+        [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_particles), _P(tess_weights),
+         _P(tess_params)],
+    ),
     "tess_g2p": (
         ctypes.c_int32,
         [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_particles), _P(tess_weights),

@@ -667,6 +667,20 @@ def G2P(grid: Grid, particles: Particles, weights, params: MPMParams, ctx=None) 
                                      ctypes.byref(P), Wp, ctypes.byref(params.abi())))
 
 
+def G2P2G(grid: Grid, particles: Particles, weights, params: MPMParams, ctx=None) -> None:
+    """@G2P2G of the README program (src/Transfer/g2p2g.jl:100-159): the G2P half, then
+    the P2G and grid-node halves of the next step, both halves with the weights at the
+    positions the particles have before the call.  Dense grids."""
+    basis, Wp = _resolve_weights(weights)
+    _require_device(particles.x, grid.m)
+    if isinstance(grid, SpGrid):
+        raise UnsupportedError("G2P2G on an SpGrid is not supported")
+    ctx = ctx or default_context()
+    P, G = particles.abi(), grid.abi()
+    _check(ctx.lib, ctx.lib.tess_g2p2g(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
+                                       ctypes.byref(P), Wp, ctypes.byref(params.abi())))
+
+
 def _resolve_weights(weights):
     """(basis, pointer-to-tess_weights or None for deferred evaluation)."""
     if isinstance(weights, Basis):

@@ -129,6 +129,7 @@ class Oracle:
                                                  ctypes.c_void_p, ctypes.c_void_p]),
             "oref_p2g": (ctypes.c_int, [ctypes.c_int, M, G, PP, W, PR]),
             "oref_g2p": (ctypes.c_int, [ctypes.c_int, M, G, PP, W, PR]),
+            "oref_g2p2g": (ctypes.c_int, [ctypes.c_int, M, G, PP, W, PR]),
             "oref_step": (ctypes.c_int, [ctypes.c_int, M, G, PP, PR, ctypes.c_int]),
             "oref_step_threaded": (ctypes.c_int, [ctypes.c_int, M, G, PP, PR, ctypes.c_int, ctypes.c_int, W]),
         }
@@ -216,6 +217,12 @@ class Oracle:
         Wp = ctypes.byref(weights.abi()) if weights is not None else None
         self._ok(self.lib.oref_g2p(basis.kind, ctypes.byref(st.mesh_abi()), ctypes.byref(G), ctypes.byref(P), Wp,
                                    ctypes.byref(params.abi())))
+
+    def g2p2g(self, st: HostState, basis, params, weights: HostWeights | None = None):
+        P, G = st.particles_abi(), st.grid_abi()
+        Wp = ctypes.byref(weights.abi()) if weights is not None else None
+        self._ok(self.lib.oref_g2p2g(basis.kind, ctypes.byref(st.mesh_abi()), ctypes.byref(G), ctypes.byref(P), Wp,
+                                     ctypes.byref(params.abi())))
 
     def step(self, st: HostState, basis, params, nsteps=1):
         P, G = st.particles_abi(), st.grid_abi()
