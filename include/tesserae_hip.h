@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define TESS_ABI_VERSION 1
+#define TESS_ABI_VERSION 2
 
 typedef int32_t tess_status;
 enum {
@@ -56,6 +56,22 @@ enum {
   TESS_BSPLINE_QUADRATIC = 2,
   TESS_BSPLINE_CUBIC = 3,
   TESS_UGIMP = 16
+};
+
+/* Corrected bases: one of the kernel codes above OR'd with a wrapper and,
+ * optionally, the polynomial flag.
+ *   WLS(kernel[, poly]) (src/Basis/wls.jl:1-17): the weighted least-squares fit
+ *     of poly over the kernel's support; poly defaults to Polynomial(Linear()).
+ *   KernelCorrection(kernel[, poly]) (src/Basis/kernelcorrection.jl:1-24): the
+ *     plain kernel on a whole support, WLS(kernel, poly) where the mesh (or the
+ *     node filter) cuts it; poly defaults to Polynomial(MultiLinear()).
+ * TESS_POLY_MULTILINEAR selects Polynomial(MultiLinear()) (1, x, y, xy[, z, yz,
+ * zx, xyz], src/Basis/polynomials.jl:42-46); without it the polynomial is
+ * Polynomial(Linear()) (1, x[, y, z]).  Corrected weights carry w and ∇w. */
+enum {
+  TESS_WLS = 0x100,
+  TESS_KERNEL_CORRECTION = 0x200,
+  TESS_POLY_MULTILINEAR = 0x400
 };
 
 /* CartesianMesh{dim,T} (src/Mesh/cartesian.jl:38-42, ctor :89-103).
@@ -100,11 +116,19 @@ typedef struct tess_grid {
  * window the per-particle CartesianIndices as dim (start, stop) Int64 pairs,
  * 1-based inclusive, (1,0) when empty.  Entry ip of particle p is element
  * ip + sw^dim * p, ip being the column-major position of the node inside the
- * (possibly truncated) window placed at the origin of the sw^dim storage. */
+ * (possibly truncated) window placed at the origin of the sw^dim storage.
+ * filter is the node mask of update!(weights, particles, mesh, filter)
+ * (basis_weight.jl:714-752): one byte per mesh node in column-major node
+ * order, nonzero = the node takes part in the corrected fit; NULL = Trues.
+ * Only corrected bases accept a filter (supports_filtered_updates).
+ * A tess_weights whose w.ptr is NULL stands for deferred weights that
+ * recorded a filter (update!(...; deferred=true), basis_weight.jl:398,716-721):
+ * transfers evaluate the basis and read only filter. */
 typedef struct tess_weights {
   tess_field w;
   tess_field grad_w;
   int64_t* window;
+  const uint8_t* filter;
 } tess_weights;
 
 /* Constants of the README program (README.md:46-49, 71-87) plus the config
@@ -138,10 +162,12 @@ int32_t tess_abi_version(void);
 
 /* ---- basis weights ------------------------------------------------------ */
 
-/* update!(weights, particles, mesh) in stored mode
+/* update!(weights, particles, mesh[, filter]) in stored mode
  * (src/Basis/basis_weight.jl:714-752; per particle :634-654, B-spline fast path
- * src/Basis/bspline.jl:102-170, truncated windows basis_weight.jl:624-632).
- * Writes w, grad_w and window for particles 0..n-1. */
+ * src/Basis/bspline.jl:102-170, truncated windows basis_weight.jl:624-632;
+ * corrected bases src/Basis/wls.jl:19-128, kernelcorrection.jl:26-35).
+ * Writes w, grad_w and window for particles 0..n-1.  A weights->filter on a
+ * basis that is not corrected fails with TESS_ERR_ARGUMENT. */
 tess_status tess_update_weights(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh,
                                 const tess_particles* particles, const tess_weights* weights);
 
@@ -171,9 +197,12 @@ tess_status tess_reorder_particles(tess_ctx* ctx, const tess_mesh* mesh, const t
 /* @P2G of the README program (README.md:71-77; lowering src/Transfer/p2g.jl:145-178,
  * grid-node half :377-395): m = Σ w mₚ, mv = Σ w (mₚ v), f = Σ ((-V σ)·∇w), then on
  * every node m⁻¹ = m>0 ? 1/m : 0, vⁿ = mv m⁻¹, v = vⁿ + Δt (f m⁻¹ + g[m>0]), slip
- * walls.  weights == NULL evaluates the basis inside the transfer (deferred
- * weights, src/Transfer/runtime.jl:21-24,55-130).  Deterministic: no float
- * atomics; the same inputs give bitwise-identical grids. */
+ * walls.  weights == NULL (or weights->w.ptr == NULL, carrying a filter)
+ * evaluates the basis inside the transfer (deferred weights,
+ * src/Transfer/runtime.jl:21-24,55-130).  Deterministic: no float atomics; the
+ * same inputs give bitwise-identical grids.  Corrected bases (TESS_WLS,
+ * TESS_KERNEL_CORRECTION) are accepted by every forward entry point except the
+ * batch ones. */
 tess_status tess_p2g(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
                      const tess_particles* particles, const tess_weights* weights,
                      const tess_params* params);

@@ -92,9 +92,10 @@ struct Mesh {
 // ---- basis (bspline.jl, gimp.jl, basis_weight.jl) -------------------------
 
 constexpr int support_width(int basis) {
+  basis &= 0xff;  // the kernel of a corrected basis
   return basis == TESS_BSPLINE_LINEAR ? 2 : basis == TESS_BSPLINE_QUADRATIC ? 3 : basis == TESS_BSPLINE_CUBIC ? 4 : 3;
 }
-constexpr int degree_of(int basis) { return basis == TESS_UGIMP ? 2 : basis; }
+constexpr int degree_of(int basis) { return (basis & 0xff) == TESS_UGIMP ? 2 : (basis & 0xff); }
 
 // Julia evalpoly(x, (a0, a1, ...)) = muladd Horner from the top coefficient.
 template <typename T>
@@ -295,6 +296,248 @@ void for_window(const PW<T, D>& pw, const Mesh<T, D>& mesh, Fn&& f) {
   }
 }
 
+// ---- corrected bases (wls.jl, kernelcorrection.jl, polynomials.jl) --------
+
+constexpr int kernel_of(int basis) { return basis & 0xff; }
+
+// value(poly, x) (polynomials.jl:32-46): Linear (1, x1..xD); MultiLinear in 2D
+// (1, x1, x2, x1x2), in 3D (1, x1, x2, x3, x1x2, x2x3, x3x1, x1x2x3).
+template <typename T, int D>
+int poly_value(bool multilinear, const T* x, T* P) {
+  P[0] = T(1);
+  for (int d = 0; d < D; ++d) P[1 + d] = x[d];
+  if (!multilinear) return D + 1;
+  if (D == 2) {
+    P[3] = x[0] * x[1];
+    return 4;
+  }
+  P[4] = x[0] * x[1];
+  P[5] = x[1] * x[2];
+  P[6] = x[2] * x[0];
+  P[7] = x[0] * x[1] * x[2];
+  return 8;
+}
+
+// inv(M) of an n×n matrix (row-major): Gauss-Jordan elimination with partial
+// pivoting in T.  The reference calls Tensorial.jl's inv (not in its tree); a
+// singular M gives non-finite entries in both.
+template <typename T>
+void invert(int n, const T* M, T* Minv) {
+  T a[8][16];
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < 2 * n; ++j) a[i][j] = j < n ? M[i * n + j] : (j - n == i ? T(1) : T(0));
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(a[i][k]) > std::fabs(a[piv][k])) piv = i;
+    if (piv != k)
+      for (int j = 0; j < 2 * n; ++j) std::swap(a[k][j], a[piv][j]);
+    const T p = a[k][k];
+    for (int j = 0; j < 2 * n; ++j) a[k][j] = a[k][j] / p;
+    for (int i = 0; i < n; ++i) {
+      if (i == k) continue;
+      const T f = a[i][k];
+      for (int j = 0; j < 2 * n; ++j) a[i][j] = a[i][j] - f * a[k][j];
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) Minv[i * n + j] = a[i][n + j];
+}
+
+// filter[i] over the window (alltrue, basis_weight.jl:593-607); NULL = Trues.
+template <typename T, int D>
+bool filter_alltrue(const uint8_t* filter, const PW<T, D>& pw, const Mesh<T, D>& mesh) {
+  if (!filter) return true;
+  bool all = true;
+  for_window<T, D>(pw, mesh, [&](int, int64_t i) { all = all && filter[i] != 0; });
+  return all;
+}
+
+// Nodewise kernel value (nodal_basis_jet(Order(0), kernel, pt, mesh, i),
+// basis_weight.jl:49-60): the left-associated product of the axis values.
+template <typename T, int D>
+T kernel_value(int kernel, const T* x, T lp, const Mesh<T, D>& mesh, const int64_t* I) {
+  const T l = kernel == TESS_UGIMP ? lp / mesh.h : T(0);
+  T w = T(1);
+  for (int d = 0; d < D; ++d) {
+    T val, der;
+    piecewise_jet<T>(kernel, (x[d] - mesh.node(d, I[d])) * mesh.h_inv, l, &val, &der);
+    w = d == 0 ? val : w * val;
+  }
+  return w;
+}
+
+// apply_wls_correction! (wls.jl:66-77): wq = w·(M⁻¹P(xᵢ-xₚ)); w ← wq⊡P₀ = wq[0],
+// ∇w ← wq⊡∇P₀ = wq[1..D] (the jets of Linear / MultiLinear at 0, polynomials.jl:35-46).
+template <typename T, int D>
+void apply_correction(bool multilinear, const T* x, const Mesh<T, D>& mesh, const T* Minv, int np, PW<T, D>& pw) {
+  for_window<T, D>(pw, mesh, [&](int s, int64_t i) {
+    int64_t I[D], r = i;
+    for (int d = 0; d < D; ++d) { I[d] = r % mesh.dims[d]; r /= mesh.dims[d]; }
+    T dx[D], P[8], q[8];
+    for (int d = 0; d < D; ++d) dx[d] = mesh.node(d, I[d]) - x[d];
+    poly_value<T, D>(multilinear, dx, P);
+    for (int a = 0; a < np; ++a) {
+      T acc = Minv[a * np] * P[0];
+      for (int b = 1; b < np; ++b) acc = acc + Minv[a * np + b] * P[b];
+      q[a] = acc;
+    }
+    const T w = pw.w[s];
+    pw.w[s] = w * q[0];
+    for (int c = 0; c < D; ++c) pw.gw[s][c] = w * q[1 + c];
+  });
+}
+
+// update_wls_values! (wls.jl:51-64) and the deferred state's fit (:131-148):
+// M = Σ w P⊗P over the window with w = kernel value × filter, then the correction.
+template <typename T, int D>
+void wls_general(int kernel, bool multilinear, const T* x, T lp, const Mesh<T, D>& mesh, const uint8_t* filter,
+                 PW<T, D>& pw) {
+  const int np = multilinear ? (1 << D) : D + 1;
+  T M[64] = {};
+  for_window<T, D>(pw, mesh, [&](int s, int64_t i) {
+    int64_t I[D], r = i;
+    for (int d = 0; d < D; ++d) { I[d] = r % mesh.dims[d]; r /= mesh.dims[d]; }
+    T w = kernel_value<T, D>(kernel, x, lp, mesh, I);
+    if (filter) w = w * T(filter[i] ? 1 : 0);
+    pw.w[s] = w;
+    T dx[D], P[8];
+    for (int d = 0; d < D; ++d) dx[d] = mesh.node(d, I[d]) - x[d];
+    poly_value<T, D>(multilinear, dx, P);
+    for (int a = 0; a < np; ++a)
+      for (int b = 0; b < np; ++b) M[a * np + b] += w * P[a] * P[b];
+  });
+  T Minv[64];
+  invert<T>(np, M, Minv);
+  apply_correction<T, D>(multilinear, x, mesh, Minv, np, pw);
+}
+
+// Closed-form moments of a whole B-spline support (wls.jl:34-49):
+// M⁻¹ = diag(1, 12/(n+1)/h², ...), applied to the nodewise kernel values.
+template <typename T, int D>
+void wls_closed(int kernel, const T* x, const Mesh<T, D>& mesh, PW<T, D>& pw) {
+  const int np = D + 1;
+  T Minv[16] = {};
+  Minv[0] = T(1);
+  const T c = T(12) / T(degree_of(kernel) + 1) / (mesh.h * mesh.h);
+  for (int d = 0; d < D; ++d) Minv[(1 + d) * np + 1 + d] = c;
+  for_window<T, D>(pw, mesh, [&](int s, int64_t i) {
+    int64_t I[D], r = i;
+    for (int d = 0; d < D; ++d) { I[d] = r % mesh.dims[d]; r /= mesh.dims[d]; }
+    pw.w[s] = kernel_value<T, D>(kernel, x, T(0), mesh, I);
+  });
+  apply_correction<T, D>(false, x, mesh, Minv, np, pw);
+}
+
+// MultiLinear fit of a quadratic or cubic B-spline without a filter
+// (wls.jl:79-99, deferred :166-189): dim independent 1-D WLS(kernel, Linear)
+// fits (closed form on a whole axis support, else the 2×2 fit), then the
+// tensor product of the axis values and derivatives.
+template <typename T, int D>
+void wls_separable(int kernel, const T* x, const Mesh<T, D>& mesh, PW<T, D>& pw) {
+  const int sw = pw.sw;
+  T v1[D][PW<T, D>::MAXSW], d1[D][PW<T, D>::MAXSW];
+  int64_t ext[D];
+  for (int d = 0; d < D; ++d) {
+    ext[d] = pw.hi[d] - pw.lo[d] + 1;
+    T N[PW<T, D>::MAXSW], dx[PW<T, D>::MAXSW];
+    for (int64_t a = 0; a < ext[d]; ++a) {
+      const T xi = mesh.node(d, pw.lo[d] + a);
+      T der;
+      piecewise_jet<T>(kernel, (x[d] - xi) * mesh.h_inv, T(0), &N[a], &der);
+      dx[a] = xi - x[d];
+    }
+    T Minv[4];
+    if (ext[d] == sw) {
+      Minv[0] = T(1);
+      Minv[1] = Minv[2] = T(0);
+      Minv[3] = T(12) / T(degree_of(kernel) + 1) / (mesh.h * mesh.h);
+    } else {
+      T M[4] = {};
+      for (int64_t a = 0; a < ext[d]; ++a) {
+        const T P[2] = {T(1), dx[a]};
+        for (int i = 0; i < 2; ++i)
+          for (int j = 0; j < 2; ++j) M[i * 2 + j] += N[a] * P[i] * P[j];
+      }
+      invert<T>(2, M, Minv);
+    }
+    for (int64_t a = 0; a < ext[d]; ++a) {
+      const T q0 = Minv[0] * T(1) + Minv[1] * dx[a];
+      const T q1 = Minv[2] * T(1) + Minv[3] * dx[a];
+      v1[d][a] = N[a] * q0;
+      d1[d][a] = N[a] * q1;
+    }
+  }
+  int64_t cnt = 1;
+  for (int d = 0; d < D; ++d) cnt *= ext[d];
+  for (int64_t k = 0; k < cnt; ++k) {
+    int64_t a[D], r = k;
+    for (int d = 0; d < D; ++d) { a[d] = r % ext[d]; r /= ext[d]; }
+    int64_t slot = 0;
+    for (int d = D - 1; d >= 0; --d) slot = slot * sw + a[d];
+    T w = v1[0][a[0]];
+    for (int d = 1; d < D; ++d) w = w * v1[d][a[d]];
+    pw.w[slot] = w;
+    for (int c = 0; c < D; ++c) {
+      T g = (c == 0) ? d1[0][a[0]] : v1[0][a[0]];
+      for (int d = 1; d < D; ++d) g = g * ((d == c) ? d1[d][a[d]] : v1[d][a[d]]);
+      pw.gw[slot][c] = g;
+    }
+  }
+}
+
+// One particle's corrected weights.  Which branch applies follows the
+// reference's dispatch:
+//   KernelCorrection: the plain kernel when the window is whole and the filter
+//     passes all of it (kernelcorrection.jl:27-35, deferred :41-49); otherwise
+//     WLS(kernel, poly) with the filter, which for MultiLinear over a quadratic
+//     or cubic B-spline without a filter is the separable fit (:53-63,
+//     wls.jl:79-99);
+//   WLS(kernel, Linear) over a quadratic or cubic B-spline: closed-form moments
+//     on a whole window that the filter passes (stored, wls.jl:30-40) or a
+//     whole window without a filter (deferred, :131-138);
+//   WLS(kernel, MultiLinear) over a quadratic or cubic B-spline, stored, no
+//     filter: the separable fit (wls.jl:79-99);
+//   otherwise the general fit (wls.jl:51-64, deferred :139-148).
+template <typename T, int D>
+void eval_corrected(int basis, const T* x, T lp, const Mesh<T, D>& mesh, bool deferred, const uint8_t* filter,
+                    PW<T, D>& pw) {
+  const int kernel = kernel_of(basis);
+  const bool ml = (basis & TESS_POLY_MULTILINEAR) != 0;
+  const bool qc = kernel == TESS_BSPLINE_QUADRATIC || kernel == TESS_BSPLINE_CUBIC;
+  support_window<T, D>(kernel, x, lp, mesh, pw);
+  if (pw.empty) return;
+  const bool ft = filter_alltrue<T, D>(filter, pw, mesh);
+  if (basis & TESS_KERNEL_CORRECTION) {
+    if (pw.full && ft) {
+      eval_weights<T, D>(kernel, x, lp, mesh, deferred, pw);
+    } else if (ml && qc && !filter) {
+      wls_separable<T, D>(kernel, x, mesh, pw);
+    } else {
+      wls_general<T, D>(kernel, ml, x, lp, mesh, filter, pw);
+    }
+    return;
+  }
+  if (!ml && qc && pw.full && (deferred ? !filter : ft)) {
+    wls_closed<T, D>(kernel, x, mesh, pw);
+  } else if (ml && qc && !filter && !deferred) {
+    wls_separable<T, D>(kernel, x, mesh, pw);
+  } else {
+    wls_general<T, D>(kernel, ml, x, lp, mesh, filter, pw);
+  }
+}
+
+// Plain kernels ignore the filter (their deferred state is `nothing`,
+// basis_weight.jl:442-446); corrected bases read it.
+template <typename T, int D>
+void eval_any(int basis, const T* x, T lp, const Mesh<T, D>& mesh, bool deferred, const uint8_t* filter,
+              PW<T, D>& pw) {
+  if (basis & (TESS_WLS | TESS_KERNEL_CORRECTION))
+    eval_corrected<T, D>(basis, x, lp, mesh, deferred, filter, pw);
+  else
+    eval_weights<T, D>(basis, x, lp, mesh, deferred, pw);
+}
+
 template <typename T, int D>
 void load_vec(const Fld<T>& f, int64_t e, T* out) {
   for (int c = 0; c < D; ++c) out[c] = f(e, c);
@@ -308,8 +551,8 @@ void particle_weights(int basis, const Mesh<T, D>& mesh, const tess_particles& P
   T x[D];
   load_vec<T, D>(X, p, x);
   T lp = Lf.ok() ? Lf(p) : T(0);
-  if (!W) {
-    eval_weights<T, D>(basis, x, lp, mesh, true, pw);
+  if (!W || !W->w.ptr) {  // deferred; a weights struct without storage carries the recorded filter
+    eval_any<T, D>(basis, x, lp, mesh, true, W ? W->filter : nullptr, pw);
     return;
   }
   // stored: window and values as update! wrote them (runtime.jl:15-18, 39-44)
@@ -474,7 +717,7 @@ int g2p_seq(int basis, const tess_mesh& tm, const tess_grid& G, const tess_parti
 
 template <typename T, int D>
 int update_weights_impl(int basis, const tess_mesh& tm, const tess_particles& P, const tess_weights& W,
-                        int64_t p0, int64_t p1) {
+                        int64_t p0, int64_t p1, bool deferred = false) {
   Mesh<T, D> mesh(tm);
   Fld<T> X(P.x), Lf(P.l), Wf(W.w), Gf(W.grad_w);
   const int sw = support_width(basis);
@@ -484,7 +727,7 @@ int update_weights_impl(int basis, const tess_mesh& tm, const tess_particles& P,
     T x[D];
     load_vec<T, D>(X, p, x);
     PW<T, D> pw;
-    eval_weights<T, D>(basis, x, Lf.ok() ? Lf(p) : T(0), mesh, false, pw);
+    eval_any<T, D>(basis, x, Lf.ok() ? Lf(p) : T(0), mesh, deferred, W.filter, pw);
     int64_t* win = W.window + p * 2 * D;
     for (int d = 0; d < D; ++d) {
       if (pw.empty) { win[2 * d] = 1; win[2 * d + 1] = 0; }
@@ -714,7 +957,7 @@ void inverse_d(const double* F, double* Fi) {  // column-major
 template <typename T, int D>
 int step_adjoint_impl(int basis, const tess_mesh& tm, const tess_grid& G, const tess_particles& P,
                       const tess_params& prm, double* ax, double* av, double* aF, double* ath) {
-  if (basis == TESS_UGIMP) return fail(TESS_ERR_UNSUPPORTED, "adjoint: B-spline bases only");
+  if (basis == TESS_UGIMP || (basis & ~0xff)) return fail(TESS_ERR_UNSUPPORTED, "adjoint: B-spline bases only");
   Mesh<T, D> mesh(tm);
   p2g_seq<T, D>(basis, tm, G, P, nullptr, prm);  // the forward grid of this step
   Fld<T> X(P.x), v(P.v), Ff(P.F), V(P.V), sg(P.sigma), LE(P.logE);
@@ -859,9 +1102,16 @@ int dispatch(const tess_mesh* m, Args&&... args) {
   return fail(TESS_ERR_UNSUPPORTED, "unsupported dim/dtype");
 }
 
+// A kernel, optionally wrapped in exactly one of WLS / KernelCorrection, the
+// polynomial flag only with a wrapper.
 bool basis_ok(int basis) {
-  return basis == TESS_BSPLINE_LINEAR || basis == TESS_BSPLINE_QUADRATIC || basis == TESS_BSPLINE_CUBIC ||
-         basis == TESS_UGIMP;
+  const int k = kernel_of(basis), flags = basis & ~0xff;
+  const bool kernel = k == TESS_BSPLINE_LINEAR || k == TESS_BSPLINE_QUADRATIC || k == TESS_BSPLINE_CUBIC ||
+                      k == TESS_UGIMP;
+  const int wrap = flags & (TESS_WLS | TESS_KERNEL_CORRECTION);
+  const bool ok_flags = (flags & ~(TESS_WLS | TESS_KERNEL_CORRECTION | TESS_POLY_MULTILINEAR)) == 0 &&
+                        (wrap == 0 ? flags == 0 : wrap != (TESS_WLS | TESS_KERNEL_CORRECTION));
+  return kernel && ok_flags;
 }
 
 template <typename T, int D>
@@ -892,8 +1142,9 @@ struct RunG2P2G {
                  const tess_params* prm) {
     std::vector<T> w, gw;
     std::vector<int64_t> win;
-    tess_weights own;
-    if (!W) {
+    tess_weights own{};
+    if (!W || !W->w.ptr) {
+      own.filter = W ? W->filter : nullptr;
       int64_t nsl = 1;
       for (int d = 0; d < D; ++d) nsl *= support_width(basis);
       w.assign(size_t(nsl * P->n), T(0));
@@ -902,7 +1153,7 @@ struct RunG2P2G {
       own.w = tess_field{w.data(), 1, 0};
       own.grad_w = tess_field{gw.data(), D, 1};
       own.window = win.data();
-      const int st = update_weights_impl<T, D>(basis, m, *P, own, 0, P->n);
+      const int st = update_weights_impl<T, D>(basis, m, *P, own, 0, P->n, true);
       if (st != TESS_OK) return st;
       W = &own;
     }
@@ -1003,13 +1254,13 @@ struct RunThreaded {
 };
 template <typename T, int D>
 struct RunJet {
-  static int run(const tess_mesh& m, int basis, const double* x, double l, int deferred, int64_t* window,
-                 double* w, double* gw) {
+  static int run(const tess_mesh& m, int basis, const double* x, double l, int deferred, const uint8_t* filter,
+                 int64_t* window, double* w, double* gw) {
     Mesh<T, D> mesh(m);
     T xt[D];
     for (int d = 0; d < D; ++d) xt[d] = T(x[d]);
     PW<T, D> pw;
-    eval_weights<T, D>(basis, xt, T(l), mesh, deferred != 0, pw);
+    eval_any<T, D>(basis, xt, T(l), mesh, deferred != 0, filter, pw);
     for (int d = 0; d < D; ++d) {
       window[2 * d] = pw.empty ? 1 : pw.lo[d] + 1;
       window[2 * d + 1] = pw.empty ? 0 : pw.hi[d] + 1;
@@ -1039,12 +1290,23 @@ const char* oref_last_error(void) { return g_err.c_str(); }
 int oref_basis_weight(const tess_mesh* m, int basis, const double* x, double l, int deferred, int64_t* window,
                       double* w, double* gw) {
   if (!basis_ok(basis)) return fail(TESS_ERR_UNSUPPORTED, "basis");
-  return dispatch<RunJet>(m, basis, x, l, deferred, window, w, gw);
+  return dispatch<RunJet>(m, basis, x, l, deferred, static_cast<const uint8_t*>(nullptr), window, w, gw);
+}
+
+// The same with a node filter (update!(bw, pt, mesh, filter),
+// basis_weight.jl:683-695): one byte per mesh node, column-major; corrected
+// bases only.
+int oref_basis_weight_filtered(const tess_mesh* m, int basis, const double* x, double l, int deferred,
+                               const uint8_t* filter, int64_t* window, double* w, double* gw) {
+  if (!basis_ok(basis)) return fail(TESS_ERR_UNSUPPORTED, "basis");
+  if (filter && !(basis & (TESS_WLS | TESS_KERNEL_CORRECTION)))
+    return fail(TESS_ERR_ARGUMENT, "the basis does not support filtered updates");
+  return dispatch<RunJet>(m, basis, x, l, deferred, filter, window, w, gw);
 }
 
 // 1-D jet(Order(1), kernel, ξ[, l]) (basis_weight.jl:35-37).
 int oref_jet1d(int basis, double xi, double l, double* val, double* der) {
-  if (!basis_ok(basis)) return fail(TESS_ERR_UNSUPPORTED, "basis");
+  if (!basis_ok(basis) || (basis & ~0xff)) return fail(TESS_ERR_UNSUPPORTED, "basis");
   piecewise_jet<double>(basis, xi, l, val, der);
   return TESS_OK;
 }
@@ -1087,6 +1349,8 @@ int oref_supportnodes_r(const tess_mesh* m, const double* x, double r, int64_t* 
 
 int oref_update_weights(int basis, const tess_mesh* m, const tess_particles* P, const tess_weights* W) {
   if (!basis_ok(basis)) return fail(TESS_ERR_UNSUPPORTED, "basis");
+  if (W && W->filter && !(basis & (TESS_WLS | TESS_KERNEL_CORRECTION)))  // require_filtered_update_support
+    return fail(TESS_ERR_ARGUMENT, "the basis does not support filtered updates");
   return dispatch<RunWeights>(m, basis, P, W);
 }
 

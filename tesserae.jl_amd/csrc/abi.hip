@@ -125,10 +125,22 @@ void check_mesh(const tess_mesh* m) {
   if (!(m->h_inv > 0) || !std::isfinite(m->h_inv)) raise(TESS_ERR_ARGUMENT, "mesh inverse spacing must be positive");
 }
 
+// A kernel, optionally wrapped in one of WLS / KernelCorrection (the
+// polynomial flag only with a wrapper).
 void check_basis(int32_t basis) {
-  if (basis != TESS_BSPLINE_LINEAR && basis != TESS_BSPLINE_QUADRATIC && basis != TESS_BSPLINE_CUBIC &&
-      basis != TESS_UGIMP)
+  const int32_t k = basis & 0xff, flags = basis & ~0xff;
+  if (k != TESS_BSPLINE_LINEAR && k != TESS_BSPLINE_QUADRATIC && k != TESS_BSPLINE_CUBIC && k != TESS_UGIMP)
     raise(TESS_ERR_UNSUPPORTED, "basis is not one of BSpline(Linear/Quadratic/Cubic) or uGIMP");
+  const int32_t wrap = flags & (TESS_WLS | TESS_KERNEL_CORRECTION);
+  if ((flags & ~(TESS_WLS | TESS_KERNEL_CORRECTION | TESS_POLY_MULTILINEAR)) != 0 ||
+      wrap == (TESS_WLS | TESS_KERNEL_CORRECTION) || (wrap == 0 && flags != 0))
+    raise(TESS_ERR_UNSUPPORTED, "basis flags: one of TESS_WLS / TESS_KERNEL_CORRECTION, with an optional "
+                                "TESS_POLY_MULTILINEAR");
+}
+
+void check_plain_basis(int32_t basis, const char* what) {
+  check_basis(basis);
+  if (basis & ~0xff) raise(TESS_ERR_UNSUPPORTED, std::string(what) + " supports the plain kernels only");
 }
 
 void need(const tess_field& f, const char* what) {
@@ -146,7 +158,7 @@ void check_particles(const tess_particles* P, int32_t basis, bool transfer) {
     need(P->F, "F");
     need(P->sigma, "σ");
   }
-  if (basis == TESS_UGIMP && !P->l.ptr)
+  if ((basis & 0xff) == TESS_UGIMP && !P->l.ptr)
     raise(TESS_ERR_ARGUMENT, "uGIMP requires the initial particle length `l` in the particle property");
 }
 
@@ -160,7 +172,7 @@ void check_grid(const tess_grid* G) {
 }
 
 void check_weights(const tess_weights* W) {
-  if (!W) return;
+  if (!W || !W->w.ptr) return;  // deferred (a weights struct without storage carries only a filter)
   if (!W->window) raise(TESS_ERR_ARGUMENT, "stored weights need the support-window array");
   need(W->w, "w");
   need(W->grad_w, "∇w");
@@ -262,7 +274,10 @@ tess_status tess_update_weights(tess_ctx* ctx, int32_t basis, const tess_mesh* m
     check_basis(basis);
     check_particles(particles, basis, false);
     if (!weights) raise(TESS_ERR_ARGUMENT, "weights is NULL");
+    if (!weights->w.ptr) raise(TESS_ERR_ARGUMENT, "update! of stored weights needs the w array");
     check_weights(weights);
+    if (weights->filter && !(basis & (TESS_WLS | TESS_KERNEL_CORRECTION)))  // require_filtered_update_support
+      raise(TESS_ERR_ARGUMENT, "the basis does not support filtered updates");
     launch_update_weights(ctx, basis, *mesh, *particles, *weights);
   });
 }
@@ -418,7 +433,7 @@ tess_status tess_step_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh
 namespace tess {
 namespace {
 void check_adjoint_basis(int32_t basis) {
-  check_basis(basis);
+  check_plain_basis(basis, "the adjoint");
   if (basis == TESS_UGIMP) raise(TESS_ERR_UNSUPPORTED, "the adjoint supports the B-spline bases");
 }
 
@@ -480,7 +495,7 @@ tess_status tess_batch_create(tess_ctx* ctx, int32_t basis, int32_t nscenes, con
                               tess_batch** out) {
   return guarded([&] {
     check_ctx(ctx);
-    check_basis(basis);
+    check_plain_basis(basis, "a batch");
     if (!out) raise(TESS_ERR_ARGUMENT, "out is NULL");
     *out = nullptr;
     if (nscenes < 1 || !scenes) raise(TESS_ERR_ARGUMENT, "a batch needs at least one scene");

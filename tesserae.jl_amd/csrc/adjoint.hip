@@ -910,11 +910,15 @@ struct RolloutAdjointOp {
 
 void run_step_adjoint(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
                       const tess_params& prm, const tess_adjoint& adj) {
+  ctx->corr = 0;  // plain kernels only
+  ctx->corr_filter = nullptr;
   dispatch<AdjointOp>(basis, mesh, false, ctx, mesh, G, P, prm, adj);
 }
 
 void run_rollout(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
                  const tess_params& prm, int nsteps, int every) {
+  ctx->corr = 0;
+  ctx->corr_filter = nullptr;
   dispatch<RolloutOp>(basis, mesh, false, ctx, mesh, G, P, prm, nsteps, every);
 }
 
@@ -925,6 +929,8 @@ void run_rollout_adjoint(tess_ctx* ctx, int basis, const tess_mesh& mesh, const 
   bool same = ro.dim == mesh.dim && ro.dtype == mesh.dtype && ro.basis == basis;
   for (int d = 0; d < mesh.dim; ++d) same = same && ro.dims[d] == mesh.dims[d];
   if (!same) raise(TESS_ERR_ARGUMENT, "mesh, dtype or basis differ from the recorded rollout");
+  ctx->corr = 0;
+  ctx->corr_filter = nullptr;
   dispatch<RolloutAdjointOp>(basis, mesh, false, ctx, mesh, G, prm, adj);
 }
 

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "tess_correct.hpp"
 #include "tess_device.hpp"
 #include "tess_internal.hpp"
 
@@ -30,11 +31,15 @@ struct WArgs {
   int64_t* window;
   int64_t n;
   uint32_t* err;
+  int32_t corr;            // corrected basis bits (CORR kernels)
+  const uint8_t* filter;   // node filter of update!(..., filter), NULL = Trues
 };
 
 // Per particle: support window, then either the full-support polynomial fill
-// (bspline.jl:146-170) or the nodewise jet (basis_weight.jl:624-632).
-template <typename T, int D, int BASIS>
+// (bspline.jl:146-170) or the nodewise jet (basis_weight.jl:624-632).  CORR:
+// a corrected basis, whose plain branch is the kernel's and whose other
+// branches correct the nodewise jets (wls.jl:19-128, kernelcorrection.jl:26-35).
+template <typename T, int D, int BASIS, bool CORR = false>
 __global__ __launch_bounds__(256) void k_update_weights(WArgs<T, D> a) {
   constexpr int SW = support_width(BASIS);
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -62,13 +67,20 @@ __global__ __launch_bounds__(256) void k_update_weights(WArgs<T, D> a) {
       return;
     }
   }
-  int lo[D], hi[D];
-  bool full = BASIS != TESS_UGIMP;
+  int lo[D], hi[D], cnt[D];
+  bool whole = true;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     axis_window<T, BASIS>(xi[d], l, a.mesh.dims[d], lo[d], hi[d]);
-    full = full && (hi[d] - lo[d] + 1 == SW);
+    cnt[d] = hi[d] - lo[d] + 1;
+    whole = whole && cnt[d] == SW;
   }
+  bool plain = true;
+  if constexpr (CORR) {
+    const bool fa = filter_alltrue<D, SW>(a.filter, a.mesh.dims, lo, cnt);
+    plain = corr_mode(a.corr, BASIS, whole, a.filter != nullptr, fa, true) == CORR_PLAIN;
+  }
+  const bool full = BASIS != TESS_UGIMP && whole && plain;
   T val[D][SW], der[D][SW];
   if (full) {
     if constexpr (BASIS != TESS_UGIMP) {
@@ -93,6 +105,18 @@ __global__ __launch_bounds__(256) void k_update_weights(WArgs<T, D> a) {
       }
   }
   const T hinv = a.mesh.h_inv;
+  Corrected<T, D, SW> cs;
+  cs.mode = CORR_PLAIN;
+  if constexpr (CORR) {
+    if (!plain) {
+      T xin[D][SW];
+#pragma unroll
+      for (int d = 0; d < D; ++d)
+#pragma unroll
+        for (int k = 0; k < SW; ++k) xin[d][k] = (x[d] - a.mesh.node(d, lo[d] + k)) * hinv;
+      cs.prepare(a.corr, BASIS, true, a.filter, a.mesh.dims, lo, cnt, xin, val, der);
+    }
+  }
 #pragma unroll
   for (int o = 0; o < IPow<SW, D>::v; ++o) {
     int aa[D], r = o;
@@ -104,10 +128,18 @@ __global__ __launch_bounds__(256) void k_update_weights(WArgs<T, D> a) {
       ok = ok && aa[d] <= hi[d] - lo[d];
     }
     if (!ok) continue;
+    const int64_t e = int64_t(o) + int64_t(IPow<SW, D>::v) * p;
+    if (CORR && cs.mode == CORR_GENERAL) {
+      T w, g[D];
+      cs.node(val, aa, filter_pass<D>(a.filter, a.mesh.dims, lo, aa), hinv, w, g);
+      a.w(e) = w;
+#pragma unroll
+      for (int c = 0; c < D; ++c) a.gw(e, c) = g[c];
+      continue;
+    }
     T w = val[0][aa[0]];
 #pragma unroll
     for (int d = 1; d < D; ++d) w = w * val[d][aa[d]];
-    const int64_t e = int64_t(o) + int64_t(IPow<SW, D>::v) * p;
     a.w(e) = w;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
@@ -248,27 +280,37 @@ void update_weights_typed(tess_ctx* ctx, int basis, const tess_mesh& mesh, const
   a.gw = dfield<T>(W.grad_w);
   a.window = W.window;
   a.n = P.n;
+  a.corr = basis & ~0xff;
+  a.filter = W.filter;
   uint32_t* err = ctx->counters.as<uint32_t>(4);
   a.err = err;
   TESS_HIP(hipMemsetAsync(err, 0, sizeof(uint32_t), ctx->stream));
   if (P.n == 0) return;
   const dim3 g(nblk(P.n, 256)), b(256);
-  switch (basis) {
+  const int kernel = basis & 0xff;
+  const bool corr = is_corrected(basis);
+#define TESS_UPD(K)                                                                           \
+  if (corr)                                                                                   \
+    hipLaunchKernelGGL((k_update_weights<T, D, K, true>), g, b, 0, ctx->stream, a);           \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_update_weights<T, D, K>), g, b, 0, ctx->stream, a);
+  switch (kernel) {
     case TESS_BSPLINE_LINEAR:
-      hipLaunchKernelGGL((k_update_weights<T, D, TESS_BSPLINE_LINEAR>), g, b, 0, ctx->stream, a);
+      TESS_UPD(TESS_BSPLINE_LINEAR)
       break;
     case TESS_BSPLINE_QUADRATIC:
-      hipLaunchKernelGGL((k_update_weights<T, D, TESS_BSPLINE_QUADRATIC>), g, b, 0, ctx->stream, a);
+      TESS_UPD(TESS_BSPLINE_QUADRATIC)
       break;
     case TESS_BSPLINE_CUBIC:
-      hipLaunchKernelGGL((k_update_weights<T, D, TESS_BSPLINE_CUBIC>), g, b, 0, ctx->stream, a);
+      TESS_UPD(TESS_BSPLINE_CUBIC)
       break;
     default:
-      hipLaunchKernelGGL((k_update_weights<T, D, TESS_UGIMP>), g, b, 0, ctx->stream, a);
+      TESS_UPD(TESS_UGIMP)
       break;
   }
+#undef TESS_UPD
   TESS_HIP(hipGetLastError());
-  if (basis == TESS_UGIMP) {
+  if (kernel == TESS_UGIMP) {
     uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
     TESS_HIP(hipMemcpyAsync(hc, err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     TESS_HIP(hipStreamSynchronize(ctx->stream));

@@ -108,6 +108,10 @@ struct tess_ctx {
   int64_t kept_binnings = 0;  // binnings that kept the previous layout (no particle changed its key)
   tess::Buffer counters;   // small device scalars
   tess::Buffer g2p2g_x;    // positions before a G2P2G pass (its P2G half's weights)
+  // corrected basis of the current call (TESS_WLS / TESS_KERNEL_CORRECTION /
+  // TESS_POLY_MULTILINEAR bits, 0 for a plain kernel) and its node filter
+  int32_t corr = 0;
+  const uint8_t* corr_filter = nullptr;
   tess::Buffer movers;     // G2P's changed-key counts (spread over cache lines)
   tess::Buffer host_pinned_dummy;
   int64_t* counters_host = nullptr;  // pinned readback slots

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "tess_correct.hpp"
 #include "tess_device.hpp"
 #include "tess_internal.hpp"
 
@@ -124,6 +125,9 @@ struct Args {
   DField<T> gm, gmv, gf, gvn, gv;
   const int64_t* window;
   DField<T> w, gw;
+  // corrected basis of deferred weights (0: the plain kernel) and its node filter
+  int32_t corr;
+  const uint8_t* filter;
   const uint32_t* perm;
   const uint32_t* tile_start;
   const uint32_t* tile_end;
@@ -646,6 +650,28 @@ __device__ __forceinline__ void axis_jets(T xd, int lo_local, const T* s_nodex, 
   }
 }
 
+// Corrected deferred weights of one particle inside a tile kernel: the axis
+// jets (nodewise values at the footprint's node coordinates) become the
+// corrected ones, or the general fit is prepared (src/Transfer/runtime.jl:55-80
+// with the deferred states of wls.jl:131-163, kernelcorrection.jl:41-63).
+// ext = window extent - 1 per axis, ccoord = window start in the footprint.
+template <typename T, int D, int BASIS>
+__device__ __forceinline__ void correct_jets(const Args<T, D>& a, const T* x, const int* lo, const int* ext,
+                                             const int* ccoord, const T (*s_nodex)[Geo<D>::TS],
+                                             Corrected<T, D, support_width(BASIS)>& cs,
+                                             T (&val)[D][support_width(BASIS)], T (&der)[D][support_width(BASIS)]) {
+  constexpr int SW = support_width(BASIS);
+  int cnt[D];
+  T xi[D][SW];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    cnt[d] = ext[d] + 1;
+#pragma unroll
+    for (int k = 0; k < SW; ++k) xi[d][k] = (x[d] - s_nodex[d][ccoord[d] + k]) * a.mesh.h_inv;
+  }
+  cs.prepare(a.corr, BASIS, false, a.filter, a.mesh.dims, lo, cnt, xi, val, der);
+}
+
 // ---- P2G -------------------------------------------------------------------
 
 __device__ __forceinline__ uint64_t lanemask_lt64() { return (uint64_t(1) << (threadIdx.x & 63)) - 1; }
@@ -692,7 +718,7 @@ __device__ __forceinline__ void tile_class_counts(const uint32_t* __restrict__ s
 // the class window in registers, and adds them into the LDS tile.  Lanes hold
 // distinct classes, so every LDS add instruction touches each node at most once
 // and the per-node order of the adds is the fixed program order.
-template <typename T, int D, int BASIS, bool STORED, int NPASS>
+template <typename T, int D, int BASIS, bool STORED, int NPASS, bool CORR = false>
 __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                   uint32_t* __restrict__ misplaced) {
   using G = Geo<D>;
@@ -775,6 +801,8 @@ __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a, const uint32_t* _
 #pragma unroll
         for (int d = 0; d < D; ++d) axis_jets<T, D, BASIS>(x[d], ccoord[d], s_nodex[d], l, hinv, val[d], der[d]);
       }
+      Corrected<T, D, SW> cs;
+      if constexpr (CORR) correct_jets<T, D, BASIS>(a, x, lo, ext, ccoord, s_nodex, cs, val, der);
 #pragma unroll
       for (int s2 = 0; s2 < NSL; ++s2) {
         const int o = pass * NSL + s2;
@@ -794,6 +822,8 @@ __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a, const uint32_t* _
           w = a.w(e);
 #pragma unroll
           for (int c = 0; c < D; ++c) g[c] = a.gw(e, c);
+        } else if (CORR && cs.mode == CORR_GENERAL) {
+          cs.node(val, aa, filter_pass<D>(a.filter, a.mesh.dims, lo, aa), hinv, w, g);
         } else {
           // w = Π_d N_d (left-assoc), ∇w_c = (N'_c Π_{d≠c} N_d) h⁻¹
           w = val[0][aa[0]];
@@ -1622,7 +1652,7 @@ __device__ __forceinline__ void particle_update(const Args<T, D>& a, int64_t p, 
 // One wave per tile, lanes own the tile's classes as in P2G (the walk is
 // coalesced when particles are stored in the layered order).  The footprint's
 // vⁿ and v are staged in LDS once per tile.
-template <typename T, int D, int BASIS, bool STORED>
+template <typename T, int D, int BASIS, bool STORED, bool CORR = false>
 __global__ __launch_bounds__(64) void k_g2p_tile(Args<T, D> a, const uint32_t* __restrict__ skeys) {
   using G = Geo<D>;
   constexpr int SW = support_width(BASIS);
@@ -1687,6 +1717,8 @@ __global__ __launch_bounds__(64) void k_g2p_tile(Args<T, D> a, const uint32_t* _
 #pragma unroll
       for (int d = 0; d < D; ++d) axis_jets<T, D, BASIS>(x[d], ccoord[d], s_nodex[d], l, hinv, val[d], der[d]);
     }
+    Corrected<T, D, SW> cs;
+    if constexpr (CORR) correct_jets<T, D, BASIS>(a, x, lo, ext, ccoord, s_nodex, cs, val, der);
     T dv[D], L[D * D], dx[D];
 #pragma unroll
     for (int c = 0; c < D; ++c) {
@@ -1713,6 +1745,8 @@ __global__ __launch_bounds__(64) void k_g2p_tile(Args<T, D> a, const uint32_t* _
         w = a.w(e);
 #pragma unroll
         for (int c = 0; c < D; ++c) g[c] = a.gw(e, c);
+      } else if (CORR && cs.mode == CORR_GENERAL) {
+        cs.node(val, aa, filter_pass<D>(a.filter, a.mesh.dims, lo, aa), hinv, w, g);
       } else {
         w = val[0][aa[0]];
 #pragma unroll
@@ -1844,6 +1878,14 @@ void fill_binning_ptrs(tess_ctx* ctx, Args<T, D>& a) {
   a.n_inside = b.n_inside;
   a.n_active = b.n_active;
   a.n_nb = b.n_nodeblocks;
+}
+
+// The corrected basis and node filter of the current call (deferred weights
+// only: stored weights already hold the corrected values).
+template <typename T, int D, bool STORED>
+void set_correction(tess_ctx* ctx, Args<T, D>& a) {
+  a.corr = STORED ? 0 : ctx->corr;
+  a.filter = STORED ? nullptr : ctx->corr_filter;
 }
 
 // True when every particle field is Julia-native AoS (components contiguous,
@@ -2059,6 +2101,7 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   Args<T, D> a = make_args<T, D>(mesh, &G, P, W, &prm);
   fill_binning_ptrs<T, D>(ctx, a);
   set_sparse<T, D>(a, mesh, sp);
+  set_correction<T, D, STORED>(ctx, a);
   uint32_t* counters = static_cast<uint32_t*>(ctx->counters.ptr);
   int64_t nnodes = 1;
   for (int d = 0; d < D; ++d) nnodes *= mesh.dims[d];
@@ -2101,6 +2144,9 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
     if constexpr (STORED) {
       hipLaunchKernelGGL((k_p2g_tile<T, D, BASIS, STORED, NPASS>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
+                         ctx->stream, a, sk, misplaced);
+    } else if (a.corr) {
+      hipLaunchKernelGGL((k_p2g_tile<T, D, BASIS, false, NPASS, true>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
                          ctx->stream, a, sk, misplaced);
     } else {
       // 3D: one wave per offset along z; 2D: one wave while the stencil fits
@@ -2161,11 +2207,12 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   Args<T, D> a = make_args<T, D>(mesh, &G, P, W, &prm);
   fill_binning_ptrs<T, D>(ctx, a);
   set_sparse<T, D>(a, mesh, sp);
-  // next-step keys at the sorted positions (deferred weights only)
+  set_correction<T, D, STORED>(ctx, a);
+  // next-step keys at the sorted positions (deferred weights of a plain kernel)
   uint32_t* nk = nullptr;
   uint32_t* stay = nullptr;
   uint32_t* mv = nullptr;  // movers: particles whose key changes
-  if (write_keys && !STORED && P.n > 0) {
+  if (write_keys && !STORED && !a.corr && P.n > 0) {
     nk = ctx->inc_nk.as<uint32_t>(size_t(P.n));
     stay = ctx->inc_spos.as<uint32_t>(size_t(P.n) + 1);
     mv = movers_counters(ctx);
@@ -2184,6 +2231,9 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     if constexpr (STORED) {
       hipLaunchKernelGGL((k_g2p_tile<T, D, BASIS, STORED>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream, a,
                          sk);
+    } else if (a.corr) {
+      hipLaunchKernelGGL((k_g2p_tile<T, D, BASIS, false, true>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
+                         ctx->stream, a, sk);
     } else {
       constexpr int NW = TESS_G2P_WAVES_PER_TILE;
       const uint32_t nt = uint32_t(b.ntiles);

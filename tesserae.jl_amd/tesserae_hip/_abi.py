@@ -8,6 +8,8 @@ from __future__ import annotations
 import ctypes
 import os
 
+TESS_ABI_VERSION = 2  # tess_weights carries the node filter of corrected bases
+
 TESS_OK = 0
 TESS_ERR_ARGUMENT = 1
 TESS_ERR_DIMENSION = 2
@@ -23,6 +25,10 @@ TESS_BSPLINE_LINEAR = 1
 TESS_BSPLINE_QUADRATIC = 2
 TESS_BSPLINE_CUBIC = 3
 TESS_UGIMP = 16
+
+TESS_WLS = 0x100
+TESS_KERNEL_CORRECTION = 0x200
+TESS_POLY_MULTILINEAR = 0x400
 
 LIB_NAME = "libtesserae_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
@@ -78,6 +84,7 @@ class tess_weights(ctypes.Structure):
         ("w", tess_field),
         ("grad_w", tess_field),
         ("window", ctypes.c_void_p),
+        ("filter", ctypes.c_void_p),
     ]
 
 
@@ -230,7 +237,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.tess_abi_version() != 1:
+    if lib.tess_abi_version() != TESS_ABI_VERSION:
         raise ImportError(f"{LIB_NAME} ABI version mismatch")
     if path is None:
         _lib = lib

@@ -18,7 +18,7 @@ import torch
 
 from . import _abi
 from ._abi import (TESS_BSPLINE_CUBIC, TESS_BSPLINE_LINEAR, TESS_BSPLINE_QUADRATIC, TESS_FLOAT32, TESS_FLOAT64,
-                   TESS_UGIMP)
+                   TESS_KERNEL_CORRECTION, TESS_POLY_MULTILINEAR, TESS_UGIMP, TESS_WLS)
 
 
 # ---- exceptions mirroring the reference's host-side errors -----------------
@@ -75,8 +75,17 @@ class Basis:
     name: str
 
     @property
+    def kernel(self) -> int:
+        """Code of the kernel (the basis itself, or the one a WLS / KernelCorrection wraps)."""
+        return self.kind & 0xFF
+
+    @property
+    def corrected(self) -> bool:
+        return bool(self.kind & (TESS_WLS | TESS_KERNEL_CORRECTION))
+
+    @property
     def support_width(self) -> int:
-        return {TESS_BSPLINE_LINEAR: 2, TESS_BSPLINE_QUADRATIC: 3, TESS_BSPLINE_CUBIC: 4, TESS_UGIMP: 3}[self.kind]
+        return {TESS_BSPLINE_LINEAR: 2, TESS_BSPLINE_QUADRATIC: 3, TESS_BSPLINE_CUBIC: 4, TESS_UGIMP: 3}[self.kernel]
 
     def __repr__(self) -> str:
         return self.name
@@ -105,6 +114,51 @@ def BSpline(degree) -> Basis:
 def uGIMP() -> Basis:
     """uGIMP() (src/Basis/gimp.jl:16); particles need the length field ``l``."""
     return Basis(TESS_UGIMP, "uGIMP()")
+
+
+# ---- corrected bases (src/Basis/wls.jl, kernelcorrection.jl, polynomials.jl) ----
+
+class MultiLinear:
+    """Degree tag of the multilinear polynomial (polynomials.jl:1)."""
+
+
+@dataclass(frozen=True)
+class Polynomial:
+    """Polynomial(degree) (polynomials.jl:4-7): Linear() or MultiLinear() for a correction."""
+    degree: object
+
+    def __repr__(self) -> str:
+        return f"Polynomial({type(self.degree).__name__}())"
+
+
+def _correction_polynomial(name: str, poly) -> bool:
+    """True for MultiLinear; the reference's CorrectionPolynomial check (polynomials.jl:9-21)."""
+    if isinstance(poly, Polynomial) and isinstance(poly.degree, Linear):
+        return False
+    if isinstance(poly, Polynomial) and isinstance(poly.degree, MultiLinear):
+        return True
+    raise ArgumentError(f"{name} supports Polynomial(Linear()) and Polynomial(MultiLinear()), got {poly}. "
+                        "A higher-degree basis leaves the weighted least-squares moment matrix singular on a kernel "
+                        "support, producing negative and unbounded weights.")
+
+
+def _wrap(name: str, flag: int, kernel: Basis, poly) -> Basis:
+    if not isinstance(kernel, Basis) or kernel.corrected:
+        raise ArgumentError(f"{name} wraps a kernel (BSpline or uGIMP), got {kernel!r}")
+    ml = _correction_polynomial(name, poly)
+    return Basis(kernel.kind | flag | (TESS_POLY_MULTILINEAR if ml else 0), f"{name}({kernel!r}, {poly!r})")
+
+
+def WLS(kernel: Basis, poly: Polynomial | None = None) -> Basis:
+    """WLS(kernel[, poly]) (wls.jl:1-17): weighted least-squares fit, Polynomial(Linear()) by default."""
+    return _wrap("WLS", TESS_WLS, kernel, Polynomial(Linear()) if poly is None else poly)
+
+
+def KernelCorrection(kernel: Basis, poly: Polynomial | None = None) -> Basis:
+    """KernelCorrection(kernel[, poly]) (kernelcorrection.jl:1-24): the kernel on whole supports, WLS where the
+    mesh or the filter cuts them; Polynomial(MultiLinear()) by default."""
+    return _wrap("KernelCorrection", TESS_KERNEL_CORRECTION, kernel,
+                 Polynomial(MultiLinear()) if poly is None else poly)
 
 
 # ---- mesh (src/Mesh/cartesian.jl:38-103) ----------------------------------
@@ -416,10 +470,19 @@ def generate_particles(mesh: CartesianMesh, n: int | None = None, positions=None
 
 
 class BasisWeights:
-    """BasisWeightArray (src/Basis/basis_weight.jl:345-358) in stored or deferred mode."""
+    """BasisWeightArray (src/Basis/basis_weight.jl:345-358) in stored or deferred mode.
+
+    ``deferred`` weights store nothing (``generate_basis_weights(...; deferred=true)``);
+    stored weights switched over by ``update(..., deferred=True)`` keep their
+    storage but are evaluated inside the transfers until the next plain update
+    (basis_weight.jl:700-721).  ``filter`` is the node mask the last update
+    recorded for a corrected basis (None = Trues).
+    """
 
     def __init__(self, basis: Basis, mesh: CartesianMesh, n: int, deferred: bool = False, device="cuda"):
-        self.basis, self.mesh, self.n, self.deferred = basis, mesh, int(n), deferred
+        self.basis, self.mesh, self.n, self.storageless = basis, mesh, int(n), deferred
+        self.deferring = False
+        self.filter = None
         sw, d = basis.support_width, mesh.dim
         if deferred:
             self.w = self.grad_w = self.window = None
@@ -428,13 +491,21 @@ class BasisWeights:
             self.grad_w = torch.zeros((self.n, sw ** d, d), dtype=mesh.dtype, device=device)
             self.window = torch.zeros((self.n, 2 * d), dtype=torch.int64, device=device)
 
+    @property
+    def deferred(self) -> bool:
+        """Tesserae.isdeferred: transfers evaluate the basis instead of reading stored values."""
+        return self.storageless or self.deferring
+
     def __len__(self) -> int:
         return self.n
 
     def abi(self) -> _abi.tess_weights | None:
-        if self.deferred:
-            return None
+        """The C-ABI view: None for deferred weights without a filter, a storage-less struct carrying the
+        filter for deferred ones with a filter, else the stored arrays (and the filter update! reads)."""
         W = _abi.tess_weights()
+        W.filter = self.filter.data_ptr() if self.filter is not None else None
+        if self.deferred:
+            return W if self.filter is not None else None
         W.w = _field(self.w, 1, "julia")
         W.grad_w = _field(self.grad_w, self.mesh.dim, "julia")
         W.window = self.window.data_ptr()
@@ -592,15 +663,39 @@ def update(obj, *args, ctx: Context | None = None, **kwargs):
     raise ArgumentError(f"update: no method for {type(obj).__name__}")
 
 
-def _update_weights(weights: BasisWeights, particles: Particles, mesh: CartesianMesh, ctx=None, deferred=None):
-    """update!(weights, particles, mesh) (basis_weight.jl:714-752)."""
-    if weights.deferred:
-        if deferred is False:
+def _node_filter(mesh: CartesianMesh, filter, device) -> torch.Tensor | None:
+    """A Bool array of size(mesh) as one byte per node in column-major node order; None for Trues."""
+    if filter is None:
+        return None
+    f = filter.detach().cpu().numpy() if isinstance(filter, torch.Tensor) else np.asarray(filter)
+    if tuple(f.shape) != tuple(mesh.size()):
+        raise TesseraeAssertionError(f"size(mesh) == size(filter): {tuple(mesh.size())} vs {tuple(f.shape)}")
+    return torch.from_numpy(np.ascontiguousarray(f.astype(np.uint8).ravel(order="F"))).to(device)
+
+
+def _update_weights(weights: BasisWeights, particles: Particles, mesh: CartesianMesh, filter=None, ctx=None,
+                    deferred=None):
+    """update!(weights, particles, mesh[, filter]; deferred) (basis_weight.jl:700-752)."""
+    deferred = weights.storageless if deferred is None else bool(deferred)
+    device = particles.x.device
+    if deferred or weights.storageless:
+        # recorded before the early return: deferred corrected bases read it in the transfers
+        weights.filter = _node_filter(mesh, filter, device)
+    if weights.storageless:
+        if not deferred:
             raise ErrorException("update!: these weights were built with `deferred=true` and have no storage for "
-                                 "basis values, so they cannot be materialized.")
+                                 "basis values, so they cannot be materialized. Build them without `deferred=true` "
+                                 "to store the values.")
         return weights
+    if deferred:
+        weights.deferring = True
+        return weights
+    weights.deferring = False
     if len(weights) < len(particles):
         raise TesseraeAssertionError("length(weights) ≥ n")
+    weights.filter = _node_filter(mesh, filter, device)
+    if weights.filter is not None and len(particles) != 0 and not weights.basis.corrected:
+        raise ArgumentError(f"{weights.basis!r} does not support filtered updates")
     _require_device(particles.x, weights.w)
     ctx = ctx or default_context()
     P = particles.abi()

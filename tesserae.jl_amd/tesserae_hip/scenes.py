@@ -138,7 +138,7 @@ def sphere_spin(basis: Basis, dtype=torch.float32, dx: float = 1 / 512, seed: in
     v = v + 0.05 * normal(seed, 3 * n).reshape(n, 3)
     V0 = s ** 3
     params = MPMParams(dt=5e-5, E=5e5, nu=0.35, rho0=1e3, V0=V0)
-    l = np.full(n, dx / 2) if basis.kind == 16 else None
+    l = np.full(n, dx / 2) if basis.kernel == 16 else None
     return Scene("sphere_spin", mesh, basis, params, steps=steps, l=l, **_state(x, v, V0, 3))
 
 

@@ -82,17 +82,34 @@ class HostState:
         return m
 
 
+def host_filter(mesh, filter):
+    """A Bool array of size(mesh) as one byte per node, column-major (None stays None)."""
+    if filter is None:
+        return None
+    f = np.asarray(filter, bool)
+    assert f.shape == tuple(mesh.size())
+    return np.ascontiguousarray(f.astype(np.uint8).ravel(order="F"))
+
+
 class HostWeights:
-    def __init__(self, basis, dim, n, npdt):
+    """Stored weights on the host; deferred=True keeps no storage and carries only the filter."""
+
+    def __init__(self, basis, dim, n, npdt, filter=None, deferred=False):
         sw = basis.support_width
         self.basis = basis
-        self.w = np.zeros((n, sw ** dim), npdt)
-        self.grad_w = np.zeros((n, sw ** dim, dim), npdt)
-        self.window = np.zeros((n, 2 * dim), np.int64)
+        self.deferred = deferred
+        self.filter = filter  # bytes per node (host_filter) or None
+        if not deferred:
+            self.w = np.zeros((n, sw ** dim), npdt)
+            self.grad_w = np.zeros((n, sw ** dim, dim), npdt)
+            self.window = np.zeros((n, 2 * dim), np.int64)
         self.dim = dim
 
     def abi(self):
         W = _abi.tess_weights()
+        W.filter = self.filter.ctypes.data if self.filter is not None else None
+        if self.deferred:
+            return W
         W.w = _field(self.w, 1)
         W.grad_w = _field(self.grad_w.reshape(len(self.w), -1), self.dim)
         W.w.elem_stride, W.w.comp_stride = 1, 1
@@ -116,6 +133,9 @@ class Oracle:
             "oref_last_error": (ctypes.c_char_p, []),
             "oref_basis_weight": (ctypes.c_int, [M, ctypes.c_int, _P(ctypes.c_double), ctypes.c_double, ctypes.c_int,
                                                  _P(_c64), _P(ctypes.c_double), _P(ctypes.c_double)]),
+            "oref_basis_weight_filtered": (ctypes.c_int, [M, ctypes.c_int, _P(ctypes.c_double), ctypes.c_double,
+                                                          ctypes.c_int, ctypes.c_void_p, _P(_c64),
+                                                          _P(ctypes.c_double), _P(ctypes.c_double)]),
             "oref_jet1d": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _P(ctypes.c_double),
                                           _P(ctypes.c_double)]),
             "oref_findblock": (ctypes.c_int, [M, _P(ctypes.c_double), _P(_c64)]),
@@ -143,15 +163,21 @@ class Oracle:
             raise RuntimeError(f"oracle error {status}: {self.lib.oref_last_error().decode()}")
 
     # ---- single-particle helpers
-    def basis_weight(self, mesh, basis, x, l=0.0, deferred=False):
+    def basis_weight(self, mesh, basis, x, l=0.0, deferred=False, filter=None):
+        """(window (d, 2) 1-based, w (sw^d,), ∇w (sw^d, d)) of one particle; filter: Bool array of size(mesh)."""
         d = mesh.dim
         sw = basis.support_width
         xx = (ctypes.c_double * d)(*x)
         win = (_c64 * (2 * d))()
         w = (ctypes.c_double * (sw ** d))()
         gw = (ctypes.c_double * (sw ** d * d))()
-        self._ok(self.lib.oref_basis_weight(ctypes.byref(mesh.abi()), basis.kind, xx, float(l), int(deferred), win, w,
-                                            gw))
+        fb = host_filter(mesh, filter)
+        if fb is None:
+            self._ok(self.lib.oref_basis_weight(ctypes.byref(mesh.abi()), basis.kind, xx, float(l), int(deferred),
+                                                win, w, gw))
+        else:
+            self._ok(self.lib.oref_basis_weight_filtered(ctypes.byref(mesh.abi()), basis.kind, xx, float(l),
+                                                         int(deferred), fb.ctypes.data, win, w, gw))
         return (np.array(win[:], dtype=np.int64).reshape(d, 2), np.array(w[:]),
                 np.array(gw[:]).reshape(sw ** d, d))
 
@@ -173,8 +199,8 @@ class Oracle:
         return [(out[2 * d], out[2 * d + 1]) for d in range(mesh.dim)]
 
     # ---- whole-array operations on HostState
-    def update_weights(self, st: HostState, basis):
-        W = HostWeights(basis, st.dim, st.n, st.npdt)
+    def update_weights(self, st: HostState, basis, filter=None):
+        W = HostWeights(basis, st.dim, st.n, st.npdt, filter=host_filter(st.mesh, filter))
         P = st.particles_abi()
         self._ok(self.lib.oref_update_weights(basis.kind, ctypes.byref(st.mesh_abi()), ctypes.byref(P),
                                               ctypes.byref(W.abi())))
