@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward ratio leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--no-sphere", action="store_true", help="skip the sparse-grid sphere leg")
+    ap.add_argument("--no-taylor", action="store_true", help="skip the Taylor-impact rod (KernelCorrection) leg")
     ap.add_argument("--sphere-dx", type=float, default=1 / 512)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (reference for HBM counter scaling)")
@@ -236,6 +237,45 @@ def sphere_leg(dx, steps):
             "by_basis": out}
 
 
+def taylor_leg(steps):
+    """Taylor-impact rod of the reference's GPU example (docs/src/gpu.md:202-393): 1.48M particles,
+    KernelCorrection(BSpline(Quadratic())) with deferred weights, the rod's bottom layer on the floor (truncated
+    supports: the corrected branch), FP32 and FP64, next to the plain quadratic B-spline on the same rod.  The
+    example's published whole-run times on an RTX 5090 (FP32 8 s, FP64 31 s for ~1.8k iterations) include its
+    von Mises return mapping; this engine runs the README's Neo-Hookean solid, so the comparison covers the
+    transfers and not the material update."""
+    import torch
+
+    import tesserae_hip as th
+    from gpu_util import to_gpu
+    from tesserae_hip.scenes import taylor_rod
+
+    out = {}
+    n = iters = 0
+    for dtype, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        for basis in (th.KernelCorrection(th.BSpline(th.Quadratic())), th.BSpline(th.Quadratic())):
+            sc = taylor_rod(dtype=dtype, basis=basis)
+            n, iters = sc.n, int(round(80e-6 / sc.params.dt))
+            mesh, P, G = to_gpu(sc)
+            ctx = th.Context()
+            th.step(G, P, basis, sc.params, nsteps=3, reorder_threshold=0.85, ctx=ctx)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            a.record()
+            th.step(G, P, basis, sc.params, nsteps=steps, reorder_threshold=0.85, ctx=ctx)
+            b.record()
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b) / steps
+            out[f"{tag} {basis!r}"] = {"ms_per_step": ms, "particle_steps_per_s": n / (ms * 1e-3),
+                                       "projected_run_s": ms * 1e-3 * iters}
+            ctx.close()
+            del P, G
+            torch.cuda.empty_cache()
+    return {"workload": f"Taylor-impact rod, {n} particles, mesh spacing R/12, {iters} iterations to 80 us, "
+                        "Neo-Hookean, deferred weights", "particles": n, "iterations": iters,
+            "reference_rtx5090_run_s": {"f32": 8.0, "f64": 31.0}, "by_config": out}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -337,6 +377,9 @@ def main():
     if not args.no_sphere:
         log("bench: sparse-grid sphere")
         out["sphere"] = sphere_leg(args.sphere_dx, 3)
+    if not args.no_taylor:
+        log("bench: Taylor-impact rod (KernelCorrection)")
+        out["taylor"] = taylor_leg(20)
     if not args.no_adjoint:
         log("bench: adjoint/forward ratio")
         out["adjoint"] = adjoint_ratio(args.adjoint_steps, args.checkpoint_every)

@@ -969,6 +969,56 @@ __device__ __forceinline__ void axis_jets_mixed(T xd, T xmin, int lo_local, int 
   for (int q = 0; q < SW; ++q) der[q] *= hinv;
 }
 
+// Separable corrected bases in the fast kernels: KernelCorrection or WLS with
+// Polynomial(MultiLinear()) over a quadratic or cubic B-spline and no filter
+// (the reference's SeparableKernelCorrection and the MultiLinear WLS,
+// kernelcorrection.jl:53-63, wls.jl:79-99, 166-189).  Every particle is either
+// the plain kernel or a product of per-axis 1-D fits, so the fast kernels only
+// swap the axis jets.
+__host__ __device__ constexpr bool separable_correction(int corr, int basis, bool has_filter) {
+  return (corr & TESS_POLY_MULTILINEAR) && (basis == TESS_BSPLINE_QUADRATIC || basis == TESS_BSPLINE_CUBIC) &&
+         !has_filter;
+}
+
+// Whether a particle's axis jets are corrected: always for WLS, off a whole
+// support for KernelCorrection (kernelcorrection.jl:55-63).
+template <typename T, int D, int BASIS>
+__device__ __forceinline__ bool sep_corrects(const Args<T, D>& a, const T* x) {
+  if (!(a.corr & TESS_KERNEL_CORRECTION)) return true;
+  bool full = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    int lo, hi;
+    axis_window<T, BASIS>((x[d] - a.mesh.xmin[d]) * a.mesh.h_inv, T(0), a.mesh.dims[d], lo, hi);
+    full = full && hi - lo == support_width(BASIS) - 1;
+  }
+  return !full;
+}
+
+// Axis jets of a separable corrected basis: the plain ones, or the 1-D fit of
+// the nodewise values (der in world units, as axis_jets_mixed returns it).
+template <typename T, int BASIS>
+__device__ __forceinline__ void axis_jets_sep(T xd, T xmin, int lo_local, int ext, const T* s_nodex, T hinv,
+                                              bool correct, T* val, T* der) {
+  constexpr int SW = support_width(BASIS);
+  if (!correct) {
+    axis_jets_mixed<T, BASIS>(xd, xmin, lo_local, ext, s_nodex, T(0), hinv, val, der);
+    return;
+  }
+  T dh[SW];
+#pragma unroll
+  for (int q = 0; q < SW; ++q) {
+    const T xi = (xd - s_nodex[lo_local + q]) * hinv;
+    T v0 = T(0), d0 = T(0);
+    if (q <= ext) piecewise_jet<T, BASIS>(xi, T(0), v0, d0);
+    val[q] = v0;
+    dh[q] = -xi;
+  }
+  wls_axis<T, SW>(ext + 1, degree_of(BASIS), dh, val, der);
+#pragma unroll
+  for (int q = 0; q < SW; ++q) der[q] *= hinv;
+}
+
 // One particle's P2G inputs, loaded one iteration ahead of their use.
 template <typename T, int D>
 struct P2GIn {
@@ -995,7 +1045,7 @@ __device__ __forceinline__ void p2g_load(const Args<T, D>& a, uint32_t pid, P2GI
 // the registers of one wave hold 1/NPASS of the stencil, which keeps several
 // waves per SIMD resident).  The waves add their stencils into the LDS tile
 // one after another, so the per-node order of the additions is fixed.
-template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false>
+template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false, bool SEP = false>
 __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                           uint32_t* __restrict__ misplaced) {
   using G = Geo<D>;
@@ -1096,7 +1146,13 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
         const T xi = (x[d] - a.mesh.xmin[d]) * hinv;
         int lo, hi;
         axis_window<T, BASIS>(xi, l, a.mesh.dims[d], lo, hi);
-        axis_jets_mixed<T, BASIS>(x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], l, hinv, jv, jd);
+        if constexpr (SEP)
+          // inactive lanes carry zero mass and stress: they keep the plain jets, whose values are finite
+          // whatever their position (a fit over a window that is not theirs can be singular)
+          axis_jets_sep<T, BASIS>(x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], hinv,
+                                  act && sep_corrects<T, D, BASIS>(a, x), jv, jd);
+        else
+          axis_jets_mixed<T, BASIS>(x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], l, hinv, jv, jd);
       };
       if constexpr (NPASS == 1) {
         axis_jets(std::integral_constant<int, 0>{}, val[0], der[0]);
@@ -1279,7 +1335,7 @@ __device__ __forceinline__ void g2p_load(const Args<T, D>& a, uint32_t pid, G2PI
 #ifndef TESS_G2P_WAVES_PER_TILE
 #define TESS_G2P_WAVES_PER_TILE 4
 #endif
-template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false>
+template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false, bool SEP = false>
 __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD : 1) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                        uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
                                                        uint32_t ntiles, uint32_t* __restrict__ movers) {
@@ -1366,12 +1422,19 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
       }
     }
     T val[D][SW], der[D][SW];
+    bool corrects = false;
+    if constexpr (SEP) corrects = sep_corrects<T, D, BASIS>(a, cur.x);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const T xi = (cur.x[d] - a.mesh.xmin[d]) * hinv;
       int lo, hi;
       axis_window<T, BASIS>(xi, cur.l, a.mesh.dims[d], lo, hi);
-      axis_jets_mixed<T, BASIS>(cur.x[d], a.mesh.xmin[d], cc[d], hi - lo, s_nodex[d], cur.l, hinv, val[d], der[d]);
+      if constexpr (SEP)
+        axis_jets_sep<T, BASIS>(cur.x[d], a.mesh.xmin[d], cc[d], hi - lo, s_nodex[d], hinv, corrects, val[d],
+                                der[d]);
+      else
+        axis_jets_mixed<T, BASIS>(cur.x[d], a.mesh.xmin[d], cc[d], hi - lo, s_nodex[d], cur.l, hinv, val[d],
+                                  der[d]);
     }
     // X = Σ w v_i, DV = Σ w (v_i - vⁿ_i), L = Σ v_i ⊗ ∇w, factored by axis
     T X[D], DV[D], L[D * D];
@@ -2145,18 +2208,33 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     if constexpr (STORED) {
       hipLaunchKernelGGL((k_p2g_tile<T, D, BASIS, STORED, NPASS>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
                          ctx->stream, a, sk, misplaced);
-    } else if (a.corr) {
+    } else if (a.corr && !separable_correction(a.corr, BASIS, a.filter != nullptr)) {
       hipLaunchKernelGGL((k_p2g_tile<T, D, BASIS, false, NPASS, true>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
                          ctx->stream, a, sk, misplaced);
     } else {
       // 3D: one wave per offset along z; 2D: one wave while the stencil fits
       constexpr int NPF = (D == 3 || Pow<SW, D>::v * NV * int(sizeof(T)) / 4 > 96) ? SW : 1;
-      if (julia_layout(P, D))
-        hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, true>), dim3(xcd_grid(b.n_active)), dim3(64 * NPF), 0,
-                           ctx->stream, a, sk, misplaced);
-      else
-        hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, false>), dim3(xcd_grid(b.n_active)), dim3(64 * NPF), 0,
-                           ctx->stream, a, sk, misplaced);
+      const bool aos = julia_layout(P, D);
+      bool done = false;
+      if constexpr (BASIS == TESS_BSPLINE_QUADRATIC || BASIS == TESS_BSPLINE_CUBIC) {
+        if (a.corr) {  // separable corrected basis: the same kernel with corrected axis jets
+          if (aos)
+            hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, true, false, true>), dim3(xcd_grid(b.n_active)),
+                               dim3(64 * NPF), 0, ctx->stream, a, sk, misplaced);
+          else
+            hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, false, false, true>), dim3(xcd_grid(b.n_active)),
+                               dim3(64 * NPF), 0, ctx->stream, a, sk, misplaced);
+          done = true;
+        }
+      }
+      if (!done) {
+        if (aos)
+          hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, true>), dim3(xcd_grid(b.n_active)), dim3(64 * NPF), 0,
+                             ctx->stream, a, sk, misplaced);
+        else
+          hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NPF, false>), dim3(xcd_grid(b.n_active)), dim3(64 * NPF), 0,
+                             ctx->stream, a, sk, misplaced);
+      }
     }
     TESS_HIP(hipGetLastError());
     prof_end(ctx, tok);
@@ -2208,11 +2286,12 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   fill_binning_ptrs<T, D>(ctx, a);
   set_sparse<T, D>(a, mesh, sp);
   set_correction<T, D, STORED>(ctx, a);
-  // next-step keys at the sorted positions (deferred weights of a plain kernel)
+  // next-step keys at the sorted positions (deferred weights in the fast kernel)
+  const bool generic = a.corr && !separable_correction(a.corr, BASIS, a.filter != nullptr);
   uint32_t* nk = nullptr;
   uint32_t* stay = nullptr;
   uint32_t* mv = nullptr;  // movers: particles whose key changes
-  if (write_keys && !STORED && !a.corr && P.n > 0) {
+  if (write_keys && !STORED && !generic && P.n > 0) {
     nk = ctx->inc_nk.as<uint32_t>(size_t(P.n));
     stay = ctx->inc_spos.as<uint32_t>(size_t(P.n) + 1);
     mv = movers_counters(ctx);
@@ -2231,18 +2310,32 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     if constexpr (STORED) {
       hipLaunchKernelGGL((k_g2p_tile<T, D, BASIS, STORED>), dim3(xcd_grid(b.n_active)), dim3(64), 0, ctx->stream, a,
                          sk);
-    } else if (a.corr) {
+    } else if (generic) {
       hipLaunchKernelGGL((k_g2p_tile<T, D, BASIS, false, true>), dim3(xcd_grid(b.n_active)), dim3(64), 0,
                          ctx->stream, a, sk);
     } else {
       constexpr int NW = TESS_G2P_WAVES_PER_TILE;
       const uint32_t nt = uint32_t(b.ntiles);
-      if (aos)
-        hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
-                           ctx->stream, a, sk, nk, stay, nt, mv);
-      else
-        hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, false, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
-                           ctx->stream, a, sk, nk, stay, nt, mv);
+      bool done = false;
+      if constexpr (BASIS == TESS_BSPLINE_QUADRATIC || BASIS == TESS_BSPLINE_CUBIC) {
+        if (a.corr) {  // separable corrected basis
+          if (aos)
+            hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW, false, true>), dim3(xcd_grid(b.n_active)),
+                               dim3(64 * NW), 0, ctx->stream, a, sk, nk, stay, nt, mv);
+          else
+            hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, false, NW, false, true>), dim3(xcd_grid(b.n_active)),
+                               dim3(64 * NW), 0, ctx->stream, a, sk, nk, stay, nt, mv);
+          done = true;
+        }
+      }
+      if (!done) {
+        if (aos)
+          hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
+                             ctx->stream, a, sk, nk, stay, nt, mv);
+        else
+          hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, false, NW>), dim3(xcd_grid(b.n_active)), dim3(64 * NW), 0,
+                             ctx->stream, a, sk, nk, stay, nt, mv);
+      }
     }
     TESS_HIP(hipGetLastError());
     prof_end(ctx, tok);

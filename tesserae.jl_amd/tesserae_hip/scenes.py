@@ -110,6 +110,38 @@ def block_drop(dtype=torch.float32, dx: float = 1 / 256, seed: int = 1, steps: i
                  **_state(x, np.zeros_like(x), V0, 3))
 
 
+def taylor_rod(dtype=torch.float32, radius: float = 0.0032, length: float = 0.0324, per_cell: int = 3,
+               seed: int = 1, speed: float = 227.0, basis: Basis | None = None) -> Scene:
+    """Geometry of the reference's Taylor-impact GPU example (docs/src/gpu.md:202-393): a rod of radius R and
+    length L standing on the floor of a mesh of spacing R/12 over (-3R,3R)^2 x (0,1.1L), moving down at 227 m/s,
+    with the floor a slip wall; copper constants E=117e9, nu=0.35, rho0=8.93e3 and dt = 0.8 h / (c + |v|).
+    Particles sit on a lattice of spacing h/per_cell (per_cell=3 gives the example's ~1.48M particles) jittered
+    by 0.1 of the spacing.  The material is the README's Neo-Hookean solid (the example's von Mises model is
+    outside this engine); the basis is KernelCorrection(BSpline(Quadratic())) unless given."""
+    from .core import KernelCorrection
+    h = radius / 12
+    mesh = CartesianMesh(h, (-3 * radius, 3 * radius), (-3 * radius, 3 * radius), (0, 1.1 * length), dtype=dtype)
+    s = h / per_cell
+    kr = int(math.floor(radius / s))
+    ax = (np.arange(-kr, kr) + 0.5) * s
+    az = (np.arange(int(math.floor(length / s))) + 0.5) * s
+    X, Y = np.meshgrid(ax, ax, indexing="ij")
+    keep = X ** 2 + Y ** 2 < radius ** 2
+    xy = np.stack([X[keep], Y[keep]], axis=1)
+    x = np.concatenate([np.column_stack([xy, np.full(len(xy), z)]) for z in az])
+    n = len(x)
+    x = x + (uniform01(seed, 3 * n).reshape(n, 3) * 2.0 - 1.0) * (0.1 * s)
+    E, nu, rho0 = 117e9, 0.35, 8.93e3
+    lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+    dt = 0.8 * h / (math.sqrt((lam + 2 * mu) / rho0) + speed)
+    V0 = math.pi * radius ** 2 * length / n
+    params = MPMParams(dt=dt, E=E, nu=nu, rho0=rho0, V0=V0, slip_walls=1 << 4)
+    v = np.zeros_like(x)
+    v[:, 2] = -speed
+    return Scene("taylor_rod", mesh, basis or KernelCorrection(BSpline(Quadratic())), params,
+                 **_state(x, v, V0, 3))
+
+
 def sphere_spin(basis: Basis, dtype=torch.float32, dx: float = 1 / 512, seed: int = 7, steps: int = 50,
                 block_size_log2: int = 3, radius: float = 0.3) -> Scene:
     """3D sparse-grid sphere: 2x2x2 particles per cell of [0,1]^3 (spacing dx/2) inside the
