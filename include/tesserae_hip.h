@@ -349,6 +349,55 @@ tess_status tess_batch_step(tess_ctx* ctx, tess_batch* batch, int32_t nsteps);
 tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_per_variant, int32_t* variant);
 tess_status tess_batch_destroy(tess_batch* batch);
 
+/* ---- output (src/IO/export.jl) ------------------------------------------ */
+
+/* Component layouts of export.jl's vtk_format (:73-85) for one field. */
+enum {
+  TESS_VTK_SCALAR = 1,     /* one component */
+  TESS_VTK_VECTOR = 2,     /* dim components (a Vec) */
+  TESS_VTK_POINTS = 3,     /* dim components padded with zeros to 3 (coordinates, resize(x, 3)) */
+  TESS_VTK_TENSOR = 4,     /* dim*dim components, column-major (a SecondOrderTensor) */
+  TESS_VTK_SYMMETRIC = 5   /* (xx, yy, zz, xy, yz, zx) of a symmetric tensor held in full dim*dim storage;
+                              in 2D (xx, yy, NaN, xy, NaN, NaN) (export.jl:79-81) */
+};
+
+/* f32(vtk_format(field)) (export.jl:66-85, 104-107) on the device: n elements
+ * of a particle or grid field of type dtype (any strides) become Float32
+ * components in the layout `kind`, written to dst (host or device memory,
+ * n * components floats) with one copy.  Synchronises. */
+tess_status tess_export_field(tess_ctx* ctx, int32_t dtype, int32_t dim, tess_field field, int64_t n, int32_t kind,
+                              float* dst);
+
+/* One named point-data array of the writers below (vtk[name] = data):
+ * n x ncomp Float32, the components of one point contiguous. */
+typedef struct tess_vtk_array {
+  const char* name;
+  const float* data;
+  int32_t ncomp;
+  int32_t reserved;
+} tess_vtk_array;
+
+/* openvtk(path, x::AbstractVector{<:Vec}) with point data, then close
+ * (export.jl:47-52, 89-99): "path.vtu", an UnstructuredGrid of n VTK_VERTEX
+ * cells at xyz (n x 3 Float32).  Data are appended raw, zlib-compressed when
+ * compress != 0 (vtkZLibDataCompressor, UInt64 headers).  Host only. */
+tess_status tess_vtk_write_points(const char* path, int64_t n, const float* xyz, int32_t narrays,
+                                  const tess_vtk_array* arrays, int32_t compress);
+/* openvtk(path, mesh::CartesianMesh) (export.jl:53-55): "path.vts", a
+ * StructuredGrid of the mesh nodes as Float32 points, point data over the
+ * nodes in column-major order.  Host only. */
+tess_status tess_vtk_write_mesh(const char* path, const tess_mesh* mesh, int32_t narrays,
+                                const tess_vtk_array* arrays, int32_t compress);
+/* openvtm (export.jl:101): "path.vtm", a vtkMultiBlockDataSet of the given
+ * block files (paths as they are to be written in the file).  Host only. */
+tess_status tess_vtm_write(const char* path, int32_t nblocks, const char* const* files);
+/* openpvd + pvd[t] = file (export.jl:102): "path.pvd", a ParaView collection
+ * of (time, file) entries.  Host only. */
+tess_status tess_pvd_write(const char* path, int32_t n, const double* times, const char* const* files);
+/* write_ply(name, xs) (export.jl:115-132): "name.ply" with the header lines and
+ * float x, y, z per vertex, binary little-endian (xyz: n x 3).  Host only. */
+tess_status tess_ply_write(const char* name, int64_t n, const float* xyz);
+
 /* ---- measurement (no counterpart in the reference) ----------------------- */
 
 /* Per-kernel timing with HIP events recorded on the context's stream around

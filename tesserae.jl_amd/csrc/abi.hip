@@ -251,7 +251,8 @@ tess_status tess_ctx_destroy(tess_ctx* ctx) {
                             &ctx->inc_mks,   &ctx->inc_mr,    &ctx->inc_mjs,
                             &ctx->sp_occ,    &ctx->sp_act,    &ctx->sp_scan,
                             &ctx->adj_partials, &ctx->adj_nodes, &ctx->ro_checkpoints, &ctx->ro_segment,
-                            &ctx->ro_gradv,  &ctx->ro_snapshots, &ctx->movers, &ctx->g2p2g_x};
+                            &ctx->ro_gradv,  &ctx->ro_snapshots, &ctx->movers, &ctx->g2p2g_x,
+                            &ctx->export_buf};
     for (auto* b : bufs) b->release();
     for (auto ev : ctx->prof.pool) (void)hipEventDestroy(ev);
     if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
@@ -556,6 +557,58 @@ tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_
 tess_status tess_batch_destroy(tess_batch* batch) {
   return guarded([&] {
     if (batch) run_batch_destroy(batch);
+  });
+}
+
+tess_status tess_export_field(tess_ctx* ctx, int32_t dtype, int32_t dim, tess_field field, int64_t n, int32_t kind,
+                              float* dst) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (dtype != TESS_FLOAT32 && dtype != TESS_FLOAT64) raise(TESS_ERR_ARGUMENT, "dtype");
+    if (dim != 2 && dim != 3) raise(TESS_ERR_UNSUPPORTED, "dim must be 2 or 3");
+    if (n < 0) raise(TESS_ERR_DIMENSION, "negative element count");
+    if (n > 0 && (!field.ptr || !dst)) raise(TESS_ERR_ARGUMENT, "field or destination is NULL");
+    run_export_field(ctx, dtype, dim, field, n, kind, dst);
+  });
+}
+
+tess_status tess_vtk_write_points(const char* path, int64_t n, const float* xyz, int32_t narrays,
+                                  const tess_vtk_array* arrays, int32_t compress) {
+  return guarded([&] {
+    if (!path) raise(TESS_ERR_ARGUMENT, "path is NULL");
+    if (n < 0) raise(TESS_ERR_DIMENSION, "negative point count");
+    if (n > 0 && !xyz) raise(TESS_ERR_ARGUMENT, "point coordinates are NULL");
+    run_vtk_write_points(path, n, xyz, narrays, arrays, compress);
+  });
+}
+
+tess_status tess_vtk_write_mesh(const char* path, const tess_mesh* mesh, int32_t narrays,
+                                const tess_vtk_array* arrays, int32_t compress) {
+  return guarded([&] {
+    if (!path) raise(TESS_ERR_ARGUMENT, "path is NULL");
+    check_mesh(mesh);
+    run_vtk_write_mesh(path, *mesh, narrays, arrays, compress);
+  });
+}
+
+tess_status tess_vtm_write(const char* path, int32_t nblocks, const char* const* files) {
+  return guarded([&] {
+    if (!path) raise(TESS_ERR_ARGUMENT, "path is NULL");
+    run_vtm_write(path, nblocks, files);
+  });
+}
+
+tess_status tess_pvd_write(const char* path, int32_t n, const double* times, const char* const* files) {
+  return guarded([&] {
+    if (!path) raise(TESS_ERR_ARGUMENT, "path is NULL");
+    run_pvd_write(path, n, times, files);
+  });
+}
+
+tess_status tess_ply_write(const char* name, int64_t n, const float* xyz) {
+  return guarded([&] {
+    if (!name) raise(TESS_ERR_ARGUMENT, "name is NULL");
+    run_ply_write(name, n, xyz);
   });
 }
 

@@ -108,6 +108,7 @@ struct tess_ctx {
   int64_t kept_binnings = 0;  // binnings that kept the previous layout (no particle changed its key)
   tess::Buffer counters;   // small device scalars
   tess::Buffer g2p2g_x;    // positions before a G2P2G pass (its P2G half's weights)
+  tess::Buffer export_buf; // Float32 staging of tess_export_field
   // corrected basis of the current call (TESS_WLS / TESS_KERNEL_CORRECTION /
   // TESS_POLY_MULTILINEAR bits, 0 for a plain kernel) and its node filter
   int32_t corr = 0;
@@ -205,6 +206,15 @@ void run_rollout(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_gri
 void run_rollout_adjoint(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G,
                          const tess_params& prm, const tess_adjoint& adj);
 // ---- batches (batch.hip) ----
+void run_export_field(tess_ctx* ctx, int32_t dtype, int32_t dim, tess_field field, int64_t n, int32_t kind,
+                      float* dst);
+void run_vtk_write_points(const char* path, int64_t n, const float* xyz, int32_t narrays,
+                          const tess_vtk_array* arrays, int32_t compress);
+void run_vtk_write_mesh(const char* path, const tess_mesh& mesh, int32_t narrays, const tess_vtk_array* arrays,
+                        int32_t compress);
+void run_vtm_write(const char* path, int32_t nblocks, const char* const* files);
+void run_pvd_write(const char* path, int32_t n, const double* times, const char* const* files);
+void run_ply_write(const char* name, int64_t n, const float* xyz);
 void run_batch_create(tess_ctx* ctx, int basis, int nscenes, const tess_scene* scenes, tess_batch** out);
 void run_batch_step(tess_ctx* ctx, tess_batch* b, int nsteps);
 void run_batch_set_variant(tess_batch* b, int variant);

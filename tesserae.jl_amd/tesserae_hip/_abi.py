@@ -26,6 +26,12 @@ TESS_BSPLINE_QUADRATIC = 2
 TESS_BSPLINE_CUBIC = 3
 TESS_UGIMP = 16
 
+TESS_VTK_SCALAR = 1
+TESS_VTK_VECTOR = 2
+TESS_VTK_POINTS = 3
+TESS_VTK_TENSOR = 4
+TESS_VTK_SYMMETRIC = 5
+
 TESS_WLS = 0x100
 TESS_KERNEL_CORRECTION = 0x200
 TESS_POLY_MULTILINEAR = 0x400
@@ -128,8 +134,26 @@ class tess_scene(ctypes.Structure):
 
 
 # Every symbol include/tesserae_hip.h declares, with its ctypes signature.
+class tess_vtk_array(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char_p),
+        ("data", ctypes.c_void_p),
+        ("ncomp", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 _P = ctypes.POINTER
 SIGNATURES = {
+    "tess_export_field": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, tess_field, ctypes.c_int64,
+                                           ctypes.c_int32, ctypes.c_void_p]),
+    "tess_vtk_write_points": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32,
+                                               _P(tess_vtk_array), ctypes.c_int32]),
+    "tess_vtk_write_mesh": (ctypes.c_int32, [ctypes.c_char_p, _P(tess_mesh), ctypes.c_int32, _P(tess_vtk_array),
+                                             ctypes.c_int32]),
+    "tess_vtm_write": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int32, _P(ctypes.c_char_p)]),
+    "tess_pvd_write": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int32, _P(ctypes.c_double), _P(ctypes.c_char_p)]),
+    "tess_ply_write": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_void_p]),
     "tess_abi_version": (ctypes.c_int32, []),
     "tess_last_error": (ctypes.c_char_p, []),
     "tess_ctx_create": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_void_p, _P(ctypes.c_void_p)]),
