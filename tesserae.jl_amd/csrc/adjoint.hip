@@ -660,7 +660,10 @@ void step_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G,
 //
 // While a segment is replayed, each step's binning arrays and P2G grid are
 // copied aside, so that the reversed step restores them instead of binning and
-// scattering again.  One launch copies every array (4-byte words).
+// scattering again.  One launch copies the binning arrays (4-byte words); the
+// grid is copied only on the node blocks of the step's list (nb_list), the
+// only nodes the reversed step reads: tile footprints lie inside them, and the
+// next forward step zeroes the whole grid before scattering.
 
 struct CopyTask {
   const uint32_t* src;
@@ -687,43 +690,80 @@ int64_t field_words(const tess_field& f, int64_t n, int nc) {
 
 // The arrays of one snapshot: binning (perm, sorted keys, tile runs and lists)
 // and the grid fields m, mv, f, vⁿ, v.
-template <typename T, int D>
-int snapshot_arrays(tess_ctx* ctx, const tess_grid& G, int64_t n, int64_t ntiles, int64_t nnodes, void* ptrs[14],
-                    int64_t words[14]) {
+int snapshot_arrays(tess_ctx* ctx, int64_t n, int64_t ntiles, void* ptrs[9], int64_t words[9]) {
   tess::Buffer* bufs[] = {&ctx->vals,      &ctx->keys_sorted, &ctx->tile_start, &ctx->tile_end, &ctx->active_tiles,
                           &ctx->tile_flag, &ctx->tile_slot,   &ctx->nb_list,    &ctx->nb_flag};
   const int64_t lens[] = {n, n, ntiles, ntiles, ntiles, ntiles, ntiles, ntiles, ntiles};
-  int k = 0;
   for (int i = 0; i < 9; ++i) {
-    ptrs[k] = bufs[i]->ptr;
-    words[k++] = lens[i];
+    ptrs[i] = bufs[i]->ptr;
+    words[i] = lens[i];
   }
-  const tess_field* gf[] = {&G.m, &G.mv, &G.f, &G.vn, &G.v};
-  const int gc[] = {1, D, D, D, D};
-  for (int i = 0; i < 5; ++i) {
-    ptrs[k] = gf[i]->ptr;
-    words[k++] = field_words<T>(*gf[i], nnodes, gc[i]);
-  }
-  return k;
+  return 9;
+}
+
+// words of the binning part, rounded up to keep the grid part 8-byte aligned
+inline int64_t snapshot_bin_words(int64_t n, int64_t ntiles) { return (2 * n + 7 * ntiles + 1) & ~int64_t(1); }
+
+// node values of one snapshot's grid part: m, mv, f, vⁿ, v of up to every node
+// block (BW^D nodes each)
+template <typename T, int D>
+int64_t snapshot_grid_words(int64_t ntiles) {
+  return ntiles * Pow<Geo<D>::BW, D>::v * (1 + 4 * D) * int64_t(sizeof(T) / 4);
 }
 
 template <typename T, int D>
-int64_t snapshot_words(tess_ctx* ctx, const tess_grid& G, int64_t n, int64_t ntiles, int64_t nnodes) {
-  void* p[14];
-  int64_t w[14];
-  const int k = snapshot_arrays<T, D>(ctx, G, n, ntiles, nnodes, p, w);
-  int64_t t = 0;
-  for (int i = 0; i < k; ++i) t += w[i];
-  return t;
+int64_t snapshot_words(int64_t n, int64_t ntiles) {
+  return snapshot_bin_words(n, ntiles) + snapshot_grid_words<T, D>(ntiles);
 }
 
-// save (to_store) or restore the snapshot at store
+// Grid values of the node blocks in nb_list to (save) or from (restore) a
+// compact store: node r of the j-th listed block at (j*BW^D + r)*(1+4D).
 template <typename T, int D>
-void snapshot_copy(tess_ctx* ctx, const tess_grid& G, int64_t n, int64_t ntiles, int64_t nnodes, uint32_t* store,
-                   bool to_store) {
-  void* p[14];
-  int64_t w[14];
-  const int k = snapshot_arrays<T, D>(ctx, G, n, ntiles, nnodes, p, w);
+__global__ __launch_bounds__(64) void k_snap_grid(Args<T, D> a, T* __restrict__ store, int to_store) {
+  using G = Geo<D>;
+  constexpr int NV = 1 + 4 * D;
+  const int64_t j = blockIdx.x;
+  if (j >= a.n_nb) return;
+  int Bi[D], I[D];
+  unlin<D>(a.nb_list[j], a.ntiles, Bi);
+  int r = int(threadIdx.x);
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    I[d] = Bi[d] * G::BW + r % G::BW;
+    r /= G::BW;
+    if (I[d] >= a.mesh.dims[d]) return;
+  }
+  const int64_t i = a.mesh.lin(I);
+  T* s = store + (j * int64_t(blockDim.x) + threadIdx.x) * NV;
+  if (to_store) {
+    s[0] = a.gm(i);
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      s[1 + c] = a.gmv(i, c);
+      s[1 + D + c] = a.gf(i, c);
+      s[1 + 2 * D + c] = a.gvn(i, c);
+      s[1 + 3 * D + c] = a.gv(i, c);
+    }
+  } else {
+    a.gm(i) = s[0];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      a.gmv(i, c) = s[1 + c];
+      a.gf(i, c) = s[1 + D + c];
+      a.gvn(i, c) = s[1 + 2 * D + c];
+      a.gv(i, c) = s[1 + 3 * D + c];
+    }
+  }
+}
+
+// save (to_store) or restore the snapshot at store; ctx->bin describes the
+// step whose binning is saved or restored
+template <typename T, int D>
+void snapshot_copy(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, int64_t n, int64_t ntiles,
+                   uint32_t* store, bool to_store) {
+  void* p[9];
+  int64_t w[9];
+  const int k = snapshot_arrays(ctx, n, ntiles, p, w);
   CopyTable tab;
   std::memset(static_cast<void*>(&tab), 0, sizeof(tab));
   int64_t off = 0, most = 0;
@@ -735,10 +775,22 @@ void snapshot_copy(tess_ctx* ctx, const tess_grid& G, int64_t n, int64_t ntiles,
     off += w[i];
     most = std::max(most, w[i]);
   }
-  if (most == 0) return;
-  const unsigned gx = unsigned(std::min<int64_t>(blocks_for(most, 256), 2048));
-  hipLaunchKernelGGL(k_copy_multi, dim3(gx, unsigned(k)), dim3(256), 0, ctx->stream, tab);
-  TESS_HIP(hipGetLastError());
+  if (most > 0) {
+    const unsigned gx = unsigned(std::min<int64_t>(blocks_for(most, 256), 2048));
+    hipLaunchKernelGGL(k_copy_multi, dim3(gx, unsigned(k)), dim3(256), 0, ctx->stream, tab);
+    TESS_HIP(hipGetLastError());
+  }
+  const Binning& b = ctx->bin;
+  if (b.n_nodeblocks > 0) {  // after the binning part: a restore reads the restored nb_list
+    tess_particles none;
+    std::memset(static_cast<void*>(&none), 0, sizeof(none));
+    Args<T, D> a = make_args<T, D>(mesh, &G, none, nullptr, nullptr);
+    fill_binning_ptrs<T, D>(ctx, a);
+    T* gstore = reinterpret_cast<T*>(store + snapshot_bin_words(n, ntiles));
+    hipLaunchKernelGGL((k_snap_grid<T, D>), dim3(unsigned(b.n_nodeblocks)), dim3(Pow<Geo<D>::BW, D>::v), 0,
+                       ctx->stream, a, gstore, to_store ? 1 : 0);
+    TESS_HIP(hipGetLastError());
+  }
 }
 
 // ---- checkpointed rollouts -----------------------------------------------------
@@ -854,12 +906,12 @@ void rollout_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid&
       step_impl<T, D, BASIS, false>(ctx, mesh, G, S, prm, 1, 0.0, nullptr);
       // the context now holds the binning of state k-1 and G its P2G grid
       if (snap_words < 0) {
-        const int64_t w = snapshot_words<T, D>(ctx, G, n, ctx->bin.ntiles, nnodes);
+        const int64_t w = snapshot_words<T, D>(n, ctx->bin.ntiles);
         snap_words = w * 4 * int64_t(ro.every - 1) <= kSnapBytes ? w : 0;
         if (snap_words > 0) snaps = ctx->ro_snapshots.as<uint32_t>(size_t(snap_words) * size_t(ro.every - 1));
       }
       if (snap_words > 0) {
-        snapshot_copy<T, D>(ctx, G, n, ctx->bin.ntiles, nnodes, snaps + snap_words * (k - 1), true);
+        snapshot_copy<T, D>(ctx, mesh, G, n, ctx->bin.ntiles, snaps + snap_words * (k - 1), true);
         ro.snap_bins[size_t(k - 1)] = ctx->bin;
       }
     }
@@ -868,7 +920,7 @@ void rollout_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid&
       if (snap_words > 0 && k < len - 1) {
         Binning& b = ctx->bin;
         b = ro.snap_bins[size_t(k)];
-        snapshot_copy<T, D>(ctx, G, n, b.ntiles, nnodes, snaps + snap_words * k, false);
+        snapshot_copy<T, D>(ctx, mesh, G, n, b.ntiles, snaps + snap_words * k, false);
         b.valid = true;
         b.keys_ready = false;
         b.x_ptr = S.x.ptr;
