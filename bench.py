@@ -55,6 +55,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--no-sphere", action="store_true", help="skip the sparse-grid sphere leg")
     ap.add_argument("--no-taylor", action="store_true", help="skip the Taylor-impact rod (KernelCorrection) leg")
+    ap.add_argument("--no-g2p2g", action="store_true", help="skip the G2P2G leg")
     ap.add_argument("--sphere-dx", type=float, default=1 / 512)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (reference for HBM counter scaling)")
@@ -237,6 +238,37 @@ def sphere_leg(dx, steps):
             "by_basis": out}
 
 
+def g2p2g_leg(sc, passes, layout, reorder):
+    """@G2P2G of the README program on the block drop, as the time loop of a program written as one @G2P2G
+    block (tess_g2p2g_steps, deferred weights, the step loop's reorder threshold): one P2G, warm-up passes, then
+    `passes` timed passes in one call, each the G2P half, then the P2G and grid-node halves with the weights at
+    the positions before the pass."""
+    import torch
+
+    import tesserae_hip as th
+    from gpu_util import to_gpu
+
+    mesh, P, G = to_gpu(sc, layout=layout)
+    ctx = th.Context()
+    th.P2G(G, P, sc.basis, sc.params, ctx=ctx)
+    th.g2p2g_steps(G, P, sc.basis, sc.params, nsteps=3, reorder_threshold=reorder, ctx=ctx)
+    ctx.profile(True)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
+    th.g2p2g_steps(G, P, sc.basis, sc.params, nsteps=passes, reorder_threshold=reorder, ctx=ctx)
+    b.record()
+    torch.cuda.synchronize()
+    prof = ctx.profile_read()
+    ctx.close()
+    ms = a.elapsed_time(b) / passes
+    del P, G
+    torch.cuda.empty_cache()
+    return {"workload": f"block drop dx=1/256, {sc.n} particles, quadratic, FP32, G2P2G passes", "passes": passes,
+            "ms_per_pass": ms, "particle_steps_per_s": sc.n / (ms * 1e-3),
+            "kernels_avg_us": {k: v[0] / max(v[1], 1) * 1e3 for k, v in prof.items()}}
+
+
 def taylor_leg(steps):
     """Taylor-impact rod of the reference's GPU example (docs/src/gpu.md:202-393): 1.48M particles,
     KernelCorrection(BSpline(Quadratic())) with deferred weights, the rod's bottom layer on the floor (truncated
@@ -371,6 +403,9 @@ def main():
                     "reorders": stats["reorders"]},
         "wall_s": wall,
     }
+    if not args.no_g2p2g:
+        log("bench: G2P2G passes")
+        out["g2p2g"] = g2p2g_leg(sc, 10, args.layout, args.reorder)
     if not args.no_batch:
         log("bench: batched scenes")
         out["batch"] = batch_leg(20)

@@ -224,6 +224,16 @@ tess_status tess_g2p2g(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, cons
                        const tess_particles* particles, const tess_weights* weights,
                        const tess_params* params);
 
+/* nsteps passes of tess_g2p2g with deferred weights: the time loop of a
+ * program written as one @G2P2G block (g2p2g.jl:100-159).  Within the call
+ * each pass bins with the keys the previous pass's G2P half wrote, and the
+ * particles are reordered into the engine's tile order at reorder_threshold
+ * exactly as tess_step does (the role of reorder_particles! around the
+ * reference's loop).  Reordering permutes the caller's particle arrays. */
+tess_status tess_g2p2g_steps(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                             const tess_particles* particles, const tess_params* params, int32_t nsteps,
+                             double reorder_threshold);
+
 /* One README step with deferred weights: update!(wgts,…) + @P2G + @G2P
  * (README.md:66-87), repeated nsteps times on the context's stream.
  * reorder_threshold plays the role of the reference loop's

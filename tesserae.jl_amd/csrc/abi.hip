@@ -352,6 +352,23 @@ tess_status tess_g2p(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const 
   });
 }
 
+tess_status tess_g2p2g_steps(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
+                             const tess_particles* particles, const tess_params* params, int32_t nsteps,
+                             double reorder_threshold) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_mesh(mesh);
+    check_basis(basis);
+    check_grid(grid);
+    check_particles(particles, basis, true);
+    check_params(params);
+    if (nsteps < 0) raise(TESS_ERR_ARGUMENT, "nsteps must be non-negative");
+    if (!(reorder_threshold >= 0.0 && reorder_threshold <= 1.0))
+      raise(TESS_ERR_ARGUMENT, "threshold must be in [0, 1].");
+    run_g2p2g_steps(ctx, basis, *mesh, *grid, *particles, *params, nsteps, reorder_threshold);
+  });
+}
+
 tess_status tess_step(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
                       const tess_particles* particles, const tess_params* params, int32_t nsteps,
                       double reorder_threshold) {

@@ -189,6 +189,8 @@ void run_g2p(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G
              const tess_weights* W, const tess_params& prm);
 void run_g2p2g(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
                const tess_weights* W, const tess_params& prm);
+void run_g2p2g_steps(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+                     const tess_params& prm, int nsteps, double reorder_threshold);
 void run_step(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
               const tess_params& prm, int nsteps, double reorder_threshold);
 // The same on an SpGrid: grid fields are SpArray data vectors addressed through sp.

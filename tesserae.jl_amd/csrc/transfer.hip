@@ -34,6 +34,12 @@ void run_g2p2g(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid&
   dispatch<G2P2GOp>(basis, mesh, W != nullptr, ctx, mesh, G, P, W, prm);
 }
 
+void run_g2p2g_steps(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+                     const tess_params& prm, int nsteps, double reorder_threshold) {
+  bind_basis(ctx, basis, nullptr);
+  dispatch<G2P2GStepsOp>(basis, mesh, false, ctx, mesh, G, P, prm, nsteps, reorder_threshold);
+}
+
 void run_step(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
               const tess_params& prm, int nsteps, double reorder_threshold) {
   bind_basis(ctx, basis, nullptr);

@@ -2449,6 +2449,45 @@ void g2p2g_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   ctx->bin.valid = false;  // the particles have moved
 }
 
+// nsteps @G2P2G passes of the README program (deferred weights), the time loop
+// of a program written as one @G2P2G block: each pass bins at the positions
+// before it (from the second pass on with the keys the previous pass's G2P
+// wrote), keeps those positions for its P2G half, and reorders the particles
+// into the layered order at the threshold, as step_impl does.
+template <typename T, int D, int BASIS>
+void g2p2g_steps_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+                      const tess_params& prm, int nsteps, double reorder_threshold) {
+  for (int s = 0; s < nsteps; ++s) {
+    bin_particles<T, D, BASIS, false>(ctx, mesh, P, nullptr, s > 0);
+    const Binning& b = ctx->bin;
+    if (reorder_threshold > 0.0 && b.n_inside > 0) {
+      bool go = reorder_threshold >= 1.0;
+      if (!go && ctx->last_misplaced >= 0) {
+        const double score = 1.0 - double(ctx->last_misplaced) / double(b.n_inside);
+        go = score < reorder_threshold;
+      }
+      if (go) {
+        layered_reorder<T, D>(ctx, mesh, P);
+        ctx->reorders += 1;
+      }
+    }
+    tess_particles Pold = P;
+    if (P.n > 0) {
+      T* xo = ctx->g2p2g_x.as<T>(size_t(P.n) * D);
+      Args<T, D> a = make_args<T, D>(mesh, nullptr, P, nullptr, nullptr);
+      hipLaunchKernelGGL((k_keep_positions<T, D>), dim3(blocks_for(P.n, 256)), dim3(256), 0, ctx->stream, a, xo);
+      TESS_HIP(hipGetLastError());
+      Pold.x = tess_field{xo, D, 1};
+    }
+    g2p_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, nullptr, s + 1 < nsteps);
+    ctx->bin.valid = true;  // the binning describes the old positions the P2G half reads
+    p2g_launch<T, D, BASIS, false>(ctx, mesh, G, Pold, nullptr, prm, reorder_threshold > 0.0, nullptr);
+    ctx->bin.valid = false;
+    // the storage order the P2G half measured is that of the particles themselves
+    if (reorder_threshold > 0.0) ctx->misplaced_x_ptr = P.x.ptr;
+  }
+}
+
 // nsteps of the README loop body (deferred weights).  With reorder_threshold
 // in (0, 1], particles are moved into the layered order when the fraction
 // stored in place, measured by the previous step's P2G, is below the
@@ -2531,6 +2570,13 @@ struct G2P2GOp {
   static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
                   const tess_weights* W, const tess_params& prm) {
     g2p2g_impl<T, D, B, S>(ctx, m, G, P, W, prm);
+  }
+};
+template <typename T, int D, int B, bool S>
+struct G2P2GStepsOp {
+  static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
+                  const tess_params& prm, int nsteps, double thr) {
+    if constexpr (!S) g2p2g_steps_impl<T, D, B>(ctx, m, G, P, prm, nsteps, thr);
   }
 };
 template <typename T, int D, int B, bool S>

@@ -8,7 +8,7 @@ from .core import (Adjoint, ArgumentError, Batch, Basis, BasisWeights, BSpline, 
                    DimensionMismatch, ErrorException, G2P, G2P2G, Grid, KernelCorrection, Linear, MPMParams, MultiLinear,
                    P2G, Particles, Partition, Polynomial, WLS,
                    Quadratic, SpArray, SpGrid, SpIndices, TesseraeAssertionError, TesseraeError,
-                   UnsupportedError, default_context, generate_basis_weights, generate_grid, generate_particles,
+                   UnsupportedError, default_context, g2p2g_steps, generate_basis_weights, generate_grid, generate_particles,
                    grid_sampling_points, reorder_particles, sparse_storage_index, step, uGIMP, update,
                    update_sparsity, step_adjoint, rollout, rollout_adjoint)
 from ._abi import LIB_PATH, load_library
@@ -19,6 +19,6 @@ __all__ = [
     "DimensionMismatch", "ErrorException", "G2P", "G2P2G", "Grid", "Linear", "MPMParams", "P2G", "Particles", "Partition",
     "Quadratic", "SpArray", "SpGrid", "SpIndices", "sparse_storage_index", "update_sparsity",
     "TesseraeAssertionError", "TesseraeError", "UnsupportedError", "default_context",
-    "generate_basis_weights", "generate_grid", "generate_particles", "grid_sampling_points", "reorder_particles",
+    "g2p2g_steps", "generate_basis_weights", "generate_grid", "generate_particles", "grid_sampling_points", "reorder_particles",
     "step", "uGIMP", "update", "LIB_PATH", "load_library",
 ]

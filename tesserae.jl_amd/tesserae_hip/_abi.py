@@ -188,6 +188,11 @@ SIGNATURES = {
         [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_particles), _P(tess_weights),
          _P(tess_params)],
     ),
+    "tess_g2p2g_steps": (
+        ctypes.c_int32,
+        [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_particles), _P(tess_params),
+         ctypes.c_int32, ctypes.c_double],
+    ),
     "tess_step": (
         ctypes.c_int32,
         [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_particles), _P(tess_params),

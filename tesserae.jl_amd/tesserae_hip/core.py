@@ -789,6 +789,21 @@ def _resolve_weights(weights):
     return weights.basis, ctypes.pointer(W)
 
 
+def g2p2g_steps(grid: Grid, particles: Particles, basis: Basis, params: MPMParams, nsteps: int = 1,
+                reorder_threshold: float = 0.0, ctx=None) -> None:
+    """nsteps @G2P2G passes with deferred weights: the time loop of a program written as one @G2P2G block
+    (g2p2g.jl:100-159), with the particles reordered into the engine's tile order at the threshold as in
+    :func:`step`.  Dense grids."""
+    _require_device(particles.x, grid.m)
+    if isinstance(grid, SpGrid):
+        raise UnsupportedError("G2P2G on an SpGrid is not supported")
+    ctx = ctx or default_context()
+    P, G = particles.abi(), grid.abi()
+    _check(ctx.lib, ctx.lib.tess_g2p2g_steps(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
+                                             ctypes.byref(P), ctypes.byref(params.abi()), int(nsteps),
+                                             float(reorder_threshold)))
+
+
 def step(grid: Grid, particles: Particles, basis: Basis, params: MPMParams, nsteps: int = 1,
          reorder_threshold: float = 0.0, ctx=None) -> None:
     """nsteps of the README loop body with deferred weights (README.md:66-87).

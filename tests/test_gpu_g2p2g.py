@@ -72,3 +72,38 @@ def test_g2p2g_fp32_and_deterministic(oracle):
         np.testing.assert_array_equal(a[1][name], b[1][name])
     assert rel_err(a[0]["x"], st.x) <= 1e-6
     assert rel_err(a[1]["m"], st.m) <= 1e-5
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_g2p2g_steps_match_oracle_and_single_passes(oracle, dim):
+    """tess_g2p2g_steps: four passes in one call equal the oracle's four passes (FP64) and four single-pass
+    calls; with the layered reorder the particles are the same set (up to their order)."""
+    basis = th.BSpline(th.Quadratic())
+    sc = _scene(dim, basis, 91 + dim)
+    sc.x = np.clip(sc.x, 0.02, 0.98)  # keep every particle inside so the reorder keeps them all binned
+    st = HostState(sc)
+    oracle.p2g(st, basis, sc.params)
+    for _ in range(4):
+        oracle.g2p2g(st, basis, sc.params)
+    runs = {}
+    for mode in ("loop", "single", "reorder"):
+        mesh, P, G = to_gpu(sc)
+        th.P2G(G, P, th.generate_basis_weights(basis, mesh, sc.n, deferred=True), sc.params)
+        if mode == "single":
+            for _ in range(4):
+                th.G2P2G(G, P, basis, sc.params)
+        else:
+            th.g2p2g_steps(G, P, basis, sc.params, nsteps=4, reorder_threshold=1.0 if mode == "reorder" else 0.0)
+        torch.cuda.synchronize()
+        runs[mode] = (particles_np(P), grid_np(G))
+    for name in ("x", "v", "F", "sigma"):
+        assert rel_err(runs["loop"][0][name], getattr(st, name)) <= 1e-12, name
+        assert rel_err(runs["loop"][0][name], runs["single"][0][name]) <= 1e-13, name
+    for name in ("m", "mv", "f"):
+        assert rel_err(runs["loop"][1][name], runs["single"][1][name]) <= 1e-13, name
+        assert rel_err(runs["reorder"][1][name], runs["loop"][1][name]) <= 1e-12, name
+    key = lambda p: np.lexsort(p["x"].T[::-1])
+    a, b = runs["loop"][0], runs["reorder"][0]
+    ia, ib = key(a), key(b)
+    for name in ("x", "v", "F"):
+        assert rel_err(b[name][ib], a[name][ia]) <= 1e-12, name
