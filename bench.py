@@ -140,9 +140,10 @@ def adjoint_ratio(nsteps, every):
             "grad_v0_norm": gnorm}
 
 
-def batch_leg(steps):
+def batch_leg(steps, reorder):
     """256 uneven 2D scenes (quadratic, FP32) stepped as one batch, after the autotuner
-    picks the launch variant."""
+    picks the launch variant; each scene's particles are reordered into the tile order at
+    the given threshold."""
     import torch
 
     import tesserae_hip as th
@@ -157,6 +158,7 @@ def batch_leg(steps):
     ntot = sum(sc.n for sc in scenes)
     ctx = th.Context()
     batch = th.Batch(members, th.BSpline(th.Quadratic()), ctx=ctx)
+    batch.set_reorder(reorder)  # the reference loop's reorder_particles! threshold, per scene
     variant = batch.autotune(2)
     batch.step(1)
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -408,7 +410,7 @@ def main():
         out["g2p2g"] = g2p2g_leg(sc, 10, args.layout, args.reorder)
     if not args.no_batch:
         log("bench: batched scenes")
-        out["batch"] = batch_leg(20)
+        out["batch"] = batch_leg(20, args.reorder)
     if not args.no_sphere:
         log("bench: sparse-grid sphere")
         out["sphere"] = sphere_leg(args.sphere_dx, 3)

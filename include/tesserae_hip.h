@@ -238,10 +238,12 @@ tess_status tess_g2p2g_steps(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh
  * (README.md:66-87), repeated nsteps times on the context's stream.
  * reorder_threshold plays the role of the reference loop's
  * reorder_particles!(particles, partition; threshold)
- * (src/Parallel/cpu_block_strategy.jl:248-269): before a step's P2G, the
- * particle fields are permuted into the engine's tile order when the fraction
- * of particles already in place (measured by the previous step) is below the
- * threshold; 0 never reorders, 1 reorders every step.  Reordering permutes the
+ * (src/Parallel/cpu_block_strategy.jl:222-278): before a step's P2G, the
+ * particle fields are permuted into the engine's tile order when the
+ * contiguity score measured by the previous step (the fraction of neighbours
+ * in that order whose storage indices are consecutive, the meaning of
+ * block_ordered_particle_contiguity) is below the threshold; 0 never
+ * reorders, 1 reorders every step.  Reordering permutes the
  * caller's particle arrays, exactly like reorder_particles!. */
 tess_status tess_step(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
                       const tess_particles* particles, const tess_params* params, int32_t nsteps,
@@ -351,6 +353,15 @@ tess_status tess_batch_create(tess_ctx* ctx, int32_t basis, int32_t nscenes, con
                               tess_batch** out);
 /* nsteps README steps of every scene of the batch (each with its own Δt). */
 tess_status tess_batch_step(tess_ctx* ctx, tess_batch* batch, int32_t nsteps);
+/* Reorders each scene's particles into the engine's tile order before a
+ * step's P2G when the batch's contiguity score (the fraction of neighbours in
+ * that order whose storage indices are consecutive) is below threshold, or
+ * every step at threshold >= 1 (reorder_particles! and
+ * block_ordered_particle_contiguity, src/Parallel/cpu_block_strategy.jl:222-278); particles
+ * outside their mesh follow, and every particle stays in its scene's arrays.
+ * 0 (the default) never reorders.  Reordering permutes the caller's particle
+ * arrays, exactly like reorder_particles!. */
+tess_status tess_batch_set_reorder(tess_batch* batch, double threshold);
 /* Times each launch variant (waves per tile in P2G and G2P) over
  * steps_per_variant steps of the batch after one untimed step, keeps the
  * fastest and writes its number to *variant.  The tuning steps are steps of
@@ -422,7 +433,9 @@ tess_status tess_profile_read(tess_ctx* ctx, int32_t index, char* name, int32_t 
 /* Host copies of the last binning and step: out[0] particles inside the mesh,
  * out[1] active tiles, out[2] node blocks written, out[3] grid nodes with m>0
  * written by the previous P2G, out[4] reorders done by tess_step, out[5]
- * particles found out of the layered order by the previous P2G (-1 unknown),
+ * neighbours in the layered order (the tile kernels' read order) whose
+ * storage indices are not consecutive, counted by the previous P2G (-1
+ * unknown; the contiguity score is 1 - out[5] / (out[0] - out[1])),
  * out[6] particles whose tile or window class changed at the last incremental
  * re-binning (-1 after a full sort), out[7] binnings that kept the previous
  * tile layout because no particle changed its key. */

@@ -245,7 +245,7 @@ tess_status tess_ctx_destroy(tess_ctx* ctx) {
                             &ctx->vals_alt,  &ctx->hist,        &ctx->scan_tmp,     &ctx->tile_start,
                             &ctx->tile_end,  &ctx->tile_flag,   &ctx->tile_slot,    &ctx->active_tiles,
                             &ctx->nb_flag,   &ctx->nb_slot,     &ctx->nb_list,      &ctx->nb_list_prev,
-                            &ctx->partials,  &ctx->counters,    &ctx->host_pinned_dummy, &ctx->reorder_tmp,
+                            &ctx->partials,  &ctx->counters,    &ctx->host_pinned_dummy, &ctx->reorder_tmp, &ctx->reorder_cnt,
                             &ctx->per_tile,  &ctx->per_nodeblock, &ctx->skeys2, &ctx->perm2,
                             &ctx->inc_nk,    &ctx->inc_spos,  &ctx->inc_mkey, &ctx->inc_mj,
                             &ctx->inc_mks,   &ctx->inc_mr,    &ctx->inc_mjs,
@@ -524,6 +524,14 @@ tess_status tess_batch_create(tess_ctx* ctx, int32_t basis, int32_t nscenes, con
       check_params(&scenes[s].params);
     }
     run_batch_create(ctx, basis, nscenes, scenes, out);
+  });
+}
+
+tess_status tess_batch_set_reorder(tess_batch* batch, double threshold) {
+  return guarded([&] {
+    if (!batch) raise(TESS_ERR_ARGUMENT, "batch is NULL");
+    if (!(threshold >= 0.0 && threshold <= 1.0)) raise(TESS_ERR_ARGUMENT, "threshold must be in [0, 1].");
+    run_batch_set_reorder(batch, threshold);
   });
 }
 

@@ -177,6 +177,136 @@ void batch_bin(tess_ctx* ctx, tess_batch& B, bool keys_from_g2p) {
   B.binned = true;
 }
 
+// ---- layered reorder of a batch ------------------------------------------------
+//
+// Each scene's particles into the layered order of its tiles (the order the
+// tile kernels read contiguously), its particles outside the mesh after them
+// in their previous order.  Every particle stays in its scene's arrays: the
+// layered slot L of a scene-s particle becomes global index L + (particles
+// outside the mesh in scenes before s), since the scene's inside particles
+// start at sorted position poff_s minus those.
+
+__global__ void k_mark_outside(const uint32_t* __restrict__ perm, int64_t from, int64_t n, uint32_t* __restrict__ flag) {
+  const int64_t j = from + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < n) flag[perm[j]] = 1u;
+}
+
+// new global index of the particle at each sorted position (pout) and the
+// gather order (new index -> old index)
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_batch_new_index(Args<T, D> a, const uint32_t* __restrict__ layered,
+                                                         const uint32_t* __restrict__ R, uint32_t* __restrict__ order,
+                                                         uint32_t* __restrict__ pout) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < a.n) {
+    const uint32_t g = a.perm[j];
+    const int s = scene_of(a.scene_poff, a.nscenes, int64_t(g));
+    const int64_t p0 = a.scene_poff[s], p1 = a.scene_poff[s + 1];
+    const uint32_t dst = j < a.n_inside ? uint32_t(int64_t(layered[j]) + int64_t(R[p0]))
+                                        : uint32_t(p1 - int64_t(R[p1] - R[p0]) + int64_t(R[g] - R[p0]));
+    order[dst] = g;
+    pout[j] = dst;
+  }
+}
+
+template <typename T, int D>
+__device__ __forceinline__ DField<T> scene_field(const SceneRec<T, D>& r, int fid) {
+  switch (fid) {
+    case 0: return r.x;
+    case 1: return r.v;
+    case 2: return r.V;
+    case 3: return r.gradv;
+    case 4: return r.F;
+    case 5: return r.sigma;
+    case 6: return r.l;
+    default: return r.logE;
+  }
+}
+
+// components of the particle fields x, v, V, ∇v, F, σ, l, log E
+template <int D>
+constexpr int kFieldComps[8] = {D, D, 1, D * D, D * D, D * D, 1, 1};
+template <int D>
+constexpr int kStateWidth = 2 * D + 2 + 3 * D * D + 1;
+
+// With `order`: every field of particle order[i] into row i of the packed
+// scratch (field-major: field f's rows start at its component offset times n).
+// Without: row i back into particle i.  One scene lookup per particle.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_batch_state(Args<T, D> a, T* __restrict__ scratch,
+                                                     const uint32_t* __restrict__ order) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int s = scene_of(a.scene_poff, a.nscenes, i);
+  const SceneRec<T, D>& r = a.scenes[s];
+  const int64_t li = (order ? int64_t(order[i]) : i) - r.poff;  // the permutation stays within the scene
+  int64_t off = 0;
+#pragma unroll
+  for (int fid = 0; fid < 8; ++fid) {
+    const int nc = kFieldComps<D>[fid];
+    const DField<T> f = scene_field<T, D>(r, fid);
+    if (f.p) {
+      T* row = scratch + off * a.n + i * nc;
+      if (order) {
+        for (int c = 0; c < nc; ++c) row[c] = f(li, c);
+      } else {
+        for (int c = 0; c < nc; ++c) f(li, c) = row[c];
+      }
+    }
+    off += nc;
+  }
+}
+
+// Reorders the scenes' particles when the batch's contiguity score is below
+// `threshold`, or at every call when threshold >= 1 (reorder_particles!,
+// src/Parallel/cpu_block_strategy.jl:222-278).  The score is the fraction of
+// neighbours in the layered order whose storage indices are consecutive,
+// counted by k_layered_order; the binning's permutation follows the particles.
+template <typename T, int D>
+void batch_reorder(tess_ctx* ctx, tess_batch& B, double threshold) {
+  const Binning& b = ctx->bin;
+  const int64_t n = B.ntot;
+  if (n == 0 || b.n_active == 0) return;
+  Args<T, D> a = batch_args<T, D>(ctx, B);
+  uint32_t* layered = ctx->keys_alt.as<uint32_t>(size_t(n));
+  uint32_t* order = ctx->keys.as<uint32_t>(size_t(n));  // the raw keys are dead after the sort
+  uint32_t* pout = ctx->inc_mr.as<uint32_t>(size_t(n));
+  uint32_t* R = ctx->inc_mks.as<uint32_t>(size_t(n) + 1);
+  uint32_t* broken = ctx->reorder_cnt.as<uint32_t>(size_t(b.n_active));
+  uint32_t* counters = ctx->counters.as<uint32_t>(CNT_SLOTS);
+  const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
+  hipLaunchKernelGGL((k_layered_order<T, D>), dim3(unsigned(b.n_active)), dim3(64), 0, ctx->stream, a, sk, pout,
+                     layered, threshold < 1.0 ? broken : nullptr);
+  TESS_HIP(hipGetLastError());
+  if (threshold < 1.0) {
+    sum_u32(ctx, broken, b.n_active, counters + CNT_MISPLACED);
+    uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
+    TESS_HIP(hipMemcpyAsync(hc + CNT_MISPLACED, counters + CNT_MISPLACED, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    TESS_HIP(hipStreamSynchronize(ctx->stream));
+    if (!(layered_contiguity(int64_t(hc[CNT_MISPLACED]), b.n_inside - b.n_active) < threshold)) return;
+  }
+  TESS_HIP(hipMemsetAsync(R, 0, (size_t(n) + 1) * sizeof(uint32_t), ctx->stream));
+  if (n > b.n_inside) {
+    hipLaunchKernelGGL(k_mark_outside, dim3(blocks_for(n - b.n_inside, 256)), dim3(256), 0, ctx->stream, a.perm,
+                       b.n_inside, n, R);
+    TESS_HIP(hipGetLastError());
+  }
+  exclusive_scan_u32(ctx, R, n + 1, nullptr);
+  hipLaunchKernelGGL((k_batch_new_index<T, D>), dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, a, layered, R,
+                     order, pout);
+  TESS_HIP(hipGetLastError());
+  T* scratch = ctx->reorder_tmp.as<T>(size_t(n) * kStateWidth<D>);
+  hipLaunchKernelGGL((k_batch_state<T, D>), dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, a, scratch,
+                     static_cast<const uint32_t*>(order));
+  TESS_HIP(hipGetLastError());
+  hipLaunchKernelGGL((k_batch_state<T, D>), dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, a, scratch,
+                     static_cast<const uint32_t*>(nullptr));
+  TESS_HIP(hipGetLastError());
+  TESS_HIP(hipMemcpyAsync(ctx->vals.ptr, pout, size_t(n) * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
+  B.reorders += 1;
+}
+
 template <typename T, int D, int BASIS, int NP>
 void launch_p2g(tess_ctx* ctx, const Args<T, D>& a, int64_t n_active, const uint32_t* sk) {
   hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NP, true, true>), dim3(xcd_grid(n_active)), dim3(64 * NP), 0,
@@ -187,18 +317,23 @@ void launch_p2g(tess_ctx* ctx, const Args<T, D>& a, int64_t n_active, const uint
 template <typename T, int D, int BASIS, int NW>
 void launch_g2p(tess_ctx* ctx, const Args<T, D>& a, int64_t n_active, const uint32_t* sk, uint32_t* nk, uint32_t* stay,
                 uint32_t* movers) {
+  const uint32_t nt = uint32_t(ctx->bin.ntiles);
   hipLaunchKernelGGL((k_g2p_fast<T, D, BASIS, true, NW, true>), dim3(xcd_grid(n_active)), dim3(64 * NW), 0,
-                     ctx->stream, a, sk, nk, stay, uint32_t(ctx->bin.ntiles), movers);
+                     ctx->stream, a, sk, nk, stay, nt, movers);
   TESS_HIP(hipGetLastError());
 }
 
 template <typename T, int D, int BASIS>
 void batch_step_impl(tess_ctx* ctx, tess_batch& B, int nsteps) {
-  using Gm = Geo<D>;
   constexpr int SW = support_width(BASIS);
   for (int s = 0; s < nsteps; ++s) {
     // from the second step of the call on, the previous G2P wrote the keys
     batch_bin<T, D, BASIS>(ctx, B, s > 0);
+    if (B.reorder_threshold > 0.0) {
+      const int tok = prof_begin(ctx, "reorder", 0.0);
+      batch_reorder<T, D>(ctx, B, B.reorder_threshold);
+      prof_end(ctx, tok);
+    }
     const Binning& b = ctx->bin;
     Args<T, D> a = batch_args<T, D>(ctx, B);
     // zero what this step does not overwrite: every grid the first time, then
@@ -335,6 +470,8 @@ void run_batch_step(tess_ctx* ctx, tess_batch* b, int nsteps) {
 }
 
 void run_batch_set_variant(tess_batch* b, int variant) { b->variant = variant; }
+
+void run_batch_set_reorder(tess_batch* b, double threshold) { b->reorder_threshold = threshold; }
 int run_batch_variant(const tess_batch* b) { return b->variant; }
 
 void run_batch_destroy(tess_batch* b) {

@@ -124,10 +124,11 @@ struct tess_ctx {
   int64_t prev_ntiles = -1;
   int64_t prev_nnodes = -1;
   // reorder scratch and the layout score of the last step's P2G
-  tess::Buffer reorder_tmp;
+  tess::Buffer reorder_tmp, reorder_cnt;
   const void* misplaced_x_ptr = nullptr;
   int64_t misplaced_n = 0;
-  int64_t last_misplaced = -1;
+  int64_t misplaced_pairs = 0;  // layered neighbour pairs of that P2G's binning
+  int64_t last_misplaced = -1;  // of them, pairs not consecutive in storage
   int64_t reorders = 0;
   // nodes with m > 0 written by the last gather (read back with the next binning)
   int64_t last_active_nodes = -1;
@@ -153,6 +154,10 @@ struct tess_batch {
   bool binned = false, keys_ready = false;
   int64_t kept_binnings = 0;
   int variant = 0;  // bit 0: P2G one wave per last-axis offset (2D), bit 1: G2P with 2 waves instead of 4
+  // layered reorder of the scenes' particles when the contiguity score falls
+  // below the threshold (0: never, 1: every step); reorders done so far
+  double reorder_threshold = 0.0;
+  int64_t reorders = 0;
 };
 
 namespace tess {
@@ -220,6 +225,7 @@ void run_ply_write(const char* name, int64_t n, const float* xyz);
 void run_batch_create(tess_ctx* ctx, int basis, int nscenes, const tess_scene* scenes, tess_batch** out);
 void run_batch_step(tess_ctx* ctx, tess_batch* b, int nsteps);
 void run_batch_set_variant(tess_batch* b, int variant);
+void run_batch_set_reorder(tess_batch* b, double threshold);
 int run_batch_variant(const tess_batch* b);
 void run_batch_destroy(tess_batch* b);
 // Moves every particle field so that slot k receives particle order[k].

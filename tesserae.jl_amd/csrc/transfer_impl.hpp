@@ -712,6 +712,22 @@ __device__ __forceinline__ void tile_class_counts(const uint32_t* __restrict__ s
   for (int off = 32; off >= 1; off >>= 1) maxcnt = max(maxcnt, __shfl_xor(maxcnt, off, 64));
 }
 
+// Layered-order contiguity of one layer (the estimator of
+// block_ordered_particle_contiguity, src/Parallel/cpu_block_strategy.jl:222-246):
+// a lane holding a particle (bit set in `has`) is paired with the next lower
+// such lane, the lowest one with the last particle of the previous layer
+// (`last`, updated here); returns 1 where the pair is not consecutive in
+// storage.  Called by every lane of the wave; the tile's first particle has
+// no pair.
+__device__ __forceinline__ uint32_t broken_pair(uint64_t has, int k, uint32_t pid, uint32_t& last) {
+  const uint64_t below = has & lanemask_lt64();
+  const uint32_t pl = __shfl(pid, below ? 63 - __clzll(below) : 0, 64);
+  const bool mine = (has >> (threadIdx.x & 63)) & 1u;
+  const uint32_t r = (mine && (below || k > 0) && pid != (below ? pl : last) + 1u) ? 1u : 0u;
+  last = __shfl(pid, 63 - __clzll(has), 64);
+  return r;
+}
+
 // One wave per tile.  The sort key orders each tile's particles by class
 // (window start relative to the footprint origin), so lane c owns class c: it
 // walks its particles, accumulates their contributions to the sw^dim nodes of
@@ -757,17 +773,14 @@ __global__ __launch_bounds__(64) void k_p2g_tile(Args<T, D> a, const uint32_t* _
     }
   }
   check_sparse_support<T, D, SW>(a, origin, ccoord, ccount);
-  // particles whose storage index differs from the layered position
-  // (k-th particle of every class contiguous, classes ascending)
+  // neighbours in the layered order (k-th particle of every class, classes
+  // ascending, then the next k) whose storage indices are not consecutive
   if (misplaced) {
-    uint32_t bad = 0, layer_off = 0;
+    uint32_t bad = 0, last = 0;
     for (int k = 0; k < maxcnt; ++k) {
       const uint64_t has = __ballot(k < ccount);
-      if (k < ccount) {
-        const uint32_t want = start + layer_off + uint32_t(__popcll(has & lanemask_lt64()));
-        bad += a.perm[start + coff + k] != want ? 1u : 0u;
-      }
-      layer_off += uint32_t(__popcll(has));
+      const uint32_t pid = k < ccount ? a.perm[start + coff + k] : 0u;
+      bad += broken_pair(has, k, pid, last);
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) bad += __shfl_xor(bad, off, 64);
@@ -1097,7 +1110,7 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
   }
   if (wave == 0) check_sparse_support<T, D, SW>(a, origin, ccoord, ccount);
   const T mp = a.mp, hinv = a.mesh.h_inv;
-  uint32_t bad = 0, layer_off = 0;
+  uint32_t bad = 0, last = 0;
   {
     const int pass = wave;
     T am[NSL], amv[D][NSL], af[D][NSL];
@@ -1124,10 +1137,7 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
       if (k + 1 < ccount) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + 1] - poff, nxt);
       const uint64_t has = __ballot(k < ccount);
       const bool act = k < ccount;
-      if (misplaced && pass == 0 && act) {
-        const uint32_t want = start + layer_off + uint32_t(__popcll(has & lanemask_lt64()));
-        bad += cur.pid != want ? 1u : 0u;
-      }
+      if (misplaced && pass == 0) bad += broken_pair(has, k, act ? cur.pid : 0u, last);
       T x[D], mpv[D], A[D * D];
       const T mpe = act ? mp : T(0);
 #pragma unroll
@@ -1247,7 +1257,6 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
           }
         }
       }
-      if (pass == 0) layer_off += uint32_t(__popcll(has));
     };
     for (int k = 0; k < maxcnt; k += 2) {
       layer(k, pa, pb);
@@ -2197,6 +2206,7 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     misplaced = per_tile;
     ctx->misplaced_x_ptr = P.x.ptr;
     ctx->misplaced_n = P.n;
+    ctx->misplaced_pairs = b.n_inside - b.n_active;
   } else {
     ctx->misplaced_x_ptr = nullptr;
   }
@@ -2355,7 +2365,8 @@ void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
 // storage index map.
 template <typename T, int D>
 __global__ __launch_bounds__(64) void k_layered_order(Args<T, D> a, const uint32_t* __restrict__ skeys,
-                                                       uint32_t* __restrict__ order, uint32_t* __restrict__ perm_new) {
+                                                       uint32_t* __restrict__ order, uint32_t* __restrict__ perm_new,
+                                                       uint32_t* __restrict__ broken = nullptr) {
   const int lane = threadIdx.x;
   const int64_t j = blockIdx.x;
   if (j >= a.n_active) return;
@@ -2365,15 +2376,22 @@ __global__ __launch_bounds__(64) void k_layered_order(Args<T, D> a, const uint32
   __shared__ uint32_t s_cnt[64];
   int ccount, coff, maxcnt;
   tile_class_counts(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);
-  uint32_t layer_off = 0;
+  uint32_t layer_off = 0, bad = 0, last = 0;
   for (int k = 0; k < maxcnt; ++k) {
     const uint64_t has = __ballot(k < ccount);
+    const uint32_t pid = k < ccount ? a.perm[start + coff + k] : 0u;
     if (k < ccount) {
       const uint32_t dest = start + layer_off + uint32_t(__popcll(has & lanemask_lt64()));
-      order[dest] = a.perm[start + coff + k];
+      order[dest] = pid;
       perm_new[start + coff + k] = dest;
     }
+    if (broken) bad += broken_pair(has, k, pid, last);
     layer_off += uint32_t(__popcll(has));
+  }
+  if (broken) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) bad += __shfl_xor(bad, off, 64);
+    if (lane == 0) broken[j] = bad;
   }
 }
 
@@ -2383,6 +2401,12 @@ __global__ void k_tail_identity(const uint32_t* __restrict__ perm, int64_t from,
   if (j >= n) return;
   order[j] = perm[j];
   perm_new[j] = uint32_t(j);
+}
+
+// fraction of layered neighbour pairs (particles inside the mesh minus active
+// tiles) that are consecutive in storage; 1 with no pairs
+inline double layered_contiguity(int64_t broken, int64_t pairs) {
+  return pairs > 0 ? 1.0 - double(broken) / double(pairs) : 1.0;
 }
 
 template <typename T, int D>
@@ -2463,7 +2487,7 @@ void g2p2g_steps_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, 
     if (reorder_threshold > 0.0 && b.n_inside > 0) {
       bool go = reorder_threshold >= 1.0;
       if (!go && ctx->last_misplaced >= 0) {
-        const double score = 1.0 - double(ctx->last_misplaced) / double(b.n_inside);
+        const double score = layered_contiguity(ctx->last_misplaced, ctx->misplaced_pairs);
         go = score < reorder_threshold;
       }
       if (go) {
@@ -2489,10 +2513,12 @@ void g2p2g_steps_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, 
 }
 
 // nsteps of the README loop body (deferred weights).  With reorder_threshold
-// in (0, 1], particles are moved into the layered order when the fraction
-// stored in place, measured by the previous step's P2G, is below the
-// threshold (1 reorders every step), the role of
-// reorder_particles!(particles, partition; threshold) in the reference loop.
+// in (0, 1], particles are moved into the layered order when the contiguity
+// score measured by the previous step's P2G (the fraction of neighbours in the
+// layered order whose storage indices are consecutive, the estimator of
+// block_ordered_particle_contiguity) is below the threshold (1 reorders every
+// step), the role of reorder_particles!(particles, partition; threshold) in
+// the reference loop.
 template <typename T, int D, int BASIS, bool STORED>
 void step_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
                const tess_params& prm, int nsteps, double reorder_threshold, tess_sparsity* sp) {
@@ -2511,7 +2537,7 @@ void step_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const t
     if (reorder_threshold > 0.0 && b.n_inside > 0) {
       bool go = reorder_threshold >= 1.0;
       if (!go && ctx->last_misplaced >= 0) {
-        const double score = 1.0 - double(ctx->last_misplaced) / double(b.n_inside);
+        const double score = layered_contiguity(ctx->last_misplaced, ctx->misplaced_pairs);
         go = score < reorder_threshold;
       }
       if (go) {

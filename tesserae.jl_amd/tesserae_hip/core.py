@@ -611,6 +611,7 @@ class Context:
     def stats(self) -> dict:
         vals = (ctypes.c_int64 * 8)()
         _check(self.lib, self.lib.tess_stats(self.handle, vals, 8))
+        # misplaced: layered neighbour pairs not consecutive in storage (contiguity 1 - misplaced / pairs)
         keys = ("n_inside", "active_tiles", "node_blocks", "active_nodes", "reorders", "misplaced", "movers",
                 "kept_binnings")
         return dict(zip(keys, list(vals)))
@@ -920,6 +921,12 @@ class Batch:
 
     def step(self, nsteps: int = 1) -> None:
         _check(self.ctx.lib, self.ctx.lib.tess_batch_step(self.ctx.handle, self.handle, int(nsteps)))
+
+    def set_reorder(self, threshold: float) -> None:
+        """Reorder each scene's particles into the tile order when the contiguity score (the fraction of
+        neighbours in that order whose storage indices are consecutive) is below ``threshold`` (0 never, 1 every
+        step); permutes the scenes' particle arrays like reorder_particles!."""
+        _check(self.ctx.lib, self.ctx.lib.tess_batch_set_reorder(self.handle, float(threshold)))
 
     def autotune(self, steps_per_variant: int = 2) -> int:
         v = ctypes.c_int32(0)

@@ -187,6 +187,30 @@ def test_two_disks_fp64_trajectory(oracle):
         assert err <= 1e-10, name
 
 
+def test_contiguity_score_of_the_step_loop():
+    """The reorder score (block_ordered_particle_contiguity's meaning, cpu_block_strategy.jl:222-246): right after
+    a reorder every neighbour pair of the layered order is consecutive in storage; with the positions randomly
+    permuted almost none is."""
+    def score_after(sc, threshold):
+        ctx = th.Context()
+        mesh, P, G = to_gpu(sc)
+        th.step(G, P, sc.basis, sc.params, nsteps=2, reorder_threshold=threshold, ctx=ctx)
+        ctx.sync()
+        st = ctx.stats()
+        ctx.close()
+        pairs = st["n_inside"] - st["active_tiles"]
+        assert st["misplaced"] >= 0 and pairs > 1000, st
+        return 1.0 - st["misplaced"] / pairs, st
+
+    sc = two_disks(h=0.01, steps=2)
+    score, st = score_after(sc, 1.0)  # reorders before each P2G: the counted P2G sees the layered order
+    assert st["misplaced"] == 0 and st["reorders"] == 2, st
+    sc = two_disks(h=0.01, steps=2)
+    sc.x[:] = sc.x[np.random.default_rng(5).permutation(sc.n)]
+    score, st = score_after(sc, 1e-9)  # never reorders (the score stays above 1e-9)
+    assert st["reorders"] == 0 and score < 0.02, (score, st)
+
+
 def test_step_with_reorder_matches_oracle(oracle):
     """Steps that permute particles into the engine's tile order (threshold 1: every
     step) give the oracle's trajectory once particles are matched by identity."""
