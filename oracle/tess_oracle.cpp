@@ -15,8 +15,9 @@
 //                                              colour-disjointness (test/partition.jl:53-158)
 //   P2G / grid update / G2P .................. pinned by conservation invariants, affine
 //                                              reproduction and a brute-force dense loop
-//   Neo-Hookean det/log (Tensorial.jl) ....... parity unpinned beyond closed-form
-//                                              checks (no reference test covers them)
+//   Neo-Hookean stress ....................... pinned by the strain-energy derivative
+//                                              (J σ = ∂ψ/∂F Fᵀ) with an LU determinant;
+//                                              Tensorial.jl's det rounding order unpinned
 //
 // Arithmetic order follows the reference where it is visible in its source:
 // Horner evaluation with muladd (Julia evalpoly), left-associated tensor

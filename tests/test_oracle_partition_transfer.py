@@ -210,6 +210,9 @@ def test_g2p_affine_reproduction(oracle, dim, basis):
     X = np.stack([a.reshape(-1, order="F") for a in axes], axis=1)
     st.gv[:] = X @ A.T + b
     st.vn[:] = st.gv - c
+    # a deformed F (det > 0) so that the stress check below sees finite strains
+    F0 = np.eye(dim) + 0.3 * rng.uniform(-1, 1, size=(sc.n, dim, dim))
+    st.F[:] = F0.transpose(0, 2, 1).reshape(sc.n, dim * dim)
     x0, v0 = st.x.copy(), st.v.copy()
     oracle.g2p(st, basis, sc.params)
     dt = sc.params.dt
@@ -225,6 +228,25 @@ def test_g2p_affine_reproduction(oracle, dim, basis):
         / J[:, None, None]
     np.testing.assert_allclose(st.sigma.reshape(-1, dim, dim).transpose(0, 2, 1), sig, rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(st.V, sc.params.V0 * J, rtol=1e-14)
+    # F advanced by F <- (I + dt ∇v) F
+    np.testing.assert_allclose(F, (np.eye(dim) + dt * A) @ F0, rtol=1e-12, atol=1e-12)
+    # the stress is the Kirchhoff stress of the compressible Neo-Hookean energy
+    # ψ(F) = μ/2 (tr FᵀF - d) - μ ln J + λ/2 (ln J)², by central differences:
+    # J σ = (∂ψ/∂F) Fᵀ.  A dropped term, a wrong sign or a transposed product fails it.
+    def psi(G):
+        lj = np.log(np.linalg.det(G))
+        return mu / 2 * ((G * G).sum(axis=(1, 2)) - dim) - mu * lj + lam / 2 * lj ** 2
+
+    eps = 1e-5
+    dpsi = np.zeros_like(F)
+    for a in range(dim):
+        for b in range(dim):
+            E = np.zeros((dim, dim))
+            E[a, b] = eps
+            dpsi[:, a, b] = (psi(F + E) - psi(F - E)) / (2 * eps)
+    tau = dpsi @ F.transpose(0, 2, 1)
+    got = st.sigma.reshape(-1, dim, dim).transpose(0, 2, 1) * J[:, None, None]
+    np.testing.assert_allclose(got, tau, rtol=1e-6, atol=1e-7 * np.abs(tau).max())
 
 
 def test_threaded_baseline_matches_sequential(oracle):
