@@ -366,7 +366,9 @@ tess_status tess_batch_set_reorder(tess_batch* batch, double threshold);
  * steps_per_variant steps of the batch after one untimed step, keeps the
  * fastest and writes its number to *variant.  The tuning steps are steps of
  * the simulation: the scenes advance by (variants * (steps_per_variant + 1))
- * steps.  Synchronises. */
+ * steps.  With a reorder threshold set, each untimed step reorders and the
+ * timed steps do not, so the variants are compared on the same freshly
+ * ordered layout; the threshold is restored.  Synchronises. */
 tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_per_variant, int32_t* variant);
 tess_status tess_batch_destroy(tess_batch* batch);
 

@@ -557,14 +557,20 @@ tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_
     TESS_HIP(hipEventCreate(&e1));
     float best = 0.0f;
     int best_v = run_batch_variant(batch);
+    // with a reorder threshold set, every variant is timed on a freshly
+    // reordered layout and without reorders in its timed steps
+    const double thr = batch->reorder_threshold;
     for (int k = 0; k < ncand; ++k) {
       const int v = dim == 2 ? candidates[k] : candidates[k] * 2;
       run_batch_set_variant(batch, v);
-      run_batch_step(ctx, batch, 1);  // untimed: code loading and first-use allocations
+      if (thr > 0.0) run_batch_set_reorder(batch, 1.0);
+      run_batch_step(ctx, batch, 1);  // untimed: code loading, first-use allocations, the reorder
+      run_batch_set_reorder(batch, 0.0);
       TESS_HIP(hipEventRecord(e0, ctx->stream));
       run_batch_step(ctx, batch, steps_per_variant);
       TESS_HIP(hipEventRecord(e1, ctx->stream));
       TESS_HIP(hipEventSynchronize(e1));
+      run_batch_set_reorder(batch, thr);
       float ms = 0.0f;
       TESS_HIP(hipEventElapsedTime(&ms, e0, e1));
       if (k == 0 || ms < best) {
