@@ -68,6 +68,26 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def copy_bandwidth(mib=1024, reps=5):
+    """Device-to-device copy of `mib` MiB timed with HIP events: the box's measured HBM copy rate in GB/s
+    (bytes read + bytes written per second)."""
+    import torch
+
+    src = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    gbs = 2 * src.numel() * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def cpu_baseline(threads):
     """Oracle threaded restatement (stored weights, CPU block partition, colour-phased
     P2G, threaded G2P) on a bounded sample: the same scene at dx=1/128 (1.7M
@@ -424,6 +444,10 @@ def main():
         log("bench: CPU baseline")
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(threads)
+        if threads != 8:  # the thread count of the reference's published CPU numbers
+            out["cpu_baseline_8_threads"] = cpu_baseline(8)
+    out["copy_GBs"] = copy_bandwidth()
+    log(f"bench: device copy {out['copy_GBs']:.0f} GB/s")
     print(json.dumps(out))
 
 
