@@ -954,6 +954,23 @@ __device__ __forceinline__ T& fldw(const DField<T>& f, int64_t e, int c) {
   else return f(e, c);
 }
 
+// Cache policy of the G2P particle stream (x, v, F read; v, ∇v, x, F, σ, V
+// written; gigabytes per launch, far beyond the L2 and the Infinity Cache):
+// 0 plain accesses, 1 non-temporal stores, 2 non-temporal loads and stores.
+#ifndef TESS_G2P_STREAM
+#define TESS_G2P_STREAM 1
+#endif
+template <typename T>
+__device__ __forceinline__ void st_g2p(T& dst, T v) {
+  if constexpr (TESS_G2P_STREAM >= 1) __builtin_nontemporal_store(v, &dst);
+  else dst = v;
+}
+template <typename T>
+__device__ __forceinline__ T ld_g2p(T& src) {
+  if constexpr (TESS_G2P_STREAM >= 2) return __builtin_nontemporal_load(&src);
+  else return src;
+}
+
 // 1-D jets at the window nodes lo..lo+SW-1 of one axis; entries past the
 // (truncated) window are zero.  Derivatives are returned times h⁻¹.
 template <typename T, int BASIS>
@@ -1324,11 +1341,11 @@ template <typename T, int D, int BASIS, bool AOS>
 __device__ __forceinline__ void g2p_load(const Args<T, D>& a, uint32_t pid, G2PIn<T, D, AOS>& in) {
   in.pid = pid;
 #pragma unroll
-  for (int c = 0; c < D; ++c) in.x[c] = fld<T, D, AOS>(a.x, pid, c);
+  for (int c = 0; c < D; ++c) in.x[c] = ld_g2p(fldw<T, D, AOS>(a.x, pid, c));
 #pragma unroll
-  for (int c = 0; c < D; ++c) in.v[c] = fld<T, D, AOS>(a.v, pid, c);
+  for (int c = 0; c < D; ++c) in.v[c] = ld_g2p(fldw<T, D, AOS>(a.v, pid, c));
 #pragma unroll
-  for (int q = 0; q < D * D; ++q) in.F[q] = fld<T, D * D, AOS>(a.F, pid, q);
+  for (int q = 0; q < D * D; ++q) in.F[q] = ld_g2p(fldw<T, D * D, AOS>(a.F, pid, q));
   in.l = T(0);
   if constexpr (BASIS == TESS_UGIMP) in.l = a.l(pid) / a.mesh.h;
 }
@@ -1515,14 +1532,14 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
     const uint32_t pid = cur.pid;
     // particle equations (README.md:81-86)
 #pragma unroll
-    for (int c = 0; c < D; ++c) fldw<T, D, AOS>(a.v, pid, c) = cur.v[c] + DV[c];
+    for (int c = 0; c < D; ++c) st_g2p(fldw<T, D, AOS>(a.v, pid, c), cur.v[c] + DV[c]);
 #pragma unroll
-    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.gradv, pid, q) = L[q];
+    for (int q = 0; q < D * D; ++q) st_g2p(fldw<T, D * D, AOS>(a.gradv, pid, q), L[q]);
     T xn[D];
 #pragma unroll
     for (int c = 0; c < D; ++c) {
       xn[c] = cur.x[c] + X[c] * dt;
-      fldw<T, D, AOS>(a.x, pid, c) = xn[c];
+      st_g2p(fldw<T, D, AOS>(a.x, pid, c), xn[c]);
     }
     if (nk) {
       const uint32_t jj = slot(k);
@@ -1551,12 +1568,12 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
         Fn[r + D * b] = cur.F[r + D * b] + dt * t;
       }
 #pragma unroll
-    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.F, pid, q) = Fn[q];
+    for (int q = 0; q < D * D; ++q) st_g2p(fldw<T, D * D, AOS>(a.F, pid, q), Fn[q]);
     T sg[D * D], Vn;
     neo_hookean<T, D>(Fn, a.lambda, a.mu, a.V0, a.logE.p != nullptr, a.logE.p ? a.logE(pid) : T(0), sg, Vn);
 #pragma unroll
-    for (int q = 0; q < D * D; ++q) fldw<T, D * D, AOS>(a.sigma, pid, q) = sg[q];
-    a.V(pid) = Vn;
+    for (int q = 0; q < D * D; ++q) st_g2p(fldw<T, D * D, AOS>(a.sigma, pid, q), sg[q]);
+    st_g2p(a.V(pid), Vn);
   };
   for (int k = wave; k < kend; k += 2 * NW) {
     particle(k, pa, pb);
