@@ -970,6 +970,15 @@ __device__ __forceinline__ T ld_g2p(T& src) {
   if constexpr (TESS_G2P_STREAM >= 2) return __builtin_nontemporal_load(&src);
   else return src;
 }
+// P2G's particle reads (x, v, V, σ): 0 plain, 1 non-temporal
+#ifndef TESS_P2G_STREAM
+#define TESS_P2G_STREAM 0
+#endif
+template <typename T>
+__device__ __forceinline__ T ld_p2g(T& src) {
+  if constexpr (TESS_P2G_STREAM >= 1) return __builtin_nontemporal_load(&src);
+  else return src;
+}
 
 // 1-D jets at the window nodes lo..lo+SW-1 of one axis; entries past the
 // (truncated) window are zero.  Derivatives are returned times h⁻¹.
@@ -1060,12 +1069,12 @@ template <typename T, int D, int BASIS, bool AOS>
 __device__ __forceinline__ void p2g_load(const Args<T, D>& a, uint32_t pid, P2GIn<T, D>& in) {
   in.pid = pid;
 #pragma unroll
-  for (int d = 0; d < D; ++d) in.x[d] = fld<T, D, AOS>(a.x, pid, d);
+  for (int d = 0; d < D; ++d) in.x[d] = ld_p2g(fldw<T, D, AOS>(a.x, pid, d));
 #pragma unroll
-  for (int d = 0; d < D; ++d) in.v[d] = fld<T, D, AOS>(a.v, pid, d);
-  in.V = a.V(pid);
+  for (int d = 0; d < D; ++d) in.v[d] = ld_p2g(fldw<T, D, AOS>(a.v, pid, d));
+  in.V = ld_p2g(a.V(pid));
 #pragma unroll
-  for (int q = 0; q < D * D; ++q) in.S[q] = fld<T, D * D, AOS>(a.sigma, pid, q);
+  for (int q = 0; q < D * D; ++q) in.S[q] = ld_p2g(fldw<T, D * D, AOS>(a.sigma, pid, q));
   in.l = T(0);
   if constexpr (BASIS == TESS_UGIMP) in.l = a.l(pid) / a.mesh.h;
 }
