@@ -370,6 +370,12 @@ tess_status tess_batch_set_reorder(tess_batch* batch, double threshold);
  * timed steps do not, so the variants are compared on the same freshly
  * ordered layout; the threshold is restored.  Synchronises. */
 tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_per_variant, int32_t* variant);
+/* Sets the launch variant, e.g. a number tess_batch_autotune returned and the
+ * caller cached for this workload shape: in 2D bit 0 splits P2G into one wave
+ * per last-axis offset, and G2P runs 4 waves per tile, 2 with bit 1, 1 with
+ * bit 2 (0..5); in 3D 0 or 2.  Other numbers are TESS_ERR_ARGUMENT.  No
+ * counterpart in the reference, which has no launch variants. */
+tess_status tess_batch_set_variant(tess_batch* batch, int32_t variant);
 tess_status tess_batch_destroy(tess_batch* batch);
 
 /* ---- output (src/IO/export.jl) ------------------------------------------ */

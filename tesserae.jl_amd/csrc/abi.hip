@@ -550,8 +550,10 @@ tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_
     if (!batch) raise(TESS_ERR_ARGUMENT, "batch is NULL");
     if (steps_per_variant < 1) raise(TESS_ERR_ARGUMENT, "steps_per_variant must be positive");
     const int dim = batch->dim;
-    const int candidates[4] = {0, 1, 2, 3};
-    const int ncand = dim == 2 ? 4 : 2;  // 3D: the P2G split is fixed, only the G2P waves vary
+    // 2D: P2G split (bit 0) × G2P with 4, 2 or 1 waves per tile; 3D: the P2G
+    // split is fixed, G2P with 4 or 2 waves
+    const int cand2[6] = {0, 1, 2, 3, 4, 5}, cand3[2] = {0, 2};
+    const int ncand = dim == 2 ? 6 : 2;
     hipEvent_t e0, e1;
     TESS_HIP(hipEventCreate(&e0));
     TESS_HIP(hipEventCreate(&e1));
@@ -561,7 +563,7 @@ tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_
     // reordered layout and without reorders in its timed steps
     const double thr = batch->reorder_threshold;
     for (int k = 0; k < ncand; ++k) {
-      const int v = dim == 2 ? candidates[k] : candidates[k] * 2;
+      const int v = dim == 2 ? cand2[k] : cand3[k];
       run_batch_set_variant(batch, v);
       if (thr > 0.0) run_batch_set_reorder(batch, 1.0);
       run_batch_step(ctx, batch, 1);  // untimed: code loading, first-use allocations, the reorder
@@ -582,6 +584,16 @@ tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_
     (void)hipEventDestroy(e1);
     run_batch_set_variant(batch, best_v);
     if (variant) *variant = best_v;
+  });
+}
+
+tess_status tess_batch_set_variant(tess_batch* batch, int32_t variant) {
+  return guarded([&] {
+    if (!batch) raise(TESS_ERR_ARGUMENT, "batch is NULL");
+    const bool ok = batch->dim == 2 ? (variant >= 0 && variant <= 5) : (variant == 0 || variant == 2);
+    if (!ok) raise(TESS_ERR_ARGUMENT, "no launch variant " + std::to_string(variant) + " for a " +
+                                          std::to_string(batch->dim) + "D batch");
+    run_batch_set_variant(batch, variant);
   });
 }
 

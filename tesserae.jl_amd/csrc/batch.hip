@@ -383,7 +383,8 @@ void batch_step_impl(tess_ctx* ctx, tess_batch& B, int nsteps) {
     if (b.n_active > 0) {
       const int tok = prof_begin(ctx, "k_g2p_tile",
                                  double(b.n_inside) * sizeof(T) * (2 * D + D * D + 2 * D + 3 * D * D + 1));
-      if (B.variant & 2) launch_g2p<T, D, BASIS, 2>(ctx, a, b.n_active, sk, nk, stay, mv);
+      if (B.variant & 4) launch_g2p<T, D, BASIS, 1>(ctx, a, b.n_active, sk, nk, stay, mv);
+      else if (B.variant & 2) launch_g2p<T, D, BASIS, 2>(ctx, a, b.n_active, sk, nk, stay, mv);
       else launch_g2p<T, D, BASIS, 4>(ctx, a, b.n_active, sk, nk, stay, mv);
       prof_end(ctx, tok);
     }

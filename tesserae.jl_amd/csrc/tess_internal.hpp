@@ -153,7 +153,8 @@ struct tess_batch {
   // particle's next key (stepping within one call)
   bool binned = false, keys_ready = false;
   int64_t kept_binnings = 0;
-  int variant = 0;  // bit 0: P2G one wave per last-axis offset (2D), bit 1: G2P with 2 waves instead of 4
+  // bit 0: P2G one wave per last-axis offset (2D); G2P waves per tile: 4, 2 with bit 1, 1 with bit 2
+  int variant = 0;
   // layered reorder of the scenes' particles when the contiguity score falls
   // below the threshold (0: never, 1: every step); reorders done so far
   double reorder_threshold = 0.0;

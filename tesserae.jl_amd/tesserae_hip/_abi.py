@@ -238,6 +238,7 @@ SIGNATURES = {
     "tess_batch_step": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
     "tess_batch_autotune": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, _P(ctypes.c_int32)]),
     "tess_batch_set_reorder": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double]),
+    "tess_batch_set_variant": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "tess_batch_destroy": (ctypes.c_int32, [ctypes.c_void_p]),
     "tess_profile_enable": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "tess_profile_read": (

@@ -928,6 +928,10 @@ class Batch:
         step); permutes the scenes' particle arrays like reorder_particles!."""
         _check(self.ctx.lib, self.ctx.lib.tess_batch_set_reorder(self.handle, float(threshold)))
 
+    def set_variant(self, variant: int) -> None:
+        """Launch variant to use (a number autotune returned): in 2D 0..5, in 3D 0 or 2."""
+        _check(self.ctx.lib, self.ctx.lib.tess_batch_set_variant(self.handle, int(variant)))
+
     def autotune(self, steps_per_variant: int = 2) -> int:
         v = ctypes.c_int32(0)
         _check(self.ctx.lib, self.ctx.lib.tess_batch_autotune(self.ctx.handle, self.handle, int(steps_per_variant),

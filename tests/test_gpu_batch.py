@@ -12,7 +12,7 @@ from tesserae_hip.scenes import batched_scenes, random_box
 pytestmark = pytest.mark.gpu
 
 
-def _run_and_check(oracle, scenes, basis, nsteps, tune=0):
+def _run_and_check(oracle, scenes, basis, nsteps, tune=0, variant=None):
     members, gpu = [], []
     for sc in scenes:
         mesh, P, G = to_gpu(sc)
@@ -23,8 +23,10 @@ def _run_and_check(oracle, scenes, basis, nsteps, tune=0):
     extra = 0
     if tune:
         v = batch.autotune(tune)
-        assert v in (0, 1, 2, 3)
-        extra = (4 if scenes[0].mesh.dim == 2 else 2) * (tune + 1)
+        assert v in (range(6) if scenes[0].mesh.dim == 2 else (0, 2))
+        extra = (6 if scenes[0].mesh.dim == 2 else 2) * (tune + 1)
+    if variant is not None:
+        batch.set_variant(variant)
     batch.step(nsteps)
     ctx.sync()
     for sc, P in zip(scenes, gpu):
@@ -46,6 +48,24 @@ def test_batch_2d_uneven_scenes_match_oracle(oracle):
 def test_batch_autotune_then_steps(oracle):
     scenes = batched_scenes(5, seed=9, dtype=torch.float64, cells=(16, 48), particles=(100, 2000))
     _run_and_check(oracle, scenes, th.BSpline(th.Quadratic()), 2, tune=1)
+
+
+@pytest.mark.parametrize("variant", range(6))
+def test_batch_2d_every_launch_variant_matches_oracle(oracle, variant):
+    """P2G with one wave or one per last-axis offset, G2P with 4, 2 or 1 waves per tile."""
+    scenes = batched_scenes(6, seed=11, dtype=torch.float64, cells=(16, 64), particles=(100, 3000))
+    _run_and_check(oracle, scenes, th.BSpline(th.Quadratic()), 3, variant=variant)
+
+
+def test_batch_set_variant_rejects_unknown_numbers():
+    sc = random_box(3, 200, 1, th.BSpline(th.Quadratic()), dtype=torch.float64)
+    mesh, P, G = to_gpu(sc)
+    batch = th.Batch([(G, P, sc.params)], th.BSpline(th.Quadratic()))
+    for bad in (1, 3, 4, 6, -1):
+        with pytest.raises(th.ArgumentError):
+            batch.set_variant(bad)
+    batch.set_variant(2)
+    batch.close()
 
 
 @pytest.mark.parametrize("basis", [th.BSpline(th.Linear()), th.BSpline(th.Cubic())], ids=["linear", "cubic"])
