@@ -261,7 +261,8 @@ __global__ __launch_bounds__(256) void k_batch_state(Args<T, D> a, T* __restrict
 // `threshold`, or at every call when threshold >= 1 (reorder_particles!,
 // src/Parallel/cpu_block_strategy.jl:222-278).  The score is the fraction of
 // neighbours in the layered order whose storage indices are consecutive,
-// counted by k_layered_order; the binning's permutation follows the particles.
+// counted by the previous step's P2G, or on a call's first step by
+// k_layered_order; the binning's permutation follows the particles.
 template <typename T, int D>
 void batch_reorder(tess_ctx* ctx, tess_batch& B, double threshold) {
   const Binning& b = ctx->bin;
@@ -275,12 +276,18 @@ void batch_reorder(tess_ctx* ctx, tess_batch& B, double threshold) {
   uint32_t* broken = ctx->reorder_cnt.as<uint32_t>(size_t(b.n_active));
   uint32_t* counters = ctx->counters.as<uint32_t>(CNT_SLOTS);
   const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
+  uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
+  if (threshold < 1.0 && B.counted) {
+    // the previous step's P2G counted the broken pairs; the binning's readback
+    // brought the count to the host
+    if (!(layered_contiguity(int64_t(hc[CNT_MISPLACED]), B.counted_pairs) < threshold)) return;
+  }
+  const bool count_here = threshold < 1.0 && !B.counted;
   hipLaunchKernelGGL((k_layered_order<T, D>), dim3(unsigned(b.n_active)), dim3(64), 0, ctx->stream, a, sk, pout,
-                     layered, threshold < 1.0 ? broken : nullptr);
+                     layered, count_here ? broken : nullptr);
   TESS_HIP(hipGetLastError());
-  if (threshold < 1.0) {
+  if (count_here) {
     sum_u32(ctx, broken, b.n_active, counters + CNT_MISPLACED);
-    uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
     TESS_HIP(hipMemcpyAsync(hc + CNT_MISPLACED, counters + CNT_MISPLACED, sizeof(uint32_t), hipMemcpyDeviceToHost,
                             ctx->stream));
     TESS_HIP(hipStreamSynchronize(ctx->stream));
@@ -308,9 +315,9 @@ void batch_reorder(tess_ctx* ctx, tess_batch& B, double threshold) {
 }
 
 template <typename T, int D, int BASIS, int NP>
-void launch_p2g(tess_ctx* ctx, const Args<T, D>& a, int64_t n_active, const uint32_t* sk) {
+void launch_p2g(tess_ctx* ctx, const Args<T, D>& a, int64_t n_active, const uint32_t* sk, uint32_t* broken) {
   hipLaunchKernelGGL((k_p2g_fast<T, D, BASIS, NP, true, true>), dim3(xcd_grid(n_active)), dim3(64 * NP), 0,
-                     ctx->stream, a, sk, static_cast<uint32_t*>(nullptr));
+                     ctx->stream, a, sk, broken);
   TESS_HIP(hipGetLastError());
 }
 
@@ -326,6 +333,7 @@ void launch_g2p(tess_ctx* ctx, const Args<T, D>& a, int64_t n_active, const uint
 template <typename T, int D, int BASIS>
 void batch_step_impl(tess_ctx* ctx, tess_batch& B, int nsteps) {
   constexpr int SW = support_width(BASIS);
+  B.counted = false;  // the context's counters may have served other calls since
   for (int s = 0; s < nsteps; ++s) {
     // from the second step of the call on, the previous G2P wrote the keys
     batch_bin<T, D, BASIS>(ctx, B, s > 0);
@@ -353,9 +361,20 @@ void batch_step_impl(tess_ctx* ctx, tess_batch& B, int nsteps) {
     if (b.n_active > 0) {
       a.partials = ctx->partials.as<T>(size_t(b.n_active) * Part<D, SW>::SIZE);
       const int tok = prof_begin(ctx, "k_p2g_tile", double(b.n_inside) * sizeof(T) * (2 * D + 1 + D * D));
-      if (D == 3 || (B.variant & 1)) launch_p2g<T, D, BASIS, SW>(ctx, a, b.n_active, sk);
-      else launch_p2g<T, D, BASIS, 1>(ctx, a, b.n_active, sk);
+      // with a threshold in (0, 1) P2G counts the broken layered pairs for the
+      // next step's reorder test
+      const bool count = B.reorder_threshold > 0.0 && B.reorder_threshold < 1.0;
+      uint32_t* broken = count ? ctx->reorder_cnt.as<uint32_t>(size_t(b.n_active)) : nullptr;
+      if (D == 3 || (B.variant & 1)) launch_p2g<T, D, BASIS, SW>(ctx, a, b.n_active, sk, broken);
+      else launch_p2g<T, D, BASIS, 1>(ctx, a, b.n_active, sk, broken);
       prof_end(ctx, tok);
+      if (count) {
+        sum_u32(ctx, broken, b.n_active, counters + CNT_MISPLACED);
+        B.counted_pairs = b.n_inside - b.n_active;
+      }
+      B.counted = count;
+    } else {
+      B.counted = false;
     }
     if (b.n_nodeblocks > 0) {
       uint32_t* per_nb = ctx->per_nodeblock.as<uint32_t>(size_t(b.n_nodeblocks));

@@ -185,26 +185,34 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* 
   (void)lane;
 }
 
-// *out = sum of v[0..n) (one workgroup; integer, so order-independent)
-__global__ __launch_bounds__(1024) void k_sum_u32(const uint32_t* __restrict__ v, int64_t n, uint32_t* __restrict__ out) {
-  __shared__ uint32_t s[16];
+// *out += sum of v[0..n): grid-stride partial sums, one atomic add per
+// workgroup (integer, so the result does not depend on the order)
+constexpr int SUM_THREADS = 256;
+__global__ __launch_bounds__(SUM_THREADS) void k_sum_u32(const uint32_t* __restrict__ v, int64_t n,
+                                                         uint32_t* __restrict__ out) {
+  __shared__ uint32_t s[SUM_THREADS / 64];
   uint32_t acc = 0;
-  for (int64_t i = threadIdx.x; i < n; i += 1024) acc += v[i];
+  for (int64_t i = int64_t(blockIdx.x) * SUM_THREADS + threadIdx.x; i < n; i += int64_t(gridDim.x) * SUM_THREADS)
+    acc += v[i];
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t t = 0;
-    for (int w = 0; w < 16; ++w) t += s[w];
-    *out = t;
+    for (int w = 0; w < SUM_THREADS / 64; ++w) t += s[w];
+    if (t) atomicAdd(out, t);
   }
 }
 
 }  // namespace
 
 void sum_u32(tess_ctx* ctx, const uint32_t* v, int64_t n, uint32_t* out) {
-  hipLaunchKernelGGL(k_sum_u32, dim3(1), dim3(1024), 0, ctx->stream, v, n, out);
+  TESS_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), ctx->stream));
+  if (n <= 0) return;
+  // about 8 values per thread, at most 512 workgroups
+  const int64_t blocks = std::min<int64_t>(512, std::max<int64_t>(1, ceil_div(n, int64_t(SUM_THREADS) * 8)));
+  hipLaunchKernelGGL(k_sum_u32, dim3(unsigned(blocks)), dim3(SUM_THREADS), 0, ctx->stream, v, n, out);
   TESS_HIP(hipGetLastError());
 }
 

@@ -159,6 +159,10 @@ struct tess_batch {
   // below the threshold (0: never, 1: every step); reorders done so far
   double reorder_threshold = 0.0;
   int64_t reorders = 0;
+  // the last P2G of this call counted the broken layered pairs (the sum sits in
+  // the counters), over this many pairs
+  bool counted = false;
+  int64_t counted_pairs = 0;
 };
 
 namespace tess {
