@@ -248,7 +248,7 @@ tess_status tess_ctx_destroy(tess_ctx* ctx) {
                             &ctx->partials,  &ctx->counters,    &ctx->host_pinned_dummy, &ctx->reorder_tmp, &ctx->reorder_cnt,
                             &ctx->per_tile,  &ctx->per_nodeblock, &ctx->skeys2, &ctx->perm2,
                             &ctx->inc_nk,    &ctx->inc_spos,  &ctx->inc_mkey, &ctx->inc_mj,
-                            &ctx->inc_mks,   &ctx->inc_mr,    &ctx->inc_mjs,
+                            &ctx->inc_mks,   &ctx->inc_mr,    &ctx->inc_mjs, &ctx->inc_rng,
                             &ctx->sp_occ,    &ctx->sp_act,    &ctx->sp_scan,
                             &ctx->adj_partials, &ctx->adj_nodes, &ctx->ro_checkpoints, &ctx->ro_segment,
                             &ctx->ro_gradv,  &ctx->ro_snapshots, &ctx->movers, &ctx->g2p2g_x,

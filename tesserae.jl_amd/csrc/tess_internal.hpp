@@ -103,7 +103,7 @@ struct tess_ctx {
   tess::Buffer partials;
   tess::Buffer per_tile, per_nodeblock;  // per-workgroup counts summed by a reduction
   // incremental re-binning scratch
-  tess::Buffer skeys2, perm2, inc_nk, inc_spos, inc_mkey, inc_mj, inc_mks, inc_mr, inc_mjs;
+  tess::Buffer skeys2, perm2, inc_nk, inc_spos, inc_mkey, inc_mj, inc_mks, inc_mr, inc_mjs, inc_rng;
   int64_t last_movers = -1;
   int64_t kept_binnings = 0;  // binnings that kept the previous layout (no particle changed its key)
   tess::Buffer counters;   // small device scalars

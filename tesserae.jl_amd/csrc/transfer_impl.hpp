@@ -466,20 +466,29 @@ __global__ void k_place_movers(const uint32_t* __restrict__ mks, const uint32_t*
 // Stayer j moves back by the movers removed before it and forward by the
 // movers inserted before it (sorted movers with a pair (key, j') < (key, j)).
 constexpr int STAY_BLOCK = 256, STAY_CAP = 2048;
+// The movers [ranges[2b], ranges[2b+1]) that fall among the stayers of
+// k_place_stayers' workgroup b, one thread per workgroup (the binary searches
+// run in parallel instead of at the start of every workgroup).
+__global__ void k_stay_ranges(const uint32_t* __restrict__ skeys_old, int64_t n, const uint32_t* __restrict__ mks,
+                              const uint32_t* __restrict__ mjs, int64_t M, int64_t nblocks,
+                              int64_t* __restrict__ ranges) {
+  const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  const int64_t j0 = b * STAY_BLOCK;
+  const int64_t j1 = j0 + STAY_BLOCK < n ? j0 + STAY_BLOCK : n;
+  const int64_t lo = lower_bound_pair(mks, mjs, 0, M, skeys_old[j0], uint32_t(j0));
+  ranges[2 * b] = lo;
+  ranges[2 * b + 1] = lower_bound_pair(mks, mjs, lo, M, skeys_old[j1 - 1], uint32_t(j1));
+}
+
 __global__ __launch_bounds__(STAY_BLOCK) void k_place_stayers(
     const uint32_t* __restrict__ skeys_old, const uint32_t* __restrict__ perm_old, const uint32_t* __restrict__ spos,
-    int64_t n, const uint32_t* __restrict__ mks, const uint32_t* __restrict__ mjs, int64_t M,
+    int64_t n, const uint32_t* __restrict__ mks, const uint32_t* __restrict__ mjs, const int64_t* __restrict__ ranges,
     uint32_t* __restrict__ skeys_new, uint32_t* __restrict__ perm_new) {
   __shared__ uint32_t s_k[STAY_CAP], s_j[STAY_CAP];
-  __shared__ int64_t s_range[2];
   const int64_t j0 = int64_t(blockIdx.x) * STAY_BLOCK;
   const int64_t j1 = j0 + STAY_BLOCK < n ? j0 + STAY_BLOCK : n;
-  if (threadIdx.x == 0) {
-    s_range[0] = lower_bound_pair(mks, mjs, 0, M, skeys_old[j0], uint32_t(j0));
-    s_range[1] = lower_bound_pair(mks, mjs, s_range[0], M, skeys_old[j1 - 1], uint32_t(j1));
-  }
-  __syncthreads();
-  const int64_t a0 = s_range[0], cnt = s_range[1] - s_range[0];
+  const int64_t a0 = ranges[2 * int64_t(blockIdx.x)], cnt = ranges[2 * int64_t(blockIdx.x) + 1] - a0;
   const bool in_lds = cnt <= STAY_CAP;
   if (in_lds)
     for (int64_t t = threadIdx.x; t < cnt; t += STAY_BLOCK) {
@@ -2029,8 +2038,13 @@ inline void merge_movers(tess_ctx* ctx, const uint32_t* nk, const uint32_t* spos
     hipLaunchKernelGGL(k_place_movers, dim3(blocks_for(M, 256)), dim3(256), 0, ctx->stream, mks, mr, mj, skeys, perm,
                        spos, n, M, mjs, skeys2, perm2);
     TESS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_place_stayers, dim3(blocks_for(n, STAY_BLOCK)), dim3(STAY_BLOCK), 0, ctx->stream, skeys,
-                       perm, spos, n, mks, mjs, M, skeys2, perm2);
+    const int64_t nsb = blocks_for(n, STAY_BLOCK);
+    int64_t* ranges = ctx->inc_rng.as<int64_t>(size_t(2 * nsb));
+    hipLaunchKernelGGL(k_stay_ranges, dim3(blocks_for(nsb, 256)), dim3(256), 0, ctx->stream, skeys, n, mks, mjs, M,
+                       nsb, ranges);
+    TESS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_place_stayers, dim3(unsigned(nsb)), dim3(STAY_BLOCK), 0, ctx->stream, skeys, perm, spos, n,
+                       mks, mjs, ranges, skeys2, perm2);
     TESS_HIP(hipGetLastError());
   } else {
     radix_sort_pairs(ctx, nk, perm, skeys2, perm2, n, kbits);
