@@ -86,10 +86,12 @@ def test_batch_rejects_mixed_dimensions():
         th.Batch(members, th.BSpline(th.Quadratic()))
 
 
-def test_batch_reorder_matches_oracle(oracle):
-    """With the layered reorder on (threshold 1: reorder every step), every scene still
-    equals the oracle's run of it alone, up to the order of its particles; particles outside the mesh stay in
-    their scene; the arrays were permuted."""
+@pytest.mark.parametrize("threshold", [1.0, 0.999], ids=["every_step", "by_contiguity"])
+def test_batch_reorder_matches_oracle(oracle, threshold):
+    """With the layered reorder on (threshold 1: every step; below 1: when the contiguity score counted on a
+    call's first step by the layered-order pass, later by the previous step's P2G, is below it), every scene
+    still equals the oracle's run of it alone, up to the order of its particles; particles outside the mesh
+    stay in their scene; the arrays were permuted.  Two calls of two steps each."""
     basis = th.BSpline(th.Quadratic())
     scenes = [random_box(2, 300 + 400 * k, 70 + k, basis, dtype=torch.float64, h=0.05 * (1 + k % 3), lo=-0.05,
                          hi=1.05, stress=0.5) for k in range(6)]
@@ -100,8 +102,9 @@ def test_batch_reorder_matches_oracle(oracle):
         gpu.append(P)
     ctx = th.Context()
     batch = th.Batch(members, basis, ctx=ctx)
-    batch.set_reorder(1.0)
-    batch.step(4)
+    batch.set_reorder(threshold)
+    batch.step(2)
+    batch.step(2)
     ctx.sync()
     permuted = 0
     for sc, P in zip(scenes, gpu):
