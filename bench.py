@@ -205,7 +205,8 @@ def batch_leg(steps, reorder):
 def sphere_leg(dx, steps):
     """Sparse-grid sphere (SpArray 8^3 blocks): 2x2x2 particles per cell inside r=0.3 at
     dx, rigid rotation plus noise, cubic B-spline and uGIMP, FP32.  Positions are generated
-    on the device."""
+    on the device.  Every step runs the reference loop's reorder_particles!(threshold=0.85);
+    the first reorder (out of the generation order) falls in the two warm-up steps."""
     import math
 
     import torch
@@ -239,18 +240,19 @@ def sphere_leg(dx, steps):
         params = th.MPMParams(dt=5e-5, E=5e5, nu=0.35, rho0=1e3, V0=s ** 3)
         G = th.generate_grid(th.SpArray, mesh)
         ctx = th.Context()
-        th.step(G, P, basis, params, nsteps=1, ctx=ctx)  # warm-up: storage, code loading
+        # warm-up: storage, code loading, and the first reorder (decided from step 1's count)
+        th.step(G, P, basis, params, nsteps=2, reorder_threshold=0.85, ctx=ctx)
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         a.record()
-        th.step(G, P, basis, params, nsteps=steps, ctx=ctx)
+        th.step(G, P, basis, params, nsteps=steps, reorder_threshold=0.85, ctx=ctx)
         b.record()
         torch.cuda.synchronize()
         ms = a.elapsed_time(b) / steps
         st = ctx.stats()
         # 260 B per particle (3D FP32), plus l read by both transfers for uGIMP; 104 B per active node
         bytes_step = (260 + (8 if basis.kind == 16 else 0)) * n + 104 * max(st["active_nodes"], 0)
-        out[repr(basis)] = {"ms_per_step": ms, "particle_steps_per_s": n / (ms * 1e-3),
+        out[repr(basis)] = {"ms_per_step": ms, "particle_steps_per_s": n / (ms * 1e-3), "reorders": st["reorders"],
                             "hbm_frac": bytes_step / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
                             "storage_nodes": G.spinds.storage_len, "dense_nodes": mesh.nnodes()}
         ctx.close()
