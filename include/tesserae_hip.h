@@ -284,7 +284,10 @@ tess_status tess_sparsity_update(tess_ctx* ctx, const tess_mesh* mesh, tess_fiel
  * "inactive SpGrid support node" error, runtime.jl:155-161).  G2P reads
  * inactive nodes as zero.  tess_step_sparse runs update_sparsity! before each
  * step's P2G, as the SpGrid loop does (src/Grids/grid.jl:97-101), and fails with TESS_ERR_STATE when the
- * new storage_len exceeds sp->capacity (the caller resizes and calls again). */
+ * new storage_len exceeds sp->capacity.  That check comes before any of the step's own work, so
+ * *steps_done (host, may be NULL) then holds the number of steps that did run, and the caller resizes the
+ * SpArray data vectors to sp->storage_len and calls again for the remaining nsteps - *steps_done steps.
+ * On success *steps_done = nsteps; on any other error it is the number of steps enqueued before it. */
 tess_status tess_p2g_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
                             const tess_sparsity* sp, const tess_particles* particles, const tess_weights* weights,
                             const tess_params* params);
@@ -293,7 +296,7 @@ tess_status tess_g2p_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh,
                             const tess_params* params);
 tess_status tess_step_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
                              tess_sparsity* sp, const tess_particles* particles, const tess_params* params,
-                             int32_t nsteps, double reorder_threshold);
+                             int32_t nsteps, double reorder_threshold, int32_t* steps_done);
 
 /* ---- differentiation (no counterpart in the reference) ------------------- */
 
@@ -376,6 +379,12 @@ tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_
  * bit 2 (0..5); in 3D 0 or 2.  Other numbers are TESS_ERR_ARGUMENT.  No
  * counterpart in the reference, which has no launch variants. */
 tess_status tess_batch_set_variant(tess_batch* batch, int32_t variant);
+/* Counters of the batch, host values, the first n of: [0] reorders done, [1] of
+ * those, reorders decided from the broken-pair count of the previous step's
+ * P2G (not from a layered-order pass on a call's first step), [2] binnings that
+ * kept the previous tile layout, [3] the launch variant.  No counterpart in the
+ * reference (measurement). */
+tess_status tess_batch_stats(const tess_batch* batch, int64_t* out, int32_t n);
 tess_status tess_batch_destroy(tess_batch* batch);
 
 /* ---- output (src/IO/export.jl) ------------------------------------------ */

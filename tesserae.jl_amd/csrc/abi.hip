@@ -432,8 +432,10 @@ tess_status tess_g2p_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh,
 
 tess_status tess_step_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh, const tess_grid* grid,
                              tess_sparsity* sp, const tess_particles* particles, const tess_params* params,
-                             int32_t nsteps, double reorder_threshold) {
-  return guarded([&] {
+                             int32_t nsteps, double reorder_threshold, int32_t* steps_done) {
+  int done = 0;
+  if (steps_done) *steps_done = 0;
+  const tess_status st = guarded([&] {
     check_ctx(ctx);
     check_mesh(mesh);
     check_basis(basis);
@@ -444,8 +446,10 @@ tess_status tess_step_sparse(tess_ctx* ctx, int32_t basis, const tess_mesh* mesh
     if (nsteps < 0) raise(TESS_ERR_ARGUMENT, "nsteps must be non-negative");
     if (!(reorder_threshold >= 0.0 && reorder_threshold <= 1.0))
       raise(TESS_ERR_ARGUMENT, "threshold must be in [0, 1].");
-    run_step_sparse(ctx, basis, *mesh, *grid, *sp, *particles, *params, nsteps, reorder_threshold);
+    run_step_sparse(ctx, basis, *mesh, *grid, *sp, *particles, *params, nsteps, reorder_threshold, &done);
   });
+  if (steps_done) *steps_done = done;
+  return st;
 }
 
 namespace tess {
@@ -554,14 +558,31 @@ tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_
     // split is fixed, G2P with 4 or 2 waves
     const int cand2[6] = {0, 1, 2, 3, 4, 5}, cand3[2] = {0, 2};
     const int ncand = dim == 2 ? 6 : 2;
-    hipEvent_t e0, e1;
-    TESS_HIP(hipEventCreate(&e0));
-    TESS_HIP(hipEventCreate(&e1));
+    // the caller's reorder threshold and variant come back and the events are
+    // released on every exit, a failed step included
+    struct Restore {
+      tess_batch* b;
+      double thr;
+      int variant;
+      hipEvent_t e[2] = {nullptr, nullptr};
+      bool done = false;
+      ~Restore() {
+        for (hipEvent_t ev : e)
+          if (ev) (void)hipEventDestroy(ev);
+        if (!done) {
+          run_batch_set_variant(b, variant);
+          run_batch_set_reorder(b, thr);
+        }
+      }
+    } guard{batch, batch->reorder_threshold, run_batch_variant(batch)};
+    TESS_HIP(hipEventCreate(&guard.e[0]));
+    TESS_HIP(hipEventCreate(&guard.e[1]));
+    hipEvent_t e0 = guard.e[0], e1 = guard.e[1];
     float best = 0.0f;
-    int best_v = run_batch_variant(batch);
+    int best_v = guard.variant;
     // with a reorder threshold set, every variant is timed on a freshly
     // reordered layout and without reorders in its timed steps
-    const double thr = batch->reorder_threshold;
+    const double thr = guard.thr;
     for (int k = 0; k < ncand; ++k) {
       const int v = dim == 2 ? cand2[k] : cand3[k];
       run_batch_set_variant(batch, v);
@@ -580,8 +601,7 @@ tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_
         best_v = v;
       }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    guard.done = true;
     run_batch_set_variant(batch, best_v);
     if (variant) *variant = best_v;
   });
@@ -594,6 +614,14 @@ tess_status tess_batch_set_variant(tess_batch* batch, int32_t variant) {
     if (!ok) raise(TESS_ERR_ARGUMENT, "no launch variant " + std::to_string(variant) + " for a " +
                                           std::to_string(batch->dim) + "D batch");
     run_batch_set_variant(batch, variant);
+  });
+}
+
+tess_status tess_batch_stats(const tess_batch* batch, int64_t* out, int32_t n) {
+  return guarded([&] {
+    if (!batch) raise(TESS_ERR_ARGUMENT, "batch is NULL");
+    if (n < 0 || (n > 0 && !out)) raise(TESS_ERR_ARGUMENT, "out is NULL");
+    run_batch_stats(batch, out, n);
   });
 }
 

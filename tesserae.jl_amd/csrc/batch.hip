@@ -277,7 +277,8 @@ void batch_reorder(tess_ctx* ctx, tess_batch& B, double threshold) {
   uint32_t* counters = ctx->counters.as<uint32_t>(CNT_SLOTS);
   const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
   uint32_t* hc = reinterpret_cast<uint32_t*>(ctx->counters_host);
-  if (threshold < 1.0 && B.counted) {
+  const bool by_p2g_count = threshold < 1.0 && B.counted;
+  if (by_p2g_count) {
     // the previous step's P2G counted the broken pairs; the binning's readback
     // brought the count to the host
     if (!(layered_contiguity(int64_t(hc[CNT_MISPLACED]), B.counted_pairs) < threshold)) return;
@@ -312,6 +313,7 @@ void batch_reorder(tess_ctx* ctx, tess_batch& B, double threshold) {
   TESS_HIP(hipGetLastError());
   TESS_HIP(hipMemcpyAsync(ctx->vals.ptr, pout, size_t(n) * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
   B.reorders += 1;
+  if (by_p2g_count) B.reorders_by_count += 1;
 }
 
 template <typename T, int D, int BASIS, int NP>
@@ -493,6 +495,11 @@ void run_batch_set_variant(tess_batch* b, int variant) { b->variant = variant; }
 
 void run_batch_set_reorder(tess_batch* b, double threshold) { b->reorder_threshold = threshold; }
 int run_batch_variant(const tess_batch* b) { return b->variant; }
+
+void run_batch_stats(const tess_batch* b, int64_t* out, int n) {
+  const int64_t vals[4] = {b->reorders, b->reorders_by_count, b->kept_binnings, b->variant};
+  for (int k = 0; k < n && k < 4; ++k) out[k] = vals[k];
+}
 
 void run_batch_destroy(tess_batch* b) {
   b->recs.release();

@@ -159,6 +159,8 @@ struct tess_batch {
   // below the threshold (0: never, 1: every step); reorders done so far
   double reorder_threshold = 0.0;
   int64_t reorders = 0;
+  // of those, reorders decided from the broken-pair count of the previous step's P2G
+  int64_t reorders_by_count = 0;
   // the last P2G of this call counted the broken layered pairs (the sum sits in
   // the counters), over this many pairs
   bool counted = false;
@@ -208,8 +210,10 @@ void run_p2g_sparse(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_
                     const tess_particles& P, const tess_weights* W, const tess_params& prm);
 void run_g2p_sparse(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_sparsity& sp,
                     const tess_particles& P, const tess_weights* W, const tess_params& prm);
+// *done: steps run before a TESS_ERR_STATE (storage too small), nsteps on success.
 void run_step_sparse(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, tess_sparsity& sp,
-                     const tess_particles& P, const tess_params& prm, int nsteps, double reorder_threshold);
+                     const tess_particles& P, const tess_params& prm, int nsteps, double reorder_threshold,
+                     int* done);
 // ---- adjoint (adjoint.hip) ----
 void run_step_adjoint(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
                       const tess_params& prm, const tess_adjoint& adj);
@@ -232,6 +236,8 @@ void run_batch_step(tess_ctx* ctx, tess_batch* b, int nsteps);
 void run_batch_set_variant(tess_batch* b, int variant);
 void run_batch_set_reorder(tess_batch* b, double threshold);
 int run_batch_variant(const tess_batch* b);
+// out[0..n): reorders, reorders decided from the previous P2G's count, kept binnings, variant
+void run_batch_stats(const tess_batch* b, int64_t* out, int n);
 void run_batch_destroy(tess_batch* b);
 // Moves every particle field so that slot k receives particle order[k].
 void permute_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const uint32_t* order);

@@ -44,7 +44,7 @@ void run_step(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& 
               const tess_params& prm, int nsteps, double reorder_threshold) {
   bind_basis(ctx, basis, nullptr);
   dispatch<StepOp>(basis, mesh, false, ctx, mesh, G, P, prm, nsteps, reorder_threshold,
-                   static_cast<tess_sparsity*>(nullptr));
+                   static_cast<tess_sparsity*>(nullptr), static_cast<int*>(nullptr));
 }
 
 void run_p2g_sparse(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, const tess_sparsity& sp,
@@ -60,9 +60,10 @@ void run_g2p_sparse(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_
 }
 
 void run_step_sparse(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_grid& G, tess_sparsity& sp,
-                     const tess_particles& P, const tess_params& prm, int nsteps, double reorder_threshold) {
+                     const tess_particles& P, const tess_params& prm, int nsteps, double reorder_threshold,
+                     int* done) {
   bind_basis(ctx, basis, nullptr);
-  dispatch<StepOp>(basis, mesh, false, ctx, mesh, G, P, prm, nsteps, reorder_threshold, &sp);
+  dispatch<StepOp>(basis, mesh, false, ctx, mesh, G, P, prm, nsteps, reorder_threshold, &sp, done);
 }
 
 }  // namespace tess

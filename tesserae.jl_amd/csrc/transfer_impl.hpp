@@ -2559,10 +2559,17 @@ void g2p2g_steps_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, 
 // block_ordered_particle_contiguity) is below the threshold (1 reorders every
 // step), the role of reorder_particles!(particles, partition; threshold) in
 // the reference loop.
+//
+// *done (may be NULL) counts the steps whose transfers were enqueued; an SpGrid
+// step that finds its storage too small raises before any of its own work, so
+// after the error *done steps have run and the caller resizes and runs the
+// remaining nsteps - *done.
 template <typename T, int D, int BASIS, bool STORED>
 void step_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-               const tess_params& prm, int nsteps, double reorder_threshold, tess_sparsity* sp) {
+               const tess_params& prm, int nsteps, double reorder_threshold, tess_sparsity* sp,
+               int* done = nullptr) {
   for (int s = 0; s < nsteps; ++s) {
+    if (done) *done = s;
     if (sp) {
       // update_sparsity!(grid, xₚ) at the top of the SpGrid loop (src/Grids/grid.jl:97-101)
       run_sparsity_update(ctx, mesh, P.x, P.n, *sp, nullptr);
@@ -2588,6 +2595,7 @@ void step_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const t
     p2g_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, reorder_threshold > 0.0, sp);
     g2p_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, sp, s + 1 < nsteps);
   }
+  if (done) *done = nsteps;
 }
 
 template <template <typename, int, int, bool> class Op, typename... Args_>
@@ -2648,8 +2656,8 @@ struct G2P2GStepsOp {
 template <typename T, int D, int B, bool S>
 struct StepOp {
   static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_particles& P,
-                  const tess_params& prm, int nsteps, double thr, tess_sparsity* sp) {
-    if constexpr (!S) step_impl<T, D, B, false>(ctx, m, G, P, prm, nsteps, thr, sp);
+                  const tess_params& prm, int nsteps, double thr, tess_sparsity* sp, int* done) {
+    if constexpr (!S) step_impl<T, D, B, false>(ctx, m, G, P, prm, nsteps, thr, sp, done);
   }
 };
 

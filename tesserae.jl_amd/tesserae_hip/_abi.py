@@ -215,7 +215,7 @@ SIGNATURES = {
     "tess_step_sparse": (
         ctypes.c_int32,
         [ctypes.c_void_p, ctypes.c_int32, _P(tess_mesh), _P(tess_grid), _P(tess_sparsity), _P(tess_particles),
-         _P(tess_params), ctypes.c_int32, ctypes.c_double],
+         _P(tess_params), ctypes.c_int32, ctypes.c_double, _P(ctypes.c_int32)],
     ),
     "tess_step_adjoint": (
         ctypes.c_int32,
@@ -239,6 +239,7 @@ SIGNATURES = {
     "tess_batch_autotune": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, _P(ctypes.c_int32)]),
     "tess_batch_set_reorder": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double]),
     "tess_batch_set_variant": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "tess_batch_stats": (ctypes.c_int32, [ctypes.c_void_p, _P(ctypes.c_int64), ctypes.c_int32]),
     "tess_batch_destroy": (ctypes.c_int32, [ctypes.c_void_p]),
     "tess_profile_enable": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "tess_profile_read": (

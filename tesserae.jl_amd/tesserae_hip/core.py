@@ -817,22 +817,24 @@ def step(grid: Grid, particles: Particles, basis: Basis, params: MPMParams, nste
     ctx = ctx or default_context()
     P = particles.abi()
     if isinstance(grid, SpGrid):
-        # update_sparsity!(grid, x) runs inside every step; the storage grows when the
-        # active blocks outgrow it and the step is retried (nothing of it has run yet)
+        # update_sparsity!(grid, x) runs inside every step; when the active blocks
+        # outgrow the storage the library stops before that step's work and reports
+        # how many steps ran; the storage grows and the remaining steps follow
         if grid.capacity == 0:
             update_sparsity(grid, particles.x, ctx=ctx)
         s = 0
         while s < nsteps:
             sp, G = grid.sparsity_abi(), grid.abi()
+            done = ctypes.c_int32(0)
             st = ctx.lib.tess_step_sparse(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
-                                          ctypes.byref(sp), ctypes.byref(P), ctypes.byref(params.abi()), 1,
-                                          float(reorder_threshold))
+                                          ctypes.byref(sp), ctypes.byref(P), ctypes.byref(params.abi()),
+                                          int(nsteps - s), float(reorder_threshold), ctypes.byref(done))
             grid.spinds.storage_len = int(sp.storage_len)
+            s += int(done.value)
             if st == _abi.TESS_ERR_STATE and sp.storage_len > grid.capacity:
                 grid.resize_fillzero(int(sp.storage_len * 1.25) + 1)
                 continue
             _check(ctx.lib, st)
-            s += 1
         return
     G = grid.abi()
     _check(ctx.lib, ctx.lib.tess_step(ctx.handle, basis.kind, ctypes.byref(grid.mesh.abi()), ctypes.byref(G),
@@ -931,6 +933,11 @@ class Batch:
     def set_variant(self, variant: int) -> None:
         """Launch variant to use (a number autotune returned): in 2D 0..5, in 3D 0 or 2."""
         _check(self.ctx.lib, self.ctx.lib.tess_batch_set_variant(self.handle, int(variant)))
+
+    def stats(self) -> dict:
+        vals = (ctypes.c_int64 * 4)()
+        _check(self.ctx.lib, self.ctx.lib.tess_batch_stats(self.handle, vals, 4))
+        return dict(zip(("reorders", "reorders_by_count", "kept_binnings", "variant"), list(vals)))
 
     def autotune(self, steps_per_variant: int = 2) -> int:
         v = ctypes.c_int32(0)

@@ -106,6 +106,12 @@ def test_batch_reorder_matches_oracle(oracle, threshold):
     batch.step(2)
     batch.step(2)
     ctx.sync()
+    stats = batch.stats()
+    print(f"[batch reorder] threshold={threshold} {stats}", flush=True)
+    assert stats["reorders"] >= 1
+    if threshold < 1.0:
+        # within a call, steps after the first decide from the previous step's P2G count of broken pairs
+        assert stats["reorders_by_count"] >= 1
     permuted = 0
     for sc, P in zip(scenes, gpu):
         st = HostState(sc)

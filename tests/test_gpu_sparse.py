@@ -124,3 +124,50 @@ def test_sparse_errors():
     G0 = _sp_grid(m0)
     with pytest.raises(th.UnsupportedError):
         th.update_sparsity(G0, P.x)
+
+
+def test_step_sparse_storage_grows_mid_call():
+    """A fast-moving box crosses into new blocks during a 12-step call whose storage fits only the first
+    active set: tess_step_sparse stops before the step that needs more storage and reports the steps that
+    ran; after resizing, the remaining steps complete the run, which equals 12 dense-grid steps."""
+    import ctypes
+
+    from tesserae_hip import _abi
+    basis = th.BSpline(th.Quadratic())
+    sc = random_box(3, 3000, 21, basis, dtype=torch.float64, h=0.02, lo=0.2, hi=0.3)
+    sc.mesh.block_size_log2 = 2
+    sc.v[:] = (2.0, 0.5, 0.0)
+    sc.params = th.MPMParams(dt=5e-3, E=1e3, nu=0.3, rho0=1e3, V0=sc.params.V0)
+    nsteps = 12
+    mesh, Pd, Gd = to_gpu(sc)
+    _, Ps, _ = to_gpu(sc)
+    Gs = _sp_grid(mesh)
+    ctx = th.Context()
+    th.update_sparsity(Gs, Ps.x, ctx=ctx)
+    assert Gs.capacity == Gs.spinds.storage_len
+    lib = ctx.lib
+    P = Ps.abi()
+    done_total, calls, stops = 0, 0, []
+    while done_total < nsteps:
+        sp, G = Gs.sparsity_abi(), Gs.abi()
+        done = ctypes.c_int32(-1)
+        st = lib.tess_step_sparse(ctx.handle, basis.kind, ctypes.byref(mesh.abi()), ctypes.byref(G),
+                                  ctypes.byref(sp), ctypes.byref(P), ctypes.byref(sc.params.abi()),
+                                  nsteps - done_total, 0.0, ctypes.byref(done))
+        calls += 1
+        Gs.spinds.storage_len = int(sp.storage_len)
+        done_total += done.value
+        if st == _abi.TESS_ERR_STATE:
+            assert 0 <= done.value < nsteps and sp.storage_len > Gs.capacity
+            stops.append(done.value)
+            Gs.resize_fillzero(int(sp.storage_len))
+        else:
+            assert st == _abi.TESS_OK and done_total == nsteps
+    print(f"[storage growth] calls={calls} stopped after {stops} steps", flush=True)
+    assert stops and stops[0] > 0  # the storage had to grow after steps of the same call had run
+    th.step(Gd, Pd, basis, sc.params, nsteps=nsteps, ctx=th.Context())
+    torch.cuda.synchronize()
+    a, b = particles_np(Pd), particles_np(Ps)
+    for name in a:
+        assert rel_err(b[name], a[name]) <= 1e-12, name
+    ctx.close()
