@@ -27,18 +27,39 @@ for p in (os.path.join(ROOT, "tesserae.jl_amd"), os.path.join(ROOT, "tests")):
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the block-drop kernels from two rocprofv3 --pmc runs
-# (FETCH_SIZE, WRITE_SIZE) scaled by a plain 1 GiB copy; written next to the
-# rocprofv3 summaries it comes from
+# (FETCH_SIZE, then WRITE_SIZE) scaled by the same counters' readings for the
+# library's plain copy kernel; written by `bench.py --traffic-from` with a digest
+# of the sources it measured, and used only while the sources still match
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "block_drop_hbm_traffic.json")
+SOURCE_GLOBS = ("tesserae.jl_amd/csrc/*.hip", "tesserae.jl_amd/csrc/*.hpp", "tesserae.jl_amd/csrc/Makefile",
+                "include/*.h")
+
+
+def sources_digest():
+    """sha256 over the library's sources (path and content, sorted by path)."""
+    import glob
+    import hashlib
+
+    h = hashlib.sha256()
+    for pat in SOURCE_GLOBS:
+        for f in sorted(glob.glob(os.path.join(ROOT, pat))):
+            h.update(os.path.relpath(f, ROOT).encode())
+            with open(f, "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
 def profiled_traffic(kernel):
+    """(bytes per launch, note): None with the reason when the file is missing or measured other sources."""
     try:
         with open(TRAFFIC_FILE) as f:
-            rec = json.load(f)["kernels"][kernel]
-        return float(rec["bytes"])
+            doc = json.load(f)
+        rec = doc["kernels"][kernel]
     except (OSError, KeyError, ValueError, TypeError):
-        return None
+        return None, "no counter record for this kernel"
+    if doc.get("sources_digest") != sources_digest():
+        return None, f"counter record measured sources {doc.get('sources_digest')}, these are {sources_digest()}"
+    return float(rec["bytes"]), os.path.relpath(TRAFFIC_FILE, ROOT)
 
 
 def parse():
@@ -58,7 +79,11 @@ def parse():
     ap.add_argument("--no-g2p2g", action="store_true", help="skip the G2P2G leg")
     ap.add_argument("--sphere-dx", type=float, default=1 / 512)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="first run a plain device copy of MIB MiB (reference for HBM counter scaling)")
+                    help="first run the library's plain copy of MIB MiB three times (the reference that scales "
+                         "HBM counters to bytes)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write the HBM traffic record from two rocprofv3 --pmc output folders and exit")
+    ap.add_argument("--cpu-steps", type=int, default=5, help="timed steps of each CPU baseline run")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--checkpoint-every", type=int, default=20)
     return ap.parse_args()
@@ -68,54 +93,118 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def copy_bandwidth(mib=1024, reps=5):
-    """Device-to-device copy of `mib` MiB timed with HIP events: the box's measured HBM copy rate in GB/s
-    (bytes read + bytes written per second)."""
+def copy_bandwidth(ctx, mib=1024, reps=5):
+    """The library's plain device copy (tess_device_copy) of `mib` MiB timed with HIP events on its stream: the
+    box's measured copy rate in GB/s (bytes read + bytes written per second)."""
     import torch
 
     src = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
-    dst.copy_(src)
+    ctx.device_copy(src, dst)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
     e0.record()
     for _ in range(reps):
-        dst.copy_(src)
+        ctx.device_copy(src, dst)
     e1.record()
     e1.synchronize()
     gbs = 2 * src.numel() * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    assert bool(torch.equal(src, dst))
     del src, dst
     torch.cuda.empty_cache()
     return gbs
 
 
-def cpu_baseline(threads):
-    """Oracle threaded restatement (stored weights, CPU block partition, colour-phased
-    P2G, threaded G2P) on a bounded sample: the same scene at dx=1/128 (1.7M
-    particles), Float32, 2 steps."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(threads, dtype, steps):
+    """The oracle's threaded restatement of the reference CPU algorithm (stored weights, the CPU block
+    partition, colour-phased P2G, threaded G2P) on the headline workload itself (13,434,880 particles):
+    one warm-up step, then `steps` steps timed one by one."""
     import numpy as np
     import torch
 
     from oracle_bind import HostState, Oracle
     from tesserae_hip.scenes import block_drop
 
-    sc = block_drop(dtype=torch.float32, dx=1 / 128, steps=2)
+    sc = block_drop(dtype=dtype)
     st = HostState(sc)
     orc = Oracle()
     W = orc.step_threaded(st, sc.basis, sc.params, threads, nsteps=1)  # warm-up (allocations, page faults)
-    t0 = time.perf_counter()
-    orc.step_threaded(st, sc.basis, sc.params, threads, nsteps=2, weights=W)
-    dt = time.perf_counter() - t0
-    return {"value": sc.n * 2 / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"block drop dx=1/128 ({sc.n} particles), Float32, 2 timed steps after 1 warm-up, "
-                      f"{threads} threads, {dt:.2f} s"}
+    per = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        orc.step_threaded(st, sc.basis, sc.params, threads, nsteps=1, weights=W)
+        per.append(time.perf_counter() - t0)
+    med, mean = float(np.median(per)), float(np.mean(per))
+    tag = "Float32" if dtype == torch.float32 else "Float64"
+    return {"value": sc.n / med, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "median_s_per_step": med, "mean_s_per_step": mean, "mean_value": sc.n / mean,
+            "sample": f"block drop dx=1/256 ({sc.n} particles), {tag}, {steps} steps timed one by one after 1 "
+                      f"warm-up step, {threads} threads, {sum(per):.1f} s; value from the median step"}
+
+
+def traffic_record(fetch_dir, write_dir):
+    """Per-kernel HBM bytes per launch from the counter CSVs of two rocprofv3 runs of
+    `bench.py --steps 3 --warmup 2 --no-... --copy-probe 1024`: each kernel's first dispatch (the cold first step
+    after the scene set-up) is left out; FETCH_SIZE and WRITE_SIZE (KiB) are scaled by their readings for the
+    library's copy kernel, which reads and writes exactly 1 GiB per dispatch."""
+    import csv
+    import glob
+
+    def load(d, counter):
+        rows = []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for r in csv.DictReader(fh):
+                    if r.get("Counter_Name") == counter:
+                        rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+        rows.sort()
+        per = {}
+        for _, name, v in rows:
+            per.setdefault(name, []).append(v)
+        return per
+
+    fetch, write = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    copy_kib = 1 << 20
+    cf = [v for k, vs in fetch.items() if "k_stream_copy" in k for v in vs]
+    cw = [v for k, vs in write.items() if "k_stream_copy" in k for v in vs]
+    fscale, wscale = copy_kib / (sum(cf) / len(cf)), copy_kib / (sum(cw) / len(cw))
+    alg = {"k_g2p_fast": 196 * 13434880, "k_p2g_fast": 64 * 13434880, "k_gather_grid": None}
+    out = {}
+    for key, label in (("k_g2p_fast", "k_g2p_tile"), ("k_p2g_fast", "k_p2g_tile"), ("k_gather_grid", "k_gather_grid")):
+        name = next((k for k in fetch if key + "<float, 3" in k), None)
+        if name is None or name not in write:
+            continue
+        fr, wr = fetch[name][1:], write[name][1:]
+        rd = sum(fr) / len(fr) * 1024 * fscale
+        wb = sum(wr) / len(wr) * 1024 * wscale
+        out[label] = {"kernel": name, "dispatches": len(fr), "bytes_read": rd, "bytes_written": wb,
+                      "bytes": rd + wb}
+    return {"workload": "3D block drop dx=1/256 FP32, steady-state dispatches (first dispatch left out)",
+            "sources_digest": sources_digest(), "fetch_scale": fscale, "write_scale": wscale,
+            "commands": [f"rocprofv3 --pmc {c} -d {d} -o run --output-format csv -- python3 bench.py --steps 3 "
+                         "--warmup 2 --no-cpu --no-adjoint --no-batch --no-sphere --no-taylor --no-g2p2g "
+                         "--copy-probe 1024" for c, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir))],
+            "kernels": out}
 
 
 def adjoint_ratio(nsteps, every):
     """Differentiable inverse cube (dx=1/64, 132,651 particles, quadratic), nsteps steps.
-    ratio: the checkpointed pullback of the loss mean ||x(T) - (x(0) + (0.1,0,0))||^2
-    (replay of every segment from its checkpoint plus the reversed steps; FP64 adjoint)
-    over the plain FP32 forward of the same steps.  ratio_with_rollout adds the
-    checkpointed forward (the rollout that records the checkpoints) to the numerator."""
+    ratio: forward and backward, checkpoint recompute included (the checkpointed forward
+    that records the checkpoints, then the pullback of the loss mean ||x(T) - (x(0) +
+    (0.1,0,0))||^2: replay of every segment from its checkpoint plus the reversed steps,
+    FP64 adjoint) over the plain FP32 forward of the same steps.  pullback_over_forward
+    leaves the checkpointed forward out."""
     import numpy as np
     import torch
 
@@ -156,7 +245,7 @@ def adjoint_ratio(nsteps, every):
     return {"workload": f"inverse cube dx=1/64, {sc.n} particles, quadratic, {nsteps} steps, checkpoint every "
                         f"{every}, FP32 forward / FP64 adjoint",
             "t_forward_ms": t_fwd, "t_rollout_ms": t_roll, "t_pullback_ms": t_adj,
-            "ratio": t_adj / t_fwd, "ratio_with_rollout": (t_roll + t_adj) / t_fwd, "loss": loss,
+            "ratio": (t_roll + t_adj) / t_fwd, "pullback_over_forward": t_adj / t_fwd, "loss": loss,
             "grad_v0_norm": gnorm}
 
 
@@ -334,6 +423,12 @@ def taylor_leg(steps):
 
 def main():
     args = parse()
+    if args.traffic_from:
+        rec = traffic_record(*args.traffic_from)
+        with open(TRAFFIC_FILE, "w") as f:
+            json.dump(rec, f, indent=1)
+        print(json.dumps(rec, indent=1))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
@@ -347,12 +442,12 @@ def main():
 
     torch.cuda.set_device(0)
     if args.copy_probe:
-        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counter readings scale the
-        # engine's kernels' readings to bytes
+        # the library's plain copy of known size: its FETCH_SIZE / WRITE_SIZE counter readings
+        # scale the engine's kernels' readings to bytes
         src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            th.default_context().device_copy(src, dst)
         torch.cuda.synchronize()
         del src, dst
         log(f"bench: copy probe {args.copy_probe} MiB x3")
@@ -399,6 +494,7 @@ def main():
     dom_ms = prof[dom][0] / prof[dom][1]
     dom_bytes = prof[dom][2] / prof[dom][1]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic, traffic_src = profiled_traffic(dom)
     out = {
         "metric": "3D MPM particle-steps/s at 13M pcls, % of HBM peak; adjoint/forward time ratio",
         "value": value,
@@ -415,8 +511,7 @@ def main():
         "config": {"workload": "3D elastic block drop, dx=1/256, quadratic B-spline, 13,434,880 particles, FP32",
                    "particles": n, "layout": args.layout},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom),
-                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
         "step_hbm": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (ms * 1e-3) / 1e9,
                      "frac": step_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), "active_nodes": stats["active_nodes"]},
@@ -442,14 +537,20 @@ def main():
     if not args.no_adjoint:
         log("bench: adjoint/forward ratio")
         out["adjoint"] = adjoint_ratio(args.adjoint_steps, args.checkpoint_every)
+    out["copy_GBs"] = copy_bandwidth(ctx)
+    log(f"bench: device copy {out['copy_GBs']:.0f} GB/s")
     if not args.no_cpu:
         log("bench: CPU baseline")
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(threads)
+        del P, G
+        torch.cuda.empty_cache()
+        nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        # the box's CPU share for one GPU is 16 threads, whatever nproc shows
+        threads = args.cpu_threads or min(16, nproc)
+        out["cpu_baseline"] = cpu_baseline(threads, torch.float64, args.cpu_steps)
+        out["cpu_baseline"]["host"] = {"nproc": nproc, "cpu_count": os.cpu_count(), "cpu_model": cpu_model()}
+        out["cpu_baseline_f32"] = cpu_baseline(threads, torch.float32, args.cpu_steps)
         if threads != 8:  # the thread count of the reference's published CPU numbers
-            out["cpu_baseline_8_threads"] = cpu_baseline(8)
-    out["copy_GBs"] = copy_bandwidth()
-    log(f"bench: device copy {out['copy_GBs']:.0f} GB/s")
+            out["cpu_baseline_8_threads"] = cpu_baseline(8, torch.float64, max(2, args.cpu_steps // 2))
     print(json.dumps(out))
 
 

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define TESS_ABI_VERSION 2
+#define TESS_ABI_VERSION 3
 
 typedef int32_t tess_status;
 enum {
@@ -457,6 +457,11 @@ tess_status tess_profile_read(tess_ctx* ctx, int32_t index, char* name, int32_t 
  * re-binning (-1 after a full sort), out[7] binnings that kept the previous
  * tile layout because no particle changed its key. */
 tess_status tess_stats(tess_ctx* ctx, int64_t* out, int32_t count);
+/* A plain device copy of `bytes` (a multiple of 16, both pointers 16-byte
+ * aligned) on the context's stream, one 16-byte load and store per element: the
+ * library's own measure of the copy bandwidth the box reaches.  No counterpart
+ * in the reference (measurement).  Does not synchronise. */
+tess_status tess_device_copy(tess_ctx* ctx, const void* src, void* dst, int64_t bytes);
 
 #ifdef __cplusplus
 }

@@ -711,6 +711,17 @@ tess_status tess_profile_read(tess_ctx* ctx, int32_t index, char* name, int32_t 
   });
 }
 
+tess_status tess_device_copy(tess_ctx* ctx, const void* src, void* dst, int64_t bytes) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (bytes < 0 || bytes % 16) raise(TESS_ERR_ARGUMENT, "bytes must be a non-negative multiple of 16");
+    if (bytes > 0 && (!src || !dst)) raise(TESS_ERR_ARGUMENT, "src or dst is NULL");
+    if ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15)
+      raise(TESS_ERR_ARGUMENT, "src and dst must be 16-byte aligned");
+    run_device_copy(ctx, src, dst, bytes);
+  });
+}
+
 tess_status tess_stats(tess_ctx* ctx, int64_t* out, int32_t count) {
   return guarded([&] {
     check_ctx(ctx);

@@ -205,7 +205,32 @@ __global__ __launch_bounds__(SUM_THREADS) void k_sum_u32(const uint32_t* __restr
   }
 }
 
+// Copy of n 16-byte words: each thread issues four independent loads before
+// its four stores, over a grid-stride loop.
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                      int64_t n) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+    dst[i] = a;
+    dst[i + stride] = b;
+    dst[i + 2 * stride] = c;
+    dst[i + 3 * stride] = d;
+  }
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
+
+void run_device_copy(tess_ctx* ctx, const void* src, void* dst, int64_t bytes) {
+  const int64_t n = bytes / 16;
+  if (n == 0) return;
+  const int64_t blocks = std::min<int64_t>(ceil_div(n, int64_t(256)), 8192);
+  hipLaunchKernelGGL(k_stream_copy, dim3(unsigned(blocks)), dim3(256), 0, ctx->stream,
+                     static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+  TESS_HIP(hipGetLastError());
+}
 
 void sum_u32(tess_ctx* ctx, const uint32_t* v, int64_t n, uint32_t* out) {
   TESS_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), ctx->stream));
@@ -216,23 +241,21 @@ void sum_u32(tess_ctx* ctx, const uint32_t* v, int64_t n, uint32_t* out) {
   TESS_HIP(hipGetLastError());
 }
 
-void exclusive_scan_u32(tess_ctx* ctx, uint32_t* data, int64_t n, uint32_t* total_dev) {
-  if (n <= 0) {
-    if (total_dev) TESS_HIP(hipMemsetAsync(total_dev, 0, sizeof(uint32_t), ctx->stream));
-    return;
-  }
+size_t scan_work_bytes(int64_t n) {
+  size_t bytes = 256;
+  for (int64_t m = ceil_div(n, SCAN_CHUNK); m > 1; m = ceil_div(m, SCAN_CHUNK)) bytes += size_t(m) * sizeof(uint32_t) + 256;
+  return bytes;
+}
+
+void exclusive_scan_u32_ws(hipStream_t stream, uint8_t* tmp, uint32_t* data, int64_t n, uint32_t* total_dev) {
+  if (n <= 0) return;
   const int64_t nchunks = ceil_div(n, SCAN_CHUNK);
   if (nchunks == 1) {
-    hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, data, n, nullptr, total_dev);
+    hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(SCAN_THREADS), 0, stream, data, n, nullptr, total_dev);
     TESS_HIP(hipGetLastError());
     return;
   }
-  // block sums live in a per-level slice of scan_tmp (levels shrink by 4096x)
-  size_t levels_bytes = 0;
-  for (int64_t m = nchunks; m > 1; m = ceil_div(m, SCAN_CHUNK)) levels_bytes += size_t(m) * sizeof(uint32_t) + 256;
-  uint8_t* tmp = static_cast<uint8_t*>(ctx->scan_tmp.get(levels_bytes + 256));
-  (void)tmp;
-  // recursive formulation with explicit slices
+  // block sums live in a per-level slice of tmp (levels shrink by 4096x)
   struct Level {
     uint32_t* data;
     int64_t n;
@@ -243,22 +266,59 @@ void exclusive_scan_u32(tess_ctx* ctx, uint32_t* data, int64_t n, uint32_t* tota
   size_t off = 0;
   while (ceil_div(lv[nl - 1].n, SCAN_CHUNK) > 1) {
     const int64_t m = ceil_div(lv[nl - 1].n, SCAN_CHUNK);
-    uint32_t* sums = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ctx->scan_tmp.ptr) + off);
+    uint32_t* sums = reinterpret_cast<uint32_t*>(tmp + off);
     off += size_t(m) * sizeof(uint32_t) + 256;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(unsigned(m)), dim3(SCAN_THREADS), 0, ctx->stream, lv[nl - 1].data,
+    hipLaunchKernelGGL(k_scan_reduce, dim3(unsigned(m)), dim3(SCAN_THREADS), 0, stream, lv[nl - 1].data,
                        lv[nl - 1].n, sums);
     TESS_HIP(hipGetLastError());
     lv[nl++] = {sums, m};
   }
   // top level fits one chunk
-  hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, lv[nl - 1].data, lv[nl - 1].n,
-                     nullptr, nl == 1 ? total_dev : nullptr);
+  hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(SCAN_THREADS), 0, stream, lv[nl - 1].data, lv[nl - 1].n, nullptr,
+                     nl == 1 ? total_dev : nullptr);
   TESS_HIP(hipGetLastError());
   for (int l = nl - 2; l >= 0; --l) {
     const int64_t m = ceil_div(lv[l].n, SCAN_CHUNK);
-    hipLaunchKernelGGL(k_scan_apply, dim3(unsigned(m)), dim3(SCAN_THREADS), 0, ctx->stream, lv[l].data, lv[l].n,
+    hipLaunchKernelGGL(k_scan_apply, dim3(unsigned(m)), dim3(SCAN_THREADS), 0, stream, lv[l].data, lv[l].n,
                        lv[l + 1].data, l == 0 ? total_dev : nullptr);
     TESS_HIP(hipGetLastError());
+  }
+}
+
+void exclusive_scan_u32(tess_ctx* ctx, uint32_t* data, int64_t n, uint32_t* total_dev) {
+  if (n <= 0) {
+    if (total_dev) TESS_HIP(hipMemsetAsync(total_dev, 0, sizeof(uint32_t), ctx->stream));
+    return;
+  }
+  uint8_t* tmp = ceil_div(n, SCAN_CHUNK) > 1 ? static_cast<uint8_t*>(ctx->scan_tmp.get(scan_work_bytes(n))) : nullptr;
+  exclusive_scan_u32_ws(ctx->stream, tmp, data, n, total_dev);
+}
+
+size_t sort_hist_words(int64_t n) { return size_t(ceil_div(n, SORT_CHUNK)) * RADIX; }
+
+void radix_sort_pairs_ws(const SortWork& w, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                         uint32_t* vals_out, int64_t n, int key_bits) {
+  if (n <= 0) return;
+  if (n > int64_t(UINT32_MAX)) raise(TESS_ERR_ARGUMENT, "radix sort: more than 2^32-1 elements");
+  const int passes = key_bits <= 0 ? 1 : (key_bits + RADIX_BITS - 1) / RADIX_BITS;
+  const int64_t nchunks = ceil_div(n, SORT_CHUNK);
+  // ping-pong so that the last pass writes (keys_out, vals_out)
+  const uint32_t* kin = keys_in;
+  const uint32_t* vin = vals_in;
+  for (int p = 0; p < passes; ++p) {
+    const bool to_out = ((passes - 1 - p) % 2) == 0;
+    uint32_t* kout = to_out ? keys_out : w.keys_alt;
+    uint32_t* vout = to_out ? vals_out : w.vals_alt;
+    const int shift = p * RADIX_BITS;
+    hipLaunchKernelGGL(k_radix_hist, dim3(unsigned(nchunks)), dim3(SORT_THREADS), 0, w.stream, kin, n, shift,
+                       w.hist, nchunks);
+    TESS_HIP(hipGetLastError());
+    exclusive_scan_u32_ws(w.stream, w.scan_tmp, w.hist, nchunks * RADIX, nullptr);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(unsigned(nchunks)), dim3(SORT_THREADS), 0, w.stream, kin, vin, kout,
+                       vout, n, shift, w.hist, nchunks);
+    TESS_HIP(hipGetLastError());
+    kin = kout;
+    vin = vout;
   }
 }
 
@@ -266,32 +326,15 @@ void radix_sort_pairs(tess_ctx* ctx, const uint32_t* keys_in, const uint32_t* va
                       uint32_t* vals_out, int64_t n, int key_bits) {
   if (n <= 0) return;
   if (n > int64_t(UINT32_MAX)) raise(TESS_ERR_ARGUMENT, "radix sort: more than 2^32-1 elements");
-  const int passes = key_bits <= 0 ? 1 : (key_bits + RADIX_BITS - 1) / RADIX_BITS;
-  const int64_t nchunks = ceil_div(n, SORT_CHUNK);
-  uint32_t* hist = ctx->hist.as<uint32_t>(size_t(nchunks) * RADIX);
-  uint32_t* ka = ctx->keys_alt.as<uint32_t>(size_t(n));
-  uint32_t* va = ctx->vals_alt.as<uint32_t>(size_t(n));
-  // ping-pong so that the last pass writes (keys_out, vals_out)
-  const uint32_t* kin = keys_in;
-  const uint32_t* vin = vals_in;
-  for (int p = 0; p < passes; ++p) {
-    const bool last = p == passes - 1;
-    // choose the target so that the final pass lands in *_out
-    const bool to_out = ((passes - 1 - p) % 2) == 0;
-    uint32_t* kout = to_out ? keys_out : ka;
-    uint32_t* vout = to_out ? vals_out : va;
-    const int shift = p * RADIX_BITS;
-    hipLaunchKernelGGL(k_radix_hist, dim3(unsigned(nchunks)), dim3(SORT_THREADS), 0, ctx->stream, kin, n, shift,
-                       hist, nchunks);
-    TESS_HIP(hipGetLastError());
-    exclusive_scan_u32(ctx, hist, nchunks * RADIX, nullptr);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(unsigned(nchunks)), dim3(SORT_THREADS), 0, ctx->stream, kin, vin, kout,
-                       vout, n, shift, hist, nchunks);
-    TESS_HIP(hipGetLastError());
-    kin = kout;
-    vin = vout;
-    (void)last;
-  }
+  SortWork w;
+  w.stream = ctx->stream;
+  const size_t hw = sort_hist_words(n);
+  w.hist = ctx->hist.as<uint32_t>(hw);
+  w.keys_alt = ctx->keys_alt.as<uint32_t>(size_t(n));
+  w.vals_alt = ctx->vals_alt.as<uint32_t>(size_t(n));
+  w.scan_tmp = ceil_div(int64_t(hw), SCAN_CHUNK) > 1 ? static_cast<uint8_t*>(ctx->scan_tmp.get(scan_work_bytes(int64_t(hw))))
+                                                      : nullptr;
+  radix_sort_pairs_ws(w, keys_in, vals_in, keys_out, vals_out, n, key_bits);
 }
 
 }  // namespace tess

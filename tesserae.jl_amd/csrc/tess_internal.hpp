@@ -170,6 +170,8 @@ struct tess_batch {
 namespace tess {
 
 // ---- scans and sorts (sort.hip) ----
+// Plain copy of bytes (a multiple of 16) on the context's stream.
+void run_device_copy(tess_ctx* ctx, const void* src, void* dst, int64_t bytes);
 // Exclusive prefix sum of n uint32 values in place; returns nothing, total is
 // written to *total_dev (device) when non-null.
 void exclusive_scan_u32(tess_ctx* ctx, uint32_t* data, int64_t n, uint32_t* total_dev);
@@ -179,6 +181,20 @@ void sum_u32(tess_ctx* ctx, const uint32_t* v, int64_t n, uint32_t* out);
 // vals_in == nullptr means values 0..n-1.  Result lands in (keys_out, vals_out).
 void radix_sort_pairs(tess_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                       uint32_t* vals_out, int64_t n, int key_bits);
+// The same sort over caller-provided work buffers on a stream (no allocation):
+// hist holds sort_hist_words(n) words, keys_alt/vals_alt n words each, scan_tmp
+// scan_work_bytes(sort_hist_words(n)) bytes.
+struct SortWork {
+  hipStream_t stream = nullptr;
+  uint32_t* hist = nullptr;
+  uint32_t* keys_alt = nullptr;
+  uint32_t* vals_alt = nullptr;
+  uint8_t* scan_tmp = nullptr;
+};
+size_t sort_hist_words(int64_t n);
+size_t scan_work_bytes(int64_t n);
+void radix_sort_pairs_ws(const SortWork& w, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                         uint32_t* vals_out, int64_t n, int key_bits);
 
 // ---- reference-layout weights and partition (partition.hip) ----
 void launch_update_weights(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_particles& P,

@@ -908,12 +908,13 @@ __device__ __forceinline__ double det_d(const T* F) {
 }
 
 // Neo-Hookean Cauchy stress σ = (μ(FFᵀ-I) + λ lnJ I)/J and V = V0 J of the
-// updated F (README.md:84-86), stiffness scaled by exp(logE) when has_logE,
-// evaluated in Float64 for both storage types (cofactor determinant, log(J)).
-// Float64 vector instructions issue at the rate of unpacked Float32 ones here,
-// and this form is shorter than a Float32 one that keeps the relative accuracy
-// of small strains (log1p of J - 1 from the minors of F - I).  σ is written as
-// sig[r + D*b].
+// updated F (README.md:84-86), stiffness scaled by exp(logE) when has_logE.
+// J = det F (cofactor expansion), ln J and 1/J are Float64 for both storage
+// types, so a small or large J keeps its relative accuracy.  Float64 storage
+// forms FFᵀ - I as written.  Float32 storage forms it from H = F - I as
+// H + Hᵀ + HHᵀ in Float32: the difference of FFᵀ and I would lose the digits
+// of a small strain, H itself is exact, and nothing in the sum cancels for F
+// near I.  σ is written as sig[r + D*b].
 template <typename T, int D>
 __device__ __forceinline__ void neo_hookean(const T* Fn, double lam0, double mu0, double V0, bool has_logE, T logE,
                                             T* sig, T& Vout) {
@@ -926,16 +927,36 @@ __device__ __forceinline__ void neo_hookean(const T* Fn, double lam0, double mu0
   }
   const double lnJ = log(J);
   const double invJ = 1.0 / J;
+  if constexpr (sizeof(T) == 4) {
+    const float muJ = float(mu * invJ);
+    const float lamJ = float(lam * lnJ * invJ);
+    float H[D * D];
 #pragma unroll
-  for (int b = 0; b < D; ++b)
+    for (int b = 0; b < D; ++b)
 #pragma unroll
-    for (int r = 0; r < D; ++r) {
-      double bb = 0.0;
+      for (int r = 0; r < D; ++r) H[r + D * b] = Fn[r + D * b] - (r == b ? 1.0f : 0.0f);
 #pragma unroll
-      for (int k = 0; k < D; ++k) bb += double(Fn[r + D * k]) * double(Fn[b + D * k]);
-      const double sg = mu * (bb - (r == b ? 1.0 : 0.0)) + (r == b ? lam * lnJ : 0.0);
-      sig[r + D * b] = T(sg * invJ);
-    }
+    for (int b = 0; b < D; ++b)
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        float hh = H[r] * H[b];  // (HHᵀ)_rb = Σ_k H_rk H_bk
+#pragma unroll
+        for (int k = 1; k < D; ++k) hh = fmaf(H[r + D * k], H[b + D * k], hh);
+        const float bp = (H[r + D * b] + H[b + D * r]) + hh;
+        sig[r + D * b] = r == b ? fmaf(muJ, bp, lamJ) : muJ * bp;
+      }
+  } else {
+#pragma unroll
+    for (int b = 0; b < D; ++b)
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        double bb = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) bb += double(Fn[r + D * k]) * double(Fn[b + D * k]);
+        const double sg = mu * (bb - (r == b ? 1.0 : 0.0)) + (r == b ? lam * lnJ : 0.0);
+        sig[r + D * b] = T(sg * invJ);
+      }
+  }
   Vout = T(V0 * J);
 }
 
@@ -1199,14 +1220,26 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
         else
           axis_jets_mixed<T, BASIS>(x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], l, hinv, jv, jd);
       };
-      if constexpr (NPASS == 1) {
+#ifdef TESS_P2G_NO_JET_SHARE
+      constexpr bool OWN_JETS = true;
+#else
+      constexpr bool OWN_JETS = NPASS == 1;
+#endif
+      if constexpr (OWN_JETS) {
         axis_jets(std::integral_constant<int, 0>{}, val[0], der[0]);
         axis_jets(std::integral_constant<int, 1>{}, val[1], der[1]);
         if constexpr (D == 3) axis_jets(std::integral_constant<int, D - 1>{}, val[D - 1], der[D - 1]);
 #pragma unroll
         for (int q = 0; q < KN; ++q) {
+          // this wave's last-axis offsets, selected without indexing registers by a runtime value
           vl[q] = val[D - 1][q];
           dl[q] = der[D - 1][q];
+#pragma unroll
+          for (int o = 1; o < SW; ++o)
+            if (o == pass * KN + q) {
+              vl[q] = val[D - 1][o];
+              dl[q] = der[D - 1][o];
+            }
         }
       } else {
         // the waves hold the same particles: wave d % NPASS evaluates axis d
@@ -2263,7 +2296,11 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
                          ctx->stream, a, sk, misplaced);
     } else {
       // 3D: one wave per offset along z; 2D: one wave while the stencil fits
+#ifdef TESS_P2G_ONE_WAVE_3D
+      constexpr int NPF = (Pow<SW, D>::v * NV * int(sizeof(T)) / 4 > 200) ? SW : 1;
+#else
       constexpr int NPF = (D == 3 || Pow<SW, D>::v * NV * int(sizeof(T)) / 4 > 96) ? SW : 1;
+#endif
       const bool aos = julia_layout(P, D);
       bool done = false;
       if constexpr (BASIS == TESS_BSPLINE_QUADRATIC || BASIS == TESS_BSPLINE_CUBIC) {

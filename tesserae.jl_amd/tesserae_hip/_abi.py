@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes
 import os
 
-TESS_ABI_VERSION = 2  # tess_weights carries the node filter of corrected bases
+TESS_ABI_VERSION = 3  # tess_step_sparse reports the steps it ran (v2: tess_weights carries a node filter)
 
 TESS_OK = 0
 TESS_ERR_ARGUMENT = 1
@@ -247,6 +247,7 @@ SIGNATURES = {
         [ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32, _P(ctypes.c_double), _P(ctypes.c_int64),
          _P(ctypes.c_double)],
     ),
+    "tess_device_copy": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     "tess_stats": (ctypes.c_int32, [ctypes.c_void_p, _P(ctypes.c_int64), ctypes.c_int32]),
 }
 

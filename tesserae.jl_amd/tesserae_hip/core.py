@@ -608,6 +608,14 @@ class Context:
             out[name.value.decode()] = (ms.value, calls.value, nbytes.value)
             i += 1
 
+    def device_copy(self, src: torch.Tensor, dst: torch.Tensor) -> None:
+        """The library's plain copy kernel from src to dst (same byte size, a multiple of 16)."""
+        nb = src.numel() * src.element_size()
+        if dst.numel() * dst.element_size() != nb:
+            raise DimensionMismatch("device_copy: src and dst differ in size")
+        _check(self.lib, self.lib.tess_device_copy(self.handle, ctypes.c_void_p(src.data_ptr()),
+                                                   ctypes.c_void_p(dst.data_ptr()), nb))
+
     def stats(self) -> dict:
         vals = (ctypes.c_int64 * 8)()
         _check(self.lib, self.lib.tess_stats(self.handle, vals, 8))
