@@ -182,6 +182,21 @@ tess_status tess_partition_update(tess_ctx* ctx, const tess_mesh* mesh, tess_fie
                                   int32_t* particleindices, int32_t* starts, int32_t* stops,
                                   int64_t* nassigned);
 
+/* The same partition without allocating: no counterpart in the reference.  It
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Results equal
+ * tess_partition_update's; *nassigned is DEVICE memory (one int64) and nothing
+ * is read back, so the call does not synchronise.  Work runs on `stream` (the
+ * current device's; NULL = its default stream) over a fixed work area of the
+ * library (about 21 MB per GPU, held until the process ends) sized for at most
+ * TESS_PARTITION_NOALLOC_MAX particles; larger n is TESS_ERR_ARGUMENT.  All
+ * calls on one GPU share that area: calls that overlap on a device race, so
+ * serialise them (one stream, or events).  The first call on a device resolves
+ * the area's address (tess_ctx_create does it for its device). */
+#define TESS_PARTITION_NOALLOC_MAX 1048576
+tess_status tess_partition_update_noalloc(const tess_mesh* mesh, tess_field x, int64_t n, int32_t* particleindices,
+                                          int32_t* starts, int32_t* stops, int64_t* nassigned, void* stream);
+
 /* reorder_particles!(particles, partition; threshold)
  * (src/Parallel/cpu_block_strategy.jl:248-355): permutes every particle field
  * into the partition's block order when the block-ordered contiguity is below

@@ -231,6 +231,8 @@ tess_status tess_ctx_create(int32_t device, void* stream, tess_ctx** out) {
     c->counters_host = static_cast<int64_t*>(pinned);
     // device counters start at zero (the binning reads slots written by the previous P2G)
     TESS_HIP(hipMemset(c->counters.get(64 * sizeof(uint32_t)), 0, 64 * sizeof(uint32_t)));
+    // the allocation-free partition's work area on this device
+    noalloc_scratch_resolve();
     *out = c;
   });
 }
@@ -293,6 +295,23 @@ tess_status tess_partition_update(tess_ctx* ctx, const tess_mesh* mesh, tess_fie
     if (!starts || !stops || !nassigned || (!particleindices && n > 0))
       raise(TESS_ERR_ARGUMENT, "partition output arrays must not be NULL");
     run_partition_update(ctx, *mesh, x, n, particleindices, starts, stops, nassigned);
+  });
+}
+
+tess_status tess_partition_update_noalloc(const tess_mesh* mesh, tess_field x, int64_t n, int32_t* particleindices,
+                                          int32_t* starts, int32_t* stops, int64_t* nassigned, void* stream) {
+  return guarded([&] {
+    check_mesh(mesh);
+    if (n < 0) raise(TESS_ERR_DIMENSION, "negative particle count");
+    if (n > partition_noalloc_max_particles())
+      raise(TESS_ERR_ARGUMENT, "tess_partition_update_noalloc handles at most " +
+                                   std::to_string(partition_noalloc_max_particles()) + " particles, got " +
+                                   std::to_string(n));
+    if (!x.ptr && n > 0) raise(TESS_ERR_ARGUMENT, "missing field x");
+    if (!starts || !stops || !nassigned || (!particleindices && n > 0))
+      raise(TESS_ERR_ARGUMENT, "partition output arrays must not be NULL");
+    run_partition_update_noalloc(*mesh, x, n, particleindices, starts, stops, nassigned,
+                                 static_cast<hipStream_t>(stream));
   });
 }
 

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "tess_correct.hpp"
 #include "tess_device.hpp"
@@ -358,6 +359,94 @@ void partition_typed(tess_ctx* ctx, const tess_mesh& mesh, tess_field x, int64_t
   *nassigned = n > 0 ? int64_t(hc[0]) : 0;
 }
 
+// ---- partition without allocation --------------------------------------------
+//
+// tess_partition_update_noalloc runs the partition above over a fixed work area
+// instead of the context's growing buffers: keys, sorted keys, values, the sort's
+// ping-pong keys and values (n words each), the digit histograms, the scan's
+// block sums and one count.  The area is a file-scope device array sized for
+// kNoAllocMaxParticles; every GPU that loads the library holds its own copy
+// until the process ends.  Calls on one GPU share it: calls that overlap on a
+// device race.
+
+constexpr int64_t kNoAllocMaxParticles = TESS_PARTITION_NOALLOC_MAX;
+static_assert(kNoAllocMaxParticles % 4096 == 0, "the histogram area counts whole 4096-key sort chunks");
+constexpr size_t kAlign = 256;
+constexpr size_t align_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+constexpr size_t kNoAllocWords = size_t(kNoAllocMaxParticles) * sizeof(uint32_t);
+// 4096-key sort chunks × 256 digits; the scan of those words has 16 blocks of 4096
+constexpr size_t kNoAllocHistBytes = size_t(kNoAllocMaxParticles / 4096) * 256 * sizeof(uint32_t);
+constexpr size_t kNoAllocScanBytes = 4096;
+static_assert(kNoAllocMaxParticles / 4096 * 256 <= 4096 * 4096, "the histogram scan takes two levels");
+constexpr size_t kNoAllocScratchBytes = 5 * align_up(kNoAllocWords) + align_up(kNoAllocHistBytes) + kNoAllocScanBytes +
+                                        kAlign;
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+__global__ void k_fill_i32(int32_t* __restrict__ a, int64_t n, int32_t v) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void k_store_nassigned(const uint32_t* __restrict__ count, int64_t n, int64_t* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = n > 0 ? int64_t(*count) : 0;
+}
+
+template <typename T, int D>
+void partition_noalloc_typed(const tess_mesh& mesh, tess_field x, int64_t n, int32_t* pidx, int32_t* starts,
+                             int32_t* stops, int64_t* nassigned, hipStream_t stream, uint8_t* area) {
+  const int L = mesh.block_size_log2;
+  int nbd[3] = {1, 1, 1};
+  uint64_t nbt = 1;
+  for (int d = 0; d < D; ++d) {
+    nbd[d] = int(((mesh.dims[d] - 1) >> L) + 1);  // nblocks, cartesian.jl:241-243
+    nbt *= uint64_t(nbd[d]);
+  }
+  const uint64_t outside = (uint64_t(1) << D) * nbt;
+  if (outside >= uint64_t(UINT32_MAX) || n >= int64_t(INT32_MAX))
+    raise(TESS_ERR_ARGUMENT, "Partition block count exceeds packed block id capacity.");
+  // carve the area: five n-word arrays, histograms, scan block sums, the count
+  uint8_t* q = area;
+  auto take = [&](size_t bytes) {
+    uint8_t* r = q;
+    q += align_up(bytes);
+    return r;
+  };
+  uint32_t* keys = reinterpret_cast<uint32_t*>(take(kNoAllocWords));
+  uint32_t* skeys = reinterpret_cast<uint32_t*>(take(kNoAllocWords));
+  uint32_t* vals = reinterpret_cast<uint32_t*>(take(kNoAllocWords));
+  SortWork w;
+  w.stream = stream;
+  w.keys_alt = reinterpret_cast<uint32_t*>(take(kNoAllocWords));
+  w.vals_alt = reinterpret_cast<uint32_t*>(take(kNoAllocWords));
+  w.hist = reinterpret_cast<uint32_t*>(take(kNoAllocHistBytes));
+  w.scan_tmp = take(kNoAllocScanBytes);
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t)));
+  // fill kernels rather than memset nodes, so that a captured graph replays them
+  hipLaunchKernelGGL(k_fill_i32, dim3(nblk(int64_t(nbt), 256)), dim3(256), 0, stream, starts, int64_t(nbt), 0);
+  TESS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_fill_i32, dim3(nblk(int64_t(nbt), 256)), dim3(256), 0, stream, stops, int64_t(nbt), 0);
+  TESS_HIP(hipGetLastError());
+  if (n > 0) {
+    hipLaunchKernelGGL((k_partition_keys<T, D>), dim3(nblk(n, 256)), dim3(256), 0, stream, dmesh<T, D>(mesh),
+                       dfield<T>(x), n, L, make_int3(nbd[0], nbd[1], nbd[2]), uint32_t(nbt), keys);
+    TESS_HIP(hipGetLastError());
+    int bits = 0;
+    while ((uint64_t(1) << bits) <= outside) ++bits;
+    radix_sort_pairs_ws(w, keys, nullptr, skeys, vals, n, bits);
+    // exactly one thread writes the count when n > 0
+    hipLaunchKernelGGL(k_partition_runs, dim3(nblk(n, 256)), dim3(256), 0, stream, skeys, vals, n, uint32_t(nbt),
+                       uint32_t(outside), pidx, starts, stops, cnt);
+    TESS_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_store_nassigned, dim3(1), dim3(64), 0, stream, cnt, n, nassigned);
+  TESS_HIP(hipGetLastError());
+}
+
 template <typename U>
 void permute_field(tess_ctx* ctx, const tess_field& f, int ncomp, const uint32_t* perm, int64_t n) {
   if (!f.ptr || n == 0) return;
@@ -391,6 +480,43 @@ void run_partition_update(tess_ctx* ctx, const tess_mesh& mesh, tess_field x, in
     return partition_typed<double, 2>(ctx, mesh, x, n, pidx, starts, stops, nassigned);
   if (mesh.dim == 3 && mesh.dtype == TESS_FLOAT64)
     return partition_typed<double, 3>(ctx, mesh, x, n, pidx, starts, stops, nassigned);
+  raise(TESS_ERR_UNSUPPORTED, "unsupported (dim, dtype)");
+}
+
+int64_t partition_noalloc_max_particles() { return kNoAllocMaxParticles; }
+
+void noalloc_scratch_resolve() {
+  int dev = 0;
+  TESS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) raise(TESS_ERR_DEVICE, "device ordinal beyond the library's table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return;
+  void* p = nullptr;
+  // each device holds its own copy of the array at its own address; host code
+  // uses this address, never the array's name
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    raise(TESS_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[dev] = p;
+}
+
+void run_partition_update_noalloc(const tess_mesh& mesh, tess_field x, int64_t n, int32_t* pidx, int32_t* starts,
+                                  int32_t* stops, int64_t* nassigned, hipStream_t stream) {
+  if (n > kNoAllocMaxParticles)
+    raise(TESS_ERR_ARGUMENT, "tess_partition_update_noalloc handles at most " + std::to_string(kNoAllocMaxParticles) +
+                                 " particles, got " + std::to_string(n));
+  noalloc_scratch_resolve();
+  int dev = 0;
+  TESS_HIP(hipGetDevice(&dev));
+  uint8_t* area = static_cast<uint8_t*>(g_scratch_ptr[dev]);
+  if (mesh.dim == 2 && mesh.dtype == TESS_FLOAT32)
+    return partition_noalloc_typed<float, 2>(mesh, x, n, pidx, starts, stops, nassigned, stream, area);
+  if (mesh.dim == 3 && mesh.dtype == TESS_FLOAT32)
+    return partition_noalloc_typed<float, 3>(mesh, x, n, pidx, starts, stops, nassigned, stream, area);
+  if (mesh.dim == 2 && mesh.dtype == TESS_FLOAT64)
+    return partition_noalloc_typed<double, 2>(mesh, x, n, pidx, starts, stops, nassigned, stream, area);
+  if (mesh.dim == 3 && mesh.dtype == TESS_FLOAT64)
+    return partition_noalloc_typed<double, 3>(mesh, x, n, pidx, starts, stops, nassigned, stream, area);
   raise(TESS_ERR_UNSUPPORTED, "unsupported (dim, dtype)");
 }
 

@@ -201,6 +201,14 @@ void launch_update_weights(tess_ctx* ctx, int basis, const tess_mesh& mesh, cons
                            const tess_weights& W);
 void run_partition_update(tess_ctx* ctx, const tess_mesh& mesh, tess_field x, int64_t n, int32_t* pidx,
                           int32_t* starts, int32_t* stops, int64_t* nassigned);
+// The partition over the library's fixed device work area (no allocation);
+// *nassigned is device memory.  Raises TESS_ERR_ARGUMENT beyond
+// partition_noalloc_max_particles().
+void run_partition_update_noalloc(const tess_mesh& mesh, tess_field x, int64_t n, int32_t* pidx, int32_t* starts,
+                                  int32_t* stops, int64_t* nassigned, hipStream_t stream);
+int64_t partition_noalloc_max_particles();
+// Resolves the work area's address on the current device (once per device).
+void noalloc_scratch_resolve();
 void run_reorder(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, int32_t* pidx,
                  int64_t nassigned, double threshold, int32_t* reordered);
 

@@ -36,6 +36,8 @@ TESS_WLS = 0x100
 TESS_KERNEL_CORRECTION = 0x200
 TESS_POLY_MULTILINEAR = 0x400
 
+TESS_PARTITION_NOALLOC_MAX = 1048576  # particles of tess_partition_update_noalloc
+
 LIB_NAME = "libtesserae_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
@@ -167,6 +169,11 @@ SIGNATURES = {
         ctypes.c_int32,
         [ctypes.c_void_p, _P(tess_mesh), tess_field, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
          ctypes.c_void_p, _P(ctypes.c_int64)],
+    ),
+    "tess_partition_update_noalloc": (
+        ctypes.c_int32,
+        [_P(tess_mesh), tess_field, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+         ctypes.c_void_p, ctypes.c_void_p],
     ),
     "tess_reorder_particles": (
         ctypes.c_int32,
