@@ -205,20 +205,12 @@ __global__ __launch_bounds__(SUM_THREADS) void k_sum_u32(const uint32_t* __restr
   }
 }
 
-// Copy of n 16-byte words: each thread issues four independent loads before
-// its four stores, over a grid-stride loop.
+// Copy of n 16-byte words, one per thread (the form that measured fastest
+// against grid-stride loops with several loads in flight per thread).
 __global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                       int64_t n) {
-  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n; i += 4 * stride) {
-    const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-    dst[i] = a;
-    dst[i + stride] = b;
-    dst[i + 2 * stride] = c;
-    dst[i + 3 * stride] = d;
-  }
-  for (; i < n; i += stride) dst[i] = src[i];
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
 }
 
 }  // namespace
@@ -226,7 +218,8 @@ __global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ s
 void run_device_copy(tess_ctx* ctx, const void* src, void* dst, int64_t bytes) {
   const int64_t n = bytes / 16;
   if (n == 0) return;
-  const int64_t blocks = std::min<int64_t>(ceil_div(n, int64_t(256)), 8192);
+  if (n > int64_t(UINT32_MAX) * 256) raise(TESS_ERR_ARGUMENT, "device copy larger than one launch");
+  const int64_t blocks = ceil_div(n, int64_t(256));
   hipLaunchKernelGGL(k_stream_copy, dim3(unsigned(blocks)), dim3(256), 0, ctx->stream,
                      static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
   TESS_HIP(hipGetLastError());

@@ -687,9 +687,19 @@ __device__ __forceinline__ uint64_t lanemask_lt64() { return (uint64_t(1) << (th
 
 // Per-class counts of one tile from its slice of the sorted keys (class = low 6
 // bits), by wave-level matching; lane c gets count and offset of class c.
+// sorted slots per tile whose particle ids the transfer kernels stage in LDS
+constexpr int PID_CAP = 1024;
+
 template <int NW = 1>
+// With s_pid, also stages the particle ids (perm - poff) of the tile's first
+// `cap` sorted slots in LDS, and with s_key their sort keys: the transfer
+// kernels then address their particle loads through LDS reads instead of
+// global loads that the memory counter would drain behind earlier traffic.
 __device__ __forceinline__ void tile_class_counts(const uint32_t* __restrict__ skeys, uint32_t start, int cnt,
-                                                  uint32_t* s_cnt, int& ccount, int& coff, int& maxcnt) {
+                                                  uint32_t* s_cnt, int& ccount, int& coff, int& maxcnt,
+                                                  const uint32_t* __restrict__ perm = nullptr, uint32_t poff = 0,
+                                                  uint32_t* s_pid = nullptr, uint32_t* s_key = nullptr,
+                                                  int cap = 0) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform: scalar branches
   if (wave == 0) s_cnt[lane] = 0;
@@ -697,7 +707,12 @@ __device__ __forceinline__ void tile_class_counts(const uint32_t* __restrict__ s
   for (int base = wave * 64; base < cnt; base += 64 * NW) {
     const int i = base + lane;
     const bool valid = i < cnt;
-    const uint32_t c = valid ? (skeys[start + i] & 63u) : 0u;
+    const uint32_t key = valid ? skeys[start + i] : 0u;
+    const uint32_t c = key & 63u;
+    if (s_pid && valid && i < cap) {
+      s_pid[i] = perm[start + i] - poff;
+      if (s_key) s_key[i] = key;
+    }
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
@@ -1154,7 +1169,9 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
     s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
   }
   int ccount, coff, maxcnt;
-  tile_class_counts<NPASS>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);  // barrier after the staging
+  __shared__ uint32_t s_pid[PID_CAP];
+  tile_class_counts<NPASS>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt, a.perm, poff, s_pid, nullptr,
+                           PID_CAP);  // barrier after the staging
   int ccoord[D];
   {
     int r = lane;
@@ -1186,11 +1203,22 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
     P2GIn<T, D> pa, pb;
     std::memset(static_cast<void*>(&pa), 0, sizeof(pa));
     std::memset(static_cast<void*>(&pb), 0, sizeof(pb));
-    if (ccount > 0) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff] - poff, pa);
+    // particle ids come from LDS (staged with the class counts), a layer ahead
+    // of the loads they address
+    auto pid_of = [&](int k, auto staged) {
+      const int li = coff + k;
+      if (k >= ccount) return 0u;
+      if constexpr (decltype(staged)::value)
+        return s_pid[li];
+      else
+        return a.perm[start + li] - poff;
+    };
     // one layer: the accumulation runs on every lane (a branch around it made
     // the compiler copy all accumulators at each iteration)
-    auto layer = [&](int k, const P2GIn<T, D>& cur, P2GIn<T, D>& nxt) {
-      if (k + 1 < ccount) p2g_load<T, D, BASIS, AOS>(a, a.perm[start + coff + k + 1] - poff, nxt);
+    auto layer = [&](int k, const P2GIn<T, D>& cur, P2GIn<T, D>& nxt, uint32_t q_next, uint32_t& q_after,
+                     auto staged) {
+      q_after = pid_of(k + 2, staged);
+      if (k + 1 < ccount) p2g_load<T, D, BASIS, AOS>(a, q_next, nxt);
       const uint64_t has = __ballot(k < ccount);
       const bool act = k < ccount;
       if (misplaced && pass == 0) bad += broken_pair(has, k, act ? cur.pid : 0u, last);
@@ -1220,26 +1248,14 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
         else
           axis_jets_mixed<T, BASIS>(x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], l, hinv, jv, jd);
       };
-#ifdef TESS_P2G_NO_JET_SHARE
-      constexpr bool OWN_JETS = true;
-#else
-      constexpr bool OWN_JETS = NPASS == 1;
-#endif
-      if constexpr (OWN_JETS) {
+      if constexpr (NPASS == 1) {
         axis_jets(std::integral_constant<int, 0>{}, val[0], der[0]);
         axis_jets(std::integral_constant<int, 1>{}, val[1], der[1]);
         if constexpr (D == 3) axis_jets(std::integral_constant<int, D - 1>{}, val[D - 1], der[D - 1]);
 #pragma unroll
         for (int q = 0; q < KN; ++q) {
-          // this wave's last-axis offsets, selected without indexing registers by a runtime value
           vl[q] = val[D - 1][q];
           dl[q] = der[D - 1][q];
-#pragma unroll
-          for (int o = 1; o < SW; ++o)
-            if (o == pass * KN + q) {
-              vl[q] = val[D - 1][o];
-              dl[q] = der[D - 1][o];
-            }
         }
       } else {
         // the waves hold the same particles: wave d % NPASS evaluates axis d
@@ -1326,10 +1342,18 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
         }
       }
     };
-    for (int k = 0; k < maxcnt; k += 2) {
-      layer(k, pa, pb);
-      if (k + 1 < maxcnt) layer(k + 1, pb, pa);
-    }
+    auto walk = [&](auto staged) {
+      uint32_t qa = pid_of(1, staged), qb = 0u;
+      if (ccount > 0) p2g_load<T, D, BASIS, AOS>(a, pid_of(0, staged), pa);
+      for (int k = 0; k < maxcnt; k += 2) {
+        layer(k, pa, pb, qa, qb, staged);
+        if (k + 1 < maxcnt) layer(k + 1, pb, pa, qb, qa, staged);
+      }
+    };
+    if (cnt <= PID_CAP)
+      walk(std::true_type{});
+    else
+      walk(std::false_type{});
     // lanes are distinct classes, so for one offset their target nodes are
     // distinct; offsets are applied one after another (read, add, write: the
     // LDS executes one wave's accesses in order) and the waves one after
@@ -1384,7 +1408,7 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
 // and prefetches its next particle while updating the current one.
 template <typename T, int D, bool AOS>
 struct G2PIn {
-  uint32_t pid, cls;  // particle, window-start class in the tile
+  uint32_t pid, cls, key;  // particle, window-start class in the tile, sort key at its slot
   T x[D], v[D], F[D * D], l;
 };
 
@@ -1461,7 +1485,9 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
     s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
   }
   int ccount, coff, maxcnt;
-  tile_class_counts<NW>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);  // barrier after the staging
+  __shared__ uint32_t s_pid[PID_CAP], s_key[PID_CAP];
+  tile_class_counts<NW>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt, a.perm, poff, s_pid, s_key,
+                        PID_CAP);  // barrier after the staging
   const T hinv = a.mesh.h_inv, dt = a.dt;
   // Two walks of the tile's particles.  Class order (lane = class, iteration k =
   // the k-th particle of every class) reads the layered storage contiguously
@@ -1473,21 +1499,45 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
   const int kend = by_sorted ? n_sorted : maxcnt;
   auto live = [&](int k) { return by_sorted ? k * 64 + lane < cnt : k < ccount; };
   auto slot = [&](int k) { return start + uint32_t(by_sorted ? k * 64 + lane : coff + k); };  // sorted position
-  auto fetch = [&](int k, G2PIn<T, D, AOS>& in) {
+  // a particle's id and sort key come from LDS (staged with the class
+  // counts), an iteration ahead of the loads they address: those loads do not
+  // wait behind the previous particle's stores, as a global load of the id
+  // would (the memory counter drains in issue order)
+  struct Ahead {
+    uint32_t pid, key;
+  };
+  // staged: the tile's ids are all in LDS (cnt <= PID_CAP); the walk is
+  // compiled for each case so that neither waits on the other's loads
+  auto ahead = [&](int k, auto staged) {
+    Ahead r{0u, 0u};
+    if (live(k)) {
+      const uint32_t li = slot(k) - start;
+      if constexpr (decltype(staged)::value) {
+        r.pid = s_pid[li];
+        r.key = s_key[li];
+      } else {
+        r.pid = a.perm[start + li] - poff;
+        r.key = skeys[start + li];
+      }
+    }
+    return r;
+  };
+  auto fetch = [&](int k, const Ahead& q, G2PIn<T, D, AOS>& in) {
     if (!live(k)) return;
-    const uint32_t jj = slot(k);
-    g2p_load<T, D, BASIS, AOS>(a, a.perm[jj] - poff, in);
-    in.cls = by_sorted ? skeys[jj] & 63u : uint32_t(lane);
+    g2p_load<T, D, BASIS, AOS>(a, q.pid, in);
+    in.key = q.key;
+    in.cls = by_sorted ? q.key & 63u : uint32_t(lane);
   };
   // two register sets alternate between the particle being updated and the
   // one being loaded (wave w walks k = w, w+NW, ...)
   G2PIn<T, D, AOS> pa, pb;
   std::memset(static_cast<void*>(&pa), 0, sizeof(pa));
   std::memset(static_cast<void*>(&pb), 0, sizeof(pb));
-  fetch(wave, pa);
-  auto particle = [&](int k, const G2PIn<T, D, AOS>& cur, G2PIn<T, D, AOS>& nxt) {
+  auto particle = [&](int k, const G2PIn<T, D, AOS>& cur, G2PIn<T, D, AOS>& nxt, const Ahead& q_next,
+                      Ahead& q_after, auto staged) {
     const bool act = live(k);
-    fetch(k + NW, nxt);
+    q_after = ahead(k + 2 * NW, staged);
+    fetch(k + NW, q_next, nxt);
     if (!act) return;
     int cc[D];  // the particle's class coordinates (its window's offset in the footprint)
     {
@@ -1600,7 +1650,7 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
       const uint32_t kl = key_of_position<T, D, BASIS>(a, xn, cur.l, nl);
       const uint32_t key = kl == nl * 64u ? ntiles * 64u : kl + (tile - ltile) * 64u;
       nk[jj] = key;
-      const bool moved = key != skeys[jj];
+      const bool moved = key != cur.key;
       stay[jj] = moved ? 0u : 1u;
       // particles whose key changed, counted per wave by its lowest live lane
       // (no atomic in the common case of none)
@@ -1626,10 +1676,18 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
     for (int q = 0; q < D * D; ++q) st_g2p(fldw<T, D * D, AOS>(a.sigma, pid, q), sg[q]);
     st_g2p(a.V(pid), Vn);
   };
-  for (int k = wave; k < kend; k += 2 * NW) {
-    particle(k, pa, pb);
-    if (k + NW < kend) particle(k + NW, pb, pa);
-  }
+  auto walk = [&](auto staged) {
+    Ahead qa = ahead(wave + NW, staged), qb{0u, 0u};
+    fetch(wave, ahead(wave, staged), pa);
+    for (int k = wave; k < kend; k += 2 * NW) {
+      particle(k, pa, pb, qa, qb, staged);
+      if (k + NW < kend) particle(k + NW, pb, pa, qb, qa, staged);
+    }
+  };
+  if (cnt <= PID_CAP)
+    walk(std::true_type{});
+  else
+    walk(std::false_type{});
 }
 
 // ---- gather + grid-node half ------------------------------------------------
@@ -2296,11 +2354,7 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
                          ctx->stream, a, sk, misplaced);
     } else {
       // 3D: one wave per offset along z; 2D: one wave while the stencil fits
-#ifdef TESS_P2G_ONE_WAVE_3D
-      constexpr int NPF = (Pow<SW, D>::v * NV * int(sizeof(T)) / 4 > 200) ? SW : 1;
-#else
       constexpr int NPF = (D == 3 || Pow<SW, D>::v * NV * int(sizeof(T)) / 4 > 96) ? SW : 1;
-#endif
       const bool aos = julia_layout(P, D);
       bool done = false;
       if constexpr (BASIS == TESS_BSPLINE_QUADRATIC || BASIS == TESS_BSPLINE_CUBIC) {

