@@ -204,6 +204,59 @@ void check_sparsity(const tess_mesh* m, const tess_sparsity* sp, bool transfer) 
 
 using namespace tess;
 
+namespace tess {
+
+tess_ctx* ctx_new(int device, hipStream_t stream) {
+  tess_ctx* c = new tess_ctx();
+  c->device = device;
+  c->stream = stream;
+  void* pinned = nullptr;
+  // pinned readback slots: counters (words 0-63), mover counts (words 64-319)
+  hipError_t e = hipHostMalloc(&pinned, 512 * sizeof(int64_t), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    delete c;
+    check_hip(e, "hipHostMalloc");
+  }
+  c->counters_host = static_cast<int64_t*>(pinned);
+  try {
+    // device counters start at zero (the binning reads slots written by the previous P2G)
+    TESS_HIP(hipMemset(c->counters.get(64 * sizeof(uint32_t)), 0, 64 * sizeof(uint32_t)));
+  } catch (...) {
+    ctx_free(c);
+    throw;
+  }
+  return c;
+}
+
+void ctx_free(tess_ctx* ctx) {
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  else (void)hipDeviceSynchronize();
+  if (ctx->replay) {
+    hipStream_t rs = ctx->replay->stream;
+    ctx_free(ctx->replay);
+    if (rs) (void)hipStreamDestroy(rs);
+    ctx->replay = nullptr;
+  }
+  tess::Buffer* bufs[] = {&ctx->keys,      &ctx->keys_sorted, &ctx->vals,         &ctx->keys_alt,
+                          &ctx->vals_alt,  &ctx->hist,        &ctx->scan_tmp,     &ctx->tile_start,
+                          &ctx->tile_end,  &ctx->tile_flag,   &ctx->tile_slot,    &ctx->active_tiles,
+                          &ctx->nb_flag,   &ctx->nb_slot,     &ctx->nb_list,      &ctx->nb_list_prev,
+                          &ctx->partials,  &ctx->counters,    &ctx->host_pinned_dummy, &ctx->reorder_tmp, &ctx->reorder_cnt,
+                          &ctx->per_tile,  &ctx->per_nodeblock, &ctx->skeys2, &ctx->perm2,
+                          &ctx->inc_nk,    &ctx->inc_spos,  &ctx->inc_mkey, &ctx->inc_mj,
+                          &ctx->inc_mks,   &ctx->inc_mr,    &ctx->inc_mjs, &ctx->inc_rng,
+                          &ctx->sp_occ,    &ctx->sp_act,    &ctx->sp_scan,
+                          &ctx->adj_partials, &ctx->adj_nodes, &ctx->ro_checkpoints, &ctx->ro_segment,
+                          &ctx->ro_gradv,  &ctx->ro_snapshots, &ctx->ro_grid, &ctx->movers, &ctx->g2p2g_x,
+                          &ctx->export_buf};
+  for (auto* b : bufs) b->release();
+  for (auto ev : ctx->prof.pool) (void)hipEventDestroy(ev);
+  if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
+  delete ctx;
+}
+
+}  // namespace tess
+
 extern "C" {
 
 int32_t tess_abi_version(void) { return TESS_ABI_VERSION; }
@@ -218,21 +271,14 @@ tess_status tess_ctx_create(int32_t device, void* stream, tess_ctx** out) {
     TESS_HIP(hipGetDeviceCount(&count));
     if (device < 0 || device >= count) raise(TESS_ERR_ARGUMENT, "no such HIP device");
     TESS_HIP(hipSetDevice(device));
-    tess_ctx* c = new tess_ctx();
-    c->device = device;
-    c->stream = static_cast<hipStream_t>(stream);
-    void* pinned = nullptr;
-    // pinned readback slots: counters (words 0-63), mover counts (words 64-319)
-    hipError_t e = hipHostMalloc(&pinned, 512 * sizeof(int64_t), hipHostMallocDefault);
-    if (e != hipSuccess) {
-      delete c;
-      check_hip(e, "hipHostMalloc");
+    tess_ctx* c = ctx_new(device, static_cast<hipStream_t>(stream));
+    try {
+      // the allocation-free partition's work area on this device
+      noalloc_scratch_resolve();
+    } catch (...) {
+      ctx_free(c);
+      throw;
     }
-    c->counters_host = static_cast<int64_t*>(pinned);
-    // device counters start at zero (the binning reads slots written by the previous P2G)
-    TESS_HIP(hipMemset(c->counters.get(64 * sizeof(uint32_t)), 0, 64 * sizeof(uint32_t)));
-    // the allocation-free partition's work area on this device
-    noalloc_scratch_resolve();
     *out = c;
   });
 }
@@ -241,24 +287,7 @@ tess_status tess_ctx_destroy(tess_ctx* ctx) {
   return guarded([&] {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    else (void)hipDeviceSynchronize();
-    tess::Buffer* bufs[] = {&ctx->keys,      &ctx->keys_sorted, &ctx->vals,         &ctx->keys_alt,
-                            &ctx->vals_alt,  &ctx->hist,        &ctx->scan_tmp,     &ctx->tile_start,
-                            &ctx->tile_end,  &ctx->tile_flag,   &ctx->tile_slot,    &ctx->active_tiles,
-                            &ctx->nb_flag,   &ctx->nb_slot,     &ctx->nb_list,      &ctx->nb_list_prev,
-                            &ctx->partials,  &ctx->counters,    &ctx->host_pinned_dummy, &ctx->reorder_tmp, &ctx->reorder_cnt,
-                            &ctx->per_tile,  &ctx->per_nodeblock, &ctx->skeys2, &ctx->perm2,
-                            &ctx->inc_nk,    &ctx->inc_spos,  &ctx->inc_mkey, &ctx->inc_mj,
-                            &ctx->inc_mks,   &ctx->inc_mr,    &ctx->inc_mjs, &ctx->inc_rng,
-                            &ctx->sp_occ,    &ctx->sp_act,    &ctx->sp_scan,
-                            &ctx->adj_partials, &ctx->adj_nodes, &ctx->ro_checkpoints, &ctx->ro_segment,
-                            &ctx->ro_gradv,  &ctx->ro_snapshots, &ctx->movers, &ctx->g2p2g_x,
-                            &ctx->export_buf};
-    for (auto* b : bufs) b->release();
-    for (auto ev : ctx->prof.pool) (void)hipEventDestroy(ev);
-    if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
-    delete ctx;
+    ctx_free(ctx);
   });
 }
 

@@ -68,7 +68,7 @@ __device__ __forceinline__ void node_jet2(const Args<T, D>& a, T x, int d, int n
                                           double& n2) {
   n0 = n1 = n2 = 0.0;
   if (node <= hi) {
-    const double hinv = 1.0 / a.mesh.h_d;
+    const double hinv = a.mesh.hinv_d;
     const double r = (double(x) - (a.mesh.xmin_d[d] + double(node) * a.mesh.h_d)) * hinv;
     bspline_jet2<BASIS>(r, n0, n1, n2);
     n1 *= hinv;
@@ -82,7 +82,7 @@ __device__ __forceinline__ void window_jets2(const Args<T, D>& a, const T* x, co
                                              double (*N)[support_width(BASIS)], double (*dN)[support_width(BASIS)],
                                              double (*ddN)[support_width(BASIS)]) {
   constexpr int SW = support_width(BASIS);
-  const double hinv = 1.0 / a.mesh.h_d;
+  const double hinv = a.mesh.hinv_d;
 #pragma unroll
   for (int d = 0; d < D; ++d)
 #pragma unroll
@@ -126,7 +126,8 @@ __global__ __launch_bounds__(64 * support_width(BASIS)) void k_adj_scatter(Args<
   __shared__ uint32_t s_cnt[64];
   for (int q = int(threadIdx.x); q < NA * G::NFOOT; q += NT) (&s_tile[0][0])[q] = 0.0;
   int ccount, coff, maxcnt;
-  tile_class_counts<SW>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt);
+  __shared__ uint32_t s_pid[PID_CAP];
+  tile_class_counts<SW>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt, a.perm, 0u, s_pid, nullptr, PID_CAP);
   int ccoord[D];
   {
     int r = lane;
@@ -146,7 +147,8 @@ __global__ __launch_bounds__(64 * support_width(BASIS)) void k_adj_scatter(Args<
       for (int v = 0; v < NA; ++v) acc[s][v] = 0.0;
     for (int k = 0; k < maxcnt; ++k) {
       if (k >= ccount) continue;
-      const uint32_t pid = a.perm[start + coff + k];
+      const int li = coff + k;
+      const uint32_t pid = li < PID_CAP ? s_pid[li] : a.perm[start + li];
       T x[D], l;
       int lo[D], hi[D];
       particle_window<T, D, BASIS, false>(a, pid, x, l, lo, hi, nullptr);
@@ -358,15 +360,25 @@ __global__ __launch_bounds__(ADJ_GATHER_THREADS) void k_adj_gather(Args<T, D> a,
   __syncthreads();
   const double dt = double(a.dt), mp = double(a.mp);
   constexpr int NR = Pow<SW, D - 1>::v;
+  // each thread's jets of the axes after x (value, first, second derivative per
+  // window node), evaluated once per particle and read by both passes' rows
+  __shared__ double s_rj[ADJ_GATHER_THREADS][D - 1][SW][3];
   for (int t = int(threadIdx.x); t < cnt; t += int(blockDim.x)) {
     const uint32_t p = a.perm[start + t];
     T x[D], l;
     int lo[D], hi[D];
     particle_window<T, D, BASIS, false>(a, p, x, l, lo, hi, nullptr);
-    // x-axis jets kept in registers; the other axes are evaluated per offset
+    // x-axis jets kept in registers, the other axes' in this thread's LDS row
     double N[1][SW], dN[1][SW], ddN[1][SW];
 #pragma unroll
     for (int k = 0; k < SW; ++k) node_jet2<T, D, BASIS>(a, x[0], 0, lo[0] + k, hi[0], N[0][k], dN[0][k], ddN[0][k]);
+#pragma unroll
+    for (int d = 1; d < D; ++d)
+#pragma unroll
+      for (int k = 0; k < SW; ++k) {
+        double* q = s_rj[threadIdx.x][d - 1][k];
+        node_jet2<T, D, BASIS>(a, x[d], d, lo[d] + k, hi[d], q[0], q[1], q[2]);
+      }
     int base[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) base[d] = lo[d] - origin[d];  // the class coordinate, <= BW-1
@@ -407,7 +419,12 @@ __global__ __launch_bounds__(ADJ_GATHER_THREADS) void k_adj_gather(Args<T, D> a,
         }
         double rn[D], rd[D], rdd[D];
 #pragma unroll
-        for (int d = 1; d < D; ++d) node_jet2<T, D, BASIS>(a, x[d], d, lo[d] + o[d], hi[d], rn[d], rd[d], rdd[d]);
+        for (int d = 1; d < D; ++d) {
+          const double* q = s_rj[threadIdx.x][d - 1][o[d]];
+          rn[d] = q[0];
+          rd[d] = q[1];
+          rdd[d] = q[2];
+        }
         // A/B x-sums of v and f̄, and C of u0 = (L̄ᵀ v - Pᵀ f̄)_0
         double Av[D], Bv[D], Af[D], Bf[D], Cu0 = 0.0;
 #pragma unroll
@@ -493,9 +510,13 @@ __global__ __launch_bounds__(ADJ_GATHER_THREADS) void k_adj_gather(Args<T, D> a,
           qrow += (base[d] + o[d]) * mul;
           mul *= G::TS;
         }
-        double rn[D], rd[D], rdd_unused;
+        double rn[D], rd[D];
 #pragma unroll
-        for (int d = 1; d < D; ++d) node_jet2<T, D, BASIS>(a, x[d], d, lo[d] + o[d], hi[d], rn[d], rd[d], rdd_unused);
+        for (int d = 1; d < D; ++d) {
+          const double* q = s_rj[threadIdx.x][d - 1][o[d]];
+          rn[d] = q[0];
+          rd[d] = q[1];
+        }
         // A/B x-sums of v, f̄ and s = v̄'·Δv + Δt x̄'·v + mₚ m̄ + mₚ q̄·v_p, A of q̄
         double Av[D], Bv[D], Af[D], Bf[D], Aq[D], As = 0.0, Bs = 0.0;
 #pragma unroll
@@ -875,6 +896,38 @@ void rollout_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, cons
   for (int d = 0; d < 3; ++d) ro.dims[d] = mesh.dims[d];
 }
 
+// The binning arrays and the adjoint's per-tile and per-node buffers of a
+// context at their largest sizes for n particles over ntiles tiles, so that the
+// reversed steps never grow (and free) a buffer the other stream may still use.
+template <typename T, int D>
+void reserve_reverse_buffers(tess_ctx* ctx, int64_t n, int64_t ntiles, int64_t nnodes) {
+  using Gm = Geo<D>;
+  const size_t sn = size_t(std::max<int64_t>(n, 1)), st = size_t(std::max<int64_t>(ntiles, 1));
+  ctx->vals.as<uint32_t>(sn);
+  ctx->keys_sorted.as<uint32_t>(sn);
+  tess::Buffer* per_tile[] = {&ctx->tile_start, &ctx->tile_end, &ctx->active_tiles, &ctx->tile_flag,
+                              &ctx->tile_slot,  &ctx->nb_list,  &ctx->nb_flag};
+  for (auto* b : per_tile) b->as<uint32_t>(st);
+  ctx->adj_partials.as<double>(st * 2 * D * Gm::NFOOT);
+  ctx->adj_nodes.as<double>(size_t(nnodes) * (2 * D + 1));
+}
+
+// The replay context of ctx: a second context on the same device with a
+// non-blocking stream of default priority.
+tess_ctx* replay_context(tess_ctx* ctx) {
+  if (!ctx->replay) {
+    hipStream_t rs = nullptr;
+    TESS_HIP(hipStreamCreateWithFlags(&rs, hipStreamNonBlocking));
+    try {
+      ctx->replay = ctx_new(ctx->device, rs);
+    } catch (...) {
+      (void)hipStreamDestroy(rs);
+      throw;
+    }
+  }
+  return ctx->replay;
+}
+
 template <typename T, int D, int BASIS>
 void rollout_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_params& prm,
                           const tess_adjoint& adj) {
@@ -883,54 +936,120 @@ void rollout_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid&
   const int64_t per = state_scalars<T, D>() * n;
   const int ncheck = (ro.nsteps + ro.every - 1) / ro.every;
   T* ck = static_cast<T*>(ctx->ro_checkpoints.ptr);
-  T* seg = ctx->ro_segment.as<T>(size_t(std::max<int64_t>(per * ro.every, 1)));
-  T* gradv = ctx->ro_gradv.as<T>(size_t(std::max<int64_t>(n * D * D, 1)));
-  int64_t nnodes = 1;
-  for (int d = 0; d < D; ++d) nnodes *= mesh.dims[d];
-  // The replay of step k bins state k and scatters its grid: both are kept for
-  // the reversed step k when the every-1 snapshots of a segment fit in 4 GiB;
-  // otherwise the reversed step bins and scatters again.
-  constexpr int64_t kSnapBytes = int64_t(4) << 30;
-  int64_t snap_words = -1;  // per snapshot; 0: not kept
-  uint32_t* snaps = nullptr;
-  ro.snap_bins.assign(size_t(std::max(ro.every - 1, 0)), Binning{});
-  for (int c = ncheck - 1; c >= 0; --c) {
-    const int s0 = c * ro.every;
-    const int len = std::min(ro.every, ro.nsteps - s0);
-    // replay the segment: slot k holds the state at the start of step s0 + k
-    TESS_HIP(hipMemcpyAsync(seg, ck + per * c, size_t(per) * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
-    for (int k = 1; k < len; ++k) {
-      TESS_HIP(hipMemcpyAsync(seg + per * k, seg + per * (k - 1), size_t(per) * sizeof(T), hipMemcpyDeviceToDevice,
-                              ctx->stream));
-      const tess_particles S = state_view<T, D>(seg + per * k, n, ro, gradv);
-      step_impl<T, D, BASIS, false>(ctx, mesh, G, S, prm, 1, 0.0, nullptr);
-      // the context now holds the binning of state k-1 and G its P2G grid
-      if (snap_words < 0) {
-        const int64_t w = snapshot_words<T, D>(n, ctx->bin.ntiles);
-        snap_words = w * 4 * int64_t(ro.every - 1) <= kSnapBytes ? w : 0;
-        if (snap_words > 0) snaps = ctx->ro_snapshots.as<uint32_t>(size_t(snap_words) * size_t(ro.every - 1));
-      }
-      if (snap_words > 0) {
-        snapshot_copy<T, D>(ctx, mesh, G, n, ctx->bin.ntiles, snaps + snap_words * (k - 1), true);
-        ro.snap_bins[size_t(k - 1)] = ctx->bin;
-      }
-    }
-    for (int k = len - 1; k >= 0; --k) {
-      const tess_particles S = state_view<T, D>(seg + per * k, n, ro, gradv);
-      if (snap_words > 0 && k < len - 1) {
-        Binning& b = ctx->bin;
-        b = ro.snap_bins[size_t(k)];
-        snapshot_copy<T, D>(ctx, mesh, G, n, b.ntiles, snaps + snap_words * k, false);
-        b.valid = true;
-        b.keys_ready = false;
-        b.x_ptr = S.x.ptr;
-        ctx->last_grid_m = nullptr;  // G holds a restored grid: the next P2G zeroes it whole
-        adjoint_kernels<T, D, BASIS>(ctx, mesh, G, S, prm, adj);
-      } else {
-        step_adjoint_impl<T, D, BASIS>(ctx, mesh, G, S, prm, adj);
-      }
-    }
+  int64_t nnodes = 1, ntiles = 1;
+  for (int d = 0; d < D; ++d) {
+    nnodes *= mesh.dims[d];
+    ntiles *= ((mesh.dims[d] - 1) >> Geo<D>::LT) + 1;
   }
+  // Each reversed step k restores the binning and P2G grid of state k, which
+  // the replay of its segment computed: the replay advances a copy of every
+  // state of the segment by one step (the last one included) and keeps the
+  // context's binning arrays and the grid on the step's node blocks.  The
+  // replay of segment c-1 runs on a second context and stream while this
+  // context reverses segment c: both are latency-bound at this size, and they
+  // use separate state, snapshot and scratch buffers (two sets, alternating by
+  // segment), ordered by events.  When two segments of snapshots exceed 8 GiB,
+  // each reversed step bins and scatters again instead, on this stream alone.
+  constexpr int64_t kSnapBytes = int64_t(8) << 30;
+  const int64_t snap_words = snapshot_words<T, D>(n, ntiles);
+  const int every = ro.every;
+  const bool kept = snap_words * 4 * int64_t(every) * 2 <= kSnapBytes;
+  if (!kept) {
+    T* seg = ctx->ro_segment.as<T>(size_t(std::max<int64_t>(per * every, 1)));
+    T* gradv = ctx->ro_gradv.as<T>(size_t(std::max<int64_t>(n * D * D, 1)));
+    for (int c = ncheck - 1; c >= 0; --c) {
+      const int s0 = c * every;
+      const int len = std::min(every, ro.nsteps - s0);
+      TESS_HIP(hipMemcpyAsync(seg, ck + per * c, size_t(per) * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+      for (int k = 1; k < len; ++k) {
+        TESS_HIP(hipMemcpyAsync(seg + per * k, seg + per * (k - 1), size_t(per) * sizeof(T),
+                                hipMemcpyDeviceToDevice, ctx->stream));
+        step_impl<T, D, BASIS, false>(ctx, mesh, G, state_view<T, D>(seg + per * k, n, ro, gradv), prm, 1, 0.0,
+                                      nullptr);
+      }
+      for (int k = len - 1; k >= 0; --k)
+        step_adjoint_impl<T, D, BASIS>(ctx, mesh, G, state_view<T, D>(seg + per * k, n, ro, gradv), prm, adj);
+    }
+    return;
+  }
+  tess_ctx* rc = replay_context(ctx);
+  // per set: every + 1 state slots (the last one a scratch copy stepped for the
+  // segment's last binning), every snapshots
+  const int64_t seg_set = per * (every + 1);
+  T* seg = ctx->ro_segment.as<T>(size_t(std::max<int64_t>(2 * seg_set, 1)));
+  T* gradv = ctx->ro_gradv.as<T>(size_t(std::max<int64_t>(2 * n * D * D, 1)));  // replay, reverse
+  uint32_t* snaps = ctx->ro_snapshots.as<uint32_t>(size_t(snap_words) * size_t(every) * 2);
+  // the replay's private grid, Julia layout: m, then mv, f, vⁿ, v with D components
+  T* gr = ctx->ro_grid.as<T>(size_t(nnodes) * (1 + 4 * D));
+  tess_grid Gr;
+  Gr.m = tess_field{gr, 1, 1};
+  Gr.mv = tess_field{gr + nnodes, D, 1};
+  Gr.f = tess_field{gr + nnodes * (1 + D), D, 1};
+  Gr.vn = tess_field{gr + nnodes * (1 + 2 * D), D, 1};
+  Gr.v = tess_field{gr + nnodes * (1 + 3 * D), D, 1};
+  rc->last_grid_m = nullptr;  // the replay's first P2G zeroes its grid whole
+  reserve_reverse_buffers<T, D>(ctx, n, ntiles, nnodes);
+  std::vector<Binning> bins[2];
+  bins[0].assign(size_t(every), Binning{});
+  bins[1].assign(size_t(every), Binning{});
+  struct Events {
+    hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};  // replay done (set 0, 1), reverse done (set 0, 1)
+    ~Events() {
+      for (hipEvent_t ev : e)
+        if (ev) (void)hipEventDestroy(ev);
+    }
+  } ev;
+  for (auto& e : ev.e) TESS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  bool reversed[2] = {false, false};
+  // the caller's work on ctx's stream (the rollout, the adjoint's inputs) comes before the first replay
+  TESS_HIP(hipEventRecord(ev.e[2], ctx->stream));
+  TESS_HIP(hipStreamWaitEvent(rc->stream, ev.e[2], 0));
+  auto replay = [&](int c, int set) {
+    if (reversed[set]) TESS_HIP(hipStreamWaitEvent(rc->stream, ev.e[2 + set], 0));  // its buffers are free again
+    const int len = std::min(every, ro.nsteps - c * every);
+    T* sg = seg + seg_set * set;
+    T* gv = gradv;
+    TESS_HIP(hipMemcpyAsync(sg, ck + per * c, size_t(per) * sizeof(T), hipMemcpyDeviceToDevice, rc->stream));
+    for (int k = 1; k <= len; ++k) {
+      // slot k: state k-1 advanced by one step; its binning and grid are those of state k-1
+      TESS_HIP(hipMemcpyAsync(sg + per * k, sg + per * (k - 1), size_t(per) * sizeof(T), hipMemcpyDeviceToDevice,
+                              rc->stream));
+      step_impl<T, D, BASIS, false>(rc, mesh, Gr, state_view<T, D>(sg + per * k, n, ro, gv), prm, 1, 0.0, nullptr);
+      snapshot_copy<T, D>(rc, mesh, Gr, n, rc->bin.ntiles, snaps + snap_words * (int64_t(every) * set + k - 1), true);
+      bins[set][size_t(k - 1)] = rc->bin;
+    }
+    TESS_HIP(hipEventRecord(ev.e[set], rc->stream));
+  };
+  auto reverse = [&](int c, int set) {
+    TESS_HIP(hipStreamWaitEvent(ctx->stream, ev.e[set], 0));
+    const int len = std::min(every, ro.nsteps - c * every);
+    T* sg = seg + seg_set * set;
+    for (int k = len - 1; k >= 0; --k) {
+      const tess_particles S = state_view<T, D>(sg + per * k, n, ro, gradv + n * D * D);
+      Binning& b = ctx->bin;
+      b = bins[set][size_t(k)];
+      snapshot_copy<T, D>(ctx, mesh, G, n, b.ntiles, snaps + snap_words * (int64_t(every) * set + k), false);
+      b.valid = true;
+      b.keys_ready = false;
+      b.state_valid = false;
+      b.x_ptr = S.x.ptr;
+      ctx->last_grid_m = nullptr;  // G holds a restored grid: the next P2G zeroes it whole
+      adjoint_kernels<T, D, BASIS>(ctx, mesh, G, S, prm, adj);
+    }
+    TESS_HIP(hipEventRecord(ev.e[2 + set], ctx->stream));
+    reversed[set] = true;
+  };
+  replay(ncheck - 1, (ncheck - 1) & 1);
+  for (int c = ncheck - 1; c >= 0; --c) {
+    reverse(c, c & 1);
+    if (c > 0) replay(c - 1, (c - 1) & 1);
+  }
+  // nothing of the replay context is left running when the call returns
+  TESS_HIP(hipStreamWaitEvent(ctx->stream, ev.e[0], 0));
+  TESS_HIP(hipStreamWaitEvent(ctx->stream, ev.e[1], 0));
+  TESS_HIP(hipStreamSynchronize(rc->stream));
+  ctx->bin.valid = false;
+  ctx->bin.state_valid = false;
 }
 
 template <typename T, int D, int B, bool S>

@@ -31,7 +31,7 @@ struct DMesh {
   T xmin[D];
   T h, h_inv;
   double xmin_d[D];
-  double h_d;
+  double h_d, hinv_d;  // h and 1/h in Float64 (1/h rounded once, on the host)
   __device__ T node(int d, int64_t i) const { return T(xmin_d[d] + double(i) * h_d); }
   __device__ int64_t lin(const int* I) const {
     int64_t l = I[D - 1];
@@ -51,6 +51,7 @@ inline DMesh<T, D> dmesh(const tess_mesh& m) {
   r.h = T(m.h);
   r.h_inv = T(m.h_inv);
   r.h_d = m.h;
+  r.hinv_d = 1.0 / m.h;
   return r;
 }
 

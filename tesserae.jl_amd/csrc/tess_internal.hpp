@@ -84,9 +84,6 @@ struct Rollout {
   int32_t dim = 0, dtype = 0, basis = 0;
   int64_t dims[3] = {0, 0, 0};
   tess_field logE{nullptr, 0, 0};
-  // binnings of the replayed steps of one segment (host part; the arrays and
-  // the P2G grid of each step live in tess_ctx::ro_snapshots)
-  std::vector<Binning> snap_bins;
 };
 
 }  // namespace tess
@@ -137,7 +134,17 @@ struct tess_ctx {
   // adjoint: per-tile Float64 partials, node adjoints, rollout checkpoints and replay
   tess::Buffer adj_partials, adj_nodes, ro_checkpoints, ro_segment, ro_gradv, ro_snapshots;
   tess::Rollout rollout;
+  // the pullback replays segments on a second context (own stream and
+  // scratch, private grid ro_grid) while this context reverses the previous one
+  tess_ctx* replay = nullptr;
+  tess::Buffer ro_grid;
 };
+
+namespace tess {
+// A context's state without the C-ABI checks (tess_ctx_create/destroy use them).
+tess_ctx* ctx_new(int device, hipStream_t stream);
+void ctx_free(tess_ctx* ctx);
+}  // namespace tess
 
 // A batch of scenes (batch.hip): the scenes as given, where each scene's tiles
 // and particles start in the batch-wide index spaces, and the device copies.
