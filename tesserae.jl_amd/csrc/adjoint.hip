@@ -62,6 +62,11 @@ __device__ __forceinline__ void bspline_jet2(double r, double& N, double& dN, do
   }
 }
 
+// Doubles per node of the node-adjoint array: f̄ (D), q̄ (D), m̄, then the
+// forward v (D) and v - vⁿ (D) that the gather stages.
+template <int D>
+constexpr int NODE_ADJ = 4 * D + 1;
+
 // Jets of node lo+k along axis d (zero past hi), derivatives in x.
 template <typename T, int D, int BASIS>
 __device__ __forceinline__ void node_jet2(const Args<T, D>& a, T x, int d, int node, int hi, double& n0, double& n1,
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(64 * support_width(BASIS)) void k_adj_scatter(Args<
 // grid update reversed.
 template <typename T, int D>
 __global__ __launch_bounds__(64) void k_adj_grid(Args<T, D> a, const double* __restrict__ partials,
-                                                 double* __restrict__ nodeadj) {
+                                                 double* __restrict__ nodeadj, const T* __restrict__ snap_grid) {
   using G = Geo<D>;
   constexpr int NA = 2 * D;
   constexpr int NB = (D == 3) ? 27 : 9;
@@ -291,8 +296,19 @@ __global__ __launch_bounds__(64) void k_adj_grid(Args<T, D> a, const double* __r
         for (int v = 0; v < NA; ++v) acc[v] += P[v];
       }
   const int64_t i = a.mesh.lin(I);
-  double* out = nodeadj + i * (2 * D + 1);
-  const double m = double(a.gm(i));
+  double* out = nodeadj + i * NODE_ADJ<D>;
+  // the step's forward grid at this node: the grid itself, or node r of the
+  // j-th listed block of a compact snapshot (m, mv, f, vⁿ, v)
+  const T* sg = snap_grid ? snap_grid + (j * int64_t(blockDim.x) + threadIdx.x) * (1 + 4 * D) : nullptr;
+  const double m = sg ? double(sg[0]) : double(a.gm(i));
+  // v and v - vⁿ for the gather's footprint (zero where m = 0, as the forward grid is)
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    const double vv = sg ? double(sg[1 + 3 * D + c]) : double(a.gv(i, c));
+    const double vn = sg ? double(sg[1 + 2 * D + c]) : double(a.gvn(i, c));
+    out[2 * D + 1 + c] = vv;
+    out[3 * D + 1 + c] = vv - vn;
+  }
   if (!(m > 0.0)) {
 #pragma unroll
     for (int v = 0; v < 2 * D + 1; ++v) out[v] = 0.0;
@@ -309,7 +325,9 @@ __global__ __launch_bounds__(64) void k_adj_grid(Args<T, D> a, const double* __r
     const double vnb = s - dvb;               // vⁿ enters v and Δv = v - vⁿ
     out[c] = s * dt * im;                     // f̄
     out[D + c] = vnb * im;                    // q̄ (of mv)
-    mb -= (s * dt * double(a.gf(i, c)) + vnb * double(a.gmv(i, c))) * im * im;
+    const double fc = sg ? double(sg[1 + D + c]) : double(a.gf(i, c));
+    const double mvc = sg ? double(sg[1 + c]) : double(a.gmv(i, c));
+    mb -= (s * dt * fc + vnb * mvc) * im * im;
   }
   out[2 * D] = mb;
 }
@@ -348,14 +366,14 @@ __global__ __launch_bounds__(ADJ_GATHER_THREADS) void k_adj_gather(Args<T, D> a,
       in = in && I[d] >= 0 && I[d] < a.mesh.dims[d];
     }
     const int64_t i = in ? a.mesh.lin(I) : 0;
+    const double* na = nodeadj + i * NODE_ADJ<D>;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      const double vv = in ? double(a.gv(i, c)) : 0.0;
-      s_n[q][c] = vv;
-      s_n[q][D + c] = in ? vv - double(a.gvn(i, c)) : 0.0;
+      s_n[q][c] = in ? na[2 * D + 1 + c] : 0.0;       // v
+      s_n[q][D + c] = in ? na[3 * D + 1 + c] : 0.0;   // v - vⁿ
     }
 #pragma unroll
-    for (int v = 0; v < 2 * D + 1; ++v) s_n[q][2 * D + v] = in ? nodeadj[i * (2 * D + 1) + v] : 0.0;
+    for (int v = 0; v < 2 * D + 1; ++v) s_n[q][2 * D + v] = in ? na[v] : 0.0;
   }
   __syncthreads();
   const double dt = double(a.dt), mp = double(a.mp);
@@ -633,22 +651,52 @@ inline DField<double> dfield_d(const tess_field& f) {
   return DField<double>{static_cast<double*>(f.ptr), f.elem_stride, f.comp_stride};
 }
 
+// Layout of a snapshot (binning words, then the grid of the listed node
+// blocks): perm and sorted keys (n words each), then tile_start, tile_end, the
+// active tiles, tile flags and slots, the node-block list and flags (ntiles
+// words each).
+inline int64_t snapshot_bin_words(int64_t n, int64_t ntiles) { return (2 * n + 7 * ntiles + 1) & ~int64_t(1); }
+inline void snapshot_views(const uint32_t* snap, int64_t n, int64_t ntiles, const uint32_t* w[9]) {
+  const int64_t lens[9] = {n, n, ntiles, ntiles, ntiles, ntiles, ntiles, ntiles, ntiles};
+  int64_t off = 0;
+  for (int i = 0; i < 9; ++i) {
+    w[i] = snap + off;
+    off += lens[i];
+  }
+}
+
 // The reversed kernels of one step, after the binning of S and its forward P2G
-// grid are in place.
+// grid are in place, or read from the step's snapshot.
 template <typename T, int D, int BASIS>
 void adjoint_kernels(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& S,
-                     const tess_params& prm, const tess_adjoint& adj) {
+                     const tess_params& prm, const tess_adjoint& adj, const uint32_t* snap = nullptr) {
   using Gm = Geo<D>;
   const Binning& b = ctx->bin;
   if (b.n_active == 0) return;  // every particle outside: the step is the identity on the state
   Args<T, D> a = make_args<T, D>(mesh, &G, S, nullptr, &prm);
   fill_binning_ptrs<T, D>(ctx, a);
+  const T* snap_grid = nullptr;
+  const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
+  if (snap) {
+    // the step's binning arrays and forward grid straight from its snapshot
+    const uint32_t* w[9];
+    snapshot_views(snap, b.n, b.ntiles, w);
+    a.perm = w[0];
+    sk = w[1];
+    a.tile_start = w[2];
+    a.tile_end = w[3];
+    a.active = w[4];
+    a.tile_flag = w[5];
+    a.tile_slot = w[6];
+    a.nb_list = w[7];
+    a.nb_flag = w[8];
+    snap_grid = reinterpret_cast<const T*>(snap + snapshot_bin_words(b.n, b.ntiles));
+  }
   int64_t nnodes = 1;
   for (int d = 0; d < D; ++d) nnodes *= mesh.dims[d];
   double* partials = ctx->adj_partials.as<double>(size_t(b.n_active) * 2 * D * Gm::NFOOT);
-  double* nodeadj = ctx->adj_nodes.as<double>(size_t(nnodes) * (2 * D + 1));
+  double* nodeadj = ctx->adj_nodes.as<double>(size_t(nnodes) * NODE_ADJ<D>);
   const DField<double> ax = dfield_d(adj.x), av = dfield_d(adj.v), aF = dfield_d(adj.F), ath = dfield_d(adj.logE);
-  const uint32_t* sk = static_cast<const uint32_t*>(ctx->keys_sorted.ptr);
   int tok = prof_begin(ctx, "k_adj_scatter", 0.0);
   hipLaunchKernelGGL((k_adj_scatter<T, D, BASIS>), dim3(xcd_grid(b.n_active)), dim3(64 * support_width(BASIS)), 0,
                      ctx->stream, a, sk, ax,
@@ -657,7 +705,7 @@ void adjoint_kernels(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, c
   prof_end(ctx, tok);
   tok = prof_begin(ctx, "k_adj_grid", 0.0);
   hipLaunchKernelGGL((k_adj_grid<T, D>), dim3(xcd_grid(b.n_nodeblocks)), dim3(64), 0, ctx->stream, a, partials,
-                     nodeadj);
+                     nodeadj, snap_grid);
   TESS_HIP(hipGetLastError());
   prof_end(ctx, tok);
   tok = prof_begin(ctx, "k_adj_gather", 0.0);
@@ -722,8 +770,6 @@ int snapshot_arrays(tess_ctx* ctx, int64_t n, int64_t ntiles, void* ptrs[9], int
   return 9;
 }
 
-// words of the binning part, rounded up to keep the grid part 8-byte aligned
-inline int64_t snapshot_bin_words(int64_t n, int64_t ntiles) { return (2 * n + 7 * ntiles + 1) & ~int64_t(1); }
 
 // node values of one snapshot's grid part: m, mv, f, vⁿ, v of up to every node
 // block (BW^D nodes each)
@@ -896,20 +942,14 @@ void rollout_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, cons
   for (int d = 0; d < 3; ++d) ro.dims[d] = mesh.dims[d];
 }
 
-// The binning arrays and the adjoint's per-tile and per-node buffers of a
-// context at their largest sizes for n particles over ntiles tiles, so that the
-// reversed steps never grow (and free) a buffer the other stream may still use.
+// The adjoint's per-tile and per-node buffers of a context at their largest
+// sizes for ntiles tiles, so that the reversed steps never grow (and free) a
+// buffer while the other stream runs.
 template <typename T, int D>
-void reserve_reverse_buffers(tess_ctx* ctx, int64_t n, int64_t ntiles, int64_t nnodes) {
+void reserve_reverse_buffers(tess_ctx* ctx, int64_t ntiles, int64_t nnodes) {
   using Gm = Geo<D>;
-  const size_t sn = size_t(std::max<int64_t>(n, 1)), st = size_t(std::max<int64_t>(ntiles, 1));
-  ctx->vals.as<uint32_t>(sn);
-  ctx->keys_sorted.as<uint32_t>(sn);
-  tess::Buffer* per_tile[] = {&ctx->tile_start, &ctx->tile_end, &ctx->active_tiles, &ctx->tile_flag,
-                              &ctx->tile_slot,  &ctx->nb_list,  &ctx->nb_flag};
-  for (auto* b : per_tile) b->as<uint32_t>(st);
-  ctx->adj_partials.as<double>(st * 2 * D * Gm::NFOOT);
-  ctx->adj_nodes.as<double>(size_t(nnodes) * (2 * D + 1));
+  ctx->adj_partials.as<double>(size_t(std::max<int64_t>(ntiles, 1)) * 2 * D * Gm::NFOOT);
+  ctx->adj_nodes.as<double>(size_t(nnodes) * NODE_ADJ<D>);
 }
 
 // The replay context of ctx: a second context on the same device with a
@@ -941,10 +981,10 @@ void rollout_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid&
     nnodes *= mesh.dims[d];
     ntiles *= ((mesh.dims[d] - 1) >> Geo<D>::LT) + 1;
   }
-  // Each reversed step k restores the binning and P2G grid of state k, which
-  // the replay of its segment computed: the replay advances a copy of every
-  // state of the segment by one step (the last one included) and keeps the
-  // context's binning arrays and the grid on the step's node blocks.  The
+  // Each reversed step k reads the binning and P2G grid of state k from the
+  // snapshot the replay of its segment kept: the replay advances a copy of
+  // every state of the segment by one step (the last one included) and copies
+  // its context's binning arrays and the grid on the step's node blocks.  The
   // replay of segment c-1 runs on a second context and stream while this
   // context reverses segment c: both are latency-bound at this size, and they
   // use separate state, snapshot and scratch buffers (two sets, alternating by
@@ -988,7 +1028,7 @@ void rollout_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid&
   Gr.vn = tess_field{gr + nnodes * (1 + 2 * D), D, 1};
   Gr.v = tess_field{gr + nnodes * (1 + 3 * D), D, 1};
   rc->last_grid_m = nullptr;  // the replay's first P2G zeroes its grid whole
-  reserve_reverse_buffers<T, D>(ctx, n, ntiles, nnodes);
+  reserve_reverse_buffers<T, D>(ctx, ntiles, nnodes);
   std::vector<Binning> bins[2];
   bins[0].assign(size_t(every), Binning{});
   bins[1].assign(size_t(every), Binning{});
@@ -1004,45 +1044,65 @@ void rollout_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid&
   // the caller's work on ctx's stream (the rollout, the adjoint's inputs) comes before the first replay
   TESS_HIP(hipEventRecord(ev.e[2], ctx->stream));
   TESS_HIP(hipStreamWaitEvent(rc->stream, ev.e[2], 0));
-  auto replay = [&](int c, int set) {
-    if (reversed[set]) TESS_HIP(hipStreamWaitEvent(rc->stream, ev.e[2 + set], 0));  // its buffers are free again
-    const int len = std::min(every, ro.nsteps - c * every);
-    T* sg = seg + seg_set * set;
-    T* gv = gradv;
-    TESS_HIP(hipMemcpyAsync(sg, ck + per * c, size_t(per) * sizeof(T), hipMemcpyDeviceToDevice, rc->stream));
-    for (int k = 1; k <= len; ++k) {
-      // slot k: state k-1 advanced by one step; its binning and grid are those of state k-1
-      TESS_HIP(hipMemcpyAsync(sg + per * k, sg + per * (k - 1), size_t(per) * sizeof(T), hipMemcpyDeviceToDevice,
+  // one reversed step k of the segment in buffer set `set` (enqueued only)
+  auto reverse_step = [&](int set, int k) {
+    const tess_particles S = state_view<T, D>(seg + seg_set * set + per * k, n, ro, gradv + n * D * D);
+    Binning& b = ctx->bin;
+    b = bins[set][size_t(k)];
+    b.valid = false;  // the context's own arrays do not hold this binning
+    b.state_valid = false;
+    adjoint_kernels<T, D, BASIS>(ctx, mesh, G, S, prm, adj, snaps + snap_words * (int64_t(every) * set + k));
+  };
+  auto seg_len = [&](int c) { return std::min(every, ro.nsteps - c * every); };
+  // Segment c is replayed as one step loop (the binning carried from step to
+  // step as in the forward) on a working copy of its checkpoint, the last slot
+  // of its buffer set: before each step the working state is copied to the
+  // step's slot, after its P2G the binning and grid are copied aside.  While
+  // segment c-1 is replayed, segment c is reversed: each replay step enqueues
+  // one reversed step on this context's stream before its G2P, so the two
+  // streams stay in step without either queueing far ahead.
+  // the segment being reversed during a replay: its buffer set, next step to
+  // enqueue, length; and the replay's length
+  int rev_set = -1, rev_k = -1, rev_len = 0, rep_len = 1;
+  std::function<void(int, int)> hook;
+  int cur_set = 0;
+  hook = [&](int phase, int s) {
+    T* sg = seg + seg_set * cur_set;
+    if (phase == 0) {
+      TESS_HIP(hipMemcpyAsync(sg + per * s, sg + per * every, size_t(per) * sizeof(T), hipMemcpyDeviceToDevice,
                               rc->stream));
-      step_impl<T, D, BASIS, false>(rc, mesh, Gr, state_view<T, D>(sg + per * k, n, ro, gv), prm, 1, 0.0, nullptr);
-      snapshot_copy<T, D>(rc, mesh, Gr, n, rc->bin.ntiles, snaps + snap_words * (int64_t(every) * set + k - 1), true);
-      bins[set][size_t(k - 1)] = rc->bin;
+    } else {
+      snapshot_copy<T, D>(rc, mesh, Gr, n, rc->bin.ntiles, snaps + snap_words * (int64_t(every) * cur_set + s), true);
+      bins[cur_set][size_t(s)] = rc->bin;
+      // keep the reversal two steps ahead of its share of the replay
+      const int target = std::min(rev_len, (s + 1) * rev_len / rep_len + 2);
+      while (rev_k >= 0 && rev_len - 1 - rev_k < target) reverse_step(rev_set, rev_k--);
     }
+  };
+  auto replay = [&](int c) {
+    const int set = c & 1;
+    if (reversed[set]) TESS_HIP(hipStreamWaitEvent(rc->stream, ev.e[2 + set], 0));  // its buffers are free again
+    cur_set = set;
+    T* work = seg + seg_set * set + per * every;
+    TESS_HIP(hipMemcpyAsync(work, ck + per * c, size_t(per) * sizeof(T), hipMemcpyDeviceToDevice, rc->stream));
+    step_impl<T, D, BASIS, false>(rc, mesh, Gr, state_view<T, D>(work, n, ro, gradv), prm, seg_len(c), 0.0, nullptr,
+                                  nullptr, &hook);
     TESS_HIP(hipEventRecord(ev.e[set], rc->stream));
   };
-  auto reverse = [&](int c, int set) {
+  replay(ncheck - 1);
+  for (int c = ncheck - 1; c >= 0; --c) {
+    const int set = c & 1;
     TESS_HIP(hipStreamWaitEvent(ctx->stream, ev.e[set], 0));
-    const int len = std::min(every, ro.nsteps - c * every);
-    T* sg = seg + seg_set * set;
-    for (int k = len - 1; k >= 0; --k) {
-      const tess_particles S = state_view<T, D>(sg + per * k, n, ro, gradv + n * D * D);
-      Binning& b = ctx->bin;
-      b = bins[set][size_t(k)];
-      snapshot_copy<T, D>(ctx, mesh, G, n, b.ntiles, snaps + snap_words * (int64_t(every) * set + k), false);
-      b.valid = true;
-      b.keys_ready = false;
-      b.state_valid = false;
-      b.x_ptr = S.x.ptr;
-      ctx->last_grid_m = nullptr;  // G holds a restored grid: the next P2G zeroes it whole
-      adjoint_kernels<T, D, BASIS>(ctx, mesh, G, S, prm, adj);
+    rev_set = set;
+    rev_len = seg_len(c);
+    rev_k = rev_len - 1;
+    if (c > 0) {
+      rep_len = seg_len(c - 1);
+      replay(c - 1);
     }
+    while (rev_k >= 0) reverse_step(set, rev_k--);
     TESS_HIP(hipEventRecord(ev.e[2 + set], ctx->stream));
     reversed[set] = true;
-  };
-  replay(ncheck - 1, (ncheck - 1) & 1);
-  for (int c = ncheck - 1; c >= 0; --c) {
-    reverse(c, c & 1);
-    if (c > 0) replay(c - 1, (c - 1) & 1);
   }
   // nothing of the replay context is left running when the call returns
   TESS_HIP(hipStreamWaitEvent(ctx->stream, ev.e[0], 0));

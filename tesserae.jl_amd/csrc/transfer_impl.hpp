@@ -25,6 +25,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
+
 #include <algorithm>
 #include <cstring>
 
@@ -2655,12 +2657,17 @@ void g2p2g_steps_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, 
 // step that finds its storage too small raises before any of its own work, so
 // after the error *done steps have run and the caller resizes and runs the
 // remaining nsteps - *done.
+//
+// hook (may be NULL) is called as (*hook)(0, s) before step s touches the
+// particles and (*hook)(1, s) once its binning and P2G grid are in place,
+// before its G2P (the adjoint's replays copy states and snapshots there).
 template <typename T, int D, int BASIS, bool STORED>
 void step_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
                const tess_params& prm, int nsteps, double reorder_threshold, tess_sparsity* sp,
-               int* done = nullptr) {
+               int* done = nullptr, const std::function<void(int, int)>* hook = nullptr) {
   for (int s = 0; s < nsteps; ++s) {
     if (done) *done = s;
+    if (hook) (*hook)(0, s);
     if (sp) {
       // update_sparsity!(grid, xₚ) at the top of the SpGrid loop (src/Grids/grid.jl:97-101)
       run_sparsity_update(ctx, mesh, P.x, P.n, *sp, nullptr);
@@ -2684,6 +2691,7 @@ void step_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const t
       }
     }
     p2g_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, reorder_threshold > 0.0, sp);
+    if (hook) (*hook)(1, s);
     g2p_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, sp, s + 1 < nsteps);
   }
   if (done) *done = nsteps;
