@@ -21,6 +21,11 @@
 // reversing it.
 #include "transfer_impl.hpp"
 
+// Reversed steps a segment replay keeps enqueued ahead of its own progress.
+#ifndef TESS_ADJ_REVERSE_LEAD
+#define TESS_ADJ_REVERSE_LEAD 2
+#endif
+
 namespace tess {
 namespace {
 
@@ -377,26 +382,21 @@ __global__ __launch_bounds__(ADJ_GATHER_THREADS) void k_adj_gather(Args<T, D> a,
   }
   __syncthreads();
   const double dt = double(a.dt), mp = double(a.mp);
-  constexpr int NR = Pow<SW, D - 1>::v;
-  // each thread's jets of the axes after x (value, first, second derivative per
-  // window node), evaluated once per particle and read by both passes' rows
-  __shared__ double s_rj[ADJ_GATHER_THREADS][D - 1][SW][3];
+  // rows of the window: offsets along y (unrolled, jets in registers) inside
+  // offsets along z (3D; that axis' jet evaluated per row plane and pass)
+  constexpr int NZ = (D == 3) ? SW : 1;
   for (int t = int(threadIdx.x); t < cnt; t += int(blockDim.x)) {
     const uint32_t p = a.perm[start + t];
     T x[D], l;
     int lo[D], hi[D];
     particle_window<T, D, BASIS, false>(a, p, x, l, lo, hi, nullptr);
-    // x-axis jets kept in registers, the other axes' in this thread's LDS row
-    double N[1][SW], dN[1][SW], ddN[1][SW];
+    // x- and y-axis jets (value, first, second derivative per window node)
+    double N[1][SW], dN[1][SW], ddN[1][SW], J1[SW][3];
 #pragma unroll
-    for (int k = 0; k < SW; ++k) node_jet2<T, D, BASIS>(a, x[0], 0, lo[0] + k, hi[0], N[0][k], dN[0][k], ddN[0][k]);
-#pragma unroll
-    for (int d = 1; d < D; ++d)
-#pragma unroll
-      for (int k = 0; k < SW; ++k) {
-        double* q = s_rj[threadIdx.x][d - 1][k];
-        node_jet2<T, D, BASIS>(a, x[d], d, lo[d] + k, hi[d], q[0], q[1], q[2]);
-      }
+    for (int k = 0; k < SW; ++k) {
+      node_jet2<T, D, BASIS>(a, x[0], 0, lo[0] + k, hi[0], N[0][k], dN[0][k], ddN[0][k]);
+      node_jet2<T, D, BASIS>(a, x[1], 1, lo[1] + k, hi[1], J1[k][0], J1[k][1], J1[k][2]);
+    }
     int base[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) base[d] = lo[d] - origin[d];  // the class coordinate, <= BW-1
@@ -424,24 +424,24 @@ __global__ __launch_bounds__(ADJ_GATHER_THREADS) void k_adj_gather(Args<T, D> a,
           Pm[r2 + D * c] = Vp * double(a.sigma(p, r2 + D * c));
         }
 #pragma unroll 1
-      for (int rs = 0; rs < NR; ++rs) {
-        int o[D];
-        o[0] = 0;
-        int r = rs, qrow = 0, mul = G::TS;
-#pragma unroll
-        for (int d = 1; d < D; ++d) {
-          o[d] = r % SW;
-          r /= SW;
-          qrow += (base[d] + o[d]) * mul;
-          mul *= G::TS;
+      for (int oz = 0; oz < NZ; ++oz) {
+        double J2[3] = {1.0, 0.0, 0.0};
+        int qz = 0;
+        if constexpr (D == 3) {
+          node_jet2<T, D, BASIS>(a, x[D - 1], D - 1, lo[D - 1] + oz, hi[D - 1], J2[0], J2[1], J2[2]);
+          qz = (base[D - 1] + oz) * G::TS * G::TS;
         }
-        double rn[D], rd[D], rdd[D];
 #pragma unroll
-        for (int d = 1; d < D; ++d) {
-          const double* q = s_rj[threadIdx.x][d - 1][o[d]];
-          rn[d] = q[0];
-          rd[d] = q[1];
-          rdd[d] = q[2];
+      for (int oy = 0; oy < SW; ++oy) {
+        const int qrow = qz + (base[1] + oy) * G::TS;
+        double rn[D], rd[D], rdd[D];
+        rn[1] = J1[oy][0];
+        rd[1] = J1[oy][1];
+        rdd[1] = J1[oy][2];
+        if constexpr (D == 3) {
+          rn[D - 1] = J2[0];
+          rd[D - 1] = J2[1];
+          rdd[D - 1] = J2[2];
         }
         // A/B x-sums of v and f̄, and C of u0 = (L̄ᵀ v - Pᵀ f̄)_0
         double Av[D], Bv[D], Af[D], Bf[D], Cu0 = 0.0;
@@ -502,6 +502,7 @@ __global__ __launch_bounds__(ADJ_GATHER_THREADS) void k_adj_gather(Args<T, D> a,
           gx[c] += tt;
         }
       }
+      }
     }
     double gq[D], Pb[D * D], L[D * D];
 #pragma unroll
@@ -517,23 +518,22 @@ __global__ __launch_bounds__(ADJ_GATHER_THREADS) void k_adj_gather(Args<T, D> a,
         mpvp[c] = mp * double(a.v(p, c));
       }
 #pragma unroll 1
-      for (int rs = 0; rs < NR; ++rs) {
-        int o[D];
-        o[0] = 0;
-        int r = rs, qrow = 0, mul = G::TS;
-#pragma unroll
-        for (int d = 1; d < D; ++d) {
-          o[d] = r % SW;
-          r /= SW;
-          qrow += (base[d] + o[d]) * mul;
-          mul *= G::TS;
+      for (int oz = 0; oz < NZ; ++oz) {
+        double J2[2] = {1.0, 0.0}, unused;
+        int qz = 0;
+        if constexpr (D == 3) {
+          node_jet2<T, D, BASIS>(a, x[D - 1], D - 1, lo[D - 1] + oz, hi[D - 1], J2[0], J2[1], unused);
+          qz = (base[D - 1] + oz) * G::TS * G::TS;
         }
-        double rn[D], rd[D];
 #pragma unroll
-        for (int d = 1; d < D; ++d) {
-          const double* q = s_rj[threadIdx.x][d - 1][o[d]];
-          rn[d] = q[0];
-          rd[d] = q[1];
+      for (int oy = 0; oy < SW; ++oy) {
+        const int qrow = qz + (base[1] + oy) * G::TS;
+        double rn[D], rd[D];
+        rn[1] = J1[oy][0];
+        rd[1] = J1[oy][1];
+        if constexpr (D == 3) {
+          rn[D - 1] = J2[0];
+          rd[D - 1] = J2[1];
         }
         // A/B x-sums of v, f̄ and s = v̄'·Δv + Δt x̄'·v + mₚ m̄ + mₚ q̄·v_p, A of q̄
         double Av[D], Bv[D], Af[D], Bf[D], Aq[D], As = 0.0, Bs = 0.0;
@@ -580,6 +580,7 @@ __global__ __launch_bounds__(ADJ_GATHER_THREADS) void k_adj_gather(Args<T, D> a,
             Pb[r2 + D * c] -= Rc * Af[r2];
           }
         }
+      }
       }
     }
     // particle side: x̄, v̄, F̄ = (I+ΔtL)ᵀ F̄' + V0 (μ (P̄+P̄ᵀ) F + λ tr(P̄) F⁻ᵀ), ∂/∂logE += P̄ : P
@@ -743,10 +744,19 @@ struct CopyTable {
   CopyTask t[16];
 };
 
+// Task blockIdx.y of the table; 16-byte words where both ends are 16-byte aligned.
 __global__ __launch_bounds__(256) void k_copy_multi(CopyTable tab) {
   const CopyTask c = tab.t[blockIdx.y];
-  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < c.words; i += int64_t(gridDim.x) * 256)
-    c.dst[i] = c.src[i];
+  const int64_t i0 = int64_t(blockIdx.x) * 256 + threadIdx.x, stride = int64_t(gridDim.x) * 256;
+  int64_t done = 0;
+  if (((reinterpret_cast<uintptr_t>(c.src) | reinterpret_cast<uintptr_t>(c.dst)) & 15) == 0) {
+    const int64_t n4 = c.words / 4;
+    const uint4* s4 = reinterpret_cast<const uint4*>(c.src);
+    uint4* d4 = reinterpret_cast<uint4*>(c.dst);
+    for (int64_t i = i0; i < n4; i += stride) d4[i] = s4[i];
+    done = n4 * 4;
+  }
+  for (int64_t i = done + i0; i < c.words; i += stride) c.dst[i] = c.src[i];
 }
 
 // words of a field of nc components over n elements, whatever its strides
@@ -824,10 +834,12 @@ __global__ __launch_bounds__(64) void k_snap_grid(Args<T, D> a, T* __restrict__ 
 }
 
 // save (to_store) or restore the snapshot at store; ctx->bin describes the
-// step whose binning is saved or restored
+// step whose binning is saved or restored.  A further copy of xwords 32-bit
+// words from xsrc to xdst rides along in the same launch.
 template <typename T, int D>
 void snapshot_copy(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, int64_t n, int64_t ntiles,
-                   uint32_t* store, bool to_store) {
+                   uint32_t* store, bool to_store, const void* xsrc = nullptr, void* xdst = nullptr,
+                   int64_t xwords = 0) {
   void* p[9];
   int64_t w[9];
   const int k = snapshot_arrays(ctx, n, ntiles, p, w);
@@ -842,9 +854,17 @@ void snapshot_copy(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, int
     off += w[i];
     most = std::max(most, w[i]);
   }
+  int ntask = k;
+  if (xwords > 0) {
+    tab.t[ntask].src = static_cast<const uint32_t*>(xsrc);
+    tab.t[ntask].dst = static_cast<uint32_t*>(xdst);
+    tab.t[ntask].words = xwords;
+    ++ntask;
+    most = std::max(most, xwords / 4);
+  }
   if (most > 0) {
     const unsigned gx = unsigned(std::min<int64_t>(blocks_for(most, 256), 2048));
-    hipLaunchKernelGGL(k_copy_multi, dim3(gx, unsigned(k)), dim3(256), 0, ctx->stream, tab);
+    hipLaunchKernelGGL(k_copy_multi, dim3(gx, unsigned(ntask)), dim3(256), 0, ctx->stream, tab);
     TESS_HIP(hipGetLastError());
   }
   const Binning& b = ctx->bin;
@@ -1056,26 +1076,28 @@ void rollout_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid&
   auto seg_len = [&](int c) { return std::min(every, ro.nsteps - c * every); };
   // Segment c is replayed as one step loop (the binning carried from step to
   // step as in the forward) on a working copy of its checkpoint, the last slot
-  // of its buffer set: before each step the working state is copied to the
-  // step's slot, after its P2G the binning and grid are copied aside.  While
-  // segment c-1 is replayed, segment c is reversed: each replay step enqueues
-  // one reversed step on this context's stream before its G2P, so the two
-  // streams stay in step without either queueing far ahead.
+  // of its buffer set: after each step's P2G the working state is copied to
+  // the step's slot and the binning and grid aside.  While segment c-1 is
+  // replayed, segment c is reversed: each replay step enqueues reversed steps
+  // on this context's stream before its G2P, so the two streams stay in step
+  // without either queueing far ahead.
   // the segment being reversed during a replay: its buffer set, next step to
   // enqueue, length; and the replay's length
   int rev_set = -1, rev_k = -1, rev_len = 0, rep_len = 1;
   std::function<void(int, int)> hook;
   int cur_set = 0;
   hook = [&](int phase, int s) {
+    if (phase == 0) return;
+    // after the step's P2G (which leaves the particles as they were at the
+    // step's start): the working state into the step's slot, with the binning
+    // and grid, in one launch
     T* sg = seg + seg_set * cur_set;
-    if (phase == 0) {
-      TESS_HIP(hipMemcpyAsync(sg + per * s, sg + per * every, size_t(per) * sizeof(T), hipMemcpyDeviceToDevice,
-                              rc->stream));
-    } else {
-      snapshot_copy<T, D>(rc, mesh, Gr, n, rc->bin.ntiles, snaps + snap_words * (int64_t(every) * cur_set + s), true);
-      bins[cur_set][size_t(s)] = rc->bin;
-      // keep the reversal two steps ahead of its share of the replay
-      const int target = std::min(rev_len, (s + 1) * rev_len / rep_len + 2);
+    snapshot_copy<T, D>(rc, mesh, Gr, n, rc->bin.ntiles, snaps + snap_words * (int64_t(every) * cur_set + s), true,
+                        sg + per * every, sg + per * s, per * int64_t(sizeof(T)) / 4);
+    bins[cur_set][size_t(s)] = rc->bin;
+    {
+      // keep the reversal TESS_ADJ_REVERSE_LEAD steps ahead of its share of the replay
+      const int target = std::min(rev_len, (s + 1) * rev_len / rep_len + TESS_ADJ_REVERSE_LEAD);
       while (rev_k >= 0 && rev_len - 1 - rev_k < target) reverse_step(rev_set, rev_k--);
     }
   };
