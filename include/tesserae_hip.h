@@ -380,7 +380,7 @@ tess_status tess_batch_step(tess_ctx* ctx, tess_batch* batch, int32_t nsteps);
  * 0 (the default) never reorders.  Reordering permutes the caller's particle
  * arrays, exactly like reorder_particles!. */
 tess_status tess_batch_set_reorder(tess_batch* batch, double threshold);
-/* Times each launch variant (waves per tile in P2G and G2P) over
+/* Times each launch variant (waves per tile in P2G and G2P, eager or graph mode) over
  * steps_per_variant steps of the batch after one untimed step, keeps the
  * fastest and writes its number to *variant.  The tuning steps are steps of
  * the simulation: the scenes advance by (variants * (steps_per_variant + 1))
@@ -391,14 +391,19 @@ tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_
 /* Sets the launch variant, e.g. a number tess_batch_autotune returned and the
  * caller cached for this workload shape: in 2D bit 0 splits P2G into one wave
  * per last-axis offset, and G2P runs 4 waves per tile, 2 with bit 1, 1 with
- * bit 2 (0..5); in 3D 0 or 2.  Other numbers are TESS_ERR_ARGUMENT.  No
- * counterpart in the reference, which has no launch variants. */
+ * bit 2 (0..5); in 3D 0 or 2.  Bit 3 (+8) selects graph mode: the batch steps
+ * on a private stream and scratch, each step one replay of a captured hipGraph
+ * whose binning sorts every particle and whose tile and node-block kernels are
+ * persistent launches reading their counts on the device (no host readback
+ * inside a step; the reorder test and one error-flag readback run per call and
+ * per 16 replays).  Other numbers are TESS_ERR_ARGUMENT.  No counterpart in the
+ * reference, which has no launch variants. */
 tess_status tess_batch_set_variant(tess_batch* batch, int32_t variant);
 /* Counters of the batch, host values, the first n of: [0] reorders done, [1] of
  * those, reorders decided from the broken-pair count of the previous step's
  * P2G (not from a layered-order pass on a call's first step), [2] binnings that
- * kept the previous tile layout, [3] the launch variant.  No counterpart in the
- * reference (measurement). */
+ * kept the previous tile layout, [3] the launch variant, [4] graph replays,
+ * [5] graph captures.  No counterpart in the reference (measurement). */
 tess_status tess_batch_stats(const tess_batch* batch, int64_t* out, int32_t n);
 tess_status tess_batch_destroy(tess_batch* batch);
 

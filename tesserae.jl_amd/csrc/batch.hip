@@ -336,6 +336,11 @@ template <typename T, int D, int BASIS>
 void batch_step_impl(tess_ctx* ctx, tess_batch& B, int nsteps) {
   constexpr int SW = support_width(BASIS);
   B.counted = false;  // the context's counters may have served other calls since
+  if (B.last_graph) {
+    // graph-mode steps wrote node blocks the eager stale list does not name
+    B.zeroed = false;
+    B.last_graph = false;
+  }
   for (int s = 0; s < nsteps; ++s) {
     // from the second step of the call on, the previous G2P wrote the keys
     batch_bin<T, D, BASIS>(ctx, B, s > 0);
@@ -421,10 +426,321 @@ void batch_step_impl(tess_ctx* ctx, tess_batch& B, int nsteps) {
   ctx->bin.valid = false;
 }
 
+// ---- graph mode ----------------------------------------------------------------
+//
+// With bit 3 of the variant a batch steps on a private context (its own stream
+// and scratch, so no other call can move the buffers a captured step points
+// at).  Its step reads nothing back to the host: the binning sorts every
+// particle again (every launch size fixed by the batch), the counts of inside
+// particles, active tiles and touched node blocks stay on the device, and the
+// tile and node-block kernels are persistent launches (a grid sized to the
+// device by its occupancy) that walk those lists up to the device-side counts.
+// Zeroing is done by kernels.  The step is captured once into a hipGraph and
+// replayed; the reorder test, which needs the host, runs between chunks of
+// graph_chunk replays.
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) p[i] = v;
+}
+
+void fill_u32(hipStream_t st, uint32_t* p, int64_t n, uint32_t v) {
+  if (n <= 0) return;
+  const unsigned g = unsigned(std::min<int64_t>(blocks_for(n, 256), 4096));
+  hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, st, p, n, v);
+  TESS_HIP(hipGetLastError());
+}
+
+// The nodes of the node blocks listed by the previous step and not flagged by
+// this one, back to zero (the count read on the device).
+template <typename T, int D>
+__global__ __launch_bounds__(64) void k_zero_stale_persist(Args<T, D> a, const uint32_t* __restrict__ prev_list,
+                                                           const uint32_t* __restrict__ prev_count) {
+  using G = Geo<D>;
+  const int64_t np = int64_t(*prev_count);
+  for (int64_t j = blockIdx.x; j < np; j += gridDim.x) {
+    const uint32_t B = prev_list[j];
+    if (a.nb_flag[B]) continue;
+    uint32_t poff;
+    const uint32_t Bl = bind_tile<true>(a, B, poff);
+    int Bi[D], I[D];
+    unlin<D>(Bl, a.ntiles, Bi);
+    int r = int(threadIdx.x);
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      I[d] = Bi[d] * G::BW + r % G::BW;
+      r /= G::BW;
+      in = in && I[d] < a.mesh.dims[d];
+    }
+    if (in) {
+      const int64_t i = grid_slot(a, I);
+      if (i >= 0) zero_node(a, i);
+    }
+  }
+}
+
+// This step's node-block list and count become the next step's stale list.
+__global__ __launch_bounds__(256) void k_save_list(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                                                   uint32_t* __restrict__ prev_list, uint32_t* __restrict__ prev_count) {
+  const int64_t n = int64_t(*count);
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+    prev_list[i] = list[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *prev_count = uint32_t(n);
+}
+
+// Particles outside their mesh (sorted positions from the device count of
+// inside particles to n): the update with zero grid increments.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_g2p_outside_persist(Args<T, D> a, const uint32_t* __restrict__ n_inside) {
+  const int64_t from = int64_t(*n_inside);
+  for (int64_t j = from + int64_t(blockIdx.x) * 256 + threadIdx.x; j < a.n; j += int64_t(gridDim.x) * 256) {
+    uint32_t pid = a.perm[j];
+    pid -= uint32_t(bind_scene(a, scene_of(a.scene_poff, a.nscenes, pid)).poff);
+    T dv[D], L[D * D], dx[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      dv[c] = T(0);
+      dx[c] = T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) L[k] = T(0);
+    particle_update<T, D>(a, pid, dv, L, dx);
+  }
+}
+
+int device_cus(int dev) {
+  int c = 0;
+  TESS_HIP(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
+  return std::max(c, 1);
+}
+
+// Workgroups of a persistent launch: as many as the device holds at once (its
+// occupancy times the CUs), at most one per work item, a multiple of 8.
+unsigned persist_grid(const void* kern, int threads, int64_t max_items, int cus) {
+  int per = 0;
+  TESS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, 0));
+  int64_t g = int64_t(std::max(per, 1)) * cus;
+  g = std::min<int64_t>(g, int64_t(xcd_grid(std::max<int64_t>(max_items, 1))));
+  return unsigned(((g + 7) / 8) * 8);
+}
+
+template <typename T, int D, int BASIS, int NP>
+const void* p2g_persist_fn() {
+  return reinterpret_cast<const void*>(&k_p2g_fast_persist<T, D, BASIS, NP, true, true>);
+}
+template <typename T, int D, int BASIS, int NW>
+const void* g2p_persist_fn() {
+  return reinterpret_cast<const void*>(&k_g2p_fast_persist<T, D, BASIS, true, NW, true>);
+}
+
+template <typename T, int D, int BASIS>
+void batch_graph_grids(tess_ctx* g, tess_batch& B) {
+  constexpr int SW = support_width(BASIS);
+  const int cus = device_cus(g->device);
+  const int64_t nt = B.tiles_total;
+  const bool split = D == 3 || (B.variant & 1);
+  B.grid_p2g = split ? persist_grid(p2g_persist_fn<T, D, BASIS, SW>(), 64 * SW, nt, cus)
+                     : persist_grid(p2g_persist_fn<T, D, BASIS, 1>(), 64, nt, cus);
+  if (B.variant & 4) B.grid_g2p = persist_grid(g2p_persist_fn<T, D, BASIS, 1>(), 64, nt, cus);
+  else if (B.variant & 2) B.grid_g2p = persist_grid(g2p_persist_fn<T, D, BASIS, 2>(), 128, nt, cus);
+  else B.grid_g2p = persist_grid(g2p_persist_fn<T, D, BASIS, 4>(), 256, nt, cus);
+  B.grid_gather = persist_grid(reinterpret_cast<const void*>(&k_gather_grid_persist<T, D, SW, true, true>), 64, nt,
+                               cus);
+  B.grid_zero = persist_grid(reinterpret_cast<const void*>(&k_zero_stale_persist<T, D>), 64, nt, cus);
+  B.grid_out = persist_grid(reinterpret_cast<const void*>(&k_g2p_outside_persist<T, D>), 256,
+                            ceil_div(std::max<int64_t>(B.ntot, 1), int64_t(256)), cus);
+}
+
+// One step of the batch on context g, every launch size fixed and every count
+// on the device (the form that is captured).
+template <typename T, int D, int BASIS>
+void batch_graph_step(tess_ctx* g, tess_batch& B) {
+  constexpr int SW = support_width(BASIS);
+  const int64_t n = B.ntot;
+  const uint64_t ntiles = uint64_t(B.tiles_total);
+  hipStream_t st = g->stream;
+  uint32_t* counters = g->counters.as<uint32_t>(CNT_SLOTS);
+  uint32_t* keys = g->keys.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
+  uint32_t* skeys = g->keys_sorted.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
+  uint32_t* perm = g->vals.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
+  uint32_t* tstart = g->tile_start.as<uint32_t>(ntiles);
+  uint32_t* tend = g->tile_end.as<uint32_t>(ntiles);
+  uint32_t* tflag = g->tile_flag.as<uint32_t>(ntiles);
+  uint32_t* tslot = g->tile_slot.as<uint32_t>(ntiles);
+  uint32_t* active = g->active_tiles.as<uint32_t>(ntiles);
+  uint32_t* nbflag = g->nb_flag.as<uint32_t>(ntiles);
+  uint32_t* nbslot = g->nb_slot.as<uint32_t>(ntiles);
+  uint32_t* nblist = g->nb_list.as<uint32_t>(ntiles);
+  uint32_t* per_nb = g->per_nodeblock.as<uint32_t>(ntiles);
+  uint32_t* prev_list = B.g_prev_list.as<uint32_t>(ntiles);
+  uint32_t* prev_count = B.g_prev_count.as<uint32_t>(64);
+  T* partials = g->partials.as<T>(size_t(ntiles) * Part<D, SW>::SIZE);
+  Args<T, D> a = batch_args<T, D>(g, B);
+  a.partials = partials;
+  const int kbits = key_bits_for(ntiles * 64);
+  // binning: every particle sorted again
+  fill_u32(st, counters, CNT_BIN_SLOTS, 0u);
+  if (n > 0) {
+    hipLaunchKernelGGL((k_bin_keys<T, D, BASIS, false, true>), dim3(blocks_for(n, 256)), dim3(256), 0, st, a, keys,
+                       uint32_t(ntiles), counters);
+    TESS_HIP(hipGetLastError());
+    radix_sort_pairs(g, keys, nullptr, skeys, perm, n, kbits);
+  }
+  fill_u32(st, tstart, int64_t(ntiles), 0u);
+  fill_u32(st, tend, int64_t(ntiles), 0u);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_tile_runs, dim3(blocks_for(n, 256)), dim3(256), 0, st, skeys, n, uint32_t(ntiles), tstart,
+                       tend, counters);
+    TESS_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_tile_flags, dim3(blocks_for(ntiles, 256)), dim3(256), 0, st, tstart, tend, uint32_t(ntiles),
+                     tflag, tslot);
+  TESS_HIP(hipGetLastError());
+  exclusive_scan_u32(g, tslot, int64_t(ntiles), counters + CNT_ACTIVE);
+  hipLaunchKernelGGL(k_compact, dim3(blocks_for(ntiles, 256)), dim3(256), 0, st, tflag, tslot, uint32_t(ntiles),
+                     active);
+  TESS_HIP(hipGetLastError());
+  fill_u32(st, nbflag, int64_t(ntiles), 0u);
+  hipLaunchKernelGGL((k_nb_mark_batch<T, D>), dim3(blocks_for(ntiles, 256)), dim3(256), 0, st, a, tflag,
+                     uint32_t(ntiles), nbflag);
+  TESS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_copy_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, st, nbflag, nbslot, int64_t(ntiles));
+  TESS_HIP(hipGetLastError());
+  exclusive_scan_u32(g, nbslot, int64_t(ntiles), counters + CNT_NB);
+  hipLaunchKernelGGL(k_compact, dim3(blocks_for(ntiles, 256)), dim3(256), 0, st, nbflag, nbslot, uint32_t(ntiles),
+                     nblist);
+  TESS_HIP(hipGetLastError());
+  // grids: the previous step's stale node blocks, P2G, the grid-node half
+  hipLaunchKernelGGL((k_zero_stale_persist<T, D>), dim3(B.grid_zero), dim3(64), 0, st, a,
+                     static_cast<const uint32_t*>(prev_list), static_cast<const uint32_t*>(prev_count));
+  TESS_HIP(hipGetLastError());
+  const uint32_t* sk = skeys;
+  const uint32_t* n_active = counters + CNT_ACTIVE;
+  if (D == 3 || (B.variant & 1))
+    hipLaunchKernelGGL((k_p2g_fast_persist<T, D, BASIS, SW, true, true>), dim3(B.grid_p2g), dim3(64 * SW), 0, st, a,
+                       sk, static_cast<uint32_t*>(nullptr), n_active);
+  else
+    hipLaunchKernelGGL((k_p2g_fast_persist<T, D, BASIS, 1, true, true>), dim3(B.grid_p2g), dim3(64), 0, st, a, sk,
+                       static_cast<uint32_t*>(nullptr), n_active);
+  TESS_HIP(hipGetLastError());
+  hipLaunchKernelGGL((k_gather_grid_persist<T, D, SW, true, true>), dim3(B.grid_gather), dim3(64), 0, st, a, per_nb,
+                     static_cast<const uint32_t*>(counters + CNT_NB));
+  TESS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_save_list, dim3(unsigned(std::min<int64_t>(blocks_for(int64_t(ntiles), 256), 1024))), dim3(256),
+                     0, st, static_cast<const uint32_t*>(nblist), static_cast<const uint32_t*>(counters + CNT_NB),
+                     prev_list, prev_count);
+  TESS_HIP(hipGetLastError());
+  // G2P over the active tiles, then the particles outside every mesh
+  uint32_t* none = nullptr;
+  const uint32_t nt32 = uint32_t(ntiles);
+  if (B.variant & 4)
+    hipLaunchKernelGGL((k_g2p_fast_persist<T, D, BASIS, true, 1, true>), dim3(B.grid_g2p), dim3(64), 0, st, a, sk,
+                       none, none, nt32, none, n_active);
+  else if (B.variant & 2)
+    hipLaunchKernelGGL((k_g2p_fast_persist<T, D, BASIS, true, 2, true>), dim3(B.grid_g2p), dim3(128), 0, st, a, sk,
+                       none, none, nt32, none, n_active);
+  else
+    hipLaunchKernelGGL((k_g2p_fast_persist<T, D, BASIS, true, 4, true>), dim3(B.grid_g2p), dim3(256), 0, st, a, sk,
+                       none, none, nt32, none, n_active);
+  TESS_HIP(hipGetLastError());
+  if (n > 0) {
+    hipLaunchKernelGGL((k_g2p_outside_persist<T, D>), dim3(B.grid_out), dim3(256), 0, st, a,
+                       static_cast<const uint32_t*>(counters + CNT_INSIDE));
+    TESS_HIP(hipGetLastError());
+  }
+}
+
+void graph_release(tess_batch& B) {
+  if (B.graph_exec) (void)hipGraphExecDestroy(B.graph_exec);
+  if (B.graph) (void)hipGraphDestroy(B.graph);
+  B.graph_exec = nullptr;
+  B.graph = nullptr;
+  B.captured_variant = -1;
+}
+
+template <typename T, int D, int BASIS>
+void batch_graph_run(tess_ctx* ctx, tess_batch& B, int nsteps) {
+  if (!B.gctx) {
+    hipStream_t gs = nullptr;
+    TESS_HIP(hipStreamCreateWithFlags(&gs, hipStreamNonBlocking));
+    try {
+      B.gctx = ctx_new(ctx->device, gs);
+    } catch (...) {
+      (void)hipStreamDestroy(gs);
+      throw;
+    }
+    for (auto& e : B.graph_ev) TESS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  tess_ctx* g = B.gctx;
+  hipStream_t gs = g->stream;
+  // the caller's work on its stream comes first
+  TESS_HIP(hipEventRecord(B.graph_ev[0], ctx->stream));
+  TESS_HIP(hipStreamWaitEvent(gs, B.graph_ev[0], 0));
+  uint32_t* prev_count = B.g_prev_count.as<uint32_t>(64);
+  B.g_prev_list.as<uint32_t>(size_t(std::max<int64_t>(B.tiles_total, 1)));
+  if (!B.last_graph) {
+    // eager steps wrote node blocks this mode's stale list does not name
+    Args<T, D> a = batch_args<T, D>(g, B);
+    hipLaunchKernelGGL((k_zero_grid_batch<T, D>), dim3(blocks_for(B.max_nodes, 256), unsigned(B.nscenes)), dim3(256),
+                       0, gs, a);
+    TESS_HIP(hipGetLastError());
+    fill_u32(gs, prev_count, 1, 0u);
+    B.last_graph = true;
+  }
+  int done = 0;
+  while (done < nsteps) {
+    if (B.reorder_threshold > 0.0 && B.ntot > 0) {
+      batch_bin<T, D, BASIS>(g, B, false);
+      batch_reorder<T, D>(g, B, B.reorder_threshold);
+    }
+    if (!B.graph_exec || B.captured_variant != B.variant) {
+      // one ordinary step sizes every buffer; the same step is then captured
+      graph_release(B);
+      batch_graph_grids<T, D, BASIS>(g, B);
+      batch_graph_step<T, D, BASIS>(g, B);
+      ++done;
+      TESS_HIP(hipStreamSynchronize(gs));
+      TESS_HIP(hipStreamBeginCapture(gs, hipStreamCaptureModeRelaxed));
+      try {
+        batch_graph_step<T, D, BASIS>(g, B);
+      } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(gs, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+      }
+      TESS_HIP(hipStreamEndCapture(gs, &B.graph));
+      TESS_HIP(hipGraphInstantiate(&B.graph_exec, B.graph, nullptr, nullptr, 0));
+      B.captured_variant = B.variant;
+      B.captures += 1;
+      continue;
+    }
+    const int chunk = std::min(B.graph_chunk, nsteps - done);
+    for (int k = 0; k < chunk; ++k) TESS_HIP(hipGraphLaunch(B.graph_exec, gs));
+    B.graph_replays += chunk;
+    done += chunk;
+  }
+  TESS_HIP(hipEventRecord(B.graph_ev[1], gs));
+  TESS_HIP(hipStreamWaitEvent(ctx->stream, B.graph_ev[1], 0));
+  // the error flags of the call's binnings (one readback per call)
+  uint32_t* hc = reinterpret_cast<uint32_t*>(g->counters_host);
+  TESS_HIP(hipMemcpyAsync(hc, g->counters.as<uint32_t>(CNT_SLOTS), CNT_SLOTS * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, gs));
+  TESS_HIP(hipStreamSynchronize(gs));
+  if (hc[CNT_ERR] & ERR_GIMP_LENGTH) raise(TESS_ERR_ARGUMENT, "uGIMP requires 0 ≤ pt.l ≤ spacing(mesh)");
+  // the eager path's binning and zeroing bookkeeping no longer hold
+  B.keys_ready = false;
+  B.binned = false;
+  B.counted = false;
+  g->bin.valid = false;
+  ctx->bin.valid = false;
+}
+
 template <typename T, int D, int B_, bool S>
 struct BatchStepOp {
   static void run(tess_ctx* ctx, const tess_mesh&, tess_batch& b, int nsteps) {
     if constexpr (S) raise(TESS_ERR_UNSUPPORTED, "batches evaluate the basis inside the transfers");
+    else if (b.variant & 8) batch_graph_run<T, D, B_>(ctx, b, nsteps);
     else batch_step_impl<T, D, B_>(ctx, b, nsteps);
   }
 };
@@ -497,11 +813,23 @@ void run_batch_set_reorder(tess_batch* b, double threshold) { b->reorder_thresho
 int run_batch_variant(const tess_batch* b) { return b->variant; }
 
 void run_batch_stats(const tess_batch* b, int64_t* out, int n) {
-  const int64_t vals[4] = {b->reorders, b->reorders_by_count, b->kept_binnings, b->variant};
-  for (int k = 0; k < n && k < 4; ++k) out[k] = vals[k];
+  const int64_t vals[6] = {b->reorders, b->reorders_by_count, b->kept_binnings, b->variant, b->graph_replays,
+                           b->captures};
+  for (int k = 0; k < n && k < 6; ++k) out[k] = vals[k];
 }
 
 void run_batch_destroy(tess_batch* b) {
+  if (b->gctx) {
+    hipStream_t gs = b->gctx->stream;
+    graph_release(*b);
+    ctx_free(b->gctx);  // synchronises its stream
+    if (gs) (void)hipStreamDestroy(gs);
+    b->gctx = nullptr;
+  }
+  for (hipEvent_t e : b->graph_ev)
+    if (e) (void)hipEventDestroy(e);
+  b->g_prev_list.release();
+  b->g_prev_count.release();
   b->recs.release();
   b->d_toff.release();
   b->d_poff.release();

@@ -172,6 +172,21 @@ struct tess_batch {
   // the counters), over this many pairs
   bool counted = false;
   int64_t counted_pairs = 0;
+  // graph mode (variant bit 3): a private context on its own stream steps the
+  // batch as a captured hipGraph of persistent launches; the previous step's
+  // node-block list and count live on the device
+  tess_ctx* gctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  hipEvent_t graph_ev[2] = {nullptr, nullptr};
+  int captured_variant = -1;
+  tess::Buffer g_prev_list, g_prev_count;
+  unsigned grid_p2g = 0, grid_g2p = 0, grid_gather = 0, grid_zero = 0, grid_out = 0;
+  // the grids were last written by graph-mode steps (the eager bookkeeping does not know them)
+  bool last_graph = false;
+  // replays between two reorder tests (each test bins on the host side)
+  int graph_chunk = 16;
+  int64_t graph_replays = 0, captures = 0;
 };
 
 namespace tess {

@@ -1131,9 +1131,9 @@ __device__ __forceinline__ void p2g_load(const Args<T, D>& a, uint32_t pid, P2GI
 // the registers of one wave hold 1/NPASS of the stencil, which keeps several
 // waves per SIMD resident).  The waves add their stencils into the LDS tile
 // one after another, so the per-node order of the additions is fixed.
-template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false, bool SEP = false>
-__global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
-                                                          uint32_t* __restrict__ misplaced) {
+template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH, bool SEP>
+__device__ __forceinline__ void k_p2g_fast_body(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                          uint32_t* __restrict__ misplaced, const int64_t j) {
   using G = Geo<D>;
   using PT = Part<D, support_width(BASIS)>;
   constexpr int SW = support_width(BASIS);
@@ -1143,8 +1143,6 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
   constexpr int NT = 64 * NPASS;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform: scalar branches
-  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
-  if (j >= a.n_active) return;
   const uint32_t tile = a.active[j];
   const uint32_t start = a.tile_start[tile];
   const int cnt = int(a.tile_end[tile] - start);
@@ -1404,6 +1402,28 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
   }
 }
 
+template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false, bool SEP = false>
+__global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                          uint32_t* __restrict__ misplaced) {
+  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
+  if (j < a.n_active) k_p2g_fast_body<T, D, BASIS, NPASS, AOS, BATCH, SEP>(a, skeys, misplaced, j);
+}
+
+// Persistent form: a fixed grid walks the work items up to the count the
+// device holds at `count` (a multiple of 8 workgroups keeps each XCD on a
+// contiguous run of them, as xcd_remap does for one workgroup per item).
+template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false, bool SEP = false>
+__global__ __launch_bounds__(64 * NPASS) void k_p2g_fast_persist(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                          uint32_t* __restrict__ misplaced, const uint32_t* __restrict__ count) {
+  const int64_t N = int64_t(*count), R = ((N + 7) >> 3) << 3;
+  for (int64_t it = blockIdx.x; it < R; it += gridDim.x) {
+    const int64_t j = xcd_remap(it, N);
+    if (j < N) k_p2g_fast_body<T, D, BASIS, NPASS, AOS, BATCH, SEP>(a, skeys, misplaced, j);
+    __syncthreads();  // the next item reuses the workgroup's LDS
+  }
+}
+
+
 // G2P with the footprint's v and v - vⁿ staged per node in LDS; the FLIP
 // increment is formed from the node differences as the reference does.  NW
 // waves share one tile: wave w walks the class-major order at k = w, w+NW, ...
@@ -1438,16 +1458,14 @@ __device__ __forceinline__ void g2p_load(const Args<T, D>& a, uint32_t pid, G2PI
 #ifndef TESS_G2P_WAVES_PER_TILE
 #define TESS_G2P_WAVES_PER_TILE 4
 #endif
-template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false, bool SEP = false>
-__global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD : 1) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
+template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH, bool SEP>
+__device__ __forceinline__ void k_g2p_fast_body(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                        uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
-                                                       uint32_t ntiles, uint32_t* __restrict__ movers) {
+                                                       uint32_t ntiles, uint32_t* __restrict__ movers, const int64_t j) {
   using G = Geo<D>;
   constexpr int SW = support_width(BASIS);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform: scalar branches
-  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
-  if (j >= a.n_active) return;
   const uint32_t tile = a.active[j];
   const uint32_t start = a.tile_start[tile];
   const int cnt = int(a.tile_end[tile] - start);
@@ -1692,15 +1710,37 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
     walk(std::false_type{});
 }
 
+template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false, bool SEP = false>
+__global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD : 1) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                       uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
+                                                       uint32_t ntiles, uint32_t* __restrict__ movers) {
+  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
+  if (j < a.n_active) k_g2p_fast_body<T, D, BASIS, AOS, NW, BATCH, SEP>(a, skeys, nk, stay, ntiles, movers, j);
+}
+
+// Persistent form: a fixed grid walks the work items up to the count the
+// device holds at `count` (a multiple of 8 workgroups keeps each XCD on a
+// contiguous run of them, as xcd_remap does for one workgroup per item).
+template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false, bool SEP = false>
+__global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD : 1) void k_g2p_fast_persist(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                       uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
+                                                       uint32_t ntiles, uint32_t* __restrict__ movers, const uint32_t* __restrict__ count) {
+  const int64_t N = int64_t(*count), R = ((N + 7) >> 3) << 3;
+  for (int64_t it = blockIdx.x; it < R; it += gridDim.x) {
+    const int64_t j = xcd_remap(it, N);
+    if (j < N) k_g2p_fast_body<T, D, BASIS, AOS, NW, BATCH, SEP>(a, skeys, nk, stay, ntiles, movers, j);
+    __syncthreads();  // the next item reuses the workgroup's LDS
+  }
+}
+
+
 // ---- gather + grid-node half ------------------------------------------------
-template <typename T, int D, int SW, bool GAOS, bool BATCH = false>
-__global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __restrict__ nodes_per_block) {
+template <typename T, int D, int SW, bool GAOS, bool BATCH>
+__device__ __forceinline__ void k_gather_grid_body(Args<T, D> a, uint32_t* __restrict__ nodes_per_block, const int64_t j) {
   using G = Geo<D>;
   using PT = Part<D, SW>;
   constexpr int NV = 1 + 2 * D;
   constexpr int NB = (D == 3) ? 27 : 9;
-  const int64_t j = xcd_remap(blockIdx.x, a.n_nb);
-  if (j >= a.n_nb) return;
   const uint32_t B = a.nb_list[j];
   uint32_t poff;  // a batch's scene: local node block; tiles of the scene start at toff
   const uint32_t Bl = bind_tile<BATCH>(a, B, poff);
@@ -1810,6 +1850,26 @@ __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __re
     fldw<T, D, GAOS>(a.gv, i, c) = vo[c];
   }
 }
+
+template <typename T, int D, int SW, bool GAOS, bool BATCH = false>
+__global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __restrict__ nodes_per_block) {
+  const int64_t j = xcd_remap(blockIdx.x, a.n_nb);
+  if (j < a.n_nb) k_gather_grid_body<T, D, SW, GAOS, BATCH>(a, nodes_per_block, j);
+}
+
+// Persistent form: a fixed grid walks the work items up to the count the
+// device holds at `count` (a multiple of 8 workgroups keeps each XCD on a
+// contiguous run of them, as xcd_remap does for one workgroup per item).
+template <typename T, int D, int SW, bool GAOS, bool BATCH = false>
+__global__ __launch_bounds__(64) void k_gather_grid_persist(Args<T, D> a, uint32_t* __restrict__ nodes_per_block, const uint32_t* __restrict__ count) {
+  const int64_t N = int64_t(*count), R = ((N + 7) >> 3) << 3;
+  for (int64_t it = blockIdx.x; it < R; it += gridDim.x) {
+    const int64_t j = xcd_remap(it, N);
+    if (j < N) k_gather_grid_body<T, D, SW, GAOS, BATCH>(a, nodes_per_block, j);
+    __syncthreads();  // the next item reuses the workgroup's LDS
+  }
+}
+
 
 // ---- G2P -------------------------------------------------------------------
 

@@ -939,13 +939,16 @@ class Batch:
         _check(self.ctx.lib, self.ctx.lib.tess_batch_set_reorder(self.handle, float(threshold)))
 
     def set_variant(self, variant: int) -> None:
-        """Launch variant to use (a number autotune returned): in 2D 0..5, in 3D 0 or 2."""
+        """Launch variant to use (a number autotune returned): in 2D 0..5, in 3D 0 or 2, each
+        optionally with bit 3 (+8): graph mode, every step one replay of a captured hipGraph of
+        persistent launches with no host readback inside the step."""
         _check(self.ctx.lib, self.ctx.lib.tess_batch_set_variant(self.handle, int(variant)))
 
     def stats(self) -> dict:
-        vals = (ctypes.c_int64 * 4)()
-        _check(self.ctx.lib, self.ctx.lib.tess_batch_stats(self.handle, vals, 4))
-        return dict(zip(("reorders", "reorders_by_count", "kept_binnings", "variant"), list(vals)))
+        vals = (ctypes.c_int64 * 6)()
+        _check(self.ctx.lib, self.ctx.lib.tess_batch_stats(self.handle, vals, 6))
+        return dict(zip(("reorders", "reorders_by_count", "kept_binnings", "variant", "graph_replays", "captures"),
+                        list(vals)))
 
     def autotune(self, steps_per_variant: int = 2) -> int:
         v = ctypes.c_int32(0)

@@ -12,7 +12,7 @@ from tesserae_hip.scenes import batched_scenes, random_box
 pytestmark = pytest.mark.gpu
 
 
-def _run_and_check(oracle, scenes, basis, nsteps, tune=0, variant=None):
+def _run_and_check(oracle, scenes, basis, nsteps, tune=0, variant=None, calls=None):
     members, gpu = [], []
     for sc in scenes:
         mesh, P, G = to_gpu(sc)
@@ -23,12 +23,18 @@ def _run_and_check(oracle, scenes, basis, nsteps, tune=0, variant=None):
     extra = 0
     if tune:
         v = batch.autotune(tune)
-        assert v in (range(6) if scenes[0].mesh.dim == 2 else (0, 2))
-        extra = (6 if scenes[0].mesh.dim == 2 else 2) * (tune + 1)
+        eager = list(range(6)) if scenes[0].mesh.dim == 2 else [0, 2]
+        assert v in eager + [e | 8 for e in eager]
+        extra = 2 * len(eager) * (tune + 1)
     if variant is not None:
         batch.set_variant(variant)
-    batch.step(nsteps)
+    for k in calls or (nsteps,):
+        batch.step(k)
     ctx.sync()
+    if variant is not None and variant & 8:
+        st = batch.stats()
+        print(f"[batch graph] variant={variant} {st}", flush=True)
+        assert st["captures"] == 1 and st["graph_replays"] == nsteps - 1
     for sc, P in zip(scenes, gpu):
         st = HostState(sc)
         oracle.step(st, basis, sc.params, nsteps + extra)
@@ -61,10 +67,11 @@ def test_batch_set_variant_rejects_unknown_numbers():
     sc = random_box(3, 200, 1, th.BSpline(th.Quadratic()), dtype=torch.float64)
     mesh, P, G = to_gpu(sc)
     batch = th.Batch([(G, P, sc.params)], th.BSpline(th.Quadratic()))
-    for bad in (1, 3, 4, 6, -1):
+    for bad in (1, 3, 4, 6, 9, 16, -1):
         with pytest.raises(th.ArgumentError):
             batch.set_variant(bad)
     batch.set_variant(2)
+    batch.set_variant(10)
     batch.close()
 
 
@@ -123,5 +130,83 @@ def test_batch_reorder_matches_oracle(oracle, threshold):
             assert rel_err(got[name][ig], getattr(st, name)[io]) <= 1e-10, (sc.name, name)
         permuted += int(np.any(ig != io))
     assert permuted > 0
+    batch.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("variant", [8, 9, 10, 13])
+def test_batch_graph_mode_matches_oracle(oracle, variant):
+    """Graph mode: one ordinary step sizes the buffers and the same step is captured; the rest are replays of
+    the captured hipGraph (persistent launches over device-side counts, every particle sorted again each
+    step).  Two calls, of two and three steps: the second call replays the graph captured by the first."""
+    scenes = batched_scenes(7, seed=21, dtype=torch.float64, cells=(16, 64), particles=(100, 3000))
+    _run_and_check(oracle, scenes, th.BSpline(th.Quadratic()), 5, variant=variant, calls=(2, 3))
+
+
+def test_batch_graph_mode_3d_matches_oracle(oracle):
+    basis = th.BSpline(th.Quadratic())
+    scenes = [random_box(3, 700 + 400 * k, 80 + k, basis, dtype=torch.float64, h=0.1 / (k + 1), lo=0.05, hi=0.9,
+                         stress=0.5) for k in range(3)]
+    _run_and_check(oracle, scenes, basis, 4, variant=8)
+
+
+def test_batch_switches_between_eager_and_graph_mode(oracle):
+    """Eager, graph, eager again on the same batch: each mode zeroes the node blocks the other left behind."""
+    basis = th.BSpline(th.Quadratic())
+    scenes = batched_scenes(5, seed=23, dtype=torch.float64, cells=(16, 48), particles=(100, 2500))
+    members, gpu = [], []
+    for sc in scenes:
+        mesh, P, G = to_gpu(sc)
+        members.append((G, P, sc.params))
+        gpu.append(P)
+    ctx = th.Context()
+    batch = th.Batch(members, basis, ctx=ctx)
+    batch.set_variant(2)
+    batch.step(2)
+    batch.set_variant(10)
+    batch.step(3)
+    batch.set_variant(2)
+    batch.step(2)
+    ctx.sync()
+    for sc, P in zip(scenes, gpu):
+        st = HostState(sc)
+        oracle.step(st, basis, sc.params, 7)
+        got = particles_np(P)
+        for name in ("x", "v", "F", "sigma", "V"):
+            assert rel_err(got[name], getattr(st, name)) <= 1e-10, (sc.name, name)
+    batch.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("threshold", [1.0, 0.999], ids=["every_chunk", "by_contiguity"])
+def test_batch_graph_mode_reorder_matches_oracle(oracle, threshold):
+    """Graph mode with the layered reorder: the test runs before the call's replays, on the batch's private
+    stream; every scene equals the oracle's run of it up to the order of its particles."""
+    basis = th.BSpline(th.Quadratic())
+    scenes = [random_box(2, 300 + 400 * k, 90 + k, basis, dtype=torch.float64, h=0.05 * (1 + k % 3), lo=-0.05,
+                         hi=1.05, stress=0.5, vscale=3.0) for k in range(6)]
+    members, gpu = [], []
+    for sc in scenes:
+        mesh, P, G = to_gpu(sc)
+        members.append((G, P, sc.params))
+        gpu.append(P)
+    ctx = th.Context()
+    batch = th.Batch(members, basis, ctx=ctx)
+    batch.set_variant(8)
+    batch.set_reorder(threshold)
+    batch.step(3)
+    batch.step(3)
+    ctx.sync()
+    stats = batch.stats()
+    print(f"[batch graph reorder] threshold={threshold} {stats}", flush=True)
+    if threshold >= 1.0:
+        assert stats["reorders"] >= 2
+    for sc, P in zip(scenes, gpu):
+        st = HostState(sc)
+        oracle.step(st, basis, sc.params, 6)
+        got = particles_np(P)
+        ig, io = np.lexsort(got["x"].T[::-1]), np.lexsort(st.x.T[::-1])
+        for name in ("x", "v", "F", "sigma", "V"):
+            assert rel_err(got[name][ig], getattr(st, name)[io]) <= 1e-10, (sc.name, name)
     batch.close()
     ctx.close()
