@@ -393,11 +393,13 @@ tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_
  * per last-axis offset, and G2P runs 4 waves per tile, 2 with bit 1, 1 with
  * bit 2 (0..5); in 3D 0 or 2.  Bit 3 (+8) selects graph mode: the batch steps
  * on a private stream and scratch, each step one replay of a captured hipGraph
- * whose binning sorts every particle and whose tile and node-block kernels are
- * persistent launches reading their counts on the device (no host readback
- * inside a step; the reorder test and one error-flag readback run per call and
- * per 16 replays).  Other numbers are TESS_ERR_ARGUMENT.  No counterpart in the
- * reference, which has no launch variants. */
+ * whose binning sorts every particle and whose tile and node-block kernels
+ * read their counts on the device, launched with one workgroup per possible
+ * tile or, with bit 4 as well (+24), as persistent grids sized to the device
+ * (no host readback inside a step; the reorder test and one error-flag
+ * readback run per call and per 16 replays).  Other numbers are
+ * TESS_ERR_ARGUMENT.  No counterpart in the reference, which has no launch
+ * variants. */
 tess_status tess_batch_set_variant(tess_batch* batch, int32_t variant);
 /* Counters of the batch, host values, the first n of: [0] reorders done, [1] of
  * those, reorders decided from the broken-pair count of the previous step's

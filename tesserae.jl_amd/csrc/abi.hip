@@ -603,10 +603,12 @@ tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_
     if (steps_per_variant < 1) raise(TESS_ERR_ARGUMENT, "steps_per_variant must be positive");
     const int dim = batch->dim;
     // 2D: P2G split (bit 0) × G2P with 4, 2 or 1 waves per tile; 3D: the P2G
-    // split is fixed, G2P with 4 or 2 waves; each eagerly launched and as
-    // captured graph replays of persistent launches (bit 3)
-    const int cand2[12] = {0, 1, 2, 3, 4, 5, 8, 9, 10, 11, 12, 13}, cand3[4] = {0, 2, 8, 10};
-    const int ncand = dim == 2 ? 12 : 4;
+    // split is fixed, G2P with 4 or 2 waves; each launched eagerly, as
+    // captured graph replays (bit 3) with one workgroup per possible tile, and
+    // as graph replays of persistent launches (bits 3 and 4)
+    const int cand2[18] = {0, 1, 2, 3, 4, 5, 8, 9, 10, 11, 12, 13, 24, 25, 26, 27, 28, 29};
+    const int cand3[6] = {0, 2, 8, 10, 24, 26};
+    const int ncand = dim == 2 ? 18 : 6;
     // the caller's reorder threshold and variant come back and the events are
     // released on every exit, a failed step included
     struct Restore {
@@ -659,8 +661,9 @@ tess_status tess_batch_autotune(tess_ctx* ctx, tess_batch* batch, int32_t steps_
 tess_status tess_batch_set_variant(tess_batch* batch, int32_t variant) {
   return guarded([&] {
     if (!batch) raise(TESS_ERR_ARGUMENT, "batch is NULL");
-    const int v = variant & ~8;  // bit 3: graph mode
-    const bool ok = variant >= 0 && (batch->dim == 2 ? v <= 5 : (v == 0 || v == 2));
+    // bit 3: graph mode; bit 4 (with bit 3): its persistent launches
+    const int v = variant & ~24;
+    const bool ok = variant >= 0 && ((variant & 24) != 16) && (batch->dim == 2 ? v <= 5 : (v == 0 || v == 2));
     if (!ok) raise(TESS_ERR_ARGUMENT, "no launch variant " + std::to_string(variant) + " for a " +
                                           std::to_string(batch->dim) + "D batch");
     run_batch_set_variant(batch, variant);

@@ -9,6 +9,7 @@
 // variants (waves per tile in P2G and G2P) on the batch itself.
 #include "transfer_impl.hpp"
 
+#include <algorithm>
 #include <vector>
 
 namespace tess {
@@ -491,9 +492,13 @@ __global__ __launch_bounds__(256) void k_save_list(const uint32_t* __restrict__ 
 // Particles outside their mesh (sorted positions from the device count of
 // inside particles to n): the update with zero grid increments.
 template <typename T, int D>
-__global__ __launch_bounds__(256) void k_g2p_outside_persist(Args<T, D> a, const uint32_t* __restrict__ n_inside) {
+__global__ __launch_bounds__(256) void k_g2p_outside_persist(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                             uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
+                                                             const uint32_t* __restrict__ n_inside) {
   const int64_t from = int64_t(*n_inside);
   for (int64_t j = from + int64_t(blockIdx.x) * 256 + threadIdx.x; j < a.n; j += int64_t(gridDim.x) * 256) {
+    nk[j] = skeys[j];  // the position does not change: the key stays
+    stay[j] = 1u;
     uint32_t pid = a.perm[j];
     pid -= uint32_t(bind_scene(a, scene_of(a.scene_poff, a.nscenes, pid)).poff);
     T dv[D], L[D * D], dx[D];
@@ -514,12 +519,12 @@ int device_cus(int dev) {
   return std::max(c, 1);
 }
 
-// Workgroups of a persistent launch: as many as the device holds at once (its
-// occupancy times the CUs), at most one per work item, a multiple of 8.
+// Workgroups of a persistent launch: twice what the occupancy query reports
+// the CUs hold at once, at most one per work item, a multiple of 8.
 unsigned persist_grid(const void* kern, int threads, int64_t max_items, int cus) {
   int per = 0;
   TESS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, 0));
-  int64_t g = int64_t(std::max(per, 1)) * cus;
+  int64_t g = 2 * int64_t(std::max(per, 1)) * cus;
   g = std::min<int64_t>(g, int64_t(xcd_grid(std::max<int64_t>(max_items, 1))));
   return unsigned(((g + 7) / 8) * 8);
 }
@@ -551,18 +556,49 @@ void batch_graph_grids(tess_ctx* g, tess_batch& B) {
                             ceil_div(std::max<int64_t>(B.ntot, 1), int64_t(256)), cus);
 }
 
+__global__ void k_mover_count(const uint32_t* __restrict__ stay_total, int64_t n, uint32_t* __restrict__ movers) {
+  if (threadIdx.x == 0) *movers = uint32_t(n - int64_t(*stay_total));
+}
+
+// Graph mode's binning double buffer: (sorted keys, permutation) in set 0 =
+// (keys_sorted, vals) or set 1 = (skeys2, perm2).
+struct BinSets {
+  uint32_t* keys[2];
+  uint32_t* perm[2];
+};
+inline BinSets bin_sets(tess_ctx* g, int64_t n) {
+  const size_t m = size_t(std::max<int64_t>(n, 1));
+  return BinSets{{g->keys_sorted.as<uint32_t>(m), g->skeys2.as<uint32_t>(m)},
+                 {g->vals.as<uint32_t>(m), g->perm2.as<uint32_t>(m)}};
+}
+constexpr int CNT_MOVERS = 12;  // counter slot of the mover count (graph mode)
+
 // One step of the batch on context g, every launch size fixed and every count
-// on the device (the form that is captured).
+// on the device.  full: the binning sorts every particle's key into set 0
+// (the step that starts each call); else it merges the particles whose key
+// the previous step's G2P changed (written with the stayer flags at the
+// sorted positions of set `in`) into set 1-in, the form that is captured.
+// Either way the G2P writes the next keys and stayer flags.
 template <typename T, int D, int BASIS>
-void batch_graph_step(tess_ctx* g, tess_batch& B) {
+void batch_graph_step(tess_ctx* g, tess_batch& B, bool full, int in) {
   constexpr int SW = support_width(BASIS);
   const int64_t n = B.ntot;
   const uint64_t ntiles = uint64_t(B.tiles_total);
+  const size_t m = size_t(std::max<int64_t>(n, 1));
   hipStream_t st = g->stream;
   uint32_t* counters = g->counters.as<uint32_t>(CNT_SLOTS);
-  uint32_t* keys = g->keys.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
-  uint32_t* skeys = g->keys_sorted.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
-  uint32_t* perm = g->vals.as<uint32_t>(size_t(std::max<int64_t>(n, 1)));
+  uint32_t* keys = g->keys.as<uint32_t>(m);
+  const BinSets sets = bin_sets(g, n);
+  uint32_t* nk = g->inc_nk.as<uint32_t>(m);
+  uint32_t* spos = g->inc_spos.as<uint32_t>(m + 1);
+  uint32_t* mkey = g->inc_mkey.as<uint32_t>(m);
+  uint32_t* mj = g->inc_mj.as<uint32_t>(m);
+  uint32_t* mks = g->inc_mks.as<uint32_t>(m + 1);  // the reorder uses it with n + 1 words
+  uint32_t* mr = g->inc_mr.as<uint32_t>(m);
+  uint32_t* mjs = g->inc_mjs.as<uint32_t>(m);
+  const int64_t nsb = blocks_for(std::max<int64_t>(n, 1), STAY_BLOCK);
+  int64_t* ranges = g->inc_rng.as<int64_t>(size_t(2 * nsb));
+  uint32_t* movers = movers_counters(g);  // G2P's mover tally (unused here)
   uint32_t* tstart = g->tile_start.as<uint32_t>(ntiles);
   uint32_t* tend = g->tile_end.as<uint32_t>(ntiles);
   uint32_t* tflag = g->tile_flag.as<uint32_t>(ntiles);
@@ -575,16 +611,52 @@ void batch_graph_step(tess_ctx* g, tess_batch& B) {
   uint32_t* prev_list = B.g_prev_list.as<uint32_t>(ntiles);
   uint32_t* prev_count = B.g_prev_count.as<uint32_t>(64);
   T* partials = g->partials.as<T>(size_t(ntiles) * Part<D, SW>::SIZE);
+  // the scans' work area at its largest use (spos, the tile lists, the sort's
+  // histograms, the reorder's ranks), so no later call moves it
+  g->scan_tmp.get(std::max({scan_work_bytes(n + 1), scan_work_bytes(int64_t(ntiles)),
+                            scan_work_bytes(int64_t(sort_hist_words(std::max<int64_t>(n, 1))))}));
+  const int out = full ? 0 : 1 - in;
+  uint32_t* skeys = sets.keys[out];
+  uint32_t* perm = sets.perm[out];
   Args<T, D> a = batch_args<T, D>(g, B);
+  a.perm = perm;
   a.partials = partials;
   const int kbits = key_bits_for(ntiles * 64);
-  // binning: every particle sorted again
   fill_u32(st, counters, CNT_BIN_SLOTS, 0u);
   if (n > 0) {
-    hipLaunchKernelGGL((k_bin_keys<T, D, BASIS, false, true>), dim3(blocks_for(n, 256)), dim3(256), 0, st, a, keys,
-                       uint32_t(ntiles), counters);
-    TESS_HIP(hipGetLastError());
-    radix_sort_pairs(g, keys, nullptr, skeys, perm, n, kbits);
+    if (full) {
+      hipLaunchKernelGGL((k_bin_keys<T, D, BASIS, false, true>), dim3(blocks_for(n, 256)), dim3(256), 0, st, a, keys,
+                         uint32_t(ntiles), counters);
+      TESS_HIP(hipGetLastError());
+      radix_sort_pairs(g, keys, nullptr, skeys, perm, n, kbits);
+    } else {
+      // stayers keep their order; the movers, sorted by (key, previous
+      // position), are merged in (the mover count M stays on the device)
+      const uint32_t* M = counters + CNT_MOVERS;
+      fill_u32(st, spos + n, 1, 0u);
+      exclusive_scan_u32(g, spos, n + 1, counters + CNT_STAY);
+      hipLaunchKernelGGL(k_mover_count, dim3(1), dim3(64), 0, st, static_cast<const uint32_t*>(counters + CNT_STAY), n,
+                         counters + CNT_MOVERS);
+      TESS_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_compact_movers, dim3(blocks_for(n, 256)), dim3(256), 0, st, static_cast<const uint32_t*>(nk),
+                         static_cast<const uint32_t*>(spos), n, mkey, mj);
+      TESS_HIP(hipGetLastError());
+      radix_sort_pairs(g, mkey, nullptr, mks, mr, n, kbits, M);
+      hipLaunchKernelGGL(k_place_movers, dim3(blocks_for(n, 256)), dim3(256), 0, st, static_cast<const uint32_t*>(mks),
+                         static_cast<const uint32_t*>(mr), static_cast<const uint32_t*>(mj),
+                         static_cast<const uint32_t*>(sets.keys[in]), static_cast<const uint32_t*>(sets.perm[in]),
+                         static_cast<const uint32_t*>(spos), n, n, mjs, skeys, perm, M);
+      TESS_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_stay_ranges, dim3(blocks_for(nsb, 256)), dim3(256), 0, st,
+                         static_cast<const uint32_t*>(sets.keys[in]), n, static_cast<const uint32_t*>(mks),
+                         static_cast<const uint32_t*>(mjs), n, nsb, ranges, M);
+      TESS_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_place_stayers, dim3(unsigned(nsb)), dim3(STAY_BLOCK), 0, st,
+                         static_cast<const uint32_t*>(sets.keys[in]), static_cast<const uint32_t*>(sets.perm[in]),
+                         static_cast<const uint32_t*>(spos), n, static_cast<const uint32_t*>(mks),
+                         static_cast<const uint32_t*>(mjs), static_cast<const int64_t*>(ranges), skeys, perm);
+      TESS_HIP(hipGetLastError());
+    }
   }
   fill_u32(st, tstart, int64_t(ntiles), 0u);
   fill_u32(st, tend, int64_t(ntiles), 0u);
@@ -616,50 +688,130 @@ void batch_graph_step(tess_ctx* g, tess_batch& B) {
   TESS_HIP(hipGetLastError());
   const uint32_t* sk = skeys;
   const uint32_t* n_active = counters + CNT_ACTIVE;
-  if (D == 3 || (B.variant & 1))
-    hipLaunchKernelGGL((k_p2g_fast_persist<T, D, BASIS, SW, true, true>), dim3(B.grid_p2g), dim3(64 * SW), 0, st, a,
-                       sk, static_cast<uint32_t*>(nullptr), n_active);
-  else
-    hipLaunchKernelGGL((k_p2g_fast_persist<T, D, BASIS, 1, true, true>), dim3(B.grid_p2g), dim3(64), 0, st, a, sk,
-                       static_cast<uint32_t*>(nullptr), n_active);
+  const uint32_t* n_nb = counters + CNT_NB;
+  // bit 4: persistent grids; else one workgroup per possible tile or node block
+  const bool persist = (B.variant & 16) != 0;
+  const dim3 all_tiles(xcd_grid(int64_t(ntiles)));
+  uint32_t* none = nullptr;
+  if (D == 3 || (B.variant & 1)) {
+    if (persist)
+      hipLaunchKernelGGL((k_p2g_fast_persist<T, D, BASIS, SW, true, true>), dim3(B.grid_p2g), dim3(64 * SW), 0, st, a,
+                         sk, none, n_active);
+    else
+      hipLaunchKernelGGL((k_p2g_fast_grid<T, D, BASIS, SW, true, true>), all_tiles, dim3(64 * SW), 0, st, a, sk, none,
+                         n_active);
+  } else {
+    if (persist)
+      hipLaunchKernelGGL((k_p2g_fast_persist<T, D, BASIS, 1, true, true>), dim3(B.grid_p2g), dim3(64), 0, st, a, sk,
+                         none, n_active);
+    else
+      hipLaunchKernelGGL((k_p2g_fast_grid<T, D, BASIS, 1, true, true>), all_tiles, dim3(64), 0, st, a, sk, none,
+                         n_active);
+  }
   TESS_HIP(hipGetLastError());
-  hipLaunchKernelGGL((k_gather_grid_persist<T, D, SW, true, true>), dim3(B.grid_gather), dim3(64), 0, st, a, per_nb,
-                     static_cast<const uint32_t*>(counters + CNT_NB));
+  if (persist)
+    hipLaunchKernelGGL((k_gather_grid_persist<T, D, SW, true, true>), dim3(B.grid_gather), dim3(64), 0, st, a, per_nb,
+                       n_nb);
+  else
+    hipLaunchKernelGGL((k_gather_grid_grid<T, D, SW, true, true>), all_tiles, dim3(64), 0, st, a, per_nb, n_nb);
   TESS_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_save_list, dim3(unsigned(std::min<int64_t>(blocks_for(int64_t(ntiles), 256), 1024))), dim3(256),
-                     0, st, static_cast<const uint32_t*>(nblist), static_cast<const uint32_t*>(counters + CNT_NB),
-                     prev_list, prev_count);
+                     0, st, static_cast<const uint32_t*>(nblist), n_nb, prev_list, prev_count);
   TESS_HIP(hipGetLastError());
-  // G2P over the active tiles, then the particles outside every mesh
-  uint32_t* none = nullptr;
+  // G2P over the active tiles (next keys and stayer flags at the sorted
+  // positions), then the particles outside every mesh
   const uint32_t nt32 = uint32_t(ntiles);
-  if (B.variant & 4)
-    hipLaunchKernelGGL((k_g2p_fast_persist<T, D, BASIS, true, 1, true>), dim3(B.grid_g2p), dim3(64), 0, st, a, sk,
-                       none, none, nt32, none, n_active);
-  else if (B.variant & 2)
-    hipLaunchKernelGGL((k_g2p_fast_persist<T, D, BASIS, true, 2, true>), dim3(B.grid_g2p), dim3(128), 0, st, a, sk,
-                       none, none, nt32, none, n_active);
-  else
-    hipLaunchKernelGGL((k_g2p_fast_persist<T, D, BASIS, true, 4, true>), dim3(B.grid_g2p), dim3(256), 0, st, a, sk,
-                       none, none, nt32, none, n_active);
+  const int nw = (B.variant & 4) ? 1 : (B.variant & 2) ? 2 : 4;
+  auto g2p = [&](auto nwc) {
+    constexpr int NW = decltype(nwc)::value;
+    if (persist)
+      hipLaunchKernelGGL((k_g2p_fast_persist<T, D, BASIS, true, NW, true>), dim3(B.grid_g2p), dim3(64 * NW), 0, st, a,
+                         sk, nk, spos, nt32, movers, n_active);
+    else
+      hipLaunchKernelGGL((k_g2p_fast_grid<T, D, BASIS, true, NW, true>), all_tiles, dim3(64 * NW), 0, st, a, sk, nk,
+                         spos, nt32, movers, n_active);
+  };
+  if (nw == 1) g2p(std::integral_constant<int, 1>{});
+  else if (nw == 2) g2p(std::integral_constant<int, 2>{});
+  else g2p(std::integral_constant<int, 4>{});
   TESS_HIP(hipGetLastError());
   if (n > 0) {
-    hipLaunchKernelGGL((k_g2p_outside_persist<T, D>), dim3(B.grid_out), dim3(256), 0, st, a,
+    hipLaunchKernelGGL((k_g2p_outside_persist<T, D>), dim3(B.grid_out), dim3(256), 0, st, a, sk, nk, spos,
                        static_cast<const uint32_t*>(counters + CNT_INSIDE));
     TESS_HIP(hipGetLastError());
   }
 }
 
 void graph_release(tess_batch& B) {
-  if (B.graph_exec) (void)hipGraphExecDestroy(B.graph_exec);
-  if (B.graph) (void)hipGraphDestroy(B.graph);
-  B.graph_exec = nullptr;
-  B.graph = nullptr;
+  for (int p = 0; p < 2; ++p) {
+    if (B.graph_exec[p]) (void)hipGraphExecDestroy(B.graph_exec[p]);
+    if (B.graph[p]) (void)hipGraphDestroy(B.graph[p]);
+    B.graph_exec[p] = nullptr;
+    B.graph[p] = nullptr;
+  }
   B.captured_variant = -1;
+  B.graph_deps.clear();
+}
+
+// Every buffer a captured step points at, with its address at the capture.
+std::vector<std::pair<const Buffer*, void*>> graph_buffers(tess_ctx* g, tess_batch& B) {
+  const Buffer* bufs[] = {&g->counters,  &g->keys,       &g->keys_sorted,  &g->vals,         &g->skeys2,
+                          &g->perm2,     &g->keys_alt,   &g->vals_alt,     &g->hist,         &g->scan_tmp,
+                          &g->inc_nk,    &g->inc_spos,   &g->inc_mkey,     &g->inc_mj,       &g->inc_mks,
+                          &g->inc_mr,    &g->inc_mjs,    &g->inc_rng,      &g->movers,       &g->tile_start,
+                          &g->tile_end,  &g->tile_flag,  &g->tile_slot,    &g->active_tiles, &g->nb_flag,
+                          &g->nb_slot,   &g->nb_list,    &g->per_nodeblock, &g->partials,    &B.g_prev_list,
+                          &B.g_prev_count, &B.recs,      &B.d_toff,        &B.d_poff};
+  std::vector<std::pair<const Buffer*, void*>> v;
+  for (const Buffer* b : bufs) v.emplace_back(b, b->ptr);
+  return v;
+}
+
+bool graph_deps_hold(const tess_batch& B) {
+  for (const auto& d : B.graph_deps)
+    if (d.first->ptr != d.second) return false;
+  return !B.graph_deps.empty();
+}
+
+// The reorder test between replays, on the binning the last step left in set
+// `cur` (its counts read back), exactly as an eager step would run it.
+template <typename T, int D>
+void graph_reorder_check(tess_ctx* g, tess_batch& B, int cur) {
+  uint32_t* hc = reinterpret_cast<uint32_t*>(g->counters_host);
+  TESS_HIP(hipMemcpyAsync(hc, g->counters.as<uint32_t>(CNT_SLOTS), CNT_SLOTS * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, g->stream));
+  TESS_HIP(hipStreamSynchronize(g->stream));
+  Binning& b = g->bin;
+  b.n = B.ntot;
+  b.ntiles = B.tiles_total;
+  b.n_inside = B.ntot > 0 ? int64_t(hc[CNT_INSIDE]) : 0;
+  b.n_active = int64_t(hc[CNT_ACTIVE]);
+  b.n_nodeblocks = int64_t(hc[CNT_NB]);
+  b.valid = true;
+  if (cur == 1) {
+    std::swap(g->keys_sorted, g->skeys2);
+    std::swap(g->vals, g->perm2);
+  }
+  B.counted = false;
+  try {
+    batch_reorder<T, D>(g, B, B.reorder_threshold);
+  } catch (...) {
+    if (cur == 1) {
+      std::swap(g->keys_sorted, g->skeys2);
+      std::swap(g->vals, g->perm2);
+    }
+    throw;
+  }
+  if (cur == 1) {
+    std::swap(g->keys_sorted, g->skeys2);
+    std::swap(g->vals, g->perm2);
+  }
+  b.valid = false;
 }
 
 template <typename T, int D, int BASIS>
 void batch_graph_run(tess_ctx* ctx, tess_batch& B, int nsteps) {
+  if (nsteps <= 0) return;
   if (!B.gctx) {
     hipStream_t gs = nullptr;
     TESS_HIP(hipStreamCreateWithFlags(&gs, hipStreamNonBlocking));
@@ -687,38 +839,51 @@ void batch_graph_run(tess_ctx* ctx, tess_batch& B, int nsteps) {
     fill_u32(gs, prev_count, 1, 0u);
     B.last_graph = true;
   }
-  int done = 0;
+  if (B.captured_variant != B.variant) graph_release(B);
+  batch_graph_grids<T, D, BASIS>(g, B);
+  // each call starts with an ordinary step whose binning sorts every particle
+  // (the positions may have changed since the last call); it leaves set 0
+  batch_graph_step<T, D, BASIS>(g, B, true, 0);
+  int done = 1, cur = 0, since_test = 0;
   while (done < nsteps) {
-    if (B.reorder_threshold > 0.0 && B.ntot > 0) {
-      batch_bin<T, D, BASIS>(g, B, false);
-      batch_reorder<T, D>(g, B, B.reorder_threshold);
-    }
-    if (!B.graph_exec || B.captured_variant != B.variant) {
-      // one ordinary step sizes every buffer; the same step is then captured
+    if (!graph_deps_hold(B)) {
+      // first use, another variant, or a buffer moved: capture both parities
+      // (capturing launches nothing)
       graph_release(B);
-      batch_graph_grids<T, D, BASIS>(g, B);
-      batch_graph_step<T, D, BASIS>(g, B);
-      ++done;
       TESS_HIP(hipStreamSynchronize(gs));
-      TESS_HIP(hipStreamBeginCapture(gs, hipStreamCaptureModeRelaxed));
-      try {
-        batch_graph_step<T, D, BASIS>(g, B);
-      } catch (...) {
-        hipGraph_t dead = nullptr;
-        (void)hipStreamEndCapture(gs, &dead);
-        if (dead) (void)hipGraphDestroy(dead);
-        throw;
+      for (int p = 0; p < 2; ++p) {
+        TESS_HIP(hipStreamBeginCapture(gs, hipStreamCaptureModeRelaxed));
+        try {
+          batch_graph_step<T, D, BASIS>(g, B, false, p);
+        } catch (...) {
+          hipGraph_t dead = nullptr;
+          (void)hipStreamEndCapture(gs, &dead);
+          if (dead) (void)hipGraphDestroy(dead);
+          graph_release(B);
+          throw;
+        }
+        TESS_HIP(hipStreamEndCapture(gs, &B.graph[p]));
+        TESS_HIP(hipGraphInstantiate(&B.graph_exec[p], B.graph[p], nullptr, nullptr, 0));
       }
-      TESS_HIP(hipStreamEndCapture(gs, &B.graph));
-      TESS_HIP(hipGraphInstantiate(&B.graph_exec, B.graph, nullptr, nullptr, 0));
+      B.graph_deps = graph_buffers(g, B);
       B.captured_variant = B.variant;
       B.captures += 1;
-      continue;
     }
-    const int chunk = std::min(B.graph_chunk, nsteps - done);
-    for (int k = 0; k < chunk; ++k) TESS_HIP(hipGraphLaunch(B.graph_exec, gs));
-    B.graph_replays += chunk;
-    done += chunk;
+    if (B.reorder_threshold > 0.0 && B.ntot > 0 && since_test == 0) {
+      graph_reorder_check<T, D>(g, B, cur);
+      if (!graph_deps_hold(B)) {
+        // the test moved a buffer the graphs use: start over from the positions
+        batch_graph_step<T, D, BASIS>(g, B, true, 0);
+        ++done;
+        cur = 0;
+        continue;
+      }
+    }
+    TESS_HIP(hipGraphLaunch(B.graph_exec[cur], gs));
+    cur = 1 - cur;
+    ++done;
+    B.graph_replays += 1;
+    since_test = (since_test + 1) % std::max(B.graph_chunk, 1);
   }
   TESS_HIP(hipEventRecord(B.graph_ev[1], gs));
   TESS_HIP(hipStreamWaitEvent(ctx->stream, B.graph_ev[1], 0));

@@ -127,12 +127,20 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t digit, bool valid) {
   return peers;
 }
 
+// n_dev (when set): the element count read on the device, at most n; chunks
+// past it hold an empty histogram.
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
-                                                              uint32_t* __restrict__ hist, int64_t nchunks) {
+                                                              uint32_t* __restrict__ hist, int64_t nchunks,
+                                                              const uint32_t* __restrict__ n_dev) {
   __shared__ uint32_t s_hist[RADIX];
+  if (n_dev) n = int64_t(*n_dev);
+  const int64_t base = int64_t(blockIdx.x) * SORT_CHUNK;
+  if (base >= n) {
+    hist[int64_t(threadIdx.x) * nchunks + blockIdx.x] = 0u;
+    return;
+  }
   s_hist[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t base = int64_t(blockIdx.x) * SORT_CHUNK;
 #pragma unroll 4
   for (int r = 0; r < SORT_ROUNDS; ++r) {
     const int64_t i = base + int64_t(r) * SORT_THREADS + threadIdx.x;
@@ -150,12 +158,14 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* 
                                                                  uint32_t* __restrict__ keys_out,
                                                                  uint32_t* __restrict__ vals_out, int64_t n,
                                                                  int shift, const uint32_t* __restrict__ offsets,
-                                                                 int64_t nchunks) {
+                                                                 int64_t nchunks, const uint32_t* __restrict__ n_dev) {
   __shared__ uint32_t s_base[RADIX];
   __shared__ uint32_t s_wcount[SORT_WAVES][RADIX];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  s_base[threadIdx.x] = offsets[int64_t(threadIdx.x) * nchunks + blockIdx.x];
+  if (n_dev) n = int64_t(*n_dev);
   const int64_t base = int64_t(blockIdx.x) * SORT_CHUNK;
+  if (base >= n) return;  // uniform
+  s_base[threadIdx.x] = offsets[int64_t(threadIdx.x) * nchunks + blockIdx.x];
   for (int r = 0; r < SORT_ROUNDS; ++r) {
     const int64_t i = base + int64_t(r) * SORT_THREADS + threadIdx.x;
     const bool valid = i < n;
@@ -290,7 +300,7 @@ void exclusive_scan_u32(tess_ctx* ctx, uint32_t* data, int64_t n, uint32_t* tota
 size_t sort_hist_words(int64_t n) { return size_t(ceil_div(n, SORT_CHUNK)) * RADIX; }
 
 void radix_sort_pairs_ws(const SortWork& w, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                         uint32_t* vals_out, int64_t n, int key_bits) {
+                         uint32_t* vals_out, int64_t n, int key_bits, const uint32_t* n_dev) {
   if (n <= 0) return;
   if (n > int64_t(UINT32_MAX)) raise(TESS_ERR_ARGUMENT, "radix sort: more than 2^32-1 elements");
   const int passes = key_bits <= 0 ? 1 : (key_bits + RADIX_BITS - 1) / RADIX_BITS;
@@ -304,11 +314,11 @@ void radix_sort_pairs_ws(const SortWork& w, const uint32_t* keys_in, const uint3
     uint32_t* vout = to_out ? vals_out : w.vals_alt;
     const int shift = p * RADIX_BITS;
     hipLaunchKernelGGL(k_radix_hist, dim3(unsigned(nchunks)), dim3(SORT_THREADS), 0, w.stream, kin, n, shift,
-                       w.hist, nchunks);
+                       w.hist, nchunks, n_dev);
     TESS_HIP(hipGetLastError());
     exclusive_scan_u32_ws(w.stream, w.scan_tmp, w.hist, nchunks * RADIX, nullptr);
     hipLaunchKernelGGL(k_radix_scatter, dim3(unsigned(nchunks)), dim3(SORT_THREADS), 0, w.stream, kin, vin, kout,
-                       vout, n, shift, w.hist, nchunks);
+                       vout, n, shift, w.hist, nchunks, n_dev);
     TESS_HIP(hipGetLastError());
     kin = kout;
     vin = vout;
@@ -316,7 +326,7 @@ void radix_sort_pairs_ws(const SortWork& w, const uint32_t* keys_in, const uint3
 }
 
 void radix_sort_pairs(tess_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                      uint32_t* vals_out, int64_t n, int key_bits) {
+                      uint32_t* vals_out, int64_t n, int key_bits, const uint32_t* n_dev) {
   if (n <= 0) return;
   if (n > int64_t(UINT32_MAX)) raise(TESS_ERR_ARGUMENT, "radix sort: more than 2^32-1 elements");
   SortWork w;
@@ -327,7 +337,7 @@ void radix_sort_pairs(tess_ctx* ctx, const uint32_t* keys_in, const uint32_t* va
   w.vals_alt = ctx->vals_alt.as<uint32_t>(size_t(n));
   w.scan_tmp = ceil_div(int64_t(hw), SCAN_CHUNK) > 1 ? static_cast<uint8_t*>(ctx->scan_tmp.get(scan_work_bytes(int64_t(hw))))
                                                       : nullptr;
-  radix_sort_pairs_ws(w, keys_in, vals_in, keys_out, vals_out, n, key_bits);
+  radix_sort_pairs_ws(w, keys_in, vals_in, keys_out, vals_out, n, key_bits, n_dev);
 }
 
 }  // namespace tess

@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/tesserae_hip.h"
@@ -173,13 +174,16 @@ struct tess_batch {
   bool counted = false;
   int64_t counted_pairs = 0;
   // graph mode (variant bit 3): a private context on its own stream steps the
-  // batch as a captured hipGraph of persistent launches; the previous step's
-  // node-block list and count live on the device
+  // batch as replays of captured hipGraphs, one per parity of the binning's
+  // double buffer (graph[p] reads binning set p and writes set 1-p); the
+  // previous step's node-block list and count live on the device, and the
+  // buffers the graphs point at are recorded to detect a reallocation
   tess_ctx* gctx = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   hipEvent_t graph_ev[2] = {nullptr, nullptr};
   int captured_variant = -1;
+  std::vector<std::pair<const tess::Buffer*, void*>> graph_deps;
   tess::Buffer g_prev_list, g_prev_count;
   unsigned grid_p2g = 0, grid_g2p = 0, grid_gather = 0, grid_zero = 0, grid_out = 0;
   // the grids were last written by graph-mode steps (the eager bookkeeping does not know them)
@@ -201,8 +205,10 @@ void exclusive_scan_u32(tess_ctx* ctx, uint32_t* data, int64_t n, uint32_t* tota
 void sum_u32(tess_ctx* ctx, const uint32_t* v, int64_t n, uint32_t* out);
 // Stable LSD radix sort of (key, value) pairs on the low key_bits of key.
 // vals_in == nullptr means values 0..n-1.  Result lands in (keys_out, vals_out).
+// With n_dev the element count is read on the device (at most n, which sizes
+// the launches), so the sort can be captured in a graph.
 void radix_sort_pairs(tess_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                      uint32_t* vals_out, int64_t n, int key_bits);
+                      uint32_t* vals_out, int64_t n, int key_bits, const uint32_t* n_dev = nullptr);
 // The same sort over caller-provided work buffers on a stream (no allocation):
 // hist holds sort_hist_words(n) words, keys_alt/vals_alt n words each, scan_tmp
 // scan_work_bytes(sort_hist_words(n)) bytes.
@@ -216,7 +222,7 @@ struct SortWork {
 size_t sort_hist_words(int64_t n);
 size_t scan_work_bytes(int64_t n);
 void radix_sort_pairs_ws(const SortWork& w, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                         uint32_t* vals_out, int64_t n, int key_bits);
+                         uint32_t* vals_out, int64_t n, int key_bits, const uint32_t* n_dev = nullptr);
 
 // ---- reference-layout weights and partition (partition.hip) ----
 void launch_update_weights(tess_ctx* ctx, int basis, const tess_mesh& mesh, const tess_particles& P,

@@ -448,12 +448,14 @@ __device__ __forceinline__ int64_t bound_key(const uint32_t* k, int64_t n, uint3
 // Sorted mover i (key K, previous position j) lands after every stayer with a
 // smaller key or an equal key and smaller previous position, and after the
 // movers before it.
+// (M_dev, when set: the mover count read on the device, at most M.)
 __global__ void k_place_movers(const uint32_t* __restrict__ mks, const uint32_t* __restrict__ mr,
                                const uint32_t* __restrict__ mj, const uint32_t* __restrict__ skeys_old,
                                const uint32_t* __restrict__ perm_old, const uint32_t* __restrict__ spos, int64_t n,
                                int64_t M, uint32_t* __restrict__ mjs, uint32_t* __restrict__ skeys_new,
-                               uint32_t* __restrict__ perm_new) {
+                               uint32_t* __restrict__ perm_new, const uint32_t* __restrict__ M_dev) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (M_dev) M = int64_t(*M_dev);
   if (i >= M) return;
   const uint32_t K = mks[i];
   const int64_t j = mj[mr[i]];
@@ -473,9 +475,10 @@ constexpr int STAY_BLOCK = 256, STAY_CAP = 2048;
 // run in parallel instead of at the start of every workgroup).
 __global__ void k_stay_ranges(const uint32_t* __restrict__ skeys_old, int64_t n, const uint32_t* __restrict__ mks,
                               const uint32_t* __restrict__ mjs, int64_t M, int64_t nblocks,
-                              int64_t* __restrict__ ranges) {
+                              int64_t* __restrict__ ranges, const uint32_t* __restrict__ M_dev) {
   const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (b >= nblocks) return;
+  if (M_dev) M = int64_t(*M_dev);
   const int64_t j0 = b * STAY_BLOCK;
   const int64_t j1 = j0 + STAY_BLOCK < n ? j0 + STAY_BLOCK : n;
   const int64_t lo = lower_bound_pair(mks, mjs, 0, M, skeys_old[j0], uint32_t(j0));
@@ -1409,9 +1412,19 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uin
   if (j < a.n_active) k_p2g_fast_body<T, D, BASIS, NPASS, AOS, BATCH, SEP>(a, skeys, misplaced, j);
 }
 
-// Persistent form: a fixed grid walks the work items up to the count the
-// device holds at `count` (a multiple of 8 workgroups keeps each XCD on a
-// contiguous run of them, as xcd_remap does for one workgroup per item).
+// Forms whose item count is read on the device at `count` (launch sizes fixed
+// in advance): one workgroup per possible item, those past the count ending
+// at once (_grid), or a persistent grid sized to the device walking the items
+// (_persist; a multiple of 8 workgroups keeps each XCD on a contiguous run of
+// them, as xcd_remap does for one workgroup per item).
+template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false, bool SEP = false>
+__global__ __launch_bounds__(64 * NPASS) void k_p2g_fast_grid(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                          uint32_t* __restrict__ misplaced, const uint32_t* __restrict__ count) {
+  const int64_t N = int64_t(*count), R = ((N + 7) >> 3) << 3;
+  // xcd_remap is a bijection of [0, R) only: workgroups past R end here
+  const int64_t j = xcd_remap(blockIdx.x, N);
+  if (int64_t(blockIdx.x) < R && j < N) k_p2g_fast_body<T, D, BASIS, NPASS, AOS, BATCH, SEP>(a, skeys, misplaced, j);
+}
 template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false, bool SEP = false>
 __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast_persist(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                           uint32_t* __restrict__ misplaced, const uint32_t* __restrict__ count) {
@@ -1718,9 +1731,20 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
   if (j < a.n_active) k_g2p_fast_body<T, D, BASIS, AOS, NW, BATCH, SEP>(a, skeys, nk, stay, ntiles, movers, j);
 }
 
-// Persistent form: a fixed grid walks the work items up to the count the
-// device holds at `count` (a multiple of 8 workgroups keeps each XCD on a
-// contiguous run of them, as xcd_remap does for one workgroup per item).
+// Forms whose item count is read on the device at `count` (launch sizes fixed
+// in advance): one workgroup per possible item, those past the count ending
+// at once (_grid), or a persistent grid sized to the device walking the items
+// (_persist; a multiple of 8 workgroups keeps each XCD on a contiguous run of
+// them, as xcd_remap does for one workgroup per item).
+template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false, bool SEP = false>
+__global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD : 1) void k_g2p_fast_grid(Args<T, D> a, const uint32_t* __restrict__ skeys,
+                                                       uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
+                                                       uint32_t ntiles, uint32_t* __restrict__ movers, const uint32_t* __restrict__ count) {
+  const int64_t N = int64_t(*count), R = ((N + 7) >> 3) << 3;
+  // xcd_remap is a bijection of [0, R) only: workgroups past R end here
+  const int64_t j = xcd_remap(blockIdx.x, N);
+  if (int64_t(blockIdx.x) < R && j < N) k_g2p_fast_body<T, D, BASIS, AOS, NW, BATCH, SEP>(a, skeys, nk, stay, ntiles, movers, j);
+}
 template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false, bool SEP = false>
 __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD : 1) void k_g2p_fast_persist(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                        uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
@@ -1857,9 +1881,18 @@ __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __re
   if (j < a.n_nb) k_gather_grid_body<T, D, SW, GAOS, BATCH>(a, nodes_per_block, j);
 }
 
-// Persistent form: a fixed grid walks the work items up to the count the
-// device holds at `count` (a multiple of 8 workgroups keeps each XCD on a
-// contiguous run of them, as xcd_remap does for one workgroup per item).
+// Forms whose item count is read on the device at `count` (launch sizes fixed
+// in advance): one workgroup per possible item, those past the count ending
+// at once (_grid), or a persistent grid sized to the device walking the items
+// (_persist; a multiple of 8 workgroups keeps each XCD on a contiguous run of
+// them, as xcd_remap does for one workgroup per item).
+template <typename T, int D, int SW, bool GAOS, bool BATCH = false>
+__global__ __launch_bounds__(64) void k_gather_grid_grid(Args<T, D> a, uint32_t* __restrict__ nodes_per_block, const uint32_t* __restrict__ count) {
+  const int64_t N = int64_t(*count), R = ((N + 7) >> 3) << 3;
+  // xcd_remap is a bijection of [0, R) only: workgroups past R end here
+  const int64_t j = xcd_remap(blockIdx.x, N);
+  if (int64_t(blockIdx.x) < R && j < N) k_gather_grid_body<T, D, SW, GAOS, BATCH>(a, nodes_per_block, j);
+}
 template <typename T, int D, int SW, bool GAOS, bool BATCH = false>
 __global__ __launch_bounds__(64) void k_gather_grid_persist(Args<T, D> a, uint32_t* __restrict__ nodes_per_block, const uint32_t* __restrict__ count) {
   const int64_t N = int64_t(*count), R = ((N + 7) >> 3) << 3;
@@ -2189,12 +2222,12 @@ inline void merge_movers(tess_ctx* ctx, const uint32_t* nk, const uint32_t* spos
     TESS_HIP(hipGetLastError());
     radix_sort_pairs(ctx, mkey, nullptr, mks, mr, M, kbits);
     hipLaunchKernelGGL(k_place_movers, dim3(blocks_for(M, 256)), dim3(256), 0, ctx->stream, mks, mr, mj, skeys, perm,
-                       spos, n, M, mjs, skeys2, perm2);
+                       spos, n, M, mjs, skeys2, perm2, static_cast<const uint32_t*>(nullptr));
     TESS_HIP(hipGetLastError());
     const int64_t nsb = blocks_for(n, STAY_BLOCK);
     int64_t* ranges = ctx->inc_rng.as<int64_t>(size_t(2 * nsb));
     hipLaunchKernelGGL(k_stay_ranges, dim3(blocks_for(nsb, 256)), dim3(256), 0, ctx->stream, skeys, n, mks, mjs, M,
-                       nsb, ranges);
+                       nsb, ranges, static_cast<const uint32_t*>(nullptr));
     TESS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_place_stayers, dim3(unsigned(nsb)), dim3(STAY_BLOCK), 0, ctx->stream, skeys, perm, spos, n,
                        mks, mjs, ranges, skeys2, perm2);

@@ -940,8 +940,8 @@ class Batch:
 
     def set_variant(self, variant: int) -> None:
         """Launch variant to use (a number autotune returned): in 2D 0..5, in 3D 0 or 2, each
-        optionally with bit 3 (+8): graph mode, every step one replay of a captured hipGraph of
-        persistent launches with no host readback inside the step."""
+        optionally with bit 3 (+8): graph mode, every step one replay of a captured hipGraph with no
+        host readback inside the step (one workgroup per possible tile; +24: persistent grids)."""
         _check(self.ctx.lib, self.ctx.lib.tess_batch_set_variant(self.handle, int(variant)))
 
     def stats(self) -> dict:

@@ -24,8 +24,8 @@ def _run_and_check(oracle, scenes, basis, nsteps, tune=0, variant=None, calls=No
     if tune:
         v = batch.autotune(tune)
         eager = list(range(6)) if scenes[0].mesh.dim == 2 else [0, 2]
-        assert v in eager + [e | 8 for e in eager]
-        extra = 2 * len(eager) * (tune + 1)
+        assert v in eager + [e | 8 for e in eager] + [e | 24 for e in eager]
+        extra = 3 * len(eager) * (tune + 1)
     if variant is not None:
         batch.set_variant(variant)
     for k in calls or (nsteps,):
@@ -34,7 +34,8 @@ def _run_and_check(oracle, scenes, basis, nsteps, tune=0, variant=None, calls=No
     if variant is not None and variant & 8:
         st = batch.stats()
         print(f"[batch graph] variant={variant} {st}", flush=True)
-        assert st["captures"] == 1 and st["graph_replays"] == nsteps - 1
+        # each call's first step sorts every particle again outside the graphs
+        assert st["captures"] == 1 and st["graph_replays"] == nsteps - len(calls or (nsteps,))
     for sc, P in zip(scenes, gpu):
         st = HostState(sc)
         oracle.step(st, basis, sc.params, nsteps + extra)
@@ -67,11 +68,12 @@ def test_batch_set_variant_rejects_unknown_numbers():
     sc = random_box(3, 200, 1, th.BSpline(th.Quadratic()), dtype=torch.float64)
     mesh, P, G = to_gpu(sc)
     batch = th.Batch([(G, P, sc.params)], th.BSpline(th.Quadratic()))
-    for bad in (1, 3, 4, 6, 9, 16, -1):
+    for bad in (1, 3, 4, 6, 9, 16, 18, 32, -1):
         with pytest.raises(th.ArgumentError):
             batch.set_variant(bad)
     batch.set_variant(2)
     batch.set_variant(10)
+    batch.set_variant(26)
     batch.close()
 
 
@@ -134,11 +136,12 @@ def test_batch_reorder_matches_oracle(oracle, threshold):
     ctx.close()
 
 
-@pytest.mark.parametrize("variant", [8, 9, 10, 13])
+@pytest.mark.parametrize("variant", [8, 9, 10, 13, 24, 26, 29])
 def test_batch_graph_mode_matches_oracle(oracle, variant):
-    """Graph mode: one ordinary step sizes the buffers and the same step is captured; the rest are replays of
-    the captured hipGraph (persistent launches over device-side counts, every particle sorted again each
-    step).  Two calls, of two and three steps: the second call replays the graph captured by the first."""
+    """Graph mode: each call's first step sorts every particle; the rest are replays of the captured hipGraphs
+    (the binning merges the particles whose key changed into the previous order; launches over device-side
+    counts: one workgroup per possible tile, or persistent grids with bit 4).  Two calls, of two and three
+    steps: the second call replays the graphs captured by the first."""
     scenes = batched_scenes(7, seed=21, dtype=torch.float64, cells=(16, 64), particles=(100, 3000))
     _run_and_check(oracle, scenes, th.BSpline(th.Quadratic()), 5, variant=variant, calls=(2, 3))
 
@@ -148,6 +151,7 @@ def test_batch_graph_mode_3d_matches_oracle(oracle):
     scenes = [random_box(3, 700 + 400 * k, 80 + k, basis, dtype=torch.float64, h=0.1 / (k + 1), lo=0.05, hi=0.9,
                          stress=0.5) for k in range(3)]
     _run_and_check(oracle, scenes, basis, 4, variant=8)
+    _run_and_check(oracle, scenes, basis, 4, variant=26)
 
 
 def test_batch_switches_between_eager_and_graph_mode(oracle):
