@@ -238,6 +238,20 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t i, int64_t N) {
 }
 inline unsigned xcd_grid(int64_t N) { return unsigned(((N + 7) >> 3) << 3); }
 
+// Workgroup i -> work item of the tile kernels.  One scene: xcd_remap (each
+// XCD on a contiguous run of tiles, which share node blocks in its L2).  A
+// batch (TESS_BATCH_SPREAD): consecutive items, dealt round-robin to the XCDs,
+// since one scene's tiles carry loads that differ from the next scene's by
+// orders of magnitude and contiguous runs would leave XCDs with unequal work.
+#ifndef TESS_BATCH_SPREAD
+#define TESS_BATCH_SPREAD 1
+#endif
+template <bool BATCH>
+__device__ __forceinline__ int64_t item_of(int64_t i, int64_t N) {
+  if constexpr (BATCH && TESS_BATCH_SPREAD) return i;
+  return xcd_remap(i, N);
+}
+
 template <int D>
 __device__ __forceinline__ void unlin(int64_t l, const int32_t* dims, int* I) {
 #pragma unroll
@@ -1408,7 +1422,7 @@ __device__ __forceinline__ void k_p2g_fast_body(Args<T, D> a, const uint32_t* __
 template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false, bool SEP = false>
 __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                           uint32_t* __restrict__ misplaced) {
-  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
+  const int64_t j = item_of<BATCH>(blockIdx.x, a.n_active);
   if (j < a.n_active) k_p2g_fast_body<T, D, BASIS, NPASS, AOS, BATCH, SEP>(a, skeys, misplaced, j);
 }
 
@@ -1422,7 +1436,7 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast_grid(Args<T, D> a, cons
                                                           uint32_t* __restrict__ misplaced, const uint32_t* __restrict__ count) {
   const int64_t N = int64_t(*count), R = ((N + 7) >> 3) << 3;
   // xcd_remap is a bijection of [0, R) only: workgroups past R end here
-  const int64_t j = xcd_remap(blockIdx.x, N);
+  const int64_t j = item_of<BATCH>(blockIdx.x, N);
   if (int64_t(blockIdx.x) < R && j < N) k_p2g_fast_body<T, D, BASIS, NPASS, AOS, BATCH, SEP>(a, skeys, misplaced, j);
 }
 template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false, bool SEP = false>
@@ -1430,7 +1444,7 @@ __global__ __launch_bounds__(64 * NPASS) void k_p2g_fast_persist(Args<T, D> a, c
                                                           uint32_t* __restrict__ misplaced, const uint32_t* __restrict__ count) {
   const int64_t N = int64_t(*count), R = ((N + 7) >> 3) << 3;
   for (int64_t it = blockIdx.x; it < R; it += gridDim.x) {
-    const int64_t j = xcd_remap(it, N);
+    const int64_t j = item_of<BATCH>(it, N);
     if (j < N) k_p2g_fast_body<T, D, BASIS, NPASS, AOS, BATCH, SEP>(a, skeys, misplaced, j);
     __syncthreads();  // the next item reuses the workgroup's LDS
   }
@@ -1727,7 +1741,7 @@ template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false, bo
 __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD : 1) void k_g2p_fast(Args<T, D> a, const uint32_t* __restrict__ skeys,
                                                        uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
                                                        uint32_t ntiles, uint32_t* __restrict__ movers) {
-  const int64_t j = xcd_remap(blockIdx.x, a.n_active);
+  const int64_t j = item_of<BATCH>(blockIdx.x, a.n_active);
   if (j < a.n_active) k_g2p_fast_body<T, D, BASIS, AOS, NW, BATCH, SEP>(a, skeys, nk, stay, ntiles, movers, j);
 }
 
@@ -1742,7 +1756,7 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
                                                        uint32_t ntiles, uint32_t* __restrict__ movers, const uint32_t* __restrict__ count) {
   const int64_t N = int64_t(*count), R = ((N + 7) >> 3) << 3;
   // xcd_remap is a bijection of [0, R) only: workgroups past R end here
-  const int64_t j = xcd_remap(blockIdx.x, N);
+  const int64_t j = item_of<BATCH>(blockIdx.x, N);
   if (int64_t(blockIdx.x) < R && j < N) k_g2p_fast_body<T, D, BASIS, AOS, NW, BATCH, SEP>(a, skeys, nk, stay, ntiles, movers, j);
 }
 template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH = false, bool SEP = false>
@@ -1751,7 +1765,7 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
                                                        uint32_t ntiles, uint32_t* __restrict__ movers, const uint32_t* __restrict__ count) {
   const int64_t N = int64_t(*count), R = ((N + 7) >> 3) << 3;
   for (int64_t it = blockIdx.x; it < R; it += gridDim.x) {
-    const int64_t j = xcd_remap(it, N);
+    const int64_t j = item_of<BATCH>(it, N);
     if (j < N) k_g2p_fast_body<T, D, BASIS, AOS, NW, BATCH, SEP>(a, skeys, nk, stay, ntiles, movers, j);
     __syncthreads();  // the next item reuses the workgroup's LDS
   }
@@ -1877,7 +1891,7 @@ __device__ __forceinline__ void k_gather_grid_body(Args<T, D> a, uint32_t* __res
 
 template <typename T, int D, int SW, bool GAOS, bool BATCH = false>
 __global__ __launch_bounds__(64) void k_gather_grid(Args<T, D> a, uint32_t* __restrict__ nodes_per_block) {
-  const int64_t j = xcd_remap(blockIdx.x, a.n_nb);
+  const int64_t j = item_of<BATCH>(blockIdx.x, a.n_nb);
   if (j < a.n_nb) k_gather_grid_body<T, D, SW, GAOS, BATCH>(a, nodes_per_block, j);
 }
 
@@ -1890,14 +1904,14 @@ template <typename T, int D, int SW, bool GAOS, bool BATCH = false>
 __global__ __launch_bounds__(64) void k_gather_grid_grid(Args<T, D> a, uint32_t* __restrict__ nodes_per_block, const uint32_t* __restrict__ count) {
   const int64_t N = int64_t(*count), R = ((N + 7) >> 3) << 3;
   // xcd_remap is a bijection of [0, R) only: workgroups past R end here
-  const int64_t j = xcd_remap(blockIdx.x, N);
+  const int64_t j = item_of<BATCH>(blockIdx.x, N);
   if (int64_t(blockIdx.x) < R && j < N) k_gather_grid_body<T, D, SW, GAOS, BATCH>(a, nodes_per_block, j);
 }
 template <typename T, int D, int SW, bool GAOS, bool BATCH = false>
 __global__ __launch_bounds__(64) void k_gather_grid_persist(Args<T, D> a, uint32_t* __restrict__ nodes_per_block, const uint32_t* __restrict__ count) {
   const int64_t N = int64_t(*count), R = ((N + 7) >> 3) << 3;
   for (int64_t it = blockIdx.x; it < R; it += gridDim.x) {
-    const int64_t j = xcd_remap(it, N);
+    const int64_t j = item_of<BATCH>(it, N);
     if (j < N) k_gather_grid_body<T, D, SW, GAOS, BATCH>(a, nodes_per_block, j);
     __syncthreads();  // the next item reuses the workgroup's LDS
   }
