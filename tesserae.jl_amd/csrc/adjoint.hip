@@ -155,59 +155,101 @@ __global__ __launch_bounds__(64 * support_width(BASIS)) void k_adj_scatter(Args<
     for (int s = 0; s < NSL; ++s)
 #pragma unroll
       for (int v = 0; v < NA; ++v) acc[s][v] = 0.0;
-    for (int k = 0; k < maxcnt; ++k) {
-      if (k >= ccount) continue;
+    // the inputs of a lane's next particle are loaded while its current one
+    // is processed (the loads of one particle would otherwise stall each step)
+    struct In {
+      T x[D];
+      double vb[D], xb[D], aF[D * D];
+      T F[D * D];
+    };
+    auto load = [&](int k, In& in) {
       const int li = coff + k;
       const uint32_t pid = li < PID_CAP ? s_pid[li] : a.perm[start + li];
-      T x[D], l;
-      int lo[D], hi[D];
-      particle_window<T, D, BASIS, false>(a, pid, x, l, lo, hi, nullptr);
-      double N[D][SW], dN[D][SW];
-      window_jets2<T, D, BASIS, false>(a, x, lo, hi, N, dN, nullptr);
-      double vb[D], xb[D], Lb[D * D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) in.x[d] = a.x(pid, d);
 #pragma unroll
       for (int c = 0; c < D; ++c) {
-        vb[c] = av(pid, c);
-        xb[c] = dt * ax(pid, c);
+        in.vb[c] = av(pid, c);
+        in.xb[c] = dt * ax(pid, c);
       }
+#pragma unroll
+      for (int q = 0; q < D * D; ++q) {
+        in.aF[q] = aF(pid, q);
+        in.F[q] = a.F(pid, q);
+      }
+    };
+    In cur, nxt;
+    if (ccount > 0) load(0, cur);
+    for (int k = 0; k < maxcnt; ++k) {
+      if (k + 1 < ccount) load(k + 1, nxt);
+      if (k < ccount) {
+      const T* x = cur.x;
+      int lo[D], hi[D];  // the window (a binned particle lies inside the mesh)
+      {
+        T xi[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) xi[d] = (x[d] - a.mesh.xmin[d]) * a.mesh.h_inv;
+#pragma unroll
+        for (int d = 0; d < D; ++d) axis_window<T, BASIS>(xi[d], T(0), a.mesh.dims[d], lo[d], hi[d]);
+      }
+      double N[D][SW], dN[D][SW];
+      window_jets2<T, D, BASIS, false>(a, x, lo, hi, N, dN, nullptr);
+      const double* vb = cur.vb;
+      const double* xb = cur.xb;
+      double Lb[D * D];
 #pragma unroll
       for (int r = 0; r < D; ++r)
 #pragma unroll
         for (int c = 0; c < D; ++c) {
           double s = 0.0;
 #pragma unroll
-          for (int q = 0; q < D; ++q) s += aF(pid, r + D * q) * double(a.F(pid, c + D * q));
+          for (int q = 0; q < D; ++q) s += cur.aF[r + D * q] * double(cur.F[c + D * q]);
           Lb[r + D * c] = dt * s;  // L̄ = Δt F̄' Fᵀ
         }
+      // rows of this wave's offsets: the last axis at `pass`, the middle axis
+      // (3D) enumerated, the nodes along x inside a row.  With w = N_x R and
+      // ∇w = (N_x' R, N_x R_c) per node along x:  Δv̄ += N_x γ,
+      // v̄ += N_x α + N_x' β,  γ = R v̄',  α = R Δt x̄' + Σ_{c>0} L̄_{:c} R_c,  β = L̄_{:0} R
+      constexpr int NROW = NSL / SW;
 #pragma unroll
-      for (int s = 0; s < NSL; ++s) {
+      for (int row = 0; row < NROW; ++row) {
         int o[D];
-        int r = s;
-#pragma unroll
-        for (int d = 0; d < D - 1; ++d) {
-          o[d] = r % SW;
-          r /= SW;
-        }
+        o[0] = 0;
+        if constexpr (D == 3) o[1] = row;
         o[D - 1] = pass;
-        double w = 1.0, g[D];
+        double R = 1.0, Rc[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) w *= N[d][o[d]];
+        for (int d = 1; d < D; ++d) R *= N[d][o[d]];
 #pragma unroll
-        for (int c = 0; c < D; ++c) {
-          double gg = 1.0;
+        for (int c = 1; c < D; ++c) {
+          double t = dN[c][o[c]];
 #pragma unroll
-          for (int d = 0; d < D; ++d) gg *= (d == c) ? dN[d][o[d]] : N[d][o[d]];
-          g[c] = gg;
+          for (int d = 1; d < D; ++d)
+            if (d != c) t *= N[d][o[d]];
+          Rc[c] = t;
         }
+        double al[D], be[D], ga[D];
 #pragma unroll
         for (int r2 = 0; r2 < D; ++r2) {
-          acc[s][r2] += w * vb[r2];
-          double t = w * xb[r2];
+          ga[r2] = R * vb[r2];
+          double t = R * xb[r2];
 #pragma unroll
-          for (int c = 0; c < D; ++c) t += Lb[r2 + D * c] * g[c];
-          acc[s][D + r2] += t;
+          for (int c = 1; c < D; ++c) t += Lb[r2 + D * c] * Rc[c];
+          al[r2] = t;
+          be[r2] = Lb[r2] * R;
+        }
+#pragma unroll
+        for (int ox = 0; ox < SW; ++ox) {
+          const int sl = ox + SW * row;
+#pragma unroll
+          for (int r2 = 0; r2 < D; ++r2) {
+            acc[sl][r2] += N[0][ox] * ga[r2];
+            acc[sl][D + r2] += N[0][ox] * al[r2] + dN[0][ox] * be[r2];
+          }
         }
       }
+      }
+      cur = nxt;
     }
     // lanes own distinct classes: one add instruction never hits a node twice;
     // the waves add one after another, so the per-node order is fixed

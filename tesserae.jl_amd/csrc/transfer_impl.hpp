@@ -1149,7 +1149,7 @@ __device__ __forceinline__ void p2g_load(const Args<T, D>& a, uint32_t pid, P2GI
 // waves per SIMD resident).  The waves add their stencils into the LDS tile
 // one after another, so the per-node order of the additions is fixed.
 template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH, bool SEP>
-__device__ __forceinline__ void k_p2g_fast_body(Args<T, D> a, const uint32_t* __restrict__ skeys,
+__device__ __forceinline__ void k_p2g_fast_body(Args<T, D>& a, const uint32_t* __restrict__ skeys,
                                                           uint32_t* __restrict__ misplaced, const int64_t j) {
   using G = Geo<D>;
   using PT = Part<D, support_width(BASIS)>;
@@ -1486,7 +1486,7 @@ __device__ __forceinline__ void g2p_load(const Args<T, D>& a, uint32_t pid, G2PI
 #define TESS_G2P_WAVES_PER_TILE 4
 #endif
 template <typename T, int D, int BASIS, bool AOS, int NW, bool BATCH, bool SEP>
-__device__ __forceinline__ void k_g2p_fast_body(Args<T, D> a, const uint32_t* __restrict__ skeys,
+__device__ __forceinline__ void k_g2p_fast_body(Args<T, D>& a, const uint32_t* __restrict__ skeys,
                                                        uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
                                                        uint32_t ntiles, uint32_t* __restrict__ movers, const int64_t j) {
   using G = Geo<D>;
@@ -1774,7 +1774,7 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
 
 // ---- gather + grid-node half ------------------------------------------------
 template <typename T, int D, int SW, bool GAOS, bool BATCH>
-__device__ __forceinline__ void k_gather_grid_body(Args<T, D> a, uint32_t* __restrict__ nodes_per_block, const int64_t j) {
+__device__ __forceinline__ void k_gather_grid_body(Args<T, D>& a, uint32_t* __restrict__ nodes_per_block, const int64_t j) {
   using G = Geo<D>;
   using PT = Part<D, SW>;
   constexpr int NV = 1 + 2 * D;
