@@ -139,6 +139,9 @@ struct Args {
   const uint32_t* nb_list;
   const uint32_t* nb_flag;
   T* partials;
+  // when set, G2P also writes each particle's position before the update here
+  // (dense, D per particle): the positions the P2G half of a @G2P2G pass reads
+  T* x_keep;
   // SpGrid: block numbering (nullptr for a dense grid), block_size_log2, blocks per axis
   const int32_t* numbering;
   int32_t spL;
@@ -1689,6 +1692,10 @@ __device__ __forceinline__ void k_g2p_fast_body(Args<T, D>& a, const uint32_t* _
       xn[c] = cur.x[c] + X[c] * dt;
       st_g2p(fldw<T, D, AOS>(a.x, pid, c), xn[c]);
     }
+    if (a.x_keep) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) a.x_keep[int64_t(pid) * D + c] = cur.x[c];
+    }
     if (nk) {
       const uint32_t jj = slot(k);
       uint32_t nl = 1;  // the scene's tiles; in a batch its keys start at its first global tile
@@ -1929,6 +1936,10 @@ __device__ __forceinline__ void particle_update(const Args<T, D>& a, int64_t p, 
   for (int c = 0; c < D; ++c) a.v(p, c) += dv[c];
 #pragma unroll
   for (int k = 0; k < D * D; ++k) a.gradv(p, k) = L[k];
+  if (a.x_keep) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) a.x_keep[p * D + c] = a.x(p, c);
+  }
 #pragma unroll
   for (int c = 0; c < D; ++c) a.x(p, c) += dx[c];
   T Fo[D * D], M[D * D], Fn[D * D];
@@ -2110,15 +2121,6 @@ __global__ __launch_bounds__(256) void k_g2p_outside(Args<T, D> a, const uint32_
 #pragma unroll
   for (int k = 0; k < D * D; ++k) L[k] = T(0);
   particle_update<T, D>(a, pid, dv, L, dx);
-}
-
-// x of every particle into a dense (x, y[, z]) array
-template <typename T, int D>
-__global__ __launch_bounds__(256) void k_keep_positions(Args<T, D> a, T* __restrict__ xo) {
-  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= a.n) return;
-#pragma unroll
-  for (int d = 0; d < D; ++d) xo[p * D + d] = a.x(p, d);
 }
 
 // ---- host side ----------------------------------------------------------------
@@ -2530,9 +2532,11 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
 
 template <typename T, int D, int BASIS, bool STORED>
 void g2p_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-                const tess_weights* W, const tess_params& prm, const tess_sparsity* sp, bool write_keys = false) {
+                const tess_weights* W, const tess_params& prm, const tess_sparsity* sp, bool write_keys = false,
+                T* x_keep = nullptr) {
   const Binning& b = ctx->bin;
   Args<T, D> a = make_args<T, D>(mesh, &G, P, W, &prm);
+  a.x_keep = x_keep;
   fill_binning_ptrs<T, D>(ctx, a);
   set_sparse<T, D>(a, mesh, sp);
   set_correction<T, D, STORED>(ctx, a);
@@ -2693,21 +2697,21 @@ void g2p_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const te
 // weights, at the positions before the pass, serves the G2P half and the P2G
 // half.  One binning at those positions orders both halves: the G2P kernel
 // updates the particles, then the P2G kernels scatter the updated v, V, σ with
-// the weights of the old positions (stored windows, or the positions kept
-// aside before the G2P half moved them), and the grid-node half follows.
+// the weights of the old positions (stored windows, or the positions the G2P
+// kernel writes aside as it moves the particles), and the grid-node half follows.
 template <typename T, int D, int BASIS, bool STORED>
 void g2p2g_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
                 const tess_weights* W, const tess_params& prm) {
   bin_particles<T, D, BASIS, STORED>(ctx, mesh, P, W);
+  // the G2P half keeps each particle's position before its update aside
+  // (deferred weights), for the P2G half's weights
   tess_particles Pold = P;
+  T* xo = nullptr;
   if (!STORED && P.n > 0) {
-    T* xo = ctx->g2p2g_x.as<T>(size_t(P.n) * D);
-    Args<T, D> a = make_args<T, D>(mesh, nullptr, P, nullptr, nullptr);
-    hipLaunchKernelGGL((k_keep_positions<T, D>), dim3(blocks_for(P.n, 256)), dim3(256), 0, ctx->stream, a, xo);
-    TESS_HIP(hipGetLastError());
+    xo = ctx->g2p2g_x.as<T>(size_t(P.n) * D);
     Pold.x = tess_field{xo, D, 1};
   }
-  g2p_launch<T, D, BASIS, STORED>(ctx, mesh, G, P, W, prm, nullptr);
+  g2p_launch<T, D, BASIS, STORED>(ctx, mesh, G, P, W, prm, nullptr, false, xo);
   ctx->bin.valid = true;  // the binning describes the old positions the P2G half reads
   p2g_launch<T, D, BASIS, STORED>(ctx, mesh, G, Pold, W, prm, false, nullptr);
   ctx->bin.valid = false;  // the particles have moved
@@ -2736,14 +2740,13 @@ void g2p2g_steps_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, 
       }
     }
     tess_particles Pold = P;
+    T* xo = nullptr;
     if (P.n > 0) {
-      T* xo = ctx->g2p2g_x.as<T>(size_t(P.n) * D);
-      Args<T, D> a = make_args<T, D>(mesh, nullptr, P, nullptr, nullptr);
-      hipLaunchKernelGGL((k_keep_positions<T, D>), dim3(blocks_for(P.n, 256)), dim3(256), 0, ctx->stream, a, xo);
-      TESS_HIP(hipGetLastError());
+      xo = ctx->g2p2g_x.as<T>(size_t(P.n) * D);
       Pold.x = tess_field{xo, D, 1};
     }
-    g2p_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, nullptr, s + 1 < nsteps);
+    // the G2P half writes the positions before its update to xo
+    g2p_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, nullptr, s + 1 < nsteps, xo);
     ctx->bin.valid = true;  // the binning describes the old positions the P2G half reads
     p2g_launch<T, D, BASIS, false>(ctx, mesh, G, Pold, nullptr, prm, reorder_threshold > 0.0, nullptr);
     ctx->bin.valid = false;
