@@ -979,20 +979,80 @@ void copy_state(tess_ctx* ctx, const tess_particles& src, const tess_particles& 
   }
 }
 
+// The pullback's kept layout: whether two segments of snapshots fit the 8 GiB
+// budget, the words of one snapshot, and the state slots of one buffer set
+// (every + 1: the last one the replay's working state).
+struct KeptLayout {
+  bool kept;
+  int64_t snap_words, seg_set;
+};
+template <typename T, int D>
+KeptLayout kept_layout(const tess_mesh& mesh, int64_t n, int every) {
+  constexpr int64_t kSnapBytes = int64_t(8) << 30;
+  int64_t ntiles = 1;
+  for (int d = 0; d < D; ++d) ntiles *= ((mesh.dims[d] - 1) >> Geo<D>::LT) + 1;
+  KeptLayout k;
+  k.snap_words = snapshot_words<T, D>(n, ntiles);
+  k.kept = k.snap_words * 4 * int64_t(every) * 2 <= kSnapBytes;
+  k.seg_set = state_scalars<T, D>() * n * (every + 1);
+  return k;
+}
+
 template <typename T, int D, int BASIS>
 void rollout_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
                   const tess_params& prm, int nsteps, int every) {
   Rollout& ro = ctx->rollout;
   ro.valid = false;
+  ro.last_kept = false;
   const int64_t n = P.n;
   const int ncheck = (nsteps + every - 1) / every;
   const int64_t per = state_scalars<T, D>() * n;
   T* ck = ctx->ro_checkpoints.as<T>(size_t(std::max<int64_t>(per * ncheck, 1)));
   ro.logE = P.logE;
+  // The last segment is the first the pullback reverses: its states, binnings
+  // and P2G grids are kept as this loop makes them, in the slots of the
+  // pullback's buffer set for that segment, so that the pullback starts
+  // reversing at once instead of replaying it alone.
+  const KeptLayout kl = kept_layout<T, D>(mesh, n, every);
+  const int last = ncheck - 1, lset = last & 1;
+  T* seg = nullptr;
+  uint32_t* snaps = nullptr;
+  if (kl.kept && ncheck > 0) {
+    seg = ctx->ro_segment.as<T>(size_t(std::max<int64_t>(2 * kl.seg_set, 1)));
+    snaps = ctx->ro_snapshots.as<uint32_t>(size_t(kl.snap_words) * size_t(every) * 2);
+    ro.last_bins.assign(size_t(every), Binning{});
+  }
   // one step loop per segment between checkpoints
   for (int c = 0; c < ncheck; ++c) {
+    const int len = std::min(every, nsteps - c * every);
     copy_state<T, D>(ctx, P, state_view<T, D>(ck + per * c, n, ro, nullptr));
-    step_impl<T, D, BASIS, false>(ctx, mesh, G, P, prm, std::min(every, nsteps - c * every), 0.0, nullptr);
+    if (c < last || !seg) {
+      step_impl<T, D, BASIS, false>(ctx, mesh, G, P, prm, len, 0.0, nullptr);
+      continue;
+    }
+    // the last segment runs on the set's working slot (packed, as the
+    // pullback's replays do); after each step's P2G, which leaves the
+    // particles as at the step's start, one launch copies the state into the
+    // step's slot and the binning and grid aside.  The final state goes back
+    // to the caller's arrays; ∇v is written to them directly.
+    T* sg = seg + kl.seg_set * lset;
+    T* work = sg + per * every;
+    copy_state<T, D>(ctx, P, state_view<T, D>(work, n, ro, nullptr));
+    tess_particles W = state_view<T, D>(work, n, ro, nullptr);
+    W.grad_v = P.grad_v;
+    std::function<void(int, int)> hook = [&](int phase, int s) {
+      if (phase == 0) return;
+      snapshot_copy<T, D>(ctx, mesh, G, n, ctx->bin.ntiles, snaps + kl.snap_words * (int64_t(every) * lset + s),
+                          true, work, sg + per * s, per * int64_t(sizeof(T)) / 4);
+      ro.last_bins[size_t(s)] = ctx->bin;
+    };
+    step_impl<T, D, BASIS, false>(ctx, mesh, G, W, prm, len, 0.0, nullptr, nullptr, &hook);
+    copy_state<T, D>(ctx, W, P);
+  }
+  if (seg) {
+    ro.last_kept = true;
+    ro.last_seg = seg;
+    ro.last_snaps = snaps;
   }
   ro.valid = true;
   ro.nsteps = nsteps;
@@ -1052,10 +1112,10 @@ void rollout_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid&
   // use separate state, snapshot and scratch buffers (two sets, alternating by
   // segment), ordered by events.  When two segments of snapshots exceed 8 GiB,
   // each reversed step bins and scatters again instead, on this stream alone.
-  constexpr int64_t kSnapBytes = int64_t(8) << 30;
-  const int64_t snap_words = snapshot_words<T, D>(n, ntiles);
+  const KeptLayout kl = kept_layout<T, D>(mesh, n, ro.every);
+  const int64_t snap_words = kl.snap_words;
   const int every = ro.every;
-  const bool kept = snap_words * 4 * int64_t(every) * 2 <= kSnapBytes;
+  const bool kept = kl.kept;
   if (!kept) {
     T* seg = ctx->ro_segment.as<T>(size_t(std::max<int64_t>(per * every, 1)));
     T* gradv = ctx->ro_gradv.as<T>(size_t(std::max<int64_t>(n * D * D, 1)));
@@ -1077,7 +1137,7 @@ void rollout_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid&
   tess_ctx* rc = replay_context(ctx);
   // per set: every + 1 state slots (the last one a scratch copy stepped for the
   // segment's last binning), every snapshots
-  const int64_t seg_set = per * (every + 1);
+  const int64_t seg_set = kl.seg_set;
   T* seg = ctx->ro_segment.as<T>(size_t(std::max<int64_t>(2 * seg_set, 1)));
   T* gradv = ctx->ro_gradv.as<T>(size_t(std::max<int64_t>(2 * n * D * D, 1)));  // replay, reverse
   uint32_t* snaps = ctx->ro_snapshots.as<uint32_t>(size_t(snap_words) * size_t(every) * 2);
@@ -1153,7 +1213,18 @@ void rollout_adjoint_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid&
                                   nullptr, &hook);
     TESS_HIP(hipEventRecord(ev.e[set], rc->stream));
   };
-  replay(ncheck - 1);
+  // the last segment as the rollout ran it, when it kept it in these buffers;
+  // else its replay runs first, alone
+  const bool last_from_rollout = ro.last_kept && ro.last_seg == seg && ro.last_snaps == snaps &&
+                                 ro.last_bins.size() == size_t(every);
+  if (last_from_rollout) {
+    bins[(ncheck - 1) & 1] = ro.last_bins;
+    TESS_HIP(hipEventRecord(ev.e[(ncheck - 1) & 1], ctx->stream));
+  } else {
+    replay(ncheck - 1);
+  }
+  // the pullback's replays overwrite those slots
+  ro.last_kept = false;
   for (int c = ncheck - 1; c >= 0; --c) {
     const int set = c & 1;
     TESS_HIP(hipStreamWaitEvent(ctx->stream, ev.e[set], 0));

@@ -85,6 +85,13 @@ struct Rollout {
   int32_t dim = 0, dtype = 0, basis = 0;
   int64_t dims[3] = {0, 0, 0};
   tess_field logE{nullptr, 0, 0};
+  // the last segment's states, binnings and grids as the rollout ran it, in
+  // the pullback's slots (so the pullback need not replay that segment), and
+  // the buffers they were written to
+  bool last_kept = false;
+  std::vector<Binning> last_bins;
+  const void* last_seg = nullptr;
+  const void* last_snaps = nullptr;
 };
 
 }  // namespace tess

@@ -145,6 +145,30 @@ def test_rollout_adjoint_checkpointing_and_determinism():
         assert rel_err(res[3][k], res[1][k]) <= 1e-12
 
 
+def test_rollout_keeps_last_segment_for_the_first_pullback():
+    """The rollout keeps its last segment's states, binnings and grids for the pullback, which then reverses
+    that segment without replaying it; a second pullback of the same rollout replays it.  Both give the same
+    gradients (Float64, to rounding: the replay may order ties differently), with a last segment shorter than
+    the checkpoint spacing (30 steps, checkpoints every 7)."""
+    sc = inverse_cube(dtype=torch.float64, dx=1 / 32)
+    target = sc.x + np.array([0.1, 0.0, 0.0])
+    ctx = th.Context()
+    mesh, P, G = to_gpu(sc)
+    th.rollout(G, P, sc.basis, sc.params, 30, checkpoint_every=7, ctx=ctx)
+    x = P.get("x").double()
+    t = torch.from_numpy(target).cuda()
+    res = []
+    for _ in range(2):
+        adj = th.Adjoint(sc.n, 3)
+        adj.x.copy_(2.0 * (x - t) / sc.n)
+        th.rollout_adjoint(G, sc.basis, sc.params, adj, ctx=ctx)
+        torch.cuda.synchronize()
+        res.append((adj.v.cpu().numpy(), adj.logE.cpu().numpy(), adj.x.cpu().numpy()))
+    for k in range(3):
+        assert rel_err(res[0][k], res[1][k]) <= 1e-12
+    ctx.close()
+
+
 def test_adjoint_errors():
     sc = inverse_cube(dtype=torch.float64, dx=1 / 16)
     mesh, P, G = to_gpu(sc)
