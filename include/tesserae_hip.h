@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define TESS_ABI_VERSION 3
+#define TESS_ABI_VERSION 4
 
 typedef int32_t tess_status;
 enum {
@@ -159,6 +159,15 @@ tess_status tess_sync(tess_ctx* ctx);
 /* Message of the last failed call on this host thread ("" if none). */
 const char* tess_last_error(void);
 int32_t tess_abi_version(void);
+/* Failure injection for tests of the error paths (no counterpart in the
+ * reference, whose tests inject `error("boom")` into equations and phases,
+ * test/transfer.jl:587-604, test/parallel_region.jl:76-170): the n-th HIP
+ * call the library checks from now on, in this process, reports
+ * TESS_ERR_DEVICE ("injected failure") after the call itself went through; 0
+ * turns it off, negative n is TESS_ERR_ARGUMENT.  Any failed call makes every
+ * context and batch rebuild its cached binning and zero its grids on its next
+ * call, so a failure never leaves a later call on stale state. */
+tess_status tess_debug_fail_after(int64_t n);
 
 /* ---- basis weights ------------------------------------------------------ */
 

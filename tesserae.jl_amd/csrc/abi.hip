@@ -4,6 +4,7 @@
 // a thread-local message.  No exception crosses this boundary.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
@@ -16,13 +17,27 @@ namespace tess {
 
 namespace {
 thread_local std::string g_last_error;
-}
+// failure injection (tess_debug_fail_after): the n-th checked HIP call from
+// now on reports a failure; 0 is off
+std::atomic<int64_t> g_fail_after{0};
+// bumped by every call that returns an error: contexts and batches that saw
+// an older value drop their cached binnings and zeroing bookkeeping
+std::atomic<uint64_t> g_error_epoch{0};
+}  // namespace
 
 void raise(tess_status code, const std::string& msg) { throw Error{code, msg}; }
 
 void check_hip(hipError_t e, const char* what) {
   if (e != hipSuccess) raise(TESS_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  if (g_fail_after.load(std::memory_order_relaxed) > 0 && g_fail_after.fetch_sub(1) == 1)
+    raise(TESS_ERR_DEVICE, std::string(what) + ": injected failure");
 }
+
+void check_hip_exact(hipError_t e, const char* what) {
+  if (e != hipSuccess) raise(TESS_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+uint64_t error_epoch() { return g_error_epoch.load(std::memory_order_relaxed); }
 
 void* Buffer::get(size_t need) {
   if (need == 0) need = 1;
@@ -30,13 +45,19 @@ void* Buffer::get(size_t need) {
   if (ptr) {
     // the previous contents are dropped: callers treat scratch as uninitialised
     TESS_HIP(hipDeviceSynchronize());
-    TESS_HIP(hipFree(ptr));
-    ptr = nullptr;
+    const hipError_t e = hipFree(ptr);
+    ptr = nullptr;  // forgotten before any error is raised
     bytes = 0;
+    TESS_HIP(e);
   }
   const size_t cap = need + need / 8;
-  TESS_HIP(hipMalloc(&ptr, cap));
-  bytes = cap;
+  void* p = nullptr;
+  const hipError_t e = hipMalloc(&p, cap);
+  if (e == hipSuccess) {
+    ptr = p;
+    bytes = cap;
+  }
+  TESS_HIP(e);
   return ptr;
 }
 
@@ -59,9 +80,10 @@ int prof_begin(tess_ctx* ctx, const char* name, double bytes) {
   P.entries[e].bytes += bytes;
   P.entries[e].calls += 1;
   while (P.pool.size() < P.next_event + 2) {
-    hipEvent_t ev;
-    TESS_HIP(hipEventCreate(&ev));
-    P.pool.push_back(ev);
+    hipEvent_t ev = nullptr;
+    const hipError_t err = hipEventCreate(&ev);
+    if (err == hipSuccess) P.pool.push_back(ev);  // owned before any error is raised
+    TESS_HIP(err);
   }
   hipEvent_t a = P.pool[P.next_event], b = P.pool[P.next_event + 1];
   P.next_event += 2;
@@ -98,19 +120,46 @@ tess_status guarded(F&& f) {
     return TESS_OK;
   } catch (const Error& e) {
     g_last_error = e.msg;
+    g_error_epoch.fetch_add(1);
     return e.code;
   } catch (const std::bad_alloc&) {
     g_last_error = "host allocation failed";
+    g_error_epoch.fetch_add(1);
     return TESS_ERR_DEVICE;
   } catch (...) {
     g_last_error = "unexpected internal error";
+    g_error_epoch.fetch_add(1);
     return TESS_ERR_STATE;
   }
 }
 
-void check_ctx(const tess_ctx* ctx) {
-  if (!ctx) raise(TESS_ERR_ARGUMENT, "context is NULL");
-  TESS_HIP(hipSetDevice(ctx->device));
+// A context's caches after a failed call somewhere: the binning, the grid's
+// zeroing bookkeeping and the rollout's kept segment may describe work that a
+// failure cut short, so the next call rebuilds them (full binning, whole-grid
+// zero), after its streams have drained.
+void ctx_drop_caches(tess_ctx* c) {
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  else (void)hipDeviceSynchronize();
+  c->bin.valid = false;
+  c->bin.state_valid = false;
+  c->bin.keys_ready = false;
+  c->last_grid_m = nullptr;
+  c->prev_nodeblocks = 0;
+  c->misplaced_x_ptr = nullptr;
+  c->last_misplaced = -1;
+  c->rollout.last_kept = false;
+}
+
+void check_ctx(const tess_ctx* cctx) {
+  if (!cctx) raise(TESS_ERR_ARGUMENT, "context is NULL");
+  TESS_HIP(hipSetDevice(cctx->device));
+  tess_ctx* ctx = const_cast<tess_ctx*>(cctx);
+  const uint64_t ep = error_epoch();
+  if (ctx->seen_epoch != ep) {
+    ctx_drop_caches(ctx);
+    if (ctx->replay) ctx_drop_caches(ctx->replay);
+    ctx->seen_epoch = ep;
+  }
 }
 
 void check_mesh(const tess_mesh* m) {
@@ -210,6 +259,7 @@ tess_ctx* ctx_new(int device, hipStream_t stream) {
   tess_ctx* c = new tess_ctx();
   c->device = device;
   c->stream = stream;
+  c->seen_epoch = error_epoch();
   void* pinned = nullptr;
   // pinned readback slots: counters (words 0-63), mover counts (words 64-319)
   hipError_t e = hipHostMalloc(&pinned, 512 * sizeof(int64_t), hipHostMallocDefault);
@@ -262,6 +312,13 @@ extern "C" {
 int32_t tess_abi_version(void) { return TESS_ABI_VERSION; }
 
 const char* tess_last_error(void) { return g_last_error.c_str(); }
+
+tess_status tess_debug_fail_after(int64_t n) {
+  return guarded([&] {
+    if (n < 0) raise(TESS_ERR_ARGUMENT, "n must be non-negative");
+    g_fail_after.store(n);
+  });
+}
 
 tess_status tess_ctx_create(int32_t device, void* stream, tess_ctx** out) {
   return guarded([&] {

@@ -1079,11 +1079,12 @@ void reserve_reverse_buffers(tess_ctx* ctx, int64_t ntiles, int64_t nnodes) {
 tess_ctx* replay_context(tess_ctx* ctx) {
   if (!ctx->replay) {
     hipStream_t rs = nullptr;
-    TESS_HIP(hipStreamCreateWithFlags(&rs, hipStreamNonBlocking));
+    const hipError_t err = hipStreamCreateWithFlags(&rs, hipStreamNonBlocking);
     try {
+      TESS_HIP(err);
       ctx->replay = ctx_new(ctx->device, rs);
     } catch (...) {
-      (void)hipStreamDestroy(rs);
+      if (rs) (void)hipStreamDestroy(rs);
       throw;
     }
   }
@@ -1268,7 +1269,15 @@ struct RolloutAdjointOp {
   static void run(tess_ctx* ctx, const tess_mesh& m, const tess_grid& G, const tess_params& prm,
                   const tess_adjoint& adj) {
     if constexpr (B == TESS_UGIMP || S) raise(TESS_ERR_UNSUPPORTED, "the adjoint supports the B-spline bases");
-    else rollout_adjoint_impl<T, D, B>(ctx, m, G, prm, adj);
+    else {
+      try {
+        rollout_adjoint_impl<T, D, B>(ctx, m, G, prm, adj);
+      } catch (...) {
+        // nothing of a failed pullback keeps running on the replay's stream
+        if (ctx->replay) (void)hipStreamSynchronize(ctx->replay->stream);
+        throw;
+      }
+    }
   }
 };
 

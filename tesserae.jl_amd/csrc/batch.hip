@@ -812,16 +812,23 @@ void graph_reorder_check(tess_ctx* g, tess_batch& B, int cur) {
 template <typename T, int D, int BASIS>
 void batch_graph_run(tess_ctx* ctx, tess_batch& B, int nsteps) {
   if (nsteps <= 0) return;
+  for (auto& e : B.graph_ev)
+    if (!e) {
+      hipEvent_t ev = nullptr;
+      const hipError_t err = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (err == hipSuccess) e = ev;  // kept before any error is raised
+      TESS_HIP(err);
+    }
   if (!B.gctx) {
     hipStream_t gs = nullptr;
-    TESS_HIP(hipStreamCreateWithFlags(&gs, hipStreamNonBlocking));
+    const hipError_t err = hipStreamCreateWithFlags(&gs, hipStreamNonBlocking);
     try {
+      TESS_HIP(err);
       B.gctx = ctx_new(ctx->device, gs);
     } catch (...) {
-      (void)hipStreamDestroy(gs);
+      if (gs) (void)hipStreamDestroy(gs);
       throw;
     }
-    for (auto& e : B.graph_ev) TESS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   tess_ctx* g = B.gctx;
   hipStream_t gs = g->stream;
@@ -852,7 +859,7 @@ void batch_graph_run(tess_ctx* ctx, tess_batch& B, int nsteps) {
       graph_release(B);
       TESS_HIP(hipStreamSynchronize(gs));
       for (int p = 0; p < 2; ++p) {
-        TESS_HIP(hipStreamBeginCapture(gs, hipStreamCaptureModeRelaxed));
+        TESS_HIP_EXACT(hipStreamBeginCapture(gs, hipStreamCaptureModeRelaxed));
         try {
           batch_graph_step<T, D, BASIS>(g, B, false, p);
         } catch (...) {
@@ -862,7 +869,7 @@ void batch_graph_run(tess_ctx* ctx, tess_batch& B, int nsteps) {
           graph_release(B);
           throw;
         }
-        TESS_HIP(hipStreamEndCapture(gs, &B.graph[p]));
+        TESS_HIP_EXACT(hipStreamEndCapture(gs, &B.graph[p]));
         TESS_HIP(hipGraphInstantiate(&B.graph_exec[p], B.graph[p], nullptr, nullptr, 0));
       }
       B.graph_deps = graph_buffers(g, B);
@@ -905,7 +912,16 @@ template <typename T, int D, int B_, bool S>
 struct BatchStepOp {
   static void run(tess_ctx* ctx, const tess_mesh&, tess_batch& b, int nsteps) {
     if constexpr (S) raise(TESS_ERR_UNSUPPORTED, "batches evaluate the basis inside the transfers");
-    else if (b.variant & 8) batch_graph_run<T, D, B_>(ctx, b, nsteps);
+    else if (b.variant & 8) {
+      try {
+        batch_graph_run<T, D, B_>(ctx, b, nsteps);
+      } catch (...) {
+        // a failed call leaves nothing of its own running on the batch's
+        // stream: the caller may reuse the scene arrays at once
+        if (b.gctx) (void)hipStreamSynchronize(b.gctx->stream);
+        throw;
+      }
+    }
     else batch_step_impl<T, D, B_>(ctx, b, nsteps);
   }
 };
@@ -929,6 +945,7 @@ bool julia_fields(const tess_scene& sc, int D) {
 
 void run_batch_create(tess_ctx* ctx, int basis, int nscenes, const tess_scene* scenes, tess_batch** out) {
   auto* b = new tess_batch();
+  b->seen_epoch = error_epoch();
   try {
     b->basis = basis;
     b->nscenes = nscenes;
@@ -969,6 +986,23 @@ void run_batch_create(tess_ctx* ctx, int basis, int nscenes, const tess_scene* s
 }
 
 void run_batch_step(tess_ctx* ctx, tess_batch* b, int nsteps) {
+  const uint64_t ep = error_epoch();
+  if (b->seen_epoch != ep) {
+    // a call failed since this batch's last step: its binning, zeroing lists
+    // and graph-mode device state may describe work cut short, so the next
+    // step zeroes every grid and bins from scratch in either mode
+    if (b->gctx) {
+      (void)hipStreamSynchronize(b->gctx->stream);
+      b->gctx->bin.valid = false;
+      b->gctx->bin.state_valid = false;
+    }
+    b->zeroed = false;
+    b->binned = false;
+    b->keys_ready = false;
+    b->counted = false;
+    b->last_graph = false;
+    b->seen_epoch = ep;
+  }
   dispatch<BatchStepOp>(b->basis, b->scenes[0].mesh, false, ctx, b->scenes[0].mesh, *b, nsteps);
 }
 

@@ -24,6 +24,11 @@ struct Error {
 [[noreturn]] void raise(tess_status code, const std::string& msg);
 void check_hip(hipError_t e, const char* what);
 #define TESS_HIP(call) ::tess::check_hip((call), #call)
+// the same check, never subject to failure injection (stream-capture control)
+void check_hip_exact(hipError_t e, const char* what);
+#define TESS_HIP_EXACT(call) ::tess::check_hip_exact((call), #call)
+// incremented by every C-ABI call that fails (see check_ctx)
+uint64_t error_epoch();
 
 // A device buffer owned by the context; grows on demand, never shrinks.
 struct Buffer {
@@ -146,6 +151,8 @@ struct tess_ctx {
   // scratch, private grid ro_grid) while this context reverses the previous one
   tess_ctx* replay = nullptr;
   tess::Buffer ro_grid;
+  // the failure epoch this context's caches belong to
+  uint64_t seen_epoch = 0;
 };
 
 namespace tess {
@@ -198,6 +205,8 @@ struct tess_batch {
   // replays between two reorder tests (each test bins on the host side)
   int graph_chunk = 16;
   int64_t graph_replays = 0, captures = 0;
+  // the failure epoch the batch's caches belong to
+  uint64_t seen_epoch = 0;
 };
 
 namespace tess {

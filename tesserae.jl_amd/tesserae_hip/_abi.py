@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes
 import os
 
-TESS_ABI_VERSION = 3  # tess_step_sparse reports the steps it ran (v2: tess_weights carries a node filter)
+TESS_ABI_VERSION = 4  # tess_debug_fail_after (v3: tess_step_sparse reports the steps it ran; v2: weight filter)
 
 TESS_OK = 0
 TESS_ERR_ARGUMENT = 1
@@ -158,6 +158,7 @@ SIGNATURES = {
     "tess_ply_write": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_void_p]),
     "tess_abi_version": (ctypes.c_int32, []),
     "tess_last_error": (ctypes.c_char_p, []),
+    "tess_debug_fail_after": (ctypes.c_int32, [ctypes.c_int64]),
     "tess_ctx_create": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_void_p, _P(ctypes.c_void_p)]),
     "tess_ctx_destroy": (ctypes.c_int32, [ctypes.c_void_p]),
     "tess_sync": (ctypes.c_int32, [ctypes.c_void_p]),

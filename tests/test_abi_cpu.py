@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_loads_and_reports_abi_version():
     lib = _abi.load_library()
-    assert lib.tess_abi_version() == _abi.TESS_ABI_VERSION == 3
+    assert lib.tess_abi_version() == _abi.TESS_ABI_VERSION == 4
     assert lib.tess_last_error() == b""
 
 
