@@ -168,6 +168,13 @@ int32_t tess_abi_version(void);
  * context and batch rebuild its cached binning and zero its grids on its next
  * call, so a failure never leaves a later call on stale state. */
 tess_status tess_debug_fail_after(int64_t n);
+/* How this build was made: TESS_BUILD_DEBUG_CHECKS for the debug build
+ * (`make debug`, libtesserae_hip_debug.so), whose every binning is read back
+ * and its invariants checked on the host (permutation, sorted keys, tile runs,
+ * active-tile list), failing with TESS_ERR_STATE; the counterpart of the
+ * reference's debug_mode preference (src/Core/debug.jl:5-15). */
+#define TESS_BUILD_DEBUG_CHECKS 1
+int32_t tess_build_flags(void);
 
 /* ---- basis weights ------------------------------------------------------ */
 

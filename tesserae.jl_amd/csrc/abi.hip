@@ -313,6 +313,11 @@ int32_t tess_abi_version(void) { return TESS_ABI_VERSION; }
 
 const char* tess_last_error(void) { return g_last_error.c_str(); }
 
+#ifndef TESS_DEBUG
+#define TESS_DEBUG 0
+#endif
+int32_t tess_build_flags(void) { return TESS_DEBUG ? TESS_BUILD_DEBUG_CHECKS : 0; }
+
 tess_status tess_debug_fail_after(int64_t n) {
   return guarded([&] {
     if (n < 0) raise(TESS_ERR_ARGUMENT, "n must be non-negative");

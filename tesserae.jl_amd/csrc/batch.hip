@@ -81,7 +81,23 @@ Args<T, D> batch_args(tess_ctx* ctx, tess_batch& b) {
 // Full sort of the batch's keys every step (the batch-wide tile space), then the
 // tile runs, active tiles and touched node blocks, as bin_particles does.
 template <typename T, int D, int BASIS>
+void batch_bin_run(tess_ctx* ctx, tess_batch& B, bool keys_from_g2p);
+
+template <typename T, int D, int BASIS>
 void batch_bin(tess_ctx* ctx, tess_batch& B, bool keys_from_g2p) {
+  batch_bin_run<T, D, BASIS>(ctx, B, keys_from_g2p);
+  if (TESS_DEBUG && B.ntot > 0) {
+    const Binning& b = ctx->bin;
+    debug_check_binning(ctx->stream, static_cast<const uint32_t*>(ctx->keys_sorted.ptr),
+                        static_cast<const uint32_t*>(ctx->vals.ptr), static_cast<const uint32_t*>(ctx->tile_start.ptr),
+                        static_cast<const uint32_t*>(ctx->tile_end.ptr),
+                        static_cast<const uint32_t*>(ctx->active_tiles.ptr), B.ntot, B.tiles_total, b.n_active,
+                        b.n_inside, "batch_bin");
+  }
+}
+
+template <typename T, int D, int BASIS>
+void batch_bin_run(tess_ctx* ctx, tess_batch& B, bool keys_from_g2p) {
   const int64_t n = B.ntot;
   const uint64_t ntiles = uint64_t(B.tiles_total);
   Binning& b = ctx->bin;
@@ -900,6 +916,14 @@ void batch_graph_run(tess_ctx* ctx, tess_batch& B, int nsteps) {
                           hipMemcpyDeviceToHost, gs));
   TESS_HIP(hipStreamSynchronize(gs));
   if (hc[CNT_ERR] & ERR_GIMP_LENGTH) raise(TESS_ERR_ARGUMENT, "uGIMP requires 0 ≤ pt.l ≤ spacing(mesh)");
+  if (TESS_DEBUG && B.ntot > 0) {
+    // the last step's binning, in set `cur` (the graphs' merges checked through it)
+    const BinSets sets = bin_sets(g, B.ntot);
+    debug_check_binning(gs, sets.keys[cur], sets.perm[cur], static_cast<const uint32_t*>(g->tile_start.ptr),
+                        static_cast<const uint32_t*>(g->tile_end.ptr),
+                        static_cast<const uint32_t*>(g->active_tiles.ptr), B.ntot, B.tiles_total,
+                        int64_t(hc[CNT_ACTIVE]), int64_t(hc[CNT_INSIDE]), "batch graph step");
+  }
   // the eager path's binning and zeroing bookkeeping no longer hold
   B.keys_ready = false;
   B.binned = false;

@@ -25,10 +25,11 @@
 
 #include <hip/hip_runtime.h>
 
-#include <functional>
-
 #include <algorithm>
 #include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
 
 #include "tess_correct.hpp"
 #include "tess_device.hpp"
@@ -2255,9 +2256,77 @@ inline void merge_movers(tess_ctx* ctx, const uint32_t* nk, const uint32_t* spos
   std::swap(ctx->vals, ctx->perm2);
 }
 
+// Debug builds (-DTESS_DEBUG=1, `make debug`): after every binning the host
+// reads the binning back and checks its invariants (the permutation, sorted
+// keys, each tile's run of keys, the active-tile list), raising
+// TESS_ERR_STATE on a violation instead of letting a later kernel index
+// through it.
+#ifndef TESS_DEBUG
+#define TESS_DEBUG 0
+#endif
+inline void debug_check_binning(hipStream_t st, const uint32_t* skeys, const uint32_t* perm, const uint32_t* tstart,
+                                const uint32_t* tend, const uint32_t* active, int64_t n, int64_t ntiles,
+                                int64_t n_active, int64_t n_inside, const char* where) {
+  if (!TESS_DEBUG) return;
+  auto fail = [&](const std::string& what) { raise(TESS_ERR_STATE, std::string("debug check (") + where + "): " + what); };
+  TESS_HIP(hipStreamSynchronize(st));
+  const size_t nn = size_t(std::max<int64_t>(n, 0)), nt = size_t(std::max<int64_t>(ntiles, 0));
+  std::vector<uint32_t> sk(nn), pm(nn), ts(nt), te(nt), act(size_t(std::max<int64_t>(n_active, 0)));
+  auto get = [&](std::vector<uint32_t>& v, const uint32_t* src) {
+    if (!v.empty()) TESS_HIP(hipMemcpy(v.data(), src, v.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  };
+  get(sk, skeys);
+  get(pm, perm);
+  get(ts, tstart);
+  get(te, tend);
+  get(act, active);
+  const uint64_t outside = uint64_t(ntiles) * 64u;
+  std::vector<uint8_t> seen(sk.size(), 0);
+  int64_t inside = 0;
+  for (int64_t j = 0; j < n; ++j) {
+    if (pm[size_t(j)] >= uint64_t(n) || seen[pm[size_t(j)]]) fail("the permutation repeats or leaves [0, n)");
+    seen[pm[size_t(j)]] = 1;
+    if (sk[size_t(j)] > outside) fail("a sort key past the tiles");
+    if (j > 0 && sk[size_t(j)] < sk[size_t(j - 1)]) fail("sort keys out of order at " + std::to_string(j));
+    inside += sk[size_t(j)] < outside ? 1 : 0;
+  }
+  if (n_inside >= 0 && inside != n_inside) fail("inside count " + std::to_string(n_inside) + " != " + std::to_string(inside));
+  int64_t nact = 0, j = 0;
+  for (int64_t t = 0; t < ntiles; ++t) {
+    const int64_t a = j;
+    while (j < inside && int64_t(sk[size_t(j)] >> 6) == t) ++j;
+    if (j > a) {
+      if (ts[size_t(t)] != uint64_t(a) || te[size_t(t)] != uint64_t(j)) fail("tile run of " + std::to_string(t));
+      if (nact >= n_active || act[size_t(nact)] != uint64_t(t)) fail("active-tile list at " + std::to_string(nact));
+      ++nact;
+    } else if (te[size_t(t)] > ts[size_t(t)]) {
+      fail("a particle run for the empty tile " + std::to_string(t));
+    }
+  }
+  if (nact != n_active) fail("active tiles " + std::to_string(n_active) + " != " + std::to_string(nact));
+}
+
+template <typename T, int D, int BASIS, bool STORED>
+void bin_particles_run(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const tess_weights* W,
+                       bool keys_from_g2p);
+
 template <typename T, int D, int BASIS, bool STORED>
 void bin_particles(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const tess_weights* W,
                    bool keys_from_g2p = false) {
+  bin_particles_run<T, D, BASIS, STORED>(ctx, mesh, P, W, keys_from_g2p);
+  if (TESS_DEBUG && P.n > 0) {
+    const Binning& b = ctx->bin;
+    debug_check_binning(ctx->stream, static_cast<const uint32_t*>(ctx->keys_sorted.ptr),
+                        static_cast<const uint32_t*>(ctx->vals.ptr), static_cast<const uint32_t*>(ctx->tile_start.ptr),
+                        static_cast<const uint32_t*>(ctx->tile_end.ptr),
+                        static_cast<const uint32_t*>(ctx->active_tiles.ptr), P.n, b.ntiles, b.n_active, b.n_inside,
+                        "bin_particles");
+  }
+}
+
+template <typename T, int D, int BASIS, bool STORED>
+void bin_particles_run(tess_ctx* ctx, const tess_mesh& mesh, const tess_particles& P, const tess_weights* W,
+                       bool keys_from_g2p) {
   Args<T, D> a = make_args<T, D>(mesh, nullptr, P, W, nullptr);
   const int64_t n = P.n;
   uint64_t ntiles = 1;

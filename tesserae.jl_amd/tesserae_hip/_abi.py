@@ -8,6 +8,7 @@ from __future__ import annotations
 import ctypes
 import os
 
+TESS_BUILD_DEBUG_CHECKS = 1  # tess_build_flags bit of the debug build
 TESS_ABI_VERSION = 4  # tess_debug_fail_after (v3: tess_step_sparse reports the steps it ran; v2: weight filter)
 
 TESS_OK = 0
@@ -40,6 +41,8 @@ TESS_PARTITION_NOALLOC_MAX = 1048576  # particles of tess_partition_update_noall
 
 LIB_NAME = "libtesserae_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+# the debug build (`make debug`): every binning read back and checked on the host
+DEBUG_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtesserae_hip_debug.so")
 
 
 class tess_mesh(ctypes.Structure):
@@ -159,6 +162,7 @@ SIGNATURES = {
     "tess_abi_version": (ctypes.c_int32, []),
     "tess_last_error": (ctypes.c_char_p, []),
     "tess_debug_fail_after": (ctypes.c_int32, [ctypes.c_int64]),
+    "tess_build_flags": (ctypes.c_int32, []),
     "tess_ctx_create": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_void_p, _P(ctypes.c_void_p)]),
     "tess_ctx_destroy": (ctypes.c_int32, [ctypes.c_void_p]),
     "tess_sync": (ctypes.c_int32, [ctypes.c_void_p]),

@@ -567,8 +567,9 @@ class MPMParams:
 class Context:
     """Owns one tess_ctx bound to a device and a HIP stream."""
 
-    def __init__(self, device: int | None = None, stream=None):
-        self.lib = _abi.load_library()
+    def __init__(self, device: int | None = None, stream=None, library: str | None = None):
+        # library: the path of another build of libtesserae_hip (e.g. the debug build) for this context
+        self.lib = _abi.load_library(library)
         self.device = torch.cuda.current_device() if device is None else int(device)
         s = torch.cuda.current_stream(self.device) if stream is None else stream
         self.stream = s
