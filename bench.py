@@ -203,8 +203,8 @@ def adjoint_ratio(nsteps, every):
     ratio: forward and backward, checkpoint recompute included (the checkpointed forward
     that records the checkpoints, then the pullback of the loss mean ||x(T) - (x(0) +
     (0.1,0,0))||^2: replay of every segment from its checkpoint plus the reversed steps,
-    FP64 adjoint) over the plain FP32 forward of the same steps.  pullback_over_forward
-    leaves the checkpointed forward out."""
+    FP64 adjoint) over the plain FP32 forward of the same steps, each the median of three
+    rounds.  pullback_over_forward leaves the checkpointed forward out."""
     import numpy as np
     import torch
 
@@ -231,22 +231,29 @@ def adjoint_ratio(nsteps, every):
     th.rollout(G, P, sc.basis, sc.params, 2, checkpoint_every=every, ctx=ctx)
     adj = th.Adjoint(sc.n, 3)
     th.rollout_adjoint(G, sc.basis, sc.params, adj, ctx=ctx)
-    mesh, P, G = to_gpu(sc)
-    t_fwd = timed(lambda: th.step(G, P, sc.basis, sc.params, nsteps=nsteps, ctx=ctx))
-    mesh, P, G = to_gpu(sc)
-    t_roll = timed(lambda: th.rollout(G, P, sc.basis, sc.params, nsteps, checkpoint_every=every, ctx=ctx))
-    x = P.get("x").double()
-    loss = float(((x - target) ** 2).sum(dim=1).mean())
-    adj = th.Adjoint(sc.n, 3)
-    adj.x.copy_(2.0 * (x - target) / sc.n)
-    t_adj = timed(lambda: th.rollout_adjoint(G, sc.basis, sc.params, adj, ctx=ctx))
-    gnorm = float(adj.v.norm())
+    # three rounds of (plain forward, checkpointed forward, pullback), each from the initial state; the
+    # figures are each column's median (the forward's launch-bound steps vary by a few percent per run)
+    fwd, roll, pull = [], [], []
+    for _ in range(3):
+        mesh, P, G = to_gpu(sc)
+        fwd.append(timed(lambda: th.step(G, P, sc.basis, sc.params, nsteps=nsteps, ctx=ctx)))
+        mesh, P, G = to_gpu(sc)
+        roll.append(timed(lambda: th.rollout(G, P, sc.basis, sc.params, nsteps, checkpoint_every=every, ctx=ctx)))
+        x = P.get("x").double()
+        loss = float(((x - target) ** 2).sum(dim=1).mean())
+        adj = th.Adjoint(sc.n, 3)
+        adj.x.copy_(2.0 * (x - target) / sc.n)
+        pull.append(timed(lambda: th.rollout_adjoint(G, sc.basis, sc.params, adj, ctx=ctx)))
+        gnorm = float(adj.v.norm())
     ctx.close()
+    t_fwd, t_roll, t_adj = (float(np.median(v)) for v in (fwd, roll, pull))
     return {"workload": f"inverse cube dx=1/64, {sc.n} particles, quadratic, {nsteps} steps, checkpoint every "
                         f"{every}, FP32 forward / FP64 adjoint",
             "t_forward_ms": t_fwd, "t_rollout_ms": t_roll, "t_pullback_ms": t_adj,
             "ratio": (t_roll + t_adj) / t_fwd, "pullback_over_forward": t_adj / t_fwd, "loss": loss,
-            "grad_v0_norm": gnorm}
+            "grad_v0_norm": gnorm,
+            "rounds_ms": {"forward": fwd, "rollout": roll, "pullback": pull},
+            "ratio_per_round": [(r + p) / f for f, r, p in zip(fwd, roll, pull)]}
 
 
 def batch_leg(steps, reorder):
