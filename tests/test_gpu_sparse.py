@@ -169,5 +169,10 @@ def test_step_sparse_storage_grows_mid_call():
     torch.cuda.synchronize()
     a, b = particles_np(Pd), particles_np(Ps)
     for name in a:
-        assert rel_err(b[name], a[name]) <= 1e-12, name
+        if name == "grad_v":
+            # the box moves rigidly: ∇v is rounding noise (~1e-14), whose bits follow the binning order (a
+            # failed call makes the next one sort from scratch); bounded against the flow's own scale |v|/h
+            assert np.abs(b[name] - a[name]).max() <= 1e-12 * 2.0 / 0.02, name
+        else:
+            assert rel_err(b[name], a[name]) <= 1e-12, name
     ctx.close()
