@@ -168,11 +168,13 @@ def test_step_sparse_storage_grows_mid_call():
     th.step(Gd, Pd, basis, sc.params, nsteps=nsteps, ctx=th.Context())
     torch.cuda.synchronize()
     a, b = particles_np(Pd), particles_np(Ps)
+    # the box moves rigidly and unstrained: ∇v and σ are rounding noise (~1e-14, ~1e-13) whose bits follow
+    # the binning order (a failed call makes the next one sort from scratch), so they are bounded against
+    # the flow's own scales, |v|/h and E; x, v, F and V relative to themselves
+    scale = {"grad_v": 2.0 / 0.02, "sigma": 1e3}
     for name in a:
-        if name == "grad_v":
-            # the box moves rigidly: ∇v is rounding noise (~1e-14), whose bits follow the binning order (a
-            # failed call makes the next one sort from scratch); bounded against the flow's own scale |v|/h
-            assert np.abs(b[name] - a[name]).max() <= 1e-12 * 2.0 / 0.02, name
+        if name in scale:
+            assert np.abs(b[name] - a[name]).max() <= 1e-12 * scale[name], name
         else:
             assert rel_err(b[name], a[name]) <= 1e-12, name
     ctx.close()
