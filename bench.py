@@ -275,7 +275,7 @@ def batch_leg(steps, reorder):
     ctx = th.Context()
     batch = th.Batch(members, th.BSpline(th.Quadratic()), ctx=ctx)
     batch.set_reorder(reorder)  # the reference loop's reorder_particles! threshold, per scene
-    variant = batch.autotune(2)
+    variant = batch.autotune(4)  # four timed steps per variant: eager and graph mode are close
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     times, prof, active_nodes = {}, {}, -1
     # the tuned launch shape both eagerly (one launch per kernel, host-sized) and in graph mode (replays of
