@@ -374,31 +374,44 @@ def g2p2g_leg(sc, passes, layout, reorder):
     """@G2P2G of the README program on the block drop, as the time loop of a program written as one @G2P2G
     block (tess_g2p2g_steps, deferred weights, the step loop's reorder threshold): one P2G, warm-up passes, then
     `passes` timed passes in one call, each the G2P half, then the P2G and grid-node halves with the weights at
-    the positions before the pass."""
+    the positions before the pass.  Timed as the library runs it (the G2P kernel, then the P2G kernels) and
+    as one particle sweep per pass (TESS_G2P2G_SWEEP=1)."""
     import torch
 
     import tesserae_hip as th
     from gpu_util import to_gpu
 
-    mesh, P, G = to_gpu(sc, layout=layout)
-    ctx = th.Context()
-    th.P2G(G, P, sc.basis, sc.params, ctx=ctx)
-    th.g2p2g_steps(G, P, sc.basis, sc.params, nsteps=3, reorder_threshold=reorder, ctx=ctx)
-    ctx.profile(True)
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    a.record()
-    th.g2p2g_steps(G, P, sc.basis, sc.params, nsteps=passes, reorder_threshold=reorder, ctx=ctx)
-    b.record()
-    torch.cuda.synchronize()
-    prof = ctx.profile_read()
-    ctx.close()
-    ms = a.elapsed_time(b) / passes
-    del P, G
-    torch.cuda.empty_cache()
+    def run(sweep):
+        old = os.environ.get("TESS_G2P2G_SWEEP")
+        os.environ["TESS_G2P2G_SWEEP"] = "1" if sweep else "0"
+        try:
+            mesh, P, G = to_gpu(sc, layout=layout)
+            ctx = th.Context()
+            th.P2G(G, P, sc.basis, sc.params, ctx=ctx)
+            th.g2p2g_steps(G, P, sc.basis, sc.params, nsteps=3, reorder_threshold=reorder, ctx=ctx)
+            ctx.profile(True)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            a.record()
+            th.g2p2g_steps(G, P, sc.basis, sc.params, nsteps=passes, reorder_threshold=reorder, ctx=ctx)
+            b.record()
+            torch.cuda.synchronize()
+            prof = ctx.profile_read()
+            ctx.close()
+            del P, G
+            torch.cuda.empty_cache()
+        finally:
+            if old is None:
+                del os.environ["TESS_G2P2G_SWEEP"]
+            else:
+                os.environ["TESS_G2P2G_SWEEP"] = old
+        return a.elapsed_time(b) / passes, {k: v[0] / max(v[1], 1) * 1e3 for k, v in prof.items()}
+
+    ms, kernels = run(False)
+    ms_s, kernels_s = run(True)
     return {"workload": f"block drop dx=1/256, {sc.n} particles, quadratic, FP32, G2P2G passes", "passes": passes,
-            "ms_per_pass": ms, "particle_steps_per_s": sc.n / (ms * 1e-3),
-            "kernels_avg_us": {k: v[0] / max(v[1], 1) * 1e3 for k, v in prof.items()}}
+            "ms_per_pass": ms, "particle_steps_per_s": sc.n / (ms * 1e-3), "kernels_avg_us": kernels,
+            "sweep_ms_per_pass": ms_s, "sweep_kernels_avg_us": kernels_s}
 
 
 def taylor_leg(steps):

@@ -21,11 +21,15 @@
 //                   the grid-node half (Float64 m⁻¹, vⁿ, v, gravity, slip walls)
 // G2P: k_g2p_tile stages the footprint's vⁿ, v in LDS and updates the tile's
 // particles; k_g2p_outside handles particles outside the mesh (empty window).
+// @G2P2G: the G2P kernel, then the P2G kernels at the kept positions; or, on
+// request, k_g2p2g_fast: the G2P half and the P2G half's tile partials in one
+// sweep over the particles, then k_gather_grid as for P2G.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <string>
@@ -1147,6 +1151,119 @@ __device__ __forceinline__ void p2g_load(const Args<T, D>& a, uint32_t pid, P2GI
   if constexpr (BASIS == TESS_UGIMP) in.l = a.l(pid) / a.mesh.h;
 }
 
+// One layer of class lanes into the stencil offsets a wave owns, in
+// registers: the KN offsets along the last axis from vl/dl (that axis's jets
+// there), all SW along the others from val/der (axes 0..D-2).  mpe is the
+// particle's mass (zero on a lane without a particle, whose jets are finite),
+// mpv = mpe v and A = -V σ (column-major).  Offset o = ix + SW*(jy + SW*kz).
+template <typename T, int D, int SW, int KN>
+__device__ __forceinline__ void p2g_accumulate(const T (*val)[SW], const T (*der)[SW], const T* vl, const T* dl,
+                                               const T mpe, const T* mpv, const T* A, T* am,
+                                               T (*amv)[Pow<SW, D - 1>::v * KN], T (*af)[Pow<SW, D - 1>::v * KN]) {
+  if constexpr (D == 3) {
+#pragma unroll
+    for (int kz = 0; kz < KN; ++kz) {
+#pragma unroll
+      for (int jy = 0; jy < SW; ++jy) {
+        const T YZ = val[1][jy] * vl[kz];
+        const T YdZ = der[1][jy] * vl[kz];
+        const T YZd = val[1][jy] * dl[kz];
+        const T cm = YZ * mpe;
+        T cmv[3], U[3], Wv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          cmv[c] = YZ * mpv[c];
+          U[c] = A[c] * YZ;
+          Wv[c] = A[c + 3] * YdZ + A[c + 6] * YZd;
+        }
+#pragma unroll
+        for (int ix = 0; ix < SW; ++ix) {
+          const int o = ix + SW * (jy + SW * kz);
+          am[o] += val[0][ix] * cm;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            amv[c][o] += val[0][ix] * cmv[c];
+            af[c][o] = fma(val[0][ix], Wv[c], fma(der[0][ix], U[c], af[c][o]));
+          }
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int jy = 0; jy < KN; ++jy) {
+      const T Y = vl[jy], Yd = dl[jy];
+      const T cm = Y * mpe;
+      T cmv[2], U[2], Wv[2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        cmv[c] = Y * mpv[c];
+        U[c] = A[c] * Y;
+        Wv[c] = A[c + 2] * Yd;
+      }
+#pragma unroll
+      for (int ix = 0; ix < SW; ++ix) {
+        const int o = ix + SW * jy;
+        am[o] += val[0][ix] * cm;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          amv[c][o] += val[0][ix] * cmv[c];
+          af[c][o] = fma(val[0][ix], Wv[c], fma(der[0][ix], U[c], af[c][o]));
+        }
+      }
+    }
+  }
+}
+
+// Adds one wave's stencil (offsets pass*KN .. pass*KN+KN-1 along the last
+// axis) into the tile's node sums in LDS.  Lanes are distinct classes, so for
+// one offset their target nodes are distinct; offsets are applied one after
+// another (read, add, write: the LDS executes one wave's accesses in order).
+template <typename T, int D, int SW, int KN>
+__device__ __forceinline__ void p2g_stencil_to_lds(T (*s_tile)[Geo<D>::PSZ], const int* ccoord, int pass,
+                                                   const T* am, const T (*amv)[Pow<SW, D - 1>::v * KN],
+                                                   const T (*af)[Pow<SW, D - 1>::v * KN]) {
+  constexpr int NV = 1 + 2 * D;
+  constexpr int NSL = Pow<SW, D - 1>::v * KN;
+#pragma unroll
+  for (int o = 0; o < NSL; ++o) {
+    int r = o, nI[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int off = (d == D - 1) ? pass * KN + r : r % SW;
+      nI[d] = ccoord[d] + off;
+      r /= SW;
+    }
+    const int q = pad_node<D>(nI[0], nI[1], nI[2]);
+    T t[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) t[v] = s_tile[v][q];
+    t[0] += am[o];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      t[1 + c] += amv[c][o];
+      t[1 + D + c] += af[c][o];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s_tile[v][q] = t[v];
+    __asm__ volatile("" ::: "memory");  // keep the offsets in program order
+  }
+}
+
+// The tile's compact partial (Part) from its node sums in LDS: the NT threads
+// of the workgroup write one node's NV values each.
+template <typename T, int D, int SW, int NT>
+__device__ __forceinline__ void p2g_write_partial(const Args<T, D>& a, const int64_t j, T (*s_tile)[Geo<D>::PSZ]) {
+  using PT = Part<D, SW>;
+  constexpr int NV = 1 + 2 * D;
+  T* out = a.partials + int64_t(j) * PT::SIZE;
+  for (int n = int(threadIdx.x); n < PT::NODES; n += NT) {
+    const int q = pad_node<D>(n % PT::E, (n / PT::E) % PT::E, D == 3 ? n / (PT::E * PT::E) : 0);
+    T* o = out + int64_t(n) * NV;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) o[v] = s_tile[v][q];
+  }
+}
+
 // NPASS waves per tile: wave w accumulates the offsets w*KN .. w*KN+KN-1
 // along the last axis for every class (the particles are read by each wave;
 // the registers of one wave hold 1/NPASS of the stencil, which keeps several
@@ -1310,58 +1427,7 @@ __device__ __forceinline__ void k_p2g_fast_body(Args<T, D>& a, const uint32_t* _
           dl[q] = s_jet[buf][D - 1][SW + pass * KN + q][lane];
         }
       }
-      if constexpr (D == 3) {
-#pragma unroll
-        for (int kz = 0; kz < KN; ++kz) {
-#pragma unroll
-          for (int jy = 0; jy < SW; ++jy) {
-            const T YZ = val[1][jy] * vl[kz];
-            const T YdZ = der[1][jy] * vl[kz];
-            const T YZd = val[1][jy] * dl[kz];
-            const T cm = YZ * mpe;
-            T cmv[3], U[3], Wv[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              cmv[c] = YZ * mpv[c];
-              U[c] = A[c] * YZ;
-              Wv[c] = A[c + 3] * YdZ + A[c + 6] * YZd;
-            }
-#pragma unroll
-            for (int ix = 0; ix < SW; ++ix) {
-              const int o = ix + SW * (jy + SW * kz);
-              am[o] += val[0][ix] * cm;
-#pragma unroll
-              for (int c = 0; c < 3; ++c) {
-                amv[c][o] += val[0][ix] * cmv[c];
-                af[c][o] = fma(val[0][ix], Wv[c], fma(der[0][ix], U[c], af[c][o]));
-              }
-            }
-          }
-        }
-      } else {
-#pragma unroll
-        for (int jy = 0; jy < KN; ++jy) {
-          const T Y = vl[jy], Yd = dl[jy];
-          const T cm = Y * mpe;
-          T cmv[2], U[2], Wv[2];
-#pragma unroll
-          for (int c = 0; c < 2; ++c) {
-            cmv[c] = Y * mpv[c];
-            U[c] = A[c] * Y;
-            Wv[c] = A[c + 2] * Yd;
-          }
-#pragma unroll
-          for (int ix = 0; ix < SW; ++ix) {
-            const int o = ix + SW * jy;
-            am[o] += val[0][ix] * cm;
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-              amv[c][o] += val[0][ix] * cmv[c];
-              af[c][o] = fma(val[0][ix], Wv[c], fma(der[0][ix], U[c], af[c][o]));
-            }
-          }
-        }
-      }
+      p2g_accumulate<T, D, SW, KN>(val, der, vl, dl, mpe, mpv, A, am, amv, af);
     };
     auto walk = [&](auto staged) {
       uint32_t qa = pid_of(1, staged), qb = 0u;
@@ -1380,31 +1446,7 @@ __device__ __forceinline__ void k_p2g_fast_body(Args<T, D>& a, const uint32_t* _
     // LDS executes one wave's accesses in order) and the waves one after
     // another, which fixes the per-node order of the additions
     for (int w = 0; w < NPASS; ++w) {
-      if (w == wave && ccount > 0) {
-#pragma unroll
-        for (int o = 0; o < NSL; ++o) {
-          int r = o, nI[3] = {0, 0, 0};
-#pragma unroll
-          for (int d = 0; d < D; ++d) {
-            const int off = (d == D - 1) ? pass * KN + r : r % SW;
-            nI[d] = ccoord[d] + off;
-            r /= SW;
-          }
-          const int q = pad_node<D>(nI[0], nI[1], nI[2]);
-          T t[NV];
-#pragma unroll
-          for (int v = 0; v < NV; ++v) t[v] = s_tile[v][q];
-          t[0] += am[o];
-#pragma unroll
-          for (int c = 0; c < D; ++c) {
-            t[1 + c] += amv[c][o];
-            t[1 + D + c] += af[c][o];
-          }
-#pragma unroll
-          for (int v = 0; v < NV; ++v) s_tile[v][q] = t[v];
-          __asm__ volatile("" ::: "memory");  // keep the offsets in program order
-        }
-      }
+      if (w == wave && ccount > 0) p2g_stencil_to_lds<T, D, SW, KN>(s_tile, ccoord, pass, am, amv, af);
       __syncthreads();
     }
   }
@@ -1413,14 +1455,7 @@ __device__ __forceinline__ void k_p2g_fast_body(Args<T, D>& a, const uint32_t* _
     for (int off = 32; off >= 1; off >>= 1) bad += __shfl_xor(bad, off, 64);
     if (lane == 0) misplaced[j] = bad;  // per tile, summed later
   }
-  // the partial: one thread per node writes its NV values
-  T* out = a.partials + int64_t(j) * PT::SIZE;
-  for (int n = int(threadIdx.x); n < PT::NODES; n += NT) {
-    const int q = pad_node<D>(n % PT::E, (n / PT::E) % PT::E, D == 3 ? n / (PT::E * PT::E) : 0);
-    T* o = out + int64_t(n) * NV;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) o[v] = s_tile[v][q];
-  }
+  p2g_write_partial<T, D, SW, NT>(a, j, s_tile);
 }
 
 template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false, bool SEP = false>
@@ -1478,6 +1513,200 @@ __device__ __forceinline__ void g2p_load(const Args<T, D>& a, uint32_t pid, G2PI
   if constexpr (BASIS == TESS_UGIMP) in.l = a.l(pid) / a.mesh.h;
 }
 
+// Stages a tile's footprint for G2P: v and v - vⁿ per component in the padded
+// node order of Geo (zeros outside the grid), and the footprint's node
+// coordinates per axis; NT threads.  The caller synchronises before use.
+template <typename T, int D, int NT>
+__device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const int* origin, T (*s_n)[Geo<D>::PSZ],
+                                          T (*s_nodex)[Geo<D>::TS]) {
+  using G = Geo<D>;
+  for (int q = int(threadIdx.x); q < G::NFOOT; q += NT) {
+    int I[D], r = q;
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      I[d] = origin[d] + r % G::TS;
+      r /= G::TS;
+      in = in && I[d] >= 0 && I[d] < a.mesh.dims[d];
+    }
+    const int64_t i = in ? grid_slot(a, I) : -1;
+    in = i >= 0;
+    const int pq = pad_node<D>(q % G::TS, (q / G::TS) % G::TS, D == 3 ? q / (G::TS * G::TS) : 0);
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      const T vv = in ? a.gv(i, c) : T(0);
+      const T vn = in ? a.gvn(i, c) : T(0);
+      s_n[c][pq] = vv;
+      s_n[D + c][pq] = vv - vn;
+    }
+  }
+  for (int q = int(threadIdx.x); q < D * G::TS; q += NT) {
+    const int d = q / G::TS, k = q % G::TS;
+    s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
+  }
+}
+
+// One particle's G2P and update (README.md:81-86) from the tile's staged
+// footprint (s_n: v and v - vⁿ per component; s_nodex: node coordinates): its
+// axis jets at its class window, the axis-factored sums of Δv, ∇v and the x
+// rate, the writes of v, ∇v, x, F, σ, V and, with nk, of the next sort key and
+// the stayer flag at its sorted slot jj.  emit(val, der, v, σ, V) receives the
+// jets and the updated state.
+template <typename T, int D, int BASIS, bool AOS, bool SEP, typename Emit>
+__device__ __forceinline__ void g2p_particle(const Args<T, D>& a, const G2PIn<T, D, AOS>& cur,
+                                             const T (*s_n)[Geo<D>::PSZ], const T (*s_nodex)[Geo<D>::TS],
+                                             const uint32_t jj, const uint32_t tile, const uint32_t ltile,
+                                             uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
+                                             const uint32_t ntiles, uint32_t* __restrict__ movers, Emit&& emit) {
+  using G = Geo<D>;
+  constexpr int SW = support_width(BASIS);
+  const T hinv = a.mesh.h_inv, dt = a.dt;
+  int cc[D];  // the particle's class coordinates (its window's offset in the footprint)
+  {
+    uint32_t r = cur.cls;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      cc[d] = int(r % G::BW);
+      r /= G::BW;
+    }
+  }
+  T val[D][SW], der[D][SW];
+  bool corrects = false;
+  if constexpr (SEP) corrects = sep_corrects<T, D, BASIS>(a, cur.x);
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const T xi = (cur.x[d] - a.mesh.xmin[d]) * hinv;
+    int lo, hi;
+    axis_window<T, BASIS>(xi, cur.l, a.mesh.dims[d], lo, hi);
+    if constexpr (SEP)
+      axis_jets_sep<T, BASIS>(cur.x[d], a.mesh.xmin[d], cc[d], hi - lo, s_nodex[d], hinv, corrects, val[d],
+                              der[d]);
+    else
+      axis_jets_mixed<T, BASIS>(cur.x[d], a.mesh.xmin[d], cc[d], hi - lo, s_nodex[d], cur.l, hinv, val[d],
+                                der[d]);
+  }
+  // X = Σ w v_i, DV = Σ w (v_i - vⁿ_i), L = Σ v_i ⊗ ∇w, factored by axis
+  T X[D], DV[D], L[D * D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    X[c] = T(0);
+    DV[c] = T(0);
+  }
+#pragma unroll
+  for (int q = 0; q < D * D; ++q) L[q] = T(0);
+  if constexpr (D == 3) {
+#pragma unroll
+    for (int kz = 0; kz < SW; ++kz)
+#pragma unroll
+      for (int jy = 0; jy < SW; ++jy) {
+        T S1[3] = {T(0), T(0), T(0)}, S2[3] = {T(0), T(0), T(0)}, Sd[3] = {T(0), T(0), T(0)};
+#pragma unroll
+        for (int ix = 0; ix < SW; ++ix) {
+          const int q = pad_node<3>(cc[0] + ix, cc[1] + jy, cc[2] + kz);
+          T nd[6];
+#pragma unroll
+          for (int u = 0; u < 6; ++u) nd[u] = s_n[u][q];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            S1[c] += val[0][ix] * nd[c];
+            S2[c] += der[0][ix] * nd[c];
+            Sd[c] += val[0][ix] * nd[3 + c];
+          }
+        }
+        const T YZ = val[1][jy] * val[2][kz];
+        const T YdZ = der[1][jy] * val[2][kz];
+        const T YZd = val[1][jy] * der[2][kz];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          X[c] += YZ * S1[c];
+          DV[c] += YZ * Sd[c];
+          L[c] += YZ * S2[c];
+          L[c + 3] += YdZ * S1[c];
+          L[c + 6] += YZd * S1[c];
+        }
+      }
+  } else {
+#pragma unroll
+    for (int jy = 0; jy < SW; ++jy) {
+      T S1[2] = {T(0), T(0)}, S2[2] = {T(0), T(0)}, Sd[2] = {T(0), T(0)};
+#pragma unroll
+      for (int ix = 0; ix < SW; ++ix) {
+        const int q = pad_node<2>(cc[0] + ix, cc[1] + jy, 0);
+        T nd[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) nd[u] = s_n[u][q];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          S1[c] += val[0][ix] * nd[c];
+          S2[c] += der[0][ix] * nd[c];
+          Sd[c] += val[0][ix] * nd[2 + c];
+        }
+      }
+      const T Y = val[1][jy], Yd = der[1][jy];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        X[c] += Y * S1[c];
+        DV[c] += Y * Sd[c];
+        L[c] += Y * S2[c];
+        L[c + 2] += Yd * S1[c];
+      }
+    }
+  }
+  const uint32_t pid = cur.pid;
+  // particle equations (README.md:81-86)
+  T vn[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    vn[c] = cur.v[c] + DV[c];
+    st_g2p(fldw<T, D, AOS>(a.v, pid, c), vn[c]);
+  }
+#pragma unroll
+  for (int q = 0; q < D * D; ++q) st_g2p(fldw<T, D * D, AOS>(a.gradv, pid, q), L[q]);
+  T xn[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    xn[c] = cur.x[c] + X[c] * dt;
+    st_g2p(fldw<T, D, AOS>(a.x, pid, c), xn[c]);
+  }
+  if (a.x_keep) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) a.x_keep[int64_t(pid) * D + c] = cur.x[c];
+  }
+  if (nk) {
+    uint32_t nl = 1;  // the scene's tiles; in a batch its keys start at its first global tile
+#pragma unroll
+    for (int d = 0; d < D; ++d) nl *= uint32_t(a.ntiles[d]);
+    const uint32_t kl = key_of_position<T, D, BASIS>(a, xn, cur.l, nl);
+    const uint32_t key = kl == nl * 64u ? ntiles * 64u : kl + (tile - ltile) * 64u;
+    nk[jj] = key;
+    const bool moved = key != cur.key;
+    stay[jj] = moved ? 0u : 1u;
+    // particles whose key changed, counted per wave by its lowest live lane
+    // (no atomic in the common case of none)
+    const uint64_t mv = __ballot(moved);
+    if (mv && (__ballot(true) & lanemask_lt64()) == 0)
+      atomicAdd(movers + (blockIdx.x % MOVER_LANES) * MOVER_STRIDE, uint32_t(__popcll(mv)));
+  }
+  T Fn[D * D];
+#pragma unroll
+  for (int b = 0; b < D; ++b)
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      T t = L[r] * cur.F[D * b];  // (I + Δt∇v) F = F + Δt (∇v F)
+#pragma unroll
+      for (int kk = 1; kk < D; ++kk) t += L[r + D * kk] * cur.F[kk + D * b];
+      Fn[r + D * b] = cur.F[r + D * b] + dt * t;
+    }
+#pragma unroll
+  for (int q = 0; q < D * D; ++q) st_g2p(fldw<T, D * D, AOS>(a.F, pid, q), Fn[q]);
+  T sg[D * D], Vn;
+  neo_hookean<T, D>(Fn, a.lambda, a.mu, a.V0, a.logE.p != nullptr, a.logE.p ? a.logE(pid) : T(0), sg, Vn);
+#pragma unroll
+  for (int q = 0; q < D * D; ++q) st_g2p(fldw<T, D * D, AOS>(a.sigma, pid, q), sg[q]);
+  st_g2p(a.V(pid), Vn);
+  emit(val, der, vn, sg, Vn);
+}
+
 // With nk != nullptr the kernel also writes, at each particle's sorted position
 // j, the sort key of its new position (nk[j]) and whether it equals the
 // current one (stay[j]): the next step's incremental binning starts from them.
@@ -1511,35 +1740,11 @@ __device__ __forceinline__ void k_g2p_fast_body(Args<T, D>& a, const uint32_t* _
   __shared__ T s_n[2 * D][G::PSZ];
   __shared__ T s_nodex[D][G::TS];
   __shared__ uint32_t s_cnt[64];
-  for (int q = int(threadIdx.x); q < G::NFOOT; q += 64 * NW) {
-    int I[D], r = q;
-    bool in = true;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      I[d] = origin[d] + r % G::TS;
-      r /= G::TS;
-      in = in && I[d] >= 0 && I[d] < a.mesh.dims[d];
-    }
-    const int64_t i = in ? grid_slot(a, I) : -1;
-    in = i >= 0;
-    const int pq = pad_node<D>(q % G::TS, (q / G::TS) % G::TS, D == 3 ? q / (G::TS * G::TS) : 0);
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      const T vv = in ? a.gv(i, c) : T(0);
-      const T vn = in ? a.gvn(i, c) : T(0);
-      s_n[c][pq] = vv;
-      s_n[D + c][pq] = vv - vn;
-    }
-  }
-  for (int q = int(threadIdx.x); q < D * G::TS; q += 64 * NW) {
-    const int d = q / G::TS, k = q % G::TS;
-    s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
-  }
+  g2p_stage<T, D, 64 * NW>(a, origin, s_n, s_nodex);
   int ccount, coff, maxcnt;
   __shared__ uint32_t s_pid[PID_CAP], s_key[PID_CAP];
   tile_class_counts<NW>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt, a.perm, poff, s_pid, s_key,
                         PID_CAP);  // barrier after the staging
-  const T hinv = a.mesh.h_inv, dt = a.dt;
   // Two walks of the tile's particles.  Class order (lane = class, iteration k =
   // the k-th particle of every class) reads the layered storage contiguously
   // and takes maxcnt iterations.  Sorted order (iteration k = sorted positions
@@ -1590,146 +1795,8 @@ __device__ __forceinline__ void k_g2p_fast_body(Args<T, D>& a, const uint32_t* _
     q_after = ahead(k + 2 * NW, staged);
     fetch(k + NW, q_next, nxt);
     if (!act) return;
-    int cc[D];  // the particle's class coordinates (its window's offset in the footprint)
-    {
-      uint32_t r = cur.cls;
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        cc[d] = int(r % G::BW);
-        r /= G::BW;
-      }
-    }
-    T val[D][SW], der[D][SW];
-    bool corrects = false;
-    if constexpr (SEP) corrects = sep_corrects<T, D, BASIS>(a, cur.x);
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const T xi = (cur.x[d] - a.mesh.xmin[d]) * hinv;
-      int lo, hi;
-      axis_window<T, BASIS>(xi, cur.l, a.mesh.dims[d], lo, hi);
-      if constexpr (SEP)
-        axis_jets_sep<T, BASIS>(cur.x[d], a.mesh.xmin[d], cc[d], hi - lo, s_nodex[d], hinv, corrects, val[d],
-                                der[d]);
-      else
-        axis_jets_mixed<T, BASIS>(cur.x[d], a.mesh.xmin[d], cc[d], hi - lo, s_nodex[d], cur.l, hinv, val[d],
-                                  der[d]);
-    }
-    // X = Σ w v_i, DV = Σ w (v_i - vⁿ_i), L = Σ v_i ⊗ ∇w, factored by axis
-    T X[D], DV[D], L[D * D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      X[c] = T(0);
-      DV[c] = T(0);
-    }
-#pragma unroll
-    for (int q = 0; q < D * D; ++q) L[q] = T(0);
-    if constexpr (D == 3) {
-#pragma unroll
-      for (int kz = 0; kz < SW; ++kz)
-#pragma unroll
-        for (int jy = 0; jy < SW; ++jy) {
-          T S1[3] = {T(0), T(0), T(0)}, S2[3] = {T(0), T(0), T(0)}, Sd[3] = {T(0), T(0), T(0)};
-#pragma unroll
-          for (int ix = 0; ix < SW; ++ix) {
-            const int q = pad_node<3>(cc[0] + ix, cc[1] + jy, cc[2] + kz);
-            T nd[6];
-#pragma unroll
-            for (int u = 0; u < 6; ++u) nd[u] = s_n[u][q];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              S1[c] += val[0][ix] * nd[c];
-              S2[c] += der[0][ix] * nd[c];
-              Sd[c] += val[0][ix] * nd[3 + c];
-            }
-          }
-          const T YZ = val[1][jy] * val[2][kz];
-          const T YdZ = der[1][jy] * val[2][kz];
-          const T YZd = val[1][jy] * der[2][kz];
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            X[c] += YZ * S1[c];
-            DV[c] += YZ * Sd[c];
-            L[c] += YZ * S2[c];
-            L[c + 3] += YdZ * S1[c];
-            L[c + 6] += YZd * S1[c];
-          }
-        }
-    } else {
-#pragma unroll
-      for (int jy = 0; jy < SW; ++jy) {
-        T S1[2] = {T(0), T(0)}, S2[2] = {T(0), T(0)}, Sd[2] = {T(0), T(0)};
-#pragma unroll
-        for (int ix = 0; ix < SW; ++ix) {
-          const int q = pad_node<2>(cc[0] + ix, cc[1] + jy, 0);
-          T nd[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) nd[u] = s_n[u][q];
-#pragma unroll
-          for (int c = 0; c < 2; ++c) {
-            S1[c] += val[0][ix] * nd[c];
-            S2[c] += der[0][ix] * nd[c];
-            Sd[c] += val[0][ix] * nd[2 + c];
-          }
-        }
-        const T Y = val[1][jy], Yd = der[1][jy];
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          X[c] += Y * S1[c];
-          DV[c] += Y * Sd[c];
-          L[c] += Y * S2[c];
-          L[c + 2] += Yd * S1[c];
-        }
-      }
-    }
-    const uint32_t pid = cur.pid;
-    // particle equations (README.md:81-86)
-#pragma unroll
-    for (int c = 0; c < D; ++c) st_g2p(fldw<T, D, AOS>(a.v, pid, c), cur.v[c] + DV[c]);
-#pragma unroll
-    for (int q = 0; q < D * D; ++q) st_g2p(fldw<T, D * D, AOS>(a.gradv, pid, q), L[q]);
-    T xn[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      xn[c] = cur.x[c] + X[c] * dt;
-      st_g2p(fldw<T, D, AOS>(a.x, pid, c), xn[c]);
-    }
-    if (a.x_keep) {
-#pragma unroll
-      for (int c = 0; c < D; ++c) a.x_keep[int64_t(pid) * D + c] = cur.x[c];
-    }
-    if (nk) {
-      const uint32_t jj = slot(k);
-      uint32_t nl = 1;  // the scene's tiles; in a batch its keys start at its first global tile
-#pragma unroll
-      for (int d = 0; d < D; ++d) nl *= uint32_t(a.ntiles[d]);
-      const uint32_t kl = key_of_position<T, D, BASIS>(a, xn, cur.l, nl);
-      const uint32_t key = kl == nl * 64u ? ntiles * 64u : kl + (tile - ltile) * 64u;
-      nk[jj] = key;
-      const bool moved = key != cur.key;
-      stay[jj] = moved ? 0u : 1u;
-      // particles whose key changed, counted per wave by its lowest live lane
-      // (no atomic in the common case of none)
-      const uint64_t mv = __ballot(moved);
-      if (mv && (__ballot(true) & lanemask_lt64()) == 0)
-        atomicAdd(movers + (blockIdx.x % MOVER_LANES) * MOVER_STRIDE, uint32_t(__popcll(mv)));
-    }
-    T Fn[D * D];
-#pragma unroll
-    for (int b = 0; b < D; ++b)
-#pragma unroll
-      for (int r = 0; r < D; ++r) {
-        T t = L[r] * cur.F[D * b];  // (I + Δt∇v) F = F + Δt (∇v F)
-#pragma unroll
-        for (int kk = 1; kk < D; ++kk) t += L[r + D * kk] * cur.F[kk + D * b];
-        Fn[r + D * b] = cur.F[r + D * b] + dt * t;
-      }
-#pragma unroll
-    for (int q = 0; q < D * D; ++q) st_g2p(fldw<T, D * D, AOS>(a.F, pid, q), Fn[q]);
-    T sg[D * D], Vn;
-    neo_hookean<T, D>(Fn, a.lambda, a.mu, a.V0, a.logE.p != nullptr, a.logE.p ? a.logE(pid) : T(0), sg, Vn);
-#pragma unroll
-    for (int q = 0; q < D * D; ++q) st_g2p(fldw<T, D * D, AOS>(a.sigma, pid, q), sg[q]);
-    st_g2p(a.V(pid), Vn);
+    g2p_particle<T, D, BASIS, AOS, SEP>(a, cur, s_n, s_nodex, nk ? slot(k) : 0u, tile, ltile, nk, stay, ntiles,
+                                        movers, [](const auto&...) {});
   };
   auto walk = [&](auto staged) {
     Ahead qa = ahead(wave + NW, staged), qb{0u, 0u};
@@ -1781,6 +1848,264 @@ __global__ __launch_bounds__(64 * NW, sizeof(T) == 4 ? TESS_G2P_WAVES_PER_SIMD :
 
 
 // ---- gather + grid-node half ------------------------------------------------
+// ---- single-sweep @G2P2G -------------------------------------------------------
+//
+// One pass over a tile's particles runs both halves of a @G2P2G program
+// (src/Transfer/g2p2g.jl:100-159) with one set of weights, those of the
+// positions before the pass.  NP producer waves run the G2P half, one layer of
+// class lanes each per round (the k-th particle of every class, as k_g2p_fast
+// walks them: gather, update, writes).  They hand each particle's position
+// before the update, mass, mₚv and -Vσ to NC consumer waves through LDS.  The
+// consumers evaluate the jets of that position, own the stencil offsets of
+// k_p2g_fast's waves and accumulate those layers one round behind the
+// producers (double-buffered exchange, one barrier per round).
+// They see the layers in k_p2g_fast's order and run its arithmetic, so the
+// tile's partial is the one that the G2P kernel followed by the P2G kernel at
+// the kept positions produces.  The particles are read once per pass instead
+// of twice, and the G2P half's memory waits overlap the P2G half's arithmetic.
+//
+// σ is exchanged as its D(D+1)/2 distinct entries: neo_hookean forms it
+// symmetric to the last bit (both triangles from the same products).
+
+// index of σ's entry (r, b), r >= b, among the distinct entries
+__host__ __device__ constexpr int sym_index(int r, int b, int D) { return b * D - b * (b - 1) / 2 + (r - b); }
+
+// producer waves of the single-sweep kernel (0: as many as the LDS budget allows)
+#ifndef TESS_G2P2G_PRODUCERS
+#define TESS_G2P2G_PRODUCERS 0
+#endif
+template <typename T, int D, int BASIS>
+struct G2P2GShape {
+  static constexpr int SW = support_width(BASIS);
+  static constexpr int NV = 1 + 2 * D;
+  static constexpr int NS = D * (D + 1) / 2;
+  static constexpr bool HAS_L = BASIS == TESS_UGIMP;
+  // exchange rows per particle: x[D] (and l) before the update, mass, mₚv[D], -Vσ[NS]
+  static constexpr int XL = D, XM = D + (HAS_L ? 1 : 0), XV = XM + 1, XA = XV + D, XR = XA + NS;
+  // consumer waves: those of k_p2g_fast (3D: one per last-axis offset; 2D: one
+  // while the stencil's accumulators fit)
+  static constexpr int NC = (D == 3 || Pow<SW, D>::v * NV * int(sizeof(T)) / 4 > 96) ? SW : 1;
+  static constexpr int lds(int np) {
+    return int(sizeof(T)) * (2 * np * XR * 64 + 2 * D * Geo<D>::PSZ + D * Geo<D>::TS) + 4 * (2 * PID_CAP + 64);
+  }
+  // producer waves: eight waves per workgroup, two workgroups per CU at 128
+  // VGPRs (Float32)
+  static constexpr int NP_AUTO = 8 - NC;
+  static constexpr int NP = TESS_G2P2G_PRODUCERS > 0 ? TESS_G2P2G_PRODUCERS : NP_AUTO;
+  // waves per SIMD to compile for: as many workgroups per CU as the LDS holds
+  // (at most two), at most 4 waves per SIMD; Float64 1
+  static constexpr int wps(int np) {
+    const int wg = lds(np) <= 80 * 1024 ? 2 : 1;
+    const int w = (wg * (np + NC) + 3) / 4;
+    return sizeof(T) != 4 ? 1 : w < 1 ? 1 : w > 4 ? 4 : w;
+  }
+};
+
+template <typename T, int D, int BASIS, bool AOS, int NP, int NC>
+__device__ __forceinline__ void k_g2p2g_fast_body(Args<T, D>& a, const uint32_t* __restrict__ skeys,
+                                                  uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
+                                                  uint32_t ntiles, uint32_t* __restrict__ movers,
+                                                  uint32_t* __restrict__ misplaced, const int64_t j) {
+  using G = Geo<D>;
+  using SH = G2P2GShape<T, D, BASIS>;
+  constexpr int SW = SH::SW, NV = SH::NV, XR = SH::XR, XM = SH::XM, XV = SH::XV, XA = SH::XA;
+  constexpr int KN = SW / NC;                   // last-axis offsets per consumer
+  constexpr int NSL = Pow<SW, D - 1>::v * KN;  // offsets per consumer
+  constexpr int NT = 64 * (NP + NC);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform: scalar branches
+  const uint32_t tile = a.active[j];
+  const uint32_t start = a.tile_start[tile];
+  const int cnt = int(a.tile_end[tile] - start);
+  uint32_t poff;
+  const uint32_t ltile = bind_tile<false>(a, tile, poff);
+  int tI[D], origin[D];
+  unlin<D>(ltile, a.ntiles, tI);
+#pragma unroll
+  for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
+  __shared__ T s_n[2 * D][G::PSZ];
+  __shared__ T s_nodex[D][G::TS];
+  __shared__ uint32_t s_cnt[64];
+  __shared__ uint32_t s_pid[PID_CAP], s_key[PID_CAP];
+  // [round parity][producer][row][class lane]; after the last round the
+  // tile's node sums (one array per component) take its place
+  __shared__ __align__(16) T s_x[2][NP][XR][64];
+  static_assert(NV * G::PSZ <= 2 * NP * XR * 64, "the node sums fit in the exchange");
+  T(*s_tile)[G::PSZ] = reinterpret_cast<T(*)[G::PSZ]>(&s_x[0][0][0][0]);
+  g2p_stage<T, D, NT>(a, origin, s_n, s_nodex);
+  int ccount, coff, maxcnt;
+  tile_class_counts<NP + NC>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt, a.perm, poff, s_pid, s_key,
+                             PID_CAP);  // barrier after the staging
+  const int nround = (maxcnt + NP - 1) / NP;  // the same in every wave
+  auto zero_tile = [&]() {
+    static_assert(NV * G::PSZ * sizeof(T) % 16 == 0, "node sums in 16-byte words");
+    for (int q = int(threadIdx.x); q < int(NV * G::PSZ * sizeof(T) / 16); q += NT)
+      reinterpret_cast<uint4*>(&s_tile[0][0])[q] = make_uint4(0u, 0u, 0u, 0u);
+  };
+  // Both roles pass the same barriers: nround + 1 in the rounds, one after the
+  // node sums are zeroed, and one per consumer as the stencils are added.
+  if (wave < NP) {
+    // producer: layers k = wave, wave + NP, ... (round r takes k = r*NP + wave)
+    auto ahead = [&](int k, auto staged) {
+      uint32_t pid = 0u, key = 0u;
+      if (k < ccount) {
+        if constexpr (decltype(staged)::value) {
+          pid = s_pid[coff + k];
+          key = s_key[coff + k];
+        } else {
+          pid = a.perm[start + coff + k] - poff;
+          key = skeys[start + coff + k];
+        }
+      }
+      return uint2{pid, key};
+    };
+    auto fetch = [&](int k, const uint2& q, G2PIn<T, D, AOS>& in) {
+      if (k >= ccount) return;
+      g2p_load<T, D, BASIS, AOS>(a, q.x, in);
+      in.key = q.y;
+      in.cls = uint32_t(lane);
+    };
+    G2PIn<T, D, AOS> pa, pb;
+    std::memset(static_cast<void*>(&pa), 0, sizeof(pa));
+    std::memset(static_cast<void*>(&pb), 0, sizeof(pb));
+    auto produce = [&](int r, const G2PIn<T, D, AOS>& cur, G2PIn<T, D, AOS>& nxt, const uint2& q_next, uint2& q_after,
+                     auto staged) {
+      if (r < nround) {
+        const int k = r * NP + wave;
+        q_after = ahead(k + 2 * NP, staged);
+        fetch(k + NP, q_next, nxt);
+        T(*xr)[64] = s_x[r & 1][wave];
+        if (k < ccount) {
+          g2p_particle<T, D, BASIS, AOS, false>(
+              a, cur, s_n, s_nodex, start + uint32_t(coff + k), tile, ltile, nk, stay, ntiles, movers,
+              [&](const auto&, const auto&, const auto& vn, const auto& sg, const T Vn) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) xr[d][lane] = cur.x[d];
+                if constexpr (SH::HAS_L) xr[SH::XL][lane] = cur.l;
+                const T mp = a.mp;
+                xr[XM][lane] = mp;
+#pragma unroll
+                for (int c = 0; c < D; ++c) xr[XV + c][lane] = mp * vn[c];
+                const T negV = -Vn;
+#pragma unroll
+                for (int b = 0; b < D; ++b)
+#pragma unroll
+                  for (int rr = b; rr < D; ++rr) xr[XA + sym_index(rr, b, D)][lane] = negV * sg[rr + D * b];
+              });
+        } else {
+          // no particle: a finite position, zero mass and stress (exact zero contributions)
+#pragma unroll
+          for (int q = 0; q < XR; ++q) xr[q][lane] = T(0);
+        }
+      }
+      __syncthreads();
+    };
+    auto walk = [&](auto staged) {
+      uint2 qa = ahead(wave + NP, staged), qb{0u, 0u};
+      fetch(wave, ahead(wave, staged), pa);
+      for (int r = 0; r <= nround; r += 2) {
+        produce(r, pa, pb, qa, qb, staged);
+        if (r + 1 <= nround) produce(r + 1, pb, pa, qb, qa, staged);
+      }
+    };
+    if (cnt <= PID_CAP)
+      walk(std::true_type{});
+    else
+      walk(std::false_type{});
+    zero_tile();
+    __syncthreads();
+    for (int w = 0; w < NC; ++w) __syncthreads();
+  } else {
+    // consumer `pass`: the stencil offsets with last-axis offset pass*KN .. pass*KN+KN-1
+    const int pass = wave - NP;
+    int ccoord[D];
+    {
+      int r = lane;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        ccoord[d] = r % G::BW;
+        r /= G::BW;
+      }
+    }
+    T am[NSL], amv[D][NSL], af[D][NSL];
+#pragma unroll
+    for (int o = 0; o < NSL; ++o) {
+      am[o] = T(0);
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        amv[c][o] = T(0);
+        af[c][o] = T(0);
+      }
+    }
+    const T hinv = a.mesh.h_inv;
+    uint32_t bad = 0, last = 0;
+    for (int r = 0; r <= nround; ++r) {
+      if (r > 0) {
+        for (int p = 0; p < NP; ++p) {
+          const int k = (r - 1) * NP + p;
+          if (k >= maxcnt) break;  // uniform
+          const T(*xr)[64] = s_x[(r - 1) & 1][p];
+          // the jets of the position before the update, as k_p2g_fast evaluates them
+          T x[D], l = T(0);
+#pragma unroll
+          for (int d = 0; d < D; ++d) x[d] = xr[d][lane];
+          if constexpr (SH::HAS_L) l = xr[SH::XL][lane];
+          T val[D][SW], der[D][SW], vl[KN], dl[KN], mpv[D], A[D * D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            const T xi = (x[d] - a.mesh.xmin[d]) * hinv;
+            int lo, hi;
+            axis_window<T, BASIS>(xi, l, a.mesh.dims[d], lo, hi);
+            axis_jets_mixed<T, BASIS>(x[d], a.mesh.xmin[d], ccoord[d], hi - lo, s_nodex[d], l, hinv, val[d], der[d]);
+          }
+#pragma unroll
+          for (int q = 0; q < KN; ++q) {
+            vl[q] = val[D - 1][pass * KN + q];
+            dl[q] = der[D - 1][pass * KN + q];
+          }
+          const T mpe = xr[XM][lane];
+#pragma unroll
+          for (int c = 0; c < D; ++c) mpv[c] = xr[XV + c][lane];
+#pragma unroll
+          for (int b = 0; b < D; ++b)
+#pragma unroll
+            for (int rr = 0; rr < D; ++rr)
+              A[rr + D * b] = xr[XA + (rr >= b ? sym_index(rr, b, D) : sym_index(b, rr, D))][lane];
+          p2g_accumulate<T, D, SW, KN>(val, der, vl, dl, mpe, mpv, A, am, amv, af);
+          if (misplaced && pass == 0) {
+            const bool act = k < ccount;
+            uint32_t pid = 0u;
+            if (act) pid = cnt <= PID_CAP ? s_pid[coff + k] : a.perm[start + coff + k] - poff;
+            bad += broken_pair(__ballot(act), k, pid, last);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    zero_tile();
+    __syncthreads();
+    // the consumers add their stencils one after another (fixed per-node order)
+    for (int w = 0; w < NC; ++w) {
+      if (w == pass && ccount > 0) p2g_stencil_to_lds<T, D, SW, KN>(s_tile, ccoord, pass, am, amv, af);
+      __syncthreads();
+    }
+    if (misplaced && pass == 0) {
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) bad += __shfl_xor(bad, off, 64);
+      if (lane == 0) misplaced[j] = bad;  // per tile, summed later
+    }
+  }
+  p2g_write_partial<T, D, SW, NT>(a, j, s_tile);
+}
+
+template <typename T, int D, int BASIS, bool AOS, int NP, int NC>
+__global__ __launch_bounds__(64 * (NP + NC), (G2P2GShape<T, D, BASIS>::wps(NP))) void k_g2p2g_fast(
+    Args<T, D> a, const uint32_t* __restrict__ skeys, uint32_t* __restrict__ nk, uint32_t* __restrict__ stay,
+    uint32_t ntiles, uint32_t* __restrict__ movers, uint32_t* __restrict__ misplaced) {
+  const int64_t j = item_of<false>(blockIdx.x, a.n_active);
+  if (j < a.n_active) k_g2p2g_fast_body<T, D, BASIS, AOS, NP, NC>(a, skeys, nk, stay, ntiles, movers, misplaced, j);
+}
+
 template <typename T, int D, int SW, bool GAOS, bool BATCH>
 __device__ __forceinline__ void k_gather_grid_body(Args<T, D>& a, uint32_t* __restrict__ nodes_per_block, const int64_t j) {
   using G = Geo<D>;
@@ -2472,9 +2797,16 @@ void bin_particles_run(tess_ctx* ctx, const tess_mesh& mesh, const tess_particle
 // P2G after binning: zeroing of what this call does not overwrite, the tile
 // kernel and the gather.  count_misplaced asks the tile kernel to count
 // particles not stored in the layered order (for the step driver's reorder).
+// With `sweep`, the tile partials come from sweep(a, sorted keys, misplaced)
+// in place of the P2G kernel (the single-sweep @G2P2G kernel, which reads the
+// grid's v and vⁿ: the grid is zeroed after it).
+template <typename T, int D>
+using SweepFn = std::function<void(Args<T, D>&, const uint32_t*, uint32_t*)>;
+
 template <typename T, int D, int BASIS, bool STORED>
 void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
-                const tess_weights* W, const tess_params& prm, bool count_misplaced, const tess_sparsity* sp) {
+                const tess_weights* W, const tess_params& prm, bool count_misplaced, const tess_sparsity* sp,
+                const SweepFn<T, D>* sweep = nullptr) {
   using Gm = Geo<D>;
   constexpr int NV = 1 + 2 * D;
   constexpr int SW = support_width(BASIS);
@@ -2496,19 +2828,24 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   if (sp) {
     a.p2g_err = counters + CNT_P2GERR;
     TESS_HIP(hipMemsetAsync(a.p2g_err, 0, sizeof(uint32_t), ctx->stream));
-    if (sp->storage_len > 0) {
-      hipLaunchKernelGGL((k_zero_grid<T, D>), dim3(blocks_for(sp->storage_len, 256)), dim3(256), 0, ctx->stream, a,
-                         sp->storage_len);
+  }
+  auto zero_grid = [&]() {
+    if (sp) {
+      if (sp->storage_len > 0) {
+        hipLaunchKernelGGL((k_zero_grid<T, D>), dim3(blocks_for(sp->storage_len, 256)), dim3(256), 0, ctx->stream,
+                           a, sp->storage_len);
+        TESS_HIP(hipGetLastError());
+      }
+    } else if (fresh) {
+      hipLaunchKernelGGL((k_zero_grid<T, D>), dim3(blocks_for(nnodes, 256)), dim3(256), 0, ctx->stream, a, nnodes);
+      TESS_HIP(hipGetLastError());
+    } else if (ctx->prev_nodeblocks > 0) {
+      hipLaunchKernelGGL((k_zero_stale<T, D>), dim3(unsigned(ctx->prev_nodeblocks)), dim3(64), 0, ctx->stream, a,
+                         static_cast<const uint32_t*>(ctx->nb_list_prev.ptr), ctx->prev_nodeblocks);
       TESS_HIP(hipGetLastError());
     }
-  } else if (fresh) {
-    hipLaunchKernelGGL((k_zero_grid<T, D>), dim3(blocks_for(nnodes, 256)), dim3(256), 0, ctx->stream, a, nnodes);
-    TESS_HIP(hipGetLastError());
-  } else if (ctx->prev_nodeblocks > 0) {
-    hipLaunchKernelGGL((k_zero_stale<T, D>), dim3(unsigned(ctx->prev_nodeblocks)), dim3(64), 0, ctx->stream, a,
-                       static_cast<const uint32_t*>(ctx->nb_list_prev.ptr), ctx->prev_nodeblocks);
-    TESS_HIP(hipGetLastError());
-  }
+  };
+  if (!sweep) zero_grid();
   const Binning& b = ctx->bin;
   uint32_t* misplaced = nullptr;
   uint32_t* per_tile = ctx->per_tile.as<uint32_t>(size_t(std::max<int64_t>(b.n_active, 1)));
@@ -2521,7 +2858,11 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   } else {
     ctx->misplaced_x_ptr = nullptr;
   }
-  if (b.n_active > 0) {
+  if (b.n_active > 0 && sweep) {
+    a.partials = ctx->partials.as<T>(size_t(b.n_active) * Part<D, SW>::SIZE);
+    (*sweep)(a, static_cast<const uint32_t*>(ctx->keys_sorted.ptr), misplaced);
+    if (misplaced) sum_u32(ctx, misplaced, b.n_active, counters + CNT_MISPLACED);
+  } else if (b.n_active > 0) {
     a.partials = ctx->partials.as<T>(size_t(b.n_active) * Part<D, SW>::SIZE);
     // algorithmic bytes: x, v, V, σ per particle
     const int tok = prof_begin(ctx, "k_p2g_tile", double(b.n_inside) * sizeof(T) * (2 * D + 1 + D * D));
@@ -2563,6 +2904,7 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
       sum_u32(ctx, misplaced, b.n_active, counters + CNT_MISPLACED);
     }
   }
+  if (sweep) zero_grid();
   TESS_HIP(hipMemsetAsync(counters + CNT_NODES, 0, sizeof(uint32_t), ctx->stream));
   if (b.n_nodeblocks > 0) {
     // algorithmic bytes per active node: m, mv, f written, read back by the
@@ -2762,6 +3104,68 @@ void g2p_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const te
   g2p_launch<T, D, BASIS, STORED>(ctx, mesh, G, P, W, prm, sp);
 }
 
+// A @G2P2G pass as one particle sweep (k_g2p2g_fast) where it is asked for
+// (TESS_G2P2G_SWEEP=1 in the environment) and applies (deferred weights
+// without kernel correction).  The particles were binned at their current
+// positions.  The sweep runs both halves and writes the tile partials;
+// p2g_launch then zeroes the grid and runs the grid-node half.  With
+// write_keys the sweep also writes the next binning's keys.  Returns false
+// (nothing enqueued) where the caller composes the G2P and P2G kernels, the
+// default: on the 13.4 M-particle block drop the sweep takes 1.35 ms against
+// 0.82 + 0.50 ms for the two kernels (its producer waves, 10 per CU against
+// the G2P kernel's 12, wait on memory, and the per-round barrier couples them
+// to the consumers).
+inline bool g2p2g_sweep_enabled() {
+  const char* e = std::getenv("TESS_G2P2G_SWEEP");
+  return e && e[0] == '1' && e[1] == '\0';
+}
+
+template <typename T, int D, int BASIS>
+bool g2p2g_sweep(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
+                 const tess_params& prm, bool write_keys, bool count_misplaced) {
+  if (ctx->corr != 0 || !g2p2g_sweep_enabled()) return false;
+  using SH = G2P2GShape<T, D, BASIS>;
+  const Binning& b = ctx->bin;
+  uint32_t* nk = nullptr;
+  uint32_t* stay = nullptr;
+  uint32_t* mv = nullptr;  // movers: particles whose key changes
+  if (write_keys && P.n > 0) {
+    nk = ctx->inc_nk.as<uint32_t>(size_t(P.n));
+    stay = ctx->inc_spos.as<uint32_t>(size_t(P.n) + 1);
+    mv = movers_counters(ctx);
+    movers_zero(ctx);
+  }
+  const bool aos = julia_layout(P, D);
+  const uint32_t nt = uint32_t(b.ntiles);
+  const SweepFn<T, D> sweep = [&](Args<T, D>& a, const uint32_t* sk, uint32_t* misplaced) {
+    // algorithmic bytes: those of the G2P kernel (x, v, F read; v, ∇v, x, F,
+    // σ, V written; vⁿ, v read per active node); the P2G half reads nothing
+    const double nodes = double(std::max<int64_t>(ctx->last_active_nodes, 0));
+    const int per_p = 2 * D + D * D + 2 * D + 3 * D * D + 1;
+    const int tok = prof_begin(ctx, "k_g2p2g_tile", double(b.n_inside) * sizeof(T) * per_p + nodes * sizeof(T) * 2 * D);
+    const dim3 grid(xcd_grid(b.n_active)), block(64 * (SH::NP + SH::NC));
+    if (aos)
+      hipLaunchKernelGGL((k_g2p2g_fast<T, D, BASIS, true, SH::NP, SH::NC>), grid, block, 0, ctx->stream, a, sk, nk,
+                         stay, nt, mv, misplaced);
+    else
+      hipLaunchKernelGGL((k_g2p2g_fast<T, D, BASIS, false, SH::NP, SH::NC>), grid, block, 0, ctx->stream, a, sk, nk,
+                         stay, nt, mv, misplaced);
+    TESS_HIP(hipGetLastError());
+    prof_end(ctx, tok);
+  };
+  p2g_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, count_misplaced, nullptr, &sweep);
+  if (P.n > b.n_inside) {  // particles outside the mesh: the G2P half's update with zero sums
+    Args<T, D> a = make_args<T, D>(mesh, &G, P, nullptr, &prm);
+    fill_binning_ptrs<T, D>(ctx, a);
+    hipLaunchKernelGGL((k_g2p_outside<T, D>), dim3(blocks_for(P.n - b.n_inside, 256)), dim3(256), 0, ctx->stream,
+                       a, static_cast<const uint32_t*>(ctx->keys_sorted.ptr), nk, stay);
+    TESS_HIP(hipGetLastError());
+  }
+  ctx->bin.valid = false;  // positions moved
+  ctx->bin.keys_ready = nk != nullptr;
+  return true;
+}
+
 // @G2P2G of the README program (src/Transfer/g2p2g.jl:100-159): one set of
 // weights, at the positions before the pass, serves the G2P half and the P2G
 // half.  One binning at those positions orders both halves: the G2P kernel
@@ -2772,6 +3176,9 @@ template <typename T, int D, int BASIS, bool STORED>
 void g2p2g_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const tess_particles& P,
                 const tess_weights* W, const tess_params& prm) {
   bin_particles<T, D, BASIS, STORED>(ctx, mesh, P, W);
+  if constexpr (!STORED) {
+    if (g2p2g_sweep<T, D, BASIS>(ctx, mesh, G, P, prm, false, false)) return;
+  }
   // the G2P half keeps each particle's position before its update aside
   // (deferred weights), for the P2G half's weights
   tess_particles Pold = P;
@@ -2808,17 +3215,19 @@ void g2p2g_steps_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, 
         ctx->reorders += 1;
       }
     }
-    tess_particles Pold = P;
-    T* xo = nullptr;
-    if (P.n > 0) {
-      xo = ctx->g2p2g_x.as<T>(size_t(P.n) * D);
-      Pold.x = tess_field{xo, D, 1};
+    if (!g2p2g_sweep<T, D, BASIS>(ctx, mesh, G, P, prm, s + 1 < nsteps, reorder_threshold > 0.0)) {
+      tess_particles Pold = P;
+      T* xo = nullptr;
+      if (P.n > 0) {
+        xo = ctx->g2p2g_x.as<T>(size_t(P.n) * D);
+        Pold.x = tess_field{xo, D, 1};
+      }
+      // the G2P half writes the positions before its update to xo
+      g2p_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, nullptr, s + 1 < nsteps, xo);
+      ctx->bin.valid = true;  // the binning describes the old positions the P2G half reads
+      p2g_launch<T, D, BASIS, false>(ctx, mesh, G, Pold, nullptr, prm, reorder_threshold > 0.0, nullptr);
+      ctx->bin.valid = false;
     }
-    // the G2P half writes the positions before its update to xo
-    g2p_launch<T, D, BASIS, false>(ctx, mesh, G, P, nullptr, prm, nullptr, s + 1 < nsteps, xo);
-    ctx->bin.valid = true;  // the binning describes the old positions the P2G half reads
-    p2g_launch<T, D, BASIS, false>(ctx, mesh, G, Pold, nullptr, prm, reorder_threshold > 0.0, nullptr);
-    ctx->bin.valid = false;
     // the storage order the P2G half measured is that of the particles themselves
     if (reorder_threshold > 0.0) ctx->misplaced_x_ptr = P.x.ptr;
   }

@@ -1,7 +1,19 @@
 """GPU @G2P2G (tess_g2p2g) against the oracle's: one README P2G, then one G2P2G
 pass (the G2P half, then the P2G and grid-node halves with the weights at the
 positions before the pass).  Float64 within 1e-12 relative to each field's
-largest entry; stored and deferred weights agree; run-to-run bitwise equal."""
+largest entry; stored and deferred weights agree; run-to-run bitwise equal.
+
+The library runs a pass as the G2P kernel followed by the P2G kernels at the
+kept positions; with TESS_G2P2G_SWEEP=1 and deferred weights it runs one
+particle sweep (k_g2p2g_fast) instead.  After one pass the two give the same
+particles and grid, bit for bit.  Over several passes they stay within a few
+units in the last place: the G2P kernel's two walks of a tile (class order and
+sorted order) are separately compiled copies of the same arithmetic, which the
+compiler may contract differently, and the two paths walk some tiles
+differently."""
+import contextlib
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -107,3 +119,104 @@ def test_g2p2g_steps_match_oracle_and_single_passes(oracle, dim):
     ia, ib = key(a), key(b)
     for name in ("x", "v", "F"):
         assert rel_err(b[name][ib], a[name][ia]) <= 1e-12, name
+
+
+@contextlib.contextmanager
+def _sweep(on):
+    """TESS_G2P2G_SWEEP=1 (one particle sweep per pass) or 0 (the G2P kernel, then the P2G kernels)."""
+    old = os.environ.get("TESS_G2P2G_SWEEP")
+    os.environ["TESS_G2P2G_SWEEP"] = "1" if on else "0"
+    try:
+        yield
+    finally:
+        if old is None:
+            del os.environ["TESS_G2P2G_SWEEP"]
+        else:
+            os.environ["TESS_G2P2G_SWEEP"] = old
+
+
+def _assert_same(a, b):
+    for part_a, part_b in zip(a, b):
+        assert part_a.keys() == part_b.keys()
+        for name in part_a:
+            np.testing.assert_array_equal(part_a[name], part_b[name], err_msg=name)
+
+
+def _assert_close(a, b, tol):
+    for part_a, part_b in zip(a, b):
+        assert part_a.keys() == part_b.keys()
+        for name in part_a:
+            assert rel_err(part_a[name], part_b[name]) <= tol, name
+
+
+def _passes(sc, basis, dtype, layout, sweep, npass):
+    mesh, P, G = to_gpu(sc, layout=layout, dtype=dtype)
+    W = th.generate_basis_weights(basis, mesh, sc.n, deferred=True)
+    th.P2G(G, P, W, sc.params)
+    with _sweep(sweep):
+        for _ in range(npass):
+            th.G2P2G(G, P, W, sc.params)
+    torch.cuda.synchronize()
+    return particles_np(P), grid_np(G)
+
+
+@pytest.mark.parametrize("layout", ["julia", "soa"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("basis", BASES, ids=BASIS_IDS)
+def test_g2p2g_sweep_equals_composed_kernels(dim, basis, dtype, layout):
+    """One G2P2G pass: the single sweep and the composed kernels agree bit for bit on every particle and grid
+    field (truncated windows and particles outside the mesh included); over three passes within a few units in
+    the last place (1e-13 Float64, 1e-6 Float32, relative to each field's largest entry)."""
+    print(f"[g2p2g sweep] dim={dim} basis={basis} dtype={dtype} layout={layout}", flush=True)
+    sc = _scene(dim, basis, 131 + dim)
+    _assert_same(_passes(sc, basis, dtype, layout, True, 1), _passes(sc, basis, dtype, layout, False, 1))
+    tol = 1e-13 if dtype == torch.float64 else 1e-6
+    _assert_close(_passes(sc, basis, dtype, layout, True, 3), _passes(sc, basis, dtype, layout, False, 3), tol)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_g2p2g_sweep_dense_tiles(dim):
+    """Tiles holding more particles than the kernels stage in LDS (over 1024, read through the global
+    permutation): one sweep equals the composed kernels bit for bit (Float32)."""
+    basis = th.BSpline(th.Quadratic())
+    n = 60000 if dim == 3 else 20000
+    sc = random_box(dim, n, 17 + dim, basis, h=0.1, lo=0.0, hi=1.0, stress=1.0)
+    runs = []
+    for sweep in (True, False):
+        ctx = th.Context()
+        mesh, P, G = to_gpu(sc, dtype=torch.float32)
+        W = th.generate_basis_weights(basis, mesh, sc.n, deferred=True)
+        th.P2G(G, P, W, sc.params, ctx=ctx)
+        tiles = ctx.stats()["active_tiles"]
+        with _sweep(sweep):
+            th.G2P2G(G, P, W, sc.params, ctx=ctx)
+        ctx.sync()
+        runs.append((particles_np(P), grid_np(G)))
+        ctx.close()
+    assert n / tiles > 1024, (n, tiles)
+    _assert_same(runs[0], runs[1])
+
+
+@pytest.mark.parametrize("threshold", [0.0, 0.85, 1.0])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_g2p2g_steps_sweep_matches_composed_kernels(dim, threshold):
+    """tess_g2p2g_steps with the sweep (next keys written by its G2P half, the contiguity count of its P2G
+    half, reorders at the threshold) against the composed kernels: the same reorders, movers and kept
+    binnings, and fields within 1e-6 (Float32, six passes)."""
+    basis = th.BSpline(th.Quadratic())
+    sc = _scene(dim, basis, 151 + dim)
+    runs, stats = [], []
+    for sweep in (True, False):
+        ctx = th.Context()
+        mesh, P, G = to_gpu(sc, dtype=torch.float32)
+        th.P2G(G, P, th.generate_basis_weights(basis, mesh, sc.n, deferred=True), sc.params, ctx=ctx)
+        with _sweep(sweep):
+            th.g2p2g_steps(G, P, basis, sc.params, nsteps=6, reorder_threshold=threshold, ctx=ctx)
+        ctx.sync()
+        runs.append((particles_np(P), grid_np(G)))
+        st = ctx.stats()
+        stats.append({k: st[k] for k in ("reorders", "misplaced", "movers", "kept_binnings")})
+        ctx.close()
+    assert stats[0] == stats[1]
+    _assert_close(runs[0], runs[1], 1e-6)
