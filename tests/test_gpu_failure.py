@@ -174,3 +174,43 @@ def test_batch_recovers_from_every_injected_failure(variant):
     check()
     batch.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("sweep", ["0", "1"], ids=["composed", "sweep"])
+def test_g2p2g_steps_recover_from_every_injected_failure(sweep, monkeypatch):
+    """tess_g2p2g_steps (three passes, reorder at 0.85), with the composed kernels and with the single-sweep kernel
+    (TESS_G2P2G_SWEEP=1)."""
+    monkeypatch.setenv("TESS_G2P2G_SWEEP", sweep)
+    sc = two_disks(dtype=torch.float64, h=0.04, steps=3)
+
+    def fresh_run(ctx, P, G):
+        _restore(P, sc)
+        th.P2G(G, P, sc.basis, sc.params, ctx=ctx)
+        th.g2p2g_steps(G, P, sc.basis, sc.params, nsteps=3, reorder_threshold=0.85, ctx=ctx)
+        ctx.sync()
+
+    ref_ctx = th.Context()
+    mesh, Pr, Gr = to_gpu(sc)
+    fresh_run(ref_ctx, Pr, Gr)
+    ref = particles_np(Pr)
+    ref_ctx.close()
+
+    ctx = th.Context()
+    mesh, P, G = to_gpu(sc)
+
+    def run():
+        fresh_run(ctx, P, G)
+
+    def check():
+        run()
+        got = particles_np(P)
+        key = lambda p: np.lexsort(p["x"].T[::-1])  # noqa: E731  (the reorder may permute the particles)
+        ia, ib = key(got), key(ref)
+        for name in FIELDS:
+            assert rel_err(got[name][ia], ref[name][ib]) <= 1e-12, name
+
+    fired = _inject_everywhere(ctx.lib, run, check)
+    print(f"[failure] g2p2g_steps sweep={sweep}: {fired} injected failures recovered", flush=True)
+    assert fired >= 10
+    check()
+    ctx.close()
