@@ -729,15 +729,12 @@ __device__ __forceinline__ void tile_class_counts(const uint32_t* __restrict__ s
                                                   int cap = 0) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform: scalar branches
-  if (wave == 0) s_cnt[lane] = 0;
-  __syncthreads();
-  for (int base = wave * 64; base < cnt; base += 64 * NW) {
-    const int i = base + lane;
+  // one chunk of 64 sorted slots: stage ids and keys, count the classes
+  auto chunk = [&](int i, uint32_t key, uint32_t pid) {
     const bool valid = i < cnt;
-    const uint32_t key = valid ? skeys[start + i] : 0u;
     const uint32_t c = key & 63u;
     if (s_pid && valid && i < cap) {
-      s_pid[i] = perm[start + i] - poff;
+      s_pid[i] = pid - poff;
       if (s_key) s_key[i] = key;
     }
     uint64_t peers = __ballot(valid);
@@ -748,6 +745,29 @@ __device__ __forceinline__ void tile_class_counts(const uint32_t* __restrict__ s
       peers &= bit ? m : ~m;
     }
     if (valid && (peers & lanemask_lt64()) == 0) atomicAdd(&s_cnt[c], uint32_t(__popcll(peers)));
+  };
+  // the keys and ids of a wave's first CC chunks are loaded together, before
+  // the barrier (it waits only on LDS operations), so one memory round trip
+  // serves them (and whatever the caller has in flight)
+  constexpr int CC = 4;
+  uint32_t kk[CC], pp[CC];
+#pragma unroll
+  for (int r = 0; r < CC; ++r) {
+    const int i = wave * 64 + r * 64 * NW + lane;
+    kk[r] = i < cnt ? skeys[start + i] : 0u;
+    pp[r] = (s_pid && i < cnt && i < cap) ? perm[start + i] : 0u;
+  }
+  if (wave == 0) s_cnt[lane] = 0;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < CC; ++r) {
+    const int i = wave * 64 + r * 64 * NW + lane;
+    if (wave * 64 + r * 64 * NW < cnt) chunk(i, kk[r], pp[r]);  // uniform
+  }
+  for (int base = wave * 64 + CC * 64 * NW; base < cnt; base += 64 * NW) {
+    const int i = base + lane;
+    const bool valid = i < cnt;
+    chunk(i, valid ? skeys[start + i] : 0u, (s_pid && valid && i < cap) ? perm[start + i] : 0u);
   }
   __syncthreads();
   ccount = int(s_cnt[lane]);
@@ -1513,16 +1533,28 @@ __device__ __forceinline__ void g2p_load(const Args<T, D>& a, uint32_t pid, G2PI
   if constexpr (BASIS == TESS_UGIMP) in.l = a.l(pid) / a.mesh.h;
 }
 
-// Stages a tile's footprint for G2P: v and v - vⁿ per component in the padded
-// node order of Geo (zeros outside the grid), and the footprint's node
-// coordinates per axis; NT threads.  The caller synchronises before use.
+// Stages a tile's footprint for G2P in two halves: g2p_stage_load issues the
+// grid loads of v and vⁿ (zeros outside the grid) into registers, and
+// g2p_stage_store writes v and v - vⁿ per component in the padded node order
+// of Geo, with the footprint's node coordinates per axis; NT threads.  The
+// loads stay in flight across the barriers in between (a barrier waits only on
+// LDS operations), so the caller issues the tile's key and id loads before the
+// grid values are needed, and one memory round trip serves both.  The caller
+// synchronises after the store.
 template <typename T, int D, int NT>
-__device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const int* origin, T (*s_n)[Geo<D>::PSZ],
-                                          T (*s_nodex)[Geo<D>::TS]) {
+struct G2PStage {
+  static constexpr int NPER = (Geo<D>::NFOOT + NT - 1) / NT;
+  T v[NPER][D], vn[NPER][D];
+};
+
+template <typename T, int D, int NT>
+__device__ __forceinline__ void g2p_stage_load(const Args<T, D>& a, const int* origin, G2PStage<T, D, NT>& st) {
   using G = Geo<D>;
-  for (int q = int(threadIdx.x); q < G::NFOOT; q += NT) {
+#pragma unroll
+  for (int u = 0; u < G2PStage<T, D, NT>::NPER; ++u) {
+    const int q = int(threadIdx.x) + u * NT;
     int I[D], r = q;
-    bool in = true;
+    bool in = q < G::NFOOT;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       I[d] = origin[d] + r % G::TS;
@@ -1531,13 +1563,29 @@ __device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const int* origin
     }
     const int64_t i = in ? grid_slot(a, I) : -1;
     in = i >= 0;
-    const int pq = pad_node<D>(q % G::TS, (q / G::TS) % G::TS, D == 3 ? q / (G::TS * G::TS) : 0);
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      const T vv = in ? a.gv(i, c) : T(0);
-      const T vn = in ? a.gvn(i, c) : T(0);
-      s_n[c][pq] = vv;
-      s_n[D + c][pq] = vv - vn;
+      st.v[u][c] = in ? a.gv(i, c) : T(0);
+      st.vn[u][c] = in ? a.gvn(i, c) : T(0);
+    }
+  }
+}
+
+template <typename T, int D, int NT>
+__device__ __forceinline__ void g2p_stage_store(const Args<T, D>& a, const int* origin,
+                                                const G2PStage<T, D, NT>& st, T (*s_n)[Geo<D>::PSZ],
+                                                T (*s_nodex)[Geo<D>::TS]) {
+  using G = Geo<D>;
+#pragma unroll
+  for (int u = 0; u < G2PStage<T, D, NT>::NPER; ++u) {
+    const int q = int(threadIdx.x) + u * NT;
+    if (q < G::NFOOT) {
+      const int pq = pad_node<D>(q % G::TS, (q / G::TS) % G::TS, D == 3 ? q / (G::TS * G::TS) : 0);
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        s_n[c][pq] = st.v[u][c];
+        s_n[D + c][pq] = st.v[u][c] - st.vn[u][c];
+      }
     }
   }
   for (int q = int(threadIdx.x); q < D * G::TS; q += NT) {
@@ -1740,11 +1788,14 @@ __device__ __forceinline__ void k_g2p_fast_body(Args<T, D>& a, const uint32_t* _
   __shared__ T s_n[2 * D][G::PSZ];
   __shared__ T s_nodex[D][G::TS];
   __shared__ uint32_t s_cnt[64];
-  g2p_stage<T, D, 64 * NW>(a, origin, s_n, s_nodex);
+  G2PStage<T, D, 64 * NW> stage;
+  g2p_stage_load<T, D, 64 * NW>(a, origin, stage);
   int ccount, coff, maxcnt;
   __shared__ uint32_t s_pid[PID_CAP], s_key[PID_CAP];
   tile_class_counts<NW>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt, a.perm, poff, s_pid, s_key,
-                        PID_CAP);  // barrier after the staging
+                        PID_CAP);  // barrier after the staging of ids and keys
+  g2p_stage_store<T, D, 64 * NW>(a, origin, stage, s_n, s_nodex);
+  __syncthreads();
   // Two walks of the tile's particles.  Class order (lane = class, iteration k =
   // the k-th particle of every class) reads the layered storage contiguously
   // and takes maxcnt iterations.  Sorted order (iteration k = sorted positions
@@ -1932,10 +1983,13 @@ __device__ __forceinline__ void k_g2p2g_fast_body(Args<T, D>& a, const uint32_t*
   __shared__ __align__(16) T s_x[2][NP][XR][64];
   static_assert(NV * G::PSZ <= 2 * NP * XR * 64, "the node sums fit in the exchange");
   T(*s_tile)[G::PSZ] = reinterpret_cast<T(*)[G::PSZ]>(&s_x[0][0][0][0]);
-  g2p_stage<T, D, NT>(a, origin, s_n, s_nodex);
+  G2PStage<T, D, NT> stage;
+  g2p_stage_load<T, D, NT>(a, origin, stage);
   int ccount, coff, maxcnt;
   tile_class_counts<NP + NC>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt, a.perm, poff, s_pid, s_key,
-                             PID_CAP);  // barrier after the staging
+                             PID_CAP);  // barrier after the staging of ids and keys
+  g2p_stage_store<T, D, NT>(a, origin, stage, s_n, s_nodex);
+  __syncthreads();
   const int nround = (maxcnt + NP - 1) / NP;  // the same in every wave
   auto zero_tile = [&]() {
     static_assert(NV * G::PSZ * sizeof(T) % 16 == 0, "node sums in 16-byte words");
