@@ -1284,6 +1284,80 @@ __device__ __forceinline__ void p2g_write_partial(const Args<T, D>& a, const int
   }
 }
 
+// With several waves per tile, each owning KN offsets along the last axis,
+// wave `pass` adds its stencil into a slab of its own: the PL = KN + BW - 1
+// planes (last-axis node coordinates pass*KN ..) that its offsets reach, in
+// the padded node order of Geo, one array of PL planes per component.  The
+// waves fill their slabs at the same time; the partial is then the sum of the
+// slabs in wave order (fixed per-node order, so the sums are deterministic).
+template <typename T, int D, int SW, int KN>
+struct Slab {
+  static constexpr int PL = KN + Geo<D>::BW - 1;
+  static constexpr int PITCH = D == 3 ? Geo<D>::PZ : Geo<D>::PY;  // words per plane
+  static constexpr int VSTRIDE = PL * PITCH;                     // words per component
+  static constexpr int SIZE = (1 + 2 * D) * VSTRIDE;             // words per wave
+};
+
+template <typename T, int D, int SW, int KN>
+__device__ __forceinline__ void p2g_stencil_to_slab(T* slab, const int* ccoord, int pass, const T* am,
+                                                    const T (*amv)[Pow<SW, D - 1>::v * KN],
+                                                    const T (*af)[Pow<SW, D - 1>::v * KN]) {
+  using SL = Slab<T, D, SW, KN>;
+  constexpr int NV = 1 + 2 * D;
+  constexpr int NSL = Pow<SW, D - 1>::v * KN;
+#pragma unroll
+  for (int o = 0; o < NSL; ++o) {
+    int r = o, nI[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      nI[d] = ccoord[d] + ((d == D - 1) ? r : r % SW);  // the last axis relative to the slab's first plane
+      r /= SW;
+    }
+    const int q = pad_node<D>(nI[0], nI[1], nI[2]);
+    T t[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) t[v] = slab[v * SL::VSTRIDE + q];
+    t[0] += am[o];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      t[1 + c] += amv[c][o];
+      t[1 + D + c] += af[c][o];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) slab[v * SL::VSTRIDE + q] = t[v];
+    __asm__ volatile("" ::: "memory");  // keep the offsets in program order
+  }
+}
+
+// The tile's compact partial from the NPASS waves' slabs, summed in wave order;
+// NT threads, one node each.
+template <typename T, int D, int SW, int KN, int NPASS, int NT>
+__device__ __forceinline__ void p2g_write_partial_slabs(const Args<T, D>& a, const int64_t j, const T* slabs) {
+  using PT = Part<D, SW>;
+  using SL = Slab<T, D, SW, KN>;
+  constexpr int NV = 1 + 2 * D;
+  T* out = a.partials + int64_t(j) * PT::SIZE;
+  for (int n = int(threadIdx.x); n < PT::NODES; n += NT) {
+    const int x = n % PT::E, y = (n / PT::E) % PT::E, z = D == 3 ? n / (PT::E * PT::E) : 0;
+    const int last = D == 3 ? z : y;
+    T sum[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) sum[v] = T(0);
+#pragma unroll
+    for (int w = 0; w < NPASS; ++w) {
+      const int pl = last - w * KN;
+      if (pl >= 0 && pl < SL::PL) {
+        const int q = D == 3 ? pad_node<3>(x, y, pl) : pad_node<2>(x, pl, 0);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sum[v] += slabs[w * SL::SIZE + v * SL::VSTRIDE + q];
+      }
+    }
+    T* o = out + int64_t(n) * NV;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) o[v] = sum[v];
+  }
+}
+
 // NPASS waves per tile: wave w accumulates the offsets w*KN .. w*KN+KN-1
 // along the last axis for every class (the particles are read by each wave;
 // the registers of one wave hold 1/NPASS of the stencil, which keeps several
@@ -1311,23 +1385,35 @@ __device__ __forceinline__ void k_p2g_fast_body(Args<T, D>& a, const uint32_t* _
 #pragma unroll
   for (int d = 0; d < D; ++d) origin[d] = tI[d] * G::BW - 1;
 
-  // the tile's node sums m, mv[D], f[D], one array per component over the
-  // padded footprint (bank-conflict-free for the class lanes)
-  __shared__ __align__(16) T s_tile[NV][G::PSZ];
+  // The tile's node sums m, mv[D], f[D].  One wave: one array per component
+  // over the padded footprint (bank-conflict-free for the class lanes).
+  // Several: one slab per wave (Slab), in LDS that the per-layer jet exchange
+  // and the staged particle ids use before.
+  constexpr bool SHJ = NPASS > 1;
+  using SL = Slab<T, D, SW, KN>;
+  constexpr int JET_WORDS = SHJ ? 2 * D * 2 * SW * 64 : 0;  // [layer parity][axis][val SW, der SW][lane]
+  constexpr int PID_WORDS = PID_CAP * int(sizeof(uint32_t)) / int(sizeof(T));
+  constexpr int SUM_WORDS = SHJ ? NPASS * SL::SIZE : NV * G::PSZ;
+  constexpr int RAW_WORDS = SHJ ? (SUM_WORDS > JET_WORDS + PID_WORDS ? SUM_WORDS : JET_WORDS + PID_WORDS)
+                                : SUM_WORDS + JET_WORDS + PID_WORDS;
+  __shared__ __align__(16) T s_raw[RAW_WORDS];
+  T* const s_sums = s_raw;
+  T(*s_tile)[G::PSZ] = reinterpret_cast<T(*)[G::PSZ]>(s_raw);  // one wave
+  T(*s_jet)[D][2 * SW][64] = reinterpret_cast<T(*)[D][2 * SW][64]>(s_raw + (SHJ ? 0 : SUM_WORDS));
+  uint32_t* const s_pid = reinterpret_cast<uint32_t*>(s_raw + (SHJ ? JET_WORDS : SUM_WORDS + JET_WORDS));
   __shared__ T s_nodex[D][G::TS];
   __shared__ uint32_t s_cnt[64];
-  // per-layer axis jets shared by the waves (NPASS > 1): [layer parity][axis][val SW, der SW][lane]
-  constexpr bool SHJ = NPASS > 1;
-  __shared__ T s_jet[SHJ ? 2 : 1][SHJ ? D : 1][SHJ ? 2 * SW : 1][SHJ ? 64 : 1];
   static_assert(NV * G::PSZ * sizeof(T) % 16 == 0, "LDS tile in 16-byte words");
-  for (int q = int(threadIdx.x); q < int(NV * G::PSZ * sizeof(T) / 16); q += NT)
-    reinterpret_cast<uint4*>(&s_tile[0][0])[q] = make_uint4(0u, 0u, 0u, 0u);
+  static_assert(JET_WORDS * sizeof(T) % 16 == 0 && SUM_WORDS * sizeof(T) % 16 == 0, "16-byte LDS sections");
+  if constexpr (!SHJ) {
+    for (int q = int(threadIdx.x); q < int(NV * G::PSZ * sizeof(T) / 16); q += NT)
+      reinterpret_cast<uint4*>(&s_tile[0][0])[q] = make_uint4(0u, 0u, 0u, 0u);
+  }
   for (int q = int(threadIdx.x); q < D * G::TS; q += NT) {
     const int d = q / G::TS, k = q % G::TS;
     s_nodex[d][k] = a.mesh.node(d, origin[d] + k);
   }
   int ccount, coff, maxcnt;
-  __shared__ uint32_t s_pid[PID_CAP];
   tile_class_counts<NPASS>(skeys, start, cnt, s_cnt, ccount, coff, maxcnt, a.perm, poff, s_pid, nullptr,
                            PID_CAP);  // barrier after the staging
   int ccoord[D];
@@ -1461,12 +1547,22 @@ __device__ __forceinline__ void k_p2g_fast_body(Args<T, D>& a, const uint32_t* _
       walk(std::true_type{});
     else
       walk(std::false_type{});
-    // lanes are distinct classes, so for one offset their target nodes are
-    // distinct; offsets are applied one after another (read, add, write: the
-    // LDS executes one wave's accesses in order) and the waves one after
-    // another, which fixes the per-node order of the additions
-    for (int w = 0; w < NPASS; ++w) {
-      if (w == wave && ccount > 0) p2g_stencil_to_lds<T, D, SW, KN>(s_tile, ccoord, pass, am, amv, af);
+    if constexpr (SHJ) {
+      // the slabs take the LDS of the jet exchange and the ids: after every
+      // wave's last layer.  Each wave zeroes and fills its own slab (the LDS
+      // executes one wave's accesses in order)
+      __syncthreads();
+      T* const slab = s_sums + pass * SL::SIZE;
+      static_assert(SL::SIZE * sizeof(T) % 16 == 0, "slabs in 16-byte words");
+      for (int q = lane; q < int(SL::SIZE * sizeof(T) / 16); q += 64)
+        reinterpret_cast<uint4*>(slab)[q] = make_uint4(0u, 0u, 0u, 0u);
+      if (ccount > 0) p2g_stencil_to_slab<T, D, SW, KN>(slab, ccoord, pass, am, amv, af);
+      __syncthreads();
+    } else {
+      // lanes are distinct classes, so for one offset their target nodes are
+      // distinct; offsets are applied one after another (read, add, write: the
+      // LDS executes one wave's accesses in order)
+      if (ccount > 0) p2g_stencil_to_lds<T, D, SW, KN>(s_tile, ccoord, pass, am, amv, af);
       __syncthreads();
     }
   }
@@ -1475,7 +1571,10 @@ __device__ __forceinline__ void k_p2g_fast_body(Args<T, D>& a, const uint32_t* _
     for (int off = 32; off >= 1; off >>= 1) bad += __shfl_xor(bad, off, 64);
     if (lane == 0) misplaced[j] = bad;  // per tile, summed later
   }
-  p2g_write_partial<T, D, SW, NT>(a, j, s_tile);
+  if constexpr (SHJ)
+    p2g_write_partial_slabs<T, D, SW, KN, NPASS, NT>(a, j, s_sums);
+  else
+    p2g_write_partial<T, D, SW, NT>(a, j, s_tile);
 }
 
 template <typename T, int D, int BASIS, int NPASS, bool AOS, bool BATCH = false, bool SEP = false>
@@ -1979,10 +2078,14 @@ __device__ __forceinline__ void k_g2p2g_fast_body(Args<T, D>& a, const uint32_t*
   __shared__ uint32_t s_cnt[64];
   __shared__ uint32_t s_pid[PID_CAP], s_key[PID_CAP];
   // [round parity][producer][row][class lane]; after the last round the
-  // tile's node sums (one array per component) take its place
-  __shared__ __align__(16) T s_x[2][NP][XR][64];
-  static_assert(NV * G::PSZ <= 2 * NP * XR * 64, "the node sums fit in the exchange");
-  T(*s_tile)[G::PSZ] = reinterpret_cast<T(*)[G::PSZ]>(&s_x[0][0][0][0]);
+  // tile's node sums take its place: the consumers' slabs (several consumers)
+  // or one array per component over the footprint (one)
+  using SL = Slab<T, D, SW, KN>;
+  constexpr int XWORDS = 2 * NP * XR * 64;
+  constexpr int SUMW = NC > 1 ? NC * SL::SIZE : NV * G::PSZ;
+  __shared__ __align__(16) T s_xraw[XWORDS > SUMW ? XWORDS : SUMW];
+  T(*s_x)[NP][XR][64] = reinterpret_cast<T(*)[NP][XR][64]>(s_xraw);
+  T(*s_tile)[G::PSZ] = reinterpret_cast<T(*)[G::PSZ]>(s_xraw);
   G2PStage<T, D, NT> stage;
   g2p_stage_load<T, D, NT>(a, origin, stage);
   int ccount, coff, maxcnt;
@@ -1996,8 +2099,9 @@ __device__ __forceinline__ void k_g2p2g_fast_body(Args<T, D>& a, const uint32_t*
     for (int q = int(threadIdx.x); q < int(NV * G::PSZ * sizeof(T) / 16); q += NT)
       reinterpret_cast<uint4*>(&s_tile[0][0])[q] = make_uint4(0u, 0u, 0u, 0u);
   };
-  // Both roles pass the same barriers: nround + 1 in the rounds, one after the
-  // node sums are zeroed, and one per consumer as the stencils are added.
+  // Both roles pass the same barriers: nround + 1 in the rounds, then one after
+  // the consumers fill their slabs (several consumers), or one after the node
+  // sums are zeroed and one as the consumer adds its stencil (one).
   if (wave < NP) {
     // producer: layers k = wave, wave + NP, ... (round r takes k = r*NP + wave)
     auto ahead = [&](int k, auto staged) {
@@ -2066,9 +2170,13 @@ __device__ __forceinline__ void k_g2p2g_fast_body(Args<T, D>& a, const uint32_t*
       walk(std::true_type{});
     else
       walk(std::false_type{});
-    zero_tile();
-    __syncthreads();
-    for (int w = 0; w < NC; ++w) __syncthreads();
+    if constexpr (NC > 1) {
+      __syncthreads();
+    } else {
+      zero_tile();
+      __syncthreads();
+      __syncthreads();
+    }
   } else {
     // consumer `pass`: the stencil offsets with last-axis offset pass*KN .. pass*KN+KN-1
     const int pass = wave - NP;
@@ -2136,11 +2244,18 @@ __device__ __forceinline__ void k_g2p2g_fast_body(Args<T, D>& a, const uint32_t*
       }
       __syncthreads();
     }
-    zero_tile();
-    __syncthreads();
-    // the consumers add their stencils one after another (fixed per-node order)
-    for (int w = 0; w < NC; ++w) {
-      if (w == pass && ccount > 0) p2g_stencil_to_lds<T, D, SW, KN>(s_tile, ccoord, pass, am, amv, af);
+    if constexpr (NC > 1) {
+      // each consumer zeroes and fills its own slab (the exchange is read out:
+      // the rounds ended with a barrier)
+      T* const slab = s_xraw + pass * SL::SIZE;
+      for (int q = lane; q < int(SL::SIZE * sizeof(T) / 16); q += 64)
+        reinterpret_cast<uint4*>(slab)[q] = make_uint4(0u, 0u, 0u, 0u);
+      if (ccount > 0) p2g_stencil_to_slab<T, D, SW, KN>(slab, ccoord, pass, am, amv, af);
+      __syncthreads();
+    } else {
+      zero_tile();
+      __syncthreads();
+      if (ccount > 0) p2g_stencil_to_lds<T, D, SW, KN>(s_tile, ccoord, pass, am, amv, af);
       __syncthreads();
     }
     if (misplaced && pass == 0) {
@@ -2149,7 +2264,10 @@ __device__ __forceinline__ void k_g2p2g_fast_body(Args<T, D>& a, const uint32_t*
       if (lane == 0) misplaced[j] = bad;  // per tile, summed later
     }
   }
-  p2g_write_partial<T, D, SW, NT>(a, j, s_tile);
+  if constexpr (NC > 1)
+    p2g_write_partial_slabs<T, D, SW, KN, NC, NT>(a, j, s_xraw);
+  else
+    p2g_write_partial<T, D, SW, NT>(a, j, s_tile);
 }
 
 template <typename T, int D, int BASIS, bool AOS, int NP, int NC>
