@@ -3283,8 +3283,8 @@ void g2p_impl(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const te
 // p2g_launch then zeroes the grid and runs the grid-node half.  With
 // write_keys the sweep also writes the next binning's keys.  Returns false
 // (nothing enqueued) where the caller composes the G2P and P2G kernels, the
-// default: on the 13.4 M-particle block drop the sweep takes 1.35 ms against
-// 0.82 + 0.50 ms for the two kernels (its producer waves, 10 per CU against
+// default: on the 13.4 M-particle block drop the sweep takes 1.23 ms against
+// 0.72 + 0.43 ms for the two kernels (its producer waves, 10 per CU against
 // the G2P kernel's 12, wait on memory, and the per-round barrier couples them
 // to the consumers).
 inline bool g2p2g_sweep_enabled() {
