@@ -22,8 +22,10 @@
 #include "transfer_impl.hpp"
 
 // Reversed steps a segment replay keeps enqueued ahead of its own progress.
+// The replay's host thread waits at each step's binning readback; the lead is
+// what the reversing stream has queued meanwhile.
 #ifndef TESS_ADJ_REVERSE_LEAD
-#define TESS_ADJ_REVERSE_LEAD 2
+#define TESS_ADJ_REVERSE_LEAD 6
 #endif
 
 namespace tess {
