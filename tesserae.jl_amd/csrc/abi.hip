@@ -145,7 +145,6 @@ void ctx_drop_caches(tess_ctx* c) {
   c->bin.keys_ready = false;
   c->last_grid_m = nullptr;
   c->prev_nodeblocks = 0;
-  c->prev_bin_gen = -1;
   c->misplaced_x_ptr = nullptr;
   c->last_misplaced = -1;
   c->rollout.last_kept = false;

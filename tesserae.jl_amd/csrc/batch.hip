@@ -104,7 +104,6 @@ void batch_bin_run(tess_ctx* ctx, tess_batch& B, bool keys_from_g2p) {
   const bool incremental = keys_from_g2p && B.keys_ready && B.binned && b.n == n && b.ntiles == int64_t(ntiles) &&
                            n > 0;
   B.keys_ready = false;
-  ctx->bin_gen += 1;  // the context's node-block list now describes the batch
   b.valid = false;
   b.state_valid = false;  // the single-scene incremental state does not describe a batch
   b.keys_ready = false;
@@ -799,7 +798,6 @@ void graph_reorder_check(tess_ctx* g, tess_batch& B, int cur) {
                           hipMemcpyDeviceToHost, g->stream));
   TESS_HIP(hipStreamSynchronize(g->stream));
   Binning& b = g->bin;
-  g->bin_gen += 1;
   b.n = B.ntot;
   b.ntiles = B.tiles_total;
   b.n_inside = B.ntot > 0 ? int64_t(hc[CNT_INSIDE]) : 0;

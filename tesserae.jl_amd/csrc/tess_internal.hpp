@@ -130,9 +130,6 @@ struct tess_ctx {
   // zeroing bookkeeping for the grid of the previous P2G
   const void* last_grid_m = nullptr;
   int64_t prev_nodeblocks = 0;
-  // incremented whenever a binning rebuilds the node-block list; the value at
-  // the last P2G (its list is then still the current one)
-  int64_t bin_gen = 0, prev_bin_gen = -1;
   int32_t prev_tile_log2 = -1;
   int64_t prev_ntiles = -1;
   int64_t prev_nnodes = -1;

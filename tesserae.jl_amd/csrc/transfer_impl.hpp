@@ -2911,7 +2911,6 @@ void bin_particles_run(tess_ctx* ctx, const tess_mesh& mesh, const tess_particle
     radix_sort_pairs(ctx, keys, nullptr, skeys, perm, n, kbits);
     ctx->last_movers = -1;
   }
-  ctx->bin_gen += 1;  // the tile and node-block lists are rebuilt
   TESS_HIP(hipMemsetAsync(tstart, 0, ntiles * sizeof(uint32_t), ctx->stream));
   TESS_HIP(hipMemsetAsync(tend, 0, ntiles * sizeof(uint32_t), ctx->stream));
   if (n > 0) {
@@ -2998,11 +2997,6 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   // An SpGrid's storage is zeroed whole (its numbering may have changed).
   const bool fresh = ctx->last_grid_m != G.m.ptr || ctx->prev_tile_log2 != Gm::LT ||
                      ctx->prev_ntiles != ctx->bin.ntiles || ctx->prev_nnodes != nnodes;
-  // the previous call on this grid used the same node-block list (no binning
-  // rebuilt it since): no block is stale, and the list kept for the next call
-  // is already the current one
-  const bool same_blocks = !sp && !fresh && ctx->prev_bin_gen == ctx->bin_gen &&
-                           ctx->prev_nodeblocks == ctx->bin.n_nodeblocks;
   if (sp) {
     a.p2g_err = counters + CNT_P2GERR;
     TESS_HIP(hipMemsetAsync(a.p2g_err, 0, sizeof(uint32_t), ctx->stream));
@@ -3017,7 +3011,7 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
     } else if (fresh) {
       hipLaunchKernelGGL((k_zero_grid<T, D>), dim3(blocks_for(nnodes, 256)), dim3(256), 0, ctx->stream, a, nnodes);
       TESS_HIP(hipGetLastError());
-    } else if (ctx->prev_nodeblocks > 0 && !same_blocks) {
+    } else if (ctx->prev_nodeblocks > 0) {
       hipLaunchKernelGGL((k_zero_stale<T, D>), dim3(unsigned(ctx->prev_nodeblocks)), dim3(64), 0, ctx->stream, a,
                          static_cast<const uint32_t*>(ctx->nb_list_prev.ptr), ctx->prev_nodeblocks);
       TESS_HIP(hipGetLastError());
@@ -3101,11 +3095,10 @@ void p2g_launch(tess_ctx* ctx, const tess_mesh& mesh, const tess_grid& G, const 
   }
   // remember this call's node blocks for the next call's stale zeroing
   uint32_t* prev = ctx->nb_list_prev.as<uint32_t>(size_t(std::max<int64_t>(b.ntiles, 1)));
-  if (b.n_nodeblocks > 0 && !same_blocks)
+  if (b.n_nodeblocks > 0)
     TESS_HIP(hipMemcpyAsync(prev, ctx->nb_list.ptr, size_t(b.n_nodeblocks) * sizeof(uint32_t),
                             hipMemcpyDeviceToDevice, ctx->stream));
   ctx->prev_nodeblocks = b.n_nodeblocks;
-  ctx->prev_bin_gen = ctx->bin_gen;
   ctx->last_grid_m = sp ? nullptr : G.m.ptr;  // a dense grid after an SpGrid call starts from a full zero
   ctx->prev_tile_log2 = Gm::LT;
   ctx->prev_ntiles = b.ntiles;
